@@ -1,0 +1,409 @@
+/*
+ * ops_mi355x.h — C ABI of the MI355X (gfx950) execution backend for the OPS
+ * structured-mesh API.
+ *
+ * Every entry point is a plain C function over plain pointers and sizes.  The
+ * names, argument meaning and error behaviour follow the reference OPS API;
+ * each declaration cites the reference declaration it replaces
+ * (paths relative to the OPS source tree).  Object structs keep the public
+ * fields of the reference types so application code that reads e.g.
+ * dat->size[0] keeps compiling; backend state hangs off the trailing impl
+ * pointer.
+ *
+ * Per-loop entry points have the shape of the reference translator's
+ * generated host stubs (ops_translator/resources/templates/cpp/
+ * loop_host.cpp.j2:12-33):
+ *   void ops_par_loop_<kernel>(const char *name, ops_block block, int dim,
+ *                              int *range, ops_arg arg0, ...);
+ * Only the hand-written kernels declared below exist; each validates its
+ * arguments (kind, access mode, type, stencil coverage and bounds) before it
+ * touches the GPU.
+ *
+ * Errors: by default an error prints "OPS error <code>: <msg>" and aborts,
+ * the outcome of the reference's uncaught OPSException
+ * (ops/c/include/ops_exceptions.h:46-53).  ops_mi_set_error_mode(1) makes the
+ * failing call return instead; ops_mi_last_error() then reports it.
+ *
+ * Threading: one host thread drives the API (ops/c/src/core/
+ * ops_instance.cpp:13,37-40).  All device work goes to one in-order HIP
+ * stream owned by the library (ops_mi_stream()).
+ */
+#ifndef OPS_MI355X_H
+#define OPS_MI355X_H
+
+#include <stddef.h>
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- constants: ops/c/include/ops_lib_core.h:57-92,163-165 ---- */
+#define OPS_MAX_DIM 5
+#define OPS_READ 0
+#define OPS_WRITE 1
+#define OPS_RW 2
+#define OPS_INC 3
+#define OPS_MIN 4
+#define OPS_MAX 5
+#define OPS_ARG_GBL 0
+#define OPS_ARG_DAT 1
+#define OPS_ARG_IDX 2
+#define OPS_HOST 1
+#define OPS_DEVICE 2
+/* error codes: ops/c/include/ops_exceptions.h:46-53 */
+#define OPS_NOT_IMPLEMENTED 1
+#define OPS_RUNTIME_CONFIGURATION_ERROR 2
+#define OPS_INTERNAL_ERROR 3
+#define OPS_RUNTIME_ERROR 4
+#define OPS_INVALID_ARGUMENT 6
+
+typedef int ops_access;
+typedef int ops_arg_type;
+typedef int ops_memspace;
+
+/* ---- object types: ops/c/include/ops_lib_core.h:181-552 ---- */
+typedef struct ops_block_core {
+  int index;
+  int dims;
+  const char *name;
+  void *instance;
+} ops_block_core;
+typedef ops_block_core *ops_block;
+
+typedef struct ops_dat_core {
+  int index;
+  ops_block block;
+  int dim;                  /* components per grid point (AoS) */
+  int type_size;            /* bytes per component */
+  int elem_size;            /* bytes per grid point */
+  int size[OPS_MAX_DIM];    /* allocated points per dimension, incl. halo and x padding */
+  int base[OPS_MAX_DIM];
+  int d_m[OPS_MAX_DIM];     /* negative halo depths (<= 0) */
+  int d_p[OPS_MAX_DIM];     /* positive halo depths, x includes x_pad */
+  int x_pad;
+  char *data;               /* host mirror (allocated on demand) */
+  char *data_d;             /* HBM allocation start of the array */
+  const char *name;
+  const char *type;
+  int dirty_hd;             /* 0 in sync, 1 host newer, 2 device newer */
+  int locked_hd;
+  int user_managed;
+  int is_hdf5;
+  const char *hdf5_file;
+  int e_dat;
+  size_t mem;               /* bytes of the array */
+  size_t base_offset;       /* bytes from data(_d) to point (0,...,0) */
+  int stride[OPS_MAX_DIM];
+  void *impl;
+} ops_dat_core;
+typedef ops_dat_core *ops_dat;
+
+typedef struct ops_stencil_core {
+  int index;
+  int dims;
+  int points;
+  const char *name;
+  int *stencil;
+  int *stride;
+  int *mgrid_stride;
+  int type;
+} ops_stencil_core;
+typedef ops_stencil_core *ops_stencil;
+
+typedef struct ops_reduction_core {
+  char *data;               /* host value (size bytes) */
+  int size;
+  int initialized;
+  int index;
+  ops_access acc;
+  char *type;
+  char *name;
+  void *instance;
+  void *impl;
+} ops_reduction_core;
+typedef ops_reduction_core *ops_reduction;
+
+typedef struct ops_halo_core {
+  ops_dat from;
+  ops_dat to;
+  int iter_size[OPS_MAX_DIM];
+  int from_base[OPS_MAX_DIM];
+  int to_base[OPS_MAX_DIM];
+  int from_dir[OPS_MAX_DIM];
+  int to_dir[OPS_MAX_DIM];
+  int index;
+} ops_halo_core;
+typedef ops_halo_core *ops_halo;
+
+typedef struct ops_halo_group_core {
+  int nhalos;
+  ops_halo *halos;
+  int index;
+  void *instance;
+} ops_halo_group_core;
+typedef ops_halo_group_core *ops_halo_group;
+
+/* ops/c/include/ops_lib_core.h:509-520 */
+typedef struct ops_arg {
+  ops_dat dat;
+  ops_stencil stencil;
+  int dim;
+  int elem_size;
+  char *data;
+  char *data_d;
+  ops_access acc;
+  ops_arg_type argtype;
+  int opt;
+} ops_arg;
+
+/* ---- lifecycle ---- */
+/* ops/c/include/ops_lib_core.h:601-608; argv tokens OPS_DIAGS=, OPS_BLOCK_SIZE_{X,Y,Z}=,
+ * OPS_TILING, OPS_DEVICE= are honoured as in ops/c/src/core/ops_lib_core.cpp:100-249 */
+void ops_init(const int argc, const char *const argv[], const int diags_level);
+void ops_exit(void);
+/* ops/c/include/ops_lib_core.h:1076-1077 (single GPU: no decomposition) */
+void ops_partition(const char *routine);
+
+/* ---- declarations ---- */
+/* ops/c/include/ops_lib_core.h:620-621 */
+ops_block ops_decl_block(int dims, const char *name);
+/* ops/c/include/ops_internal1.h (behind the ops_decl_dat<T> templates,
+ * ops/c/include/ops_lib_core.h:647-694); data may be NULL (zero-filled) or a
+ * host array laid out with halos, size[d] - d_m[d] + d_p[d] points per dim */
+ops_dat ops_decl_dat_char(ops_block block, int dim, int *size, int *base, int *d_m, int *d_p, int *stride,
+                          char *data, int type_size, const char *type, const char *name);
+/* ops/c/include/ops_lib_core.h:702-703 */
+void ops_free_dat(ops_dat dat);
+/* ops/c/include/ops_lib_core.h:824-852 */
+ops_stencil ops_decl_stencil(int dims, int points, int *stencil, const char *name);
+ops_stencil ops_decl_strided_stencil(int dims, int points, int *sten, int *stride, const char *name);
+/* ops/c/include/ops_lib_core.h:810-811 */
+ops_reduction ops_decl_reduction_handle(int size, const char *type, const char *name);
+/* ops/c/include/ops_lib_core.h:888-902 */
+ops_halo ops_decl_halo(ops_dat from, ops_dat to, int *iter_size, int *from_base, int *to_base, int *from_dir,
+                       int *to_dir);
+ops_halo_group ops_decl_halo_group(int nhalos, ops_halo *halos);
+/* ops/c/include/ops_internal1.h:99 (instance argument dropped: one instance) */
+void ops_decl_const_char(int dim, const char *type, int type_size, char *data, const char *name);
+
+/* ---- loop arguments: ops/c/include/ops_lib_core.h:732-796 ---- */
+ops_arg ops_arg_dat(ops_dat dat, int dim, ops_stencil stencil, const char *type, ops_access acc);
+ops_arg ops_arg_dat_opt(ops_dat dat, int dim, ops_stencil stencil, const char *type, ops_access acc, int flag);
+ops_arg ops_arg_idx(void);
+ops_arg ops_arg_reduce(ops_reduction handle, int dim, const char *type, ops_access acc);
+/* behind ops_arg_gbl<T> (ops/c/include/ops_lib_core.h:793-796); READ data is
+ * copied when the loop is queued (ops/c/src/core/ops_lazy.cpp:1433-1438) */
+ops_arg ops_arg_gbl_char(char *data, int dim, int size, ops_access acc);
+
+/* ---- synchronisation and data access ---- */
+/* ops/c/include/ops_lib_core.h:910-911 */
+void ops_halo_transfer(ops_halo_group group);
+/* ops/c/include/ops_lib_core.h:922-934, ops/c/src/core/ops_lib_core.cpp:1143-1149 */
+void ops_reduction_result_char(ops_reduction handle, int type_size, char *ptr);
+/* flushes queued loops (ops/c/src/core/ops_lazy.cpp:1248) */
+void ops_execute(void);
+/* ops/c/include/ops_lib_core.h:1104-1296 */
+void ops_dat_get_extents(ops_dat dat, int part, int *disp, int *size);
+void ops_dat_get_raw_metadata(ops_dat dat, int part, int *disp, int *size, int *stride, int *d_m, int *d_p);
+char *ops_dat_get_raw_pointer(ops_dat dat, int part, ops_stencil stencil, ops_memspace *memspace);
+void ops_dat_release_raw_data(ops_dat dat, int part, ops_access acc);
+void ops_dat_release_raw_data_memspace(ops_dat dat, int part, ops_access acc, ops_memspace *memspace);
+void ops_dat_fetch_data(ops_dat dat, int part, char *data);
+void ops_dat_fetch_data_memspace(ops_dat dat, int part, char *data, ops_memspace memspace);
+void ops_dat_fetch_data_slab_memspace(ops_dat dat, int part, char *data, int *range, ops_memspace memspace);
+/* interior-sized buffer, as the reference device path and the documentation
+ * specify (ops/c/src/core/ops_device_singlenode_common.cpp:227-255) */
+void ops_dat_set_data(ops_dat dat, int part, char *data);
+void ops_dat_set_data_memspace(ops_dat dat, int part, char *data, ops_memspace memspace);
+void ops_dat_set_data_slab_memspace(ops_dat dat, int part, char *data, int *range, ops_memspace memspace);
+int ops_dat_get_local_npartitions(ops_dat dat);
+int ops_dat_get_global_npartitions(ops_dat dat);
+
+/* ---- random fills: ops/c/include/ops_lib_core.h:1302-1309.  Counter-based
+ * (SplitMix64 finaliser) so values depend only on (seed, dat index, point),
+ * unlike the reference's std::mt19937 stream; fills the whole allocation ---- */
+void ops_randomgen_init(unsigned int seed, int options);
+void ops_fill_random_uniform(ops_dat dat);
+void ops_fill_random_normal(ops_dat dat);
+void ops_randomgen_exit(void);
+
+/* ---- utilities: ops/c/include/ops_lib_core.h:990-1061 ---- */
+void ops_timers(double *cpu, double *et);
+void ops_timers_core(double *cpu, double *et);
+void ops_printf(const char *format, ...);
+void ops_fprintf(FILE *stream, const char *format, ...);
+void ops_timing_output_stdout(void);
+void ops_diagnostic_output(void);
+int ops_is_root(void);
+
+/* ---- tridiagonal: ops/c/include/ops_tridiag.h:45-83.  The reference's
+ * ops_tridsolver_params is a C++ class; in C it is an opaque handle made by
+ * ops_tridsolver_params_create (the C++ class below wraps it). ---- */
+typedef struct ops_tridsolver_params_c ops_tridsolver_params_c;
+/* strategy: the reference SolveStrategy values; PCR (3) selects parallel
+ * cyclic reduction, every other value the Thomas algorithm */
+ops_tridsolver_params_c *ops_tridsolver_params_create(ops_block block, int strategy);
+void ops_tridsolver_params_destroy(ops_tridsolver_params_c *p);
+/* ops/c/include/ops_tridiag.h:73-76, ops/c/src/tridiag/ops_tridiag.cpp:78-160 */
+void ops_tridMultiDimBatch_c(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c, ops_dat d,
+                             ops_tridsolver_params_c *ctx);
+/* ops/c/include/ops_tridiag.h:78-81, ops/c/src/tridiag/ops_tridiag.cpp:168-249 */
+void ops_tridMultiDimBatch_Inc_c(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c,
+                                 ops_dat d, ops_dat u, ops_tridsolver_params_c *ctx);
+
+/* ==== hand-written per-loop entry points ============================== */
+
+/* Laplace tutorial kernels: apps/c/laplace2d_tutorial/step7/laplace_kernels.h
+ * (double or float dats, 2D) */
+/* :4-6  A(0,0) = 0 (any dims) */
+void ops_par_loop_set_zero(const char *name, ops_block block, int dim, int *range, ops_arg arg0);
+/* :8-10 A(0,0) = Anew(0,0) (any dims) */
+void ops_par_loop_copy(const char *name, ops_block block, int dim, int *range, ops_arg arg0, ops_arg arg1);
+/* :12-14, constants pi, jmax (ops_decl_const) */
+void ops_par_loop_left_bndcon(const char *name, ops_block block, int dim, int *range, ops_arg arg0, ops_arg arg1);
+/* :16-18 */
+void ops_par_loop_right_bndcon(const char *name, ops_block block, int dim, int *range, ops_arg arg0, ops_arg arg1);
+/* top edge u = sin(pi (i+1)/(imax+1)) (this project's Laplace problem) */
+void ops_par_loop_top_bndcon(const char *name, ops_block block, int dim, int *range, ops_arg arg0, ops_arg arg1);
+/* :20-24 Anew = 0.25 (A(1,0)+A(-1,0)+A(0,-1)+A(0,1)); err = max(err,|Anew-A|) */
+void ops_par_loop_apply_stencil(const char *name, ops_block block, int dim, int *range, ops_arg arg0,
+                                ops_arg arg1, ops_arg arg2);
+
+/* TeaLeaf kernels: apps/c/TeaLeaf/tea_leaf_kernels.h, tea_leaf_cg_kernels.h,
+ * tea_leaf_common_kernels.h (2D as in the reference; 3D appends z terms) */
+void ops_par_loop_tea_leaf_init_zero_kernel(const char *name, ops_block block, int dim, int *range, ops_arg p);
+void ops_par_loop_tea_leaf_init_zero2_kernel(const char *name, ops_block block, int dim, int *range, ops_arg p,
+                                             ops_arg z);
+void ops_par_loop_tea_leaf_yeqx_kernel(const char *name, ops_block block, int dim, int *range, ops_arg p,
+                                       ops_arg x);
+void ops_par_loop_tea_leaf_recip_kernel(const char *name, ops_block block, int dim, int *range, ops_arg u,
+                                        ops_arg p);
+void ops_par_loop_tea_leaf_dot_kernel(const char *name, ops_block block, int dim, int *range, ops_arg r,
+                                      ops_arg p, ops_arg rro);
+void ops_par_loop_tea_leaf_norm2_kernel(const char *name, ops_block block, int dim, int *range, ops_arg x,
+                                        ops_arg norm);
+void ops_par_loop_tea_leaf_axpy_kernel(const char *name, ops_block block, int dim, int *range, ops_arg u,
+                                       ops_arg p, ops_arg alpha);
+void ops_par_loop_tea_leaf_axpby_kernel(const char *name, ops_block block, int dim, int *range, ops_arg u,
+                                        ops_arg p, ops_arg alpha, ops_arg beta);
+void ops_par_loop_tea_leaf_zeqxty_kernel(const char *name, ops_block block, int dim, int *range, ops_arg z,
+                                         ops_arg x, ops_arg y);
+void ops_par_loop_tea_leaf_common_init_Kx_Ky_kernel(const char *name, ops_block block, int dim, int *range,
+                                                    ops_arg Kx, ops_arg Ky, ops_arg w);
+void ops_par_loop_tea_leaf_common_init_Kx_Ky_Kz_kernel(const char *name, ops_block block, int dim, int *range,
+                                                       ops_arg Kx, ops_arg Ky, ops_arg Kz, ops_arg w);
+void ops_par_loop_tea_leaf_common_init_kernel(const char *name, ops_block block, int dim, int *range, ops_arg w,
+                                              ops_arg r, ops_arg Kx, ops_arg Ky, ops_arg u, ops_arg rx,
+                                              ops_arg ry);
+void ops_par_loop_tea_leaf_common_init_kernel_3d(const char *name, ops_block block, int dim, int *range,
+                                                 ops_arg w, ops_arg r, ops_arg Kx, ops_arg Ky, ops_arg Kz,
+                                                 ops_arg u, ops_arg rx, ops_arg ry, ops_arg rz);
+void ops_par_loop_tea_leaf_cg_calc_w_reduce_kernel(const char *name, ops_block block, int dim, int *range,
+                                                   ops_arg w, ops_arg Kx, ops_arg Ky, ops_arg p, ops_arg rx,
+                                                   ops_arg ry, ops_arg pw);
+void ops_par_loop_tea_leaf_cg_calc_w_reduce_kernel_3d(const char *name, ops_block block, int dim, int *range,
+                                                      ops_arg w, ops_arg Kx, ops_arg Ky, ops_arg Kz, ops_arg p,
+                                                      ops_arg rx, ops_arg ry, ops_arg rz, ops_arg pw);
+void ops_par_loop_tea_leaf_cg_calc_ur_r_reduce_kernel(const char *name, ops_block block, int dim, int *range,
+                                                      ops_arg r, ops_arg w, ops_arg alpha, ops_arg rnn);
+
+/* constant-coefficient Poisson (this project's problem): 2D 5-point
+ * w = 4p - (p(-1,0)+p(1,0)) - (p(0,-1)+p(0,1)), 3D 7-point with 6p and z
+ * pair appended; double or float dats, pw always accumulated in double */
+void ops_par_loop_poisson_calc_w_reduce_kernel(const char *name, ops_block block, int dim, int *range,
+                                               ops_arg w, ops_arg p, ops_arg pw);
+
+/* ADI right-hand side and coefficients: apps/c/adi/preproc_kernel.h:4-39
+ * (constants nx, ny, nz, lambda) */
+void ops_par_loop_preproc_kernel(const char *name, ops_block block, int dim, int *range, ops_arg u, ops_arg du,
+                                 ops_arg ax, ops_arg bx, ops_arg cx, ops_arg ay, ops_arg by, ops_arg cy,
+                                 ops_arg az, ops_arg bz, ops_arg cz, ops_arg idx);
+/* 27-point weighted box sweep (weights 1/8,1/16,1/32,1/64 by Manhattan
+ * distance; summed z-outer, x-inner) */
+void ops_par_loop_stencil27_kernel(const char *name, ops_block block, int dim, int *range, ops_arg out,
+                                   ops_arg u);
+
+/* small reductions used by reduction-semantics tests (sum, sum of squares,
+ * sum of |x|, min, max, (sum, count) pair, max of i*j, sum of scale*x) */
+void ops_par_loop_red_sum(const char *name, ops_block block, int dim, int *range, ops_arg x, ops_arg s);
+void ops_par_loop_red_sumsq(const char *name, ops_block block, int dim, int *range, ops_arg x, ops_arg s);
+void ops_par_loop_red_abs(const char *name, ops_block block, int dim, int *range, ops_arg x, ops_arg s);
+void ops_par_loop_red_min(const char *name, ops_block block, int dim, int *range, ops_arg x, ops_arg m);
+void ops_par_loop_red_max(const char *name, ops_block block, int dim, int *range, ops_arg x, ops_arg m);
+void ops_par_loop_red_two(const char *name, ops_block block, int dim, int *range, ops_arg x, ops_arg s2);
+void ops_par_loop_red_imax(const char *name, ops_block block, int dim, int *range, ops_arg idx, ops_arg m);
+void ops_par_loop_red_scaled(const char *name, ops_block block, int dim, int *range, ops_arg x, ops_arg scale,
+                             ops_arg s);
+
+/* ==== backend extensions (no reference counterpart) =================== */
+/* 0: print and abort on error (default); 1: record and return */
+void ops_mi_set_error_mode(int mode);
+/* last error code (0 = none) and message; reading clears it */
+int ops_mi_last_error(char *msg, int msg_len);
+/* the library's HIP stream (hipStream_t) */
+void *ops_mi_stream(void);
+/* blocks until all queued device work is done */
+void ops_mi_sync(void);
+/* per-loop statistics gathered when diags > 1 (the reference's
+ * ops_timing_output table, ops/c/src/core/ops_lib_core.cpp:1468-1624):
+ * launches, summed kernel milliseconds (HIP events on the library stream) and
+ * algorithmic bytes by ops_compute_transfer's rule (:1687-1697).  Returns 0 if
+ * the name is unknown. */
+int ops_mi_loop_stats(const char *name, long *count, double *kernel_ms, double *bytes);
+void ops_mi_reset_stats(void);
+/* loop-chain fusion in the queue: 1 on (default), 0 off */
+void ops_mi_set_fusion(int on);
+
+#ifdef __cplusplus
+} /* extern "C" */
+
+/* ---- C++ conveniences with the reference's template signatures
+ *      (ops/c/include/ops_lib_core.h:647-694,793-796,922-934,947-975) ---- */
+#include <string.h>
+template <class T>
+static inline ops_dat ops_decl_dat(ops_block block, int data_size, int *block_size, int *base, int *d_m,
+                                   int *d_p, T *data, const char *type, const char *name) {
+  int stride[OPS_MAX_DIM] = {1, 1, 1, 1, 1};
+  return ops_decl_dat_char(block, data_size, block_size, base, d_m, d_p, stride, (char *)data, sizeof(T), type,
+                           name);
+}
+template <class T>
+static inline ops_arg ops_arg_gbl(T *data, int dim, const char *type, ops_access acc) {
+  (void)type;
+  return ops_arg_gbl_char((char *)data, dim, sizeof(T), acc);
+}
+template <class T>
+static inline void ops_reduction_result(ops_reduction handle, T *ptr) {
+  ops_reduction_result_char(handle, sizeof(T), (char *)ptr);
+}
+template <class T>
+static inline void ops_decl_const(const char *name, int dim, const char *type, T *data) {
+  ops_decl_const_char(dim, type, sizeof(T), (char *)data, name);
+}
+/* ops/c/include/ops_tridiag.h:45-71 */
+class ops_tridsolver_params {
+ public:
+  enum SolveStrategy { GATHER_SCATTER = 0, ALLGATHER, JACOBI, PCR, LATENCY_HIDING_INTERLEAVED,
+                       LATENCY_HIDING_TWO_STEP };
+  explicit ops_tridsolver_params(ops_block block) : tridsolver_params(ops_tridsolver_params_create(block, 0)) {}
+  ops_tridsolver_params(ops_block block, SolveStrategy s)
+      : tridsolver_params(ops_tridsolver_params_create(block, (int)s)) {}
+  ~ops_tridsolver_params() { ops_tridsolver_params_destroy(tridsolver_params); }
+  void set_jacobi_params(double, double, int) {}
+  void set_batch_size(int) {}
+  void set_cuda_opts(int, int, int) {}
+  void set_cuda_sync(int) {}
+  ops_tridsolver_params_c *tridsolver_params;
+};
+static inline void ops_tridMultiDimBatch(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c,
+                                         ops_dat d, ops_tridsolver_params *ctx) {
+  ops_tridMultiDimBatch_c(ndim, solvedim, range, a, b, c, d, ctx->tridsolver_params);
+}
+static inline void ops_tridMultiDimBatch_Inc(int ndim, int solvedim, int *range, ops_dat a, ops_dat b,
+                                             ops_dat c, ops_dat d, ops_dat u, ops_tridsolver_params *ctx) {
+  ops_tridMultiDimBatch_Inc_c(ndim, solvedim, range, a, b, c, d, u, ctx->tridsolver_params);
+}
+#endif
+
+#endif /* OPS_MI355X_H */

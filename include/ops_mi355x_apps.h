@@ -1,0 +1,57 @@
+/*
+ * ops_mi355x_apps.h — application drivers built on the OPS C ABI
+ * (libopsmi_apps.so).  They play the role of the reference's example apps and
+ * have no counterpart in the OPS library itself; each cites the reference app
+ * it follows.  Inputs are generated from the problem definitions in
+ * ops_amd/apps/problems.h when the caller passes NULL arrays.
+ */
+#ifndef OPS_MI355X_APPS_H
+#define OPS_MI355X_APPS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* TeaLeaf-form implicit heat CG (apps/c/TeaLeaf/tea_leaf_common.cpp:76-146,
+ * tea_leaf_cg.cpp:54-214, tea_solve.cpp:300-325), 2D or 3D.  K_ext holds the
+ * conductivity over the halo-extended box ((n+2) points per side), rhs the
+ * interior right-hand side.  create runs the setup loops and r0.p0. */
+void *opsapp_heat_create(int ndim, int nx, int ny, int nz, uint64_t seed, double rx, double ry, double rz,
+                         const double *K_ext, const double *rhs);
+/* up to max_iters CG iterations; stops once sqrt(r.r) <= tol sqrt(r0.r0)
+ * (tol <= 0: never).  Histories (may be NULL) get one entry per iteration;
+ * rrn_h[i] is r.r after iteration i.  Returns iterations run. */
+int opsapp_heat_iterate(void *ctx, int max_iters, double tol, double *pw_h, double *rrn_h, double *alpha_h,
+                        double *beta_h);
+double opsapp_heat_rr(void *ctx, int initial);
+void opsapp_heat_fetch_u(void *ctx, double *out);
+void opsapp_heat_destroy(void *ctx);
+
+/* constant-coefficient Poisson CG (5-point 2D / 7-point 3D, zero Dirichlet) */
+void *opsapp_poisson_create(int ndim, int nx, int ny, int nz, uint64_t seed, const double *f);
+int opsapp_poisson_iterate(void *ctx, int max_iters, double tol, double *rrn_h, double *alpha_h, double *beta_h);
+void opsapp_poisson_fetch_x(void *ctx, double *out);
+void opsapp_poisson_destroy(void *ctx);
+
+/* Laplace Jacobi (apps/c/laplace2d_tutorial/step7/laplace2d.cpp:37-139):
+ * mode 0 top-edge boundary, 1 the tutorial's left/right boundary; init_ext
+ * (halo-including) replaces the boundary loops when given.  Returns the last
+ * max-abs-diff. */
+double opsapp_laplace2d(int nx, int ny, int iters, int mode, const double *init_ext, double *err_hist,
+                        double *A_out);
+
+/* batched tridiagonal ADI sweeps (apps/c/adi/adi.cpp:344-424 with random
+ * coefficients); strategy 3 selects PCR */
+void *opsapp_trid_create(int nx, int ny, int nz, uint64_t seed, int strategy);
+void opsapp_trid_sweep(void *ctx, int inc);
+void opsapp_trid_solve(void *ctx, int solvedim, int inc);
+void opsapp_trid_fetch(void *ctx, double *d_out, double *u_out);
+void opsapp_trid_destroy(void *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

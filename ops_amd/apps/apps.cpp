@@ -1,0 +1,456 @@
+// Application drivers written only against the public C ABI
+// (include/ops_mi355x.h), the way the reference apps drive OPS:
+//   Laplace 2D Jacobi   apps/c/laplace2d_tutorial/step7/laplace2d.cpp:37-139
+//   TeaLeaf-form CG     apps/c/TeaLeaf/tea_leaf_common.cpp:76-146,
+//                       tea_leaf_cg.cpp:54-214, tea_solve.cpp:300-325 (2D; 3D
+//                       generalisation with the z pair appended)
+//   Poisson CG          constant-coefficient 5/7-point operator
+//   ADI tridiagonal     apps/c/adi/adi.cpp:344-424 (random coefficients)
+// Exported through include/ops_mi355x_apps.h for the benchmark and tests.
+#include "../../include/ops_mi355x_apps.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/ops_mi355x.h"
+#include "problems.h"
+
+using namespace opsapp;
+
+namespace {
+
+int S000[] = {0, 0, 0};
+int S7[] = {0, 0, 0, -1, 0, 0, 1, 0, 0, 0, -1, 0, 0, 1, 0, 0, 0, -1, 0, 0, 1};
+int S5[] = {0, 0, 1, 0, -1, 0, 0, 1, 0, -1};
+int S00[] = {0, 0};
+
+struct Geo {
+  int ndim, nx, ny, nz;
+  int full[6], inner[6], ext[6];
+  Geo(int d, int x, int y, int z) : ndim(d), nx(x), ny(y), nz(d == 3 ? z : 1) {
+    int n[3] = {nx, ny, nz};
+    for (int q = 0; q < 3; q++) {
+      full[2 * q] = -1;
+      full[2 * q + 1] = n[q] + 1;
+      inner[2 * q] = 0;
+      inner[2 * q + 1] = n[q];
+      ext[2 * q] = 0;
+      ext[2 * q + 1] = n[q] + 1;
+    }
+  }
+  size_t n_ext() const { return (size_t)(nx + 2) * (ny + 2) * (ndim == 3 ? nz + 2 : 1); }
+  size_t n_int() const { return (size_t)nx * ny * nz; }
+};
+
+ops_dat decl(ops_block b, const Geo &g, const char *name, const char *type = "double", int ts = 8) {
+  int size[3] = {g.nx, g.ny, g.nz}, base[3] = {0, 0, 0}, dm[3] = {-1, -1, -1}, dp[3] = {1, 1, 1};
+  int stride[3] = {1, 1, 1};
+  return ops_decl_dat_char(b, 1, size, base, dm, dp, stride, nullptr, ts, type, name);
+}
+
+ops_arg gbl(double *v) { return ops_arg_gbl_char((char *)v, 1, sizeof(double), OPS_READ); }
+
+// ---------------------------------------------------------------- heat CG
+struct Heat {
+  Geo g;
+  double rx, ry, rz;
+  ops_block blk;
+  ops_dat density, u, u0, w, Kx, Ky, Kz, r, p;
+  ops_stencil s0, sp, sx, sy, sz, sm;
+  ops_reduction red;
+  double rro = 0.0, rro0 = 0.0;
+  double one = 1.0;
+  Heat(int d, int x, int y, int z) : g(d, x, y, z) {}
+};
+
+}  // namespace
+
+extern "C" {
+
+void *opsapp_heat_create(int ndim, int nx, int ny, int nz, uint64_t seed, double rx, double ry, double rz,
+                         const double *K_ext, const double *rhs) {
+  Heat *h = new Heat(ndim, nx, ny, nz);
+  Geo &g = h->g;
+  h->rx = rx;
+  h->ry = ry;
+  h->rz = rz;
+  h->blk = ops_decl_block(ndim, "tea_grid");
+  h->density = decl(h->blk, g, "density");
+  h->u = decl(h->blk, g, "u");
+  h->u0 = decl(h->blk, g, "u0");
+  h->w = decl(h->blk, g, "w");
+  h->Kx = decl(h->blk, g, "Kx");
+  h->Ky = decl(h->blk, g, "Ky");
+  h->Kz = ndim == 3 ? decl(h->blk, g, "Kz") : nullptr;
+  h->r = decl(h->blk, g, "r");
+  h->p = decl(h->blk, g, "p");
+  if (ndim == 3) {
+    int sx[] = {0, 0, 0, 1, 0, 0}, sy[] = {0, 0, 0, 0, 1, 0}, sz[] = {0, 0, 0, 0, 0, 1};
+    int sm[] = {0, 0, 0, -1, 0, 0, 0, -1, 0, 0, 0, -1};
+    h->s0 = ops_decl_stencil(3, 1, S000, "S3D_000");
+    h->sp = ops_decl_stencil(3, 7, S7, "S3D_7PT");
+    h->sx = ops_decl_stencil(3, 2, sx, "S3D_000_P100");
+    h->sy = ops_decl_stencil(3, 2, sy, "S3D_000_0P10");
+    h->sz = ops_decl_stencil(3, 2, sz, "S3D_000_00P1");
+    h->sm = ops_decl_stencil(3, 4, sm, "S3D_000_M100_0M10_00M1");
+  } else {
+    int sx[] = {0, 0, 1, 0}, sy[] = {0, 0, 0, 1}, sm[] = {0, 0, -1, 0, 0, -1};
+    h->s0 = ops_decl_stencil(2, 1, S00, "S2D_00");
+    h->sp = ops_decl_stencil(2, 5, S5, "S2D_00_0M1_M10_P10_0P1");
+    h->sx = ops_decl_stencil(2, 2, sx, "S2D_00_P10");
+    h->sy = ops_decl_stencil(2, 2, sy, "S2D_00_0P1");
+    h->sz = nullptr;
+    h->sm = ops_decl_stencil(2, 3, sm, "S2D_00_M10_0M1");
+  }
+  h->red = ops_decl_reduction_handle(sizeof(double), "double", "red_temp");
+
+  // inputs: given, or generated from the problem definition
+  std::vector<double> Kv, fv;
+  if (!K_ext || !rhs) {
+    Kv.resize(g.n_ext());
+    fv.resize(g.n_int());
+    int kl = ndim == 3 ? -1 : 0, kh = ndim == 3 ? nz : 0;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = kl; k <= kh; k++)
+      for (int j = -1; j <= ny; j++)
+        for (int i = -1; i <= nx; i++)
+          Kv[(size_t)(i + 1) + (size_t)(nx + 2) * ((j + 1) + (size_t)(ny + 2) * (k - kl))] =
+              heat_K(seed, i, j, k, nx, ny);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < g.nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) fv[lin(i, j, k, nx, ny)] = heat_rhs(seed, i, j, k, nx, ny, g.nz, ndim);
+    K_ext = Kv.data();
+    rhs = fv.data();
+  }
+  ops_dat_set_data_slab_memspace(h->density, 0, (char *)K_ext, g.full, OPS_HOST);
+  ops_dat_set_data(h->u0, 0, (char *)rhs);
+  Kv.clear();
+  Kv.shrink_to_fit();
+  fv.clear();
+  fv.shrink_to_fit();
+
+  ops_block b = h->blk;
+  int d = ndim;
+  ops_par_loop_tea_leaf_yeqx_kernel("tea_leaf_yeqx_kernel", b, d, g.inner, ops_arg_dat(h->u, 1, h->s0, "double", OPS_WRITE),
+                                    ops_arg_dat(h->u0, 1, h->s0, "double", OPS_READ));
+  // tea_leaf_common_init, RECIP_CONDUCTIVITY, halo depth 1
+  ops_par_loop_tea_leaf_recip_kernel("tea_leaf_recip_kernel", b, d, g.full, ops_arg_dat(h->w, 1, h->s0, "double", OPS_WRITE),
+                                     ops_arg_dat(h->density, 1, h->s0, "double", OPS_READ));
+  if (ndim == 3)
+    ops_par_loop_tea_leaf_common_init_Kx_Ky_Kz_kernel(
+        "tea_leaf_common_init_Kx_Ky_Kz_kernel", b, d, g.ext, ops_arg_dat(h->Kx, 1, h->s0, "double", OPS_WRITE),
+        ops_arg_dat(h->Ky, 1, h->s0, "double", OPS_WRITE), ops_arg_dat(h->Kz, 1, h->s0, "double", OPS_WRITE),
+        ops_arg_dat(h->w, 1, h->sm, "double", OPS_READ));
+  else
+    ops_par_loop_tea_leaf_common_init_Kx_Ky_kernel(
+        "tea_leaf_common_init_Kx_Ky_kernel", b, d, g.ext, ops_arg_dat(h->Kx, 1, h->s0, "double", OPS_WRITE),
+        ops_arg_dat(h->Ky, 1, h->s0, "double", OPS_WRITE), ops_arg_dat(h->w, 1, h->sm, "double", OPS_READ));
+  // zeroed external faces
+  ops_dat K3[3] = {h->Kx, h->Ky, h->Kz};
+  for (int q = 0; q < ndim; q++) {
+    int lo[6], hi[6];
+    memcpy(lo, g.full, sizeof(lo));
+    memcpy(hi, g.full, sizeof(hi));
+    lo[2 * q] = -1;
+    lo[2 * q + 1] = 1;
+    int n = q == 0 ? nx : (q == 1 ? ny : nz);
+    hi[2 * q] = n;
+    hi[2 * q + 1] = n + 1;
+    ops_par_loop_tea_leaf_init_zero_kernel("tea_leaf_init_zero_kernel", b, d, lo, ops_arg_dat(K3[q], 1, h->s0, "double", OPS_WRITE));
+    ops_par_loop_tea_leaf_init_zero_kernel("tea_leaf_init_zero_kernel", b, d, hi, ops_arg_dat(K3[q], 1, h->s0, "double", OPS_WRITE));
+  }
+  if (ndim == 3)
+    ops_par_loop_tea_leaf_common_init_kernel_3d(
+        "tea_leaf_common_init_kernel", b, d, g.inner, ops_arg_dat(h->w, 1, h->s0, "double", OPS_WRITE),
+        ops_arg_dat(h->r, 1, h->s0, "double", OPS_WRITE), ops_arg_dat(h->Kx, 1, h->sx, "double", OPS_READ),
+        ops_arg_dat(h->Ky, 1, h->sy, "double", OPS_READ), ops_arg_dat(h->Kz, 1, h->sz, "double", OPS_READ),
+        ops_arg_dat(h->u, 1, h->sp, "double", OPS_READ), gbl(&h->rx), gbl(&h->ry), gbl(&h->rz));
+  else
+    ops_par_loop_tea_leaf_common_init_kernel(
+        "tea_leaf_common_init_kernel", b, d, g.inner, ops_arg_dat(h->w, 1, h->s0, "double", OPS_WRITE),
+        ops_arg_dat(h->r, 1, h->s0, "double", OPS_WRITE), ops_arg_dat(h->Kx, 1, h->sx, "double", OPS_READ),
+        ops_arg_dat(h->Ky, 1, h->sy, "double", OPS_READ), ops_arg_dat(h->u, 1, h->sp, "double", OPS_READ),
+        gbl(&h->rx), gbl(&h->ry));
+  // tea_leaf_cg_init without preconditioner
+  ops_par_loop_tea_leaf_yeqx_kernel("tea_leaf_yeqx_kernel", b, d, g.inner, ops_arg_dat(h->p, 1, h->s0, "double", OPS_WRITE),
+                                    ops_arg_dat(h->r, 1, h->s0, "double", OPS_READ));
+  ops_par_loop_tea_leaf_dot_kernel("tea_leaf_dot_kernel", b, d, g.inner, ops_arg_dat(h->r, 1, h->s0, "double", OPS_READ),
+                                   ops_arg_dat(h->p, 1, h->s0, "double", OPS_READ),
+                                   ops_arg_reduce(h->red, 1, "double", OPS_INC));
+  ops_reduction_result_char(h->red, sizeof(double), (char *)&h->rro);
+  h->rro0 = h->rro;
+  return h;
+}
+
+int opsapp_heat_iterate(void *ctx, int max_iters, double tol, double *pw_h, double *rrn_h, double *alpha_h,
+                        double *beta_h) {
+  Heat *h = (Heat *)ctx;
+  Geo &g = h->g;
+  ops_block b = h->blk;
+  int d = g.ndim;
+  const double stop = tol > 0 ? tol * sqrt(h->rro0) : -1.0;
+  int n = 0;
+  while (n < max_iters) {
+    double pw, rrn;
+    if (d == 3)
+      ops_par_loop_tea_leaf_cg_calc_w_reduce_kernel_3d(
+          "tea_leaf_cg_calc_w_reduce_kernel", b, d, g.inner, ops_arg_dat(h->w, 1, h->s0, "double", OPS_WRITE),
+          ops_arg_dat(h->Kx, 1, h->sx, "double", OPS_READ), ops_arg_dat(h->Ky, 1, h->sy, "double", OPS_READ),
+          ops_arg_dat(h->Kz, 1, h->sz, "double", OPS_READ), ops_arg_dat(h->p, 1, h->sp, "double", OPS_READ),
+          gbl(&h->rx), gbl(&h->ry), gbl(&h->rz), ops_arg_reduce(h->red, 1, "double", OPS_INC));
+    else
+      ops_par_loop_tea_leaf_cg_calc_w_reduce_kernel(
+          "tea_leaf_cg_calc_w_reduce_kernel", b, d, g.inner, ops_arg_dat(h->w, 1, h->s0, "double", OPS_WRITE),
+          ops_arg_dat(h->Kx, 1, h->sx, "double", OPS_READ), ops_arg_dat(h->Ky, 1, h->sy, "double", OPS_READ),
+          ops_arg_dat(h->p, 1, h->sp, "double", OPS_READ), gbl(&h->rx), gbl(&h->ry),
+          ops_arg_reduce(h->red, 1, "double", OPS_INC));
+    ops_reduction_result_char(h->red, sizeof(double), (char *)&pw);
+    double alpha = h->rro / pw;
+    ops_par_loop_tea_leaf_axpy_kernel("tea_leaf_axpy_kernel", b, d, g.inner, ops_arg_dat(h->u, 1, h->s0, "double", OPS_INC),
+                                      ops_arg_dat(h->p, 1, h->s0, "double", OPS_READ), gbl(&alpha));
+    ops_par_loop_tea_leaf_cg_calc_ur_r_reduce_kernel(
+        "tea_leaf_cg_calc_ur_r_reduce_kernel", b, d, g.inner, ops_arg_dat(h->r, 1, h->s0, "double", OPS_INC),
+        ops_arg_dat(h->w, 1, h->s0, "double", OPS_READ), gbl(&alpha), ops_arg_reduce(h->red, 1, "double", OPS_INC));
+    ops_reduction_result_char(h->red, sizeof(double), (char *)&rrn);
+    double beta = rrn / h->rro;
+    ops_par_loop_tea_leaf_axpby_kernel("tea_leaf_axpby_kernel", b, d, g.inner, ops_arg_dat(h->p, 1, h->s0, "double", OPS_RW),
+                                       ops_arg_dat(h->r, 1, h->s0, "double", OPS_READ), gbl(&beta), gbl(&h->one));
+    if (pw_h) pw_h[n] = pw;
+    if (alpha_h) alpha_h[n] = alpha;
+    if (beta_h) beta_h[n] = beta;
+    if (rrn_h) rrn_h[n] = rrn;
+    h->rro = rrn;
+    n++;
+    if (sqrt(rrn) <= stop) break;
+  }
+  return n;
+}
+
+double opsapp_heat_rr(void *ctx, int initial) { return initial ? ((Heat *)ctx)->rro0 : ((Heat *)ctx)->rro; }
+
+void opsapp_heat_fetch_u(void *ctx, double *out) { ops_dat_fetch_data(((Heat *)ctx)->u, 0, (char *)out); }
+
+void opsapp_heat_destroy(void *ctx) {
+  Heat *h = (Heat *)ctx;
+  ops_dat all[9] = {h->density, h->u, h->u0, h->w, h->Kx, h->Ky, h->Kz, h->r, h->p};
+  for (ops_dat q : all)
+    if (q) ops_free_dat(q);
+  delete h;
+}
+
+// ---------------------------------------------------------------- Poisson CG
+struct Poisson {
+  Geo g;
+  ops_block blk;
+  ops_dat x, r, p, w;
+  ops_stencil s0, sp;
+  ops_reduction red;
+  double rro = 0, rro0 = 0, one = 1.0;
+  Poisson(int d, int a, int b, int c) : g(d, a, b, c) {}
+};
+
+void *opsapp_poisson_create(int ndim, int nx, int ny, int nz, uint64_t seed, const double *f) {
+  Poisson *P = new Poisson(ndim, nx, ny, nz);
+  Geo &g = P->g;
+  P->blk = ops_decl_block(ndim, "poisson");
+  P->x = decl(P->blk, g, "x");
+  P->r = decl(P->blk, g, "r");
+  P->p = decl(P->blk, g, "p");
+  P->w = decl(P->blk, g, "w");
+  P->s0 = ndim == 3 ? ops_decl_stencil(3, 1, S000, "000") : ops_decl_stencil(2, 1, S00, "00");
+  P->sp = ndim == 3 ? ops_decl_stencil(3, 7, S7, "7pt") : ops_decl_stencil(2, 5, S5, "5pt");
+  P->red = ops_decl_reduction_handle(sizeof(double), "double", "red");
+  std::vector<double> fv;
+  if (!f) {
+    fv.resize(g.n_int());
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < g.nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) fv[lin(i, j, k, nx, ny)] = poisson_rhs(seed, i, j, k, nx, ny);
+    f = fv.data();
+  }
+  ops_dat_set_data(P->r, 0, (char *)f);
+  int d = ndim;
+  ops_par_loop_tea_leaf_yeqx_kernel("tea_leaf_yeqx_kernel", P->blk, d, g.inner, ops_arg_dat(P->p, 1, P->s0, "double", OPS_WRITE),
+                                    ops_arg_dat(P->r, 1, P->s0, "double", OPS_READ));
+  ops_par_loop_tea_leaf_dot_kernel("tea_leaf_dot_kernel", P->blk, d, g.inner, ops_arg_dat(P->r, 1, P->s0, "double", OPS_READ),
+                                   ops_arg_dat(P->p, 1, P->s0, "double", OPS_READ),
+                                   ops_arg_reduce(P->red, 1, "double", OPS_INC));
+  ops_reduction_result_char(P->red, sizeof(double), (char *)&P->rro);
+  P->rro0 = P->rro;
+  return P;
+}
+
+int opsapp_poisson_iterate(void *ctx, int max_iters, double tol, double *rrn_h, double *alpha_h, double *beta_h) {
+  Poisson *P = (Poisson *)ctx;
+  Geo &g = P->g;
+  int d = g.ndim;
+  const double stop = tol > 0 ? tol * sqrt(P->rro0) : -1.0;
+  int n = 0;
+  while (n < max_iters) {
+    double pw, rrn;
+    ops_par_loop_poisson_calc_w_reduce_kernel("poisson_calc_w_reduce_kernel", P->blk, d, g.inner,
+                                              ops_arg_dat(P->w, 1, P->s0, "double", OPS_WRITE),
+                                              ops_arg_dat(P->p, 1, P->sp, "double", OPS_READ),
+                                              ops_arg_reduce(P->red, 1, "double", OPS_INC));
+    ops_reduction_result_char(P->red, sizeof(double), (char *)&pw);
+    double alpha = P->rro / pw;
+    ops_par_loop_tea_leaf_axpy_kernel("tea_leaf_axpy_kernel", P->blk, d, g.inner, ops_arg_dat(P->x, 1, P->s0, "double", OPS_INC),
+                                      ops_arg_dat(P->p, 1, P->s0, "double", OPS_READ), gbl(&alpha));
+    ops_par_loop_tea_leaf_cg_calc_ur_r_reduce_kernel(
+        "tea_leaf_cg_calc_ur_r_reduce_kernel", P->blk, d, g.inner, ops_arg_dat(P->r, 1, P->s0, "double", OPS_INC),
+        ops_arg_dat(P->w, 1, P->s0, "double", OPS_READ), gbl(&alpha), ops_arg_reduce(P->red, 1, "double", OPS_INC));
+    ops_reduction_result_char(P->red, sizeof(double), (char *)&rrn);
+    double beta = rrn / P->rro;
+    ops_par_loop_tea_leaf_axpby_kernel("tea_leaf_axpby_kernel", P->blk, d, g.inner, ops_arg_dat(P->p, 1, P->s0, "double", OPS_RW),
+                                       ops_arg_dat(P->r, 1, P->s0, "double", OPS_READ), gbl(&beta), gbl(&P->one));
+    if (alpha_h) alpha_h[n] = alpha;
+    if (beta_h) beta_h[n] = beta;
+    if (rrn_h) rrn_h[n] = rrn;
+    P->rro = rrn;
+    n++;
+    if (sqrt(rrn) <= stop) break;
+  }
+  return n;
+}
+
+void opsapp_poisson_fetch_x(void *ctx, double *out) { ops_dat_fetch_data(((Poisson *)ctx)->x, 0, (char *)out); }
+
+void opsapp_poisson_destroy(void *ctx) {
+  Poisson *P = (Poisson *)ctx;
+  ops_free_dat(P->x);
+  ops_free_dat(P->r);
+  ops_free_dat(P->p);
+  ops_free_dat(P->w);
+  delete P;
+}
+
+// ---------------------------------------------------------------- Laplace
+double opsapp_laplace2d(int nx, int ny, int iters, int mode, const double *init_ext, double *err_hist,
+                        double *A_out) {
+  Geo g(2, nx, ny, 1);
+  ops_block blk = ops_decl_block(2, "my_grid");
+  ops_dat A = decl(blk, g, "A"), An = decl(blk, g, "Anew");
+  ops_stencil s00 = ops_decl_stencil(2, 1, S00, "00");
+  ops_stencil s5 = ops_decl_stencil(2, 5, S5, "5pt");
+  ops_reduction h_err = ops_decl_reduction_handle(sizeof(double), "double", "error");
+  double pi = 2.0 * asin(1.0);
+  ops_decl_const_char(1, "int", sizeof(int), (char *)&nx, "imax");
+  ops_decl_const_char(1, "int", sizeof(int), (char *)&ny, "jmax");
+  ops_decl_const_char(1, "double", sizeof(double), (char *)&pi, "pi");
+  ops_dat both[2] = {A, An};
+  if (init_ext) {
+    for (ops_dat q : both) ops_dat_set_data_slab_memspace(q, 0, (char *)init_ext, g.full, OPS_HOST);
+  } else {
+    int full[] = {-1, nx + 1, -1, ny + 1}, top[] = {-1, nx + 1, ny, ny + 1};
+    int left[] = {-1, 0, -1, ny + 1}, right[] = {nx, nx + 1, -1, ny + 1};
+    for (ops_dat q : both) {
+      ops_par_loop_set_zero("set zero", blk, 2, full, ops_arg_dat(q, 1, s00, "double", OPS_WRITE));
+      if (mode == 0) {
+        ops_par_loop_top_bndcon("top bndcon", blk, 2, top, ops_arg_dat(q, 1, s00, "double", OPS_WRITE), ops_arg_idx());
+      } else {
+        ops_par_loop_left_bndcon("left bndcon", blk, 2, left, ops_arg_dat(q, 1, s00, "double", OPS_WRITE), ops_arg_idx());
+        ops_par_loop_right_bndcon("right bndcon", blk, 2, right, ops_arg_dat(q, 1, s00, "double", OPS_WRITE), ops_arg_idx());
+      }
+    }
+  }
+  int interior[] = {0, nx, 0, ny};
+  double error = 1.0;
+  for (int it = 0; it < iters; it++) {
+    ops_par_loop_apply_stencil("apply stencil", blk, 2, interior, ops_arg_dat(A, 1, s5, "double", OPS_READ),
+                               ops_arg_dat(An, 1, s00, "double", OPS_WRITE),
+                               ops_arg_reduce(h_err, 1, "double", OPS_MAX));
+    ops_reduction_result_char(h_err, sizeof(double), (char *)&error);
+    ops_par_loop_copy("copy", blk, 2, interior, ops_arg_dat(A, 1, s00, "double", OPS_WRITE),
+                      ops_arg_dat(An, 1, s00, "double", OPS_READ));
+    if (err_hist) err_hist[it] = error;
+  }
+  if (A_out) ops_dat_fetch_data(A, 0, (char *)A_out);
+  ops_free_dat(A);
+  ops_free_dat(An);
+  return error;
+}
+
+// ---------------------------------------------------------------- tridiagonal
+struct Trid {
+  Geo g;
+  ops_block blk;
+  ops_dat a[3], b[3], c[3], d, u;
+  ops_tridsolver_params_c *ctx;
+  Trid(int x, int y, int z) : g(3, x, y, z) {}
+};
+
+void *opsapp_trid_create(int nx, int ny, int nz, uint64_t seed, int strategy) {
+  Trid *T = new Trid(nx, ny, nz);
+  Geo &g = T->g;
+  T->blk = ops_decl_block(3, "adi");
+  const char *an[3] = {"ax", "ay", "az"}, *bn[3] = {"bx", "by", "bz"}, *cn[3] = {"cx", "cy", "cz"};
+  size_t n = g.n_int();
+  std::vector<double> va(n), vb(n), vc(n);
+  for (int s = 0; s < 3; s++) {
+    T->a[s] = decl(T->blk, g, an[s]);
+    T->b[s] = decl(T->blk, g, bn[s]);
+    T->c[s] = decl(T->blk, g, cn[s]);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+          size_t l = lin(i, j, k, nx, ny);
+          trid_coefs(seed, s, i, j, k, nx, ny, &va[l], &vb[l], &vc[l]);
+        }
+    ops_dat_set_data(T->a[s], 0, (char *)va.data());
+    ops_dat_set_data(T->b[s], 0, (char *)vb.data());
+    ops_dat_set_data(T->c[s], 0, (char *)vc.data());
+  }
+  T->d = decl(T->blk, g, "du");
+  T->u = decl(T->blk, g, "u");
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) va[lin(i, j, k, nx, ny)] = trid_rhs(seed, i, j, k, nx, ny);
+  ops_dat_set_data(T->d, 0, (char *)va.data());
+  T->ctx = ops_tridsolver_params_create(T->blk, strategy);
+  return T;
+}
+
+// one ADI sweep: solves along x, then y, then z (z accumulating into u when inc)
+void opsapp_trid_sweep(void *ctx, int inc) {
+  Trid *T = (Trid *)ctx;
+  int *r = T->g.inner;
+  ops_tridMultiDimBatch_c(3, 0, r, T->a[0], T->b[0], T->c[0], T->d, T->ctx);
+  ops_tridMultiDimBatch_c(3, 1, r, T->a[1], T->b[1], T->c[1], T->d, T->ctx);
+  if (inc) ops_tridMultiDimBatch_Inc_c(3, 2, r, T->a[2], T->b[2], T->c[2], T->d, T->u, T->ctx);
+  else ops_tridMultiDimBatch_c(3, 2, r, T->a[2], T->b[2], T->c[2], T->d, T->ctx);
+}
+
+void opsapp_trid_solve(void *ctx, int solvedim, int inc) {
+  Trid *T = (Trid *)ctx;
+  int s = solvedim;
+  if (inc) ops_tridMultiDimBatch_Inc_c(3, s, T->g.inner, T->a[s], T->b[s], T->c[s], T->d, T->u, T->ctx);
+  else ops_tridMultiDimBatch_c(3, s, T->g.inner, T->a[s], T->b[s], T->c[s], T->d, T->ctx);
+}
+
+void opsapp_trid_fetch(void *ctx, double *d_out, double *u_out) {
+  Trid *T = (Trid *)ctx;
+  if (d_out) ops_dat_fetch_data(T->d, 0, (char *)d_out);
+  if (u_out) ops_dat_fetch_data(T->u, 0, (char *)u_out);
+}
+
+void opsapp_trid_destroy(void *ctx) {
+  Trid *T = (Trid *)ctx;
+  for (int s = 0; s < 3; s++) {
+    ops_free_dat(T->a[s]);
+    ops_free_dat(T->b[s]);
+    ops_free_dat(T->c[s]);
+  }
+  ops_free_dat(T->d);
+  ops_free_dat(T->u);
+  ops_tridsolver_params_destroy(T->ctx);
+  delete T;
+}
+
+}  // extern "C"

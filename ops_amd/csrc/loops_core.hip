@@ -1,0 +1,680 @@
+// Hand-written ops_par_loop kernels: Laplace tutorial, TeaLeaf CG family,
+// constant-coefficient Poisson, ADI preproc, 27-point sweep and the small
+// reduction loops.  Each functor restates the arithmetic of the reference
+// kernel it names, in the same association order (compiled with FP
+// contraction off), so results match the reference sequential backend bit
+// for bit; sums differ only by reduction order.
+#include "engine.h"
+
+using namespace opsmi;
+
+namespace {
+
+// ------------------------------------------------------------ Laplace family
+// apps/c/laplace2d_tutorial/step7/laplace_kernels.h:4-24
+template <class T>
+struct SetZero {
+  OPSMI_NO_RED
+  Ptr<T> a;
+  __device__ void operator()(int x, int y, int z, double (&)[1]) const { a(x, y, z) = T(0.0); }
+};
+template <class T>
+struct Copy {
+  OPSMI_NO_RED
+  Ptr<T> a;
+  Ptr<const T> b;
+  __device__ void operator()(int x, int y, int z, double (&)[1]) const { a(x, y, z) = b(x, y, z); }
+};
+// mode 0 left: sin(pi (y+1)/(jmax+1)); 1 right: same * exp(-pi);
+// 2 top: sin(pi (x+1)/(imax+1))
+template <class T>
+struct Bndcon {
+  OPSMI_NO_RED
+  Ptr<T> a;
+  double pi;
+  int n;
+  int mode;
+  __device__ void operator()(int x, int y, int z, double (&)[1]) const {
+    double v;
+    if (mode == 2) v = sin(pi * (x + 1) / (n + 1));
+    else {
+      v = sin(pi * (y + 1) / (n + 1));
+      if (mode == 1) v = v * exp(-pi);
+    }
+    a(x, y, z) = (T)v;
+  }
+};
+template <class T>
+struct ApplyStencil {
+  using red_t = T;
+  static constexpr int red_op = OPS_MAX;
+  static constexpr int red_dim = 1;
+  Ptr<const T> A;
+  Ptr<T> An;
+  __device__ void operator()(int x, int y, int z, T (&err)[1]) const {
+    const T v = (T)0.25f * (A(x + 1, y, z) + A(x - 1, y, z) + A(x, y - 1, z) + A(x, y + 1, z));
+    An(x, y, z) = v;
+    err[0] = RedOp<T, OPS_MAX>::combine(err[0], fabs(v - A(x, y, z)));
+  }
+};
+
+// ------------------------------------------------------------ TeaLeaf family
+// apps/c/TeaLeaf/tea_leaf_kernels.h:8-62, tea_leaf_cg_kernels.h:8-21,
+// tea_leaf_common_kernels.h:12-25
+struct InitZero2 {
+  OPSMI_NO_RED
+  Ptr<double> p, z2;
+  __device__ void operator()(int x, int y, int z, double (&)[1]) const {
+    p(x, y, z) = 0.0;
+    z2(x, y, z) = 0.0;
+  }
+};
+struct Recip {
+  OPSMI_NO_RED
+  Ptr<double> u;
+  Ptr<const double> p;
+  __device__ void operator()(int x, int y, int z, double (&)[1]) const { u(x, y, z) = 1.0 / p(x, y, z); }
+};
+struct Dot {
+  using red_t = double;
+  static constexpr int red_op = OPS_INC;
+  static constexpr int red_dim = 1;
+  Ptr<const double> r, p;
+  __device__ void operator()(int x, int y, int z, double (&s)[1]) const { s[0] = s[0] + r(x, y, z) * p(x, y, z); }
+};
+struct Norm2 {
+  using red_t = double;
+  static constexpr int red_op = OPS_INC;
+  static constexpr int red_dim = 1;
+  Ptr<const double> a;
+  __device__ void operator()(int x, int y, int z, double (&s)[1]) const {
+    s[0] = s[0] + a(x, y, z) * a(x, y, z);
+  }
+};
+struct Axpy {
+  OPSMI_NO_RED
+  Ptr<double> u;
+  Ptr<const double> p;
+  double alpha;
+  __device__ void operator()(int x, int y, int z, double (&)[1]) const { u(x, y, z) = u(x, y, z) + alpha * p(x, y, z); }
+};
+struct Axpby {
+  OPSMI_NO_RED
+  Ptr<double> u;
+  Ptr<const double> p;
+  double alpha, beta;
+  __device__ void operator()(int x, int y, int z, double (&)[1]) const {
+    u(x, y, z) = alpha * u(x, y, z) + beta * p(x, y, z);
+  }
+};
+struct Zeqxty {
+  OPSMI_NO_RED
+  Ptr<double> zz;
+  Ptr<const double> a, b;
+  __device__ void operator()(int x, int y, int z, double (&)[1]) const { zz(x, y, z) = a(x, y, z) * b(x, y, z); }
+};
+template <int ND>
+struct InitK {
+  OPSMI_NO_RED
+  Ptr<double> Kx, Ky, Kz;
+  Ptr<const double> w;
+  __device__ void operator()(int x, int y, int z, double (&)[1]) const {
+    const double c = w(x, y, z);
+    Kx(x, y, z) = (w(x - 1, y, z) + c) / (2.0 * w(x - 1, y, z) * c);
+    Ky(x, y, z) = (w(x, y - 1, z) + c) / (2.0 * w(x, y - 1, z) * c);
+    if constexpr (ND == 3) Kz(x, y, z) = (w(x, y, z - 1) + c) / (2.0 * w(x, y, z - 1) * c);
+  }
+};
+// the TeaLeaf operator; 3D appends the z pair after the x pair
+template <int ND>
+__device__ __forceinline__ double heat_op(const Ptr<const double> &Kx, const Ptr<const double> &Ky,
+                                          const Ptr<const double> &Kz, const Ptr<const double> &p, double rx,
+                                          double ry, double rz, int x, int y, int z) {
+  if constexpr (ND == 2) {
+    return (1.0 + ry * (Ky(x, y + 1, z) + Ky(x, y, z)) + rx * (Kx(x + 1, y, z) + Kx(x, y, z))) * p(x, y, z) -
+           ry * (Ky(x, y + 1, z) * p(x, y + 1, z) + Ky(x, y, z) * p(x, y - 1, z)) -
+           rx * (Kx(x + 1, y, z) * p(x + 1, y, z) + Kx(x, y, z) * p(x - 1, y, z));
+  } else {
+    return (1.0 + ry * (Ky(x, y + 1, z) + Ky(x, y, z)) + rx * (Kx(x + 1, y, z) + Kx(x, y, z)) +
+            rz * (Kz(x, y, z + 1) + Kz(x, y, z))) * p(x, y, z) -
+           ry * (Ky(x, y + 1, z) * p(x, y + 1, z) + Ky(x, y, z) * p(x, y - 1, z)) -
+           rx * (Kx(x + 1, y, z) * p(x + 1, y, z) + Kx(x, y, z) * p(x - 1, y, z)) -
+           rz * (Kz(x, y, z + 1) * p(x, y, z + 1) + Kz(x, y, z) * p(x, y, z - 1));
+  }
+}
+template <int ND>
+struct CommonInit {
+  OPSMI_NO_RED
+  Ptr<double> w, r;
+  Ptr<const double> Kx, Ky, Kz, u;
+  double rx, ry, rz;
+  __device__ void operator()(int x, int y, int z, double (&)[1]) const {
+    const double v = heat_op<ND>(Kx, Ky, Kz, u, rx, ry, rz, x, y, z);
+    w(x, y, z) = v;
+    r(x, y, z) = u(x, y, z) - v;
+  }
+};
+template <int ND>
+struct CalcW {
+  using red_t = double;
+  static constexpr int red_op = OPS_INC;
+  static constexpr int red_dim = 1;
+  Ptr<double> w;
+  Ptr<const double> Kx, Ky, Kz, p;
+  double rx, ry, rz;
+  __device__ void operator()(int x, int y, int z, double (&s)[1]) const {
+    const double v = heat_op<ND>(Kx, Ky, Kz, p, rx, ry, rz, x, y, z);
+    w(x, y, z) = v;
+    s[0] = s[0] + v * p(x, y, z);
+  }
+};
+struct CalcUrR {
+  using red_t = double;
+  static constexpr int red_op = OPS_INC;
+  static constexpr int red_dim = 1;
+  Ptr<double> r;
+  Ptr<const double> w;
+  double alpha;
+  __device__ void operator()(int x, int y, int z, double (&s)[1]) const {
+    const double v = r(x, y, z) - alpha * w(x, y, z);
+    r(x, y, z) = v;
+    s[0] = s[0] + v * v;
+  }
+};
+
+// ------------------------------------------------------------ Poisson
+template <class T, int ND>
+struct PoissonW {
+  using red_t = double;
+  static constexpr int red_op = OPS_INC;
+  static constexpr int red_dim = 1;
+  Ptr<T> w;
+  Ptr<const T> p;
+  __device__ void operator()(int x, int y, int z, double (&s)[1]) const {
+    T v;
+    if constexpr (ND == 2)
+      v = (T)4.0 * p(x, y, z) - (p(x - 1, y, z) + p(x + 1, y, z)) - (p(x, y - 1, z) + p(x, y + 1, z));
+    else
+      v = (T)6.0 * p(x, y, z) - (p(x - 1, y, z) + p(x + 1, y, z)) - (p(x, y - 1, z) + p(x, y + 1, z)) -
+          (p(x, y, z - 1) + p(x, y, z + 1));
+    w(x, y, z) = v;
+    s[0] = s[0] + (double)v * (double)p(x, y, z);
+  }
+};
+
+// ------------------------------------------------------------ ADI preproc
+// apps/c/adi/preproc_kernel.h:4-39
+struct Preproc {
+  OPSMI_NO_RED
+  Ptr<const double> u;
+  Ptr<double> du, ax, bx, cx, ay, by, cy, az, bz, cz;
+  int nx, ny, nz;
+  double lambda;
+  __device__ void operator()(int x, int y, int z, double (&)[1]) const {
+    double a, b, c, d;
+    if (x == 0 || x == nx - 1 || y == 0 || y == ny - 1 || z == 0 || z == nz - 1) {
+      d = 0.0f;
+      a = 0.0f;
+      b = 1.0f;
+      c = 0.0f;
+    } else {
+      d = lambda * (u(x - 1, y, z) + u(x + 1, y, z) + u(x, y - 1, z) + u(x, y + 1, z) + u(x, y, z - 1) +
+                    u(x, y, z + 1) - 6.0f * u(x, y, z));
+      a = -0.5f * lambda;
+      b = 1.0f + lambda;
+      c = -0.5f * lambda;
+    }
+    du(x, y, z) = d;
+    ax(x, y, z) = a; bx(x, y, z) = b; cx(x, y, z) = c;
+    ay(x, y, z) = a; by(x, y, z) = b; cy(x, y, z) = c;
+    az(x, y, z) = a; bz(x, y, z) = b; cz(x, y, z) = c;
+  }
+};
+
+struct Stencil27 {
+  OPSMI_NO_RED
+  Ptr<double> out;
+  Ptr<const double> u;
+  __device__ void operator()(int x, int y, int z, double (&)[1]) const {
+    const double W[4] = {0.125, 0.0625, 0.03125, 0.015625};
+    double acc = 0.0;
+#pragma unroll
+    for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
+      for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++)
+          acc = acc + W[(dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy) + (dz < 0 ? -dz : dz)] * u(x + dx, y + dy, z + dz);
+    out(x, y, z) = acc;
+  }
+};
+
+// ------------------------------------------------------------ small reductions
+template <int KIND, int OP>
+struct RedX {
+  using red_t = double;
+  static constexpr int red_op = OP;
+  static constexpr int red_dim = 1;
+  Ptr<const double> a;
+  double scale;
+  __device__ void operator()(int x, int y, int z, double (&s)[1]) const {
+    const double v = a(x, y, z);
+    if constexpr (KIND == 0) s[0] = s[0] + v;
+    if constexpr (KIND == 1) s[0] = s[0] + v * v;
+    if constexpr (KIND == 2) s[0] = s[0] + fabs(v);
+    if constexpr (KIND == 3) s[0] = fmin(s[0], v);
+    if constexpr (KIND == 4) s[0] = fmax(s[0], v);
+    if constexpr (KIND == 5) s[0] = s[0] + scale * v;
+  }
+};
+struct RedTwo {
+  using red_t = double;
+  static constexpr int red_op = OPS_INC;
+  static constexpr int red_dim = 2;
+  Ptr<const double> a;
+  __device__ void operator()(int x, int y, int z, double (&s)[2]) const {
+    s[0] = s[0] + a(x, y, z);
+    s[1] = s[1] + 1.0;
+  }
+};
+struct RedImax {
+  using red_t = int;
+  static constexpr int red_op = OPS_MAX;
+  static constexpr int red_dim = 1;
+  __device__ void operator()(int x, int y, int z, int (&s)[1]) const {
+    const int v = x * y;
+    s[0] = s[0] > v ? s[0] : v;
+  }
+};
+
+}  // namespace
+
+// small-reduction entry helper
+template <int KIND, int OP>
+static void red_x(const char *name, ops_block block, int dim, int *range, ops_arg x, ops_arg s, double scale,
+                  const ops_arg *gb) {
+  if (gb) {
+    ops_arg args[] = {x, *gb, s};
+    Range3 r = check_loop(name, block, dim, range, args,
+                          {dat_r(pts_point(), DT_F64), gbl_r(), red_a(OP, DT_F64)}, nullptr);
+    launch_loop(name, transfer_bytes(args, 3, r), r, RedX<KIND, OP>{ptr_of<const double>(x.dat), scale}, &s);
+  } else {
+    ops_arg args[] = {x, s};
+    Range3 r = check_loop(name, block, dim, range, args, {dat_r(pts_point(), DT_F64), red_a(OP, DT_F64)}, nullptr);
+    launch_loop(name, transfer_bytes(args, 2, r), r, RedX<KIND, OP>{ptr_of<const double>(x.dat), scale}, &s);
+  }
+}
+
+// ============================================================ entry points
+#define ARGS(...) \
+  ops_arg args[] = {__VA_ARGS__}; \
+  const int nargs = (int)(sizeof(args) / sizeof(args[0]));
+#define BYTES(r) transfer_bytes(args, nargs, r)
+
+extern "C" {
+
+void ops_par_loop_set_zero(const char *name, ops_block block, int dim, int *range, ops_arg a0) {
+  OPSMI_API_BEGIN
+  ARGS(a0)
+  DType dt;
+  Range3 r = check_loop(name, block, dim, range, args, {dat_w()}, &dt);
+  if (dt == DT_F64) launch_loop(name, BYTES(r), r, SetZero<double>{ptr_of<double>(a0.dat)});
+  else launch_loop(name, BYTES(r), r, SetZero<float>{ptr_of<float>(a0.dat)});
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+
+void ops_par_loop_copy(const char *name, ops_block block, int dim, int *range, ops_arg a0, ops_arg a1) {
+  OPSMI_API_BEGIN
+  ARGS(a0, a1)
+  DType dt;
+  Range3 r = check_loop(name, block, dim, range, args, {dat_w(), dat_r()}, &dt);
+  if (dt == DT_F64)
+    launch_loop(name, BYTES(r), r, Copy<double>{ptr_of<double>(a0.dat), ptr_of<const double>(a1.dat)});
+  else
+    launch_loop(name, BYTES(r), r, Copy<float>{ptr_of<float>(a0.dat), ptr_of<const float>(a1.dat)});
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+
+static void bndcon(const char *name, ops_block block, int dim, int *range, ops_arg a0, ops_arg a1, int mode) {
+  ARGS(a0, a1)
+  DType dt;
+  Range3 r = check_loop(name, block, dim, range, args, {dat_w(), idx_a()}, &dt);
+  if (dim != 2) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 2D kernel on a ", dim, "D block"));
+  double pi = const_as_double("pi");
+  int n = const_as_int(mode == 2 ? "imax" : "jmax");
+  if (dt == DT_F64) launch_loop(name, BYTES(r), r, Bndcon<double>{ptr_of<double>(a0.dat), pi, n, mode});
+  else launch_loop(name, BYTES(r), r, Bndcon<float>{ptr_of<float>(a0.dat), pi, n, mode});
+  finish_loop(args, nargs);
+}
+
+void ops_par_loop_left_bndcon(const char *name, ops_block block, int dim, int *range, ops_arg a0, ops_arg a1) {
+  OPSMI_API_BEGIN
+  bndcon(name, block, dim, range, a0, a1, 0);
+  OPSMI_API_END()
+}
+void ops_par_loop_right_bndcon(const char *name, ops_block block, int dim, int *range, ops_arg a0, ops_arg a1) {
+  OPSMI_API_BEGIN
+  bndcon(name, block, dim, range, a0, a1, 1);
+  OPSMI_API_END()
+}
+void ops_par_loop_top_bndcon(const char *name, ops_block block, int dim, int *range, ops_arg a0, ops_arg a1) {
+  OPSMI_API_BEGIN
+  bndcon(name, block, dim, range, a0, a1, 2);
+  OPSMI_API_END()
+}
+
+void ops_par_loop_apply_stencil(const char *name, ops_block block, int dim, int *range, ops_arg a0, ops_arg a1,
+                                ops_arg a2) {
+  OPSMI_API_BEGIN
+  ARGS(a0, a1, a2)
+  DType dt;
+  std::vector<int> five = {0, 0, 0, 1, 0, 0, -1, 0, 0, 0, -1, 0, 0, 1, 0};
+  Range3 r = check_loop(name, block, dim, range, args, {dat_r(five), dat_w(), red_a(OPS_MAX)}, &dt);
+  if (dim != 2) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 2D kernel on a ", dim, "D block"));
+  if (dt == DT_F64)
+    launch_loop(name, BYTES(r), r, ApplyStencil<double>{ptr_of<const double>(a0.dat), ptr_of<double>(a1.dat)}, &a2);
+  else
+    launch_loop(name, BYTES(r), r, ApplyStencil<float>{ptr_of<const float>(a0.dat), ptr_of<float>(a1.dat)}, &a2);
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+
+// ---- TeaLeaf
+void ops_par_loop_tea_leaf_init_zero_kernel(const char *name, ops_block block, int dim, int *range, ops_arg p) {
+  OPSMI_API_BEGIN
+  ARGS(p)
+  Range3 r = check_loop(name, block, dim, range, args, {dat_w(DT_F64)}, nullptr);
+  launch_loop(name, BYTES(r), r, SetZero<double>{ptr_of<double>(p.dat)});
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+void ops_par_loop_tea_leaf_init_zero2_kernel(const char *name, ops_block block, int dim, int *range, ops_arg p,
+                                             ops_arg z) {
+  OPSMI_API_BEGIN
+  ARGS(p, z)
+  Range3 r = check_loop(name, block, dim, range, args, {dat_w(DT_F64), dat_w(DT_F64)}, nullptr);
+  launch_loop(name, BYTES(r), r, InitZero2{ptr_of<double>(p.dat), ptr_of<double>(z.dat)});
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+void ops_par_loop_tea_leaf_yeqx_kernel(const char *name, ops_block block, int dim, int *range, ops_arg p,
+                                       ops_arg x) {
+  OPSMI_API_BEGIN
+  ARGS(p, x)
+  Range3 r = check_loop(name, block, dim, range, args, {dat_w(DT_F64), dat_r(pts_point(), DT_F64)}, nullptr);
+  launch_loop(name, BYTES(r), r, Copy<double>{ptr_of<double>(p.dat), ptr_of<const double>(x.dat)});
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+void ops_par_loop_tea_leaf_recip_kernel(const char *name, ops_block block, int dim, int *range, ops_arg u,
+                                        ops_arg p) {
+  OPSMI_API_BEGIN
+  ARGS(u, p)
+  Range3 r = check_loop(name, block, dim, range, args, {dat_w(DT_F64), dat_r(pts_point(), DT_F64)}, nullptr);
+  launch_loop(name, BYTES(r), r, Recip{ptr_of<double>(u.dat), ptr_of<const double>(p.dat)});
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+void ops_par_loop_tea_leaf_dot_kernel(const char *name, ops_block block, int dim, int *range, ops_arg rr,
+                                      ops_arg p, ops_arg rro) {
+  OPSMI_API_BEGIN
+  ARGS(rr, p, rro)
+  Range3 r = check_loop(name, block, dim, range, args,
+                        {dat_r(pts_point(), DT_F64), dat_r(pts_point(), DT_F64), red_a(OPS_INC, DT_F64)}, nullptr);
+  launch_loop(name, BYTES(r), r, Dot{ptr_of<const double>(rr.dat), ptr_of<const double>(p.dat)}, &rro);
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+void ops_par_loop_tea_leaf_norm2_kernel(const char *name, ops_block block, int dim, int *range, ops_arg x,
+                                        ops_arg norm) {
+  OPSMI_API_BEGIN
+  ARGS(x, norm)
+  Range3 r = check_loop(name, block, dim, range, args, {dat_r(pts_point(), DT_F64), red_a(OPS_INC, DT_F64)}, nullptr);
+  launch_loop(name, BYTES(r), r, Norm2{ptr_of<const double>(x.dat)}, &norm);
+  OPSMI_API_END()
+}
+void ops_par_loop_tea_leaf_axpy_kernel(const char *name, ops_block block, int dim, int *range, ops_arg u,
+                                       ops_arg p, ops_arg alpha) {
+  OPSMI_API_BEGIN
+  ARGS(u, p, alpha)
+  Range3 r = check_loop(name, block, dim, range, args, {dat_w(DT_F64), dat_r(pts_point(), DT_F64), gbl_r()}, nullptr);
+  launch_loop(name, BYTES(r), r, Axpy{ptr_of<double>(u.dat), ptr_of<const double>(p.dat), gbl<double>(alpha)});
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+void ops_par_loop_tea_leaf_axpby_kernel(const char *name, ops_block block, int dim, int *range, ops_arg u,
+                                        ops_arg p, ops_arg alpha, ops_arg beta) {
+  OPSMI_API_BEGIN
+  ARGS(u, p, alpha, beta)
+  Range3 r = check_loop(name, block, dim, range, args,
+                        {dat_w(DT_F64), dat_r(pts_point(), DT_F64), gbl_r(), gbl_r()}, nullptr);
+  launch_loop(name, BYTES(r), r,
+              Axpby{ptr_of<double>(u.dat), ptr_of<const double>(p.dat), gbl<double>(alpha), gbl<double>(beta)});
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+void ops_par_loop_tea_leaf_zeqxty_kernel(const char *name, ops_block block, int dim, int *range, ops_arg z,
+                                         ops_arg x, ops_arg y) {
+  OPSMI_API_BEGIN
+  ARGS(z, x, y)
+  Range3 r = check_loop(name, block, dim, range, args,
+                        {dat_w(DT_F64), dat_r(pts_point(), DT_F64), dat_r(pts_point(), DT_F64)}, nullptr);
+  launch_loop(name, BYTES(r), r,
+              Zeqxty{ptr_of<double>(z.dat), ptr_of<const double>(x.dat), ptr_of<const double>(y.dat)});
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+void ops_par_loop_tea_leaf_common_init_Kx_Ky_kernel(const char *name, ops_block block, int dim, int *range,
+                                                    ops_arg Kx, ops_arg Ky, ops_arg w) {
+  OPSMI_API_BEGIN
+  ARGS(Kx, Ky, w)
+  std::vector<int> m = {0, 0, 0, -1, 0, 0, 0, -1, 0};
+  Range3 r = check_loop(name, block, dim, range, args, {dat_w(DT_F64), dat_w(DT_F64), dat_r(m, DT_F64)}, nullptr);
+  if (dim != 2) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 2D kernel on a ", dim, "D block"));
+  Ptr<double> none{nullptr, 0, 0};
+  launch_loop(name, BYTES(r), r, InitK<2>{ptr_of<double>(Kx.dat), ptr_of<double>(Ky.dat), none, ptr_of<const double>(w.dat)});
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+void ops_par_loop_tea_leaf_common_init_Kx_Ky_Kz_kernel(const char *name, ops_block block, int dim, int *range,
+                                                       ops_arg Kx, ops_arg Ky, ops_arg Kz, ops_arg w) {
+  OPSMI_API_BEGIN
+  ARGS(Kx, Ky, Kz, w)
+  std::vector<int> m = {0, 0, 0, -1, 0, 0, 0, -1, 0, 0, 0, -1};
+  Range3 r = check_loop(name, block, dim, range, args,
+                        {dat_w(DT_F64), dat_w(DT_F64), dat_w(DT_F64), dat_r(m, DT_F64)}, nullptr);
+  if (dim != 3) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 3D kernel on a ", dim, "D block"));
+  launch_loop(name, BYTES(r), r,
+              InitK<3>{ptr_of<double>(Kx.dat), ptr_of<double>(Ky.dat), ptr_of<double>(Kz.dat), ptr_of<const double>(w.dat)});
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+
+static std::vector<int> kx_pts() { return {0, 0, 0, 1, 0, 0}; }
+static std::vector<int> ky_pts() { return {0, 0, 0, 0, 1, 0}; }
+static std::vector<int> kz_pts() { return {0, 0, 0, 0, 0, 1}; }
+
+void ops_par_loop_tea_leaf_common_init_kernel(const char *name, ops_block block, int dim, int *range, ops_arg w,
+                                              ops_arg rr, ops_arg Kx, ops_arg Ky, ops_arg u, ops_arg rx,
+                                              ops_arg ry) {
+  OPSMI_API_BEGIN
+  ARGS(w, rr, Kx, Ky, u, rx, ry)
+  Range3 r = check_loop(name, block, dim, range, args,
+                        {dat_w(DT_F64), dat_w(DT_F64), dat_r(kx_pts(), DT_F64), dat_r(ky_pts(), DT_F64),
+                         dat_r(pts_5pt(), DT_F64), gbl_r(), gbl_r()},
+                        nullptr);
+  if (dim != 2) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 2D kernel on a ", dim, "D block"));
+  Ptr<const double> none{nullptr, 0, 0};
+  launch_loop(name, BYTES(r), r,
+              CommonInit<2>{ptr_of<double>(w.dat), ptr_of<double>(rr.dat), ptr_of<const double>(Kx.dat),
+                            ptr_of<const double>(Ky.dat), none, ptr_of<const double>(u.dat), gbl<double>(rx),
+                            gbl<double>(ry), 0.0});
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+void ops_par_loop_tea_leaf_common_init_kernel_3d(const char *name, ops_block block, int dim, int *range,
+                                                 ops_arg w, ops_arg rr, ops_arg Kx, ops_arg Ky, ops_arg Kz,
+                                                 ops_arg u, ops_arg rx, ops_arg ry, ops_arg rz) {
+  OPSMI_API_BEGIN
+  ARGS(w, rr, Kx, Ky, Kz, u, rx, ry, rz)
+  Range3 r = check_loop(name, block, dim, range, args,
+                        {dat_w(DT_F64), dat_w(DT_F64), dat_r(kx_pts(), DT_F64), dat_r(ky_pts(), DT_F64),
+                         dat_r(kz_pts(), DT_F64), dat_r(pts_7pt(), DT_F64), gbl_r(), gbl_r(), gbl_r()},
+                        nullptr);
+  if (dim != 3) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 3D kernel on a ", dim, "D block"));
+  launch_loop(name, BYTES(r), r,
+              CommonInit<3>{ptr_of<double>(w.dat), ptr_of<double>(rr.dat), ptr_of<const double>(Kx.dat),
+                            ptr_of<const double>(Ky.dat), ptr_of<const double>(Kz.dat), ptr_of<const double>(u.dat),
+                            gbl<double>(rx), gbl<double>(ry), gbl<double>(rz)});
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+void ops_par_loop_tea_leaf_cg_calc_w_reduce_kernel(const char *name, ops_block block, int dim, int *range,
+                                                   ops_arg w, ops_arg Kx, ops_arg Ky, ops_arg p, ops_arg rx,
+                                                   ops_arg ry, ops_arg pw) {
+  OPSMI_API_BEGIN
+  ARGS(w, Kx, Ky, p, rx, ry, pw)
+  Range3 r = check_loop(name, block, dim, range, args,
+                        {dat_w(DT_F64), dat_r(kx_pts(), DT_F64), dat_r(ky_pts(), DT_F64), dat_r(pts_5pt(), DT_F64),
+                         gbl_r(), gbl_r(), red_a(OPS_INC, DT_F64)},
+                        nullptr);
+  if (dim != 2) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 2D kernel on a ", dim, "D block"));
+  Ptr<const double> none{nullptr, 0, 0};
+  launch_loop(name, BYTES(r), r,
+              CalcW<2>{ptr_of<double>(w.dat), ptr_of<const double>(Kx.dat), ptr_of<const double>(Ky.dat), none,
+                       ptr_of<const double>(p.dat), gbl<double>(rx), gbl<double>(ry), 0.0},
+              &pw);
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+void ops_par_loop_tea_leaf_cg_calc_w_reduce_kernel_3d(const char *name, ops_block block, int dim, int *range,
+                                                      ops_arg w, ops_arg Kx, ops_arg Ky, ops_arg Kz, ops_arg p,
+                                                      ops_arg rx, ops_arg ry, ops_arg rz, ops_arg pw) {
+  OPSMI_API_BEGIN
+  ARGS(w, Kx, Ky, Kz, p, rx, ry, rz, pw)
+  Range3 r = check_loop(name, block, dim, range, args,
+                        {dat_w(DT_F64), dat_r(kx_pts(), DT_F64), dat_r(ky_pts(), DT_F64), dat_r(kz_pts(), DT_F64),
+                         dat_r(pts_7pt(), DT_F64), gbl_r(), gbl_r(), gbl_r(), red_a(OPS_INC, DT_F64)},
+                        nullptr);
+  if (dim != 3) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 3D kernel on a ", dim, "D block"));
+  launch_loop(name, BYTES(r), r,
+              CalcW<3>{ptr_of<double>(w.dat), ptr_of<const double>(Kx.dat), ptr_of<const double>(Ky.dat),
+                       ptr_of<const double>(Kz.dat), ptr_of<const double>(p.dat), gbl<double>(rx), gbl<double>(ry),
+                       gbl<double>(rz)},
+              &pw);
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+void ops_par_loop_tea_leaf_cg_calc_ur_r_reduce_kernel(const char *name, ops_block block, int dim, int *range,
+                                                      ops_arg rr, ops_arg w, ops_arg alpha, ops_arg rnn) {
+  OPSMI_API_BEGIN
+  ARGS(rr, w, alpha, rnn)
+  Range3 r = check_loop(name, block, dim, range, args,
+                        {dat_w(DT_F64), dat_r(pts_point(), DT_F64), gbl_r(), red_a(OPS_INC, DT_F64)}, nullptr);
+  launch_loop(name, BYTES(r), r, CalcUrR{ptr_of<double>(rr.dat), ptr_of<const double>(w.dat), gbl<double>(alpha)},
+              &rnn);
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+
+// ---- Poisson
+void ops_par_loop_poisson_calc_w_reduce_kernel(const char *name, ops_block block, int dim, int *range, ops_arg w,
+                                               ops_arg p, ops_arg pw) {
+  OPSMI_API_BEGIN
+  ARGS(w, p, pw)
+  DType dt;
+  Range3 r = check_loop(name, block, dim, range, args,
+                        {dat_w(), dat_r(dim == 3 ? pts_7pt() : pts_5pt()), red_a(OPS_INC, DT_F64)}, &dt);
+  if (dim < 2) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": needs a 2D or 3D block"));
+  if (dt == DT_F64) {
+    if (dim == 2) launch_loop(name, BYTES(r), r, PoissonW<double, 2>{ptr_of<double>(w.dat), ptr_of<const double>(p.dat)}, &pw);
+    else launch_loop(name, BYTES(r), r, PoissonW<double, 3>{ptr_of<double>(w.dat), ptr_of<const double>(p.dat)}, &pw);
+  } else {
+    if (dim == 2) launch_loop(name, BYTES(r), r, PoissonW<float, 2>{ptr_of<float>(w.dat), ptr_of<const float>(p.dat)}, &pw);
+    else launch_loop(name, BYTES(r), r, PoissonW<float, 3>{ptr_of<float>(w.dat), ptr_of<const float>(p.dat)}, &pw);
+  }
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+
+// ---- ADI preproc, 27-point
+void ops_par_loop_preproc_kernel(const char *name, ops_block block, int dim, int *range, ops_arg u, ops_arg du,
+                                 ops_arg ax, ops_arg bx, ops_arg cx, ops_arg ay, ops_arg by, ops_arg cy,
+                                 ops_arg az, ops_arg bz, ops_arg cz, ops_arg idx) {
+  OPSMI_API_BEGIN
+  ARGS(u, du, ax, bx, cx, ay, by, cy, az, bz, cz, idx)
+  std::vector<ArgSpec> specs = {dat_r(pts_7pt(), DT_F64)};
+  for (int i = 0; i < 10; i++) specs.push_back(dat_w(DT_F64));
+  specs.push_back(idx_a());
+  Range3 r = check_loop(name, block, dim, range, args, specs, nullptr);
+  if (dim != 3) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 3D kernel on a ", dim, "D block"));
+  Preproc f{ptr_of<const double>(u.dat), ptr_of<double>(du.dat), ptr_of<double>(ax.dat), ptr_of<double>(bx.dat),
+            ptr_of<double>(cx.dat), ptr_of<double>(ay.dat), ptr_of<double>(by.dat), ptr_of<double>(cy.dat),
+            ptr_of<double>(az.dat), ptr_of<double>(bz.dat), ptr_of<double>(cz.dat), const_as_int("nx"),
+            const_as_int("ny"), const_as_int("nz"), const_as_double("lambda")};
+  launch_loop(name, BYTES(r), r, f);
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+
+void ops_par_loop_stencil27_kernel(const char *name, ops_block block, int dim, int *range, ops_arg out, ops_arg u) {
+  OPSMI_API_BEGIN
+  ARGS(out, u)
+  Range3 r = check_loop(name, block, dim, range, args, {dat_w(DT_F64), dat_r(pts_27pt(), DT_F64)}, nullptr);
+  if (dim != 3) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 3D kernel on a ", dim, "D block"));
+  launch_loop(name, BYTES(r), r, Stencil27{ptr_of<double>(out.dat), ptr_of<const double>(u.dat)});
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+
+void ops_par_loop_red_sum(const char *name, ops_block block, int dim, int *range, ops_arg x, ops_arg s) {
+  OPSMI_API_BEGIN
+  red_x<0, OPS_INC>(name, block, dim, range, x, s, 0.0, nullptr);
+  OPSMI_API_END()
+}
+void ops_par_loop_red_sumsq(const char *name, ops_block block, int dim, int *range, ops_arg x, ops_arg s) {
+  OPSMI_API_BEGIN
+  red_x<1, OPS_INC>(name, block, dim, range, x, s, 0.0, nullptr);
+  OPSMI_API_END()
+}
+void ops_par_loop_red_abs(const char *name, ops_block block, int dim, int *range, ops_arg x, ops_arg s) {
+  OPSMI_API_BEGIN
+  red_x<2, OPS_INC>(name, block, dim, range, x, s, 0.0, nullptr);
+  OPSMI_API_END()
+}
+void ops_par_loop_red_min(const char *name, ops_block block, int dim, int *range, ops_arg x, ops_arg m) {
+  OPSMI_API_BEGIN
+  red_x<3, OPS_MIN>(name, block, dim, range, x, m, 0.0, nullptr);
+  OPSMI_API_END()
+}
+void ops_par_loop_red_max(const char *name, ops_block block, int dim, int *range, ops_arg x, ops_arg m) {
+  OPSMI_API_BEGIN
+  red_x<4, OPS_MAX>(name, block, dim, range, x, m, 0.0, nullptr);
+  OPSMI_API_END()
+}
+void ops_par_loop_red_scaled(const char *name, ops_block block, int dim, int *range, ops_arg x, ops_arg scale,
+                             ops_arg s) {
+  OPSMI_API_BEGIN
+  if (scale.argtype != OPS_ARG_GBL || !scale.data || scale.elem_size != 8)
+    fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": arg 1 must be an OPS_READ ops_arg_gbl of one double"));
+  red_x<5, OPS_INC>(name, block, dim, range, x, s, gbl<double>(scale), &scale);
+  OPSMI_API_END()
+}
+void ops_par_loop_red_two(const char *name, ops_block block, int dim, int *range, ops_arg x, ops_arg s2) {
+  OPSMI_API_BEGIN
+  ARGS(x, s2)
+  Range3 r = check_loop(name, block, dim, range, args, {dat_r(pts_point(), DT_F64), red_a(OPS_INC, DT_F64, 2)}, nullptr);
+  launch_loop(name, BYTES(r), r, RedTwo{ptr_of<const double>(x.dat)}, &s2);
+  OPSMI_API_END()
+}
+void ops_par_loop_red_imax(const char *name, ops_block block, int dim, int *range, ops_arg idx, ops_arg m) {
+  OPSMI_API_BEGIN
+  ARGS(idx, m)
+  Range3 r = check_loop(name, block, dim, range, args, {idx_a(), red_a(OPS_MAX, DT_I32)}, nullptr);
+  launch_loop(name, BYTES(r), r, RedImax{}, &m);
+  OPSMI_API_END()
+}
+
+}  // extern "C"

@@ -1,0 +1,206 @@
+// Internal runtime of the MI355X OPS backend: object tables, device memory,
+// argument validation, the loop queue and reduction bookkeeping.  Not part of
+// the public ABI (include/ops_mi355x.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/ops_mi355x.h"
+
+namespace opsmi {
+
+// ---------------------------------------------------------------- errors
+struct Error {
+  int code;
+  std::string msg;
+};
+[[noreturn]] void fail(int code, const std::string &msg);
+void report(const Error &e);
+
+#define OPSMI_HIP(call)                                                                          \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      ::opsmi::fail(OPS_RUNTIME_ERROR, std::string("HIP error in " #call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// every extern "C" entry point runs inside API_BEGIN/API_END so no C++
+// exception crosses the ABI
+#define OPSMI_API_BEGIN try {
+#define OPSMI_API_END(ret)              \
+  }                                     \
+  catch (const ::opsmi::Error &e_) {    \
+    ::opsmi::report(e_);                \
+    return ret;                         \
+  }                                     \
+  catch (const std::exception &e_) {    \
+    ::opsmi::report({OPS_INTERNAL_ERROR, e_.what()}); \
+    return ret;                         \
+  }
+
+template <class... A>
+std::string cat(const A &...a) {
+  std::ostringstream s;
+  (s << ... << a);
+  return s.str();
+}
+
+// ---------------------------------------------------------------- types
+enum DType { DT_F64 = 0, DT_F32 = 1, DT_I32 = 2, DT_OTHER = 3 };
+DType dtype_of(const char *type, int type_size);
+
+// device-side view of one dat argument: pointer to point (0,0,0) and strides
+// in elements (x stride is the component count for AoS dats)
+struct DatView {
+  char *base;
+  long sy, sz;
+};
+
+struct RedImpl {
+  void *d_slot = nullptr;  // device value of the handle (size bytes)
+  bool dev_valid = false;  // device slot holds the running value since re-arm
+  DType dt = DT_OTHER;
+};
+
+struct DatImpl {
+  char *alloc = nullptr;  // hipMalloc'd pointer (data_d may be offset for alignment)
+  DType dt = DT_OTHER;
+  size_t host_bytes = 0;
+};
+
+struct LoopStat {
+  long count = 0;
+  double ms = 0.0;
+  double bytes = 0.0;
+};
+
+struct Const {
+  std::string type;
+  std::vector<char> bytes;
+};
+
+struct Instance {
+  bool initialised = false;
+  int diags = 1;
+  int device = 0;
+  int num_cus = 256;
+  hipStream_t stream = nullptr;
+  int error_mode = 0;
+  int last_error = 0;
+  std::string last_msg;
+  bool fusion = true;
+  int block_x = 64, block_y = 4, block_z = 1;  // OPS_BLOCK_SIZE_* (x must be 64)
+
+  std::vector<ops_block> blocks;
+  std::vector<ops_dat> dats;  // index -> dat (nullptr after free)
+  std::vector<ops_stencil> stencils;
+  std::vector<ops_reduction> reductions;
+  std::vector<ops_halo> halos;
+  std::vector<ops_halo_group> groups;
+  std::map<std::string, Const> consts;
+  std::map<std::string, LoopStat> stats;
+
+  // reduction scratch for the one in-order stream: block partials + ticket
+  void *d_partials = nullptr;
+  size_t partials_bytes = 0;
+  unsigned int *d_ticket = nullptr;
+
+  // host staging for reduction results
+  void *h_pinned = nullptr;
+
+  // random fills
+  uint64_t rng_seed = 0;
+  bool rng_init = false;
+
+  // event pool for diags > 1 timing
+  std::vector<hipEvent_t> ev_free;
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+};
+
+Instance &inst();
+void require_init();
+
+// ---------------------------------------------------------------- dats
+DatView view_of(ops_dat d);
+void ensure_host(ops_dat d);      // host mirror allocated and current
+void mark_device_dirty(ops_dat d);  // device now holds the newest data
+void sync_to_device(ops_dat d);   // host newer -> copy up
+
+// ---------------------------------------------------------------- args
+// Per-argument contract of a hand-written kernel.
+enum ArgKind { AK_DAT, AK_GBL, AK_RED, AK_IDX };
+struct ArgSpec {
+  ArgKind kind;
+  bool write = false;          // dat written by the kernel
+  std::vector<int> offsets;    // dat: flattened (x,y,z) offsets the kernel reads/writes
+  int red_op = OPS_INC;        // reduce: required access
+  int dim = 1;                 // gbl/reduce components
+  DType dt = DT_F64;           // expected element type (DT_OTHER: any of f64/f32)
+};
+std::vector<int> pts_point();
+std::vector<int> pts_5pt();
+std::vector<int> pts_7pt();
+std::vector<int> pts_27pt();
+
+struct Range3 {
+  int x0, x1, y0, y1, z0, z1;
+  long npts() const {
+    long n = 1;
+    n *= (x1 > x0 ? x1 - x0 : 0);
+    n *= (y1 > y0 ? y1 - y0 : 0);
+    n *= (z1 > z0 ? z1 - z0 : 0);
+    return n;
+  }
+};
+
+// Validates args against specs (kind, access, type, stencil coverage, bounds)
+// and returns the loop range.  dat_dt receives the common dat element type
+// for specs with dt == DT_OTHER.
+Range3 check_loop(const char *name, ops_block block, int dim, const int *range, const ops_arg *args,
+                  const std::vector<ArgSpec> &specs, DType *dat_dt);
+
+// value of a READ gbl arg
+template <class T>
+T gbl(const ops_arg &a) {
+  T v;
+  memcpy(&v, a.data, sizeof(T));
+  return v;
+}
+const Const &get_const(const char *name);
+double const_as_double(const char *name);
+int const_as_int(const char *name);
+
+// ---------------------------------------------------------------- reductions
+// Device-side control block passed to reducing kernels.
+struct RedCtl {
+  void *slot;          // handle's device value
+  void *partials;      // per-block partials (dim values per block)
+  unsigned *ticket;    // zero between launches
+  int use_init;        // 1: start from init[] (first use since re-arm)
+  double init[4];      // initial values, stored as the reduction type
+  int init_i[4];
+};
+// Prepares a handle for a launch (device slot, init values).  Marks the
+// handle's device value as live; the result is copied back at result time.
+RedCtl red_prepare(const ops_arg &a, int dim, DType dt, int nblocks);
+void ensure_partials(size_t bytes);
+
+// ---------------------------------------------------------------- loops
+// Records one executed loop for statistics; with diags > 1 the launches made
+// by fn are bracketed by events on the library stream.
+void run_loop(const char *name, double bytes, const std::function<void(hipStream_t)> &fn);
+double transfer_bytes(const ops_arg *args, int nargs, const Range3 &r);
+int grid_blocks(long rows, int rows_per_block);
+
+}  // namespace opsmi

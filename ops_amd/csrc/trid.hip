@@ -1,0 +1,258 @@
+// Batched tridiagonal solves behind ops_tridMultiDimBatch[_Inc]
+// (ops/c/include/ops_tridiag.h:45-83, ops/c/src/tridiag/ops_tridiag.cpp:
+// 78-249).  The reference hands raw pointers to the external libtrid
+// (tridDmtsvStridedBatch); this file is the backend's own solver.
+//
+// Thomas: one thread per system.  Lines are numbered so that consecutive
+// threads take systems that are neighbours in x (for solves along y and z the
+// loads of a wave are then contiguous).  The forward sweep keeps the modified
+// super-diagonal c' and right-hand side d' in a step-major scratch array
+// ([step][line]), so scratch traffic is coalesced for every direction.
+// Arithmetic is the reciprocal form
+//   r = 1/(b_i - a_i c'_{i-1});  d'_i = r (d_i - a_i d'_{i-1});  c'_i = r c_i
+//   x_i = d'_i - c'_i x_{i+1}
+// with contraction off, so results equal the CPU restatement bit for bit.
+//
+// PCR (strategy PCR): one workgroup per system held in LDS, log2(n) steps of
+// parallel cyclic reduction (Hockney 1965); a_0 and c_{n-1} are taken as 0.
+#include "engine.h"
+
+using namespace opsmi;
+
+struct ops_tridsolver_params_c {
+  int strategy;
+};
+
+namespace {
+
+struct LineGeo {
+  long n;         // system length
+  long stride;    // element stride along the solve dimension
+  long count;     // number of systems
+  int e0;         // extent of the first other dimension (fastest)
+  long s0, s1;    // element strides of the two other dimensions
+};
+
+__device__ __forceinline__ long line_offset(const LineGeo &g, long id) {
+  long i0 = id % g.e0, i1 = id / g.e0;
+  return i0 * g.s0 + i1 * g.s1;
+}
+
+template <class T, bool INC>
+__global__ __launch_bounds__(256) void k_thomas(const T *__restrict__ a, const T *__restrict__ b,
+                                                const T *__restrict__ c, T *__restrict__ d, T *__restrict__ u,
+                                                T *__restrict__ cs, T *__restrict__ ds, LineGeo g) {
+  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= g.count) return;
+  const long o = line_offset(g, id);
+  const long s = g.stride, L = g.count;
+  const T *ap = a + o, *bp = b + o, *cp = c + o;
+  T *dp = d + o;
+  T bb = (T)1.0 / bp[0];
+  T cprev = bb * cp[0];
+  T dprev = bb * dp[0];
+  cs[id] = cprev;
+  ds[id] = dprev;
+  for (long i = 1; i < g.n; i++) {
+    const T ai = ap[i * s];
+    bb = (T)1.0 / (bp[i * s] - ai * cprev);
+    dprev = bb * (dp[i * s] - ai * dprev);
+    cprev = bb * cp[i * s];
+    cs[i * L + id] = cprev;
+    ds[i * L + id] = dprev;
+  }
+  T x = dprev;
+  if (INC) u[o + (g.n - 1) * s] = u[o + (g.n - 1) * s] + x;
+  else dp[(g.n - 1) * s] = x;
+  for (long i = g.n - 2; i >= 0; i--) {
+    x = ds[i * L + id] - cs[i * L + id] * x;
+    if (INC) u[o + i * s] = u[o + i * s] + x;
+    else dp[i * s] = x;
+  }
+}
+
+template <class T, bool INC>
+__global__ __launch_bounds__(1024) void k_pcr(const T *__restrict__ a, const T *__restrict__ b,
+                                              const T *__restrict__ c, T *__restrict__ d, T *__restrict__ u,
+                                              LineGeo g) {
+  extern __shared__ unsigned char smem[];
+  T *A = (T *)smem;
+  T *B = A + 2 * g.n;
+  T *C = B + 2 * g.n;
+  T *D = C + 2 * g.n;
+  const long o = line_offset(g, blockIdx.x);
+  const int n = (int)g.n;
+  const int i = threadIdx.x;
+  const long s = g.stride;
+  if (i < n) {
+    A[i] = i == 0 ? (T)0 : a[o + i * s];
+    B[i] = b[o + i * s];
+    C[i] = i == n - 1 ? (T)0 : c[o + i * s];
+    D[i] = d[o + i * s];
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int st = 1; st < n; st *= 2) {
+    const int nx = cur ^ n;  // offset of the other buffer half (0 or n)
+    if (i < n) {
+      const T *Ac = A + cur, *Bc = B + cur, *Cc = C + cur, *Dc = D + cur;
+      const bool lo = i - st >= 0, hi = i + st < n;
+      const T k1 = lo ? Ac[i] / Bc[i - st] : (T)0;
+      const T k2 = hi ? Cc[i] / Bc[i + st] : (T)0;
+      T bv = Bc[i], dv = Dc[i];
+      if (lo) {
+        bv = bv - Cc[i - st] * k1;
+        dv = dv - Dc[i - st] * k1;
+      }
+      if (hi) {
+        bv = bv - Ac[i + st] * k2;
+        dv = dv - Dc[i + st] * k2;
+      }
+      A[nx + i] = lo ? -Ac[i - st] * k1 : (T)0;
+      C[nx + i] = hi ? -Cc[i + st] * k2 : (T)0;
+      B[nx + i] = bv;
+      D[nx + i] = dv;
+    }
+    __syncthreads();
+    cur = nx;
+  }
+  if (i < n) {
+    const T x = D[cur + i] / B[cur + i];
+    if (INC) u[o + i * s] = u[o + i * s] + x;
+    else d[o + i * s] = x;
+  }
+}
+
+struct TridScratch {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+TridScratch g_scratch;
+
+void *trid_scratch(size_t bytes) {
+  if (bytes > g_scratch.bytes) {
+    if (g_scratch.p) {
+      OPSMI_HIP(hipStreamSynchronize(inst().stream));
+      OPSMI_HIP(hipFree(g_scratch.p));
+    }
+    OPSMI_HIP(hipMalloc(&g_scratch.p, bytes));
+    g_scratch.bytes = bytes;
+  }
+  return g_scratch.p;
+}
+
+void trid_run(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c, ops_dat d, ops_dat u,
+              ops_tridsolver_params_c *ctx) {
+  require_init();
+  if (!a || !b || !c || !d || !range) fail(OPS_INVALID_ARGUMENT, "Tridsolver error: NULL argument");
+  if (ndim < 1 || ndim > 3 || ndim != a->block->dims)
+    fail(OPS_INVALID_ARGUMENT, "Tridsolver error: ndim must match the block (1-3D)");
+  if (solvedim < 0 || solvedim >= ndim) fail(OPS_INVALID_ARGUMENT, "Tridsolver error: solvedim out of range");
+  ops_dat all[5] = {a, b, c, d, u};
+  int nd = u ? 5 : 4;
+  // ops/c/src/tridiag/ops_tridiag.cpp:91-101
+  for (int i = 0; i < ndim; i++)
+    for (int k = 1; k < nd; k++)
+      if (all[k]->size[i] != a->size[i])
+        fail(OPS_RUNTIME_ERROR, "Tridsolver error: the a,b,c,d datasets all need to be the same size");
+  for (int k = 1; k < nd; k++)
+    if (strcmp(all[k]->type, a->type) != 0)
+      fail(OPS_RUNTIME_ERROR, "Tridsolver error: the a,b,c,d datasets must all be of the same type");
+  DType dt = dtype_of(a->type, a->type_size);
+  if (dt != DT_F64 && dt != DT_F32)
+    fail(OPS_RUNTIME_ERROR, "Tridsolver error: unsupported type. Dataset type must either be \"double\" or \"float\"");
+  for (int k = 0; k < nd; k++) {
+    if (all[k]->dim != 1) fail(OPS_NOT_IMPLEMENTED, "Tridsolver error: multi-component dats are not supported");
+    if (all[k]->locked_hd) fail(OPS_RUNTIME_ERROR, cat("Tridsolver error: dat ", all[k]->name, " is locked"));
+  }
+  int dims[3] = {1, 1, 1};
+  long offset = 0;
+  long st[3] = {1, a->size[0], (long)a->size[0] * a->size[1]};
+  for (int i = 0; i < ndim; i++) {
+    dims[i] = range[2 * i + 1] - range[2 * i];
+    int first = a->d_m[i] + a->base[i], last = a->size[i] + a->d_m[i] + a->base[i];
+    if (dims[i] < 0 || range[2 * i] < first || range[2 * i + 1] > last)
+      fail(OPS_INVALID_ARGUMENT, cat("Tridsolver error: range outside the dats in dimension ", i));
+    offset += (long)range[2 * i] * st[i];
+  }
+  for (int k = 0; k < nd; k++) sync_to_device(all[k]);
+  LineGeo g;
+  g.n = dims[solvedim];
+  g.stride = st[solvedim];
+  int od[2], q = 0;
+  for (int i = 0; i < 3; i++)
+    if (i != solvedim) od[q++] = i;
+  g.e0 = dims[od[0]];
+  g.s0 = st[od[0]];
+  g.s1 = st[od[1]];
+  g.count = (long)dims[od[0]] * dims[od[1]];
+  const char *nm = u ? "ops_tridMultiDimBatch_Inc" : "ops_tridMultiDimBatch";
+  if (g.n <= 0 || g.count <= 0) {
+    run_loop(nm, 0.0, [](hipStream_t) {});
+    return;
+  }
+  double bytes = (double)g.n * g.count * a->type_size * (u ? 6.0 : 5.0);
+  size_t es = a->type_size;
+  char *pa = a->data_d + a->base_offset + offset * es, *pb = b->data_d + b->base_offset + offset * es;
+  char *pc = c->data_d + c->base_offset + offset * es, *pd = d->data_d + d->base_offset + offset * es;
+  char *pu = u ? u->data_d + u->base_offset + offset * es : nullptr;
+  bool pcr = ctx && ctx->strategy == 3;
+  if (pcr) {
+    if (g.n > 1024) fail(OPS_NOT_IMPLEMENTED, "Tridsolver error: PCR supports systems of at most 1024 rows");
+    int threads = (int)((g.n + 63) / 64 * 64);
+    size_t sm = 8 * g.n * es;
+    run_loop(nm, bytes, [&](hipStream_t s) {
+      if (dt == DT_F64) {
+        if (u) hipLaunchKernelGGL((k_pcr<double, true>), dim3(g.count), dim3(threads), sm, s, (double *)pa, (double *)pb, (double *)pc, (double *)pd, (double *)pu, g);
+        else hipLaunchKernelGGL((k_pcr<double, false>), dim3(g.count), dim3(threads), sm, s, (double *)pa, (double *)pb, (double *)pc, (double *)pd, (double *)pu, g);
+      } else {
+        if (u) hipLaunchKernelGGL((k_pcr<float, true>), dim3(g.count), dim3(threads), sm, s, (float *)pa, (float *)pb, (float *)pc, (float *)pd, (float *)pu, g);
+        else hipLaunchKernelGGL((k_pcr<float, false>), dim3(g.count), dim3(threads), sm, s, (float *)pa, (float *)pb, (float *)pc, (float *)pd, (float *)pu, g);
+      }
+    });
+  } else {
+    size_t sb = (size_t)g.n * g.count * es;
+    char *scr = (char *)trid_scratch(2 * sb);
+    int nblk = (int)((g.count + 255) / 256);
+    run_loop(nm, bytes, [&](hipStream_t s) {
+      if (dt == DT_F64) {
+        if (u) hipLaunchKernelGGL((k_thomas<double, true>), dim3(nblk), dim3(256), 0, s, (double *)pa, (double *)pb, (double *)pc, (double *)pd, (double *)pu, (double *)scr, (double *)(scr + sb), g);
+        else hipLaunchKernelGGL((k_thomas<double, false>), dim3(nblk), dim3(256), 0, s, (double *)pa, (double *)pb, (double *)pc, (double *)pd, (double *)pu, (double *)scr, (double *)(scr + sb), g);
+      } else {
+        if (u) hipLaunchKernelGGL((k_thomas<float, true>), dim3(nblk), dim3(256), 0, s, (float *)pa, (float *)pb, (float *)pc, (float *)pd, (float *)pu, (float *)scr, (float *)(scr + sb), g);
+        else hipLaunchKernelGGL((k_thomas<float, false>), dim3(nblk), dim3(256), 0, s, (float *)pa, (float *)pb, (float *)pc, (float *)pd, (float *)pu, (float *)scr, (float *)(scr + sb), g);
+      }
+    });
+  }
+  mark_device_dirty(u ? u : d);
+}
+
+}  // namespace
+
+extern "C" {
+
+ops_tridsolver_params_c *ops_tridsolver_params_create(ops_block block, int strategy) {
+  (void)block;
+  ops_tridsolver_params_c *p = new ops_tridsolver_params_c;
+  p->strategy = strategy;
+  return p;
+}
+
+void ops_tridsolver_params_destroy(ops_tridsolver_params_c *p) { delete p; }
+
+void ops_tridMultiDimBatch_c(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c, ops_dat d,
+                             ops_tridsolver_params_c *ctx) {
+  OPSMI_API_BEGIN
+  trid_run(ndim, solvedim, range, a, b, c, d, nullptr, ctx);
+  OPSMI_API_END()
+}
+
+void ops_tridMultiDimBatch_Inc_c(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c, ops_dat d,
+                                 ops_dat u, ops_tridsolver_params_c *ctx) {
+  OPSMI_API_BEGIN
+  if (!u) fail(OPS_INVALID_ARGUMENT, "Tridsolver error: NULL u");
+  trid_run(ndim, solvedim, range, a, b, c, d, u, ctx);
+  OPSMI_API_END()
+}
+
+}  // extern "C"

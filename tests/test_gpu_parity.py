@@ -1,0 +1,409 @@
+"""GPU parity: the HIP backend through the C ABI against the oracle and the
+reference's own libops_seq fixtures on the same seeded inputs.
+
+Bars: stencil sweeps bit-exact (FP contraction off); MIN/MAX reductions
+bit-exact; sum reductions and CG scalars within 1e-12 relative."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+NT = min(16, os.cpu_count() or 1)
+RTOL_RED = 1e-12
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return O.load_golden()
+
+
+@pytest.fixture(scope="module")
+def apps(gpu_ops):
+    from ops_amd import apps as A
+    return A
+
+
+def relerr(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+# ------------------------------------------------------------------ Laplace
+@pytest.mark.parametrize("key,mode", [("laplace_top", 0), ("laplace_app", 1)])
+def test_laplace_fixture_bit_exact(apps, golden, key, mode):
+    g, m = golden
+    c = m[key]
+    init = O.laplace_init(c["nx"], c["ny"], mode)
+    err, hist, A = apps.laplace2d(c["nx"], c["ny"], c["iters"], mode, init)
+    assert np.array_equal(hist, g[key + "_err"])
+    assert np.array_equal(A.ravel(), g[key + "_A"])
+
+
+@pytest.mark.parametrize("key,mode", [("laplace_top", 0), ("laplace_app", 1)])
+def test_laplace_gpu_boundary_kernels(apps, golden, key, mode):
+    # boundary values come from the GPU's sin/exp; they may differ from glibc
+    # by an ulp, so the sweep is compared at 1e-14 relative
+    g, m = golden
+    c = m[key]
+    err, hist, A = apps.laplace2d(c["nx"], c["ny"], c["iters"], mode, None)
+    assert relerr(hist, g[key + "_err"]) < 1e-14
+    assert np.max(np.abs(A.ravel() - g[key + "_A"])) < 1e-14
+
+
+def test_laplace_f32_fixture_bit_exact(gpu_ops, golden):
+    ops = gpu_ops
+    g, m = golden
+    c = m["laplace_f32"]
+    nx, ny = c["nx"], c["ny"]
+    init = g["laplace_f32_init"].reshape(ny + 2, nx + 2)
+    blk = ops.ops_decl_block(2, "gridf")
+    A = ops.ops_decl_dat(blk, [nx, ny], dtype=np.float32, name="A")
+    An = ops.ops_decl_dat(blk, [nx, ny], dtype=np.float32, name="Anew")
+    s00 = ops.ops_decl_stencil(2, 1, [0, 0], "00")
+    s5 = ops.ops_decl_stencil(2, 5, [0, 0, 1, 0, -1, 0, 0, 1, 0, -1], "5pt")
+    h = ops.ops_decl_reduction_handle(4, "float", "err")
+    ops.set_ext(A, init)
+    rng = [0, nx, 0, ny]
+    hist = []
+    for _ in range(c["iters"]):
+        ops.ops_par_loop("apply_stencil", "apply_stencil", blk, 2, rng, ops.ops_arg_dat(A, 1, s5, "float", ops.OPS_READ),
+                         ops.ops_arg_dat(An, 1, s00, "float", ops.OPS_WRITE),
+                         ops.ops_arg_reduce(h, 1, "float", ops.OPS_MAX))
+        hist.append(ops.ops_reduction_result(h))
+        ops.ops_par_loop("copy", "copy", blk, 2, rng, ops.ops_arg_dat(A, 1, s00, "float", ops.OPS_WRITE),
+                         ops.ops_arg_dat(An, 1, s00, "float", ops.OPS_READ))
+    assert np.array_equal(np.array(hist, np.float32), g["laplace_f32_err"])
+    assert np.array_equal(ops.ops_dat_fetch_data(A).ravel(), g["laplace_f32_A"])
+    ops.ops_free_dat(A)
+    ops.ops_free_dat(An)
+
+
+def test_laplace_full_size_bit_exact(apps):
+    # the 2048^2 Laplace workload, 100 sweeps, against the threaded oracle
+    nx = ny = 2048
+    it = 100
+    init = O.laplace_init(nx, ny, 0)
+    err, hist, A = apps.laplace2d(nx, ny, it, 0, init)
+    A_ref, h_ref = O.laplace_run(init, it, nthreads=NT)
+    assert np.array_equal(hist, h_ref)
+    assert np.array_equal(A, O.interior(A_ref))
+
+
+def test_laplace_reference_known_answer(apps, golden):
+    # apps/c/laplace2d_tutorial/step7: 4094^2, 100 iterations
+    g, m = golden
+    c = m["laplace_kat"]
+    init = O.laplace_init(c["nx"], c["ny"], 1)
+    err, hist, _ = apps.laplace2d(c["nx"], c["ny"], c["iters"], 1, init, want_field=False)
+    assert err == g["laplace_kat_err"][-1]
+    err2, _, _ = apps.laplace2d(c["nx"], c["ny"], c["iters"], 1, None, want_field=False)
+    assert abs(100.0 * err2 / c["expected_app"] - 100.0) < 0.001
+
+
+# ------------------------------------------------------------------ CG
+@pytest.mark.parametrize("ndim,key", [(2, "cg2d"), (3, "heat3d")])
+def test_heat_cg_fixture(apps, golden, ndim, key):
+    g, m = golden
+    c = m[key]
+    nz = c.get("nz", 1)
+    K, rhs = O.heat_inputs(m["seed"], ndim, c["nx"], c["ny"], nz)
+    h = apps.Heat(ndim, c["nx"], c["ny"], nz, K_ext=K, rhs=rhs)
+    assert relerr(h.rr(initial=True), g[key + "_rrn"][0]) < RTOL_RED
+    k, hist = h.iterate(c["iters"], history=True)
+    assert k == c["iters"]
+    for name in ("pw", "alpha", "beta"):
+        assert relerr(hist[name], g[f"{key}_{name}"]) < RTOL_RED, name
+    assert relerr(hist["rrn"], g[key + "_rrn"][1:]) < 1e-10
+    u = h.u()
+    h.close()
+    ref = g[key + "_u"]
+    assert np.linalg.norm(u.ravel() - ref) / np.linalg.norm(ref) < 1e-12
+
+
+@pytest.mark.parametrize("ndim,key", [(2, "poisson2d"), (3, "poisson3d")])
+def test_poisson_cg_fixture(apps, golden, ndim, key):
+    g, m = golden
+    c = m[key]
+    nz = c.get("nz", 1)
+    f = O.poisson_rhs(m["seed"], ndim, c["nx"], c["ny"], nz)
+    P = apps.Poisson(ndim, c["nx"], c["ny"], nz, f=f)
+    k, hist = P.iterate(c["iters"], history=True)
+    for name in ("alpha", "beta"):
+        assert relerr(hist[name], g[f"{key}_{name}"]) < RTOL_RED, name
+    assert relerr(hist["rrn"], g[key + "_rrn"][1:]) < 1e-10
+    x = P.x()
+    P.close()
+    assert np.linalg.norm(x.ravel() - g[key + "_x"]) / np.linalg.norm(g[key + "_x"]) < 1e-12
+
+
+def test_heat_cg_app_inputs_match_oracle_generator(apps):
+    # the app's own input generator (used at full size) equals the oracle's:
+    # compare a few CG scalars started from generated inputs
+    n = (18, 14, 10)
+    h = apps.Heat(3, *n, seed=99)
+    k, hist = h.iterate(8, history=True)
+    h.close()
+    K, rhs = O.heat_inputs(99, 3, *n)
+    ref = O.heat_cg(3, *n, K, rhs, max_iters=8)
+    assert relerr(hist["alpha"], ref["alpha"]) < RTOL_RED
+
+
+# ------------------------------------------------------------------ stencils
+def _block3(ops, n, name):
+    blk = ops.ops_decl_block(3, name)
+    s000 = ops.ops_decl_stencil(3, 1, [0, 0, 0], "000")
+    return blk, s000
+
+
+def test_preproc_and_27pt_fixture(gpu_ops, golden):
+    ops = gpu_ops
+    g, m = golden
+    c = m["adi"]
+    nx, ny, nz = c["nx"], c["ny"], c["nz"]
+    blk, s000 = _block3(ops, (nx, ny, nz), "adi")
+    s7 = ops.ops_decl_stencil(3, 7, [0, 0, 0, -1, 0, 0, 1, 0, 0, 0, -1, 0, 0, 1, 0, 0, 0, -1, 0, 0, 1], "7pt")
+    pts = [v for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1) for v in (dx, dy, dz)]
+    s27 = ops.ops_decl_stencil(3, 27, pts, "27pt")
+    names = ["u", "du", "ax", "bx", "cx", "ay", "by", "cy", "az", "bz", "cz", "o"]
+    D = {n: ops.ops_decl_dat(blk, [nx, ny, nz], name=n) for n in names}
+    ops.set_ext(D["u"], g["adi_u_ext"].reshape(nz + 2, ny + 2, nx + 2))
+    ops.ops_decl_const("nx", 1, "int", nx)
+    ops.ops_decl_const("ny", 1, "int", ny)
+    ops.ops_decl_const("nz", 1, "int", nz)
+    ops.ops_decl_const("lambda", 1, "double", c["lambda"])
+    rng = [0, nx, 0, ny, 0, nz]
+    args = [ops.ops_arg_dat(D["u"], 1, s7, "double", ops.OPS_READ)]
+    args += [ops.ops_arg_dat(D[n], 1, s000, "double", ops.OPS_WRITE) for n in names[1:11]]
+    args += [ops.ops_arg_idx()]
+    ops.ops_par_loop("preproc_kernel", "preproc_kernel", blk, 3, rng, *args)
+    assert np.array_equal(ops.ops_dat_fetch_data(D["du"]).ravel(), g["adi_du"])
+    assert np.array_equal(ops.ops_dat_fetch_data(D["ax"]).ravel(), g["adi_ax"])
+    assert np.array_equal(ops.ops_dat_fetch_data(D["bx"]).ravel(), g["adi_bx"])
+    ops.ops_par_loop("stencil27_kernel", "stencil27", blk, 3, rng, ops.ops_arg_dat(D["o"], 1, s000, "double", ops.OPS_WRITE),
+                     ops.ops_arg_dat(D["u"], 1, s27, "double", ops.OPS_READ))
+    assert np.array_equal(ops.ops_dat_fetch_data(D["o"]).ravel(), g["st27_out"])
+    for d in D.values():
+        ops.ops_free_dat(d)
+
+
+def test_poisson_f32_sweep_fixture(gpu_ops, golden):
+    ops = gpu_ops
+    g, m = golden
+    c = m["poisson3d"]
+    nx, ny, nz = c["nx"], c["ny"], c["nz"]
+    blk, s000 = _block3(ops, (nx, ny, nz), "pf")
+    s7 = ops.ops_decl_stencil(3, 7, [0, 0, 0, -1, 0, 0, 1, 0, 0, 0, -1, 0, 0, 1, 0, 0, 0, -1, 0, 0, 1], "7pt")
+    p = ops.ops_decl_dat(blk, [nx, ny, nz], dtype=np.float32, name="pf")
+    w = ops.ops_decl_dat(blk, [nx, ny, nz], dtype=np.float32, name="wf")
+    f = O.poisson_rhs(m["seed"], 3, nx, ny, nz)
+    ops.ops_dat_set_data(p, f.astype(np.float32))
+    h = ops.ops_decl_reduction_handle(8, "double", "pw")
+    ops.ops_par_loop("poisson_calc_w_reduce_kernel", "pw", blk, 3, [0, nx, 0, ny, 0, nz],
+                     ops.ops_arg_dat(w, 1, s000, "float", ops.OPS_WRITE), ops.ops_arg_dat(p, 1, s7, "float", ops.OPS_READ),
+                     ops.ops_arg_reduce(h, 1, "double", ops.OPS_INC))
+    pw = ops.ops_reduction_result(h)
+    assert np.array_equal(ops.ops_dat_fetch_data(w).ravel(), g["poisson3d_f32_w"])
+    assert relerr(pw, g["poisson3d_f32_pw"][0]) < RTOL_RED
+
+
+# ------------------------------------------------------------------ reductions
+def test_reduction_semantics_fixture(gpu_ops, golden):
+    ops = gpu_ops
+    g, m = golden
+    c = m["red"]
+    nx, ny = c["nx"], c["ny"]
+    blk = ops.ops_decl_block(2, "red")
+    s00 = ops.ops_decl_stencil(2, 1, [0, 0], "00")
+    x = ops.ops_decl_dat(blk, [nx, ny], name="x")
+    ops.ops_dat_set_data(x, g["red_x"])
+    R = lambda n, sz=8, t="double": ops.ops_decl_reduction_handle(sz, t, n)
+    h_sum, h_min, h_max, h_two, h_imax = R("sum"), R("min"), R("max"), R("two", 16), R("imax", 4, "int")
+    inner, sub = [0, nx, 0, ny], [2, nx - 3, 1, ny - 2]
+    A = lambda acc: ops.ops_arg_dat(x, 1, s00, "double", acc)
+    red = lambda h, acc, dim=1, t="double": ops.ops_arg_reduce(h, dim, t, acc)
+    out = []
+    ops.ops_par_loop("red_sum", "sum", blk, 2, inner, A(ops.OPS_READ), red(h_sum, ops.OPS_INC))
+    out.append(ops.ops_reduction_result(h_sum))
+    ops.ops_par_loop("red_min", "min", blk, 2, inner, A(ops.OPS_READ), red(h_min, ops.OPS_MIN))
+    out.append(ops.ops_reduction_result(h_min))
+    ops.ops_par_loop("red_max", "max", blk, 2, inner, A(ops.OPS_READ), red(h_max, ops.OPS_MAX))
+    out.append(ops.ops_reduction_result(h_max))
+    ops.ops_par_loop("red_sum", "sum", blk, 2, inner, A(ops.OPS_READ), red(h_sum, ops.OPS_INC))
+    ops.ops_par_loop("red_sumsq", "sumsq", blk, 2, sub, A(ops.OPS_READ), red(h_sum, ops.OPS_INC))
+    out.append(ops.ops_reduction_result(h_sum))
+    ops.ops_par_loop("red_abs", "abs", blk, 2, inner, A(ops.OPS_READ), red(h_sum, ops.OPS_INC))
+    out.append(ops.ops_reduction_result(h_sum))
+    ops.ops_par_loop("red_min", "min", blk, 2, sub, A(ops.OPS_READ), red(h_min, ops.OPS_MIN))
+    out.append(ops.ops_reduction_result(h_min))
+    ops.ops_par_loop("red_max", "max", blk, 2, sub, A(ops.OPS_READ), red(h_max, ops.OPS_MAX))
+    out.append(ops.ops_reduction_result(h_max))
+    ops.ops_par_loop("red_two", "two", blk, 2, inner, A(ops.OPS_READ), red(h_two, ops.OPS_INC, 2))
+    out += list(ops.ops_reduction_result(h_two))
+    ops.ops_par_loop("red_imax", "imax", blk, 2, inner, ops.ops_arg_idx(), red(h_imax, ops.OPS_MAX, 1, "int"))
+    out.append(float(ops.ops_reduction_result(h_imax)))
+    ops.ops_par_loop("red_scaled", "scaled", blk, 2, inner, A(ops.OPS_READ), ops.ops_arg_gbl(-2.5, 1, "double"),
+                     red(h_sum, ops.OPS_INC))
+    out.append(ops.ops_reduction_result(h_sum))
+    ops.ops_par_loop("red_min", "min", blk, 2, [3, 3, 0, ny], A(ops.OPS_READ), red(h_min, ops.OPS_MIN))
+    out.append(ops.ops_reduction_result(h_min))
+    ref = g["red_out"]
+    exact = [1, 2, 5, 6, 8, 9, 11]
+    for i, (a, b) in enumerate(zip(out, ref)):
+        if i in exact:
+            assert a == b, i
+        else:
+            assert relerr(a, b) < RTOL_RED, i
+
+
+def test_reduction_large_deterministic(gpu_ops):
+    # many blocks: the last-block combine is order-fixed, so repeats agree
+    ops = gpu_ops
+    blk = ops.ops_decl_block(3, "big")
+    s000 = ops.ops_decl_stencil(3, 1, [0, 0, 0], "000")
+    n = (300, 200, 150)
+    x = ops.ops_decl_dat(blk, list(n), name="x")
+    v = O.u01(5, 5, n[0] * n[1] * n[2]) * 2 - 1
+    ops.ops_dat_set_data(x, v)
+    h = ops.ops_decl_reduction_handle(8, "double", "s")
+    vals = []
+    for _ in range(3):
+        ops.ops_par_loop("red_sum", "sum", blk, 3, [0, n[0], 0, n[1], 0, n[2]],
+                         ops.ops_arg_dat(x, 1, s000, "double", ops.OPS_READ), ops.ops_arg_reduce(h, 1, "double", ops.OPS_INC))
+        vals.append(ops.ops_reduction_result(h))
+    assert vals[0] == vals[1] == vals[2]
+    assert relerr(vals[0], np.sum(v)) < 1e-11
+    ops.ops_free_dat(x)
+
+
+# ------------------------------------------------------------------ data access
+def test_fetch_set_roundtrip_and_padding(gpu_ops):
+    ops = gpu_ops
+    blk = ops.ops_decl_block(3, "rt")
+    n = (130, 7, 5)
+    x = ops.ops_decl_dat(blk, list(n), name="x")
+    core = x.core
+    assert core.size[0] >= n[0] + 2 and core.d_m[0] == -1
+    v = np.arange(np.prod(n), dtype=np.float64).reshape(n[2], n[1], n[0])
+    ops.ops_dat_set_data(x, v)
+    assert np.array_equal(ops.ops_dat_fetch_data(x), v)
+    ext = ops.get_ext(x)
+    assert ext.shape == (n[2] + 2, n[1] + 2, n[0] + 2)
+    assert np.array_equal(ext[1:-1, 1:-1, 1:-1], v) and np.all(ext[0] == 0)
+    e2 = np.random.default_rng(1).random(ext.shape)
+    ops.set_ext(x, e2)
+    assert np.array_equal(ops.get_ext(x), e2)
+    ops.ops_free_dat(x)
+
+
+def test_random_fill_uniform_matches_generator(gpu_ops):
+    ops = gpu_ops
+    blk = ops.ops_decl_block(2, "rng")
+    nx, ny = 150, 9
+    x = ops.ops_decl_dat(blk, [nx, ny], name="x")
+    ops.ops_randomgen_init(77)
+    ops.ops_fill_random_uniform(x)
+    ext = ops.get_ext(x)
+    ref = O.u01(77, 0, (nx + 2) * (ny + 2)).reshape(ny + 2, nx + 2)
+    assert np.array_equal(ext, ref)
+    ops.ops_fill_random_normal(x)
+    ext = ops.get_ext(x)
+    refn = O.normal(77, 1, (nx + 2) * (ny + 2)).reshape(ny + 2, nx + 2)
+    assert np.max(np.abs(ext - refn)) < 1e-13
+
+
+# ------------------------------------------------------------------ tridiagonal
+def _trid_host(seed, nx, ny, nz, sd):
+    a, b, c = O.trid_inputs(seed, sd, nx, ny, nz)
+    return a, b, c, O.trid_rhs(seed, nx, ny, nz)
+
+
+@pytest.mark.parametrize("strategy", [0, 3])
+def test_trid_directions_bit_exact(apps, strategy):
+    nx, ny, nz = 40, 36, 30
+    T = apps.Trid(nx, ny, nz, seed=13, strategy=strategy)
+    fn = O.pcr_batch if strategy == 3 else O.thomas_batch
+    d_ref = O.trid_rhs(13, nx, ny, nz)
+    for sd in range(3):
+        T.solve(sd)
+        a, b, c = O.trid_inputs(13, sd, nx, ny, nz)
+        fn(3, sd, [nx, ny, nz], [nx, ny, nz], a, b, c, d_ref, nthreads=NT)
+        d, _ = T.fetch()
+        assert np.array_equal(d, d_ref), sd
+    # z accumulate into u
+    u_ref = np.zeros_like(d_ref)
+    T.solve(2, inc=True)
+    a, b, c = O.trid_inputs(13, 2, nx, ny, nz)
+    fn(3, 2, [nx, ny, nz], [nx, ny, nz], a, b, c, d_ref, u_ref, nthreads=NT)
+    d, u = T.fetch()
+    assert np.array_equal(u, u_ref) and np.array_equal(d, d_ref)
+    T.close()
+
+
+def test_trid_full_size_thomas_and_pcr(apps):
+    # the 256^3 ADI workload: Thomas bit-exact vs the CPU path, PCR within 1e-13
+    n = 256
+    Tt = apps.Trid(n, n, n, seed=21, strategy=0)
+    Tp = apps.Trid(n, n, n, seed=21, strategy=3)
+    Tt.sweep()
+    Tp.sweep()
+    dt, _ = Tt.fetch()
+    dp, _ = Tp.fetch()
+    Tt.close()
+    Tp.close()
+    d = O.trid_rhs(21, n, n, n)
+    for sd in range(3):
+        a, b, c = O.trid_inputs(21, sd, n, n, n)
+        O.thomas_batch(3, sd, [n, n, n], [n, n, n], a, b, c, d, nthreads=NT)
+    assert np.array_equal(dt, d)
+    assert np.max(np.abs(dp - d)) <= 1e-13 * np.max(np.abs(d))
+
+
+# ------------------------------------------------------------------ halo transfer
+def test_halo_transfer_flipped(gpu_ops):
+    ops = gpu_ops
+    blk = ops.ops_decl_block(2, "hb")
+    a = ops.ops_decl_dat(blk, [6, 5], name="a")
+    b = ops.ops_decl_dat(blk, [6, 5], name="b")
+    v = np.arange(30, dtype=np.float64).reshape(5, 6)
+    ops.ops_dat_set_data(a, v)
+    # copy a's last interior column into b's left halo, y reversed
+    h = ops.lib().ops_decl_halo(a.ptr, b.ptr, ops._ints([1, 5]), ops._ints([5, 0]), ops._ints([-1, 0]),
+                                ops._ints([1, 2]), ops._ints([1, -2]))
+    ops._check()
+    grp = ops.lib().ops_decl_halo_group(1, (ops.C.c_void_p * 1)(h))
+    ops._check()
+    ops._call("ops_halo_transfer", grp)
+    ext = ops.get_ext(b)
+    assert np.array_equal(ext[1:6, 0], v[::-1, 5])
+
+
+# ------------------------------------------------------------------ errors
+def test_validation_errors(gpu_ops):
+    ops = gpu_ops
+    blk = ops.ops_decl_block(2, "err")
+    s00 = ops.ops_decl_stencil(2, 1, [0, 0], "00")
+    A = ops.ops_decl_dat(blk, [10, 10], name="A")
+    An = ops.ops_decl_dat(blk, [10, 10], name="An")
+    h = ops.ops_decl_reduction_handle(8, "double", "e")
+    # stencil lacks the neighbours read by apply_stencil
+    with pytest.raises(ops.OPSError, match="lacks point"):
+        ops.ops_par_loop("apply_stencil", "a", blk, 2, [0, 10, 0, 10], ops.ops_arg_dat(A, 1, s00, "double", ops.OPS_READ),
+                         ops.ops_arg_dat(An, 1, s00, "double", ops.OPS_WRITE), ops.ops_arg_reduce(h, 1, "double", ops.OPS_MAX))
+    ops.ops_reduction_result(h)
+    s5 = ops.ops_decl_stencil(2, 5, [0, 0, 1, 0, -1, 0, 0, 1, 0, -1], "5pt")
+    # range reaching past the halo
+    with pytest.raises(ops.OPSError, match="leaves dat"):
+        ops.ops_par_loop("apply_stencil", "a", blk, 2, [0, 11, 0, 10], ops.ops_arg_dat(A, 1, s5, "double", ops.OPS_READ),
+                         ops.ops_arg_dat(An, 1, s00, "double", ops.OPS_WRITE), ops.ops_arg_reduce(h, 1, "double", ops.OPS_MAX))
+    ops.ops_reduction_result(h)
+    # OPS_WRITE with a non-point stencil (ops_lib_core.cpp:1054-1056)
+    with pytest.raises(ops.OPSError, match="OPS_WRITE"):
+        ops.ops_arg_dat(An, 1, s5, "double", ops.OPS_WRITE)
+    # a reduction handle reused with another access type
+    ops.ops_arg_reduce(h, 1, "double", ops.OPS_MAX)
+    with pytest.raises(ops.OPSError, match="different access type"):
+        ops.ops_arg_reduce(h, 1, "double", ops.OPS_MIN)
