@@ -1,20 +1,19 @@
 // Device loop engine shared by the hand-written ops_par_loop kernels.
 //
-// Work mapping: a workgroup is 64 x 4 threads (four wave64s).  Each wave owns
-// one (y,z) row of the loop range at a time and its 64 lanes stride along x,
-// so every global access of a wave is one contiguous 512-byte (f64) segment.
-// Rows are dealt to waves grid-stride; the grid is capped at 8 workgroups per
-// CU, so every thread visits many points and the number of reduction partials
-// stays small.
+// Work mapping: a workgroup is 64 x 4 threads (four wave64s); lanes run
+// along x, so every global access of a wave is one contiguous 512-byte (f64)
+// segment.  Non-reducing loops give each thread one point: short-lived waves
+// keep the HBM pipeline full.  Workgroups are dealt to the 8 XCDs
+// round-robin by the hardware, so the engine renumbers them: XCD k works
+// through the k-th contiguous eighth of the tiles, and stencil neighbours
+// (adjacent rows and planes) are mostly found in the same XCD's L2.
 //
-// Reductions: each thread folds its points into registers in its own order,
-// then the block reduces with wave64 butterflies (__shfl_xor) and LDS, writes
-// one partial, and takes a ticket.  The last block to finish combines all
-// partials in a fixed order (so results do not depend on scheduling), folds
-// in the handle's previous value (or the OPS initial value on first use),
-// writes the handle's device slot and resets the ticket.  Partials and the
-// ticket go through device-scope atomics so blocks on different XCDs (each
-// with its own L2) see each other's writes.
+// Reductions are two-stage and fence-free: each thread folds its points into
+// registers, the block folds them with wave64 butterflies (__shfl_xor) and
+// LDS and stores one partial; a second kernel folds the partials in a fixed
+// order (spans of 2048, then one block) and combines the result with the
+// handle's running value (or the OPS initial value on first use).  The order
+// depends only on the launch shape, so results are reproducible run to run.
 #pragma once
 
 #include <float.h>
@@ -28,6 +27,7 @@ namespace opsmi {
 
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
+constexpr int kSpan = 2048;  // partials folded by one second-stage block
 
 template <class T>
 struct Ptr {
@@ -96,15 +96,6 @@ __device__ __forceinline__ float init_value<float>(const RedCtl &c, int q) { ret
 template <>
 __device__ __forceinline__ int init_value<int>(const RedCtl &c, int q) { return c.init_i[q]; }
 
-template <class T>
-__device__ __forceinline__ void store_agent(T *p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <class T>
-__device__ __forceinline__ T load_agent(const T *p) {
-  return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 template <class T, int OP>
 __device__ __forceinline__ T wave_reduce(T v) {
 #pragma unroll
@@ -134,61 +125,163 @@ __device__ __forceinline__ void block_reduce(T (&v)[D], T (*sh)[D]) {
   }
 }
 
+__device__ __forceinline__ unsigned flat_block() {
+  return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+}
+__device__ __forceinline__ unsigned num_blocks() { return gridDim.x * gridDim.y * gridDim.z; }
+
+// XCD-aware renumbering: hardware block b runs on XCD b % 8; it takes logical
+// tile start(b % 8) + b / 8, where XCD k owns a contiguous logical range of
+// its floor/ceil share of the nb tiles.  A bijection on [0, nb).
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
+  const unsigned q8 = nb >> 3, r8 = nb & 7u, k = b & 7u;
+  return k * q8 + (k < r8 ? k : r8) + (b >> 3);
+}
+
+// stage 1: block fold + one partial per block
 template <class T, int OP, int D>
-__device__ void reduce_finish(T (&acc)[D], const RedCtl &c) {
+__device__ __forceinline__ void reduce_block_store(T (&acc)[D], const RedCtl &c) {
   __shared__ T sh[16][D];
-  __shared__ int is_last;
-  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  const int nthr = blockDim.x * blockDim.y;
-  T *part = (T *)c.partials;
   block_reduce<T, OP, D>(acc, sh);
-  if (tid == 0) {
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    T *part = (T *)c.partials;
 #pragma unroll
-    for (int q = 0; q < D; q++) store_agent(&part[(size_t)blockIdx.x * D + q], acc[q]);
-    __threadfence();
-    unsigned t = atomicAdd(c.ticket, 1u);
-    is_last = (t == gridDim.x - 1);
+    for (int q = 0; q < D; q++) part[(size_t)flat_block() * D + q] = acc[q];
   }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
+}
+
+// stage 2: block b folds in[b*kSpan, (b+1)*kSpan) (eight loads per thread)
+template <class T, int OP, int D>
+__global__ __launch_bounds__(256) void k_red_stage2(const T *__restrict__ in, unsigned n, T *__restrict__ out) {
+  __shared__ T sh[16][D];
+  const unsigned tid = threadIdx.x, base = blockIdx.x * kSpan;
+  T v[D];
+#pragma unroll
+  for (int q = 0; q < D; q++) {
+    T t8[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const unsigned i = base + tid + u * 256u;
+      t8[u] = i < n ? in[(size_t)i * D + q] : RedOp<T, OP>::identity();
+    }
+    v[q] = t8[0];
+#pragma unroll
+    for (int u = 1; u < 8; u++) v[q] = RedOp<T, OP>::combine(v[q], t8[u]);
+  }
+  block_reduce<T, OP, D>(v, sh);
+  if (tid == 0)
+#pragma unroll
+    for (int q = 0; q < D; q++) out[(size_t)blockIdx.x * D + q] = v[q];
+}
+
+// final: one block folds n values and updates the handle's device slot
+template <class T, int OP, int D>
+__global__ __launch_bounds__(256) void k_red_final(const T *__restrict__ in, unsigned n, RedCtl c) {
+  __shared__ T sh[16][D];
+  const unsigned tid = threadIdx.x;
   T v[D];
 #pragma unroll
   for (int q = 0; q < D; q++) {
     v[q] = RedOp<T, OP>::identity();
-    for (int b = tid; b < (int)gridDim.x; b += nthr) v[q] = RedOp<T, OP>::combine(v[q], load_agent(&part[(size_t)b * D + q]));
+    unsigned i = tid;
+    for (; i + 7 * 256u < n; i += 8 * 256u) {
+      T t8[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) t8[u] = in[(size_t)(i + u * 256u) * D + q];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[q] = RedOp<T, OP>::combine(v[q], t8[u]);
+    }
+    for (; i < n; i += 256u) v[q] = RedOp<T, OP>::combine(v[q], in[(size_t)i * D + q]);
   }
-  __syncthreads();
   block_reduce<T, OP, D>(v, sh);
   if (tid == 0) {
     T *slot = (T *)c.slot;
 #pragma unroll
     for (int q = 0; q < D; q++) {
-      T base = c.use_init ? init_value<T>(c, q) : load_agent(&slot[q]);
-      store_agent(&slot[q], RedOp<T, OP>::combine(base, v[q]));
+      const T base = c.use_init ? init_value<T>(c, q) : slot[q];
+      slot[q] = RedOp<T, OP>::combine(base, v[q]);
     }
-    store_agent(c.ticket, 0u);
   }
+}
+
+// launches the fold of `parts` stage-1 partials into the handle
+template <class T, int OP, int D>
+void launch_red_finish(const RedCtl &c, unsigned parts, hipStream_t s) {
+  if (parts <= 4u * kSpan) {
+    hipLaunchKernelGGL((k_red_final<T, OP, D>), dim3(1), dim3(256), 0, s, (const T *)c.partials, parts, c);
+  } else {
+    const unsigned g2 = (parts + kSpan - 1) / kSpan;
+    hipLaunchKernelGGL((k_red_stage2<T, OP, D>), dim3(g2), dim3(256), 0, s, (const T *)c.partials, parts,
+                       (T *)c.partials2);
+    hipLaunchKernelGGL((k_red_final<T, OP, D>), dim3(1), dim3(256), 0, s, (const T *)c.partials2, g2, c);
+  }
+}
+
+// Tiling of a loop range into 64 x 4 workgroups.  3D ranges: tile (tx, ty,
+// tz) covers x in 64 tx + [0,64), y in 4 ty + [0,4) and `chunk` z planes
+// from chunk*tz.  2D/1D ranges: tile (tx, ty) covers 64 x-points and `chunk`
+// groups of 4 rows.
+struct Tiling {
+  dim3 grid;
+  int chunk;
+  bool three_d;
+  unsigned blocks() const { return grid.x * grid.y * grid.z; }
+};
+
+inline Tiling make_tiling(const Range3 &r, int chunk = 1) {
+  Tiling t;
+  const long nx = r.x1 - r.x0, ny = r.y1 - r.y0, nz = r.z1 - r.z0;
+  t.three_d = nz > 1;
+  const long gx = (nx + kWave - 1) / kWave, gy = (ny + kWavesPerBlock - 1) / kWavesPerBlock;
+  const long planes = t.three_d ? nz : gy;
+  const long gc = (planes + chunk - 1) / chunk;
+  if (t.three_d) t.grid = dim3((unsigned)gx, (unsigned)gy, (unsigned)gc);
+  else t.grid = dim3((unsigned)gx, (unsigned)gc, 1);
+  t.chunk = chunk;
+  return t;
+}
+
+// logical tile coordinates of this workgroup (XCD-aware)
+struct TileId {
+  unsigned x, y, z;
+};
+__device__ __forceinline__ TileId tile_id() {
+  const unsigned l = xcd_tile(flat_block(), num_blocks());
+  TileId t;
+  t.x = l % gridDim.x;
+  const unsigned rest = l / gridDim.x;
+  t.y = rest % gridDim.y;
+  t.z = rest / gridDim.y;
+  return t;
 }
 
 // F provides: using red_t; static constexpr int red_op, red_dim;
 //   __device__ void operator()(int x, int y, int z, red_t (&acc)[red_dim or 1]) const
 template <class F>
-__global__ __launch_bounds__(256) void k_loop(Range3 r, F f, RedCtl ctl) {
+__global__ __launch_bounds__(256) void k_loop(Range3 r, F f, RedCtl ctl, int chunk, int three_d) {
   using R = typename F::red_t;
   constexpr int D = F::red_dim > 0 ? F::red_dim : 1;
   constexpr int OP = F::red_op;
   R acc[D];
 #pragma unroll
   for (int q = 0; q < D; q++) acc[q] = RedOp<R, OP>::identity();
-  const int lane = threadIdx.x;
-  const int ny = r.y1 - r.y0;
-  const long nrows = (long)ny * (r.z1 - r.z0);
-  for (long row = (long)blockIdx.x * blockDim.y + threadIdx.y; row < nrows; row += (long)gridDim.x * blockDim.y) {
-    const int y = r.y0 + (int)(row % ny), z = r.z0 + (int)(row / ny);
-    for (int x = r.x0 + lane; x < r.x1; x += kWave) f(x, y, z, acc);
+  const TileId t = tile_id();
+  const int x = r.x0 + (int)t.x * kWave + (int)threadIdx.x;
+  if (x < r.x1) {
+    if (three_d) {
+      const int y = r.y0 + (int)t.y * kWavesPerBlock + (int)threadIdx.y;
+      if (y < r.y1) {
+        const int z0 = r.z0 + (int)t.z * chunk;
+        const int z1 = min(z0 + chunk, r.z1);
+        for (int z = z0; z < z1; z++) f(x, y, z, acc);
+      }
+    } else {
+      const int y0 = r.y0 + (int)t.y * kWavesPerBlock * chunk;
+      const int y1 = min(y0 + kWavesPerBlock * chunk, r.y1);
+      for (int y = y0 + (int)threadIdx.y; y < y1; y += kWavesPerBlock) f(x, y, r.z0, acc);
+    }
   }
-  if constexpr (F::red_dim > 0) reduce_finish<R, OP, D>(acc, ctl);
+  if constexpr (F::red_dim > 0) reduce_block_store<R, OP, D>(acc, ctl);
 }
 
 template <class T>
@@ -203,12 +296,13 @@ void launch_loop(const char *name, double bytes, const Range3 &r, const F &f, co
     run_loop(name, 0.0, [](hipStream_t) {});
     return;
   }
-  long rows = (long)(r.y1 - r.y0) * (r.z1 - r.z0);
-  int nb = grid_blocks(rows, kWavesPerBlock);
+  Tiling t = make_tiling(r, 1);
   RedCtl ctl{};
-  if constexpr (F::red_dim > 0) ctl = red_prepare(*red_arg, F::red_dim, dtype_v<typename F::red_t>(), nb);
+  if constexpr (F::red_dim > 0) ctl = red_prepare(*red_arg, F::red_dim, dtype_v<typename F::red_t>(), t.blocks());
   run_loop(name, bytes, [&](hipStream_t s) {
-    hipLaunchKernelGGL(k_loop<F>, dim3(nb), dim3(kWave, kWavesPerBlock), 0, s, r, f, ctl);
+    hipLaunchKernelGGL(k_loop<F>, t.grid, dim3(kWave, kWavesPerBlock), 0, s, r, f, ctl, t.chunk, t.three_d ? 1 : 0);
+    if constexpr (F::red_dim > 0)
+      launch_red_finish<typename F::red_t, F::red_op, F::red_dim>(ctl, t.blocks(), s);
   });
 }
 

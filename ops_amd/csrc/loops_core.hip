@@ -168,6 +168,92 @@ struct CalcW {
     s[0] = s[0] + v * p(x, y, z);
   }
 };
+// 3D TeaLeaf operator, z-marching form.  A 64 x 4 workgroup owns a 64 x 4
+// column of the grid over `chunk` z planes; each thread keeps p(z-1), p(z),
+// p(z+1) and Kz(z), Kz(z+1) in registers, so only the new plane is loaded
+// per step and the z neighbours never travel from HBM twice.  x and y
+// neighbours come from the same or adjacent rows of the wave's workgroup
+// (L1/L2 hits).  Same arithmetic, in the same order, as heat_op<3>.
+// INIT = tea_leaf_common_init_kernel (w = A u, r = u - w); otherwise
+// tea_leaf_cg_calc_w_reduce_kernel (w = A p, pw += w p).
+template <bool INIT>
+__global__ __launch_bounds__(256) void k_heat3_march(Ptr<double> w, Ptr<double> r, Ptr<const double> Kx,
+                                                     Ptr<const double> Ky, Ptr<const double> Kz,
+                                                     Ptr<const double> p, double rx, double ry, double rz, Range3 rg,
+                                                     int chunk, RedCtl ctl) {
+  double acc[1] = {0.0};
+  const TileId t = tile_id();
+  const int x = rg.x0 + (int)t.x * kWave + (int)threadIdx.x;
+  const int y = rg.y0 + (int)t.y * kWavesPerBlock + (int)threadIdx.y;
+  if (x < rg.x1 && y < rg.y1) {
+    const int z0 = rg.z0 + (int)t.z * chunk;
+    const int z1 = min(z0 + chunk, rg.z1);
+    // state of plane z: centre column p(z-1), p(z), p(z+1), Kz(z), Kz(z+1)
+    // and the in-plane neighbours.  The next plane's loads are issued before
+    // this plane's stores, so they are in flight while it is written.
+    double pm = p(x, y, z0 - 1), pc = p(x, y, z0), pn = p(x, y, z0 + 1);
+    double kzc = Kz(x, y, z0), kzn = Kz(x, y, z0 + 1);
+    double kxc = Kx(x, y, z0), kxn = Kx(x + 1, y, z0), kyc = Ky(x, y, z0), kyn = Ky(x, y + 1, z0);
+    double pxm = p(x - 1, y, z0), pxp = p(x + 1, y, z0), pym = p(x, y - 1, z0), pyp = p(x, y + 1, z0);
+    for (int z = z0; z < z1; z++) {
+      const double v = (1.0 + ry * (kyn + kyc) + rx * (kxn + kxc) + rz * (kzn + kzc)) * pc -
+                       ry * (kyn * pyp + kyc * pym) - rx * (kxn * pxp + kxc * pxm) - rz * (kzn * pn + kzc * pm);
+      const double pc_v = pc;
+      if (z + 1 < z1) {
+        const int zn = z + 1;
+        const double pnn = p(x, y, zn + 1), kznn = Kz(x, y, zn + 1);
+        kxc = Kx(x, y, zn);
+        kxn = Kx(x + 1, y, zn);
+        kyc = Ky(x, y, zn);
+        kyn = Ky(x, y + 1, zn);
+        pxm = p(x - 1, y, zn);
+        pxp = p(x + 1, y, zn);
+        pym = p(x, y - 1, zn);
+        pyp = p(x, y + 1, zn);
+        pm = pc;
+        pc = pn;
+        pn = pnn;
+        kzc = kzn;
+        kzn = kznn;
+      }
+      w(x, y, z) = v;
+      if constexpr (INIT) r(x, y, z) = pc_v - v;
+      else acc[0] = acc[0] + v * pc_v;
+    }
+  }
+  if constexpr (!INIT) reduce_block_store<double, OPS_INC, 1>(acc, ctl);
+}
+
+// launches the z-marching 3D heat kernel; 16 z planes per workgroup by
+// default (measured best among 1-16 on 512^3), OPSMI_HEAT_ZCHUNK overrides
+template <bool INIT>
+void launch_heat3(const char *name, double bytes, const Range3 &r, Ptr<double> w, Ptr<double> rr,
+                  Ptr<const double> Kx, Ptr<const double> Ky, Ptr<const double> Kz, Ptr<const double> p, double rx,
+                  double ry, double rz, const ops_arg *red) {
+  if (r.npts() <= 0) {
+    run_loop(name, 0.0, [](hipStream_t) {});
+    return;
+  }
+  const int nx = r.x1 - r.x0, ny = r.y1 - r.y0, nz = r.z1 - r.z0;
+  const unsigned gx = (nx + kWave - 1) / kWave, gy = (ny + kWavesPerBlock - 1) / kWavesPerBlock;
+  int chunk = 16;
+  const char *env = getenv("OPSMI_HEAT_ZCHUNK");
+  if (env) chunk = std::max(1, atoi(env));
+  const unsigned gz = (nz + chunk - 1) / chunk;
+  RedCtl ctl{};
+  if (!INIT) ctl = red_prepare(*red, 1, DT_F64, gx * gy * gz);
+  run_loop(name, bytes, [&](hipStream_t s) {
+    hipLaunchKernelGGL(k_heat3_march<INIT>, dim3(gx, gy, gz), dim3(kWave, kWavesPerBlock), 0, s, w, rr, Kx, Ky, Kz,
+                       p, rx, ry, rz, r, chunk, ctl);
+    if (!INIT) launch_red_finish<double, OPS_INC, 1>(ctl, gx * gy * gz, s);
+  });
+}
+
+bool use_march() {
+  const char *e = getenv("OPSMI_HEAT_MARCH");
+  return !(e && e[0] == '0');
+}
+
 struct CalcUrR {
   using red_t = double;
   static constexpr int red_op = OPS_INC;
@@ -524,10 +610,15 @@ void ops_par_loop_tea_leaf_common_init_kernel_3d(const char *name, ops_block blo
                          dat_r(kz_pts(), DT_F64), dat_r(pts_7pt(), DT_F64), gbl_r(), gbl_r(), gbl_r()},
                         nullptr);
   if (dim != 3) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 3D kernel on a ", dim, "D block"));
-  launch_loop(name, BYTES(r), r,
-              CommonInit<3>{ptr_of<double>(w.dat), ptr_of<double>(rr.dat), ptr_of<const double>(Kx.dat),
-                            ptr_of<const double>(Ky.dat), ptr_of<const double>(Kz.dat), ptr_of<const double>(u.dat),
-                            gbl<double>(rx), gbl<double>(ry), gbl<double>(rz)});
+  if (use_march())
+    launch_heat3<true>(name, BYTES(r), r, ptr_of<double>(w.dat), ptr_of<double>(rr.dat), ptr_of<const double>(Kx.dat),
+                       ptr_of<const double>(Ky.dat), ptr_of<const double>(Kz.dat), ptr_of<const double>(u.dat),
+                       gbl<double>(rx), gbl<double>(ry), gbl<double>(rz), nullptr);
+  else
+    launch_loop(name, BYTES(r), r,
+                CommonInit<3>{ptr_of<double>(w.dat), ptr_of<double>(rr.dat), ptr_of<const double>(Kx.dat),
+                              ptr_of<const double>(Ky.dat), ptr_of<const double>(Kz.dat), ptr_of<const double>(u.dat),
+                              gbl<double>(rx), gbl<double>(ry), gbl<double>(rz)});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -559,11 +650,16 @@ void ops_par_loop_tea_leaf_cg_calc_w_reduce_kernel_3d(const char *name, ops_bloc
                          dat_r(pts_7pt(), DT_F64), gbl_r(), gbl_r(), gbl_r(), red_a(OPS_INC, DT_F64)},
                         nullptr);
   if (dim != 3) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 3D kernel on a ", dim, "D block"));
-  launch_loop(name, BYTES(r), r,
-              CalcW<3>{ptr_of<double>(w.dat), ptr_of<const double>(Kx.dat), ptr_of<const double>(Ky.dat),
-                       ptr_of<const double>(Kz.dat), ptr_of<const double>(p.dat), gbl<double>(rx), gbl<double>(ry),
-                       gbl<double>(rz)},
-              &pw);
+  if (use_march())
+    launch_heat3<false>(name, BYTES(r), r, ptr_of<double>(w.dat), Ptr<double>{nullptr, 0, 0},
+                        ptr_of<const double>(Kx.dat), ptr_of<const double>(Ky.dat), ptr_of<const double>(Kz.dat),
+                        ptr_of<const double>(p.dat), gbl<double>(rx), gbl<double>(ry), gbl<double>(rz), &pw);
+  else
+    launch_loop(name, BYTES(r), r,
+                CalcW<3>{ptr_of<double>(w.dat), ptr_of<const double>(Kx.dat), ptr_of<const double>(Ky.dat),
+                         ptr_of<const double>(Kz.dat), ptr_of<const double>(p.dat), gbl<double>(rx), gbl<double>(ry),
+                         gbl<double>(rz)},
+                &pw);
   finish_loop(args, nargs);
   OPSMI_API_END()
 }

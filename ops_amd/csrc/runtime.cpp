@@ -276,7 +276,7 @@ void ensure_partials(size_t bytes) {
     OPSMI_HIP(hipStreamSynchronize(I.stream));
     OPSMI_HIP(hipFree(I.d_partials));
   }
-  size_t nb = std::max(bytes, (size_t)65536);
+  size_t nb = std::max(bytes, (size_t)1 << 20);
   OPSMI_HIP(hipMalloc(&I.d_partials, nb));
   I.partials_bytes = nb;
 }
@@ -289,7 +289,7 @@ RedCtl red_prepare(const ops_arg &a, int dim, DType dt, int nblocks) {
   RedCtl c{};
   c.slot = ri->d_slot;
   c.partials = I.d_partials;
-  c.ticket = I.d_ticket;
+  c.partials2 = I.d_partials2;
   c.use_init = ri->dev_valid ? 0 : 1;
   for (int q = 0; q < dim && q < 4; q++) {
     if (dt == DT_F64) memcpy(&c.init[q], h->data + 8 * q, 8);
@@ -401,9 +401,8 @@ void ops_init(const int argc, const char *const argv[], const int diags_level) {
   OPSMI_HIP(hipGetDeviceProperties(&prop, dev));
   I.num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   OPSMI_HIP(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
-  OPSMI_HIP(hipMalloc(&I.d_ticket, 256));
-  OPSMI_HIP(hipMemset(I.d_ticket, 0, 256));
-  ensure_partials((size_t)I.num_cus * 8 * 4 * 8);
+  ensure_partials((size_t)1 << 20);
+  OPSMI_HIP(hipMalloc(&I.d_partials2, (size_t)1 << 20));
   OPSMI_HIP(hipHostMalloc(&I.h_pinned, 4096));
   I.initialised = true;
   OPSMI_API_END()
@@ -458,7 +457,8 @@ void ops_exit(void) {
   (void)hipFree(I.d_partials);
   I.d_partials = nullptr;
   I.partials_bytes = 0;
-  (void)hipFree(I.d_ticket);
+  (void)hipFree(I.d_partials2);
+  I.d_partials2 = nullptr;
   (void)hipHostFree(I.h_pinned);
   (void)hipStreamDestroy(I.stream);
   I.consts.clear();

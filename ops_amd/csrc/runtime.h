@@ -107,10 +107,11 @@ struct Instance {
   std::map<std::string, Const> consts;
   std::map<std::string, LoopStat> stats;
 
-  // reduction scratch for the one in-order stream: block partials + ticket
+  // reduction scratch for the one in-order stream: block partials of the
+  // loop kernel and of the second reduction stage
   void *d_partials = nullptr;
   size_t partials_bytes = 0;
-  unsigned int *d_ticket = nullptr;
+  void *d_partials2 = nullptr;
 
   // host staging for reduction results
   void *h_pinned = nullptr;
@@ -185,8 +186,8 @@ int const_as_int(const char *name);
 // Device-side control block passed to reducing kernels.
 struct RedCtl {
   void *slot;          // handle's device value
-  void *partials;      // per-block partials (dim values per block)
-  unsigned *ticket;    // zero between launches
+  void *partials;      // per-block partials of the loop kernel (dim values per block)
+  void *partials2;     // second-stage partials
   int use_init;        // 1: start from init[] (first use since re-arm)
   double init[4];      // initial values, stored as the reduction type
   int init_i[4];

@@ -106,22 +106,146 @@ def test_laplace_reference_known_answer(apps, golden):
 
 # ------------------------------------------------------------------ CG
 @pytest.mark.parametrize("ndim,key", [(2, "cg2d"), (3, "heat3d")])
-def test_heat_cg_fixture(apps, golden, ndim, key):
+def test_heat_cg_fixture_trajectory(apps, golden, ndim, key):
+    # CG amplifies last-bit differences of the dot products (their order
+    # differs from the serial reference), so whole trajectories are compared
+    # only over the first iterations; per-loop parity at every iteration is
+    # test_heat_cg_lockstep
     g, m = golden
     c = m[key]
     nz = c.get("nz", 1)
     K, rhs = O.heat_inputs(m["seed"], ndim, c["nx"], c["ny"], nz)
     h = apps.Heat(ndim, c["nx"], c["ny"], nz, K_ext=K, rhs=rhs)
     assert relerr(h.rr(initial=True), g[key + "_rrn"][0]) < RTOL_RED
-    k, hist = h.iterate(c["iters"], history=True)
-    assert k == c["iters"]
+    k, hist = h.iterate(10, history=True)
+    h.close()
     for name in ("pw", "alpha", "beta"):
-        assert relerr(hist[name], g[f"{key}_{name}"]) < RTOL_RED, name
-    assert relerr(hist["rrn"], g[key + "_rrn"][1:]) < 1e-10
+        assert relerr(hist[name], g[f"{key}_{name}"][:10]) < 1e-11, name
+
+
+def _heat_setup_py(ops, ndim, n, K_ext, rhs):
+    """TeaLeaf setup through the Python front end (same loops as the C++ app)."""
+    nx, ny, nz = n
+    blk = ops.ops_decl_block(ndim, "tea_lockstep")
+    size = [nx, ny] if ndim == 2 else [nx, ny, nz]
+    D = {q: ops.ops_decl_dat(blk, size, name=q) for q in ("density", "u", "w", "Kx", "Ky", "Kz", "r", "p")}
+    if ndim == 3:
+        S = dict(s0=[0, 0, 0], sp=[0, 0, 0, -1, 0, 0, 1, 0, 0, 0, -1, 0, 0, 1, 0, 0, 0, -1, 0, 0, 1],
+                 sx=[0, 0, 0, 1, 0, 0], sy=[0, 0, 0, 0, 1, 0], sz=[0, 0, 0, 0, 0, 1],
+                 sm=[0, 0, 0, -1, 0, 0, 0, -1, 0, 0, 0, -1])
+    else:
+        S = dict(s0=[0, 0], sp=[0, 0, 1, 0, -1, 0, 0, 1, 0, -1], sx=[0, 0, 1, 0], sy=[0, 0, 0, 1],
+                 sm=[0, 0, -1, 0, 0, -1])
+    st = {k: ops.ops_decl_stencil(ndim, len(v) // ndim, v, k) for k, v in S.items()}
+    red = ops.ops_decl_reduction_handle(8, "double", "red")
+    ops.set_ext(D["density"], K_ext)
+    ops.ops_dat_set_data(D["u"], rhs)
+    full = [-1, nx + 1, -1, ny + 1, -1, nz + 1][: 2 * ndim]
+    inner = [0, nx, 0, ny, 0, nz][: 2 * ndim]
+    ext = [0, nx + 1, 0, ny + 1, 0, nz + 1][: 2 * ndim]
+    A = lambda q, s, acc: ops.ops_arg_dat(D[q], 1, st[s], "double", acc)
+    R, W, RW, INC = ops.OPS_READ, ops.OPS_WRITE, ops.OPS_RW, ops.OPS_INC
+    ops.ops_par_loop("tea_leaf_recip_kernel", "recip", blk, ndim, full, A("w", "s0", W), A("density", "s0", R))
+    if ndim == 3:
+        ops.ops_par_loop("tea_leaf_common_init_Kx_Ky_Kz_kernel", "initK", blk, 3, ext, A("Kx", "s0", W),
+                         A("Ky", "s0", W), A("Kz", "s0", W), A("w", "sm", R))
+    else:
+        ops.ops_par_loop("tea_leaf_common_init_Kx_Ky_kernel", "initK", blk, 2, ext, A("Kx", "s0", W),
+                         A("Ky", "s0", W), A("w", "sm", R))
+    for q, name in enumerate(["Kx", "Ky", "Kz"][:ndim]):
+        lo, hi = list(full), list(full)
+        lo[2 * q:2 * q + 2] = [-1, 1]
+        hi[2 * q:2 * q + 2] = [n[q], n[q] + 1]
+        for rg in (lo, hi):
+            ops.ops_par_loop("tea_leaf_init_zero_kernel", "zero", blk, ndim, rg, A(name, "s0", W))
+    one = ops.ops_arg_gbl(1.0, 1, "double")
+    if ndim == 3:
+        ops.ops_par_loop("tea_leaf_common_init_kernel_3d", "init", blk, 3, inner, A("w", "s0", W), A("r", "s0", W),
+                         A("Kx", "sx", R), A("Ky", "sy", R), A("Kz", "sz", R), A("u", "sp", R), one,
+                         ops.ops_arg_gbl(1.0, 1, "double"), ops.ops_arg_gbl(1.0, 1, "double"))
+    else:
+        ops.ops_par_loop("tea_leaf_common_init_kernel", "init", blk, 2, inner, A("w", "s0", W), A("r", "s0", W),
+                         A("Kx", "sx", R), A("Ky", "sy", R), A("u", "sp", R), one, ops.ops_arg_gbl(1.0, 1, "double"))
+    ops.ops_par_loop("tea_leaf_yeqx_kernel", "yeqx", blk, ndim, inner, A("p", "s0", W), A("r", "s0", R))
+    ops.ops_par_loop("tea_leaf_dot_kernel", "dot", blk, ndim, inner, A("r", "s0", R), A("p", "s0", R),
+                     ops.ops_arg_reduce(red, 1, "double", INC))
+    rro = ops.ops_reduction_result(red)
+    return blk, D, st, red, inner, rro
+
+
+@pytest.mark.parametrize("ndim,n", [(2, (40, 34, 1)), (3, (26, 22, 18))])
+def test_heat_cg_lockstep(gpu_ops, ndim, n):
+    """Every loop of every CG iteration is checked against the oracle applied
+    to the GPU's own inputs of that loop: operator output bit-exact, vector
+    updates bit-exact, reductions within 1e-12 relative."""
+    ops = gpu_ops
+    nx, ny, nz = n
+    K, rhs = O.heat_inputs(31, ndim, nx, ny, nz)
+    blk, D, st, red, inner, rro = _heat_setup_py(ops, ndim, n, K, rhs)
+    Kx, Ky, Kz = O.heat_coefs(ndim, nx, ny, nz, K)
+    assert np.array_equal(ops.get_ext(D["Kx"])[(slice(1, -1),) * ndim], O.interior(Kx))
+    R, W, RW, INC = ops.OPS_READ, ops.OPS_WRITE, ops.OPS_RW, ops.OPS_INC
+    A = lambda q, s, acc: ops.ops_arg_dat(D[q], 1, st[s], "double", acc)
+    g = lambda v: ops.ops_arg_gbl(v, 1, "double")
+    for it in range(25):
+        p_ext = ops.get_ext(D["p"])
+        if ndim == 3:
+            ops.ops_par_loop("tea_leaf_cg_calc_w_reduce_kernel_3d", "w", blk, 3, inner, A("w", "s0", W),
+                             A("Kx", "sx", R), A("Ky", "sy", R), A("Kz", "sz", R), A("p", "sp", R), g(1.0), g(1.0),
+                             g(1.0), ops.ops_arg_reduce(red, 1, "double", INC))
+        else:
+            ops.ops_par_loop("tea_leaf_cg_calc_w_reduce_kernel", "w", blk, 2, inner, A("w", "s0", W),
+                             A("Kx", "sx", R), A("Ky", "sy", R), A("p", "sp", R), g(1.0), g(1.0),
+                             ops.ops_arg_reduce(red, 1, "double", INC))
+        pw = ops.ops_reduction_result(red)
+        w_ref, pw_ref = O.heat_apply(ndim, nx, ny, nz, Kx, Ky, Kz, p_ext)
+        w = ops.ops_dat_fetch_data(D["w"])
+        assert np.array_equal(w, O.interior(w_ref)), it
+        assert relerr(pw, pw_ref) < RTOL_RED, it
+        alpha = rro / pw
+        u0 = ops.ops_dat_fetch_data(D["u"])
+        ops.ops_par_loop("tea_leaf_axpy_kernel", "axpy", blk, ndim, inner, A("u", "s0", INC), A("p", "s0", R), g(alpha))
+        assert np.array_equal(ops.ops_dat_fetch_data(D["u"]), u0 + alpha * O.interior(p_ext)), it
+        r0 = ops.ops_dat_fetch_data(D["r"])
+        ops.ops_par_loop("tea_leaf_cg_calc_ur_r_reduce_kernel", "ur", blk, ndim, inner, A("r", "s0", INC),
+                         A("w", "s0", R), g(alpha), ops.ops_arg_reduce(red, 1, "double", INC))
+        rrn = ops.ops_reduction_result(red)
+        r1 = ops.ops_dat_fetch_data(D["r"])
+        assert np.array_equal(r1, r0 - alpha * w), it
+        assert relerr(rrn, np.sum(r1 * r1)) < RTOL_RED, it
+        beta = rrn / rro
+        ops.ops_par_loop("tea_leaf_axpby_kernel", "axpby", blk, ndim, inner, A("p", "s0", RW), A("r", "s0", R),
+                         g(beta), g(1.0))
+        assert np.array_equal(ops.ops_dat_fetch_data(D["p"]), beta * O.interior(p_ext) + 1.0 * r1), it
+        rro = rrn
+    for d in D.values():
+        ops.ops_free_dat(d)
+
+
+@pytest.mark.parametrize("ndim,n", [(2, (60, 44, 1)), (3, (30, 26, 20))])
+def test_heat_cg_converged_residual(apps, ndim, n):
+    # solved to 1e-10 on the GPU; the true residual |b - A u| recomputed by
+    # the oracle from the GPU's solution agrees with the GPU's recursive
+    # residual and meets the tolerance
+    nx, ny, nz = n
+    K, rhs = O.heat_inputs(17, ndim, nx, ny, nz)
+    h = apps.Heat(ndim, nx, ny, nz, K_ext=K, rhs=rhs)
+    k = h.iterate(5000, tol=1e-10)
+    rel = (h.rr() / h.rr(initial=True)) ** 0.5
     u = h.u()
     h.close()
-    ref = g[key + "_u"]
-    assert np.linalg.norm(u.ravel() - ref) / np.linalg.norm(ref) < 1e-12
+    assert rel <= 1e-10 and k < 5000
+    Kx, Ky, Kz = O.heat_coefs(ndim, nx, ny, nz, K)
+    u_ext = np.zeros(O.ext_shape(ndim, nx, ny, nz))
+    O.interior(u_ext)[:] = u
+    w, _ = O.heat_apply(ndim, nx, ny, nz, Kx, Ky, Kz, u_ext)
+    f_ext = np.zeros_like(u_ext)
+    O.interior(f_ext)[:] = rhs
+    Au0, _ = O.heat_apply(ndim, nx, ny, nz, Kx, Ky, Kz, f_ext)  # initial guess u0 = rhs
+    true_rel = np.linalg.norm(O.interior(w) - rhs) / np.linalg.norm(O.interior(Au0) - rhs)
+    assert true_rel <= 2e-10
+    ref = O.heat_cg(ndim, nx, ny, nz, K, rhs, max_iters=5000, tol=1e-10)
+    assert abs(ref["iters"] - k) <= max(3, k // 20)
 
 
 @pytest.mark.parametrize("ndim,key", [(2, "poisson2d"), (3, "poisson3d")])
@@ -132,11 +256,13 @@ def test_poisson_cg_fixture(apps, golden, ndim, key):
     f = O.poisson_rhs(m["seed"], ndim, c["nx"], c["ny"], nz)
     P = apps.Poisson(ndim, c["nx"], c["ny"], nz, f=f)
     k, hist = P.iterate(c["iters"], history=True)
+    x = P.x()
+    P.close()
+    # the constant-coefficient operator is well conditioned: the whole
+    # trajectory stays within the reduction tolerance
     for name in ("alpha", "beta"):
         assert relerr(hist[name], g[f"{key}_{name}"]) < RTOL_RED, name
     assert relerr(hist["rrn"], g[key + "_rrn"][1:]) < 1e-10
-    x = P.x()
-    P.close()
     assert np.linalg.norm(x.ravel() - g[key + "_x"]) / np.linalg.norm(g[key + "_x"]) < 1e-12
 
 
