@@ -289,21 +289,48 @@ constexpr DType dtype_v() {
   return std::is_same<T, double>::value ? DT_F64 : (std::is_same<T, float>::value ? DT_F32 : DT_I32);
 }
 
-// launches F over r; red_arg is the ops_arg_reduce of a reducing F
-template <class F>
-void launch_loop(const char *name, double bytes, const Range3 &r, const F &f, const ops_arg *red_arg = nullptr) {
+// reduction control with the current scratch buffers (taken at launch time)
+inline RedCtl with_scratch(RedCtl c) {
+  c.partials = inst().d_partials;
+  c.partials2 = inst().d_partials2;
+  return c;
+}
+
+// Builds the queue record of a loop over r.  make() constructs the kernel
+// functor; it runs when the loop is launched, so device pointers are those
+// of the dats at that moment.  red_arg is the ops_arg_reduce of a reducing
+// functor.
+template <class Make>
+LoopRec make_loop(const char *name, const ops_arg *args, int nargs, const Range3 &r, Make make,
+                  const ops_arg *red_arg = nullptr) {
+  using F = decltype(make());
+  LoopRec rec;
+  rec.name = name ? name : "?";
+  rec.range = r;
+  rec.bytes = transfer_bytes(args, nargs, r);
+  deps_from_args(rec, args, nargs);
   if (r.npts() <= 0) {
-    run_loop(name, 0.0, [](hipStream_t) {});
-    return;
+    rec.bytes = 0.0;
+    rec.launch = [](hipStream_t) {};
+    return rec;
   }
-  Tiling t = make_tiling(r, 1);
+  const Tiling t = make_tiling(r, 1);
   RedCtl ctl{};
   if constexpr (F::red_dim > 0) ctl = red_prepare(*red_arg, F::red_dim, dtype_v<typename F::red_t>(), t.blocks());
-  run_loop(name, bytes, [&](hipStream_t s) {
-    hipLaunchKernelGGL(k_loop<F>, t.grid, dim3(kWave, kWavesPerBlock), 0, s, r, f, ctl, t.chunk, t.three_d ? 1 : 0);
-    if constexpr (F::red_dim > 0)
-      launch_red_finish<typename F::red_t, F::red_op, F::red_dim>(ctl, t.blocks(), s);
-  });
+  rec.ctl = ctl;
+  rec.launch = [=](hipStream_t s) {
+    const F f = make();
+    const RedCtl c = with_scratch(ctl);
+    hipLaunchKernelGGL(k_loop<F>, t.grid, dim3(kWave, kWavesPerBlock), 0, s, r, f, c, t.chunk, t.three_d ? 1 : 0);
+    if constexpr (F::red_dim > 0) launch_red_finish<typename F::red_t, F::red_op, F::red_dim>(c, t.blocks(), s);
+  };
+  return rec;
+}
+
+template <class Make>
+void launch_loop(const char *name, const ops_arg *args, int nargs, const Range3 &r, Make make,
+                 const ops_arg *red_arg = nullptr) {
+  enqueue(make_loop(name, args, nargs, r, make, red_arg));
 }
 
 // marks every written dat as newest on the device

@@ -102,6 +102,7 @@ size_t g_buf_bytes = 0;
 extern "C" void ops_halo_transfer(ops_halo_group group) {
   OPSMI_API_BEGIN
   require_init();
+  flush_all();
   if (!group) fail(OPS_INVALID_ARGUMENT, "ops_halo_transfer: NULL group");
   Instance &I = inst();
   for (int h = 0; h < group->nhalos; h++) {
@@ -140,6 +141,7 @@ extern "C" void ops_halo_transfer(ops_halo_group group) {
       hipLaunchKernelGGL(k_halo<false>, dim3(nb), dim3(256), 0, s, st, box, buf, storage);
     });
     mark_device_dirty(to);
+    note_external_write(to);
   }
   OPSMI_API_END()
 }

@@ -224,29 +224,41 @@ __global__ __launch_bounds__(256) void k_heat3_march(Ptr<double> w, Ptr<double> 
   if constexpr (!INIT) reduce_block_store<double, OPS_INC, 1>(acc, ctl);
 }
 
-// launches the z-marching 3D heat kernel; 16 z planes per workgroup by
-// default (measured best among 1-16 on 512^3), OPSMI_HEAT_ZCHUNK overrides
+// Queue record for the z-marching 3D heat kernel (16 z planes per workgroup
+// by default, measured best among 1-16 on 512^3; OPSMI_HEAT_ZCHUNK
+// overrides).  INIT args: w, r, Kx, Ky, Kz, u; otherwise w, Kx, Ky, Kz, p.
+inline int heat_zchunk() { return heat_zchunk_default(); }
+
 template <bool INIT>
-void launch_heat3(const char *name, double bytes, const Range3 &r, Ptr<double> w, Ptr<double> rr,
-                  Ptr<const double> Kx, Ptr<const double> Ky, Ptr<const double> Kz, Ptr<const double> p, double rx,
-                  double ry, double rz, const ops_arg *red) {
+LoopRec make_heat3_loop(const char *name, const ops_arg *args, int nargs, const Range3 &r, ops_arg w, ops_arg rr,
+                        ops_arg Kx, ops_arg Ky, ops_arg Kz, ops_arg p, double rx, double ry, double rz,
+                        const ops_arg *red) {
+  LoopRec rec;
+  rec.name = name;
+  rec.range = r;
+  rec.bytes = transfer_bytes(args, nargs, r);
+  deps_from_args(rec, args, nargs);
   if (r.npts() <= 0) {
-    run_loop(name, 0.0, [](hipStream_t) {});
-    return;
+    rec.bytes = 0.0;
+    rec.launch = [](hipStream_t) {};
+    return rec;
   }
   const int nx = r.x1 - r.x0, ny = r.y1 - r.y0, nz = r.z1 - r.z0;
   const unsigned gx = (nx + kWave - 1) / kWave, gy = (ny + kWavesPerBlock - 1) / kWavesPerBlock;
-  int chunk = 16;
-  const char *env = getenv("OPSMI_HEAT_ZCHUNK");
-  if (env) chunk = std::max(1, atoi(env));
+  const int chunk = heat_zchunk();
   const unsigned gz = (nz + chunk - 1) / chunk;
   RedCtl ctl{};
   if (!INIT) ctl = red_prepare(*red, 1, DT_F64, gx * gy * gz);
-  run_loop(name, bytes, [&](hipStream_t s) {
-    hipLaunchKernelGGL(k_heat3_march<INIT>, dim3(gx, gy, gz), dim3(kWave, kWavesPerBlock), 0, s, w, rr, Kx, Ky, Kz,
-                       p, rx, ry, rz, r, chunk, ctl);
-    if (!INIT) launch_red_finish<double, OPS_INC, 1>(ctl, gx * gy * gz, s);
-  });
+  rec.ctl = ctl;
+  rec.launch = [=](hipStream_t s) {
+    const RedCtl c = with_scratch(ctl);
+    Ptr<double> rp = INIT ? ptr_of<double>(rr.dat) : Ptr<double>{nullptr, 0, 0};
+    hipLaunchKernelGGL(k_heat3_march<INIT>, dim3(gx, gy, gz), dim3(kWave, kWavesPerBlock), 0, s,
+                       ptr_of<double>(w.dat), rp, ptr_of<const double>(Kx.dat), ptr_of<const double>(Ky.dat),
+                       ptr_of<const double>(Kz.dat), ptr_of<const double>(p.dat), rx, ry, rz, r, chunk, c);
+    if (!INIT) launch_red_finish<double, OPS_INC, 1>(c, gx * gy * gz, s);
+  };
+  return rec;
 }
 
 bool use_march() {
@@ -383,11 +395,11 @@ static void red_x(const char *name, ops_block block, int dim, int *range, ops_ar
     ops_arg args[] = {x, *gb, s};
     Range3 r = check_loop(name, block, dim, range, args,
                           {dat_r(pts_point(), DT_F64), gbl_r(), red_a(OP, DT_F64)}, nullptr);
-    launch_loop(name, transfer_bytes(args, 3, r), r, RedX<KIND, OP>{ptr_of<const double>(x.dat), scale}, &s);
+    launch_loop(name, args, 3, r, [=] { return RedX<KIND, OP>{ptr_of<const double>(x.dat), scale}; }, &s);
   } else {
     ops_arg args[] = {x, s};
     Range3 r = check_loop(name, block, dim, range, args, {dat_r(pts_point(), DT_F64), red_a(OP, DT_F64)}, nullptr);
-    launch_loop(name, transfer_bytes(args, 2, r), r, RedX<KIND, OP>{ptr_of<const double>(x.dat), scale}, &s);
+    launch_loop(name, args, 2, r, [=] { return RedX<KIND, OP>{ptr_of<const double>(x.dat), scale}; }, &s);
   }
 }
 
@@ -395,7 +407,16 @@ static void red_x(const char *name, ops_block block, int dim, int *range, ops_ar
 #define ARGS(...) \
   ops_arg args[] = {__VA_ARGS__}; \
   const int nargs = (int)(sizeof(args) / sizeof(args[0]));
-#define BYTES(r) transfer_bytes(args, nargs, r)
+#define PD(a) ptr_of<double>((a).dat)
+#define PCD(a) ptr_of<const double>((a).dat)
+#define PF(a) ptr_of<float>((a).dat)
+#define PCF(a) ptr_of<const float>((a).dat)
+#define LAUNCH(r, ...) launch_loop(name, args, nargs, r, [=] { return __VA_ARGS__; })
+#define LAUNCH_RED(r, red, ...) launch_loop(name, args, nargs, r, [=] { return __VA_ARGS__; }, &(red))
+
+static void need_dim(const char *name, int dim, int want) {
+  if (dim != want) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": ", want, "D kernel on a ", dim, "D block"));
+}
 
 extern "C" {
 
@@ -404,8 +425,8 @@ void ops_par_loop_set_zero(const char *name, ops_block block, int dim, int *rang
   ARGS(a0)
   DType dt;
   Range3 r = check_loop(name, block, dim, range, args, {dat_w()}, &dt);
-  if (dt == DT_F64) launch_loop(name, BYTES(r), r, SetZero<double>{ptr_of<double>(a0.dat)});
-  else launch_loop(name, BYTES(r), r, SetZero<float>{ptr_of<float>(a0.dat)});
+  if (dt == DT_F64) LAUNCH(r, SetZero<double>{PD(a0)});
+  else LAUNCH(r, SetZero<float>{PF(a0)});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -415,10 +436,8 @@ void ops_par_loop_copy(const char *name, ops_block block, int dim, int *range, o
   ARGS(a0, a1)
   DType dt;
   Range3 r = check_loop(name, block, dim, range, args, {dat_w(), dat_r()}, &dt);
-  if (dt == DT_F64)
-    launch_loop(name, BYTES(r), r, Copy<double>{ptr_of<double>(a0.dat), ptr_of<const double>(a1.dat)});
-  else
-    launch_loop(name, BYTES(r), r, Copy<float>{ptr_of<float>(a0.dat), ptr_of<const float>(a1.dat)});
+  if (dt == DT_F64) LAUNCH(r, Copy<double>{PD(a0), PCD(a1)});
+  else LAUNCH(r, Copy<float>{PF(a0), PCF(a1)});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -427,11 +446,11 @@ static void bndcon(const char *name, ops_block block, int dim, int *range, ops_a
   ARGS(a0, a1)
   DType dt;
   Range3 r = check_loop(name, block, dim, range, args, {dat_w(), idx_a()}, &dt);
-  if (dim != 2) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 2D kernel on a ", dim, "D block"));
-  double pi = const_as_double("pi");
-  int n = const_as_int(mode == 2 ? "imax" : "jmax");
-  if (dt == DT_F64) launch_loop(name, BYTES(r), r, Bndcon<double>{ptr_of<double>(a0.dat), pi, n, mode});
-  else launch_loop(name, BYTES(r), r, Bndcon<float>{ptr_of<float>(a0.dat), pi, n, mode});
+  need_dim(name, dim, 2);
+  const double pi = const_as_double("pi");
+  const int n = const_as_int(mode == 2 ? "imax" : "jmax");
+  if (dt == DT_F64) LAUNCH(r, Bndcon<double>{PD(a0), pi, n, mode});
+  else LAUNCH(r, Bndcon<float>{PF(a0), pi, n, mode});
   finish_loop(args, nargs);
 }
 
@@ -456,13 +475,10 @@ void ops_par_loop_apply_stencil(const char *name, ops_block block, int dim, int 
   OPSMI_API_BEGIN
   ARGS(a0, a1, a2)
   DType dt;
-  std::vector<int> five = {0, 0, 0, 1, 0, 0, -1, 0, 0, 0, -1, 0, 0, 1, 0};
-  Range3 r = check_loop(name, block, dim, range, args, {dat_r(five), dat_w(), red_a(OPS_MAX)}, &dt);
-  if (dim != 2) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 2D kernel on a ", dim, "D block"));
-  if (dt == DT_F64)
-    launch_loop(name, BYTES(r), r, ApplyStencil<double>{ptr_of<const double>(a0.dat), ptr_of<double>(a1.dat)}, &a2);
-  else
-    launch_loop(name, BYTES(r), r, ApplyStencil<float>{ptr_of<const float>(a0.dat), ptr_of<float>(a1.dat)}, &a2);
+  Range3 r = check_loop(name, block, dim, range, args, {dat_r(pts_5pt()), dat_w(), red_a(OPS_MAX)}, &dt);
+  need_dim(name, dim, 2);
+  if (dt == DT_F64) LAUNCH_RED(r, a2, ApplyStencil<double>{PCD(a0), PD(a1)});
+  else LAUNCH_RED(r, a2, ApplyStencil<float>{PCF(a0), PF(a1)});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -472,7 +488,7 @@ void ops_par_loop_tea_leaf_init_zero_kernel(const char *name, ops_block block, i
   OPSMI_API_BEGIN
   ARGS(p)
   Range3 r = check_loop(name, block, dim, range, args, {dat_w(DT_F64)}, nullptr);
-  launch_loop(name, BYTES(r), r, SetZero<double>{ptr_of<double>(p.dat)});
+  LAUNCH(r, SetZero<double>{PD(p)});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -481,7 +497,7 @@ void ops_par_loop_tea_leaf_init_zero2_kernel(const char *name, ops_block block, 
   OPSMI_API_BEGIN
   ARGS(p, z)
   Range3 r = check_loop(name, block, dim, range, args, {dat_w(DT_F64), dat_w(DT_F64)}, nullptr);
-  launch_loop(name, BYTES(r), r, InitZero2{ptr_of<double>(p.dat), ptr_of<double>(z.dat)});
+  LAUNCH(r, InitZero2{PD(p), PD(z)});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -490,7 +506,7 @@ void ops_par_loop_tea_leaf_yeqx_kernel(const char *name, ops_block block, int di
   OPSMI_API_BEGIN
   ARGS(p, x)
   Range3 r = check_loop(name, block, dim, range, args, {dat_w(DT_F64), dat_r(pts_point(), DT_F64)}, nullptr);
-  launch_loop(name, BYTES(r), r, Copy<double>{ptr_of<double>(p.dat), ptr_of<const double>(x.dat)});
+  LAUNCH(r, Copy<double>{PD(p), PCD(x)});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -499,7 +515,7 @@ void ops_par_loop_tea_leaf_recip_kernel(const char *name, ops_block block, int d
   OPSMI_API_BEGIN
   ARGS(u, p)
   Range3 r = check_loop(name, block, dim, range, args, {dat_w(DT_F64), dat_r(pts_point(), DT_F64)}, nullptr);
-  launch_loop(name, BYTES(r), r, Recip{ptr_of<double>(u.dat), ptr_of<const double>(p.dat)});
+  LAUNCH(r, Recip{PD(u), PCD(p)});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -509,8 +525,7 @@ void ops_par_loop_tea_leaf_dot_kernel(const char *name, ops_block block, int dim
   ARGS(rr, p, rro)
   Range3 r = check_loop(name, block, dim, range, args,
                         {dat_r(pts_point(), DT_F64), dat_r(pts_point(), DT_F64), red_a(OPS_INC, DT_F64)}, nullptr);
-  launch_loop(name, BYTES(r), r, Dot{ptr_of<const double>(rr.dat), ptr_of<const double>(p.dat)}, &rro);
-  finish_loop(args, nargs);
+  LAUNCH_RED(r, rro, Dot{PCD(rr), PCD(p)});
   OPSMI_API_END()
 }
 void ops_par_loop_tea_leaf_norm2_kernel(const char *name, ops_block block, int dim, int *range, ops_arg x,
@@ -518,7 +533,7 @@ void ops_par_loop_tea_leaf_norm2_kernel(const char *name, ops_block block, int d
   OPSMI_API_BEGIN
   ARGS(x, norm)
   Range3 r = check_loop(name, block, dim, range, args, {dat_r(pts_point(), DT_F64), red_a(OPS_INC, DT_F64)}, nullptr);
-  launch_loop(name, BYTES(r), r, Norm2{ptr_of<const double>(x.dat)}, &norm);
+  LAUNCH_RED(r, norm, Norm2{PCD(x)});
   OPSMI_API_END()
 }
 void ops_par_loop_tea_leaf_axpy_kernel(const char *name, ops_block block, int dim, int *range, ops_arg u,
@@ -526,7 +541,14 @@ void ops_par_loop_tea_leaf_axpy_kernel(const char *name, ops_block block, int di
   OPSMI_API_BEGIN
   ARGS(u, p, alpha)
   Range3 r = check_loop(name, block, dim, range, args, {dat_w(DT_F64), dat_r(pts_point(), DT_F64), gbl_r()}, nullptr);
-  launch_loop(name, BYTES(r), r, Axpy{ptr_of<double>(u.dat), ptr_of<const double>(p.dat), gbl<double>(alpha)});
+  const double a = gbl<double>(alpha);
+  LoopRec rec = make_loop(name, args, nargs, r, [=] { return Axpy{PD(u), PCD(p), a}; });
+  rec.kind = LK_AXPY;
+  rec.dims = dim;
+  rec.d[0] = u.dat;
+  rec.d[1] = p.dat;
+  rec.g[0] = a;
+  enqueue(std::move(rec));
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -536,8 +558,15 @@ void ops_par_loop_tea_leaf_axpby_kernel(const char *name, ops_block block, int d
   ARGS(u, p, alpha, beta)
   Range3 r = check_loop(name, block, dim, range, args,
                         {dat_w(DT_F64), dat_r(pts_point(), DT_F64), gbl_r(), gbl_r()}, nullptr);
-  launch_loop(name, BYTES(r), r,
-              Axpby{ptr_of<double>(u.dat), ptr_of<const double>(p.dat), gbl<double>(alpha), gbl<double>(beta)});
+  const double a = gbl<double>(alpha), b = gbl<double>(beta);
+  LoopRec rec = make_loop(name, args, nargs, r, [=] { return Axpby{PD(u), PCD(p), a, b}; });
+  rec.kind = LK_AXPBY;
+  rec.dims = dim;
+  rec.d[0] = u.dat;
+  rec.d[1] = p.dat;
+  rec.g[0] = a;
+  rec.g[1] = b;
+  enqueue(std::move(rec));
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -547,8 +576,7 @@ void ops_par_loop_tea_leaf_zeqxty_kernel(const char *name, ops_block block, int 
   ARGS(z, x, y)
   Range3 r = check_loop(name, block, dim, range, args,
                         {dat_w(DT_F64), dat_r(pts_point(), DT_F64), dat_r(pts_point(), DT_F64)}, nullptr);
-  launch_loop(name, BYTES(r), r,
-              Zeqxty{ptr_of<double>(z.dat), ptr_of<const double>(x.dat), ptr_of<const double>(y.dat)});
+  LAUNCH(r, Zeqxty{PD(z), PCD(x), PCD(y)});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -558,9 +586,8 @@ void ops_par_loop_tea_leaf_common_init_Kx_Ky_kernel(const char *name, ops_block 
   ARGS(Kx, Ky, w)
   std::vector<int> m = {0, 0, 0, -1, 0, 0, 0, -1, 0};
   Range3 r = check_loop(name, block, dim, range, args, {dat_w(DT_F64), dat_w(DT_F64), dat_r(m, DT_F64)}, nullptr);
-  if (dim != 2) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 2D kernel on a ", dim, "D block"));
-  Ptr<double> none{nullptr, 0, 0};
-  launch_loop(name, BYTES(r), r, InitK<2>{ptr_of<double>(Kx.dat), ptr_of<double>(Ky.dat), none, ptr_of<const double>(w.dat)});
+  need_dim(name, dim, 2);
+  LAUNCH(r, InitK<2>{PD(Kx), PD(Ky), Ptr<double>{nullptr, 0, 0}, PCD(w)});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -571,9 +598,8 @@ void ops_par_loop_tea_leaf_common_init_Kx_Ky_Kz_kernel(const char *name, ops_blo
   std::vector<int> m = {0, 0, 0, -1, 0, 0, 0, -1, 0, 0, 0, -1};
   Range3 r = check_loop(name, block, dim, range, args,
                         {dat_w(DT_F64), dat_w(DT_F64), dat_w(DT_F64), dat_r(m, DT_F64)}, nullptr);
-  if (dim != 3) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 3D kernel on a ", dim, "D block"));
-  launch_loop(name, BYTES(r), r,
-              InitK<3>{ptr_of<double>(Kx.dat), ptr_of<double>(Ky.dat), ptr_of<double>(Kz.dat), ptr_of<const double>(w.dat)});
+  need_dim(name, dim, 3);
+  LAUNCH(r, InitK<3>{PD(Kx), PD(Ky), PD(Kz), PCD(w)});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -591,12 +617,9 @@ void ops_par_loop_tea_leaf_common_init_kernel(const char *name, ops_block block,
                         {dat_w(DT_F64), dat_w(DT_F64), dat_r(kx_pts(), DT_F64), dat_r(ky_pts(), DT_F64),
                          dat_r(pts_5pt(), DT_F64), gbl_r(), gbl_r()},
                         nullptr);
-  if (dim != 2) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 2D kernel on a ", dim, "D block"));
-  Ptr<const double> none{nullptr, 0, 0};
-  launch_loop(name, BYTES(r), r,
-              CommonInit<2>{ptr_of<double>(w.dat), ptr_of<double>(rr.dat), ptr_of<const double>(Kx.dat),
-                            ptr_of<const double>(Ky.dat), none, ptr_of<const double>(u.dat), gbl<double>(rx),
-                            gbl<double>(ry), 0.0});
+  need_dim(name, dim, 2);
+  const double vx = gbl<double>(rx), vy = gbl<double>(ry);
+  LAUNCH(r, CommonInit<2>{PD(w), PD(rr), PCD(Kx), PCD(Ky), Ptr<const double>{nullptr, 0, 0}, PCD(u), vx, vy, 0.0});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -609,16 +632,12 @@ void ops_par_loop_tea_leaf_common_init_kernel_3d(const char *name, ops_block blo
                         {dat_w(DT_F64), dat_w(DT_F64), dat_r(kx_pts(), DT_F64), dat_r(ky_pts(), DT_F64),
                          dat_r(kz_pts(), DT_F64), dat_r(pts_7pt(), DT_F64), gbl_r(), gbl_r(), gbl_r()},
                         nullptr);
-  if (dim != 3) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 3D kernel on a ", dim, "D block"));
+  need_dim(name, dim, 3);
+  const double vx = gbl<double>(rx), vy = gbl<double>(ry), vz = gbl<double>(rz);
   if (use_march())
-    launch_heat3<true>(name, BYTES(r), r, ptr_of<double>(w.dat), ptr_of<double>(rr.dat), ptr_of<const double>(Kx.dat),
-                       ptr_of<const double>(Ky.dat), ptr_of<const double>(Kz.dat), ptr_of<const double>(u.dat),
-                       gbl<double>(rx), gbl<double>(ry), gbl<double>(rz), nullptr);
+    enqueue(make_heat3_loop<true>(name, args, nargs, r, w, rr, Kx, Ky, Kz, u, vx, vy, vz, nullptr));
   else
-    launch_loop(name, BYTES(r), r,
-                CommonInit<3>{ptr_of<double>(w.dat), ptr_of<double>(rr.dat), ptr_of<const double>(Kx.dat),
-                              ptr_of<const double>(Ky.dat), ptr_of<const double>(Kz.dat), ptr_of<const double>(u.dat),
-                              gbl<double>(rx), gbl<double>(ry), gbl<double>(rz)});
+    LAUNCH(r, CommonInit<3>{PD(w), PD(rr), PCD(Kx), PCD(Ky), PCD(Kz), PCD(u), vx, vy, vz});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -631,12 +650,9 @@ void ops_par_loop_tea_leaf_cg_calc_w_reduce_kernel(const char *name, ops_block b
                         {dat_w(DT_F64), dat_r(kx_pts(), DT_F64), dat_r(ky_pts(), DT_F64), dat_r(pts_5pt(), DT_F64),
                          gbl_r(), gbl_r(), red_a(OPS_INC, DT_F64)},
                         nullptr);
-  if (dim != 2) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 2D kernel on a ", dim, "D block"));
-  Ptr<const double> none{nullptr, 0, 0};
-  launch_loop(name, BYTES(r), r,
-              CalcW<2>{ptr_of<double>(w.dat), ptr_of<const double>(Kx.dat), ptr_of<const double>(Ky.dat), none,
-                       ptr_of<const double>(p.dat), gbl<double>(rx), gbl<double>(ry), 0.0},
-              &pw);
+  need_dim(name, dim, 2);
+  const double vx = gbl<double>(rx), vy = gbl<double>(ry);
+  LAUNCH_RED(r, pw, CalcW<2>{PD(w), PCD(Kx), PCD(Ky), Ptr<const double>{nullptr, 0, 0}, PCD(p), vx, vy, 0.0});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -649,17 +665,23 @@ void ops_par_loop_tea_leaf_cg_calc_w_reduce_kernel_3d(const char *name, ops_bloc
                         {dat_w(DT_F64), dat_r(kx_pts(), DT_F64), dat_r(ky_pts(), DT_F64), dat_r(kz_pts(), DT_F64),
                          dat_r(pts_7pt(), DT_F64), gbl_r(), gbl_r(), gbl_r(), red_a(OPS_INC, DT_F64)},
                         nullptr);
-  if (dim != 3) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 3D kernel on a ", dim, "D block"));
-  if (use_march())
-    launch_heat3<false>(name, BYTES(r), r, ptr_of<double>(w.dat), Ptr<double>{nullptr, 0, 0},
-                        ptr_of<const double>(Kx.dat), ptr_of<const double>(Ky.dat), ptr_of<const double>(Kz.dat),
-                        ptr_of<const double>(p.dat), gbl<double>(rx), gbl<double>(ry), gbl<double>(rz), &pw);
-  else
-    launch_loop(name, BYTES(r), r,
-                CalcW<3>{ptr_of<double>(w.dat), ptr_of<const double>(Kx.dat), ptr_of<const double>(Ky.dat),
-                         ptr_of<const double>(Kz.dat), ptr_of<const double>(p.dat), gbl<double>(rx), gbl<double>(ry),
-                         gbl<double>(rz)},
-                &pw);
+  need_dim(name, dim, 3);
+  const double vx = gbl<double>(rx), vy = gbl<double>(ry), vz = gbl<double>(rz);
+  LoopRec rec = use_march()
+                    ? make_heat3_loop<false>(name, args, nargs, r, w, w, Kx, Ky, Kz, p, vx, vy, vz, &pw)
+                    : make_loop(name, args, nargs, r,
+                                [=] { return CalcW<3>{PD(w), PCD(Kx), PCD(Ky), PCD(Kz), PCD(p), vx, vy, vz}; }, &pw);
+  rec.kind = LK_HEAT_W3;
+  rec.dims = dim;
+  rec.d[0] = w.dat;
+  rec.d[1] = Kx.dat;
+  rec.d[2] = Ky.dat;
+  rec.d[3] = Kz.dat;
+  rec.d[4] = p.dat;
+  rec.g[0] = vx;
+  rec.g[1] = vy;
+  rec.g[2] = vz;
+  enqueue(std::move(rec));
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -669,8 +691,8 @@ void ops_par_loop_tea_leaf_cg_calc_ur_r_reduce_kernel(const char *name, ops_bloc
   ARGS(rr, w, alpha, rnn)
   Range3 r = check_loop(name, block, dim, range, args,
                         {dat_w(DT_F64), dat_r(pts_point(), DT_F64), gbl_r(), red_a(OPS_INC, DT_F64)}, nullptr);
-  launch_loop(name, BYTES(r), r, CalcUrR{ptr_of<double>(rr.dat), ptr_of<const double>(w.dat), gbl<double>(alpha)},
-              &rnn);
+  const double a = gbl<double>(alpha);
+  LAUNCH_RED(r, rnn, CalcUrR{PD(rr), PCD(w), a});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -685,11 +707,11 @@ void ops_par_loop_poisson_calc_w_reduce_kernel(const char *name, ops_block block
                         {dat_w(), dat_r(dim == 3 ? pts_7pt() : pts_5pt()), red_a(OPS_INC, DT_F64)}, &dt);
   if (dim < 2) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": needs a 2D or 3D block"));
   if (dt == DT_F64) {
-    if (dim == 2) launch_loop(name, BYTES(r), r, PoissonW<double, 2>{ptr_of<double>(w.dat), ptr_of<const double>(p.dat)}, &pw);
-    else launch_loop(name, BYTES(r), r, PoissonW<double, 3>{ptr_of<double>(w.dat), ptr_of<const double>(p.dat)}, &pw);
+    if (dim == 2) LAUNCH_RED(r, pw, PoissonW<double, 2>{PD(w), PCD(p)});
+    else LAUNCH_RED(r, pw, PoissonW<double, 3>{PD(w), PCD(p)});
   } else {
-    if (dim == 2) launch_loop(name, BYTES(r), r, PoissonW<float, 2>{ptr_of<float>(w.dat), ptr_of<const float>(p.dat)}, &pw);
-    else launch_loop(name, BYTES(r), r, PoissonW<float, 3>{ptr_of<float>(w.dat), ptr_of<const float>(p.dat)}, &pw);
+    if (dim == 2) LAUNCH_RED(r, pw, PoissonW<float, 2>{PF(w), PCF(p)});
+    else LAUNCH_RED(r, pw, PoissonW<float, 3>{PF(w), PCF(p)});
   }
   finish_loop(args, nargs);
   OPSMI_API_END()
@@ -705,12 +727,11 @@ void ops_par_loop_preproc_kernel(const char *name, ops_block block, int dim, int
   for (int i = 0; i < 10; i++) specs.push_back(dat_w(DT_F64));
   specs.push_back(idx_a());
   Range3 r = check_loop(name, block, dim, range, args, specs, nullptr);
-  if (dim != 3) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 3D kernel on a ", dim, "D block"));
-  Preproc f{ptr_of<const double>(u.dat), ptr_of<double>(du.dat), ptr_of<double>(ax.dat), ptr_of<double>(bx.dat),
-            ptr_of<double>(cx.dat), ptr_of<double>(ay.dat), ptr_of<double>(by.dat), ptr_of<double>(cy.dat),
-            ptr_of<double>(az.dat), ptr_of<double>(bz.dat), ptr_of<double>(cz.dat), const_as_int("nx"),
-            const_as_int("ny"), const_as_int("nz"), const_as_double("lambda")};
-  launch_loop(name, BYTES(r), r, f);
+  need_dim(name, dim, 3);
+  const int nx = const_as_int("nx"), ny = const_as_int("ny"), nz = const_as_int("nz");
+  const double lambda = const_as_double("lambda");
+  LAUNCH(r, Preproc{PCD(u), PD(du), PD(ax), PD(bx), PD(cx), PD(ay), PD(by), PD(cy), PD(az), PD(bz), PD(cz), nx, ny,
+                    nz, lambda});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -719,8 +740,8 @@ void ops_par_loop_stencil27_kernel(const char *name, ops_block block, int dim, i
   OPSMI_API_BEGIN
   ARGS(out, u)
   Range3 r = check_loop(name, block, dim, range, args, {dat_w(DT_F64), dat_r(pts_27pt(), DT_F64)}, nullptr);
-  if (dim != 3) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": 3D kernel on a ", dim, "D block"));
-  launch_loop(name, BYTES(r), r, Stencil27{ptr_of<double>(out.dat), ptr_of<const double>(u.dat)});
+  need_dim(name, dim, 3);
+  LAUNCH(r, Stencil27{PD(out), PCD(u)});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -762,14 +783,14 @@ void ops_par_loop_red_two(const char *name, ops_block block, int dim, int *range
   OPSMI_API_BEGIN
   ARGS(x, s2)
   Range3 r = check_loop(name, block, dim, range, args, {dat_r(pts_point(), DT_F64), red_a(OPS_INC, DT_F64, 2)}, nullptr);
-  launch_loop(name, BYTES(r), r, RedTwo{ptr_of<const double>(x.dat)}, &s2);
+  LAUNCH_RED(r, s2, RedTwo{PCD(x)});
   OPSMI_API_END()
 }
 void ops_par_loop_red_imax(const char *name, ops_block block, int dim, int *range, ops_arg idx, ops_arg m) {
   OPSMI_API_BEGIN
   ARGS(idx, m)
   Range3 r = check_loop(name, block, dim, range, args, {idx_a(), red_a(OPS_MAX, DT_I32)}, nullptr);
-  launch_loop(name, BYTES(r), r, RedImax{}, &m);
+  LAUNCH_RED(r, m, RedImax{});
   OPSMI_API_END()
 }
 

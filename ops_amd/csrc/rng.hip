@@ -58,6 +58,7 @@ unsigned long long g_fill_count = 0;
 
 void fill(ops_dat dat, bool normal) {
   require_init();
+  flush_all();
   Instance &I = inst();
   if (!I.rng_init) fail(OPS_RUNTIME_CONFIGURATION_ERROR, "ops_randomgen_init has not been called");
   if (!dat) fail(OPS_INVALID_ARGUMENT, "ops_fill_random: NULL dat");
@@ -86,6 +87,7 @@ void fill(ops_dat dat, bool normal) {
              }
            });
   mark_device_dirty(dat);
+  note_external_write(dat);
 }
 
 }  // namespace

@@ -364,6 +364,136 @@ void run_loop(const char *name, double bytes, const std::function<void(hipStream
   OPSMI_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ lazy queue
+static std::vector<FusionRule> &rules() {
+  static std::vector<FusionRule> r;
+  return r;
+}
+void register_fusion(FusionRule rule) { rules().push_back(rule); }
+
+static std::vector<LoopRec> g_queue;
+
+void deps_from_args(LoopRec &rec, const ops_arg *args, int n) {
+  for (int i = 0; i < n; i++) {
+    const ops_arg &a = args[i];
+    if (a.argtype == OPS_ARG_DAT && a.dat) {
+      if (a.acc != OPS_WRITE) rec.reads.push_back(a.dat);
+      if (a.acc != OPS_READ) rec.writes.push_back(a.dat);
+    } else if (a.argtype == OPS_ARG_GBL && a.data && (a.acc == OPS_INC || a.acc == OPS_MIN || a.acc == OPS_MAX)) {
+      rec.red = (ops_reduction)a.data;
+    }
+  }
+}
+
+static bool meet(const std::vector<ops_dat> &x, const std::vector<ops_dat> &y) {
+  for (ops_dat p : x)
+    for (ops_dat q : y)
+      if (p == q) return true;
+  return false;
+}
+// a (earlier) must run before b
+static bool conflict(const LoopRec &a, const LoopRec &b) {
+  return meet(a.writes, b.reads) || meet(a.writes, b.writes) || meet(a.reads, b.writes) ||
+         (a.red && a.red == b.red);
+}
+
+static void invalidate_spare(ops_dat d) {
+  DatImpl *m = (DatImpl *)d->impl;
+  if (m) m->spare_synced = false;
+}
+
+static void execute(std::vector<LoopRec> &run) {
+  Instance &I = inst();
+  size_t i = 0;
+  while (i < run.size()) {
+    int n = 0;
+    if (I.fusion)
+      for (FusionRule r : rules()) {
+        n = r(run, i, I.stream);
+        if (n > 0) break;
+      }
+    if (n > 0) {
+      i += n;
+      continue;
+    }
+    LoopRec &L = run[i];
+    run_loop(L.name.c_str(), L.bytes, L.launch);
+    for (ops_dat d : L.writes) invalidate_spare(d);
+    i++;
+  }
+}
+
+void enqueue(LoopRec &&rec) {
+  g_queue.push_back(std::move(rec));
+  if (g_queue.size() > 64) flush_all();
+}
+
+void flush_all() {
+  if (g_queue.empty()) return;
+  std::vector<LoopRec> run;
+  run.swap(g_queue);
+  execute(run);
+}
+
+void flush_for(ops_reduction h) {
+  const long n = (long)g_queue.size();
+  if (n == 0) return;
+  std::vector<char> need(n, 0);
+  bool any = false;
+  for (long i = n - 1; i >= 0; i--) {
+    if (g_queue[i].red == h) {
+      need[i] = 1;
+      any = true;
+      continue;
+    }
+    for (long j = i + 1; j < n; j++)
+      if (need[j] && conflict(g_queue[i], g_queue[j])) {
+        need[i] = 1;
+        break;
+      }
+  }
+  if (!any) return;
+  std::vector<LoopRec> run, rest;
+  for (long i = 0; i < n; i++) (need[i] ? run : rest).push_back(std::move(g_queue[i]));
+  g_queue.swap(rest);
+  execute(run);
+}
+
+char *spare_base(ops_dat d) {
+  Instance &I = inst();
+  DatImpl *m = (DatImpl *)d->impl;
+  size_t lead = (size_t)(d->data_d - m->alloc);
+  if (!m->spare) {
+    OPSMI_HIP(hipMalloc(&m->spare, d->mem + lead + 256));
+    m->spare_synced = false;
+  }
+  if (!m->spare_synced) {
+    OPSMI_HIP(hipMemcpyAsync(m->spare + lead, d->data_d, d->mem, hipMemcpyDeviceToDevice, I.stream));
+    m->spare_synced = true;
+  }
+  return m->spare + lead + d->base_offset;
+}
+
+void swap_spare(ops_dat d) {
+  DatImpl *m = (DatImpl *)d->impl;
+  size_t lead = (size_t)(d->data_d - m->alloc);
+  std::swap(m->alloc, m->spare);
+  d->data_d = m->alloc + lead;
+}
+
+void note_external_write(ops_dat d) { invalidate_spare(d); }
+
+void note_fused(const char *name, double bytes) {
+  LoopStat &s = inst().stats[name ? name : "?"];
+  s.count++;
+  s.bytes += bytes;
+}
+
+int heat_zchunk_default() {
+  const char *env = getenv("OPSMI_HEAT_ZCHUNK");
+  return env ? std::max(1, atoi(env)) : 16;
+}
+
 static double wall_now() {
   struct timespec t;
   clock_gettime(CLOCK_MONOTONIC, &t);
@@ -400,6 +530,8 @@ void ops_init(const int argc, const char *const argv[], const int diags_level) {
   hipDeviceProp_t prop;
   OPSMI_HIP(hipGetDeviceProperties(&prop, dev));
   I.num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  const char *fz = getenv("OPSMI_FUSION");
+  I.fusion = !(fz && fz[0] == '0');
   OPSMI_HIP(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
   ensure_partials((size_t)1 << 20);
   OPSMI_HIP(hipMalloc(&I.d_partials2, (size_t)1 << 20));
@@ -412,6 +544,7 @@ void ops_exit(void) {
   OPSMI_API_BEGIN
   Instance &I = inst();
   if (!I.initialised) return;
+  flush_all();
   OPSMI_HIP(hipStreamSynchronize(I.stream));
   for (ops_dat d : I.dats) {
     if (!d) continue;
@@ -572,10 +705,12 @@ void ops_free_dat(ops_dat d) {
   OPSMI_API_BEGIN
   if (!d) return;
   Instance &I = inst();
+  flush_all();
   OPSMI_HIP(hipStreamSynchronize(I.stream));
   DatImpl *m = (DatImpl *)d->impl;
   if (m) {
     (void)hipFree(m->alloc);
+    if (m->spare) (void)hipFree(m->spare);
     delete m;
   }
   free(d->data);
@@ -776,13 +911,18 @@ ops_arg ops_arg_reduce(ops_reduction handle, int dim, const char *type, ops_acce
 }
 
 // ------------------------------------------------------------------ sync
-void ops_execute(void) {}
+void ops_execute(void) {
+  OPSMI_API_BEGIN
+  if (inst().initialised) flush_all();
+  OPSMI_API_END()
+}
 
 void ops_reduction_result_char(ops_reduction handle, int type_size, char *ptr) {
   OPSMI_API_BEGIN
   (void)type_size;
   require_init();
   Instance &I = inst();
+  flush_for(handle);
   RedImpl *ri = (RedImpl *)handle->impl;
   if (ri->dev_valid) {
     OPSMI_HIP(hipMemcpyAsync(I.h_pinned, ri->d_slot, handle->size, hipMemcpyDeviceToHost, I.stream));
@@ -830,6 +970,7 @@ char *ops_dat_get_raw_pointer(ops_dat dat, int part, ops_stencil stencil, ops_me
   (void)part;
   (void)stencil;
   require_init();
+  flush_all();
   if (!memspace) fail(OPS_INVALID_ARGUMENT, "ops_dat_get_raw_pointer: NULL memspace");
   if (*memspace == OPS_HOST) {
     ensure_host(dat);
@@ -851,7 +992,10 @@ void ops_dat_release_raw_data(ops_dat dat, int part, ops_access acc) {
   if (dat->locked_hd == 0)
     fail(OPS_RUNTIME_ERROR, cat("Error: ops_dat_release_raw_data() called, but with no matching "
                                 "ops_dat_get_raw_pointer() beforehand: ", dat->name));
-  if (acc != OPS_READ) dat->dirty_hd = dat->locked_hd;
+  if (acc != OPS_READ) {
+    dat->dirty_hd = dat->locked_hd;
+    note_external_write(dat);
+  }
   dat->locked_hd = 0;
   OPSMI_API_END()
 }
@@ -862,7 +1006,10 @@ void ops_dat_release_raw_data_memspace(ops_dat dat, int part, ops_access acc, op
   if (dat->locked_hd == 0)
     fail(OPS_RUNTIME_ERROR, cat("Error: ops_dat_release_raw_data_memspace() called, but with no matching "
                                 "ops_dat_get_raw_pointer() beforehand: ", dat->name));
-  if (acc != OPS_READ) dat->dirty_hd = *memspace;
+  if (acc != OPS_READ) {
+    dat->dirty_hd = *memspace;
+    note_external_write(dat);
+  }
   dat->locked_hd = 0;
   OPSMI_API_END()
 }
@@ -910,6 +1057,7 @@ void ops_dat_fetch_data_memspace(ops_dat dat, int part, char *data, ops_memspace
   (void)part;
   (void)memspace;
   require_init();
+  flush_all();
   sync_to_device(dat);
   int lo[3], hi[3];
   interior_box(dat, lo, hi);
@@ -924,6 +1072,7 @@ void ops_dat_fetch_data_slab_memspace(ops_dat dat, int part, char *data, int *ra
   (void)part;
   (void)memspace;
   require_init();
+  flush_all();
   sync_to_device(dat);
   int lo[3], hi[3];
   for (int d = 0; d < dat->block->dims; d++) {
@@ -939,11 +1088,13 @@ void ops_dat_set_data_memspace(ops_dat dat, int part, char *data, ops_memspace m
   (void)part;
   (void)memspace;
   require_init();
+  flush_all();
   sync_to_device(dat);
   int lo[3], hi[3];
   interior_box(dat, lo, hi);
   slab_copy(dat, data, lo, hi, true);
   mark_device_dirty(dat);
+  note_external_write(dat);
   OPSMI_API_END()
 }
 
@@ -954,6 +1105,7 @@ void ops_dat_set_data_slab_memspace(ops_dat dat, int part, char *data, int *rang
   (void)part;
   (void)memspace;
   require_init();
+  flush_all();
   sync_to_device(dat);
   int lo[3], hi[3];
   for (int d = 0; d < dat->block->dims; d++) {
@@ -962,6 +1114,7 @@ void ops_dat_set_data_slab_memspace(ops_dat dat, int part, char *data, int *rang
   }
   slab_copy(dat, data, lo, hi, true);
   mark_device_dirty(dat);
+  note_external_write(dat);
   OPSMI_API_END()
 }
 
@@ -973,7 +1126,10 @@ void ops_timers_core(double *cpu, double *et) {
 
 void ops_timers(double *cpu, double *et) {
   OPSMI_API_BEGIN
-  if (inst().initialised) OPSMI_HIP(hipStreamSynchronize(inst().stream));
+  if (inst().initialised) {
+    flush_all();
+    OPSMI_HIP(hipStreamSynchronize(inst().stream));
+  }
   ops_timers_core(cpu, et);
   OPSMI_API_END()
 }
@@ -1007,6 +1163,7 @@ void ops_diagnostic_output(void) {
 
 void ops_timing_output_stdout(void) {
   OPSMI_API_BEGIN
+  flush_all();
   drain_events();
   // ops_timing_output layout (ops/c/src/core/ops_lib_core.cpp:1468-1624),
   // bandwidth in decimal GB/s of kernel time
@@ -1039,12 +1196,14 @@ void *ops_mi_stream(void) { return (void *)inst().stream; }
 void ops_mi_sync(void) {
   OPSMI_API_BEGIN
   require_init();
+  flush_all();
   OPSMI_HIP(hipStreamSynchronize(inst().stream));
   OPSMI_API_END()
 }
 
 int ops_mi_loop_stats(const char *name, long *count, double *kernel_ms, double *bytes) {
   OPSMI_API_BEGIN
+  flush_all();
   drain_events();
   auto it = inst().stats.find(name);
   if (it == inst().stats.end()) return 0;
@@ -1057,6 +1216,7 @@ int ops_mi_loop_stats(const char *name, long *count, double *kernel_ms, double *
 
 void ops_mi_reset_stats(void) {
   OPSMI_API_BEGIN
+  flush_all();
   drain_events();
   inst().stats.clear();
   OPSMI_API_END()

@@ -73,6 +73,8 @@ struct DatImpl {
   char *alloc = nullptr;  // hipMalloc'd pointer (data_d may be offset for alignment)
   DType dt = DT_OTHER;
   size_t host_bytes = 0;
+  char *spare = nullptr;      // second buffer for double-buffered fused chains
+  bool spare_synced = false;  // spare equals the current buffer outside fused writes
 };
 
 struct LoopStat {
@@ -203,5 +205,56 @@ void ensure_partials(size_t bytes);
 void run_loop(const char *name, double bytes, const std::function<void(hipStream_t)> &fn);
 double transfer_bytes(const ops_arg *args, int nargs, const Range3 &r);
 int grid_blocks(long rows, int rows_per_block);
+
+// ---------------------------------------------------------------- lazy queue
+// Loops are queued (the reference's OPS_LAZY mode, ops/c/src/core/
+// ops_lazy.cpp:186-276) and run when a result is needed.  A reduction result
+// runs only the queued loops it depends on: the loops reducing into the
+// handle plus, transitively, every earlier loop that conflicts with one of
+// them (read-after-write, write-after-read, write-after-write on a dat, or
+// the same reduction handle); the rest stay queued in program order.  Data
+// access (fetch/set/raw pointers/halo transfers/frees) runs everything.
+// Device pointers are resolved when a loop runs, so a dat's buffer may be
+// swapped by a fused loop chain in between.
+enum LoopKind {
+  LK_OPAQUE = 0,
+  LK_AXPY,       // tea_leaf_axpy_kernel: u = u + a p
+  LK_AXPBY,      // tea_leaf_axpby_kernel: u = a u + b p
+  LK_HEAT_W3,    // tea_leaf_cg_calc_w_reduce_kernel (3D): w = A p, pw
+  LK_POISSON_W,  // poisson_calc_w_reduce_kernel
+};
+struct LoopRec {
+  std::string name;
+  LoopKind kind = LK_OPAQUE;
+  Range3 range{};
+  int dims = 0;
+  std::vector<ops_dat> reads, writes;  // dats read (READ/RW/INC), written (WRITE/RW/INC)
+  ops_reduction red = nullptr;         // handle this loop reduces into
+  ops_dat d[8] = {};                   // kind-specific dat operands
+  double g[4] = {0, 0, 0, 0};          // kind-specific scalar operands
+  RedCtl ctl{};                        // reduction control of a reducing loop
+  double bytes = 0.0;
+  std::function<void(hipStream_t)> launch;
+};
+// a fusion rule looks at the front of a run of loops about to execute; when
+// it matches it launches one fused kernel for the first n loops and returns n
+typedef int (*FusionRule)(std::vector<LoopRec> &run, size_t at, hipStream_t s);
+void register_fusion(FusionRule rule);
+void enqueue(LoopRec &&rec);
+void flush_all();
+void flush_for(ops_reduction h);
+void deps_from_args(LoopRec &rec, const ops_arg *args, int n);
+// Double buffering for fused chains that rewrite a dat its own stencil reads:
+// spare_base(d) is point (0,...,0) of a second allocation of d whose contents
+// equal d's current buffer (copied when out of date); swap_spare(d) makes it
+// d's buffer.  Any other write to d marks the spare out of date.
+char *spare_base(ops_dat d);
+void swap_spare(ops_dat d);
+void note_external_write(ops_dat d);
+// counts a loop that ran inside a fused kernel (its time is the fused entry's)
+void note_fused(const char *name, double bytes);
+// z planes per workgroup of the z-marching 7-point kernels (16, or
+// OPSMI_HEAT_ZCHUNK)
+int heat_zchunk_default();
 
 }  // namespace opsmi

@@ -144,6 +144,7 @@ void *trid_scratch(size_t bytes) {
 void trid_run(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c, ops_dat d, ops_dat u,
               ops_tridsolver_params_c *ctx) {
   require_init();
+  flush_all();
   if (!a || !b || !c || !d || !range) fail(OPS_INVALID_ARGUMENT, "Tridsolver error: NULL argument");
   if (ndim < 1 || ndim > 3 || ndim != a->block->dims)
     fail(OPS_INVALID_ARGUMENT, "Tridsolver error: ndim must match the block (1-3D)");
@@ -225,6 +226,7 @@ void trid_run(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat 
     });
   }
   mark_device_dirty(u ? u : d);
+  note_external_write(u ? u : d);
 }
 
 }  // namespace

@@ -1,0 +1,98 @@
+"""Lazy execution and run-time loop-chain fusion.
+
+A reduction result runs only the queued loops it depends on; data access runs
+everything.  The fused CG pair (axpy + axpby) must give the same
+bits as the loops run one by one."""
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(ops, n=(37, 29)):
+    blk = ops.ops_decl_block(2, "lazy")
+    s00 = ops.ops_decl_stencil(2, 1, [0, 0], "00")
+    A = ops.ops_decl_dat(blk, list(n), name="A")
+    B = ops.ops_decl_dat(blk, list(n), name="B")
+    v = O.u01(3, 9, n[0] * n[1]).reshape(n[1], n[0])
+    ops.ops_dat_set_data(A, v)
+    h = ops.ops_decl_reduction_handle(8, "double", "s")
+    rng = [0, n[0], 0, n[1]]
+    return blk, s00, A, B, v, h, rng
+
+
+def test_deferred_loop_not_needed_by_result(gpu_ops):
+    ops = gpu_ops
+    blk, s00, A, B, v, h, rng = _setup(ops)
+    R, W = ops.OPS_READ, ops.OPS_WRITE
+    ops.ops_par_loop("copy", "copy", blk, 2, rng, ops.ops_arg_dat(B, 1, s00, "double", W),
+                     ops.ops_arg_dat(A, 1, s00, "double", R))
+    ops.ops_par_loop("set_zero", "zero", blk, 2, rng, ops.ops_arg_dat(A, 1, s00, "double", W))
+    ops.ops_par_loop("red_sum", "sum", blk, 2, rng, ops.ops_arg_dat(B, 1, s00, "double", R),
+                     ops.ops_arg_reduce(h, 1, "double", ops.OPS_INC))
+    s = ops.ops_reduction_result(h)
+    assert abs(s - v.sum()) <= 1e-12 * abs(v.sum())
+    assert np.all(ops.ops_dat_fetch_data(A) == 0.0)
+    assert np.array_equal(ops.ops_dat_fetch_data(B), v)
+
+
+def test_result_runs_its_dependencies_in_order(gpu_ops):
+    ops = gpu_ops
+    blk, s00, A, B, v, h, rng = _setup(ops)
+    R, W = ops.OPS_READ, ops.OPS_WRITE
+    ops.ops_par_loop("set_zero", "zero", blk, 2, rng, ops.ops_arg_dat(A, 1, s00, "double", W))
+    ops.ops_par_loop("copy", "copy", blk, 2, rng, ops.ops_arg_dat(B, 1, s00, "double", W),
+                     ops.ops_arg_dat(A, 1, s00, "double", R))
+    ops.ops_par_loop("red_sum", "sum", blk, 2, rng, ops.ops_arg_dat(B, 1, s00, "double", R),
+                     ops.ops_arg_reduce(h, 1, "double", ops.OPS_INC))
+    assert ops.ops_reduction_result(h) == 0.0
+
+
+def test_write_after_read_kept_in_order(gpu_ops):
+    # a later loop overwrites what an earlier queued loop reads: the earlier
+    # one must run first even though only the later one is needed
+    ops = gpu_ops
+    blk, s00, A, B, v, h, rng = _setup(ops)
+    R, W = ops.OPS_READ, ops.OPS_WRITE
+    h2 = ops.ops_decl_reduction_handle(8, "double", "s2")
+    ops.ops_par_loop("red_sum", "sumA", blk, 2, rng, ops.ops_arg_dat(A, 1, s00, "double", R),
+                     ops.ops_arg_reduce(h2, 1, "double", ops.OPS_INC))
+    ops.ops_par_loop("copy", "copy", blk, 2, rng, ops.ops_arg_dat(B, 1, s00, "double", W),
+                     ops.ops_arg_dat(A, 1, s00, "double", R))
+    ops.ops_par_loop("set_zero", "zero", blk, 2, rng, ops.ops_arg_dat(A, 1, s00, "double", W))
+    ops.ops_par_loop("red_sum", "sumB", blk, 2, rng, ops.ops_arg_dat(B, 1, s00, "double", R),
+                     ops.ops_arg_reduce(h, 1, "double", ops.OPS_INC))
+    sB = ops.ops_reduction_result(h)
+    sA = ops.ops_reduction_result(h2)
+    assert abs(sA - v.sum()) <= 1e-12 * abs(v.sum())
+    assert abs(sB - v.sum()) <= 1e-12 * abs(v.sum())
+
+
+@pytest.mark.parametrize("n", [(48, 40, 33), (130, 66, 40)])
+def test_fused_cg_chain_bit_identical(gpu_ops, n):
+    from ops_amd import apps
+    ops = gpu_ops
+    K, rhs = O.heat_inputs(23, 3, *n)
+    runs = []
+    for fusion in (0, 1):
+        ops.lib().ops_mi_set_fusion(fusion)
+        try:
+            h = apps.Heat(3, *n, K_ext=K, rhs=rhs)
+            ops.ops_mi_reset_stats()
+            k, hist = h.iterate(30, history=True)
+            assert k == 30
+            st = ops.ops_mi_loop_stats("cg_pair(axpy+axpby)")
+            u = h.u()
+            h.close()
+        finally:
+            ops.lib().ops_mi_set_fusion(1)
+        runs.append((hist, u, st))
+    (h0, u0, st0), (h1, u1, st1) = runs
+    # one axpy+axpby pair per iteration: each axpy waits in the queue past the
+    # r.r result until the axpby that overwrites p
+    assert st0 is None and st1 is not None and st1[0] == 30
+    for key in ("pw", "alpha", "beta", "rrn"):
+        assert np.array_equal(h0[key], h1[key]), key
+    assert np.array_equal(u0, u1)
