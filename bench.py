@@ -1,19 +1,32 @@
-"""Benchmark: one step = one CG iteration of the 3D TeaLeaf-form implicit heat
-solve on a 512^3 FP64 grid (7-point variable-coefficient operator, lognormal
-conductivity), i.e. the four ops_par_loops of tea_leaf_cg_calc_w / calc_ur /
-calc_p with their two FP64 reductions, driven by the C++ app over the C ABI.
+"""Benchmarks of the OPS hot path on one GPU.  Each run prints one JSON line.
 
-value = algorithmic bytes of the step (OPS ops_compute_transfer rule: 1x for
-READ/WRITE, 2x for RW/INC, 112 B/point, counted per loop as written even when
-the lazy queue fuses the axpy+axpby pair into one 40 B/point pass) / measured
-time, decimal GB/s.
-roofline = the kernel with the largest share of the step
-(tea_leaf_cg_calc_w_reduce_kernel, 40 B/point compulsory traffic) timed with
-HIP events recorded on the library's stream around every launch.
-cpu_baseline = the oracle's OpenMP restatement of the reference CPU path on
-the same 512^3 problem, a few iterations on the host cores.
+Default workload (heat3d_cg_512): one step = one CG iteration of the 3D
+TeaLeaf-form implicit heat solve on a 512^3 FP64 grid (7-point
+variable-coefficient operator, lognormal conductivity), i.e. the four
+ops_par_loops of tea_leaf_cg_calc_w / calc_ur / calc_p with their two FP64
+reductions, driven by the C++ app over the C ABI.
+  value = algorithmic bytes of the step (OPS ops_compute_transfer rule: 1x for
+  READ/WRITE, 2x for RW/INC, 112 B/point, counted per loop as written even
+  when the lazy queue fuses the axpy+axpby pair into one 40 B/point pass) /
+  measured time, decimal GB/s.
+  roofline = the kernel with the largest share of the step
+  (tea_leaf_cg_calc_w_reduce_kernel, 40 B/point compulsory traffic) timed
+  with HIP events recorded on the library's stream around every launch.
+  cpu_baseline = the oracle's OpenMP restatement of the reference CPU path on
+  the same 512^3 problem, a few iterations on the host cores.
 
-usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--size 512]
+Other workloads (--workload):
+  poisson_mixed_768  768^3 7-point Poisson to 1e-10 relative residual: FP64
+                     CG against FP64 iterative refinement around FP32 PCG
+                     (solves/s, iteration counts, error against FP64 CG)
+  laplace2d_2048     2048^2 Jacobi with max-abs-diff reduction, 1000 iterations
+  adi_trid_256       256^3 batched tridiagonal solves along x, y, z (Thomas)
+  serve_stream       request stream (60 % 2D Poisson, 40 % 3D heat, uneven
+                     sizes) on the batched engine: solves/s with all
+                     requests queued, p50/p99 latency under Poisson arrivals,
+                     against the per-solve FP64 path one solve at a time
+
+usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload heat3d_cg_512]
 """
 import argparse
 import json
@@ -25,6 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "HBM GB/s per ops_par_loop; batched solves/s + p99 latency at 1e-10 residual"
 # compulsory HBM bytes per point of each kernel: the four TeaLeaf loops, the
 # fused pointwise pair (reads u, p, r; writes u, p) and the fused chain
 # (reads u, p_old, r, Kx, Ky, Kz; writes u, p_new, w)
@@ -33,11 +47,39 @@ BYTES_PER_PT = {"tea_leaf_cg_calc_w_reduce_kernel": 40, "tea_leaf_axpy_kernel": 
                 "cg_pair(axpy+axpby)": 40, "cg_chain(axpy+axpby+calc_w)": 72}
 
 
-def cpu_baseline(n, seed, iters=3):
+def kernel_table(ops, bpp_map, pts):
+    out = {}
+    for name, bpp in bpp_map.items():
+        st = ops.ops_mi_loop_stats(name)
+        if st and st[0] and st[1] > 0:
+            cnt, kms, _ = st
+            out[name] = {"launches": cnt, "avg_us": round(kms / cnt * 1e3, 2),
+                         "GB_s": round(bpp * pts / (kms / cnt * 1e-3) / 1e9, 1)}
+    return out
+
+
+def roofline(kernels, bpp_map, pts):
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
+    avg_s = kernels[dom]["avg_us"] * 1e-6
+    achieved = bpp_map[dom] * pts / avg_s / 1e9
+    return {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": bpp_map[dom] * pts,
+            "avg_launch_us": round(avg_s * 1e6, 2)}
+
+
+def line(value, unit, ms, a, dtype, config, **extra):
+    out = {"metric": METRIC, "value": round(value, 2), "unit": unit, "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic (seeded counter-based generator)", "config": config}
+    out.update(extra)
+    return out
+
+
+# ------------------------------------------------------------------ heat CG
+def cpu_baseline_heat(n, seed, iters=3):
     """Oracle (OpenMP restatement of the reference CPU path) on the same
     problem: times `iters` CG iterations as the difference of two runs."""
-    import numpy as np
-
     from oracle import oracle_py as O
 
     nt = os.cpu_count() or 1
@@ -55,23 +97,8 @@ def cpu_baseline(n, seed, iters=3):
                       f"{dt * 1e3:.1f} ms/iteration)"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--size", type=int, default=512)
-    ap.add_argument("--seed", type=int, default=20240611)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    a = ap.parse_args()
-    if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              "(multi-GPU sharding is design-only)", file=sys.stderr)
-        sys.exit(2)
-
-    from ops_amd import apps, ops
-
-    n = a.size
+def bench_heat(a, ops, apps):
+    n = a.size or 512
     ops.ops_init(diags=2)
     h = apps.Heat(3, n, n, n, seed=a.seed)
     h.iterate(a.warmup)
@@ -84,47 +111,199 @@ def main():
     assert k == a.steps
     ms = (t1 - t0) / a.steps * 1e3
     pts = float(n) ** 3
-    step_bytes = 112.0 * pts
-    value = step_bytes / (ms * 1e-3) / 1e9
-
-    kernels = {}
-    for name, bpp in BYTES_PER_PT.items():
-        st = ops.ops_mi_loop_stats(name)
-        if st and st[0] and st[1] > 0:
-            cnt, kms, by = st
-            kernels[name] = {"launches": cnt, "avg_us": round(kms / cnt * 1e3, 2),
-                             "GB_s": round(bpp * pts / (kms / cnt * 1e-3) / 1e9, 1)}
-    # the dominant kernel: largest total time in the step
-    dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
-    bpp = BYTES_PER_PT[dom]
-    avg_s = kernels[dom]["avg_us"] * 1e-6
-    achieved = bpp * pts / avg_s / 1e9
+    value = 112.0 * pts / (ms * 1e-3) / 1e9
+    kernels = kernel_table(ops, BYTES_PER_PT, pts)
     rr_rel = (h.rr() / h.rr(initial=True)) ** 0.5
     h.close()
-
-    out = {
-        "metric": "HBM GB/s per ops_par_loop; batched solves/s + p99 latency at 1e-10 residual",
-        "value": round(value, 2),
-        "unit": "GB/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": round(ms, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded counter-based generator: lognormal conductivity, uniform + Gaussian rhs)",
-        "config": {"workload": f"heat3d_cg_{n}", "grid": [n, n, n], "operator": "7-point variable coefficient",
-                   "step": "one CG iteration (4 ops_par_loops, 2 FP64 reductions)",
-                   "bytes_per_point": 112, "relres_after_steps": rr_rel},
-        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": bpp * pts, "avg_launch_us": round(avg_s * 1e6, 2)},
-        "kernels": kernels,
-    }
+    out = line(value, "GB/s", ms, a, "f64",
+               {"workload": f"heat3d_cg_{n}", "grid": [n, n, n], "operator": "7-point variable coefficient",
+                "step": "one CG iteration (4 ops_par_loops, 2 FP64 reductions)", "bytes_per_point": 112,
+                "relres_after_steps": rr_rel},
+               roofline=roofline(kernels, BYTES_PER_PT, pts), kernels=kernels)
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(n, a.seed)
+        out["cpu_baseline"] = cpu_baseline_heat(n, a.seed)
+    return out
+
+
+# ------------------------------------------------------ mixed-precision Poisson
+MIXED_BPP = {"poisson_calc_w_reduce_kernel": 8, "mp_update_kernel": 24, "mp_p_kernel": 12,
+             "poisson_residual_kernel": 24, "mp_init_kernel": 20, "mp_correct_kernel": 20}
+FP64_BPP = {"poisson_calc_w_reduce_kernel": 16, "tea_leaf_axpy_kernel": 24,
+            "tea_leaf_cg_calc_ur_r_reduce_kernel": 24, "tea_leaf_axpby_kernel": 24, "cg_pair(axpy+axpby)": 40}
+
+
+def bench_mixed(a, ops, apps):
+    import numpy as np
+
+    n = a.size or 768
+    tol = 1e-10
+    pts = float(n) ** 3
+    ops.ops_init(diags=2)
+    # FP64 CG (x0 = 0, stop at |r|/|f| <= tol on the recursive residual)
+    P = apps.Poisson(3, n, n, n, seed=a.seed)
+    ops.ops_mi_sync()
+    ops.ops_mi_reset_stats()
+    t0 = time.perf_counter()
+    it64 = P.iterate(100000, tol=tol)
+    ops.ops_mi_sync()
+    t64 = time.perf_counter() - t0
+    k64 = kernel_table(ops, FP64_BPP, pts)
+    x64 = P.x()
+    P.close()
+    print(f"fp64 CG: {it64} iterations, {t64:.2f} s", file=sys.stderr, flush=True)
+    M = apps.PoissonMixed(3, n, n, n, seed=a.seed)
+    ops.ops_mi_sync()
+    ops.ops_mi_reset_stats()
+    t0 = time.perf_counter()
+    res = M.solve(tol=tol)
+    ops.ops_mi_sync()
+    tmp = time.perf_counter() - t0
+    kmp = kernel_table(ops, MIXED_BPP, pts)
+    xm = M.x()
+    M.close()
+    print(f"mixed: {res['outer']} outer, {res['inner_total']} inner, {tmp:.2f} s", file=sys.stderr, flush=True)
+    err = float(np.max(np.abs(xm - x64)) / np.max(np.abs(x64)))
+    del xm, x64
+    value = 1.0 / tmp
+    return line(value, "solves/s", tmp * 1e3, a, "f32 sweeps, f64 reductions/residual/correction",
+                {"workload": f"poisson_mixed_{n}", "grid": [n, n, n], "operator": "7-point constant coefficient",
+                 "tol": tol},
+                fp64_cg={"iterations": it64, "seconds": round(t64, 3), "kernels": k64},
+                mixed={"outer": res["outer"], "inner_total": res["inner_total"],
+                       "relres": [float(v) for v in res["relres"]], "seconds": round(tmp, 3), "kernels": kmp},
+                speedup_vs_fp64=round(t64 / tmp, 3), max_rel_diff_vs_fp64_cg=err,
+                roofline=roofline(kmp, MIXED_BPP, pts))
+
+
+# ------------------------------------------------------------------ Laplace
+def bench_laplace(a, ops, apps):
+    n = a.size or 2048
+    iters = 1000
+    ops.ops_init(diags=2)
+    apps.laplace2d(n, n, 10, want_field=False)
+    ops.ops_mi_sync()
+    t0 = time.perf_counter()
+    apps.laplace2d(n, n, 10, want_field=False)
+    t1 = time.perf_counter()
+    ops.ops_mi_reset_stats()
+    t2 = time.perf_counter()
+    err, _, _ = apps.laplace2d(n, n, 10 + iters, want_field=False)
+    t3 = time.perf_counter()
+    ms = ((t3 - t2) - (t1 - t0)) / iters * 1e3
+    pts = float(n) ** 2
+    bpp = {"apply stencil": 16, "copy": 16}
+    kernels = kernel_table(ops, bpp, pts)
+    value = 32.0 * pts / (ms * 1e-3) / 1e9
+    return line(value, "GB/s", ms, a, "f64",
+                {"workload": f"laplace2d_{n}", "grid": [n, n], "iterations": iters, "bytes_per_point": 32,
+                 "final_max_abs_diff": err},
+                roofline=roofline(kernels, bpp, pts), kernels=kernels)
+
+
+# --------------------------------------------------------------- tridiagonal
+def bench_trid(a, ops, apps):
+    n = a.size or 256
+    ops.ops_init(diags=2)
+    T = apps.Trid(n, n, n, seed=a.seed, strategy=0)
+    for _ in range(a.warmup):
+        T.sweep(inc=True)
+    ops.ops_mi_sync()
+    ops.ops_mi_reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        T.sweep(inc=True)
+    ops.ops_mi_sync()
+    ms = (time.perf_counter() - t0) / a.steps * 1e3
+    T.close()
+    pts = float(n) ** 3
+    value = (40.0 + 40.0 + 48.0) * pts / (ms * 1e-3) / 1e9
+    return line(value, "GB/s", ms, a, "f64",
+                {"workload": f"adi_trid_{n}", "grid": [n, n, n], "step": "Thomas solves along x, y, z (z with _Inc)",
+                 "bytes_per_point": 128})
+
+
+# ------------------------------------------------------------------ serving
+def serve_requests(nreq, seed):
+    """60 % 2D 5-point Poisson with sides log-uniform in [64, 2048], 40 % 3D
+    7-point heat with sides log-uniform in [32, 256]."""
+    import numpy as np
+
+    g = np.random.default_rng(seed)
+    kinds = np.where(g.random(nreq) < 0.6, 0, 1).astype(np.int32)
+    s2 = np.exp(g.uniform(np.log(64), np.log(2048), nreq))
+    s3 = np.exp(g.uniform(np.log(32), np.log(256), nreq))
+    sizes = np.where(kinds == 0, s2, s3).round().astype(np.int32)
+    seeds = g.integers(1, 2 ** 62, nreq, dtype=np.uint64)
+    return kinds, sizes, seeds, g
+
+
+def bench_serve(a, ops, apps):
+    import numpy as np
+
+    nreq, tol, budget = a.requests, 1e-10, 200 * 10 ** 9
+    kinds, sizes, seeds, g = serve_requests(nreq, a.seed)
+    ops.ops_init(diags=1)
+    # saturation: every request queued at t = 0
+    span, done, iters = apps.serve_stream(kinds, sizes, seeds, np.zeros(nreq), tol, budget)
+    assert np.all(iters > 0), "a solve stopped without converging"
+    sat = nreq / span
+    print(f"batched, all queued at t=0: {nreq} solves in {span:.2f} s", file=sys.stderr, flush=True)
+    # latency under Poisson arrivals at 70 % of the saturated rate
+    rate = 0.7 * sat
+    arr = np.cumsum(g.exponential(1.0 / rate, nreq))
+    arr -= arr[0]
+    span2, done2, iters2 = apps.serve_stream(kinds, sizes, seeds, arr, tol, budget)
+    lat = done2 - arr
+    print(f"batched, Poisson arrivals at {rate:.1f}/s: span {span2:.2f} s", file=sys.stderr, flush=True)
+    # per-solve FP64 path: the library apps, one solve after another
+    t_base, it_base = 0.0, []
+    for j, (k, n, s) in enumerate(zip(kinds, sizes, seeds)):
+        n = int(n)
+        P = apps.Poisson(2, n, n, 1, seed=int(s)) if k == 0 else apps.Heat(3, n, n, n, seed=int(s))
+        ops.ops_mi_sync()
+        t0 = time.perf_counter()
+        it_base.append(P.iterate(1000000, tol=tol))
+        ops.ops_mi_sync()
+        t_base += time.perf_counter() - t0
+        P.close()
+        if j % 20 == 19:
+            print(f"per-solve path: {j + 1} solves, {t_base:.2f} s", file=sys.stderr, flush=True)
+    base = nreq / t_base
+    return line(sat, "solves/s", span / nreq * 1e3, a, "f64",
+                {"workload": f"serve_stream_{nreq}", "requests": nreq, "tol": tol, "budget_bytes": budget,
+                 "mix": "60% 2D 5-pt Poisson side log-U[64,2048], 40% 3D 7-pt heat side log-U[32,256]"},
+                batched_saturated={"solves_per_s": round(sat, 3), "makespan_s": round(span, 3)},
+                batched_poisson_arrivals={"rate_per_s": round(rate, 3), "p50_latency_s": float(np.percentile(lat, 50)),
+                                          "p99_latency_s": float(np.percentile(lat, 99)),
+                                          "solves_per_s": round(nreq / span2, 3)},
+                per_solve_fp64={"solves_per_s": round(base, 3), "solve_seconds_total": round(t_base, 3)},
+                speedup_vs_per_solve=round(sat / base, 3),
+                iterations={"batched_total": int(iters.sum()), "per_solve_total": int(sum(it_base))})
+
+
+WORKLOADS = {"heat3d_cg_512": bench_heat, "poisson_mixed_768": bench_mixed, "laplace2d_2048": bench_laplace,
+             "adi_trid_256": bench_trid, "serve_stream": bench_serve}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=0, help="grid side (default: the workload's)")
+    ap.add_argument("--seed", type=int, default=20240611)
+    ap.add_argument("--workload", default="heat3d_cg_512", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--requests", type=int, default=100, help="serve_stream: number of requests")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    from ops_amd import apps, ops
+
+    out = WORKLOADS[a.workload](a, ops, apps)
     print(json.dumps(out))
 
 

@@ -314,6 +314,30 @@ void ops_par_loop_tea_leaf_cg_calc_ur_r_reduce_kernel(const char *name, ops_bloc
 void ops_par_loop_poisson_calc_w_reduce_kernel(const char *name, ops_block block, int dim, int *range,
                                                ops_arg w, ops_arg p, ops_arg pw);
 
+/* Mixed-precision Poisson (this project's solver; the reference has no
+ * mixed-precision path): FP64 iterative refinement around a
+ * Jacobi-preconditioned CG with FP32 storage and arithmetic, every dot
+ * product accumulated in FP64.  Same per-loop stub shape as above.
+ *   poisson_residual_kernel  r = f - A x (FP64 dats), rr += r*r (double)
+ *   mp_init_kernel           e = 0; rr = (float)r; p = Mi*rr; float dats
+ *                            e, rr, p; r double; Mi float gbl;
+ *                            dots[0] += rr*(Mi*rr), dots[1] += rr*rr
+ *   mp_update_kernel         e += a*p; rr -= a*w; same two dots (a, Mi
+ *                            float gbls; e, rr RW; p, w READ)
+ *   mp_p_kernel              p = Mi*rr + b*p (Mi, b float gbls)
+ *   mp_correct_kernel        x = x + (double)e
+ * The FP32 operator sweep is poisson_calc_w_reduce_kernel on float dats. */
+void ops_par_loop_poisson_residual_kernel(const char *name, ops_block block, int dim, int *range, ops_arg r,
+                                          ops_arg f, ops_arg x, ops_arg rr);
+void ops_par_loop_mp_init_kernel(const char *name, ops_block block, int dim, int *range, ops_arg e, ops_arg rr,
+                                 ops_arg p, ops_arg r, ops_arg mi, ops_arg dots);
+void ops_par_loop_mp_update_kernel(const char *name, ops_block block, int dim, int *range, ops_arg e, ops_arg rr,
+                                   ops_arg p, ops_arg w, ops_arg a, ops_arg mi, ops_arg dots);
+void ops_par_loop_mp_p_kernel(const char *name, ops_block block, int dim, int *range, ops_arg p, ops_arg rr,
+                              ops_arg mi, ops_arg b);
+void ops_par_loop_mp_correct_kernel(const char *name, ops_block block, int dim, int *range, ops_arg x,
+                                    ops_arg e);
+
 /* ADI right-hand side and coefficients: apps/c/adi/preproc_kernel.h:4-39
  * (constants nx, ny, nz, lambda) */
 void ops_par_loop_preproc_kernel(const char *name, ops_block block, int dim, int *range, ops_arg u, ops_arg du,

@@ -35,6 +35,19 @@ int opsapp_poisson_iterate(void *ctx, int max_iters, double tol, double *rrn_h, 
 void opsapp_poisson_fetch_x(void *ctx, double *out);
 void opsapp_poisson_destroy(void *ctx);
 
+/* mixed-precision Poisson: FP64 iterative refinement (residual recomputed in
+ * FP64, correction x += e in FP64) around an FP32 Jacobi-preconditioned CG
+ * with FP64 dot accumulation.  solve stops once |f - A x| <= tol |f| or after
+ * max_outer corrections; each correction runs the inner CG to
+ * max(inner_tol, tol / (2 relres)) relative, or inner_iters.  relres_h gets |f - A x|/|f| before each
+ * correction and after the last (outer + 1 entries).  Returns outer
+ * iterations; *inner_total gets the inner iterations summed. */
+void *opsapp_pmixed_create(int ndim, int nx, int ny, int nz, uint64_t seed, const double *f);
+int opsapp_pmixed_solve(void *ctx, double tol, int max_outer, int inner_iters, double inner_tol,
+                        double *relres_h, int *inner_total);
+void opsapp_pmixed_fetch_x(void *ctx, double *out);
+void opsapp_pmixed_destroy(void *ctx);
+
 /* Laplace Jacobi (apps/c/laplace2d_tutorial/step7/laplace2d.cpp:37-139):
  * mode 0 top-edge boundary, 1 the tutorial's left/right boundary; init_ext
  * (halo-including) replaces the boundary loops when given.  Returns the last
@@ -49,6 +62,16 @@ void opsapp_trid_sweep(void *ctx, int inc);
 void opsapp_trid_solve(void *ctx, int solvedim, int inc);
 void opsapp_trid_fetch(void *ctx, double *d_out, double *u_out);
 void opsapp_trid_destroy(void *ctx);
+
+/* request stream on the batched engine (include/ops_mi355x_serve.h), on the
+ * wall clock: request i (kind[i], n[i], seed[i]) arrives arrival_s[i]
+ * seconds after the start (ascending) and is submitted once it has arrived;
+ * done_s[i] gets the time of the scheduler step that reported it converged,
+ * iters[i] its iteration count (negative: stopped without converging).
+ * Returns the time from start to the last completion, or -1 on error. */
+double opsapp_serve_stream(int nreq, const int *kind, const int *n, const uint64_t *seed, const double *arrival_s,
+                           double tol, size_t budget, int max_slots, int iters_per_launch, double *done_s,
+                           int *iters);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,6 @@
 """ctypes front end of libopsmi_apps.so (include/ops_mi355x_apps.h): the
-C++ application drivers (Laplace, TeaLeaf-form heat CG, Poisson CG, ADI
-tridiagonal sweeps) that run on the OPS C ABI."""
+C++ application drivers (Laplace, TeaLeaf-form heat CG, Poisson CG,
+mixed-precision Poisson, ADI tridiagonal sweeps) that run on the OPS C ABI."""
 import ctypes as C
 import os
 
@@ -31,12 +31,18 @@ def lib():
             "opsapp_poisson_iterate": (ci, [vp, ci, cd, _d, _d, _d]),
             "opsapp_poisson_fetch_x": (None, [vp, _d]),
             "opsapp_poisson_destroy": (None, [vp]),
+            "opsapp_pmixed_create": (vp, [ci, ci, ci, ci, u64, _d]),
+            "opsapp_pmixed_solve": (ci, [vp, cd, ci, ci, cd, _d, C.POINTER(C.c_int)]),
+            "opsapp_pmixed_fetch_x": (None, [vp, _d]),
+            "opsapp_pmixed_destroy": (None, [vp]),
             "opsapp_laplace2d": (cd, [ci, ci, ci, ci, _d, _d, _d]),
             "opsapp_trid_create": (vp, [ci, ci, ci, u64, ci]),
             "opsapp_trid_sweep": (None, [vp, ci]),
             "opsapp_trid_solve": (None, [vp, ci, ci]),
             "opsapp_trid_fetch": (None, [vp, _d, _d]),
             "opsapp_trid_destroy": (None, [vp]),
+            "opsapp_serve_stream": (cd, [ci, C.POINTER(ci), C.POINTER(ci), C.POINTER(u64), _d, cd, C.c_size_t, ci,
+                                         ci, _d, C.POINTER(ci)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -117,6 +123,37 @@ class Poisson:
             self.ctx = None
 
 
+class PoissonMixed:
+    """Mixed-precision Poisson: FP64 iterative refinement around an FP32
+    Jacobi-preconditioned CG (FP64 dot accumulation)."""
+
+    def __init__(self, ndim, nx, ny, nz=1, seed=20240611, f=None):
+        self.shape = (ny, nx) if ndim == 2 else (nz, ny, nx)
+        self._f = None if f is None else np.ascontiguousarray(f, dtype=np.float64)
+        self.ctx = lib().opsapp_pmixed_create(ndim, nx, ny, nz, seed, _p(self._f))
+        _chk()
+        self._f = None
+
+    def solve(self, tol=1e-10, max_outer=30, inner_iters=100000, inner_tol=1e-4):
+        rel = np.zeros(max_outer + 1)
+        tot = C.c_int(0)
+        n = lib().opsapp_pmixed_solve(self.ctx, tol, max_outer, inner_iters, inner_tol, _p(rel), C.byref(tot))
+        _chk()
+        return dict(outer=n, relres=rel[: n + 1], inner_total=tot.value)
+
+    def x(self):
+        out = np.zeros(self.shape)
+        lib().opsapp_pmixed_fetch_x(self.ctx, _p(out))
+        _chk()
+        return out
+
+    def close(self):
+        if self.ctx:
+            lib().opsapp_pmixed_destroy(self.ctx)
+            _chk()
+            self.ctx = None
+
+
 def laplace2d(nx, ny, iters, mode=0, init_ext=None, want_field=True):
     hist = np.zeros(iters)
     A = np.zeros((ny, nx)) if want_field else None
@@ -151,3 +188,24 @@ class Trid:
             lib().opsapp_trid_destroy(self.ctx)
             _chk()
             self.ctx = None
+
+
+def serve_stream(kinds, sizes, seeds, arrival_s, tol=1e-10, budget=200 * 10 ** 9, max_slots=256,
+                 iters_per_launch=8):
+    """Runs a request stream on the batched engine (opsapp_serve_stream);
+    returns (makespan_s, done_s, iters)."""
+    n = len(kinds)
+    k = np.ascontiguousarray(kinds, dtype=np.int32)
+    s = np.ascontiguousarray(sizes, dtype=np.int32)
+    sd = np.ascontiguousarray(seeds, dtype=np.uint64)
+    arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
+    done = np.zeros(n)
+    it = np.zeros(n, dtype=np.int32)
+    ci = C.POINTER(C.c_int)
+    span = lib().opsapp_serve_stream(n, k.ctypes.data_as(ci), s.ctypes.data_as(ci),
+                                     sd.ctypes.data_as(C.POINTER(C.c_uint64)), _p(arr), tol, budget, max_slots,
+                                     iters_per_launch, _p(done), it.ctypes.data_as(ci))
+    _chk()
+    if span < 0:
+        raise RuntimeError("opsapp_serve_stream failed")
+    return span, done, it

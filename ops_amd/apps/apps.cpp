@@ -6,6 +6,8 @@
 //                       generalisation with the z pair appended)
 //   Poisson CG          constant-coefficient 5/7-point operator
 //   ADI tridiagonal     apps/c/adi/adi.cpp:344-424 (random coefficients)
+//   mixed-precision Poisson and the request stream on the batched engine
+//   (include/ops_mi355x_serve.h) have no reference app
 // Exported through include/ops_mi355x_apps.h for the benchmark and tests.
 #include "../../include/ops_mi355x_apps.h"
 
@@ -16,6 +18,7 @@
 #include <vector>
 
 #include "../../include/ops_mi355x.h"
+#include "../../include/ops_mi355x_serve.h"
 #include "problems.h"
 
 using namespace opsapp;
@@ -52,6 +55,12 @@ ops_dat decl(ops_block b, const Geo &g, const char *name, const char *type = "do
 }
 
 ops_arg gbl(double *v) { return ops_arg_gbl_char((char *)v, 1, sizeof(double), OPS_READ); }
+
+double wtime() {
+  double cpu, et;
+  ops_timers(&cpu, &et);
+  return et;
+}
 
 // ---------------------------------------------------------------- heat CG
 struct Heat {
@@ -329,6 +338,127 @@ void opsapp_poisson_destroy(void *ctx) {
   delete P;
 }
 
+// ------------------------------------------ mixed-precision Poisson (IR + FP32 PCG)
+struct PoissonMixed {
+  Geo g;
+  ops_block blk;
+  ops_dat f, x, r;      // double
+  ops_dat e, rr, p, w;  // float
+  ops_stencil s0, sp;
+  ops_reduction red, dots;
+  double fn = 0;
+  PoissonMixed(int d, int a, int b, int c) : g(d, a, b, c) {}
+};
+
+void *opsapp_pmixed_create(int ndim, int nx, int ny, int nz, uint64_t seed, const double *f) {
+  PoissonMixed *M = new PoissonMixed(ndim, nx, ny, nz);
+  Geo &g = M->g;
+  M->blk = ops_decl_block(ndim, "poisson_mixed");
+  M->f = decl(M->blk, g, "f");
+  M->x = decl(M->blk, g, "x");
+  M->r = decl(M->blk, g, "r");
+  M->e = decl(M->blk, g, "e", "float", 4);
+  M->rr = decl(M->blk, g, "rr", "float", 4);
+  M->p = decl(M->blk, g, "p32", "float", 4);
+  M->w = decl(M->blk, g, "w32", "float", 4);
+  M->s0 = ndim == 3 ? ops_decl_stencil(3, 1, S000, "000") : ops_decl_stencil(2, 1, S00, "00");
+  M->sp = ndim == 3 ? ops_decl_stencil(3, 7, S7, "7pt") : ops_decl_stencil(2, 5, S5, "5pt");
+  M->red = ops_decl_reduction_handle(sizeof(double), "double", "red");
+  M->dots = ops_decl_reduction_handle(2 * sizeof(double), "double", "dots");
+  std::vector<double> fv;
+  if (!f) {
+    fv.resize(g.n_int());
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < g.nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) fv[lin(i, j, k, nx, ny)] = poisson_rhs(seed, i, j, k, nx, ny);
+    f = fv.data();
+  }
+  ops_dat_set_data(M->f, 0, (char *)f);
+  double ff = 0;
+  ops_par_loop_red_sumsq("red_sumsq", M->blk, ndim, g.inner, ops_arg_dat(M->f, 1, M->s0, "double", OPS_READ),
+                         ops_arg_reduce(M->red, 1, "double", OPS_INC));
+  ops_reduction_result_char(M->red, sizeof(double), (char *)&ff);
+  M->fn = sqrt(ff);
+  return M;
+}
+
+// Iterative refinement from the current x: the outer loop recomputes the
+// FP64 residual and stops at |f - A x| <= tol |f|; each correction solves
+// A e = r in FP32 with Jacobi PCG to inner_tol (relative, relaxed on the last
+// correction) or inner_iters.  Same steps as the oracle's orc_poisson_mixed.
+int opsapp_pmixed_solve(void *ctx, double tol, int max_outer, int inner_iters, double inner_tol,
+                        double *relres_h, int *inner_total) {
+  PoissonMixed *M = (PoissonMixed *)ctx;
+  Geo &g = M->g;
+  const int d = g.ndim;
+  float mi = (float)(1.0 / (d == 3 ? 6.0 : 4.0));
+  int outer = 0, total = 0;
+  while (true) {
+    double rrd = 0;
+    ops_par_loop_poisson_residual_kernel("poisson_residual_kernel", M->blk, d, g.inner,
+                                         ops_arg_dat(M->r, 1, M->s0, "double", OPS_WRITE),
+                                         ops_arg_dat(M->f, 1, M->s0, "double", OPS_READ),
+                                         ops_arg_dat(M->x, 1, M->sp, "double", OPS_READ),
+                                         ops_arg_reduce(M->red, 1, "double", OPS_INC));
+    ops_reduction_result_char(M->red, sizeof(double), (char *)&rrd);
+    const double rel = sqrt(rrd) / M->fn;
+    if (relres_h) relres_h[outer] = rel;
+    if (rel <= tol || outer >= max_outer) break;
+    double dots[2];
+    ops_par_loop_mp_init_kernel("mp_init_kernel", M->blk, d, g.inner, ops_arg_dat(M->e, 1, M->s0, "float", OPS_WRITE),
+                                ops_arg_dat(M->rr, 1, M->s0, "float", OPS_WRITE),
+                                ops_arg_dat(M->p, 1, M->s0, "float", OPS_WRITE),
+                                ops_arg_dat(M->r, 1, M->s0, "double", OPS_READ),
+                                ops_arg_gbl_char((char *)&mi, 1, sizeof(float), OPS_READ),
+                                ops_arg_reduce(M->dots, 2, "double", OPS_INC));
+    ops_reduction_result_char(M->dots, 2 * sizeof(double), (char *)dots);
+    double rz = dots[0];
+    // the last correction needs only reach the outer target: stop the inner
+    // solve at max(inner_tol, tol / (2 relres)) of its initial residual
+    const double istop = fmax(inner_tol, 0.5 * tol / rel) * sqrt(dots[1]);
+    for (int it = 0; it < inner_iters; it++) {
+      double pw = 0;
+      ops_par_loop_poisson_calc_w_reduce_kernel("poisson_calc_w_reduce_kernel", M->blk, d, g.inner,
+                                                ops_arg_dat(M->w, 1, M->s0, "float", OPS_WRITE),
+                                                ops_arg_dat(M->p, 1, M->sp, "float", OPS_READ),
+                                                ops_arg_reduce(M->red, 1, "double", OPS_INC));
+      ops_reduction_result_char(M->red, sizeof(double), (char *)&pw);
+      float a = (float)(rz / pw);
+      ops_par_loop_mp_update_kernel("mp_update_kernel", M->blk, d, g.inner,
+                                    ops_arg_dat(M->e, 1, M->s0, "float", OPS_RW),
+                                    ops_arg_dat(M->rr, 1, M->s0, "float", OPS_RW),
+                                    ops_arg_dat(M->p, 1, M->s0, "float", OPS_READ),
+                                    ops_arg_dat(M->w, 1, M->s0, "float", OPS_READ),
+                                    ops_arg_gbl_char((char *)&a, 1, sizeof(float), OPS_READ),
+                                    ops_arg_gbl_char((char *)&mi, 1, sizeof(float), OPS_READ),
+                                    ops_arg_reduce(M->dots, 2, "double", OPS_INC));
+      ops_reduction_result_char(M->dots, 2 * sizeof(double), (char *)dots);
+      total++;
+      if (sqrt(dots[1]) <= istop) break;
+      float b = (float)(dots[0] / rz);
+      ops_par_loop_mp_p_kernel("mp_p_kernel", M->blk, d, g.inner, ops_arg_dat(M->p, 1, M->s0, "float", OPS_RW),
+                               ops_arg_dat(M->rr, 1, M->s0, "float", OPS_READ),
+                               ops_arg_gbl_char((char *)&mi, 1, sizeof(float), OPS_READ),
+                               ops_arg_gbl_char((char *)&b, 1, sizeof(float), OPS_READ));
+      rz = dots[0];
+    }
+    ops_par_loop_mp_correct_kernel("mp_correct_kernel", M->blk, d, g.inner, ops_arg_dat(M->x, 1, M->s0, "double", OPS_RW),
+                                   ops_arg_dat(M->e, 1, M->s0, "float", OPS_READ));
+    outer++;
+  }
+  if (inner_total) *inner_total = total;
+  return outer;
+}
+
+void opsapp_pmixed_fetch_x(void *ctx, double *out) { ops_dat_fetch_data(((PoissonMixed *)ctx)->x, 0, (char *)out); }
+
+void opsapp_pmixed_destroy(void *ctx) {
+  PoissonMixed *M = (PoissonMixed *)ctx;
+  for (ops_dat q : {M->f, M->x, M->r, M->e, M->rr, M->p, M->w}) ops_free_dat(q);
+  delete M;
+}
+
 // ---------------------------------------------------------------- Laplace
 double opsapp_laplace2d(int nx, int ny, int iters, int mode, const double *init_ext, double *err_hist,
                         double *A_out) {
@@ -451,6 +581,57 @@ void opsapp_trid_destroy(void *ctx) {
   ops_free_dat(T->u);
   ops_tridsolver_params_destroy(T->ctx);
   delete T;
+}
+
+// ---------------------------------------------------------------- serving
+// Request stream on the batched engine, on the wall clock: request i arrives
+// arrival_s[i] seconds after the start (ascending), is submitted once it has
+// arrived, and completes when a scheduler step reports it converged (its
+// memory is then released).  Between steps nothing else runs on the host.
+double opsapp_serve_stream(int nreq, const int *kind, const int *n, const uint64_t *seed, const double *arrival_s,
+                           double tol, size_t budget, int max_slots, int iters_per_launch, double *done_s,
+                           int *iters) {
+  opsmi_server S = opsmi_serve_create(budget, max_slots, iters_per_launch, 1);
+  if (!S) return -1.0;
+  std::vector<int> idx_of;
+  std::vector<opsmi_solve_done> buf(256);
+  const double t0 = wtime();
+  int next = 0, done = 0;
+  while (done < nreq) {
+    double t = wtime() - t0;
+    while (next < nreq && arrival_s[next] <= t) {
+      opsmi_solve_req q{kind[next], n[next], seed[next], tol, 1000000};
+      const int id = opsmi_serve_submit(S, &q);
+      if (id < 0) {
+        opsmi_serve_destroy(S);
+        return -1.0;
+      }
+      if ((int)idx_of.size() <= id) idx_of.resize(id + 1, -1);
+      idx_of[id] = next++;
+    }
+    if (opsmi_serve_active(S) + opsmi_serve_queued(S) == 0) {
+      while (next < nreq && wtime() - t0 < arrival_s[next]) {
+      }
+      continue;
+    }
+    if (opsmi_serve_step(S) < 0) {
+      opsmi_serve_destroy(S);
+      return -1.0;
+    }
+    const double tc = wtime() - t0;
+    int k;
+    while ((k = opsmi_serve_completed(S, buf.data(), (int)buf.size())) > 0)
+      for (int j = 0; j < k; j++) {
+        const int i = idx_of[buf[j].id];
+        done_s[i] = tc;
+        iters[i] = buf[j].status == 1 ? buf[j].iters : -buf[j].iters;
+        opsmi_serve_release(S, buf[j].id);
+        done++;
+      }
+  }
+  const double span = wtime() - t0;
+  opsmi_serve_destroy(S);
+  return span;
 }
 
 }  // extern "C"

@@ -5,6 +5,7 @@
 // contraction off), so results match the reference sequential backend bit
 // for bit; sums differ only by reduction order.
 #include "engine.h"
+#include "heat_tile.h"
 
 using namespace opsmi;
 
@@ -168,12 +169,8 @@ struct CalcW {
     s[0] = s[0] + v * p(x, y, z);
   }
 };
-// 3D TeaLeaf operator, z-marching form.  A 64 x 4 workgroup owns a 64 x 4
-// column of the grid over `chunk` z planes; each thread keeps p(z-1), p(z),
-// p(z+1) and Kz(z), Kz(z+1) in registers, so only the new plane is loaded
-// per step and the z neighbours never travel from HBM twice.  x and y
-// neighbours come from the same or adjacent rows of the wave's workgroup
-// (L1/L2 hits).  Same arithmetic, in the same order, as heat_op<3>.
+// 3D TeaLeaf operator, z-marching form (heat3_column): a 64 x 4 workgroup
+// owns a 64 x 4 column of the grid over `chunk` z planes.
 // INIT = tea_leaf_common_init_kernel (w = A u, r = u - w); otherwise
 // tea_leaf_cg_calc_w_reduce_kernel (w = A p, pw += w p).
 template <bool INIT>
@@ -181,46 +178,11 @@ __global__ __launch_bounds__(256) void k_heat3_march(Ptr<double> w, Ptr<double> 
                                                      Ptr<const double> Ky, Ptr<const double> Kz,
                                                      Ptr<const double> p, double rx, double ry, double rz, Range3 rg,
                                                      int chunk, RedCtl ctl) {
-  double acc[1] = {0.0};
+  __shared__ HeatLds lds;
   const TileId t = tile_id();
-  const int x = rg.x0 + (int)t.x * kWave + (int)threadIdx.x;
-  const int y = rg.y0 + (int)t.y * kWavesPerBlock + (int)threadIdx.y;
-  if (x < rg.x1 && y < rg.y1) {
-    const int z0 = rg.z0 + (int)t.z * chunk;
-    const int z1 = min(z0 + chunk, rg.z1);
-    // state of plane z: centre column p(z-1), p(z), p(z+1), Kz(z), Kz(z+1)
-    // and the in-plane neighbours.  The next plane's loads are issued before
-    // this plane's stores, so they are in flight while it is written.
-    double pm = p(x, y, z0 - 1), pc = p(x, y, z0), pn = p(x, y, z0 + 1);
-    double kzc = Kz(x, y, z0), kzn = Kz(x, y, z0 + 1);
-    double kxc = Kx(x, y, z0), kxn = Kx(x + 1, y, z0), kyc = Ky(x, y, z0), kyn = Ky(x, y + 1, z0);
-    double pxm = p(x - 1, y, z0), pxp = p(x + 1, y, z0), pym = p(x, y - 1, z0), pyp = p(x, y + 1, z0);
-    for (int z = z0; z < z1; z++) {
-      const double v = (1.0 + ry * (kyn + kyc) + rx * (kxn + kxc) + rz * (kzn + kzc)) * pc -
-                       ry * (kyn * pyp + kyc * pym) - rx * (kxn * pxp + kxc * pxm) - rz * (kzn * pn + kzc * pm);
-      const double pc_v = pc;
-      if (z + 1 < z1) {
-        const int zn = z + 1;
-        const double pnn = p(x, y, zn + 1), kznn = Kz(x, y, zn + 1);
-        kxc = Kx(x, y, zn);
-        kxn = Kx(x + 1, y, zn);
-        kyc = Ky(x, y, zn);
-        kyn = Ky(x, y + 1, zn);
-        pxm = p(x - 1, y, zn);
-        pxp = p(x + 1, y, zn);
-        pym = p(x, y - 1, zn);
-        pyp = p(x, y + 1, zn);
-        pm = pc;
-        pc = pn;
-        pn = pnn;
-        kzc = kzn;
-        kzn = kznn;
-      }
-      w(x, y, z) = v;
-      if constexpr (INIT) r(x, y, z) = pc_v - v;
-      else acc[0] = acc[0] + v * pc_v;
-    }
-  }
+  const int z0 = rg.z0 + (int)t.z * chunk;
+  double acc[1] = {heat3_column<INIT>(lds, w, r, Kx, Ky, Kz, p, rx, ry, rz, rg.x0 + (int)t.x * kWave,
+                                      rg.y0 + (int)t.y * kWavesPerBlock, rg.x1, rg.y1, z0, min(z0 + chunk, rg.z1))};
   if constexpr (!INIT) reduce_block_store<double, OPS_INC, 1>(acc, ctl);
 }
 

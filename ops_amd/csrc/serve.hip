@@ -1,0 +1,683 @@
+// Batched multi-block CG engine with a continuous-batching scheduler
+// (include/ops_mi355x_serve.h).
+//
+// Device side.  Each active solve (one block) is described by a SolveDev
+// (field pointers, strides, sizes, tile grid) and a SolveStat (CG scalars
+// and status) in device memory.  A tile table lists (solve, tile) pairs of
+// every active solve, contiguous per solve.  One CG iteration is five
+// launches, each persistent (a fixed grid walks the tile table; blocks are
+// renumbered so each XCD walks a contiguous stretch of tiles):
+//   sk_calc_w    w = A p per tile, partial p.w per tile
+//   sk_fold<0>   per solve: pw = fold of its partials, alpha = rro / pw
+//   sk_update    u += alpha p ; r -= alpha w, partial r.r per tile
+//   sk_fold<1>   per solve: rrn, beta = rrn / rro, rro = rrn, stop test
+//   sk_axpby     p = beta p + r
+// Solves that are done skip all work, so a graph of K iterations stops each
+// solve exactly where it converges.  Partials are folded per solve, in a
+// fixed order, by one workgroup: a solve's numbers do not depend on the rest
+// of the batch.  3D heat tiles are 64 x 4 x chunk z-marching columns
+// (heat3_column); 2D Poisson tiles are 64 x (4 chunk) rows.
+//
+// Host side.  Fields live in one arena of the HBM budget (first-fit, 256 B
+// granules, coalescing).  A step admits queued solves in FIFO order while a
+// slot and memory are free (set-up kernels generate inputs, coefficients
+// and the first residual on the device), rebuilds the tile table if the
+// batch changed, launches K iterations (one graph launch), reads back the
+// per-solve status and retires converged solves.  Tables are read by the
+// kernels from device memory, so one captured graph serves every batch.
+#include <deque>
+#include <map>
+
+#include "../../include/ops_mi355x_serve.h"
+#include "../apps/problems.h"
+#include "engine.h"
+#include "heat_tile.h"
+
+using namespace opsmi;
+
+namespace {
+
+constexpr int kRows2D = 4;  // rows per wave of a 2D tile (tile = 64 x 16)
+constexpr int kChunk3D = 16;
+
+struct SolveDev {
+  double *u, *p, *r, *w, *kx, *ky, *kz;  // element (0,0,0) of each field
+  long sy, sz;
+  int kind, nx, ny, nz;
+  int tx, ty;   // tiles along x and y
+  int chunk;    // 3D: z planes per tile; 2D: rows per wave
+  int tile0, ntiles;
+  int active;
+  double rx, ry, rz;
+};
+
+struct SolveStat {
+  double rro, rr0, stop, pw, alpha, beta;
+  int iters, done, max_iters, pad;
+};
+
+struct TileBox {
+  int x0, y0, z0, z1;
+};
+
+__device__ __forceinline__ TileBox tile_box(const SolveDev &d, int li) {
+  TileBox b;
+  const int tx = li % d.tx, rest = li / d.tx;
+  b.x0 = tx * kWave;
+  if (d.kind == OPSMI_SOLVE_HEAT3D) {
+    b.y0 = (rest % d.ty) * kWavesPerBlock;
+    b.z0 = (rest / d.ty) * d.chunk;
+    b.z1 = min(b.z0 + d.chunk, d.nz);
+  } else {
+    b.y0 = rest * kWavesPerBlock * d.chunk;
+    b.z0 = 0;
+    b.z1 = 1;
+  }
+  return b;
+}
+
+// f(x, y, z) over this thread's points of the tile
+template <class F>
+__device__ __forceinline__ void tile_points(const SolveDev &d, const TileBox &b, F f) {
+  const int x = b.x0 + (int)threadIdx.x;
+  if (x >= d.nx) return;
+  if (d.kind == OPSMI_SOLVE_HEAT3D) {
+    const int y = b.y0 + (int)threadIdx.y;
+    if (y >= d.ny) return;
+    for (int z = b.z0; z < b.z1; z++) f(x, y, z);
+  } else {
+    for (int i = 0; i < d.chunk; i++) {
+      const int y = b.y0 + (int)threadIdx.y + kWavesPerBlock * i;
+      if (y < d.ny) f(x, y, 0);
+    }
+  }
+}
+
+__device__ __forceinline__ Ptr<double> P(double *a, const SolveDev &d) { return Ptr<double>{a, d.sy, d.sz}; }
+__device__ __forceinline__ Ptr<const double> PC(const double *a, const SolveDev &d) {
+  return Ptr<const double>{a, d.sy, d.sz};
+}
+
+__device__ __forceinline__ void store_partial(double v, double *dst, double (*sh)[1]) {
+  double acc[1] = {v};
+  block_reduce<double, OPS_INC, 1>(acc, sh);
+  if (threadIdx.x == 0 && threadIdx.y == 0) *dst = acc[0];
+}
+
+// ------------------------------------------------------------ iteration
+__global__ __launch_bounds__(256) void sk_calc_w(const SolveDev *S, const SolveStat *st, const int2 *tiles,
+                                                 const int *ntiles, double *part) {
+  __shared__ HeatLds lds;
+  __shared__ double sh[16][1];
+  const int nt = *ntiles;
+  for (int t = (int)xcd_tile(blockIdx.x, gridDim.x); t < nt; t += (int)gridDim.x) {
+    const int2 e = tiles[t];
+    if (st[e.x].done) continue;
+    const SolveDev d = S[e.x];
+    const TileBox b = tile_box(d, e.y);
+    double acc = 0.0;
+    if (d.kind == OPSMI_SOLVE_HEAT3D) {
+      __syncthreads();  // LDS of the previous tile fully read
+      acc = heat3_column<false>(lds, P(d.w, d), Ptr<double>{nullptr, 0, 0}, PC(d.kx, d), PC(d.ky, d), PC(d.kz, d),
+                                PC(d.p, d), d.rx, d.ry, d.rz, b.x0, b.y0, d.nx, d.ny, b.z0, b.z1);
+    } else {
+      const Ptr<const double> p = PC(d.p, d);
+      const Ptr<double> w = P(d.w, d);
+      tile_points(d, b, [&](int x, int y, int z) {
+        const double v = 4.0 * p(x, y, z) - (p(x - 1, y, z) + p(x + 1, y, z)) - (p(x, y - 1, z) + p(x, y + 1, z));
+        w(x, y, z) = v;
+        acc = acc + v * p(x, y, z);
+      });
+    }
+    store_partial(acc, part + t, sh);
+  }
+}
+
+__global__ __launch_bounds__(256) void sk_update(const SolveDev *S, const SolveStat *st, const int2 *tiles,
+                                                 const int *ntiles, double *part) {
+  __shared__ double sh[16][1];
+  const int nt = *ntiles;
+  for (int t = (int)xcd_tile(blockIdx.x, gridDim.x); t < nt; t += (int)gridDim.x) {
+    const int2 e = tiles[t];
+    if (st[e.x].done) continue;
+    const SolveDev d = S[e.x];
+    const double alpha = st[e.x].alpha;
+    const Ptr<double> u = P(d.u, d), r = P(d.r, d);
+    const Ptr<const double> p = PC(d.p, d), w = PC(d.w, d);
+    double acc = 0.0;
+    tile_points(d, tile_box(d, e.y), [&](int x, int y, int z) {
+      u(x, y, z) = u(x, y, z) + alpha * p(x, y, z);
+      const double v = r(x, y, z) - alpha * w(x, y, z);
+      r(x, y, z) = v;
+      acc = acc + v * v;
+    });
+    store_partial(acc, part + t, sh);
+  }
+}
+
+__global__ __launch_bounds__(256) void sk_axpby(const SolveDev *S, const SolveStat *st, const int2 *tiles,
+                                                const int *ntiles) {
+  const int nt = *ntiles;
+  for (int t = (int)xcd_tile(blockIdx.x, gridDim.x); t < nt; t += (int)gridDim.x) {
+    const int2 e = tiles[t];
+    if (st[e.x].done) continue;
+    const SolveDev d = S[e.x];
+    const double beta = st[e.x].beta;
+    const Ptr<double> p = P(d.p, d);
+    const Ptr<const double> r = PC(d.r, d);
+    tile_points(d, tile_box(d, e.y), [&](int x, int y, int z) { p(x, y, z) = beta * p(x, y, z) + 1.0 * r(x, y, z); });
+  }
+}
+
+// PHASE 0: alpha; PHASE 1: beta, iteration count and stop test
+template <int PHASE>
+__global__ __launch_bounds__(256) void sk_fold(const SolveDev *S, SolveStat *st, const double *part) {
+  __shared__ double sh[16][1];
+  const int s = (int)blockIdx.x;
+  if (!S[s].active || st[s].done) return;
+  const int t0 = S[s].tile0, n = S[s].ntiles;
+  double acc[1] = {0.0};
+  for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) acc[0] = acc[0] + part[t0 + i];
+  block_reduce<double, OPS_INC, 1>(acc, sh);
+  if (threadIdx.x == 0) {
+    SolveStat &q = st[s];
+    if (PHASE == 0) {
+      q.pw = acc[0];
+      q.alpha = q.rro / acc[0];
+    } else {
+      const double rrn = acc[0];
+      q.beta = rrn / q.rro;
+      q.rro = rrn;
+      q.iters = q.iters + 1;
+      if (sqrt(rrn) <= q.stop) q.done = 1;
+      else if (q.iters >= q.max_iters) q.done = 2;
+    }
+  }
+}
+
+// ------------------------------------------------------------ set-up
+__global__ __launch_bounds__(256) void sk_zero(double *a, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = 0.0;
+}
+
+// heat: w = 1/K over the extended box ([-1, n] per side), u = u0 = rhs inside
+__global__ __launch_bounds__(256) void sk_heat_inputs(SolveDev d, uint64_t seed) {
+  const long ne = (long)(d.nx + 2) * (d.ny + 2) * (d.nz + 2);
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < ne; q += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(q % (d.nx + 2)) - 1;
+    const long rest = q / (d.nx + 2);
+    const int y = (int)(rest % (d.ny + 2)) - 1, z = (int)(rest / (d.ny + 2)) - 1;
+    P(d.w, d)(x, y, z) = 1.0 / opsapp::heat_K(seed, x, y, z, d.nx, d.ny);
+    if (x >= 0 && x < d.nx && y >= 0 && y < d.ny && z >= 0 && z < d.nz)
+      P(d.u, d)(x, y, z) = opsapp::heat_rhs(seed, x, y, z, d.nx, d.ny, d.nz, 3);
+  }
+}
+
+// heat: face coefficients over [0, n] per side (tea_leaf_common_init_Kx_Ky_Kz
+// on the app's ext range), then the zeroed external faces x, y, z in {0, n}
+__global__ __launch_bounds__(256) void sk_heat_coefs(SolveDev d) {
+  const long ne = (long)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
+  const Ptr<const double> w = PC(d.w, d);
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < ne; q += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(q % (d.nx + 1));
+    const long rest = q / (d.nx + 1);
+    const int y = (int)(rest % (d.ny + 1)), z = (int)(rest / (d.ny + 1));
+    const double c = w(x, y, z);
+    double kx = (w(x - 1, y, z) + c) / (2.0 * w(x - 1, y, z) * c);
+    double ky = (w(x, y - 1, z) + c) / (2.0 * w(x, y - 1, z) * c);
+    double kz = (w(x, y, z - 1) + c) / (2.0 * w(x, y, z - 1) * c);
+    if (x == 0 || x == d.nx) kx = 0.0;
+    if (y == 0 || y == d.ny) ky = 0.0;
+    if (z == 0 || z == d.nz) kz = 0.0;
+    P(d.kx, d)(x, y, z) = kx;
+    P(d.ky, d)(x, y, z) = ky;
+    P(d.kz, d)(x, y, z) = kz;
+  }
+}
+
+// first residual, p = r, partial r.p per tile (grid = the solve's tiles)
+__global__ __launch_bounds__(256) void sk_first_residual(SolveDev d, uint64_t seed, double *part) {
+  __shared__ HeatLds lds;
+  __shared__ double sh[16][1];
+  const int li = (int)blockIdx.x;
+  const TileBox b = tile_box(d, li);
+  if (d.kind == OPSMI_SOLVE_HEAT3D)
+    heat3_column<true>(lds, P(d.w, d), P(d.r, d), PC(d.kx, d), PC(d.ky, d), PC(d.kz, d), PC(d.u, d), d.rx, d.ry,
+                       d.rz, b.x0, b.y0, d.nx, d.ny, b.z0, b.z1);
+  const Ptr<double> r = P(d.r, d), p = P(d.p, d);
+  const bool poisson = d.kind == OPSMI_SOLVE_POISSON2D;
+  double acc = 0.0;
+  tile_points(d, b, [&](int x, int y, int z) {
+    if (poisson) r(x, y, z) = opsapp::poisson_rhs(seed, x, y, 0, d.nx, d.ny);
+    const double v = r(x, y, z);
+    p(x, y, z) = v;
+    acc = acc + v * v;
+  });
+  store_partial(acc, part + li, sh);
+}
+
+__global__ __launch_bounds__(256) void sk_first_fold(SolveStat *q, const double *part, int n, double tol,
+                                                     int max_iters) {
+  __shared__ double sh[16][1];
+  double acc[1] = {0.0};
+  for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) acc[0] = acc[0] + part[i];
+  block_reduce<double, OPS_INC, 1>(acc, sh);
+  if (threadIdx.x == 0) {
+    q->rro = acc[0];
+    q->rr0 = acc[0];
+    q->stop = tol * sqrt(acc[0]);
+    q->pw = q->alpha = q->beta = 0.0;
+    q->iters = 0;
+    q->done = 0;
+    q->max_iters = max_iters;
+    q->pad = 0;
+  }
+}
+
+// ------------------------------------------------------------ host
+struct Arena {
+  size_t size = 0;
+  std::map<size_t, size_t> free_;  // offset -> length
+  void init(size_t n) {
+    size = n;
+    free_.clear();
+    free_[0] = n;
+  }
+  bool alloc(size_t n, size_t *off) {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < n) continue;
+      *off = it->first;
+      const size_t rest = it->second - n, o = it->first + n;
+      free_.erase(it);
+      if (rest) free_[o] = rest;
+      return true;
+    }
+    return false;
+  }
+  void release(size_t off, size_t n) {
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+struct Geometry {
+  int nfields, nx, ny, nz;
+  long pitch, plane, field;  // elements
+  size_t bytes;
+};
+
+Geometry geometry(const opsmi_solve_req &q) {
+  Geometry g;
+  const bool heat = q.kind == OPSMI_SOLVE_HEAT3D;
+  g.nfields = heat ? 7 : 4;
+  g.nx = g.ny = q.n;
+  g.nz = heat ? q.n : 1;
+  g.pitch = ((long)g.nx + 2 + 31) / 32 * 32;
+  g.plane = g.pitch * (g.ny + 2);
+  // 32 lead elements put x = 0 on a 256-byte boundary; round to 256 bytes
+  g.field = (g.plane * (heat ? g.nz + 2 : 1) + 32 + 31) / 32 * 32;
+  g.bytes = (size_t)g.nfields * g.field * sizeof(double);
+  return g;
+}
+
+struct Req {
+  opsmi_solve_req q;
+  int state = 0;  // 0 queued, 1 active, 2 done (memory held until fetch/release)
+  int slot = -1;
+  size_t off = 0, bytes = 0;
+  SolveDev dev{};
+};
+
+}  // namespace
+
+struct opsmi_server_s {
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int iters_per_launch = 8, use_graph = 1, max_slots = 0, grid = 0;
+  size_t budget = 0, in_use = 0, max_tiles = 0;
+  char *arena_base = nullptr;
+  Arena arena;
+  SolveDev *d_solves = nullptr;
+  SolveStat *d_stat = nullptr;
+  int2 *d_tiles = nullptr;
+  int *d_ntiles = nullptr;
+  double *d_part = nullptr, *d_setup_part = nullptr;
+  SolveStat *h_stat = nullptr;  // pinned
+  std::vector<SolveDev> h_solves;
+  std::vector<int> slot_req;
+  std::vector<int2> h_tiles;
+  int h_ntiles = 0;
+  bool dirty = true;
+  std::deque<int> queue;
+  std::map<int, Req> reqs;
+  std::deque<opsmi_solve_done> completed;
+  int next_id = 0, n_active = 0;
+};
+
+namespace {
+
+void check_req(const opsmi_solve_req &q) {
+  if (q.kind != OPSMI_SOLVE_POISSON2D && q.kind != OPSMI_SOLVE_HEAT3D)
+    fail(OPS_INVALID_ARGUMENT, cat("opsmi_serve_submit: unknown solve kind ", q.kind));
+  if (q.n < 2 || q.n > (q.kind == OPSMI_SOLVE_HEAT3D ? 2048 : 65536))
+    fail(OPS_INVALID_ARGUMENT, cat("opsmi_serve_submit: grid side ", q.n, " out of range"));
+  if (!(q.tol > 0.0) || q.max_iters < 1)
+    fail(OPS_INVALID_ARGUMENT, "opsmi_serve_submit: tol must be > 0 and max_iters >= 1");
+}
+
+long tiles_of(const opsmi_solve_req &q, int *tx, int *ty) {
+  const Geometry g = geometry(q);
+  *tx = (g.nx + kWave - 1) / kWave;
+  if (q.kind == OPSMI_SOLVE_HEAT3D) {
+    *ty = (g.ny + kWavesPerBlock - 1) / kWavesPerBlock;
+    return (long)*tx * *ty * ((g.nz + kChunk3D - 1) / kChunk3D);
+  }
+  *ty = (g.ny + kWavesPerBlock * kRows2D - 1) / (kWavesPerBlock * kRows2D);
+  return (long)*tx * *ty;
+}
+
+void launch_iterations(opsmi_server S, hipStream_t st) {
+  const dim3 blk(kWave, kWavesPerBlock);
+  for (int k = 0; k < S->iters_per_launch; k++) {
+    hipLaunchKernelGGL(sk_calc_w, dim3(S->grid), blk, 0, st, S->d_solves, S->d_stat, S->d_tiles, S->d_ntiles,
+                       S->d_part);
+    hipLaunchKernelGGL(sk_fold<0>, dim3(S->max_slots), dim3(256), 0, st, S->d_solves, S->d_stat, S->d_part);
+    hipLaunchKernelGGL(sk_update, dim3(S->grid), blk, 0, st, S->d_solves, S->d_stat, S->d_tiles, S->d_ntiles,
+                       S->d_part);
+    hipLaunchKernelGGL(sk_fold<1>, dim3(S->max_slots), dim3(256), 0, st, S->d_solves, S->d_stat, S->d_part);
+    hipLaunchKernelGGL(sk_axpby, dim3(S->grid), blk, 0, st, S->d_solves, S->d_stat, S->d_tiles, S->d_ntiles);
+  }
+}
+
+// admits the queue head while a slot and memory are free; sets up on device
+void admit(opsmi_server S) {
+  while (!S->queue.empty()) {
+    const int id = S->queue.front();
+    Req &R = S->reqs[id];
+    int slot = -1;
+    for (int s = 0; s < S->max_slots; s++)
+      if (S->slot_req[s] < 0) {
+        slot = s;
+        break;
+      }
+    if (slot < 0) return;
+    const Geometry g = geometry(R.q);
+    size_t off;
+    if (!S->arena.alloc(g.bytes, &off)) return;
+    int tx, ty;
+    const long nt = tiles_of(R.q, &tx, &ty);
+    if ((size_t)(S->h_ntiles + nt) > S->max_tiles) {
+      S->arena.release(off, g.bytes);
+      return;
+    }
+    S->queue.pop_front();
+    R.state = 1;
+    R.slot = slot;
+    R.off = off;
+    R.bytes = g.bytes;
+    S->in_use += g.bytes;
+    S->slot_req[slot] = id;
+    // fields: u, p, r, w [, kx, ky, kz]
+    double *base = (double *)(S->arena_base + off);
+    double *f[7] = {};
+    for (int i = 0; i < g.nfields; i++) f[i] = base + (size_t)i * g.field;
+    // element (-1,-1,-1) sits at field + 31, so x = 0 starts a 256-byte line
+    const long lead = 32 + g.pitch + (R.q.kind == OPSMI_SOLVE_HEAT3D ? g.plane : 0);
+    SolveDev d{};
+    d.u = f[0] + lead;
+    d.p = f[1] + lead;
+    d.r = f[2] + lead;
+    d.w = f[3] + lead;
+    if (R.q.kind == OPSMI_SOLVE_HEAT3D) {
+      d.kx = f[4] + lead;
+      d.ky = f[5] + lead;
+      d.kz = f[6] + lead;
+    }
+    d.sy = g.pitch;
+    d.sz = g.plane;
+    d.kind = R.q.kind;
+    d.nx = g.nx;
+    d.ny = g.ny;
+    d.nz = g.nz;
+    d.tx = tx;
+    d.ty = ty;
+    d.chunk = R.q.kind == OPSMI_SOLVE_HEAT3D ? kChunk3D : kRows2D;
+    d.tile0 = 0;
+    d.ntiles = (int)nt;
+    d.active = 1;
+    d.rx = d.ry = d.rz = 1.0;
+    R.dev = d;
+    hipStream_t st = S->stream;
+    const size_t nall = (size_t)g.nfields * g.field;
+    hipLaunchKernelGGL(sk_zero, dim3((unsigned)std::min<size_t>((nall + 255) / 256, 4096)), dim3(256), 0, st, base,
+                       nall);
+    if (R.q.kind == OPSMI_SOLVE_HEAT3D) {
+      const long ne = (long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+      const unsigned gb = (unsigned)std::min<long>((ne + 255) / 256, 4096);
+      hipLaunchKernelGGL(sk_heat_inputs, dim3(gb), dim3(256), 0, st, d, R.q.seed);
+      hipLaunchKernelGGL(sk_heat_coefs, dim3(gb), dim3(256), 0, st, d);
+    }
+    hipLaunchKernelGGL(sk_first_residual, dim3((unsigned)nt), dim3(kWave, kWavesPerBlock), 0, st, d, R.q.seed,
+                       S->d_setup_part);
+    hipLaunchKernelGGL(sk_first_fold, dim3(1), dim3(256), 0, st, S->d_stat + slot, (const double *)S->d_setup_part,
+                       (int)nt, R.q.tol, R.q.max_iters);
+    OPSMI_HIP(hipGetLastError());
+    S->dirty = true;
+    S->n_active++;
+  }
+}
+
+void rebuild_tables(opsmi_server S) {
+  S->h_tiles.clear();
+  for (int s = 0; s < S->max_slots; s++) {
+    SolveDev &d = S->h_solves[s];
+    const int rid = S->slot_req[s];
+    if (rid < 0) {
+      d = SolveDev{};
+      continue;
+    }
+    d = S->reqs[rid].dev;
+    d.tile0 = (int)S->h_tiles.size();
+    for (int i = 0; i < d.ntiles; i++) S->h_tiles.push_back(make_int2(s, i));
+  }
+  S->h_ntiles = (int)S->h_tiles.size();
+  OPSMI_HIP(hipMemcpyAsync(S->d_solves, S->h_solves.data(), sizeof(SolveDev) * S->max_slots, hipMemcpyHostToDevice,
+                           S->stream));
+  if (S->h_ntiles)
+    OPSMI_HIP(hipMemcpyAsync(S->d_tiles, S->h_tiles.data(), sizeof(int2) * S->h_ntiles, hipMemcpyHostToDevice,
+                             S->stream));
+  OPSMI_HIP(hipMemcpyAsync(S->d_ntiles, &S->h_ntiles, sizeof(int), hipMemcpyHostToDevice, S->stream));
+  OPSMI_HIP(hipStreamSynchronize(S->stream));  // host vectors may change after this
+  S->dirty = false;
+}
+
+Req &find(opsmi_server S, int id) {
+  auto it = S->reqs.find(id);
+  if (it == S->reqs.end()) fail(OPS_INVALID_ARGUMENT, cat("opsmi_serve: unknown solve id ", id));
+  return it->second;
+}
+
+void free_req(opsmi_server S, int id) {
+  Req &R = find(S, id);
+  if (R.state != 2) fail(OPS_INVALID_ARGUMENT, cat("opsmi_serve: solve ", id, " has not completed"));
+  S->arena.release(R.off, R.bytes);
+  S->in_use -= R.bytes;
+  S->reqs.erase(id);
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t opsmi_serve_request_bytes(const opsmi_solve_req *req) {
+  OPSMI_API_BEGIN
+  if (!req) fail(OPS_INVALID_ARGUMENT, "opsmi_serve_request_bytes: NULL request");
+  check_req(*req);
+  return geometry(*req).bytes;
+  OPSMI_API_END(0)
+}
+
+opsmi_server opsmi_serve_create(size_t hbm_budget, int max_slots, int iters_per_launch, int use_graph) {
+  OPSMI_API_BEGIN
+  require_init();
+  if (max_slots < 1 || max_slots > 65536 || iters_per_launch < 1 || iters_per_launch > 1024)
+    fail(OPS_INVALID_ARGUMENT, "opsmi_serve_create: max_slots in [1, 65536], iters_per_launch in [1, 1024]");
+  if (hbm_budget < (1u << 20)) fail(OPS_INVALID_ARGUMENT, "opsmi_serve_create: budget below 1 MiB");
+  opsmi_server S = new opsmi_server_s();
+  S->budget = hbm_budget / 256 * 256;
+  S->max_slots = max_slots;
+  S->iters_per_launch = iters_per_launch;
+  S->use_graph = use_graph ? 1 : 0;
+  S->grid = inst().num_cus * 8;
+  // smallest tile: a 2D tile of 64 x 16 points over 4 fields
+  S->max_tiles = S->budget / (size_t)(kWave * kWavesPerBlock * kRows2D * 4 * 8) + (size_t)max_slots * 8;
+  OPSMI_HIP(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+  OPSMI_HIP(hipMalloc(&S->arena_base, S->budget));
+  S->arena.init(S->budget);
+  OPSMI_HIP(hipMalloc(&S->d_solves, sizeof(SolveDev) * max_slots));
+  OPSMI_HIP(hipMalloc(&S->d_stat, sizeof(SolveStat) * max_slots));
+  OPSMI_HIP(hipMalloc(&S->d_tiles, sizeof(int2) * S->max_tiles));
+  OPSMI_HIP(hipMalloc(&S->d_ntiles, sizeof(int)));
+  OPSMI_HIP(hipMalloc(&S->d_part, sizeof(double) * S->max_tiles));
+  OPSMI_HIP(hipMalloc(&S->d_setup_part, sizeof(double) * S->max_tiles));
+  OPSMI_HIP(hipHostMalloc((void **)&S->h_stat, sizeof(SolveStat) * max_slots, hipHostMallocDefault));
+  OPSMI_HIP(hipMemsetAsync(S->d_stat, 0, sizeof(SolveStat) * max_slots, S->stream));
+  S->h_solves.assign(max_slots, SolveDev{});
+  S->slot_req.assign(max_slots, -1);
+  rebuild_tables(S);
+  return S;
+  OPSMI_API_END(nullptr)
+}
+
+int opsmi_serve_submit(opsmi_server S, const opsmi_solve_req *req) {
+  OPSMI_API_BEGIN
+  if (!S || !req) fail(OPS_INVALID_ARGUMENT, "opsmi_serve_submit: NULL argument");
+  check_req(*req);
+  const Geometry g = geometry(*req);
+  if (g.bytes > S->budget)
+    fail(OPS_INVALID_ARGUMENT, cat("opsmi_serve_submit: solve needs ", g.bytes, " bytes, budget is ", S->budget));
+  int tx, ty;
+  if ((size_t)tiles_of(*req, &tx, &ty) > S->max_tiles)
+    fail(OPS_INVALID_ARGUMENT, "opsmi_serve_submit: solve has more tiles than the tile table holds");
+  const int id = S->next_id++;
+  Req R;
+  R.q = *req;
+  S->reqs[id] = R;
+  S->queue.push_back(id);
+  return id;
+  OPSMI_API_END(-1)
+}
+
+int opsmi_serve_step(opsmi_server S) {
+  OPSMI_API_BEGIN
+  if (!S) fail(OPS_INVALID_ARGUMENT, "opsmi_serve_step: NULL server");
+  // admission: queue head first; set-up kernels run on the server stream
+  admit(S);
+  if (S->dirty) rebuild_tables(S);
+  if (S->n_active > 0) {
+    if (S->use_graph) {
+      if (!S->exec) {
+        OPSMI_HIP(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
+        launch_iterations(S, S->stream);
+        OPSMI_HIP(hipStreamEndCapture(S->stream, &S->graph));
+        OPSMI_HIP(hipGraphInstantiate(&S->exec, S->graph, nullptr, nullptr, 0));
+      }
+      OPSMI_HIP(hipGraphLaunch(S->exec, S->stream));
+    } else {
+      launch_iterations(S, S->stream);
+      OPSMI_HIP(hipGetLastError());
+    }
+    OPSMI_HIP(hipMemcpyAsync(S->h_stat, S->d_stat, sizeof(SolveStat) * S->max_slots, hipMemcpyDeviceToHost,
+                             S->stream));
+    OPSMI_HIP(hipStreamSynchronize(S->stream));
+    for (int s = 0; s < S->max_slots; s++) {
+      const int rid = S->slot_req[s];
+      if (rid < 0 || !S->h_stat[s].done) continue;
+      Req &R = S->reqs[rid];
+      R.state = 2;
+      R.slot = -1;
+      S->slot_req[s] = -1;
+      S->n_active--;
+      S->dirty = true;
+      opsmi_solve_done c;
+      c.id = rid;
+      c.status = S->h_stat[s].done;
+      c.iters = S->h_stat[s].iters;
+      c.rr0 = S->h_stat[s].rr0;
+      c.rr = S->h_stat[s].rro;
+      S->completed.push_back(c);
+    }
+  }
+  return S->n_active + (int)S->queue.size();
+  OPSMI_API_END(-1)
+}
+
+int opsmi_serve_completed(opsmi_server S, opsmi_solve_done *out, int cap) {
+  OPSMI_API_BEGIN
+  if (!S || (!out && cap > 0)) fail(OPS_INVALID_ARGUMENT, "opsmi_serve_completed: NULL argument");
+  int n = 0;
+  while (n < cap && !S->completed.empty()) {
+    out[n++] = S->completed.front();
+    S->completed.pop_front();
+  }
+  return n;
+  OPSMI_API_END(-1)
+}
+
+int opsmi_serve_fetch(opsmi_server S, int id, double *out) {
+  OPSMI_API_BEGIN
+  if (!S || !out) fail(OPS_INVALID_ARGUMENT, "opsmi_serve_fetch: NULL argument");
+  Req &R = find(S, id);
+  if (R.state != 2) fail(OPS_INVALID_ARGUMENT, cat("opsmi_serve_fetch: solve ", id, " has not completed"));
+  const SolveDev &d = R.dev;
+  for (int z = 0; z < d.nz; z++)
+    OPSMI_HIP(hipMemcpy2DAsync(out + (size_t)z * d.nx * d.ny, (size_t)d.nx * 8, d.u + (long)z * d.sz,
+                               (size_t)d.sy * 8, (size_t)d.nx * 8, (size_t)d.ny, hipMemcpyDeviceToHost, S->stream));
+  OPSMI_HIP(hipStreamSynchronize(S->stream));
+  free_req(S, id);
+  return 0;
+  OPSMI_API_END(-1)
+}
+
+int opsmi_serve_release(opsmi_server S, int id) {
+  OPSMI_API_BEGIN
+  if (!S) fail(OPS_INVALID_ARGUMENT, "opsmi_serve_release: NULL server");
+  free_req(S, id);
+  return 0;
+  OPSMI_API_END(-1)
+}
+
+size_t opsmi_serve_bytes_in_use(opsmi_server S) { return S ? S->in_use : 0; }
+int opsmi_serve_active(opsmi_server S) { return S ? S->n_active : 0; }
+int opsmi_serve_queued(opsmi_server S) { return S ? (int)S->queue.size() : 0; }
+
+void opsmi_serve_destroy(opsmi_server S) {
+  if (!S) return;
+  if (S->stream) (void)hipStreamSynchronize(S->stream);
+  if (S->exec) (void)hipGraphExecDestroy(S->exec);
+  if (S->graph) (void)hipGraphDestroy(S->graph);
+  (void)hipFree(S->arena_base);
+  (void)hipFree(S->d_solves);
+  (void)hipFree(S->d_stat);
+  (void)hipFree(S->d_tiles);
+  (void)hipFree(S->d_ntiles);
+  (void)hipFree(S->d_part);
+  (void)hipFree(S->d_setup_part);
+  (void)hipHostFree(S->h_stat);
+  if (S->stream) (void)hipStreamDestroy(S->stream);
+  delete S;
+}
+
+}  // extern "C"

@@ -351,7 +351,9 @@ double orc_poisson_residual(int ndim, int nx, int ny, int nz, const double *f, c
  *   e = 0; rr = (float)r; z = Mi*rr; p = z; rz = rr.z
  *   loop: w = A p; pw = p.w; a = (float)(rz/pw)
  *         e = e + a*p; rr = rr - a*w; z = Mi*rr; rzn = rr.z; rrn = rr.rr
- *         stop if sqrt(rrn) <= inner_tol*sqrt(rr0.rr0) or it == inner_iters
+ *         stop if sqrt(rrn) <= t*sqrt(rr0.rr0) or it == inner_iters, with
+ *         t = max(inner_tol, tol/(2 relres)): the last correction only
+ *         reduces the residual as far as the outer target needs
  *         b = (float)(rzn/rz); p = z + b*p; rz = rzn
  * Correction (FP64): x = x + (double)e.  Dots take products of floats in
  * FP64 (double(p)*double(w)). */
@@ -387,7 +389,7 @@ int orc_poisson_mixed(int ndim, int nx, int ny, int nz, const double *f, double 
       long q = g.at(i, j, k);
       s = s + (double)rr[q] * (double)rr[q];
     });
-    double istop = inner_tol * sqrt(rr0);
+    double istop = fmax(inner_tol, 0.5 * tol / rel) * sqrt(rr0);
     for (int it = 0; it < inner_iters; it++) {
       double pw = sweep_sum(nt, 0, nx, 0, ny, 0, NZ, [&](int i, int j, int k, double &s) {
         long q = g.at(i, j, k);
