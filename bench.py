@@ -244,7 +244,7 @@ def bench_serve(a, ops, apps):
     kinds, sizes, seeds, g = serve_requests(nreq, a.seed)
     ops.ops_init(diags=1)
     # saturation: every request queued at t = 0
-    span, done, iters = apps.serve_stream(kinds, sizes, seeds, np.zeros(nreq), tol, budget)
+    span, done, iters = apps.serve_stream(kinds, sizes, seeds, np.zeros(nreq), tol, budget, autotune=True)
     assert np.all(iters > 0), "a solve stopped without converging"
     sat = nreq / span
     print(f"batched, all queued at t=0: {nreq} solves in {span:.2f} s", file=sys.stderr, flush=True)
@@ -252,7 +252,7 @@ def bench_serve(a, ops, apps):
     rate = 0.7 * sat
     arr = np.cumsum(g.exponential(1.0 / rate, nreq))
     arr -= arr[0]
-    span2, done2, iters2 = apps.serve_stream(kinds, sizes, seeds, arr, tol, budget)
+    span2, done2, iters2 = apps.serve_stream(kinds, sizes, seeds, arr, tol, budget, autotune=True)
     lat = done2 - arr
     print(f"batched, Poisson arrivals at {rate:.1f}/s: span {span2:.2f} s", file=sys.stderr, flush=True)
     # per-solve FP64 path: the library apps, one solve after another

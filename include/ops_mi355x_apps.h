@@ -8,6 +8,7 @@
 #ifndef OPS_MI355X_APPS_H
 #define OPS_MI355X_APPS_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -68,10 +69,11 @@ void opsapp_trid_destroy(void *ctx);
  * seconds after the start (ascending) and is submitted once it has arrived;
  * done_s[i] gets the time of the scheduler step that reported it converged,
  * iters[i] its iteration count (negative: stopped without converging).
+ * autotune != 0 runs the tile-shape autotuner before the clock starts.
  * Returns the time from start to the last completion, or -1 on error. */
 double opsapp_serve_stream(int nreq, const int *kind, const int *n, const uint64_t *seed, const double *arrival_s,
-                           double tol, size_t budget, int max_slots, int iters_per_launch, double *done_s,
-                           int *iters);
+                           double tol, size_t budget, int max_slots, int iters_per_launch, int autotune,
+                           double *done_s, int *iters);
 
 #ifdef __cplusplus
 }

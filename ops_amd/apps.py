@@ -42,7 +42,7 @@ def lib():
             "opsapp_trid_fetch": (None, [vp, _d, _d]),
             "opsapp_trid_destroy": (None, [vp]),
             "opsapp_serve_stream": (cd, [ci, C.POINTER(ci), C.POINTER(ci), C.POINTER(u64), _d, cd, C.c_size_t, ci,
-                                         ci, _d, C.POINTER(ci)]),
+                                         ci, ci, _d, C.POINTER(ci)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -191,7 +191,7 @@ class Trid:
 
 
 def serve_stream(kinds, sizes, seeds, arrival_s, tol=1e-10, budget=200 * 10 ** 9, max_slots=256,
-                 iters_per_launch=8):
+                 iters_per_launch=8, autotune=False):
     """Runs a request stream on the batched engine (opsapp_serve_stream);
     returns (makespan_s, done_s, iters)."""
     n = len(kinds)
@@ -204,7 +204,7 @@ def serve_stream(kinds, sizes, seeds, arrival_s, tol=1e-10, budget=200 * 10 ** 9
     ci = C.POINTER(C.c_int)
     span = lib().opsapp_serve_stream(n, k.ctypes.data_as(ci), s.ctypes.data_as(ci),
                                      sd.ctypes.data_as(C.POINTER(C.c_uint64)), _p(arr), tol, budget, max_slots,
-                                     iters_per_launch, _p(done), it.ctypes.data_as(ci))
+                                     iters_per_launch, 1 if autotune else 0, _p(done), it.ctypes.data_as(ci))
     _chk()
     if span < 0:
         raise RuntimeError("opsapp_serve_stream failed")

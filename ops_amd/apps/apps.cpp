@@ -589,10 +589,14 @@ void opsapp_trid_destroy(void *ctx) {
 // arrived, and completes when a scheduler step reports it converged (its
 // memory is then released).  Between steps nothing else runs on the host.
 double opsapp_serve_stream(int nreq, const int *kind, const int *n, const uint64_t *seed, const double *arrival_s,
-                           double tol, size_t budget, int max_slots, int iters_per_launch, double *done_s,
-                           int *iters) {
+                           double tol, size_t budget, int max_slots, int iters_per_launch, int autotune,
+                           double *done_s, int *iters) {
   opsmi_server S = opsmi_serve_create(budget, max_slots, iters_per_launch, 1);
   if (!S) return -1.0;
+  if (autotune && opsmi_serve_autotune(S, 10) != 0) {
+    opsmi_serve_destroy(S);
+    return -1.0;
+  }
   std::vector<int> idx_of;
   std::vector<opsmi_solve_done> buf(256);
   const double t0 = wtime();

@@ -364,6 +364,10 @@ struct opsmi_server_s {
   std::map<int, Req> reqs;
   std::deque<opsmi_solve_done> completed;
   int next_id = 0, n_active = 0;
+  long tiles_active = 0;  // tiles of the admitted solves (the next table)
+  // tile parameter per size class (defaults; opsmi_serve_autotune measures)
+  int chunk3d[3] = {kChunk3D, kChunk3D, kChunk3D};
+  int rows2d[3] = {kRows2D, kRows2D, kRows2D};
 };
 
 namespace {
@@ -377,20 +381,30 @@ void check_req(const opsmi_solve_req &q) {
     fail(OPS_INVALID_ARGUMENT, "opsmi_serve_submit: tol must be > 0 and max_iters >= 1");
 }
 
-long tiles_of(const opsmi_solve_req &q, int *tx, int *ty) {
+// tile parameter of a solve: z planes per 3D tile, rows per wave of a 2D
+// tile; one value per size class (three classes per kind)
+int size_class(int kind, int n) {
+  if (kind == OPSMI_SOLVE_HEAT3D) return n < 64 ? 0 : (n < 128 ? 1 : 2);
+  return n < 256 ? 0 : (n < 1024 ? 1 : 2);
+}
+int tile_param(opsmi_server S, int kind, int n) {
+  return kind == OPSMI_SOLVE_HEAT3D ? S->chunk3d[size_class(kind, n)] : S->rows2d[size_class(kind, n)];
+}
+
+long tiles_of(const opsmi_solve_req &q, int param, int *tx, int *ty) {
   const Geometry g = geometry(q);
   *tx = (g.nx + kWave - 1) / kWave;
   if (q.kind == OPSMI_SOLVE_HEAT3D) {
     *ty = (g.ny + kWavesPerBlock - 1) / kWavesPerBlock;
-    return (long)*tx * *ty * ((g.nz + kChunk3D - 1) / kChunk3D);
+    return (long)*tx * *ty * ((g.nz + param - 1) / param);
   }
-  *ty = (g.ny + kWavesPerBlock * kRows2D - 1) / (kWavesPerBlock * kRows2D);
+  *ty = (g.ny + kWavesPerBlock * param - 1) / (kWavesPerBlock * param);
   return (long)*tx * *ty;
 }
 
-void launch_iterations(opsmi_server S, hipStream_t st) {
+void launch_iterations(opsmi_server S, hipStream_t st, int iters) {
   const dim3 blk(kWave, kWavesPerBlock);
-  for (int k = 0; k < S->iters_per_launch; k++) {
+  for (int k = 0; k < iters; k++) {
     hipLaunchKernelGGL(sk_calc_w, dim3(S->grid), blk, 0, st, S->d_solves, S->d_stat, S->d_tiles, S->d_ntiles,
                        S->d_part);
     hipLaunchKernelGGL(sk_fold<0>, dim3(S->max_slots), dim3(256), 0, st, S->d_solves, S->d_stat, S->d_part);
@@ -417,11 +431,13 @@ void admit(opsmi_server S) {
     size_t off;
     if (!S->arena.alloc(g.bytes, &off)) return;
     int tx, ty;
-    const long nt = tiles_of(R.q, &tx, &ty);
-    if ((size_t)(S->h_ntiles + nt) > S->max_tiles) {
+    const int param = tile_param(S, R.q.kind, R.q.n);
+    const long nt = tiles_of(R.q, param, &tx, &ty);
+    if ((size_t)(S->tiles_active + nt) > S->max_tiles) {
       S->arena.release(off, g.bytes);
       return;
     }
+    S->tiles_active += nt;
     S->queue.pop_front();
     R.state = 1;
     R.slot = slot;
@@ -453,7 +469,7 @@ void admit(opsmi_server S) {
     d.nz = g.nz;
     d.tx = tx;
     d.ty = ty;
-    d.chunk = R.q.kind == OPSMI_SOLVE_HEAT3D ? kChunk3D : kRows2D;
+    d.chunk = param;
     d.tile0 = 0;
     d.ntiles = (int)nt;
     d.active = 1;
@@ -541,8 +557,9 @@ opsmi_server opsmi_serve_create(size_t hbm_budget, int max_slots, int iters_per_
   S->iters_per_launch = iters_per_launch;
   S->use_graph = use_graph ? 1 : 0;
   S->grid = inst().num_cus * 8;
-  // smallest tile: a 2D tile of 64 x 16 points over 4 fields
-  S->max_tiles = S->budget / (size_t)(kWave * kWavesPerBlock * kRows2D * 4 * 8) + (size_t)max_slots * 8;
+  // smallest tile: a 2D tile of 64 x 8 points (2 rows per wave) over 4 fields
+  S->max_tiles = std::min<size_t>(S->budget / (size_t)(kWave * kWavesPerBlock * 2 * 4 * 8) + (size_t)max_slots * 8,
+                                  (size_t)1 << 25);
   OPSMI_HIP(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
   OPSMI_HIP(hipMalloc(&S->arena_base, S->budget));
   S->arena.init(S->budget);
@@ -569,7 +586,7 @@ int opsmi_serve_submit(opsmi_server S, const opsmi_solve_req *req) {
   if (g.bytes > S->budget)
     fail(OPS_INVALID_ARGUMENT, cat("opsmi_serve_submit: solve needs ", g.bytes, " bytes, budget is ", S->budget));
   int tx, ty;
-  if ((size_t)tiles_of(*req, &tx, &ty) > S->max_tiles)
+  if ((size_t)tiles_of(*req, tile_param(S, req->kind, req->n), &tx, &ty) > S->max_tiles)
     fail(OPS_INVALID_ARGUMENT, "opsmi_serve_submit: solve has more tiles than the tile table holds");
   const int id = S->next_id++;
   Req R;
@@ -590,13 +607,13 @@ int opsmi_serve_step(opsmi_server S) {
     if (S->use_graph) {
       if (!S->exec) {
         OPSMI_HIP(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
-        launch_iterations(S, S->stream);
+        launch_iterations(S, S->stream, S->iters_per_launch);
         OPSMI_HIP(hipStreamEndCapture(S->stream, &S->graph));
         OPSMI_HIP(hipGraphInstantiate(&S->exec, S->graph, nullptr, nullptr, 0));
       }
       OPSMI_HIP(hipGraphLaunch(S->exec, S->stream));
     } else {
-      launch_iterations(S, S->stream);
+      launch_iterations(S, S->stream, S->iters_per_launch);
       OPSMI_HIP(hipGetLastError());
     }
     OPSMI_HIP(hipMemcpyAsync(S->h_stat, S->d_stat, sizeof(SolveStat) * S->max_slots, hipMemcpyDeviceToHost,
@@ -610,6 +627,7 @@ int opsmi_serve_step(opsmi_server S) {
       R.slot = -1;
       S->slot_req[s] = -1;
       S->n_active--;
+      S->tiles_active -= R.dev.ntiles;
       S->dirty = true;
       opsmi_solve_done c;
       c.id = rid;
@@ -656,6 +674,82 @@ int opsmi_serve_release(opsmi_server S, int id) {
   if (!S) fail(OPS_INVALID_ARGUMENT, "opsmi_serve_release: NULL server");
   free_req(S, id);
   return 0;
+  OPSMI_API_END(-1)
+}
+
+// Tile-shape autotuner: for each kind and size class, a solve of a
+// representative size is set up alone and `iters` CG iterations are timed
+// (HIP events on the server stream, after two untimed ones) for each
+// candidate tile parameter; the fastest is kept for that class.  Classes whose
+// representative does not fit the budget keep their default.
+int opsmi_serve_autotune(opsmi_server S, int iters) {
+  OPSMI_API_BEGIN
+  if (!S) fail(OPS_INVALID_ARGUMENT, "opsmi_serve_autotune: NULL server");
+  if (S->n_active || !S->queue.empty()) fail(OPS_INVALID_ARGUMENT, "opsmi_serve_autotune: the server must be idle");
+  if (iters < 1 || iters > 1000) fail(OPS_INVALID_ARGUMENT, "opsmi_serve_autotune: iters in [1, 1000]");
+  static const int rep[2][3] = {{160, 512, 1536}, {48, 96, 192}};
+  static const int cand2[] = {2, 4, 8}, cand3[] = {4, 8, 16, 32};
+  hipEvent_t e0, e1;
+  OPSMI_HIP(hipEventCreate(&e0));
+  OPSMI_HIP(hipEventCreate(&e1));
+  for (int kind = 0; kind < 2; kind++)
+    for (int cls = 0; cls < 3; cls++) {
+      opsmi_solve_req q{kind, rep[kind][cls], 12345, 1e-300, 1 << 30};
+      if (geometry(q).bytes > S->budget) continue;
+      int *slot_param = kind == OPSMI_SOLVE_HEAT3D ? &S->chunk3d[cls] : &S->rows2d[cls];
+      const int *cand = kind == OPSMI_SOLVE_HEAT3D ? cand3 : cand2;
+      const int nc = kind == OPSMI_SOLVE_HEAT3D ? 4 : 3;
+      int best = *slot_param;
+      float best_ms = 1e30f;
+      for (int c = 0; c < nc; c++) {
+        *slot_param = cand[c];
+        int tx, ty;
+        if ((size_t)tiles_of(q, cand[c], &tx, &ty) > S->max_tiles) continue;
+        const int id = S->next_id++;
+        Req R;
+        R.q = q;
+        S->reqs[id] = R;
+        S->queue.push_back(id);
+        admit(S);
+        if (S->n_active != 1) {  // memory held by results not yet fetched
+          S->queue.pop_back();
+          S->reqs.erase(id);
+          continue;
+        }
+        rebuild_tables(S);
+        launch_iterations(S, S->stream, 2);
+        OPSMI_HIP(hipEventRecord(e0, S->stream));
+        launch_iterations(S, S->stream, iters);
+        OPSMI_HIP(hipEventRecord(e1, S->stream));
+        OPSMI_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        OPSMI_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = cand[c];
+        }
+        Req &T = S->reqs[id];
+        S->slot_req[T.slot] = -1;
+        S->tiles_active -= T.dev.ntiles;
+        S->n_active--;
+        T.state = 2;
+        free_req(S, id);
+        rebuild_tables(S);
+      }
+      *slot_param = best;
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+  OPSMI_API_END(-1)
+}
+
+int opsmi_serve_tile_param(opsmi_server S, int kind, int n) {
+  OPSMI_API_BEGIN
+  if (!S) fail(OPS_INVALID_ARGUMENT, "opsmi_serve_tile_param: NULL server");
+  if (kind != OPSMI_SOLVE_POISSON2D && kind != OPSMI_SOLVE_HEAT3D)
+    fail(OPS_INVALID_ARGUMENT, cat("opsmi_serve_tile_param: unknown solve kind ", kind));
+  return tile_param(S, kind, n);
   OPSMI_API_END(-1)
 }
 

@@ -38,6 +38,8 @@ def lib():
             "opsmi_serve_active": (ci, [vp]),
             "opsmi_serve_queued": (ci, [vp]),
             "opsmi_serve_request_bytes": (sz, [C.POINTER(SolveReq)]),
+            "opsmi_serve_autotune": (ci, [vp, ci]),
+            "opsmi_serve_tile_param": (ci, [vp, ci, ci]),
             "opsmi_serve_destroy": (None, [vp]),
         }
         for name, (res, args) in sig.items():
@@ -90,6 +92,15 @@ class Server:
     def release(self, i):
         lib().opsmi_serve_release(self.h, i)
         ops._check()
+
+    def autotune(self, iters=10):
+        lib().opsmi_serve_autotune(self.h, iters)
+        ops._check()
+
+    def tile_param(self, kind, n):
+        v = lib().opsmi_serve_tile_param(self.h, kind, n)
+        ops._check()
+        return v
 
     def bytes_in_use(self):
         return lib().opsmi_serve_bytes_in_use(self.h)

@@ -111,6 +111,26 @@ def test_admission_respects_budget(serve):
         assert c["status"] == 1 and np.sqrt(c["rr"]) <= 1e-10 * np.sqrt(c["rr0"])
 
 
+def test_autotune_picks_candidates_and_solves_stay_exact(serve):
+    S = serve
+    srv = S.Server(budget_bytes=1 << 30, max_slots=8, iters_per_launch=8)
+    srv.autotune(iters=4)
+    for n in (40, 100, 200):
+        assert srv.tile_param(S.HEAT3D, n) in (4, 8, 16, 32)
+    for n in (100, 600, 1500):
+        assert srv.tile_param(S.POISSON2D, n) in (2, 4, 8)
+    assert srv.bytes_in_use() == 0 and srv.active() == 0
+    ids = [srv.submit(k, n, seed=s) for k, n, s in MIX[:5]]
+    out = srv.run()
+    # the same server, one solve at a time: bit-identical
+    for i, (k, n, s) in zip(ids, MIX[:5]):
+        j = srv.submit(k, n, seed=s)
+        alone = srv.run()[j]
+        assert alone[0]["iters"] == out[i][0]["iters"]
+        assert np.array_equal(alone[1], out[i][1])
+    srv.close()
+
+
 def test_max_iters_and_errors(serve):
     S = serve
     from ops_amd import ops
