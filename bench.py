@@ -58,12 +58,26 @@ def kernel_table(ops, bpp_map, pts):
     return out
 
 
-def roofline(kernels, bpp_map, pts):
+def profiled_traffic(profile, kernel):
+    """HBM bytes per launch measured with rocprofv3 PMC (FETCH_SIZE, WRITE_SIZE,
+    calibrated on a copy kernel) for the workload's own configuration, from the
+    summary kept under profiles/; None when not recorded."""
+    path = os.path.join(ROOT, "profiles", profile)
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        f = ln.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return float(f[2])
+    return None
+
+
+def roofline(kernels, bpp_map, pts, traffic=None):
     dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
     avg_s = kernels[dom]["avg_us"] * 1e-6
     achieved = bpp_map[dom] * pts / avg_s / 1e9
     return {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": bpp_map[dom] * pts,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_launch": bpp_map[dom] * pts,
             "avg_launch_us": round(avg_s * 1e6, 2)}
 
 
@@ -119,7 +133,10 @@ def bench_heat(a, ops, apps):
                {"workload": f"heat3d_cg_{n}", "grid": [n, n, n], "operator": "7-point variable coefficient",
                 "step": "one CG iteration (4 ops_par_loops, 2 FP64 reductions)", "bytes_per_point": 112,
                 "relres_after_steps": rr_rel},
-               roofline=roofline(kernels, BYTES_PER_PT, pts), kernels=kernels)
+               roofline=roofline(kernels, BYTES_PER_PT, pts,
+                                 profiled_traffic("heat3d_cg_512_hbm_traffic.md",
+                                                  "tea_leaf_cg_calc_w_reduce_kernel") if n == 512 else None),
+               kernels=kernels)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_heat(n, a.seed)
     return out

@@ -1,0 +1,265 @@
+// z-marching stencil columns with V points per lane (16-byte vector loads).
+//
+// A 64 x 4 workgroup owns a (64 V) x 4 column of the grid over planes
+// [z0, z1); lane l of row r holds points x0 + V l ... x0 + V l + V - 1 of row
+// y0 + r.  Each thread keeps its points' p(z-1), p(z), p(z+1) (and, for the
+// heat operator, Kz(z), Kz(z+1)) in registers, so every value is loaded from
+// memory once per plane, as one 16-byte load per lane.  The plane's p (and
+// Kx, Ky) go to LDS with the rim the stencil needs, double-buffered with one
+// barrier per plane; x neighbours inside a lane come from registers, the
+// two at a lane's ends and the y neighbours from LDS.  Next-plane loads are
+// issued before the barrier.  Outputs are written with non-temporal stores
+// when NT (the next loop reads them from HBM anyway at the sizes this path
+// is for).  Lanes past the range end keep to the barriers; a lane with fewer
+// than V points in range loads and stores them one by one, and the lane
+// holding the last point loads the far x rim.
+//
+// Requirements (checked by the launchers): every array's point (x0, y, z)
+// is 16-byte aligned for all rows and planes (row and plane pitches are
+// multiples of 16 bytes and x0 is a multiple of V).
+#pragma once
+
+#include "engine.h"
+
+namespace opsmi {
+
+// LDS row slice (16-byte aligned when n == V)
+template <class T, int V>
+__device__ __forceinline__ void lds_st(T *p, int n, const T (&v)[V]) {
+  if (n >= V) {
+    vec_t<T, V> q;
+#pragma unroll
+    for (int j = 0; j < V; j++) q[j] = v[j];
+    *reinterpret_cast<vec_t<T, V> *>(p) = q;
+  } else {
+#pragma unroll
+    for (int j = 0; j < V; j++)
+      if (j < n) p[j] = v[j];
+  }
+}
+template <class T, int V>
+__device__ __forceinline__ void lds_ld(const T *p, T (&o)[V]) {
+  const vec_t<T, V> q = *reinterpret_cast<const vec_t<T, V> *>(p);
+#pragma unroll
+  for (int j = 0; j < V; j++) o[j] = q[j];
+}
+
+// lane geometry of a column tile
+struct ColLane {
+  int n;     // points of this lane in range (0..V)
+  int cnt;   // points of the tile row in range
+  int lye;   // last row in range
+  bool hxl, hxh, hyl, hyh;
+};
+template <int V>
+__device__ __forceinline__ ColLane col_lane(int x0, int y0, int xe, int ye) {
+  ColLane c;
+  const int lx = (int)threadIdx.x, ly = (int)threadIdx.y;
+  c.cnt = min(kWave * V, xe - x0);
+  c.lye = min(kWavesPerBlock, ye - y0) - 1;
+  c.n = ly <= c.lye ? max(0, min(V, c.cnt - V * lx)) : 0;
+  c.hxl = lx == 0;
+  c.hxh = lx == (c.cnt - 1) / V;
+  c.hyl = ly == 0;
+  c.hyh = ly == c.lye;
+  return c;
+}
+
+// ------------------------------------------------------------ heat (7-point, variable coefficient, double)
+template <int V>
+struct HeatLdsV {
+  __attribute__((aligned(16))) double sp[2][kWavesPerBlock + 2][kWave * V + 2 * V];
+  __attribute__((aligned(16))) double skx[2][kWavesPerBlock][kWave * V + 2 * V];
+  __attribute__((aligned(16))) double sky[2][kWavesPerBlock + 1][kWave * V];
+};
+
+// TeaLeaf 3D operator (apps/c/TeaLeaf/tea_leaf_cg_kernels.h:8-21 with the z
+// pair after the x pair), arithmetic in the reference order.  INIT: w = A p,
+// r = p - w; otherwise w = A p and the return value is this thread's sum of
+// w p.
+template <int V, bool INIT, bool NT>
+__device__ __forceinline__ double heat3_column_v(HeatLdsV<V> &L, const Ptr<double> &w, const Ptr<double> &r,
+                                                 const Ptr<const double> &Kx, const Ptr<const double> &Ky,
+                                                 const Ptr<const double> &Kz, const Ptr<const double> &p, double rx,
+                                                 double ry, double rz, int x0, int y0, int xe, int ye, int z0,
+                                                 int z1) {
+  double acc = 0.0;
+  const int lx = (int)threadIdx.x, ly = (int)threadIdx.y;
+  const ColLane c = col_lane<V>(x0, y0, xe, ye);
+  const int n = c.n;
+  const bool act = n > 0;
+  const int x = x0 + V * lx, y = y0 + ly;
+  const int o = V + V * lx;  // LDS index of this lane's first point
+  double pm[V], pc[V], pn[V], kzc[V], kzn[V], kxc[V], kyc[V];
+  double ryl[V], ryh[V], rky[V];
+  double rxl = 0, rxh = 0, rkx = 0;
+#pragma unroll
+  for (int j = 0; j < V; j++) pm[j] = pc[j] = pn[j] = kzc[j] = kzn[j] = kxc[j] = kyc[j] = ryl[j] = ryh[j] = rky[j] = 0;
+  if (act) {
+    ldv<double, V>(p.at(x, y, z0 - 1), n, pm);
+    ldv<double, V>(p.at(x, y, z0), n, pc);
+    ldv<double, V>(p.at(x, y, z0 + 1), n, pn);
+    ldv<double, V>(Kz.at(x, y, z0), n, kzc);
+    ldv<double, V>(Kz.at(x, y, z0 + 1), n, kzn);
+    ldv<double, V>(Kx.at(x, y, z0), n, kxc);
+    ldv<double, V>(Ky.at(x, y, z0), n, kyc);
+    if (c.hxl) rxl = p(x0 - 1, y, z0);
+    if (c.hxh) {
+      rxh = p(x0 + c.cnt, y, z0);
+      rkx = Kx(x0 + c.cnt, y, z0);
+    }
+    if (c.hyl) ldv<double, V>(p.at(x, y - 1, z0), n, ryl);
+    if (c.hyh) {
+      ldv<double, V>(p.at(x, y + 1, z0), n, ryh);
+      ldv<double, V>(Ky.at(x, y + 1, z0), n, rky);
+    }
+  }
+  for (int z = z0; z < z1; z++) {
+    const int b = (z - z0) & 1;
+    if (act) {
+      lds_st<double, V>(&L.sp[b][ly + 1][o], n, pc);
+      lds_st<double, V>(&L.skx[b][ly][o], n, kxc);
+      lds_st<double, V>(&L.sky[b][ly][V * lx], n, kyc);
+      if (c.hxl) L.sp[b][ly + 1][V - 1] = rxl;
+      if (c.hxh) {
+        L.sp[b][ly + 1][V + c.cnt] = rxh;
+        L.skx[b][ly][V + c.cnt] = rkx;
+      }
+      if (c.hyl) lds_st<double, V>(&L.sp[b][0][o], n, ryl);
+      if (c.hyh) {
+        lds_st<double, V>(&L.sp[b][ly + 2][o], n, ryh);
+        lds_st<double, V>(&L.sky[b][ly + 1][V * lx], n, rky);
+      }
+    }
+    double pnn[V], kznn[V], kxn_c[V], kyn_c[V];
+#pragma unroll
+    for (int j = 0; j < V; j++) pnn[j] = kznn[j] = kxn_c[j] = kyn_c[j] = 0;
+    if (act && z + 1 < z1) {
+      const int zn = z + 1;
+      ldv<double, V>(p.at(x, y, zn + 1), n, pnn);
+      ldv<double, V>(Kz.at(x, y, zn + 1), n, kznn);
+      ldv<double, V>(Kx.at(x, y, zn), n, kxn_c);
+      ldv<double, V>(Ky.at(x, y, zn), n, kyn_c);
+      if (c.hxl) rxl = p(x0 - 1, y, zn);
+      if (c.hxh) {
+        rxh = p(x0 + c.cnt, y, zn);
+        rkx = Kx(x0 + c.cnt, y, zn);
+      }
+      if (c.hyl) ldv<double, V>(p.at(x, y - 1, zn), n, ryl);
+      if (c.hyh) {
+        ldv<double, V>(p.at(x, y + 1, zn), n, ryh);
+        ldv<double, V>(Ky.at(x, y + 1, zn), n, rky);
+      }
+    }
+    __syncthreads();
+    if (act) {
+      double pym[V], pyp[V], kyn[V], v[V], rv[V];
+      lds_ld<double, V>(&L.sp[b][ly][o], pym);
+      lds_ld<double, V>(&L.sp[b][ly + 2][o], pyp);
+      lds_ld<double, V>(&L.sky[b][ly + 1][V * lx], kyn);
+      const double left = L.sp[b][ly + 1][o - 1];
+#pragma unroll
+      for (int j = 0; j < V; j++) {
+        const double pxm = j == 0 ? left : pc[j == 0 ? 0 : j - 1];
+        const double pxp = j + 1 < n ? pc[j + 1 < V ? j + 1 : j] : L.sp[b][ly + 1][o + j + 1];
+        const double kxn = j + 1 < n ? kxc[j + 1 < V ? j + 1 : j] : L.skx[b][ly][o + j + 1];
+        v[j] = (1.0 + ry * (kyn[j] + kyc[j]) + rx * (kxn + kxc[j]) + rz * (kzn[j] + kzc[j])) * pc[j] -
+               ry * (kyn[j] * pyp[j] + kyc[j] * pym[j]) - rx * (kxn * pxp + kxc[j] * pxm) -
+               rz * (kzn[j] * pn[j] + kzc[j] * pm[j]);
+        rv[j] = pc[j] - v[j];
+        if (!INIT && j < n) acc = acc + v[j] * pc[j];
+      }
+      stv<double, V, NT>(w.at(x, y, z), n, v);
+      if constexpr (INIT) stv<double, V, NT>(r.at(x, y, z), n, rv);
+    }
+#pragma unroll
+    for (int j = 0; j < V; j++) {
+      pm[j] = pc[j];
+      pc[j] = pn[j];
+      pn[j] = pnn[j];
+      kzc[j] = kzn[j];
+      kzn[j] = kznn[j];
+      kxc[j] = kxn_c[j];
+      kyc[j] = kyn_c[j];
+    }
+  }
+  return acc;
+}
+
+// ------------------------------------------------------------ Poisson (7-point, constant coefficient)
+template <class T, int V>
+struct PoisLdsV {
+  __attribute__((aligned(16))) T sp[2][kWavesPerBlock + 2][kWave * V + 2 * V];
+};
+
+// w = 6p - (p(-1,0,0) + p(1,0,0)) - (p(0,-1,0) + p(0,1,0)) - (p(0,0,-1) +
+// p(0,0,1)) in T; returns this thread's sum of (double) w (double) p.
+template <class T, int V, bool NT>
+__device__ __forceinline__ double poisson3_column_v(PoisLdsV<T, V> &L, const Ptr<T> &w, const Ptr<const T> &p, int x0,
+                                                    int y0, int xe, int ye, int z0, int z1) {
+  double acc = 0.0;
+  const int lx = (int)threadIdx.x, ly = (int)threadIdx.y;
+  const ColLane c = col_lane<V>(x0, y0, xe, ye);
+  const int n = c.n;
+  const bool act = n > 0;
+  const int x = x0 + V * lx, y = y0 + ly;
+  const int o = V + V * lx;
+  T pm[V], pc[V], pn[V], ryl[V], ryh[V];
+  T rxl = 0, rxh = 0;
+#pragma unroll
+  for (int j = 0; j < V; j++) pm[j] = pc[j] = pn[j] = ryl[j] = ryh[j] = 0;
+  if (act) {
+    ldv<T, V>(p.at(x, y, z0 - 1), n, pm);
+    ldv<T, V>(p.at(x, y, z0), n, pc);
+    ldv<T, V>(p.at(x, y, z0 + 1), n, pn);
+    if (c.hxl) rxl = p(x0 - 1, y, z0);
+    if (c.hxh) rxh = p(x0 + c.cnt, y, z0);
+    if (c.hyl) ldv<T, V>(p.at(x, y - 1, z0), n, ryl);
+    if (c.hyh) ldv<T, V>(p.at(x, y + 1, z0), n, ryh);
+  }
+  for (int z = z0; z < z1; z++) {
+    const int b = (z - z0) & 1;
+    if (act) {
+      lds_st<T, V>(&L.sp[b][ly + 1][o], n, pc);
+      if (c.hxl) L.sp[b][ly + 1][V - 1] = rxl;
+      if (c.hxh) L.sp[b][ly + 1][V + c.cnt] = rxh;
+      if (c.hyl) lds_st<T, V>(&L.sp[b][0][o], n, ryl);
+      if (c.hyh) lds_st<T, V>(&L.sp[b][ly + 2][o], n, ryh);
+    }
+    T pnn[V];
+#pragma unroll
+    for (int j = 0; j < V; j++) pnn[j] = 0;
+    if (act && z + 1 < z1) {
+      const int zn = z + 1;
+      ldv<T, V>(p.at(x, y, zn + 1), n, pnn);
+      if (c.hxl) rxl = p(x0 - 1, y, zn);
+      if (c.hxh) rxh = p(x0 + c.cnt, y, zn);
+      if (c.hyl) ldv<T, V>(p.at(x, y - 1, zn), n, ryl);
+      if (c.hyh) ldv<T, V>(p.at(x, y + 1, zn), n, ryh);
+    }
+    __syncthreads();
+    if (act) {
+      T pym[V], pyp[V], v[V];
+      lds_ld<T, V>(&L.sp[b][ly][o], pym);
+      lds_ld<T, V>(&L.sp[b][ly + 2][o], pyp);
+      const T left = L.sp[b][ly + 1][o - 1];
+#pragma unroll
+      for (int j = 0; j < V; j++) {
+        const T pxm = j == 0 ? left : pc[j == 0 ? 0 : j - 1];
+        const T pxp = j + 1 < n ? pc[j + 1 < V ? j + 1 : j] : L.sp[b][ly + 1][o + j + 1];
+        v[j] = (T)6.0 * pc[j] - (pxm + pxp) - (pym[j] + pyp[j]) - (pm[j] + pn[j]);
+        if (j < n) acc = acc + (double)v[j] * (double)pc[j];
+      }
+      stv<T, V, NT>(w.at(x, y, z), n, v);
+    }
+#pragma unroll
+    for (int j = 0; j < V; j++) {
+      pm[j] = pc[j];
+      pc[j] = pn[j];
+      pn[j] = pnn[j];
+    }
+  }
+  return acc;
+}
+
+}  // namespace opsmi

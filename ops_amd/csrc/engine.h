@@ -255,8 +255,60 @@ __device__ __forceinline__ TileId tile_id() {
   return t;
 }
 
+// ------------------------------------------------------------ vector access
+template <class T, int V>
+using vec_t = T __attribute__((ext_vector_type(V)));
+
+// host: 16-byte loads are possible at (x0, y, z) of every row and plane
+inline bool vec16_ok(ops_dat d, int x0) {
+  if (!d) return false;
+  const DatView v = view_of(d);
+  const long es = d->type_size;
+  return (uintptr_t)(v.base + x0 * es) % 16 == 0 && (v.sy * es) % 16 == 0 && (v.sz * es) % 16 == 0;
+}
+// host: OPSMI_VEC=0 keeps the one-point-per-lane kernels
+inline bool use_vec() {
+  const char *e = getenv("OPSMI_VEC");
+  return !(e && e[0] == '0');
+}
+
+// n <= V consecutive values from p (one vector load when all V are valid)
+template <class T, int V>
+__device__ __forceinline__ void ldv(const T *p, int n, T (&o)[V]) {
+  if (n >= V) {
+    const vec_t<T, V> q = *reinterpret_cast<const vec_t<T, V> *>(p);
+#pragma unroll
+    for (int j = 0; j < V; j++) o[j] = q[j];
+  } else {
+#pragma unroll
+    for (int j = 0; j < V; j++) o[j] = j < n ? p[j] : T(0);
+  }
+}
+
+// n <= V consecutive values to p; NT: non-temporal (streaming) stores
+template <class T, int V, bool NT>
+__device__ __forceinline__ void stv(T *p, int n, const T (&v)[V]) {
+  if (n >= V) {
+    vec_t<T, V> q;
+#pragma unroll
+    for (int j = 0; j < V; j++) q[j] = v[j];
+    if constexpr (NT) __builtin_nontemporal_store(q, reinterpret_cast<vec_t<T, V> *>(p));
+    else *reinterpret_cast<vec_t<T, V> *>(p) = q;
+  } else {
+#pragma unroll
+    for (int j = 0; j < V; j++)
+      if (j < n) {
+        if constexpr (NT) __builtin_nontemporal_store(v[j], p + j);
+        else p[j] = v[j];
+      }
+  }
+}
+
 // F provides: using red_t; static constexpr int red_op, red_dim;
 //   __device__ void operator()(int x, int y, int z, red_t (&acc)[red_dim or 1]) const
+// and optionally, for the vector engine (k_loop_v), kVec > 1 and
+//   template <int V> __device__ void vop(int x, int y, int z, int n, red_t (&acc)[...]) const
+// which handles the n <= V consecutive points x .. x + n - 1.
 template <class F>
 __global__ __launch_bounds__(256) void k_loop(Range3 r, F f, RedCtl ctl, int chunk, int three_d) {
   using R = typename F::red_t;
@@ -283,6 +335,46 @@ __global__ __launch_bounds__(256) void k_loop(Range3 r, F f, RedCtl ctl, int chu
   }
   if constexpr (F::red_dim > 0) reduce_block_store<R, OP, D>(acc, ctl);
 }
+
+// Vector form of k_loop: each lane takes V consecutive x points (one 16-byte
+// load per array when the functor's element is 16 / V bytes), so a wave
+// covers 64 V points of a row.
+template <class F, int V>
+__global__ __launch_bounds__(256) void k_loop_v(Range3 r, F f, RedCtl ctl, int chunk, int three_d) {
+  using R = typename F::red_t;
+  constexpr int D = F::red_dim > 0 ? F::red_dim : 1;
+  constexpr int OP = F::red_op;
+  R acc[D];
+#pragma unroll
+  for (int q = 0; q < D; q++) acc[q] = RedOp<R, OP>::identity();
+  const TileId t = tile_id();
+  const int x = r.x0 + (int)t.x * kWave * V + V * (int)threadIdx.x;
+  const int n = min(V, r.x1 - x);
+  if (n > 0) {
+    if (three_d) {
+      const int y = r.y0 + (int)t.y * kWavesPerBlock + (int)threadIdx.y;
+      if (y < r.y1) {
+        const int z0 = r.z0 + (int)t.z * chunk;
+        const int z1 = min(z0 + chunk, r.z1);
+        for (int z = z0; z < z1; z++) f.template vop<V>(x, y, z, n, acc);
+      }
+    } else {
+      const int y0 = r.y0 + (int)t.y * kWavesPerBlock * chunk;
+      const int y1 = min(y0 + kWavesPerBlock * chunk, r.y1);
+      for (int y = y0 + (int)threadIdx.y; y < y1; y += kWavesPerBlock) f.template vop<V>(x, y, r.z0, n, acc);
+    }
+  }
+  if constexpr (F::red_dim > 0) reduce_block_store<R, OP, D>(acc, ctl);
+}
+
+template <class F, class = void>
+struct vec_width {
+  static constexpr int value = 1;
+};
+template <class F>
+struct vec_width<F, std::void_t<decltype(F::kVec)>> {
+  static constexpr int value = F::kVec;
+};
 
 template <class T>
 constexpr DType dtype_v() {
@@ -314,14 +406,27 @@ LoopRec make_loop(const char *name, const ops_arg *args, int nargs, const Range3
     rec.launch = [](hipStream_t) {};
     return rec;
   }
-  const Tiling t = make_tiling(r, 1);
+  Tiling t = make_tiling(r, 1);
+  // vector engine when the functor has one and every dat allows 16-byte
+  // loads at the range's x start
+  constexpr int V = vec_width<F>::value;
+  bool vec = V > 1 && use_vec();
+  for (int i = 0; i < nargs && vec; i++)
+    if (args[i].argtype == OPS_ARG_DAT) vec = vec16_ok(args[i].dat, r.x0);
+  if (vec) t.grid.x = (unsigned)((r.x1 - r.x0 + kWave * V - 1) / (kWave * V));
   RedCtl ctl{};
   if constexpr (F::red_dim > 0) ctl = red_prepare(*red_arg, F::red_dim, dtype_v<typename F::red_t>(), t.blocks());
   rec.ctl = ctl;
   rec.launch = [=](hipStream_t s) {
     const F f = make();
     const RedCtl c = with_scratch(ctl);
-    hipLaunchKernelGGL(k_loop<F>, t.grid, dim3(kWave, kWavesPerBlock), 0, s, r, f, c, t.chunk, t.three_d ? 1 : 0);
+    if (vec) {
+      if constexpr (V > 1)
+        hipLaunchKernelGGL((k_loop_v<F, V>), t.grid, dim3(kWave, kWavesPerBlock), 0, s, r, f, c, t.chunk,
+                           t.three_d ? 1 : 0);
+    } else {
+      hipLaunchKernelGGL(k_loop<F>, t.grid, dim3(kWave, kWavesPerBlock), 0, s, r, f, c, t.chunk, t.three_d ? 1 : 0);
+    }
     if constexpr (F::red_dim > 0) launch_red_finish<typename F::red_t, F::red_op, F::red_dim>(c, t.blocks(), s);
   };
   return rec;

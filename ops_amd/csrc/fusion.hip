@@ -153,7 +153,9 @@ int cg_chain_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
 // Pointwise pair: the deferred axpy and the axpby that overwrites its p,
 //     u = u + alpha p_old ;  p = a p_old + b r
 // in one pass: reads u, p, r and writes u, p (40 bytes per point instead of
-// 24 + 24).  Pointwise, so p is updated in place.
+// 24 + 24).  Pointwise, so p is updated in place.  One point per lane: at
+// 512^3 it measured 955 us against 983 us (16-byte loads) and 990 us
+// (16-byte loads and non-temporal stores).
 struct PairF {
   OPSMI_NO_RED
   Ptr<double> u, p;

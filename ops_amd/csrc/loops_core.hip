@@ -4,6 +4,7 @@
 // kernel it names, in the same association order (compiled with FP
 // contraction off), so results match the reference sequential backend bit
 // for bit; sums differ only by reduction order.
+#include "column.h"
 #include "engine.h"
 #include "heat_tile.h"
 
@@ -22,9 +23,16 @@ struct SetZero {
 template <class T>
 struct Copy {
   OPSMI_NO_RED
+  static constexpr int kVec = 16 / (int)sizeof(T);
   Ptr<T> a;
   Ptr<const T> b;
   __device__ void operator()(int x, int y, int z, double (&)[1]) const { a(x, y, z) = b(x, y, z); }
+  template <int V>
+  __device__ void vop(int x, int y, int z, int n, double (&)[1]) const {
+    T v[V];
+    ldv<T, V>(b.at(x, y, z), n, v);
+    stv<T, V, true>(a.at(x, y, z), n, v);
+  }
 };
 // mode 0 left: sin(pi (y+1)/(jmax+1)); 1 right: same * exp(-pi);
 // 2 top: sin(pi (x+1)/(imax+1))
@@ -94,18 +102,38 @@ struct Norm2 {
 };
 struct Axpy {
   OPSMI_NO_RED
+  static constexpr int kVec = 2;
   Ptr<double> u;
   Ptr<const double> p;
   double alpha;
   __device__ void operator()(int x, int y, int z, double (&)[1]) const { u(x, y, z) = u(x, y, z) + alpha * p(x, y, z); }
+  template <int V>
+  __device__ void vop(int x, int y, int z, int n, double (&)[1]) const {
+    double a[V], b[V];
+    ldv<double, V>(u.at(x, y, z), n, a);
+    ldv<double, V>(p.at(x, y, z), n, b);
+#pragma unroll
+    for (int j = 0; j < V; j++) a[j] = a[j] + alpha * b[j];
+    stv<double, V, true>(u.at(x, y, z), n, a);
+  }
 };
 struct Axpby {
   OPSMI_NO_RED
+  static constexpr int kVec = 2;
   Ptr<double> u;
   Ptr<const double> p;
   double alpha, beta;
   __device__ void operator()(int x, int y, int z, double (&)[1]) const {
     u(x, y, z) = alpha * u(x, y, z) + beta * p(x, y, z);
+  }
+  template <int V>
+  __device__ void vop(int x, int y, int z, int n, double (&)[1]) const {
+    double a[V], b[V];
+    ldv<double, V>(u.at(x, y, z), n, a);
+    ldv<double, V>(p.at(x, y, z), n, b);
+#pragma unroll
+    for (int j = 0; j < V; j++) a[j] = alpha * a[j] + beta * b[j];
+    stv<double, V, true>(u.at(x, y, z), n, a);
   }
 };
 struct Zeqxty {
@@ -186,9 +214,27 @@ __global__ __launch_bounds__(256) void k_heat3_march(Ptr<double> w, Ptr<double> 
   if constexpr (!INIT) reduce_block_store<double, OPS_INC, 1>(acc, ctl);
 }
 
+// Same with two points per lane (16-byte loads, 128 x 4 columns) and
+// non-temporal stores of w (and r): heat3_column_v.
+template <bool INIT>
+__global__ __launch_bounds__(256) void k_heat3_march_v(Ptr<double> w, Ptr<double> r, Ptr<const double> Kx,
+                                                       Ptr<const double> Ky, Ptr<const double> Kz,
+                                                       Ptr<const double> p, double rx, double ry, double rz, Range3 rg,
+                                                       int chunk, RedCtl ctl) {
+  __shared__ HeatLdsV<2> lds;
+  const TileId t = tile_id();
+  const int z0 = rg.z0 + (int)t.z * chunk;
+  double acc[1] = {heat3_column_v<2, INIT, true>(lds, w, r, Kx, Ky, Kz, p, rx, ry, rz, rg.x0 + (int)t.x * 2 * kWave,
+                                                 rg.y0 + (int)t.y * kWavesPerBlock, rg.x1, rg.y1, z0,
+                                                 min(z0 + chunk, rg.z1))};
+  if constexpr (!INIT) reduce_block_store<double, OPS_INC, 1>(acc, ctl);
+}
+
 // Queue record for the z-marching 3D heat kernel (16 z planes per workgroup
 // by default, measured best among 1-16 on 512^3; OPSMI_HEAT_ZCHUNK
 // overrides).  INIT args: w, r, Kx, Ky, Kz, u; otherwise w, Kx, Ky, Kz, p.
+// The two-points-per-lane form runs when every array allows 16-byte loads
+// at the range's x start (OPSMI_VEC=0 keeps one point per lane).
 inline int heat_zchunk() { return heat_zchunk_default(); }
 
 template <bool INIT>
@@ -206,7 +252,11 @@ LoopRec make_heat3_loop(const char *name, const ops_arg *args, int nargs, const 
     return rec;
   }
   const int nx = r.x1 - r.x0, ny = r.y1 - r.y0, nz = r.z1 - r.z0;
-  const unsigned gx = (nx + kWave - 1) / kWave, gy = (ny + kWavesPerBlock - 1) / kWavesPerBlock;
+  bool vec = use_vec();
+  for (ops_dat d : {w.dat, Kx.dat, Ky.dat, Kz.dat, p.dat}) vec = vec && vec16_ok(d, r.x0);
+  if (INIT) vec = vec && vec16_ok(rr.dat, r.x0);
+  const int tile_x = vec ? 2 * kWave : kWave;
+  const unsigned gx = (nx + tile_x - 1) / tile_x, gy = (ny + kWavesPerBlock - 1) / kWavesPerBlock;
   const int chunk = heat_zchunk();
   const unsigned gz = (nz + chunk - 1) / chunk;
   RedCtl ctl{};
@@ -215,9 +265,10 @@ LoopRec make_heat3_loop(const char *name, const ops_arg *args, int nargs, const 
   rec.launch = [=](hipStream_t s) {
     const RedCtl c = with_scratch(ctl);
     Ptr<double> rp = INIT ? ptr_of<double>(rr.dat) : Ptr<double>{nullptr, 0, 0};
-    hipLaunchKernelGGL(k_heat3_march<INIT>, dim3(gx, gy, gz), dim3(kWave, kWavesPerBlock), 0, s,
-                       ptr_of<double>(w.dat), rp, ptr_of<const double>(Kx.dat), ptr_of<const double>(Ky.dat),
-                       ptr_of<const double>(Kz.dat), ptr_of<const double>(p.dat), rx, ry, rz, r, chunk, c);
+    hipLaunchKernelGGL(vec ? k_heat3_march_v<INIT> : k_heat3_march<INIT>, dim3(gx, gy, gz),
+                       dim3(kWave, kWavesPerBlock), 0, s, ptr_of<double>(w.dat), rp, ptr_of<const double>(Kx.dat),
+                       ptr_of<const double>(Ky.dat), ptr_of<const double>(Kz.dat), ptr_of<const double>(p.dat), rx,
+                       ry, rz, r, chunk, c);
     if (!INIT) launch_red_finish<double, OPS_INC, 1>(c, gx * gy * gz, s);
   };
   return rec;
@@ -232,6 +283,7 @@ struct CalcUrR {
   using red_t = double;
   static constexpr int red_op = OPS_INC;
   static constexpr int red_dim = 1;
+  static constexpr int kVec = 2;
   Ptr<double> r;
   Ptr<const double> w;
   double alpha;
@@ -239,6 +291,18 @@ struct CalcUrR {
     const double v = r(x, y, z) - alpha * w(x, y, z);
     r(x, y, z) = v;
     s[0] = s[0] + v * v;
+  }
+  template <int V>
+  __device__ void vop(int x, int y, int z, int n, double (&s)[1]) const {
+    double a[V], b[V];
+    ldv<double, V>(r.at(x, y, z), n, a);
+    ldv<double, V>(w.at(x, y, z), n, b);
+#pragma unroll
+    for (int j = 0; j < V; j++) {
+      a[j] = a[j] - alpha * b[j];
+      if (j < n) s[0] = s[0] + a[j] * a[j];
+    }
+    stv<double, V, true>(r.at(x, y, z), n, a);
   }
 };
 
@@ -261,6 +325,49 @@ struct PoissonW {
     s[0] = s[0] + (double)v * (double)p(x, y, z);
   }
 };
+
+// 3D form as z-marching columns with 16 bytes per lane (2 doubles or 4
+// floats), p planes in LDS: poisson3_column_v, same arithmetic.
+template <class T>
+__global__ __launch_bounds__(256) void k_poisson3_march(Ptr<T> w, Ptr<const T> p, Range3 rg, int chunk, RedCtl ctl) {
+  constexpr int V = 16 / (int)sizeof(T);
+  __shared__ PoisLdsV<T, V> lds;
+  const TileId t = tile_id();
+  const int z0 = rg.z0 + (int)t.z * chunk;
+  double acc[1] = {poisson3_column_v<T, V, true>(lds, w, p, rg.x0 + (int)t.x * kWave * V,
+                                                 rg.y0 + (int)t.y * kWavesPerBlock, rg.x1, rg.y1, z0,
+                                                 min(z0 + chunk, rg.z1))};
+  reduce_block_store<double, OPS_INC, 1>(acc, ctl);
+}
+
+template <class T>
+LoopRec make_poisson3_loop(const char *name, const ops_arg *args, int nargs, const Range3 &r, ops_arg w, ops_arg p,
+                           const ops_arg *red) {
+  LoopRec rec;
+  rec.name = name;
+  rec.range = r;
+  rec.bytes = transfer_bytes(args, nargs, r);
+  deps_from_args(rec, args, nargs);
+  if (r.npts() <= 0) {
+    rec.bytes = 0.0;
+    rec.launch = [](hipStream_t) {};
+    return rec;
+  }
+  constexpr int V = 16 / (int)sizeof(T);
+  const int nx = r.x1 - r.x0, ny = r.y1 - r.y0, nz = r.z1 - r.z0;
+  const int chunk = heat_zchunk();
+  const unsigned gx = (nx + kWave * V - 1) / (kWave * V), gy = (ny + kWavesPerBlock - 1) / kWavesPerBlock;
+  const unsigned gz = (nz + chunk - 1) / chunk;
+  const RedCtl ctl = red_prepare(*red, 1, DT_F64, gx * gy * gz);
+  rec.ctl = ctl;
+  rec.launch = [=](hipStream_t s) {
+    const RedCtl c = with_scratch(ctl);
+    hipLaunchKernelGGL(k_poisson3_march<T>, dim3(gx, gy, gz), dim3(kWave, kWavesPerBlock), 0, s, ptr_of<T>(w.dat),
+                       ptr_of<const T>(p.dat), r, chunk, c);
+    launch_red_finish<double, OPS_INC, 1>(c, gx * gy * gz, s);
+  };
+  return rec;
+}
 
 // ------------------------------------------------------------ ADI preproc
 // apps/c/adi/preproc_kernel.h:4-39
@@ -668,7 +775,11 @@ void ops_par_loop_poisson_calc_w_reduce_kernel(const char *name, ops_block block
   Range3 r = check_loop(name, block, dim, range, args,
                         {dat_w(), dat_r(dim == 3 ? pts_7pt() : pts_5pt()), red_a(OPS_INC, DT_F64)}, &dt);
   if (dim < 2) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": needs a 2D or 3D block"));
-  if (dt == DT_F64) {
+  const bool col = dim == 3 && use_march() && use_vec() && vec16_ok(w.dat, r.x0) && vec16_ok(p.dat, r.x0);
+  if (col) {
+    if (dt == DT_F64) enqueue(make_poisson3_loop<double>(name, args, nargs, r, w, p, &pw));
+    else enqueue(make_poisson3_loop<float>(name, args, nargs, r, w, p, &pw));
+  } else if (dt == DT_F64) {
     if (dim == 2) LAUNCH_RED(r, pw, PoissonW<double, 2>{PD(w), PCD(p)});
     else LAUNCH_RED(r, pw, PoissonW<double, 3>{PD(w), PCD(p)});
   } else {

@@ -48,6 +48,7 @@ struct MpInit {
   using red_t = double;
   static constexpr int red_op = OPS_INC;
   static constexpr int red_dim = 2;
+  static constexpr int kVec = 2;
   Ptr<float> e, rr, p;
   Ptr<const double> r;
   float mi;
@@ -60,12 +61,32 @@ struct MpInit {
     s[0] = s[0] + (double)q * (double)z;
     s[1] = s[1] + (double)q * (double)q;
   }
+  template <int V>
+  __device__ void vop(int i, int j, int k, int n, double (&s)[2]) const {
+    double rv[V];
+    float q[V], z[V], zero[V];
+    ldv<double, V>(r.at(i, j, k), n, rv);
+#pragma unroll
+    for (int c = 0; c < V; c++) {
+      q[c] = (float)rv[c];
+      z[c] = mi * q[c];
+      zero[c] = 0.0f;
+      if (c < n) {
+        s[0] = s[0] + (double)q[c] * (double)z[c];
+        s[1] = s[1] + (double)q[c] * (double)q[c];
+      }
+    }
+    stv<float, V, true>(e.at(i, j, k), n, zero);
+    stv<float, V, true>(rr.at(i, j, k), n, q);
+    stv<float, V, true>(p.at(i, j, k), n, z);
+  }
 };
 
 struct MpUpdate {
   using red_t = double;
   static constexpr int red_op = OPS_INC;
   static constexpr int red_dim = 2;
+  static constexpr int kVec = 4;
   Ptr<float> e, rr;
   Ptr<const float> p, w;
   float a, mi;
@@ -77,24 +98,65 @@ struct MpUpdate {
     s[0] = s[0] + (double)q * (double)z;
     s[1] = s[1] + (double)q * (double)q;
   }
+  template <int V>
+  __device__ void vop(int i, int j, int k, int n, double (&s)[2]) const {
+    float ev[V], rv[V], pv[V], wv[V];
+    ldv<float, V>(e.at(i, j, k), n, ev);
+    ldv<float, V>(rr.at(i, j, k), n, rv);
+    ldv<float, V>(p.at(i, j, k), n, pv);
+    ldv<float, V>(w.at(i, j, k), n, wv);
+#pragma unroll
+    for (int c = 0; c < V; c++) {
+      ev[c] = ev[c] + a * pv[c];
+      rv[c] = rv[c] - a * wv[c];
+      const float z = mi * rv[c];
+      if (c < n) {
+        s[0] = s[0] + (double)rv[c] * (double)z;
+        s[1] = s[1] + (double)rv[c] * (double)rv[c];
+      }
+    }
+    stv<float, V, true>(e.at(i, j, k), n, ev);
+    stv<float, V, true>(rr.at(i, j, k), n, rv);
+  }
 };
 
 struct MpP {
   OPSMI_NO_RED
+  static constexpr int kVec = 4;
   Ptr<float> p;
   Ptr<const float> rr;
   float mi, b;
   __device__ void operator()(int i, int j, int k, double (&)[1]) const {
     p(i, j, k) = mi * rr(i, j, k) + b * p(i, j, k);
   }
+  template <int V>
+  __device__ void vop(int i, int j, int k, int n, double (&)[1]) const {
+    float pv[V], rv[V];
+    ldv<float, V>(p.at(i, j, k), n, pv);
+    ldv<float, V>(rr.at(i, j, k), n, rv);
+#pragma unroll
+    for (int c = 0; c < V; c++) pv[c] = mi * rv[c] + b * pv[c];
+    stv<float, V, true>(p.at(i, j, k), n, pv);
+  }
 };
 
 struct MpCorrect {
   OPSMI_NO_RED
+  static constexpr int kVec = 2;
   Ptr<double> x;
   Ptr<const float> e;
   __device__ void operator()(int i, int j, int k, double (&)[1]) const {
     x(i, j, k) = x(i, j, k) + (double)e(i, j, k);
+  }
+  template <int V>
+  __device__ void vop(int i, int j, int k, int n, double (&)[1]) const {
+    double xv[V];
+    float ev[V];
+    ldv<double, V>(x.at(i, j, k), n, xv);
+    ldv<float, V>(e.at(i, j, k), n, ev);
+#pragma unroll
+    for (int c = 0; c < V; c++) xv[c] = xv[c] + (double)ev[c];
+    stv<double, V, true>(x.at(i, j, k), n, xv);
   }
 };
 
