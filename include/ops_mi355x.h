@@ -338,6 +338,23 @@ void ops_par_loop_mp_p_kernel(const char *name, ops_block block, int dim, int *r
 void ops_par_loop_mp_correct_kernel(const char *name, ops_block block, int dim, int *range, ops_arg x,
                                     ops_arg e);
 
+/* Multi-block Poisson test app kernels: apps/c/poisson/poisson_kernel.h:4-38
+ * (constants dx, dy from ops_decl_const; double dats, 2D)
+ *   populate     (gbl int dispx, gbl int dispy, idx, u W, f W, ref W)
+ *   initialguess (u W)
+ *   stencil      (u R 5-point, f R, u2 W)
+ *   update       (in R, out W)
+ *   error        (u R, ref R, err INC) */
+void ops_par_loop_poisson_kernel_populate(const char *name, ops_block block, int dim, int *range, ops_arg dispx,
+                                          ops_arg dispy, ops_arg idx, ops_arg u, ops_arg f, ops_arg ref);
+void ops_par_loop_poisson_kernel_initialguess(const char *name, ops_block block, int dim, int *range, ops_arg u);
+void ops_par_loop_poisson_kernel_stencil(const char *name, ops_block block, int dim, int *range, ops_arg u,
+                                         ops_arg f, ops_arg u2);
+void ops_par_loop_poisson_kernel_update(const char *name, ops_block block, int dim, int *range, ops_arg in,
+                                        ops_arg out);
+void ops_par_loop_poisson_kernel_error(const char *name, ops_block block, int dim, int *range, ops_arg u,
+                                       ops_arg ref, ops_arg err);
+
 /* ADI right-hand side and coefficients: apps/c/adi/preproc_kernel.h:4-39
  * (constants nx, ny, nz, lambda) */
 void ops_par_loop_preproc_kernel(const char *name, ops_block block, int dim, int *range, ops_arg u, ops_arg du,
@@ -376,8 +393,10 @@ void ops_mi_sync(void);
  * the name is unknown. */
 int ops_mi_loop_stats(const char *name, long *count, double *kernel_ms, double *bytes);
 void ops_mi_reset_stats(void);
-/* loop-chain fusion in the queue: 1 on (default), 0 off */
-void ops_mi_set_fusion(int on);
+/* loop-chain fusion in the queue: 0 off, 1 the default rules (CG axpy+axpby
+ * pair), 2 also the three-loop CG chain (axpy+axpby+calc_w, measured slower
+ * on MI355X at 512^3).  Initial level from OPSMI_FUSION (default 1). */
+void ops_mi_set_fusion(int level);
 
 #ifdef __cplusplus
 } /* extern "C" */

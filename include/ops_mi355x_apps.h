@@ -64,6 +64,12 @@ void opsapp_trid_solve(void *ctx, int solvedim, int inc);
 void opsapp_trid_fetch(void *ctx, double *d_out, double *u_out);
 void opsapp_trid_destroy(void *ctx);
 
+/* the reference's multi-block Poisson test app (apps/c/poisson/poisson.cpp)
+ * on an nx x ny grid split into ngx x ngy blocks joined by halo transfers;
+ * returns the total squared error (the app's printed "Total error"); u_out
+ * (may be NULL) gets the assembled interior, x fastest */
+double opsapp_mb_poisson(int nx, int ny, int ngx, int ngy, int iters, double *u_out);
+
 /* request stream on the batched engine (include/ops_mi355x_serve.h), on the
  * wall clock: request i (kind[i], n[i], seed[i]) arrives arrival_s[i]
  * seconds after the start (ascending) and is submitted once it has arrived;

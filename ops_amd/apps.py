@@ -43,6 +43,7 @@ def lib():
             "opsapp_trid_destroy": (None, [vp]),
             "opsapp_serve_stream": (cd, [ci, C.POINTER(ci), C.POINTER(ci), C.POINTER(u64), _d, cd, C.c_size_t, ci,
                                          ci, ci, _d, C.POINTER(ci)]),
+            "opsapp_mb_poisson": (cd, [ci, ci, ci, ci, ci, _d]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -188,6 +189,14 @@ class Trid:
             lib().opsapp_trid_destroy(self.ctx)
             _chk()
             self.ctx = None
+
+
+def mb_poisson(nx=20, ny=20, ngx=1, ngy=1, iters=10, want_field=True):
+    """The reference's multi-block Poisson test app; returns (total error, u)."""
+    u = np.zeros((ny, nx)) if want_field else None
+    err = lib().opsapp_mb_poisson(nx, ny, ngx, ngy, iters, _p(u))
+    _chk()
+    return err, u
 
 
 def serve_stream(kinds, sizes, seeds, arrival_s, tol=1e-10, budget=200 * 10 ** 9, max_slots=256,

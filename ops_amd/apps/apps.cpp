@@ -583,6 +583,126 @@ void opsapp_trid_destroy(void *ctx) {
   delete T;
 }
 
+// ---------------------------------------------------------------- multi-block Poisson
+// The reference's multi-block test app (apps/c/poisson/poisson.cpp:56-307)
+// with its ngrid_x x ngrid_y partition exposed: blocks of uniform size
+// ((n-1)/ngrid + 1, the last ones shorter), dats u, u2, f, ref with depth-1
+// halos, halos between neighbouring blocks exchanged each iteration by
+// ops_halo_transfer, then Jacobi sweeps (poisson_kernel_stencil) and copies.
+// Returns the total squared error against the exact solution; u_out (may
+// be NULL) gets the whole interior, x fastest.
+double opsapp_mb_poisson(int nx, int ny, int ngx, int ngy, int iters, double *u_out) {
+  double dx = 0.01, dy = 0.01;
+  ops_decl_const_char(1, "double", sizeof(double), (char *)&dx, "dx");
+  ops_decl_const_char(1, "double", sizeof(double), (char *)&dy, "dy");
+  const int nb = ngx * ngy;
+  std::vector<ops_block> blocks(nb);
+  std::vector<ops_dat> u(nb), u2(nb), f(nb), ref(nb);
+  std::vector<int> sizes(2 * nb), disps(2 * nb);
+  int s00[] = {0, 0}, s5[] = {0, 0, 1, 0, -1, 0, 0, 1, 0, -1};
+  ops_stencil S00 = ops_decl_stencil(2, 1, s00, "00");
+  ops_stencil S5 = ops_decl_stencil(2, 5, s5, "00:10:-10:01:0-1");
+  ops_reduction red_err = ops_decl_reduction_handle(sizeof(double), "double", "err");
+  const int usz[2] = {(nx - 1) / ngx + 1, (ny - 1) / ngy + 1};
+  int dm[2] = {-1, -1}, dp[2] = {1, 1}, base[2] = {0, 0}, stride[2] = {1, 1};
+  for (int j = 0; j < ngy; j++)
+    for (int i = 0; i < ngx; i++) {
+      const int b = i + ngx * j;
+      blocks[b] = ops_decl_block(2, "block");
+      int size[2] = {usz[0], usz[1]};
+      if ((i + 1) * size[0] > nx) size[0] = nx - i * size[0];
+      if ((j + 1) * size[1] > ny) size[1] = ny - j * size[1];
+      u[b] = ops_decl_dat_char(blocks[b], 1, size, base, dm, dp, stride, nullptr, 8, "double", "u");
+      u2[b] = ops_decl_dat_char(blocks[b], 1, size, base, dm, dp, stride, nullptr, 8, "double", "u2");
+      f[b] = ops_decl_dat_char(blocks[b], 1, size, base, dm, dp, stride, nullptr, 8, "double", "f");
+      ref[b] = ops_decl_dat_char(blocks[b], 1, size, base, dm, dp, stride, nullptr, 8, "double", "ref");
+      sizes[2 * b] = size[0];
+      sizes[2 * b + 1] = size[1];
+      disps[2 * b] = i * usz[0];
+      disps[2 * b + 1] = j * usz[1];
+    }
+  std::vector<ops_halo> halos;
+  for (int j = 0; j < ngy; j++)
+    for (int i = 0; i < ngx; i++) {
+      const int b = i + ngx * j;
+      int dir[] = {1, 2};
+      if (i > 0) {
+        int iter[] = {1, sizes[2 * b + 1]}, from[] = {sizes[2 * (b - 1)] - 1, 0}, to[] = {-1, 0};
+        halos.push_back(ops_decl_halo(u[b - 1], u[b], iter, from, to, dir, dir));
+        from[0] = 0;
+        to[0] = sizes[2 * b];
+        halos.push_back(ops_decl_halo(u[b], u[b - 1], iter, from, to, dir, dir));
+      }
+      if (j > 0) {
+        int iter[] = {sizes[2 * b], 1}, from[] = {0, sizes[2 * (b - ngx) + 1] - 1}, to[] = {0, -1};
+        halos.push_back(ops_decl_halo(u[b - ngx], u[b], iter, from, to, dir, dir));
+        from[1] = 0;
+        to[1] = sizes[2 * b + 1];
+        halos.push_back(ops_decl_halo(u[b], u[b - ngx], iter, from, to, dir, dir));
+      }
+    }
+  ops_halo_group grp = halos.empty() ? nullptr : ops_decl_halo_group((int)halos.size(), halos.data());
+  for (int b = 0; b < nb; b++) {
+    int rng[] = {-1, sizes[2 * b] + 1, -1, sizes[2 * b + 1] + 1};
+    ops_par_loop_poisson_kernel_populate(
+        "poisson_kernel_populate", blocks[b], 2, rng, ops_arg_gbl_char((char *)&disps[2 * b], 1, 4, OPS_READ),
+        ops_arg_gbl_char((char *)&disps[2 * b + 1], 1, 4, OPS_READ), ops_arg_idx(),
+        ops_arg_dat(u[b], 1, S00, "double", OPS_WRITE), ops_arg_dat(f[b], 1, S00, "double", OPS_WRITE),
+        ops_arg_dat(ref[b], 1, S00, "double", OPS_WRITE));
+    ops_par_loop_poisson_kernel_update("poisson_kernel_update", blocks[b], 2, rng,
+                                       ops_arg_dat(u[b], 1, S00, "double", OPS_READ),
+                                       ops_arg_dat(u2[b], 1, S00, "double", OPS_WRITE));
+  }
+  for (int b = 0; b < nb; b++) {
+    int rng[] = {0, sizes[2 * b], 0, sizes[2 * b + 1]};
+    ops_par_loop_poisson_kernel_initialguess("poisson_kernel_initialguess", blocks[b], 2, rng,
+                                             ops_arg_dat(u[b], 1, S00, "double", OPS_WRITE));
+  }
+  for (int it = 0; it < iters; it++) {
+    if (grp) ops_halo_transfer(grp);
+    for (int b = 0; b < nb; b++) {
+      int rng[] = {0, sizes[2 * b], 0, sizes[2 * b + 1]};
+      ops_par_loop_poisson_kernel_stencil("poisson_kernel_stencil", blocks[b], 2, rng,
+                                          ops_arg_dat(u[b], 1, S5, "double", OPS_READ),
+                                          ops_arg_dat(f[b], 1, S00, "double", OPS_READ),
+                                          ops_arg_dat(u2[b], 1, S00, "double", OPS_WRITE));
+    }
+    for (int b = 0; b < nb; b++) {
+      int rng[] = {0, sizes[2 * b], 0, sizes[2 * b + 1]};
+      ops_par_loop_poisson_kernel_update("poisson_kernel_update", blocks[b], 2, rng,
+                                         ops_arg_dat(u2[b], 1, S00, "double", OPS_READ),
+                                         ops_arg_dat(u[b], 1, S00, "double", OPS_WRITE));
+    }
+  }
+  double err = 0.0;
+  for (int b = 0; b < nb; b++) {
+    int rng[] = {0, sizes[2 * b], 0, sizes[2 * b + 1]};
+    ops_par_loop_poisson_kernel_error("poisson_kernel_error", blocks[b], 2, rng,
+                                      ops_arg_dat(u[b], 1, S00, "double", OPS_READ),
+                                      ops_arg_dat(ref[b], 1, S00, "double", OPS_READ),
+                                      ops_arg_reduce(red_err, 1, "double", OPS_INC));
+  }
+  ops_reduction_result_char(red_err, sizeof(double), (char *)&err);
+  if (u_out) {
+    std::vector<double> tmp;
+    for (int b = 0; b < nb; b++) {
+      const int sx = sizes[2 * b], sy = sizes[2 * b + 1];
+      tmp.resize((size_t)sx * sy);
+      ops_dat_fetch_data(u[b], 0, (char *)tmp.data());
+      for (int y = 0; y < sy; y++)
+        memcpy(u_out + (size_t)(disps[2 * b + 1] + y) * nx + disps[2 * b], tmp.data() + (size_t)y * sx,
+               (size_t)sx * sizeof(double));
+    }
+  }
+  for (int b = 0; b < nb; b++) {
+    ops_free_dat(u[b]);
+    ops_free_dat(u2[b]);
+    ops_free_dat(f[b]);
+    ops_free_dat(ref[b]);
+  }
+  return err;
+}
+
 // ---------------------------------------------------------------- serving
 // Request stream on the batched engine, on the wall clock: request i arrives
 // arrival_s[i] seconds after the start (ascending), is submitted once it has

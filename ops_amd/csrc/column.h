@@ -186,6 +186,158 @@ __device__ __forceinline__ double heat3_column_v(HeatLdsV<V> &L, const Ptr<doubl
   return acc;
 }
 
+// ------------------------------------------------------------ fused CG chain
+// The TeaLeaf CG loops between two r.r results, as one column pass
+// (apps/c/TeaLeaf/tea_leaf_cg.cpp:129-214):
+//   [AXPY] u = u + alpha_u p_old          (tea_leaf_axpy_kernel, deferred)
+//          p_new = a p_old + b r          (tea_leaf_axpby_kernel, inside R only)
+//          w = A p_new ; return sum w p_new  (tea_leaf_cg_calc_w_reduce_kernel)
+// p_new is formed from p_old and r wherever the stencil needs it (the
+// column's planes and rims; p_old itself outside R, where the axpby does not
+// write) with the same operations as the separate loops, and written to a
+// second buffer (pnew) because neighbouring columns still read p_old.
+struct ChainPtrs {
+  Ptr<double> u, pnew, w;
+  Ptr<const double> pold, r, Kx, Ky, Kz;
+};
+
+template <int V, bool AXPY>
+__device__ __forceinline__ double heat3_chain_column_v(HeatLdsV<V> &L, const ChainPtrs &c, const Range3 &R,
+                                                       double alpha_u, double a, double b, double rx, double ry,
+                                                       double rz, int x0, int y0, int z0, int z1) {
+  double acc = 0.0;
+  const int lx = (int)threadIdx.x, ly = (int)threadIdx.y;
+  const ColLane cl = col_lane<V>(x0, y0, R.x1, R.y1);
+  const int n = cl.n;
+  const bool act = n > 0;
+  const int x = x0 + V * lx, y = y0 + ly;
+  const int o = V + V * lx;
+  // rims inside R take the axpby value
+  const bool in_xl = x0 - 1 >= R.x0, in_xh = x0 + cl.cnt < R.x1;
+  const bool in_yl = y - 1 >= R.y0, in_yh = y + 1 < R.y1;
+  auto pnew1 = [&](int xx, int yy, int zz, bool in) -> double {
+    const double po = c.pold(xx, yy, zz);
+    return in ? a * po + b * c.r(xx, yy, zz) : po;
+  };
+  // p_new of V points of row yy, plane zz into o_ (p_old into po)
+  auto pnewv = [&](int yy, int zz, bool in, double (&o_)[V], double (&po)[V]) {
+    double rv[V];
+    ldv<double, V>(c.pold.at(x, yy, zz), n, po);
+    if (in) {
+      ldv<double, V>(c.r.at(x, yy, zz), n, rv);
+#pragma unroll
+      for (int j = 0; j < V; j++) o_[j] = a * po[j] + b * rv[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < V; j++) o_[j] = po[j];
+    }
+  };
+  double pm[V], pc[V], pn[V], poc[V], pon[V], kzc[V], kzn[V], kxc[V], kyc[V], uc[V];
+  double ryl[V], ryh[V], rky[V], scratch[V];
+  double rxl = 0, rxh = 0, rkx = 0;
+#pragma unroll
+  for (int j = 0; j < V; j++)
+    pm[j] = pc[j] = pn[j] = poc[j] = pon[j] = kzc[j] = kzn[j] = kxc[j] = kyc[j] = uc[j] = ryl[j] = ryh[j] =
+        rky[j] = 0;
+  if (act) {
+    pnewv(y, z0 - 1, z0 - 1 >= R.z0, pm, scratch);
+    pnewv(y, z0, true, pc, poc);
+    pnewv(y, z0 + 1, z0 + 1 < R.z1, pn, pon);
+    ldv<double, V>(c.Kz.at(x, y, z0), n, kzc);
+    ldv<double, V>(c.Kz.at(x, y, z0 + 1), n, kzn);
+    ldv<double, V>(c.Kx.at(x, y, z0), n, kxc);
+    ldv<double, V>(c.Ky.at(x, y, z0), n, kyc);
+    if (AXPY) ldv<double, V>(c.u.at(x, y, z0), n, uc);
+    if (cl.hxl) rxl = pnew1(x0 - 1, y, z0, in_xl);
+    if (cl.hxh) {
+      rxh = pnew1(x0 + cl.cnt, y, z0, in_xh);
+      rkx = c.Kx(x0 + cl.cnt, y, z0);
+    }
+    if (cl.hyl) pnewv(y - 1, z0, in_yl, ryl, scratch);
+    if (cl.hyh) {
+      pnewv(y + 1, z0, in_yh, ryh, scratch);
+      ldv<double, V>(c.Ky.at(x, y + 1, z0), n, rky);
+    }
+  }
+  for (int z = z0; z < z1; z++) {
+    const int bb = (z - z0) & 1;
+    if (act) {
+      lds_st<double, V>(&L.sp[bb][ly + 1][o], n, pc);
+      lds_st<double, V>(&L.skx[bb][ly][o], n, kxc);
+      lds_st<double, V>(&L.sky[bb][ly][V * lx], n, kyc);
+      if (cl.hxl) L.sp[bb][ly + 1][V - 1] = rxl;
+      if (cl.hxh) {
+        L.sp[bb][ly + 1][V + cl.cnt] = rxh;
+        L.skx[bb][ly][V + cl.cnt] = rkx;
+      }
+      if (cl.hyl) lds_st<double, V>(&L.sp[bb][0][o], n, ryl);
+      if (cl.hyh) {
+        lds_st<double, V>(&L.sp[bb][ly + 2][o], n, ryh);
+        lds_st<double, V>(&L.sky[bb][ly + 1][V * lx], n, rky);
+      }
+    }
+    // next plane: p_new and p_old of z + 2 for the register window, u(z + 1)
+    // for the axpy, coefficients and rims of plane z + 1
+    double pnn[V], ponn[V], kznn[V], kxn_c[V], kyn_c[V], un[V];
+#pragma unroll
+    for (int j = 0; j < V; j++) pnn[j] = ponn[j] = kznn[j] = kxn_c[j] = kyn_c[j] = un[j] = 0;
+    if (act && z + 1 < z1) {
+      const int zn = z + 1;
+      pnewv(y, zn + 1, zn + 1 < R.z1, pnn, ponn);
+      ldv<double, V>(c.Kz.at(x, y, zn + 1), n, kznn);
+      ldv<double, V>(c.Kx.at(x, y, zn), n, kxn_c);
+      ldv<double, V>(c.Ky.at(x, y, zn), n, kyn_c);
+      if (AXPY) ldv<double, V>(c.u.at(x, y, zn), n, un);
+      if (cl.hxl) rxl = pnew1(x0 - 1, y, zn, in_xl);
+      if (cl.hxh) {
+        rxh = pnew1(x0 + cl.cnt, y, zn, in_xh);
+        rkx = c.Kx(x0 + cl.cnt, y, zn);
+      }
+      if (cl.hyl) pnewv(y - 1, zn, in_yl, ryl, scratch);
+      if (cl.hyh) {
+        pnewv(y + 1, zn, in_yh, ryh, scratch);
+        ldv<double, V>(c.Ky.at(x, y + 1, zn), n, rky);
+      }
+    }
+    __syncthreads();
+    if (act) {
+      double pym[V], pyp[V], kyn[V], v[V], unew[V];
+      lds_ld<double, V>(&L.sp[bb][ly][o], pym);
+      lds_ld<double, V>(&L.sp[bb][ly + 2][o], pyp);
+      lds_ld<double, V>(&L.sky[bb][ly + 1][V * lx], kyn);
+      const double left = L.sp[bb][ly + 1][o - 1];
+#pragma unroll
+      for (int j = 0; j < V; j++) {
+        const double pxm = j == 0 ? left : pc[j == 0 ? 0 : j - 1];
+        const double pxp = j + 1 < n ? pc[j + 1 < V ? j + 1 : j] : L.sp[bb][ly + 1][o + j + 1];
+        const double kxn = j + 1 < n ? kxc[j + 1 < V ? j + 1 : j] : L.skx[bb][ly][o + j + 1];
+        v[j] = (1.0 + ry * (kyn[j] + kyc[j]) + rx * (kxn + kxc[j]) + rz * (kzn[j] + kzc[j])) * pc[j] -
+               ry * (kyn[j] * pyp[j] + kyc[j] * pym[j]) - rx * (kxn * pxp + kxc[j] * pxm) -
+               rz * (kzn[j] * pn[j] + kzc[j] * pm[j]);
+        unew[j] = uc[j] + alpha_u * poc[j];
+        if (j < n) acc = acc + v[j] * pc[j];
+      }
+      stv<double, V, true>(c.w.at(x, y, z), n, v);
+      stv<double, V, true>(c.pnew.at(x, y, z), n, pc);
+      if (AXPY) stv<double, V, true>(c.u.at(x, y, z), n, unew);
+    }
+#pragma unroll
+    for (int j = 0; j < V; j++) {
+      pm[j] = pc[j];
+      pc[j] = pn[j];
+      pn[j] = pnn[j];
+      poc[j] = pon[j];
+      pon[j] = ponn[j];
+      kzc[j] = kzn[j];
+      kzn[j] = kznn[j];
+      kxc[j] = kxn_c[j];
+      kyc[j] = kyn_c[j];
+      uc[j] = un[j];
+    }
+  }
+  return acc;
+}
+
 // ------------------------------------------------------------ Poisson (7-point, constant coefficient)
 template <class T, int V>
 struct PoisLdsV {

@@ -10,80 +10,32 @@
 //      tea_leaf_cg_calc_w_reduce   w = A p ; pw += w p      (3D, 7-point)
 // The lazy queue defers the axpy past the r.r result (nothing it depends on
 // changes) until the axpby that overwrites p needs it gone, so the three
-// loops meet here.  One z-marching kernel reads u, p_old, r, Kx, Ky, Kz and
-// writes u, p_new, w: 72 bytes per point instead of 24 + 24 + 40 = 88.
-// p_new is needed at stencil neighbours before every thread has written it,
-// so it is recomputed there from p_old and r and written into p's spare
+// loops meet here.  One z-marching column kernel (heat3_chain_column_v)
+// reads u, p_old, r, Kx, Ky, Kz and writes u, p_new, w: 72 bytes per point
+// instead of 40 (the fused pair below) + 40; opt-in (cg_chain_rule_opt_in).
+// p_new is needed at stencil neighbours before every column has written it,
+// so each column forms it
+// from p_old and r for its planes and rims and writes it into p's spare
 // buffer, which then becomes p's buffer (double buffering).  Outside the
 // loop range (the halo) p_new equals p_old, as the axpby leaves it.  Every
 // value is computed with the same operations, in the same order, as the
-// three separate loops.
+// separate loops.
+#include "column.h"
 #include "engine.h"
 
 using namespace opsmi;
 
 namespace {
 
-struct ChainArgs {
-  Ptr<double> u, pnew, w;
-  Ptr<const double> pold, r, Kx, Ky, Kz;
-  double alpha_u, a, b, rx, ry, rz;
-};
-
 template <bool AXPY>
-__global__ __launch_bounds__(256) void k_cg_chain3(ChainArgs c, Range3 R, int chunk, RedCtl ctl) {
-  double acc[1] = {0.0};
+__global__ __launch_bounds__(256) void k_cg_chain(ChainPtrs c, Range3 R, double alpha_u, double a, double b,
+                                                  double rx, double ry, double rz, int chunk, RedCtl ctl) {
+  constexpr int V = 2;
+  __shared__ HeatLdsV<V> lds;
   const TileId t = tile_id();
-  const int x = R.x0 + (int)t.x * kWave + (int)threadIdx.x;
-  const int y = R.y0 + (int)t.y * kWavesPerBlock + (int)threadIdx.y;
-  if (x < R.x1 && y < R.y1) {
-    const int z0 = R.z0 + (int)t.z * chunk;
-    const int z1 = min(z0 + chunk, R.z1);
-    const double a = c.a, b = c.b;
-    // p_new at (x',y',z'): the axpby value inside R, p_old outside
-    auto pv = [&](int xx, int yy, int zz) -> double {
-      const double po = c.pold(xx, yy, zz);
-      if (xx < R.x0 || xx >= R.x1 || yy < R.y0 || yy >= R.y1 || zz < R.z0 || zz >= R.z1) return po;
-      return a * po + b * c.r(xx, yy, zz);
-    };
-    double pm = pv(x, y, z0 - 1);
-    double poc = c.pold(x, y, z0);
-    double pc = a * poc + b * c.r(x, y, z0);
-    double pn = pv(x, y, z0 + 1);
-    double kzc = c.Kz(x, y, z0), kzn = c.Kz(x, y, z0 + 1);
-    double kxc = c.Kx(x, y, z0), kxn = c.Kx(x + 1, y, z0), kyc = c.Ky(x, y, z0), kyn = c.Ky(x, y + 1, z0);
-    double pxm = pv(x - 1, y, z0), pxp = pv(x + 1, y, z0), pym = pv(x, y - 1, z0), pyp = pv(x, y + 1, z0);
-    double uc = AXPY ? c.u(x, y, z0) : 0.0;
-    for (int z = z0; z < z1; z++) {
-      const double v = (1.0 + c.ry * (kyn + kyc) + c.rx * (kxn + kxc) + c.rz * (kzn + kzc)) * pc -
-                       c.ry * (kyn * pyp + kyc * pym) - c.rx * (kxn * pxp + kxc * pxm) -
-                       c.rz * (kzn * pn + kzc * pm);
-      const double pc_v = pc, poc_v = poc, uc_v = uc;
-      if (z + 1 < z1) {
-        const int zn = z + 1;
-        const double pnn = pv(x, y, zn + 1), kznn = c.Kz(x, y, zn + 1);
-        poc = c.pold(x, y, zn);
-        if (AXPY) uc = c.u(x, y, zn);
-        kxc = c.Kx(x, y, zn);
-        kxn = c.Kx(x + 1, y, zn);
-        kyc = c.Ky(x, y, zn);
-        kyn = c.Ky(x, y + 1, zn);
-        pxm = pv(x - 1, y, zn);
-        pxp = pv(x + 1, y, zn);
-        pym = pv(x, y - 1, zn);
-        pyp = pv(x, y + 1, zn);
-        pm = pc;
-        pc = pn;
-        pn = pnn;
-        kzc = kzn;
-        kzn = kznn;
-      }
-      if (AXPY) c.u(x, y, z) = uc_v + c.alpha_u * poc_v;
-      c.pnew(x, y, z) = pc_v;
-      c.w(x, y, z) = v;
-      acc[0] = acc[0] + v * pc_v;
-    }
-  }
+  const int z0 = R.z0 + (int)t.z * chunk;
+  double acc[1] = {heat3_chain_column_v<V, AXPY>(lds, c, R, alpha_u, a, b, rx, ry, rz, R.x0 + (int)t.x * V * kWave,
+                                                 R.y0 + (int)t.y * kWavesPerBlock, z0, min(z0 + chunk, R.z1))};
   reduce_block_store<double, OPS_INC, 1>(acc, ctl);
 }
 
@@ -112,14 +64,21 @@ int cg_chain_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
       ax = nullptr, U = nullptr;
   }
   const Range3 R = cw.range;
+  // the column kernel takes 16-byte loads from every array
+  for (ops_dat d : {P, Rr, W, Kx, Ky, Kz})
+    if (!vec16_ok(d, R.x0)) return 0;
+  if (U && !vec16_ok(U, R.x0)) ax = nullptr, U = nullptr;
+  if (!use_vec() || cw.ctl.slot == nullptr) return 0;
   const int nx = R.x1 - R.x0, ny = R.y1 - R.y0, nz = R.z1 - R.z0;
   const int chunk = heat_zchunk_default();
-  const unsigned gx = (nx + kWave - 1) / kWave, gy = (ny + kWavesPerBlock - 1) / kWavesPerBlock;
+  constexpr int V = 2;
+  const unsigned gx = (nx + V * kWave - 1) / (V * kWave), gy = (ny + kWavesPerBlock - 1) / kWavesPerBlock;
   const unsigned gz = (nz + chunk - 1) / chunk;
-  if (cw.ctl.slot == nullptr) return 0;
-  ChainArgs c;
+  // (the calc_w record sized the reduction partials for at least this many
+  // workgroups: its own grid is the same 128-wide one or the 64-wide one)
+  ChainPtrs c;
   DatView pv = view_of(P);
-  c.pnew = Ptr<double>{(double *)spare_base(P), pv.sy, pv.sz};
+  c.pnew = Ptr<double>{(double *)spare_base(P, R), pv.sy, pv.sz};
   c.pold = ptr_of<const double>(P);
   c.r = ptr_of<const double>(Rr);
   c.w = ptr_of<double>(W);
@@ -127,21 +86,17 @@ int cg_chain_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
   c.Ky = ptr_of<const double>(Ky);
   c.Kz = ptr_of<const double>(Kz);
   c.u = U ? ptr_of<double>(U) : Ptr<double>{nullptr, 0, 0};
-  c.alpha_u = ax ? ax->g[0] : 0.0;
-  c.a = bp.g[0];
-  c.b = bp.g[1];
-  c.rx = cw.g[0];
-  c.ry = cw.g[1];
-  c.rz = cw.g[2];
+  const double alpha_u = ax ? ax->g[0] : 0.0, a = bp.g[0], b = bp.g[1];
+  const double rx = cw.g[0], ry = cw.g[1], rz = cw.g[2];
   const RedCtl ctl = with_scratch(cw.ctl);
   const double bytes = (ax ? ax->bytes : 0.0) + bp.bytes + cw.bytes;
   run_loop(ax ? "cg_chain(axpy+axpby+calc_w)" : "cg_chain(axpby+calc_w)", bytes, [&](hipStream_t st) {
-    if (ax) hipLaunchKernelGGL(k_cg_chain3<true>, dim3(gx, gy, gz), dim3(kWave, kWavesPerBlock), 0, st, c, R, chunk, ctl);
-    else hipLaunchKernelGGL(k_cg_chain3<false>, dim3(gx, gy, gz), dim3(kWave, kWavesPerBlock), 0, st, c, R, chunk, ctl);
+    hipLaunchKernelGGL(ax ? k_cg_chain<true> : k_cg_chain<false>, dim3(gx, gy, gz), dim3(kWave, kWavesPerBlock), 0, st, c, R, alpha_u, a, b, rx, ry, rz, chunk,
+                       ctl);
     launch_red_finish<double, OPS_INC, 1>(ctl, gx * gy * gz, st);
   });
   (void)s;
-  swap_spare(P);
+  swap_spare(P, R);
   if (U) note_external_write(U);
   note_external_write(W);
   if (ax) note_fused(ax->name.c_str(), ax->bytes);
@@ -190,18 +145,18 @@ int axpy_axpby_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
   return 2;
 }
 
-// the three-loop chain recomputes p_new at every stencil neighbour, which
-// costs more load instructions than it saves traffic on this GPU (measured
-// 2363 us against 2345 us for the loops one by one at 512^3), so it is only
-// tried when OPSMI_FUSE_CHAIN3=1; the pointwise pair is the default
-bool chain3_enabled() {
-  const char *e = getenv("OPSMI_FUSE_CHAIN3");
-  return e && e[0] == '1';
+// The three-loop chain moves 72 instead of 80 bytes per point but needs 154
+// VGPRs (3 waves per SIMD; capping at 128 spills): at 512^3 it measured
+// 2093 us against 1116 + 955 us for calc_w and the pair, and 2569 us with one
+// point per lane.  So it runs only at fusion level 2 (ops_mi_set_fusion(2) or
+// OPSMI_FUSION=2); the pair rule takes the axpy + axpby otherwise.
+int cg_chain_rule_opt_in(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
+  return inst().fusion >= 2 ? cg_chain_rule(run, at, s) : 0;
 }
 
 struct Register {
   Register() {
-    if (chain3_enabled()) register_fusion(cg_chain_rule);
+    register_fusion(cg_chain_rule_opt_in);
     register_fusion(axpy_axpby_rule);
   }
 } g_register;

@@ -459,7 +459,11 @@ void flush_for(ops_reduction h) {
   execute(run);
 }
 
-char *spare_base(ops_dat d) {
+static bool same_range6(const int *a, const Range3 &r) {
+  return a[0] == r.x0 && a[1] == r.x1 && a[2] == r.y0 && a[3] == r.y1 && a[4] == r.z0 && a[5] == r.z1;
+}
+
+char *spare_base(ops_dat d, const Range3 &r) {
   Instance &I = inst();
   DatImpl *m = (DatImpl *)d->impl;
   size_t lead = (size_t)(d->data_d - m->alloc);
@@ -467,18 +471,23 @@ char *spare_base(ops_dat d) {
     OPSMI_HIP(hipMalloc(&m->spare, d->mem + lead + 256));
     m->spare_synced = false;
   }
-  if (!m->spare_synced) {
+  // the spare may differ from the buffer inside the last swapped range only
+  if (!m->spare_synced || (m->spare_ranged && !same_range6(m->spare_range, r))) {
     OPSMI_HIP(hipMemcpyAsync(m->spare + lead, d->data_d, d->mem, hipMemcpyDeviceToDevice, I.stream));
     m->spare_synced = true;
+    m->spare_ranged = false;
   }
   return m->spare + lead + d->base_offset;
 }
 
-void swap_spare(ops_dat d) {
+void swap_spare(ops_dat d, const Range3 &r) {
   DatImpl *m = (DatImpl *)d->impl;
   size_t lead = (size_t)(d->data_d - m->alloc);
   std::swap(m->alloc, m->spare);
   d->data_d = m->alloc + lead;
+  const int rr[6] = {r.x0, r.x1, r.y0, r.y1, r.z0, r.z1};
+  memcpy(m->spare_range, rr, sizeof(rr));
+  m->spare_ranged = true;
 }
 
 void note_external_write(ops_dat d) { invalidate_spare(d); }
@@ -531,7 +540,7 @@ void ops_init(const int argc, const char *const argv[], const int diags_level) {
   OPSMI_HIP(hipGetDeviceProperties(&prop, dev));
   I.num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   const char *fz = getenv("OPSMI_FUSION");
-  I.fusion = !(fz && fz[0] == '0');
+  I.fusion = fz ? atoi(fz) : 1;
   OPSMI_HIP(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
   ensure_partials((size_t)1 << 20);
   OPSMI_HIP(hipMalloc(&I.d_partials2, (size_t)1 << 20));
@@ -1222,6 +1231,6 @@ void ops_mi_reset_stats(void) {
   OPSMI_API_END()
 }
 
-void ops_mi_set_fusion(int on) { inst().fusion = on != 0; }
+void ops_mi_set_fusion(int level) { inst().fusion = level < 0 ? 0 : level; }
 
 }  // extern "C"

@@ -74,7 +74,9 @@ struct DatImpl {
   DType dt = DT_OTHER;
   size_t host_bytes = 0;
   char *spare = nullptr;      // second buffer for double-buffered fused chains
-  bool spare_synced = false;  // spare equals the current buffer outside fused writes
+  bool spare_synced = false;  // spare equals the current buffer outside spare_range
+  bool spare_ranged = false;  // spare_range is set (else: equal everywhere)
+  int spare_range[6] = {0, 0, 0, 0, 0, 0};
 };
 
 struct LoopStat {
@@ -97,7 +99,7 @@ struct Instance {
   int error_mode = 0;
   int last_error = 0;
   std::string last_msg;
-  bool fusion = true;
+  int fusion = 1;  // 0 off, 1 default rules, 2 also the opt-in ones
   int block_x = 64, block_y = 4, block_z = 1;  // OPS_BLOCK_SIZE_* (x must be 64)
 
   std::vector<ops_block> blocks;
@@ -244,12 +246,15 @@ void enqueue(LoopRec &&rec);
 void flush_all();
 void flush_for(ops_reduction h);
 void deps_from_args(LoopRec &rec, const ops_arg *args, int n);
-// Double buffering for fused chains that rewrite a dat its own stencil reads:
-// spare_base(d) is point (0,...,0) of a second allocation of d whose contents
-// equal d's current buffer (copied when out of date); swap_spare(d) makes it
-// d's buffer.  Any other write to d marks the spare out of date.
-char *spare_base(ops_dat d);
-void swap_spare(ops_dat d);
+// Double buffering for fused chains that rewrite a dat its own stencil reads
+// over a range r: spare_base(d, r) is point (0,...,0) of a second allocation
+// of d that equals d's current buffer everywhere outside r (copied in full
+// when that is not known to hold); the fused kernel then writes all of r
+// into it and swap_spare(d, r) makes it d's buffer.  The old buffer, now the
+// spare, differs from the new one only inside r, so a following chain over
+// the same r needs no copy.  Any other write to d marks the spare out of date.
+char *spare_base(ops_dat d, const Range3 &r);
+void swap_spare(ops_dat d, const Range3 &r);
 void note_external_write(ops_dat d);
 // counts a loop that ran inside a fused kernel (its time is the fused entry's)
 void note_fused(const char *name, double bytes);

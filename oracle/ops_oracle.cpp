@@ -580,3 +580,41 @@ void orc_pcr_batch(int ndim, int solvedim, const int *dims, const int *pads, con
   }
 }
 
+
+/* --------------------------------------------- multi-block Poisson test app
+ * apps/c/poisson/poisson.cpp:191-288 with one block; kernels of
+ * apps/c/poisson/poisson_kernel.h:4-38 in their operation order. */
+double orc_mb_poisson(int nx, int ny, int iters, double *u_int) {
+  const double dx = 0.01, dy = 0.01, pi = 3.14159265358979323846;
+  const long W = nx + 2, H = ny + 2;
+  std::vector<double> u(W * H), u2(W * H), f(W * H), ref(W * H);
+  auto at = [&](int i, int j) { return (long)(i + 1) + W * (long)(j + 1); };
+  for (int j = -1; j <= ny; j++)
+    for (int i = -1; i <= nx; i++) {
+      const double x = dx * (double)(i + 0), y = dy * (double)(j + 0);
+      const long q = at(i, j);
+      u[q] = (sin(pi * x) * cos(2.0 * pi * y) + 1.0) - 1.0;
+      f[q] = -5.0 * pi * pi * sin(pi * x) * cos(2.0 * pi * y);
+      ref[q] = sin(pi * x) * cos(2.0 * pi * y);
+      u2[q] = u[q];
+    }
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) u[at(i, j)] = 0.0;
+  for (int it = 0; it < iters; it++) {
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++)
+        u2[at(i, j)] = (((u[at(i - 1, j)] + u[at(i + 1, j)]) * dx * dx) + ((u[at(i, j - 1)] + u[at(i, j + 1)]) * dy * dy) -
+                        (f[at(i, j)] * dx * dx * dy * dy)) /
+                       (2.0 * (dx * dx + dy * dy));
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) u[at(i, j)] = u2[at(i, j)];
+  }
+  double err = 0.0;
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      const double d = u[at(i, j)] - ref[at(i, j)];
+      err = err + d * d;
+      if (u_int) u_int[(long)i + (long)nx * j] = u[at(i, j)];
+    }
+  return err;
+}

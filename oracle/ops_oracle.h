@@ -91,6 +91,16 @@ void orc_adi_preproc(int nx, int ny, int nz, double lambda, const double *u_ext,
 /* 27-point weighted box sweep — pinned: fixture */
 void orc_stencil27(int nx, int ny, int nz, const double *u_ext, double *out_int);
 
+/* ---- the reference's multi-block Poisson test app
+ *      (apps/c/poisson/poisson.cpp:56-307, poisson_kernel.h:4-38) on one
+ *      block: populate over the halo-extended box, zero initial guess,
+ *      `iters` Jacobi sweeps + copies, returns the total squared error
+ *      against sin(pi x) cos(2 pi y) — pinned: the app's own known answer
+ *      (16.5173570176708 at 20 x 20, 10 iterations) and runs of the
+ *      reference app built from its sources at other sizes (fixtures).
+ *      u_int (may be NULL) gets the interior. ---- */
+double orc_mb_poisson(int nx, int ny, int iters, double *u_int);
+
 /* ---- batched tridiagonal, semantics of tridDmtsvStridedBatch as called at
  *      ops/c/src/tridiag/ops_tridiag.cpp:126-129: systems along solvedim of
  *      extents dims[], element strides from pads[] (the dat's padded sizes),

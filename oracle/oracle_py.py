@@ -53,6 +53,7 @@ def lib():
             "orc_poisson_mixed": (ci, [ci, ci, ci, ci, _d, cd, ci, ci, cd, ci, _d, _d, _i]),
             "orc_adi_preproc": (None, [ci, ci, ci, cd, _d, _d, _d, _d, _d]),
             "orc_stencil27": (None, [ci, ci, ci, _d, _d]),
+            "orc_mb_poisson": (cd, [ci, ci, ci, _d]),
             "orc_thomas_batch": (None, [ci, ci, _i, _i, _d, _d, _d, _d, _d, ci]),
             "orc_thomas_batch_f32": (None, [ci, ci, _i, _i, _f, _f, _f, _f, _f, ci]),
             "orc_pcr_batch": (None, [ci, ci, _i, _i, _d, _d, _d, _d, _d, ci]),
@@ -213,6 +214,14 @@ def stencil27(nx, ny, nz, u_ext):
     out = np.zeros((nz, ny, nx))
     lib().orc_stencil27(nx, ny, nz, P(np.ascontiguousarray(u_ext)), P(out))
     return out
+
+
+def mb_poisson(nx=20, ny=20, iters=10):
+    """The reference's multi-block Poisson test app on one block: (total
+    squared error, interior u)."""
+    u = np.zeros((ny, nx))
+    err = lib().orc_mb_poisson(nx, ny, iters, P(u))
+    return err, u
 
 
 def _trid(fn, ndim, solvedim, dims, pads, a, b, c, d, u, nthreads):

@@ -33,6 +33,10 @@ META = {
     "heat3d": {"nx": 24, "ny": 20, "nz": 18, "iters": 50, "rx": 1.0, "ry": 1.0, "rz": 1.0},
     "poisson3d": {"nx": 20, "ny": 18, "nz": 16, "iters": 40},
     "adi": {"nx": 16, "ny": 12, "nz": 10, "lambda": 1.0},
+    # the reference's multi-block Poisson app (apps/c/poisson/poisson.cpp),
+    # unchanged: -sizex -sizey -iters; its "Total error" line
+    "mb_poisson": {"configs": [[20, 20, 10], [97, 61, 40], [64, 50, 200], [33, 47, 25], [200, 130, 100]],
+                   "expected_app": 16.5173570176708},
 }
 F32 = {"laplace_f32_err", "laplace_f32_A", "laplace_f32_init", "poisson3d_f32_w"}
 SKIP = {"heat3d_Kz"}
@@ -52,6 +56,13 @@ def main():
                 continue
             dt = np.float32 if key in F32 else np.float64
             arrays[key] = np.fromfile(os.path.join(tmp, fn), dtype=dt)
+        errs = []
+        for nx, ny, it in META["mb_poisson"]["configs"]:
+            out = subprocess.run([os.path.join(ROOT, "oracle/_ref/ref_mb_poisson"), f"-sizex={nx}", f"-sizey={ny}",
+                                  f"-iters={it}"], cwd=tmp, check=True, capture_output=True, text=True).stdout
+            line = [ln for ln in out.splitlines() if ln.startswith("Total error:")][0]
+            errs.append(float(line.split(":")[1]))
+        arrays["mb_poisson_err"] = np.array(errs)
     arrays["meta_json"] = np.frombuffer(json.dumps(META).encode(), dtype=np.uint8)
     np.savez_compressed(OUT, **arrays)
     print("wrote", OUT, "with", len(arrays), "arrays")

@@ -1,8 +1,8 @@
 """Lazy execution and run-time loop-chain fusion.
 
 A reduction result runs only the queued loops it depends on; data access runs
-everything.  The fused CG pair (axpy + axpby) must give the same
-bits as the loops run one by one."""
+everything.  The fused CG chain (axpy + axpby + calc_w) and pair (axpy +
+axpby) must give the same bits as the loops run one by one."""
 import numpy as np
 import pytest
 
@@ -70,29 +70,37 @@ def test_write_after_read_kept_in_order(gpu_ops):
     assert abs(sB - v.sum()) <= 1e-12 * abs(v.sum())
 
 
-@pytest.mark.parametrize("n", [(48, 40, 33), (130, 66, 40)])
+@pytest.mark.parametrize("n", [(48, 40, 33), (130, 66, 40), (131, 37, 21)])
 def test_fused_cg_chain_bit_identical(gpu_ops, n):
     from ops_amd import apps
     ops = gpu_ops
     K, rhs = O.heat_inputs(23, 3, *n)
     runs = []
-    for fusion in (0, 1):
+    for fusion in (0, 1, 2):
         ops.lib().ops_mi_set_fusion(fusion)
         try:
             h = apps.Heat(3, *n, K_ext=K, rhs=rhs)
             ops.ops_mi_reset_stats()
             k, hist = h.iterate(30, history=True)
             assert k == 30
-            st = ops.ops_mi_loop_stats("cg_pair(axpy+axpby)")
+            st = [ops.ops_mi_loop_stats(name) for name in ("cg_pair(axpy+axpby)", "cg_chain(axpy+axpby+calc_w)")]
             u = h.u()
             h.close()
         finally:
             ops.lib().ops_mi_set_fusion(1)
-        runs.append((hist, u, st))
-    (h0, u0, st0), (h1, u1, st1) = runs
-    # one axpy+axpby pair per iteration: each axpy waits in the queue past the
-    # r.r result until the axpby that overwrites p
-    assert st0 is None and st1 is not None and st1[0] == 30
-    for key in ("pw", "alpha", "beta", "rrn"):
-        assert np.array_equal(h0[key], h1[key]), key
-    assert np.array_equal(u0, u1)
+        runs.append((hist, u, [s[0] if s else 0 for s in st]))
+    (h0, u0, st0), (h1, u1, st1), (h2, u2, st2) = runs
+    # each iteration's axpy waits in the queue past the r.r result until the
+    # axpby that overwrites p; at level 1 the two run as the pointwise pair;
+    # at level 2 they join the next calc_w in the three-loop chain where every
+    # array takes 16-byte lanes (rows of at least 1 KiB are padded and
+    # aligned); the last iteration's axpy + axpby is always a pair
+    assert st0 == [0, 0]
+    assert st1 == [30, 0]
+    assert st2[0] + st2[1] == 30
+    if n[0] + 2 >= 128:
+        assert st2 == [1, 29]
+    for hist, u in ((h1, u1), (h2, u2)):
+        for key in ("pw", "alpha", "beta", "rrn"):
+            assert np.array_equal(h0[key], hist[key]), key
+        assert np.array_equal(u0, u)

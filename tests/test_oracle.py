@@ -299,3 +299,15 @@ def test_mixed_precision_reaches_fp64_solution():
     O.interior(x)[:] = mp["x"]
     _, rr = O.poisson_residual(3, nx, ny, nz, f, x)
     assert np.sqrt(rr) / np.linalg.norm(f) == pytest.approx(mp["relres"][-1], rel=1e-6)
+
+
+def test_mb_poisson_matches_reference_app(golden):
+    # the reference's multi-block Poisson app, built from its sources and run
+    # unchanged (its printed total error has 15 significant digits)
+    g, m = golden
+    c = m["mb_poisson"]
+    assert abs(g["mb_poisson_err"][0] - c["expected_app"]) <= 1e-13 * c["expected_app"]
+    for (nx, ny, it), ref in zip(c["configs"], g["mb_poisson_err"]):
+        err, u = O.mb_poisson(nx, ny, it)
+        assert abs(err - ref) <= 1e-13 * ref, (nx, ny, it)
+        assert u.shape == (ny, nx)
