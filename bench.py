@@ -265,6 +265,12 @@ def bench_serve(a, ops, apps):
     assert np.all(iters > 0), "a solve stopped without converging"
     sat = nreq / span
     print(f"batched, all queued at t=0: {nreq} solves in {span:.2f} s", file=sys.stderr, flush=True)
+    # the same stream with FP32-inner refinement for the 2D Poisson requests
+    span_m, _, iters_m = apps.serve_stream(kinds, sizes, seeds, np.zeros(nreq), tol, budget, autotune=True,
+                                           mixed=True)
+    assert np.all(iters_m > 0), "a mixed solve stopped without converging"
+    sat_m = nreq / span_m
+    print(f"batched FP32-inner, all queued at t=0: {nreq} solves in {span_m:.2f} s", file=sys.stderr, flush=True)
     # latency under Poisson arrivals at 70 % of the saturated rate
     rate = 0.7 * sat
     arr = np.cumsum(g.exponential(1.0 / rate, nreq))
@@ -290,6 +296,8 @@ def bench_serve(a, ops, apps):
                 {"workload": f"serve_stream_{nreq}", "requests": nreq, "tol": tol, "budget_bytes": budget,
                  "mix": "60% 2D 5-pt Poisson side log-U[64,2048], 40% 3D 7-pt heat side log-U[32,256]"},
                 batched_saturated={"solves_per_s": round(sat, 3), "makespan_s": round(span, 3)},
+                batched_fp32_inner_saturated={"solves_per_s": round(sat_m, 3), "makespan_s": round(span_m, 3),
+                                              "speedup_vs_per_solve_fp64": round(sat_m / base, 3)},
                 batched_poisson_arrivals={"rate_per_s": round(rate, 3), "p50_latency_s": float(np.percentile(lat, 50)),
                                           "p99_latency_s": float(np.percentile(lat, 99)),
                                           "solves_per_s": round(nreq / span2, 3)},

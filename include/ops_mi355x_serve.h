@@ -44,14 +44,19 @@ typedef struct {
   int n;         /* grid side */
   uint64_t seed; /* input generator seed */
   double tol;    /* relative residual target */
-  int max_iters;
+  int max_iters; /* CG iterations (mixed: inner iterations in total) */
+  int mixed;     /* 1: FP32-inner refinement (2D Poisson only): FP64 x, f
+                    and residual, Jacobi PCG on FP32 fields with FP64 dots,
+                    inner stop max(1e-4, tol / (2 relres)), at most 30
+                    corrections — the algorithm of opsapp_pmixed_solve */
 } opsmi_solve_req;
 
 typedef struct {
   int id;
-  int status; /* 1 converged, 2 stopped at max_iters */
-  int iters;
-  double rr0, rr; /* r.r at start and at the end */
+  int status; /* 1 converged, 2 stopped at max_iters (or 30 corrections) */
+  int iters;  /* CG iterations (mixed: inner iterations) */
+  int outer;  /* mixed: corrections applied */
+  double rr0, rr; /* r.r at start and at the end (mixed: true FP64 residual) */
 } opsmi_solve_done;
 
 /* hbm_budget: bytes of device memory for solve fields (one arena);

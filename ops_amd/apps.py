@@ -42,7 +42,7 @@ def lib():
             "opsapp_trid_fetch": (None, [vp, _d, _d]),
             "opsapp_trid_destroy": (None, [vp]),
             "opsapp_serve_stream": (cd, [ci, C.POINTER(ci), C.POINTER(ci), C.POINTER(u64), _d, cd, C.c_size_t, ci,
-                                         ci, ci, _d, C.POINTER(ci)]),
+                                         ci, ci, ci, _d, C.POINTER(ci)]),
             "opsapp_mb_poisson": (cd, [ci, ci, ci, ci, ci, _d]),
         }
         for name, (res, args) in sig.items():
@@ -200,9 +200,10 @@ def mb_poisson(nx=20, ny=20, ngx=1, ngy=1, iters=10, want_field=True):
 
 
 def serve_stream(kinds, sizes, seeds, arrival_s, tol=1e-10, budget=200 * 10 ** 9, max_slots=256,
-                 iters_per_launch=8, autotune=False):
+                 iters_per_launch=8, autotune=False, mixed=False):
     """Runs a request stream on the batched engine (opsapp_serve_stream);
-    returns (makespan_s, done_s, iters)."""
+    mixed solves the 2D Poisson requests with FP32-inner refinement.
+    Returns (makespan_s, done_s, iters)."""
     n = len(kinds)
     k = np.ascontiguousarray(kinds, dtype=np.int32)
     s = np.ascontiguousarray(sizes, dtype=np.int32)
@@ -213,7 +214,8 @@ def serve_stream(kinds, sizes, seeds, arrival_s, tol=1e-10, budget=200 * 10 ** 9
     ci = C.POINTER(C.c_int)
     span = lib().opsapp_serve_stream(n, k.ctypes.data_as(ci), s.ctypes.data_as(ci),
                                      sd.ctypes.data_as(C.POINTER(C.c_uint64)), _p(arr), tol, budget, max_slots,
-                                     iters_per_launch, 1 if autotune else 0, _p(done), it.ctypes.data_as(ci))
+                                     iters_per_launch, 1 if autotune else 0, 1 if mixed else 0, _p(done),
+                                     it.ctypes.data_as(ci))
     _chk()
     if span < 0:
         raise RuntimeError("opsapp_serve_stream failed")

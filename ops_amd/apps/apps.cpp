@@ -709,7 +709,7 @@ double opsapp_mb_poisson(int nx, int ny, int ngx, int ngy, int iters, double *u_
 // arrived, and completes when a scheduler step reports it converged (its
 // memory is then released).  Between steps nothing else runs on the host.
 double opsapp_serve_stream(int nreq, const int *kind, const int *n, const uint64_t *seed, const double *arrival_s,
-                           double tol, size_t budget, int max_slots, int iters_per_launch, int autotune,
+                           double tol, size_t budget, int max_slots, int iters_per_launch, int autotune, int mixed,
                            double *done_s, int *iters) {
   opsmi_server S = opsmi_serve_create(budget, max_slots, iters_per_launch, 1);
   if (!S) return -1.0;
@@ -724,7 +724,8 @@ double opsapp_serve_stream(int nreq, const int *kind, const int *n, const uint64
   while (done < nreq) {
     double t = wtime() - t0;
     while (next < nreq && arrival_s[next] <= t) {
-      opsmi_solve_req q{kind[next], n[next], seed[next], tol, 1000000};
+      opsmi_solve_req q{kind[next], n[next], seed[next], tol, 1000000,
+                        mixed && kind[next] == OPSMI_SOLVE_POISSON2D ? 1 : 0};
       const int id = opsmi_serve_submit(S, &q);
       if (id < 0) {
         opsmi_serve_destroy(S);

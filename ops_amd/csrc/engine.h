@@ -406,7 +406,10 @@ LoopRec make_loop(const char *name, const ops_arg *args, int nargs, const Range3
     rec.launch = [](hipStream_t) {};
     return rec;
   }
-  Tiling t = make_tiling(r, 1);
+  // 2D ranges: four groups of four rows per workgroup (64 x 16 points), so a
+  // 2048^2 sweep is 4096 workgroups and its reduction one fold kernel; 3D
+  // ranges: one plane of 64 x 4 per workgroup
+  Tiling t = make_tiling(r, r.z1 - r.z0 > 1 ? 1 : 4);
   // vector engine when the functor has one and every dat allows 16-byte
   // loads at the range's x start
   constexpr int V = vec_width<F>::value;

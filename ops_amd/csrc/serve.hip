@@ -39,6 +39,7 @@ namespace {
 
 constexpr int kRows2D = 4;  // rows per wave of a 2D tile (tile = 64 x 16)
 constexpr int kChunk3D = 16;
+constexpr int kPart = 4;  // partial sums per tile (stride of the partials array)
 
 struct SolveDev {
   double *u, *p, *r, *w, *kx, *ky, *kz;  // element (0,0,0) of each field
@@ -49,11 +50,24 @@ struct SolveDev {
   int tile0, ntiles;
   int active;
   double rx, ry, rz;
+  // FP32-inner refinement (2D Poisson): u = x and f (FP64), and the FP32
+  // inner fields e, rr, p, w with their own row pitch
+  int mixed;
+  double *f;
+  float *e32, *r32, *p32, *w32;
+  long sy32;
 };
+
+// phases of a mixed solve: the next iteration runs an inner FP32 PCG step,
+// or applies the correction x += e and recomputes the FP64 residual
+enum { PH_INNER = 0, PH_RESID = 1 };
 
 struct SolveStat {
   double rro, rr0, stop, pw, alpha, beta;
   int iters, done, max_iters, pad;
+  // mixed: inner r.z, inner stop threshold, |f|, targets
+  double rz, istop, fn, tol, inner_tol;
+  int phase, outer, max_outer, axpby;  // axpby: the next p update runs
 };
 
 struct TileBox {
@@ -103,6 +117,15 @@ __device__ __forceinline__ void store_partial(double v, double *dst, double (*sh
   block_reduce<double, OPS_INC, 1>(acc, sh);
   if (threadIdx.x == 0 && threadIdx.y == 0) *dst = acc[0];
 }
+template <int D>
+__device__ __forceinline__ void store_partials(double (&acc)[D], double *dst, double (*sh)[D]) {
+  block_reduce<double, OPS_INC, D>(acc, sh);
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+#pragma unroll
+    for (int q = 0; q < D; q++) dst[q] = acc[q];
+}
+
+__device__ __forceinline__ Ptr<float> PF32(float *a, const SolveDev &d) { return Ptr<float>{a, d.sy32, 0}; }
 
 // ------------------------------------------------------------ iteration
 __global__ __launch_bounds__(256) void sk_calc_w(const SolveDev *S, const SolveStat *st, const int2 *tiles,
@@ -116,6 +139,25 @@ __global__ __launch_bounds__(256) void sk_calc_w(const SolveDev *S, const SolveS
     const SolveDev d = S[e.x];
     const TileBox b = tile_box(d, e.y);
     double acc = 0.0;
+    if (d.mixed) {
+      if (st[e.x].phase == PH_RESID) {
+        // the correction x = x + (double) e (pointwise; the residual follows
+        // in sk_update once every tile has applied it)
+        const Ptr<double> x = P(d.u, d);
+        const Ptr<float> ev = PF32(d.e32, d);
+        tile_points(d, b, [&](int i, int j, int k) { x(i, j, k) = x(i, j, k) + (double)ev(i, j, k); });
+        continue;
+      }
+      // inner sweep in FP32: w = 4p - (p(-1,0) + p(1,0)) - (p(0,-1) + p(0,1)), p.w in FP64
+      const Ptr<float> p = PF32(d.p32, d), w = PF32(d.w32, d);
+      tile_points(d, b, [&](int i, int j, int k) {
+        const float v = 4.0f * p(i, j, k) - (p(i - 1, j, k) + p(i + 1, j, k)) - (p(i, j - 1, k) + p(i, j + 1, k));
+        w(i, j, k) = v;
+        acc = acc + (double)v * (double)p(i, j, k);
+      });
+      store_partial(acc, part + (size_t)t * kPart, sh);
+      continue;
+    }
     if (d.kind == OPSMI_SOLVE_HEAT3D) {
       __syncthreads();  // LDS of the previous tile fully read
       acc = heat3_column<false>(lds, P(d.w, d), Ptr<double>{nullptr, 0, 0}, PC(d.kx, d), PC(d.ky, d), PC(d.kz, d),
@@ -129,18 +171,54 @@ __global__ __launch_bounds__(256) void sk_calc_w(const SolveDev *S, const SolveS
         acc = acc + v * p(x, y, z);
       });
     }
-    store_partial(acc, part + t, sh);
+    store_partial(acc, part + (size_t)t * kPart, sh);
   }
 }
 
 __global__ __launch_bounds__(256) void sk_update(const SolveDev *S, const SolveStat *st, const int2 *tiles,
                                                  const int *ntiles, double *part) {
   __shared__ double sh[16][1];
+  __shared__ double sh3[16][3];
   const int nt = *ntiles;
   for (int t = (int)xcd_tile(blockIdx.x, gridDim.x); t < nt; t += (int)gridDim.x) {
     const int2 e = tiles[t];
     if (st[e.x].done) continue;
     const SolveDev d = S[e.x];
+    if (d.mixed) {
+      const float mi = 0.25f;  // Jacobi: 1 / diag of the 5-point operator
+      const Ptr<float> ev = PF32(d.e32, d), rr = PF32(d.r32, d), p = PF32(d.p32, d), w = PF32(d.w32, d);
+      double acc[3] = {0.0, 0.0, 0.0};
+      if (st[e.x].phase == PH_INNER) {
+        // e += a p ; rr -= a w ; (rr . Mi rr, rr . rr) in FP64
+        const float a = (float)st[e.x].alpha;
+        tile_points(d, tile_box(d, e.y), [&](int i, int j, int k) {
+          ev(i, j, k) = ev(i, j, k) + a * p(i, j, k);
+          const float q = rr(i, j, k) - a * w(i, j, k);
+          rr(i, j, k) = q;
+          const float z = mi * q;
+          acc[0] = acc[0] + (double)q * (double)z;
+          acc[1] = acc[1] + (double)q * (double)q;
+        });
+      } else {
+        // FP64 residual r = f - A x, then the inner start from it:
+        // rr = (float) r, e = 0, p = Mi rr; (r.r, rr . Mi rr, rr . rr)
+        const Ptr<const double> x = PC(d.u, d), f = PC(d.f, d);
+        tile_points(d, tile_box(d, e.y), [&](int i, int j, int k) {
+          const double ax = 4.0 * x(i, j, k) - (x(i - 1, j, k) + x(i + 1, j, k)) - (x(i, j - 1, k) + x(i, j + 1, k));
+          const double v = f(i, j, k) - ax;
+          acc[0] = acc[0] + v * v;
+          const float q = (float)v;
+          const float z = mi * q;
+          ev(i, j, k) = 0.0f;
+          rr(i, j, k) = q;
+          p(i, j, k) = z;
+          acc[1] = acc[1] + (double)q * (double)z;
+          acc[2] = acc[2] + (double)q * (double)q;
+        });
+      }
+      store_partials<3>(acc, part + (size_t)t * kPart, sh3);
+      continue;
+    }
     const double alpha = st[e.x].alpha;
     const Ptr<double> u = P(d.u, d), r = P(d.r, d);
     const Ptr<const double> p = PC(d.p, d), w = PC(d.w, d);
@@ -151,7 +229,7 @@ __global__ __launch_bounds__(256) void sk_update(const SolveDev *S, const SolveS
       r(x, y, z) = v;
       acc = acc + v * v;
     });
-    store_partial(acc, part + t, sh);
+    store_partial(acc, part + (size_t)t * kPart, sh);
   }
 }
 
@@ -162,6 +240,14 @@ __global__ __launch_bounds__(256) void sk_axpby(const SolveDev *S, const SolveSt
     const int2 e = tiles[t];
     if (st[e.x].done) continue;
     const SolveDev d = S[e.x];
+    if (d.mixed) {
+      // p = Mi rr + b p, unless the inner solve just stopped or started
+      if (!st[e.x].axpby) continue;
+      const float mi = 0.25f, bf = (float)st[e.x].beta;
+      const Ptr<float> p = PF32(d.p32, d), rr = PF32(d.r32, d);
+      tile_points(d, tile_box(d, e.y), [&](int i, int j, int k) { p(i, j, k) = mi * rr(i, j, k) + bf * p(i, j, k); });
+      continue;
+    }
     const double beta = st[e.x].beta;
     const Ptr<double> p = P(d.p, d);
     const Ptr<const double> r = PC(d.r, d);
@@ -170,17 +256,30 @@ __global__ __launch_bounds__(256) void sk_axpby(const SolveDev *S, const SolveSt
 }
 
 // PHASE 0: alpha; PHASE 1: beta, iteration count and stop test
+// Mixed solves follow the refinement loop of the FP64-IR / FP32-PCG solver:
+//   PH_RESID: (x += e), residual r, |r|/|f| <= tol stops; else the inner
+//             solve starts with rz and its stop threshold
+//             max(inner_tol, tol / (2 relres)) |rr_0|
+//   PH_INNER: a = (float)(rz / pw); after the update, stop the inner solve at
+//             its threshold (the next iteration corrects x), else
+//             b = (float)(rzn / rz) and p = Mi rr + b p
 template <int PHASE>
 __global__ __launch_bounds__(256) void sk_fold(const SolveDev *S, SolveStat *st, const double *part) {
-  __shared__ double sh[16][1];
+  __shared__ double sh[16][3];
   const int s = (int)blockIdx.x;
   if (!S[s].active || st[s].done) return;
+  const bool mixed = S[s].mixed;
+  const int ph = st[s].phase;
+  if (mixed && PHASE == 0 && ph == PH_RESID) return;  // the correction pass has no sum
   const int t0 = S[s].tile0, n = S[s].ntiles;
-  double acc[1] = {0.0};
-  for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) acc[0] = acc[0] + part[t0 + i];
-  block_reduce<double, OPS_INC, 1>(acc, sh);
-  if (threadIdx.x == 0) {
-    SolveStat &q = st[s];
+  double acc[3] = {0.0, 0.0, 0.0};
+  const int nq = mixed && PHASE == 1 ? 3 : 1;
+  for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x)
+    for (int k = 0; k < nq; k++) acc[k] = acc[k] + part[(size_t)(t0 + i) * kPart + k];
+  block_reduce<double, OPS_INC, 3>(acc, sh);
+  if (threadIdx.x != 0) return;
+  SolveStat &q = st[s];
+  if (!mixed) {
     if (PHASE == 0) {
       q.pw = acc[0];
       q.alpha = q.rro / acc[0];
@@ -191,6 +290,44 @@ __global__ __launch_bounds__(256) void sk_fold(const SolveDev *S, SolveStat *st,
       q.iters = q.iters + 1;
       if (sqrt(rrn) <= q.stop) q.done = 1;
       else if (q.iters >= q.max_iters) q.done = 2;
+    }
+    return;
+  }
+  if (PHASE == 0) {  // PH_INNER
+    q.pw = acc[0];
+    q.alpha = (double)(float)(q.rz / acc[0]);
+    return;
+  }
+  if (ph == PH_INNER) {
+    const double rzn = acc[0], rrn = acc[1];
+    q.iters = q.iters + 1;
+    q.rro = rrn;
+    if (sqrt(rrn) <= q.istop) {
+      q.phase = PH_RESID;
+      q.outer = q.outer + 1;
+      q.axpby = 0;
+    } else if (q.iters >= q.max_iters) {
+      q.done = 2;
+    } else {
+      q.beta = (double)(float)(rzn / q.rz);
+      q.rz = rzn;
+      q.axpby = 1;
+    }
+  } else {
+    const double rrd = acc[0];
+    if (q.outer == 0) {
+      q.fn = sqrt(rrd);
+      q.rr0 = rrd;
+    }
+    q.rro = rrd;
+    const double rel = sqrt(rrd) / q.fn;
+    if (rel <= q.tol) q.done = 1;
+    else if (q.outer >= q.max_outer) q.done = 2;
+    else {
+      q.rz = acc[1];
+      q.istop = fmax(q.inner_tol, 0.5 * q.tol / rel) * sqrt(acc[2]);
+      q.phase = PH_INNER;
+      q.axpby = 0;
     }
   }
 }
@@ -256,6 +393,32 @@ __global__ __launch_bounds__(256) void sk_first_residual(SolveDev d, uint64_t se
   store_partial(acc, part + li, sh);
 }
 
+// mixed 2D Poisson: f = rhs inside (x and the FP32 fields start at zero);
+// the first iteration's residual pass finds r = f
+__global__ __launch_bounds__(256) void sk_mixed_inputs(SolveDev d, uint64_t seed) {
+  const long n = (long)d.nx * d.ny;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(q % d.nx), y = (int)(q / d.nx);
+    P(d.f, d)(x, y, 0) = opsapp::poisson_rhs(seed, x, y, 0, d.nx, d.ny);
+  }
+}
+
+__global__ void sk_mixed_stat(SolveStat *q, double tol, double inner_tol, int max_iters, int max_outer) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  q->rro = q->rr0 = q->stop = q->pw = q->alpha = q->beta = 0.0;
+  q->iters = 0;
+  q->done = 0;
+  q->max_iters = max_iters;
+  q->pad = 0;
+  q->rz = q->istop = q->fn = 0.0;
+  q->tol = tol;
+  q->inner_tol = inner_tol;
+  q->phase = PH_RESID;
+  q->outer = 0;
+  q->max_outer = max_outer;
+  q->axpby = 0;
+}
+
 __global__ __launch_bounds__(256) void sk_first_fold(SolveStat *q, const double *part, int n, double tol,
                                                      int max_iters) {
   __shared__ double sh[16][1];
@@ -271,6 +434,9 @@ __global__ __launch_bounds__(256) void sk_first_fold(SolveStat *q, const double 
     q->done = 0;
     q->max_iters = max_iters;
     q->pad = 0;
+    q->phase = PH_INNER;
+    q->outer = 0;
+    q->axpby = 0;
   }
 }
 
@@ -313,21 +479,26 @@ struct Arena {
 
 struct Geometry {
   int nfields, nx, ny, nz;
-  long pitch, plane, field;  // elements
+  long pitch, plane, field;  // double fields, elements
+  int nfields32;             // FP32 fields (mixed solves)
+  long pitch32, field32;     // elements
   size_t bytes;
 };
 
 Geometry geometry(const opsmi_solve_req &q) {
   Geometry g;
   const bool heat = q.kind == OPSMI_SOLVE_HEAT3D;
-  g.nfields = heat ? 7 : 4;
+  g.nfields = heat ? 7 : (q.mixed ? 2 : 4);
   g.nx = g.ny = q.n;
   g.nz = heat ? q.n : 1;
   g.pitch = ((long)g.nx + 2 + 31) / 32 * 32;
   g.plane = g.pitch * (g.ny + 2);
   // 32 lead elements put x = 0 on a 256-byte boundary; round to 256 bytes
   g.field = (g.plane * (heat ? g.nz + 2 : 1) + 32 + 31) / 32 * 32;
-  g.bytes = (size_t)g.nfields * g.field * sizeof(double);
+  g.nfields32 = q.mixed ? 4 : 0;
+  g.pitch32 = ((long)g.nx + 2 + 63) / 64 * 64;
+  g.field32 = (g.pitch32 * (g.ny + 2) + 64 + 63) / 64 * 64;
+  g.bytes = (size_t)g.nfields * g.field * sizeof(double) + (size_t)g.nfields32 * g.field32 * sizeof(float);
   return g;
 }
 
@@ -379,6 +550,8 @@ void check_req(const opsmi_solve_req &q) {
     fail(OPS_INVALID_ARGUMENT, cat("opsmi_serve_submit: grid side ", q.n, " out of range"));
   if (!(q.tol > 0.0) || q.max_iters < 1)
     fail(OPS_INVALID_ARGUMENT, "opsmi_serve_submit: tol must be > 0 and max_iters >= 1");
+  if (q.mixed != 0 && (q.mixed != 1 || q.kind != OPSMI_SOLVE_POISSON2D))
+    fail(OPS_INVALID_ARGUMENT, "opsmi_serve_submit: the FP32-inner mode (mixed = 1) is for 2D Poisson solves");
 }
 
 // tile parameter of a solve: z planes per 3D tile, rows per wave of a 2D
@@ -445,17 +618,30 @@ void admit(opsmi_server S) {
     R.bytes = g.bytes;
     S->in_use += g.bytes;
     S->slot_req[slot] = id;
-    // fields: u, p, r, w [, kx, ky, kz]
+    // fields: u, p, r, w [, kx, ky, kz]; mixed: x, f, then FP32 e, rr, p, w
     double *base = (double *)(S->arena_base + off);
     double *f[7] = {};
     for (int i = 0; i < g.nfields; i++) f[i] = base + (size_t)i * g.field;
     // element (-1,-1,-1) sits at field + 31, so x = 0 starts a 256-byte line
     const long lead = 32 + g.pitch + (R.q.kind == OPSMI_SOLVE_HEAT3D ? g.plane : 0);
     SolveDev d{};
-    d.u = f[0] + lead;
-    d.p = f[1] + lead;
-    d.r = f[2] + lead;
-    d.w = f[3] + lead;
+    if (R.q.mixed) {
+      float *b32 = (float *)(base + (size_t)g.nfields * g.field);
+      const long lead32 = 64 + g.pitch32;  // float (-1,-1) at +63
+      d.u = f[0] + lead;
+      d.f = f[1] + lead;
+      d.e32 = b32 + lead32;
+      d.r32 = b32 + g.field32 + lead32;
+      d.p32 = b32 + 2 * g.field32 + lead32;
+      d.w32 = b32 + 3 * g.field32 + lead32;
+      d.sy32 = g.pitch32;
+      d.mixed = 1;
+    } else {
+      d.u = f[0] + lead;
+      d.p = f[1] + lead;
+      d.r = f[2] + lead;
+      d.w = f[3] + lead;
+    }
     if (R.q.kind == OPSMI_SOLVE_HEAT3D) {
       d.kx = f[4] + lead;
       d.ky = f[5] + lead;
@@ -476,9 +662,20 @@ void admit(opsmi_server S) {
     d.rx = d.ry = d.rz = 1.0;
     R.dev = d;
     hipStream_t st = S->stream;
-    const size_t nall = (size_t)g.nfields * g.field;
+    const size_t nall = g.bytes / sizeof(double);
     hipLaunchKernelGGL(sk_zero, dim3((unsigned)std::min<size_t>((nall + 255) / 256, 4096)), dim3(256), 0, st, base,
                        nall);
+    if (R.q.mixed) {
+      const long ni = (long)g.nx * g.ny;
+      hipLaunchKernelGGL(sk_mixed_inputs, dim3((unsigned)std::min<long>((ni + 255) / 256, 4096)), dim3(256), 0, st, d,
+                         R.q.seed);
+      hipLaunchKernelGGL(sk_mixed_stat, dim3(1), dim3(64), 0, st, S->d_stat + slot, R.q.tol, 1e-4, R.q.max_iters,
+                         30);
+      OPSMI_HIP(hipGetLastError());
+      S->dirty = true;
+      S->n_active++;
+      continue;
+    }
     if (R.q.kind == OPSMI_SOLVE_HEAT3D) {
       const long ne = (long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
       const unsigned gb = (unsigned)std::min<long>((ne + 255) / 256, 4096);
@@ -633,6 +830,7 @@ int opsmi_serve_step(opsmi_server S) {
       c.id = rid;
       c.status = S->h_stat[s].done;
       c.iters = S->h_stat[s].iters;
+      c.outer = S->h_stat[s].outer;
       c.rr0 = S->h_stat[s].rr0;
       c.rr = S->h_stat[s].rro;
       S->completed.push_back(c);
@@ -694,7 +892,7 @@ int opsmi_serve_autotune(opsmi_server S, int iters) {
   OPSMI_HIP(hipEventCreate(&e1));
   for (int kind = 0; kind < 2; kind++)
     for (int cls = 0; cls < 3; cls++) {
-      opsmi_solve_req q{kind, rep[kind][cls], 12345, 1e-300, 1 << 30};
+      opsmi_solve_req q{kind, rep[kind][cls], 12345, 1e-300, 1 << 30, 0};
       if (geometry(q).bytes > S->budget) continue;
       int *slot_param = kind == OPSMI_SOLVE_HEAT3D ? &S->chunk3d[cls] : &S->rows2d[cls];
       const int *cand = kind == OPSMI_SOLVE_HEAT3D ? cand3 : cand2;

@@ -38,6 +38,14 @@ __device__ __forceinline__ long line_offset(const LineGeo &g, long id) {
   return i0 * g.s0 + i1 * g.s1;
 }
 
+// The recurrence is serial per system but its loads are not: each pass loads
+// the coefficients of kPre steps at once, then runs those steps, so a thread
+// keeps 4 kPre loads in flight instead of waiting on memory at every step
+// (the arithmetic and its order are unchanged).  Without _Inc, d' is written
+// in place into d (the backward pass overwrites it with x), so only c' goes
+// through the scratch; _Inc leaves d untouched and keeps d' in scratch too.
+constexpr int kPre = 8;
+
 template <class T, bool INC>
 __global__ __launch_bounds__(256) void k_thomas(const T *__restrict__ a, const T *__restrict__ b,
                                                 const T *__restrict__ c, T *__restrict__ d, T *__restrict__ u,
@@ -45,29 +53,63 @@ __global__ __launch_bounds__(256) void k_thomas(const T *__restrict__ a, const T
   const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= g.count) return;
   const long o = line_offset(g, id);
-  const long s = g.stride, L = g.count;
+  const long s = g.stride, L = g.count, n = g.n;
   const T *ap = a + o, *bp = b + o, *cp = c + o;
   T *dp = d + o;
   T bb = (T)1.0 / bp[0];
   T cprev = bb * cp[0];
   T dprev = bb * dp[0];
   cs[id] = cprev;
-  ds[id] = dprev;
-  for (long i = 1; i < g.n; i++) {
-    const T ai = ap[i * s];
-    bb = (T)1.0 / (bp[i * s] - ai * cprev);
-    dprev = bb * (dp[i * s] - ai * dprev);
-    cprev = bb * cp[i * s];
-    cs[i * L + id] = cprev;
-    ds[i * L + id] = dprev;
+  if (INC) ds[id] = dprev;
+  else dp[0] = dprev;
+  for (long i0 = 1; i0 < n; i0 += kPre) {
+    T av[kPre], bv[kPre], cv[kPre], dv[kPre];
+#pragma unroll
+    for (int k = 0; k < kPre; k++) {
+      const long i = i0 + k;
+      if (i < n) {
+        av[k] = ap[i * s];
+        bv[k] = bp[i * s];
+        cv[k] = cp[i * s];
+        dv[k] = dp[i * s];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kPre; k++) {
+      const long i = i0 + k;
+      if (i < n) {
+        bb = (T)1.0 / (bv[k] - av[k] * cprev);
+        dprev = bb * (dv[k] - av[k] * dprev);
+        cprev = bb * cv[k];
+        cs[i * L + id] = cprev;
+        if (INC) ds[i * L + id] = dprev;
+        else dp[i * s] = dprev;
+      }
+    }
   }
   T x = dprev;
-  if (INC) u[o + (g.n - 1) * s] = u[o + (g.n - 1) * s] + x;
-  else dp[(g.n - 1) * s] = x;
-  for (long i = g.n - 2; i >= 0; i--) {
-    x = ds[i * L + id] - cs[i * L + id] * x;
-    if (INC) u[o + i * s] = u[o + i * s] + x;
-    else dp[i * s] = x;
+  if (INC) u[o + (n - 1) * s] = u[o + (n - 1) * s] + x;
+  else dp[(n - 1) * s] = x;
+  for (long i1 = n - 2; i1 >= 0; i1 -= kPre) {
+    T cv[kPre], dv[kPre], uv[kPre];
+#pragma unroll
+    for (int k = 0; k < kPre; k++) {
+      const long i = i1 - k;
+      if (i >= 0) {
+        cv[k] = cs[i * L + id];
+        dv[k] = INC ? ds[i * L + id] : dp[i * s];
+        if (INC) uv[k] = u[o + i * s];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kPre; k++) {
+      const long i = i1 - k;
+      if (i >= 0) {
+        x = dv[k] - cv[k] * x;
+        if (INC) u[o + i * s] = uv[k] + x;
+        else dp[i * s] = x;
+      }
+    }
   }
 }
 

@@ -11,12 +11,12 @@ POISSON2D, HEAT3D = 0, 1
 
 class SolveReq(C.Structure):
     _fields_ = [("kind", C.c_int), ("n", C.c_int), ("seed", C.c_uint64), ("tol", C.c_double),
-                ("max_iters", C.c_int)]
+                ("max_iters", C.c_int), ("mixed", C.c_int)]
 
 
 class SolveDone(C.Structure):
-    _fields_ = [("id", C.c_int), ("status", C.c_int), ("iters", C.c_int), ("rr0", C.c_double),
-                ("rr", C.c_double)]
+    _fields_ = [("id", C.c_int), ("status", C.c_int), ("iters", C.c_int), ("outer", C.c_int),
+                ("rr0", C.c_double), ("rr", C.c_double)]
 
 
 _sigs = None
@@ -50,8 +50,8 @@ def lib():
     return L
 
 
-def request_bytes(kind, n, tol=1e-10, max_iters=100000):
-    r = SolveReq(kind, n, 0, tol, max_iters)
+def request_bytes(kind, n, tol=1e-10, max_iters=100000, mixed=False):
+    r = SolveReq(kind, n, 0, tol, max_iters, 1 if mixed else 0)
     v = lib().opsmi_serve_request_bytes(C.byref(r))
     ops._check()
     return v
@@ -65,8 +65,9 @@ class Server:
         ops._check()
         self.shapes = {}
 
-    def submit(self, kind, n, seed=20240611, tol=1e-10, max_iters=100000):
-        r = SolveReq(kind, n, seed, tol, max_iters)
+    def submit(self, kind, n, seed=20240611, tol=1e-10, max_iters=100000, mixed=False):
+        """mixed: FP32-inner refinement (2D Poisson)."""
+        r = SolveReq(kind, n, seed, tol, max_iters, 1 if mixed else 0)
         i = lib().opsmi_serve_submit(self.h, C.byref(r))
         ops._check()
         self.shapes[i] = (n, n) if kind == POISSON2D else (n, n, n)
@@ -81,7 +82,7 @@ class Server:
         buf = (SolveDone * cap)()
         k = lib().opsmi_serve_completed(self.h, buf, cap)
         ops._check()
-        return [dict(id=b.id, status=b.status, iters=b.iters, rr0=b.rr0, rr=b.rr) for b in buf[:k]]
+        return [dict(id=b.id, status=b.status, iters=b.iters, outer=b.outer, rr0=b.rr0, rr=b.rr) for b in buf[:k]]
 
     def fetch(self, i):
         out = np.zeros(self.shapes[i])

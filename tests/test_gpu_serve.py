@@ -111,6 +111,39 @@ def test_admission_respects_budget(serve):
         assert c["status"] == 1 and np.sqrt(c["rr"]) <= 1e-10 * np.sqrt(c["rr0"])
 
 
+@pytest.mark.parametrize("n", [60, 150])
+def test_mixed_poisson_matches_oracle(serve, n):
+    # FP32-inner refinement in the batched engine: the oracle's refinement
+    # loop (orc_poisson_mixed), iteration counts and solution
+    S = serve
+    srv = S.Server(budget_bytes=1 << 28, max_slots=4, iters_per_launch=8)
+    i = srv.submit(S.POISSON2D, n, seed=SEED, mixed=True)
+    done, x = srv.run()[i]
+    srv.close()
+    f = O.poisson_rhs(SEED, 2, n, n)
+    ref = O.poisson_mixed(2, n, n, 1, f, tol=1e-10, nthreads=8)
+    assert done["status"] == 1
+    assert done["outer"] == ref["outer"] and done["iters"] == ref["inner_total"]
+    assert np.max(np.abs(x - ref["x"])) <= 1e-12 * np.max(np.abs(ref["x"]))
+    assert np.sqrt(done["rr"]) <= 1e-10 * np.sqrt(done["rr0"])
+
+
+def test_mixed_and_fp64_in_one_batch(serve):
+    S = serve
+    reqs = [(0, 70, 31, True), (1, 21, 32, False), (0, 130, 33, True), (0, 90, 34, False)]
+    srv = S.Server(budget_bytes=1 << 28, max_slots=8, iters_per_launch=8)
+    ids = [srv.submit(k, n, seed=s, mixed=m) for k, n, s, m in reqs]
+    out = srv.run()
+    srv.close()
+    for i, (k, n, s, m) in zip(ids, reqs):
+        a = S.Server(budget_bytes=1 << 28, max_slots=2, iters_per_launch=5)
+        j = a.submit(k, n, seed=s, mixed=m)
+        alone = a.run()[j]
+        a.close()
+        assert alone[0]["iters"] == out[i][0]["iters"] and alone[0]["outer"] == out[i][0]["outer"]
+        assert np.array_equal(alone[1], out[i][1]), (k, n, s, m)
+
+
 def test_autotune_picks_candidates_and_solves_stay_exact(serve):
     S = serve
     srv = S.Server(budget_bytes=1 << 30, max_slots=8, iters_per_launch=8)
@@ -142,6 +175,8 @@ def test_max_iters_and_errors(serve):
     assert out[i][0]["status"] == 2 and out[i][0]["iters"] == 6
     with pytest.raises(ops.OPSError):
         srv.submit(7, 10)
+    with pytest.raises(ops.OPSError):
+        srv.submit(S.HEAT3D, 10, mixed=True)  # FP32-inner mode is for 2D Poisson
     with pytest.raises(ops.OPSError):
         srv.submit(S.HEAT3D, 600)  # larger than the budget
     with pytest.raises(ops.OPSError):
