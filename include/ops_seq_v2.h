@@ -1,0 +1,49 @@
+/*
+ * ops_seq_v2.h — the OPS developer-API entry header (the name apps include,
+ * ops/c/include/ops_seq_v2.h) for the MI355X backend.  An app written
+ * against the reference's dev API includes this file unchanged and links
+ * ops_amd/libopsmi.so.  On top of include/ops_mi355x.h it provides:
+ *
+ *   ops_par_loop(kernel, name, block, dim, range, args...)
+ *       forwarded to the hand-written ops_par_loop_<kernel> of the same
+ *       kernel name — the rewrite ops_translator applies to an app
+ *       (ops_translator/resources/templates/cpp/loop_host.cpp.j2:12-21), so
+ *       an app may use the kernels this backend provides;
+ *   ACC<T>
+ *       the accessor the app's own kernel definitions are written with
+ *       (ops/c/include/ops_lib_core.h:1356-2038: x fastest), so those
+ *       definitions compile; they are host code the GPU loops replace;
+ *   ops_execute(instance), ops_timing_output(std::ostream &)
+ *       the reference's C++ signatures over the single instance;
+ *   ops_checkpointing_init(...)
+ *       checkpointing is not part of this backend: returns false, as the
+ *       reference does when checkpointing is not enabled.
+ */
+#ifndef OPS_MI355X_SEQ_V2_H
+#define OPS_MI355X_SEQ_V2_H
+
+#include <iostream>
+
+#include "ops_mi355x.h"
+
+#define ops_par_loop(kernel, ...) ops_par_loop_##kernel(__VA_ARGS__)
+
+template <class T>
+class ACC {
+ public:
+  ACC(T *p = nullptr, long sx = 0, long sy = 0) : p_(p), sx_(sx), sy_(sy) {}
+  T &operator()(int x, int y = 0, int z = 0) const { return p_[x + sx_ * (y + sy_ * (long)z)]; }
+
+ private:
+  T *p_;
+  long sx_, sy_;
+};
+
+static inline void ops_execute(void *instance) {
+  (void)instance;
+  ops_execute();
+}
+static inline void ops_timing_output(std::ostream &) { ops_timing_output_stdout(); }
+static inline bool ops_checkpointing_init(const char *, double, int) { return false; }
+
+#endif

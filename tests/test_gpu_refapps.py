@@ -1,0 +1,48 @@
+"""The reference's own apps, unchanged, on this backend: the Laplace tutorial
+(apps/c/laplace2d_tutorial/step7/laplace2d.cpp) and the multi-block Poisson
+test app (apps/c/poisson/poisson.cpp), compiled against include/ops_seq_v2.h
+and linked with libopsmi.so (oracle/Makefile.ref, in the build container).
+Each prints its own known-answer check; the tests read it.  Skipped where
+the binaries were not built (no reference sources at build time)."""
+import os
+import re
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF = os.path.join(ROOT, "oracle", "_ref")
+
+
+def _run(name, *args):
+    exe = os.path.join(REF, name)
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    out = subprocess.run([exe, *args], capture_output=True, text=True, timeout=120, cwd=REF)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    return out.stdout
+
+
+def test_laplace_tutorial_app_passes_its_known_answer():
+    out = _run("app_laplace2d_opsmi")
+    assert "This run is considered PASSED" in out, out[-2000:]
+    pct = float(re.search(r"within ([0-9.eE+-]+) %", out).group(1))
+    assert pct < 1e-9  # the app accepts 1e-3 %
+
+
+def test_multiblock_poisson_app_passes_its_known_answer():
+    out = _run("app_poisson_opsmi")
+    assert "This run is considered PASSED" in out, out[-2000:]
+    err = float(re.search(r"Total error: ([0-9.eE+-]+)", out).group(1))
+    assert abs(err - 16.5173570176708) <= 1e-12 * 16.5173570176708
+
+
+def test_multiblock_poisson_app_other_sizes():
+    from oracle import oracle_py as O
+    g, m = O.load_golden()
+    for (nx, ny, it), ref in zip(m["mb_poisson"]["configs"], g["mb_poisson_err"]):
+        out = _run("app_poisson_opsmi", f"-sizex={nx}", f"-sizey={ny}", f"-iters={it}")
+        err = float(re.search(r"Total error: ([0-9.eE+-]+)", out).group(1))
+        assert abs(err - ref) <= 1e-12 * ref, (nx, ny, it)
