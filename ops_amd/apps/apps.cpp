@@ -626,18 +626,21 @@ double opsapp_mb_poisson(int nx, int ny, int ngx, int ngy, int iters, double *u_
     for (int i = 0; i < ngx; i++) {
       const int b = i + ngx * j;
       int dir[] = {1, 2};
+      // the reverse halo lands in the neighbour's far halo, at the
+      // neighbour's size (the reference app uses the sending block's size,
+      // poisson.cpp:158,167, which is the same only for equal blocks)
       if (i > 0) {
         int iter[] = {1, sizes[2 * b + 1]}, from[] = {sizes[2 * (b - 1)] - 1, 0}, to[] = {-1, 0};
         halos.push_back(ops_decl_halo(u[b - 1], u[b], iter, from, to, dir, dir));
         from[0] = 0;
-        to[0] = sizes[2 * b];
+        to[0] = sizes[2 * (b - 1)];
         halos.push_back(ops_decl_halo(u[b], u[b - 1], iter, from, to, dir, dir));
       }
       if (j > 0) {
         int iter[] = {sizes[2 * b], 1}, from[] = {0, sizes[2 * (b - ngx) + 1] - 1}, to[] = {0, -1};
         halos.push_back(ops_decl_halo(u[b - ngx], u[b], iter, from, to, dir, dir));
         from[1] = 0;
-        to[1] = sizes[2 * b + 1];
+        to[1] = sizes[2 * (b - ngx) + 1];
         halos.push_back(ops_decl_halo(u[b], u[b - ngx], iter, from, to, dir, dir));
       }
     }
