@@ -148,7 +148,7 @@ def _b(s):
 class Dat:
     """Handle of an ops_dat with its numpy dtype and interior shape."""
 
-    def __init__(self, ptr, block, size, d_m, d_p, dtype, name):
+    def __init__(self, ptr, block, size, d_m, d_p, dtype, name, dim=1):
         self.ptr = ptr
         self.block = block
         self.size = list(size)
@@ -156,6 +156,7 @@ class Dat:
         self.d_p = list(d_p)
         self.dtype = np.dtype(dtype)
         self.name = name
+        self.dim = dim
 
     @property
     def core(self):
@@ -190,7 +191,7 @@ def ops_decl_block(dims, name):
     return Block(_call("ops_decl_block", dims, _b(name)), dims, name)
 
 
-def ops_decl_dat(block, size, d_m=None, d_p=None, dtype=np.float64, name="dat", data=None, base=None):
+def ops_decl_dat(block, size, d_m=None, d_p=None, dtype=np.float64, name="dat", data=None, base=None, dim=1):
     nd = block.dims
     d_m = d_m if d_m is not None else [-1] * nd
     d_p = d_p if d_p is not None else [1] * nd
@@ -200,9 +201,9 @@ def ops_decl_dat(block, size, d_m=None, d_p=None, dtype=np.float64, name="dat", 
     if data is not None:
         data = np.ascontiguousarray(data, dtype=dt)
         ptr = data.ctypes.data
-    p = _call("ops_decl_dat_char", block.ptr, 1, _ints(size), _ints(base), _ints(d_m), _ints(d_p),
+    p = _call("ops_decl_dat_char", block.ptr, dim, _ints(size), _ints(base), _ints(d_m), _ints(d_p),
               _ints([1] * nd), ptr, dt.itemsize, _b(_TYPES[dt]), _b(name))
-    return Dat(p, block, size, d_m, d_p, dt, name)
+    return Dat(p, block, size, d_m, d_p, dt, name, dim)
 
 
 def ops_free_dat(dat):
@@ -282,7 +283,7 @@ def ops_dat_set_data(dat, arr):
 
 def ops_dat_fetch_data_slab(dat, rng):
     nd = dat.block.dims
-    shape = tuple(rng[2 * d + 1] - rng[2 * d] for d in reversed(range(nd)))
+    shape = tuple(rng[2 * d + 1] - rng[2 * d] for d in reversed(range(nd))) + ((dat.dim,) if dat.dim > 1 else ())
     out = np.empty(shape, dtype=dat.dtype)
     _call("ops_dat_fetch_data_slab_memspace", dat.ptr, 0, out.ctypes.data, _ints(rng), OPS_HOST)
     return out
@@ -308,6 +309,38 @@ def get_ext(dat):
     for d in range(nd):
         rng += [-1, dat.size[d] + 1]
     return ops_dat_fetch_data_slab(dat, rng)
+
+
+def _whole_range(dat):
+    rng = []
+    for d in range(dat.block.dims):
+        rng += [dat.d_m[d], dat.size[d] + dat.d_p[d]]
+    return rng
+
+
+def set_whole(dat, arr):
+    """Uploads the dat's whole array: halos of depth -d_m / d_p included,
+    indexed [z, y, x(, component)]."""
+    ops_dat_set_data_slab(dat, _whole_range(dat), arr)
+
+
+def get_whole(dat):
+    """Downloads the dat's whole array (halos included), as set_whole."""
+    return ops_dat_fetch_data_slab(dat, _whole_range(dat))
+
+
+# ---------------------------------------------------------------- halos
+def ops_decl_halo(frm, to, iter_size, from_base, to_base, from_dir, to_dir):
+    return _call("ops_decl_halo", frm.ptr, to.ptr, _ints(iter_size), _ints(from_base), _ints(to_base),
+                 _ints(from_dir), _ints(to_dir))
+
+
+def ops_decl_halo_group(halos):
+    return _call("ops_decl_halo_group", len(halos), (C.c_void_p * len(halos))(*halos))
+
+
+def ops_halo_transfer(group):
+    _call("ops_halo_transfer", group)
 
 
 def ops_randomgen_init(seed, options=0):

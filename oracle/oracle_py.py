@@ -249,3 +249,95 @@ def load_golden():
     g = {k: z[k] for k in z.files}
     meta = json.loads(bytes(g.pop("meta_json")).decode())
     return g, meta
+
+
+# ------------------------------------------------------------ halo transfer
+class HaloDat:
+    """A dat for the halo-transfer restatement: its whole array ``a`` (halos
+    included) indexed [z, y, x, component] from the lowest halo point, the
+    negative halo depths ``d_m`` (<= 0) and the element type."""
+
+    def __init__(self, a, d_m, ndim):
+        self.a = a
+        self.d_m = list(d_m) + [0] * (3 - len(d_m))
+        self.ndim = ndim
+
+
+def _halo_ranges(dat, base, dirs, iters):
+    # ranges/step/buffer strides of ops_host_singlenode.cpp:201-219 (source)
+    # and :327-342 (target), in raw coordinates of the dat's array
+    lo, hi, step, bstr = [], [], [], []
+    for i in range(3):
+        q = abs(dirs[i]) - 1
+        if dirs[i] > 0:
+            r0 = base[i] - dat.d_m[i]
+            r1 = r0 + iters[q]
+            step.append(1)
+        else:
+            r1 = base[i] - 1 - dat.d_m[i]
+            r0 = r1 + iters[q]
+            step.append(-1)
+        lo.append(r0)
+        hi.append(r1)
+        s = 1
+        for j in range(q):
+            s *= iters[j]
+        bstr.append(s)
+    return lo, hi, step, bstr
+
+
+def _halo_points(lo, hi):
+    # the reference's loops: min(r0, r1 + 1) <= idx < max(r0 + 1, r1)
+    rng = [range(min(lo[i], hi[i] + 1), max(lo[i] + 1, hi[i])) for i in range(3)]
+    for k in rng[2]:
+        for j in rng[1]:
+            for i in rng[0]:
+                yield i, j, k
+
+
+def halo_transfer(frm, to, iter_size, from_base, to_base, from_dir, to_dir):
+    """One halo of ops_halo_transfer (ops/c/src/sequential/
+    ops_host_singlenode.cpp:172-446), in place on ``to.a``: the source box is
+    packed into a linear buffer (buffer dimension q has extent iter_size[q],
+    source dimension i walks buffer dimension |from_dir[i]| - 1, backwards
+    when from_dir[i] < 0), then the buffer is unpacked into the target box the
+    same way.  Entries beyond the block's dimensions are 1 / 0 / i + 1, as
+    ops_decl_halo_core sets them (ops/c/src/core/ops_lib_core.cpp:1023-1029).
+    When a float and a double dat meet, values pass through the narrower type
+    (:225-228).  The reference places component d of buffer point pos at
+    pos + d in that mixed case and at pos * dim + d otherwise; the two agree
+    for one component, the only mixed case the fixtures exercise."""
+    nd = frm.ndim
+    pad = lambda v, fill: list(v[:nd]) + [fill(i) for i in range(nd, 3)]  # noqa: E731
+    it = pad(iter_size, lambda i: 1)
+    fb, tb = pad(from_base, lambda i: 0), pad(to_base, lambda i: 0)
+    fd, td = pad(from_dir, lambda i: i + 1), pad(to_dir, lambda i: i + 1)
+    dim = frm.a.shape[3]
+    storage = np.float32 if np.float32 in (frm.a.dtype.type, to.a.dtype.type) else np.float64
+    buf = np.zeros(it[0] * it[1] * it[2] * dim, dtype=storage)
+    lo, hi, st, bs = _halo_ranges(frm, fb, fd, it)
+    for p in _halo_points(lo, hi):
+        pos = sum((p[i] - lo[i]) * st[i] * bs[i] for i in range(3))
+        for d in range(dim):
+            buf[pos * dim + d] = storage(frm.a[p[2], p[1], p[0], d])
+    lo, hi, st, bs = _halo_ranges(to, tb, td, it)
+    for p in _halo_points(lo, hi):
+        pos = sum((p[i] - lo[i]) * st[i] * bs[i] for i in range(3))
+        for d in range(dim):
+            to.a[p[2], p[1], p[0], d] = buf[pos * dim + d]
+
+
+def halo_whole_shape(ndim, size, d_m, d_p, dim):
+    """Shape [z, y, x, component] of a dat's whole array (halos included)."""
+    e = [size[i] - d_m[i] + d_p[i] if i < ndim else 1 for i in range(3)]
+    return (e[2], e[1], e[0], dim)
+
+
+def halo_fixture(g, case):
+    """The reference's arrays for one halo-transfer case of the fixtures:
+    {dat name: [whole array before any group, after group 1, ...]}."""
+    out = {}
+    for name, typ, dim, size, d_m, d_p in case["dats"]:
+        shp = halo_whole_shape(case["ndim"], size, d_m, d_p, dim)
+        out[name] = [g[f"halo_{case['name']}_{name}_g{k}"].reshape(shp) for k in range(len(case["groups"]) + 1)]
+    return out

@@ -38,6 +38,100 @@ META = {
     "mb_poisson": {"configs": [[20, 20, 10], [97, 61, 40], [64, 50, 200], [33, 47, 25], [200, 130, 100]],
                    "expected_app": 16.5173570176708},
 }
+# ops_halo_transfer cases for oracle/refdrv/ref_halo.cpp.  dats: name, type,
+# components, size, d_m, d_p; groups: lists of halos (from, to, iter_size,
+# from_base, to_base, from_dir, to_dir), transferred in order.  "mblock" is
+# the five halo groups of the reference's multi-block app
+# (apps/c/mblock/mblock.cpp:60-157) on its 20 x 20 dats with depth-2 halos.
+_MB = (1, [20, 20], [-2, -2], [2, 2])
+HALO_CASES = [
+    {"name": "mblock", "ndim": 2,
+     "dats": [["data0", "double", *_MB], ["data1", "double", *_MB]],
+     "groups": [
+         [["data0", "data1", [2, 20], [18, 0], [-2, 0], [1, 2], [1, 2]],
+          ["data1", "data0", [2, 20], [0, 0], [20, 0], [1, 2], [1, 2]]],
+         [["data0", "data1", [20, 2], [0, 18], [0, -2], [1, 2], [1, 2]],
+          ["data1", "data0", [20, 2], [0, 0], [0, 20], [1, 2], [1, 2]]],
+         [["data0", "data1", [2, 20], [0, 0], [20, 0], [1, 2], [1, -2]],
+          ["data1", "data0", [2, 20], [18, 0], [-2, 0], [1, -2], [1, 2]]],
+         [["data0", "data1", [20, 2], [0, 0], [0, 20], [1, 2], [-1, 2]],
+          ["data1", "data0", [20, 2], [0, 18], [0, -2], [-1, 2], [1, 2]]],
+         [["data0", "data1", [2, 20], [18, 0], [0, -2], [1, 2], [2, 1]],
+          ["data1", "data0", [2, 20], [0, 0], [20, 0], [2, 1], [1, 2]]],
+     ]},
+    # 3D, unequal halos, permuted and flipped axes; then a self-transfer
+    # whose source and target rows overlap (all values are packed first)
+    {"name": "perm3d", "ndim": 3,
+     "dats": [["A", "double", 1, [7, 5, 4], [-1, -2, -1], [2, 1, 1]],
+              ["B", "double", 1, [5, 4, 7], [-2, -1, -1], [1, 1, 2]]],
+     "groups": [
+         [["A", "B", [3, 2, 4], [2, 1, 0], [-2, 0, 4], [1, 2, 3], [2, 3, -1]],
+          ["B", "A", [2, 3, 2], [0, -1, 5], [-1, -2, 2], [-1, 2, -3], [3, 1, 2]]],
+         [["A", "A", [7, 1, 1], [0, 0, 0], [0, -1, 0], [1, 2, 3], [-1, 2, 3]],
+          ["A", "A", [7, 2, 4], [0, 0, 0], [0, 1, 0], [1, 2, 3], [1, 2, 3]]],
+     ]},
+    # two components per point
+    {"name": "multi", "ndim": 2,
+     "dats": [["P", "double", 2, [6, 5], [-1, -1], [1, 1]], ["Q", "double", 2, [6, 5], [-1, -1], [1, 1]]],
+     "groups": [
+         [["P", "Q", [1, 5], [5, 0], [-1, 0], [1, 2], [1, -2]],
+          ["Q", "P", [6, 1], [0, 0], [0, 5], [1, 2], [-1, 2]]],
+     ]},
+    # float and double dats meeting: values pass through float
+    {"name": "mixed", "ndim": 2,
+     "dats": [["D", "double", 1, [8, 6], [-1, -1], [1, 1]], ["F", "float", 1, [8, 6], [-1, -1], [1, 1]]],
+     "groups": [
+         [["D", "F", [8, 1], [0, 5], [0, -1], [1, 2], [1, 2]],
+          ["F", "D", [1, 6], [7, 0], [8, 0], [1, 2], [1, -2]]],
+         [["D", "F", [2, 3], [1, 1], [3, 2], [1, 2], [2, 1]]],
+     ]},
+    # rows of more than 1 KiB (x-padded layouts), x of one dat against y of
+    # the other, flipped on the way back
+    {"name": "wide", "ndim": 2,
+     "dats": [["W1", "double", 1, [140, 3], [-1, -1], [1, 1]], ["W2", "double", 1, [3, 140], [-1, -1], [1, 1]]],
+     "groups": [
+         [["W1", "W2", [140, 1], [0, 2], [-1, 0], [1, 2], [2, 1]],
+          ["W2", "W1", [1, 140], [2, 0], [0, 3], [1, -2], [2, 1]]],
+     ]},
+]
+META["halo"] = HALO_CASES
+
+
+def halo_spec_text(cases):
+    """The cases in the line format oracle/refdrv/ref_halo.cpp reads."""
+    pad3 = lambda v, fill: list(v) + [fill] * (3 - len(v))  # noqa: E731
+    out = []
+    for c in cases:
+        out.append(f"case {c['name']} {c['ndim']}")
+        for name, typ, dim, size, dm, dp in c["dats"]:
+            out.append(" ".join(map(str, ["dat", name, typ, dim, *pad3(size, 1), *pad3(dm, 0), *pad3(dp, 0)])))
+        for grp in c["groups"]:
+            out.append("group")
+            for f, t, it, fb, tb, fd, td in grp:
+                out.append(" ".join(map(str, ["halo", f, t, *pad3(it, 1), *pad3(fb, 0), *pad3(tb, 0),
+                                              *pad3(fd, 3), *pad3(td, 3)])))
+        out.append("end")
+    return "\n".join(out) + "\n"
+
+
+def halo_arrays(tmp):
+    """Runs ref_halo on HALO_CASES; returns {halo_<case>_<dat>_g<k>: array}."""
+    spec = os.path.join(tmp, "halo_spec.txt")
+    with open(spec, "w") as f:
+        f.write(halo_spec_text(HALO_CASES))
+    hdir = os.path.join(tmp, "halo")
+    os.makedirs(hdir)
+    subprocess.run([os.path.join(ROOT, "oracle/_ref/ref_halo"), spec, hdir], check=True)
+    types = {(c["name"], d[0]): d[1] for c in HALO_CASES for d in c["dats"]}
+    arrays = {}
+    for fn in sorted(os.listdir(hdir)):
+        key = fn[:-4]
+        case, dat = key.split("_")[1:3]
+        dt = np.float32 if types[(case, dat)] == "float" else np.float64
+        arrays[key] = np.fromfile(os.path.join(hdir, fn), dtype=dt)
+    return arrays
+
+
 F32 = {"laplace_f32_err", "laplace_f32_A", "laplace_f32_init", "poisson3d_f32_w"}
 SKIP = {"heat3d_Kz"}
 
@@ -63,6 +157,7 @@ def main():
             line = [ln for ln in out.splitlines() if ln.startswith("Total error:")][0]
             errs.append(float(line.split(":")[1]))
         arrays["mb_poisson_err"] = np.array(errs)
+        arrays.update(halo_arrays(tmp))
     arrays["meta_json"] = np.frombuffer(json.dumps(META).encode(), dtype=np.uint8)
     np.savez_compressed(OUT, **arrays)
     print("wrote", OUT, "with", len(arrays), "arrays")
