@@ -91,19 +91,29 @@ def line(value, unit, ms, a, dtype, config, **extra):
 
 
 # ------------------------------------------------------------------ heat CG
-def cpu_baseline_heat(n, seed, iters=3):
+def host_threads():
+    """Threads for the CPU baseline: the cores this process may run on, at
+    most 16 (the host share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline_heat(n, seed, target_s=10.0):
     """Oracle (OpenMP restatement of the reference CPU path) on the same
-    problem: times `iters` CG iterations as the difference of two runs."""
+    problem: CG iterations timed by the oracle around its iteration loop
+    (set-up excluded); a 3-iteration probe sizes the sample to about
+    target_s seconds (3 to 200 iterations)."""
     from oracle import oracle_py as O
 
-    nt = os.cpu_count() or 1
+    nt = host_threads()
     K, rhs = O.heat_inputs(seed, 3, n, n, n)
-    t = []
-    for k in (1, 1 + iters):
-        t0 = time.perf_counter()
-        O.heat_cg(3, n, n, n, K, rhs, max_iters=k, nthreads=nt)
-        t.append(time.perf_counter() - t0)
-    dt = max(t[1] - t[0], 1e-9) / iters
+    O.heat_cg(3, n, n, n, K, rhs, max_iters=3, nthreads=nt)
+    iters = int(min(200, max(3, target_s / max(O.last_loop_seconds() / 3, 1e-6))))
+    O.heat_cg(3, n, n, n, K, rhs, max_iters=iters, nthreads=nt)
+    dt = max(O.last_loop_seconds(), 1e-9) / iters
     gbs = 112.0 * n ** 3 / dt / 1e9
     del K, rhs
     return {"value": round(gbs, 2), "unit": "GB/s", "cores": nt, "kind": "port",

@@ -1,14 +1,14 @@
 // z-marching stencil columns with V points per lane (16-byte vector loads).
 //
-// A 64 x 4 workgroup owns a (64 V) x 4 column of the grid over planes
-// [z0, z1); lane l of row r holds points x0 + V l ... x0 + V l + V - 1 of row
-// y0 + r.  Each thread keeps its points' p(z-1), p(z), p(z+1) (and, for the
+// A 64 x WY workgroup (WY = 4 rows, or kHeatRows for the heat columns) owns
+// a (64 V) x WY column of the grid over planes [z0, z1); lane l of row r
+// holds points x0 + V l ... x0 + V l + V - 1 of row y0 + r.  Each thread keeps its points' p(z-1), p(z), p(z+1) (and, for the
 // heat operator, Kz(z), Kz(z+1)) in registers, so every value is loaded from
 // memory once per plane, as one 16-byte load per lane.  The plane's p (and
 // Kx, Ky) go to LDS with the rim the stencil needs, double-buffered with one
 // barrier per plane; x neighbours inside a lane come from registers, the
-// two at a lane's ends and the y neighbours from LDS.  Next-plane loads are
-// issued before the barrier.  Outputs are written with non-temporal stores
+// two at a lane's ends and the y neighbours from LDS.  Loads of the next
+// plane (or the next PF planes) are issued before the barrier.  Outputs are written with non-temporal stores
 // when NT (the next loop reads them from HBM anyway at the sizes this path
 // is for).  Lanes past the range end keep to the barriers; a lane with fewer
 // than V points in range loads and stores them one by one, and the lane
@@ -51,12 +51,12 @@ struct ColLane {
   int lye;   // last row in range
   bool hxl, hxh, hyl, hyh;
 };
-template <int V>
+template <int V, int WY = kWavesPerBlock>
 __device__ __forceinline__ ColLane col_lane(int x0, int y0, int xe, int ye) {
   ColLane c;
   const int lx = (int)threadIdx.x, ly = (int)threadIdx.y;
   c.cnt = min(kWave * V, xe - x0);
-  c.lye = min(kWavesPerBlock, ye - y0) - 1;
+  c.lye = min(WY, ye - y0) - 1;
   c.n = ly <= c.lye ? max(0, min(V, c.cnt - V * lx)) : 0;
   c.hxl = lx == 0;
   c.hxh = lx == (c.cnt - 1) / V;
@@ -66,93 +66,142 @@ __device__ __forceinline__ ColLane col_lane(int x0, int y0, int xe, int ye) {
 }
 
 // ------------------------------------------------------------ heat (7-point, variable coefficient, double)
-template <int V>
+// The heat columns are kHeatRows rows tall (512-thread workgroups): against
+// 4-row tiles the y rims are half as many re-read rows per point, and at
+// 512^3 the kernel measured 987 us against 1060 us.
+constexpr int kHeatRows = 8;
+
+template <int V, int WY = kWavesPerBlock>
 struct HeatLdsV {
-  __attribute__((aligned(16))) double sp[2][kWavesPerBlock + 2][kWave * V + 2 * V];
-  __attribute__((aligned(16))) double skx[2][kWavesPerBlock][kWave * V + 2 * V];
-  __attribute__((aligned(16))) double sky[2][kWavesPerBlock + 1][kWave * V];
+  __attribute__((aligned(16))) double sp[2][WY + 2][kWave * V + 2 * V];
+  __attribute__((aligned(16))) double skx[2][WY][kWave * V + 2 * V];
+  __attribute__((aligned(16))) double sky[2][WY + 1][kWave * V];
+};
+
+// Launch grid of a heat column kernel over r, tiles of tile_x x rows points.
+// Planes per workgroup: OPSMI_HEAT_ZCHUNK when set, otherwise the longest
+// power of two in [8, 64] that leaves at least 1024 workgroups (4 per CU);
+// each chunk re-reads two planes of p and one of Kz, so longer is cheaper
+// (512^3: 64 planes, 987 us against 1003 us with 32 and 1034 us with 16).
+struct ColGrid {
+  unsigned gx, gy, gz;
+  int chunk;
+  unsigned blocks() const { return gx * gy * gz; }
+};
+inline ColGrid heat_column_grid(const Range3 &r, int tile_x, int rows) {
+  ColGrid g;
+  const long nx = r.x1 - r.x0, ny = r.y1 - r.y0, nz = r.z1 - r.z0;
+  g.gx = (unsigned)((nx + tile_x - 1) / tile_x);
+  g.gy = (unsigned)((ny + rows - 1) / rows);
+  if (getenv("OPSMI_HEAT_ZCHUNK")) {
+    g.chunk = heat_zchunk_default();
+  } else {
+    g.chunk = 64;
+    while (g.chunk > 8 && (long)g.gx * g.gy * ((nz + g.chunk - 1) / g.chunk) < 1024) g.chunk >>= 1;
+  }
+  g.gz = (unsigned)((nz + g.chunk - 1) / g.chunk);
+  return g;
+}
+
+// one plane of a heat column as a lane loads it: its V points of p, Kz, Kx,
+// Ky and the rims it owns
+template <int V>
+struct HeatPlane {
+  double p[V], kz[V], kx[V], ky[V], ryl[V], ryh[V], rky[V];
+  double rxl, rxh, rkx;
 };
 
 // TeaLeaf 3D operator (apps/c/TeaLeaf/tea_leaf_cg_kernels.h:8-21 with the z
-// pair after the x pair), arithmetic in the reference order.  INIT: w = A p,
-// r = p - w; otherwise w = A p and the return value is this thread's sum of
-// w p.
-template <int V, bool INIT, bool NT>
-__device__ __forceinline__ double heat3_column_v(HeatLdsV<V> &L, const Ptr<double> &w, const Ptr<double> &r,
-                                                 const Ptr<const double> &Kx, const Ptr<const double> &Ky,
-                                                 const Ptr<const double> &Kz, const Ptr<const double> &p, double rx,
-                                                 double ry, double rz, int x0, int y0, int xe, int ye, int z0,
-                                                 int z1) {
-  double acc = 0.0;
-  const int lx = (int)threadIdx.x, ly = (int)threadIdx.y;
-  const ColLane c = col_lane<V>(x0, y0, xe, ye);
-  const int n = c.n;
-  const bool act = n > 0;
-  const int x = x0 + V * lx, y = y0 + ly;
-  const int o = V + V * lx;  // LDS index of this lane's first point
-  double pm[V], pc[V], pn[V], kzc[V], kzn[V], kxc[V], kyc[V];
-  double ryl[V], ryh[V], rky[V];
-  double rxl = 0, rxh = 0, rkx = 0;
-#pragma unroll
-  for (int j = 0; j < V; j++) pm[j] = pc[j] = pn[j] = kzc[j] = kzn[j] = kxc[j] = kyc[j] = ryl[j] = ryh[j] = rky[j] = 0;
-  if (act) {
-    ldv<double, V>(p.at(x, y, z0 - 1), n, pm);
-    ldv<double, V>(p.at(x, y, z0), n, pc);
-    ldv<double, V>(p.at(x, y, z0 + 1), n, pn);
-    ldv<double, V>(Kz.at(x, y, z0), n, kzc);
-    ldv<double, V>(Kz.at(x, y, z0 + 1), n, kzn);
-    ldv<double, V>(Kx.at(x, y, z0), n, kxc);
-    ldv<double, V>(Ky.at(x, y, z0), n, kyc);
-    if (c.hxl) rxl = p(x0 - 1, y, z0);
+// pair after the x pair), arithmetic in the reference order, over one column
+// tile.  INIT: w = A p, r = p - w; otherwise w = A p and run() returns this
+// thread's sum of w p.  Loads run PF planes ahead of the plane being
+// computed, so a lane has PF planes of loads in flight while it waits at the
+// barrier; the PF + 2 planes held in registers form a ring indexed by the
+// step number modulo PF + 2, and the z loop is unrolled by the ring length
+// so every index is a compile-time constant (no register moves).  Only p
+// and Kz are read at plane z1 (the loop's stencils guarantee those, not the
+// rest).  FULL: the tile lies inside the range (every lane has V points, all
+// rows active), which drops the per-lane partial-load paths.
+template <int V, bool INIT, bool NT, int WY, int PF, bool FULL>
+struct HeatColumn {
+  static constexpr int RING = PF + 2;
+  HeatLdsV<V, WY> &L;
+  const Ptr<double> &w, &r;
+  const Ptr<const double> &Kx, &Ky, &Kz, &p;
+  double rx, ry, rz;
+  int x0, z0, z1;
+  ColLane c;
+  int x, y, o, lx, ly;
+  double acc;
+  double pm[V];
+  // (one rim slot shared by the low and high rims of a full tile saves 10
+  // VGPRs but makes the compiler wait for all loads before refilling it)
+  using Plane = HeatPlane<V>;
+  Plane R[RING];
+
+  __device__ __forceinline__ HeatColumn(HeatLdsV<V, WY> &L_, const Ptr<double> &w_, const Ptr<double> &r_,
+                                        const Ptr<const double> &Kx_, const Ptr<const double> &Ky_,
+                                        const Ptr<const double> &Kz_, const Ptr<const double> &p_, double rx_,
+                                        double ry_, double rz_, int x0_, int y0, int xe, int ye, int z0_, int z1_)
+      : L(L_), w(w_), r(r_), Kx(Kx_), Ky(Ky_), Kz(Kz_), p(p_), rx(rx_), ry(ry_), rz(rz_), x0(x0_), z0(z0_), z1(z1_) {
+    lx = (int)threadIdx.x;
+    ly = (int)threadIdx.y;
+    c = col_lane<V, WY>(x0, y0, xe, ye);
+    if (FULL) c.n = V;
+    x = x0 + V * lx;
+    y = y0 + ly;
+    o = V + V * lx;
+    acc = 0.0;
+  }
+  __device__ __forceinline__ int n() const { return FULL ? V : c.n; }
+  __device__ __forceinline__ bool act() const { return FULL || c.n > 0; }
+
+  // plane q <= z1; the plane-z1 values of Kx, Ky and the rims are never
+  // used and are read from plane z1 - 1 instead, so every step issues the
+  // same loads (the compiler then keeps them in flight across the barrier)
+  __device__ __forceinline__ void load(int q, Plane &P) const {
+    const int qk = min(q, z1 - 1);
+    ldv<double, V>(p.at(x, y, q), n(), P.p);
+    ldv<double, V>(Kz.at(x, y, q), n(), P.kz);
+    ldv<double, V>(Kx.at(x, y, qk), n(), P.kx);
+    ldv<double, V>(Ky.at(x, y, qk), n(), P.ky);
+    if (c.hxl) P.rxl = p(x0 - 1, y, qk);
     if (c.hxh) {
-      rxh = p(x0 + c.cnt, y, z0);
-      rkx = Kx(x0 + c.cnt, y, z0);
+      P.rxh = p(x0 + c.cnt, y, qk);
+      P.rkx = Kx(x0 + c.cnt, y, qk);
     }
-    if (c.hyl) ldv<double, V>(p.at(x, y - 1, z0), n, ryl);
+    if (c.hyl) ldv<double, V>(p.at(x, y - 1, qk), n(), P.ryl);
     if (c.hyh) {
-      ldv<double, V>(p.at(x, y + 1, z0), n, ryh);
-      ldv<double, V>(Ky.at(x, y + 1, z0), n, rky);
+      ldv<double, V>(p.at(x, y + 1, qk), n(), P.ryh);
+      ldv<double, V>(Ky.at(x, y + 1, qk), n(), P.rky);
     }
   }
-  for (int z = z0; z < z1; z++) {
-    const int b = (z - z0) & 1;
-    if (act) {
-      lds_st<double, V>(&L.sp[b][ly + 1][o], n, pc);
-      lds_st<double, V>(&L.skx[b][ly][o], n, kxc);
-      lds_st<double, V>(&L.sky[b][ly][V * lx], n, kyc);
-      if (c.hxl) L.sp[b][ly + 1][V - 1] = rxl;
+
+  // plane z = z0 + s with s % RING == K
+  template <int K>
+  __device__ __forceinline__ void step(int z) {
+    const Plane &C = R[K % RING], &N = R[(K + 1) % RING];
+    const int b = (z - z0) & 1, nn = n();
+    if (act()) {
+      lds_st<double, V>(&L.sp[b][ly + 1][o], nn, C.p);
+      lds_st<double, V>(&L.skx[b][ly][o], nn, C.kx);
+      lds_st<double, V>(&L.sky[b][ly][V * lx], nn, C.ky);
+      if (c.hxl) L.sp[b][ly + 1][V - 1] = C.rxl;
       if (c.hxh) {
-        L.sp[b][ly + 1][V + c.cnt] = rxh;
-        L.skx[b][ly][V + c.cnt] = rkx;
+        L.sp[b][ly + 1][V + c.cnt] = C.rxh;
+        L.skx[b][ly][V + c.cnt] = C.rkx;
       }
-      if (c.hyl) lds_st<double, V>(&L.sp[b][0][o], n, ryl);
+      if (c.hyl) lds_st<double, V>(&L.sp[b][0][o], nn, C.ryl);
       if (c.hyh) {
-        lds_st<double, V>(&L.sp[b][ly + 2][o], n, ryh);
-        lds_st<double, V>(&L.sky[b][ly + 1][V * lx], n, rky);
+        lds_st<double, V>(&L.sp[b][ly + 2][o], nn, C.ryh);
+        lds_st<double, V>(&L.sky[b][ly + 1][V * lx], nn, C.rky);
       }
-    }
-    double pnn[V], kznn[V], kxn_c[V], kyn_c[V];
-#pragma unroll
-    for (int j = 0; j < V; j++) pnn[j] = kznn[j] = kxn_c[j] = kyn_c[j] = 0;
-    if (act && z + 1 < z1) {
-      const int zn = z + 1;
-      ldv<double, V>(p.at(x, y, zn + 1), n, pnn);
-      ldv<double, V>(Kz.at(x, y, zn + 1), n, kznn);
-      ldv<double, V>(Kx.at(x, y, zn), n, kxn_c);
-      ldv<double, V>(Ky.at(x, y, zn), n, kyn_c);
-      if (c.hxl) rxl = p(x0 - 1, y, zn);
-      if (c.hxh) {
-        rxh = p(x0 + c.cnt, y, zn);
-        rkx = Kx(x0 + c.cnt, y, zn);
-      }
-      if (c.hyl) ldv<double, V>(p.at(x, y - 1, zn), n, ryl);
-      if (c.hyh) {
-        ldv<double, V>(p.at(x, y + 1, zn), n, ryh);
-        ldv<double, V>(Ky.at(x, y + 1, zn), n, rky);
-      }
+      // the slot takes plane z + PF + 1 (or, near the end, plane z1 again:
+      // the slot held plane z - 1, which is no longer needed)
+      load(min(z + PF + 1, z1), R[(K + PF + 1) % RING]);
     }
     __syncthreads();
-    if (act) {
+    if (act()) {
       double pym[V], pyp[V], kyn[V], v[V], rv[V];
       lds_ld<double, V>(&L.sp[b][ly][o], pym);
       lds_ld<double, V>(&L.sp[b][ly + 2][o], pyp);
@@ -160,30 +209,69 @@ __device__ __forceinline__ double heat3_column_v(HeatLdsV<V> &L, const Ptr<doubl
       const double left = L.sp[b][ly + 1][o - 1];
 #pragma unroll
       for (int j = 0; j < V; j++) {
-        const double pxm = j == 0 ? left : pc[j == 0 ? 0 : j - 1];
-        const double pxp = j + 1 < n ? pc[j + 1 < V ? j + 1 : j] : L.sp[b][ly + 1][o + j + 1];
-        const double kxn = j + 1 < n ? kxc[j + 1 < V ? j + 1 : j] : L.skx[b][ly][o + j + 1];
-        v[j] = (1.0 + ry * (kyn[j] + kyc[j]) + rx * (kxn + kxc[j]) + rz * (kzn[j] + kzc[j])) * pc[j] -
-               ry * (kyn[j] * pyp[j] + kyc[j] * pym[j]) - rx * (kxn * pxp + kxc[j] * pxm) -
-               rz * (kzn[j] * pn[j] + kzc[j] * pm[j]);
-        rv[j] = pc[j] - v[j];
-        if (!INIT && j < n) acc = acc + v[j] * pc[j];
+        const double pxm = j == 0 ? left : C.p[j == 0 ? 0 : j - 1];
+        const double pxp = j + 1 < nn ? C.p[j + 1 < V ? j + 1 : j] : L.sp[b][ly + 1][o + j + 1];
+        const double kxn = j + 1 < nn ? C.kx[j + 1 < V ? j + 1 : j] : L.skx[b][ly][o + j + 1];
+        v[j] = (1.0 + ry * (kyn[j] + C.ky[j]) + rx * (kxn + C.kx[j]) + rz * (N.kz[j] + C.kz[j])) * C.p[j] -
+               ry * (kyn[j] * pyp[j] + C.ky[j] * pym[j]) - rx * (kxn * pxp + C.kx[j] * pxm) -
+               rz * (N.kz[j] * N.p[j] + C.kz[j] * pm[j]);
+        rv[j] = C.p[j] - v[j];
+        if (!INIT && j < nn) acc = acc + v[j] * C.p[j];
       }
-      stv<double, V, NT>(w.at(x, y, z), n, v);
-      if constexpr (INIT) stv<double, V, NT>(r.at(x, y, z), n, rv);
+      stv<double, V, NT>(w.at(x, y, z), nn, v);
+      if constexpr (INIT) stv<double, V, NT>(r.at(x, y, z), nn, rv);
     }
 #pragma unroll
-    for (int j = 0; j < V; j++) {
-      pm[j] = pc[j];
-      pc[j] = pn[j];
-      pn[j] = pnn[j];
-      kzc[j] = kzn[j];
-      kzn[j] = kznn[j];
-      kxc[j] = kxn_c[j];
-      kyc[j] = kyn_c[j];
+    for (int j = 0; j < V; j++) pm[j] = C.p[j];
+  }
+  template <int K>
+  __device__ __forceinline__ void steps_all(int z) {
+    if constexpr (K < RING) {
+      step<K>(z);
+      steps_all<K + 1>(z + 1);
     }
   }
-  return acc;
+  template <int K>
+  __device__ __forceinline__ void steps_tail(int z) {
+    if constexpr (K < RING - 1) {
+      if (z < z1) {
+        step<K>(z);
+        steps_tail<K + 1>(z + 1);
+      }
+    }
+  }
+
+  __device__ __forceinline__ double run() {
+#pragma unroll
+    for (int i = 0; i < RING; i++) {
+#pragma unroll
+      for (int j = 0; j < V; j++)
+        R[i].p[j] = R[i].kz[j] = R[i].kx[j] = R[i].ky[j] = R[i].ryl[j] = R[i].ryh[j] = R[i].rky[j] = 0.0;
+      R[i].rxl = R[i].rxh = R[i].rkx = 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < V; j++) pm[j] = 0.0;
+    if (act()) {
+      ldv<double, V>(p.at(x, y, z0 - 1), n(), pm);
+#pragma unroll
+      for (int i = 0; i < PF + 1; i++) load(min(z0 + i, z1), R[i]);
+    }
+    int z = z0;
+    for (; z + RING <= z1; z += RING) steps_all<0>(z);
+    steps_tail<0>(z);
+    return acc;
+  }
+};
+
+template <int V, bool INIT, bool NT, int WY = kWavesPerBlock, int PF = 1>
+__device__ __forceinline__ double heat3_column_v(HeatLdsV<V, WY> &L, const Ptr<double> &w, const Ptr<double> &r,
+                                                 const Ptr<const double> &Kx, const Ptr<const double> &Ky,
+                                                 const Ptr<const double> &Kz, const Ptr<const double> &p, double rx,
+                                                 double ry, double rz, int x0, int y0, int xe, int ye, int z0,
+                                                 int z1) {
+  if (x0 + kWave * V <= xe && y0 + WY <= ye)
+    return HeatColumn<V, INIT, NT, WY, PF, true>(L, w, r, Kx, Ky, Kz, p, rx, ry, rz, x0, y0, xe, ye, z0, z1).run();
+  return HeatColumn<V, INIT, NT, WY, PF, false>(L, w, r, Kx, Ky, Kz, p, rx, ry, rz, x0, y0, xe, ye, z0, z1).run();
 }
 
 // ------------------------------------------------------------ fused CG chain
@@ -201,13 +289,13 @@ struct ChainPtrs {
   Ptr<const double> pold, r, Kx, Ky, Kz;
 };
 
-template <int V, bool AXPY>
-__device__ __forceinline__ double heat3_chain_column_v(HeatLdsV<V> &L, const ChainPtrs &c, const Range3 &R,
+template <int V, bool AXPY, int WY = kWavesPerBlock>
+__device__ __forceinline__ double heat3_chain_column_v(HeatLdsV<V, WY> &L, const ChainPtrs &c, const Range3 &R,
                                                        double alpha_u, double a, double b, double rx, double ry,
                                                        double rz, int x0, int y0, int z0, int z1) {
   double acc = 0.0;
   const int lx = (int)threadIdx.x, ly = (int)threadIdx.y;
-  const ColLane cl = col_lane<V>(x0, y0, R.x1, R.y1);
+  const ColLane cl = col_lane<V, WY>(x0, y0, R.x1, R.y1);
   const int n = cl.n;
   const bool act = n > 0;
   const int x = x0 + V * lx, y = y0 + ly;

@@ -27,15 +27,19 @@ using namespace opsmi;
 
 namespace {
 
+// Same tiles (128 x kHeatRows, heat_column_grid) as the calc_w column, so
+// the per-workgroup sums of w p, and the result, are the same bits.
 template <bool AXPY>
-__global__ __launch_bounds__(256) void k_cg_chain(ChainPtrs c, Range3 R, double alpha_u, double a, double b,
-                                                  double rx, double ry, double rz, int chunk, RedCtl ctl) {
+__global__ __launch_bounds__(kWave * kHeatRows) void k_cg_chain(ChainPtrs c, Range3 R, double alpha_u, double a,
+                                                                double b, double rx, double ry, double rz, int chunk,
+                                                                RedCtl ctl) {
   constexpr int V = 2;
-  __shared__ HeatLdsV<V> lds;
+  __shared__ HeatLdsV<V, kHeatRows> lds;
   const TileId t = tile_id();
   const int z0 = R.z0 + (int)t.z * chunk;
-  double acc[1] = {heat3_chain_column_v<V, AXPY>(lds, c, R, alpha_u, a, b, rx, ry, rz, R.x0 + (int)t.x * V * kWave,
-                                                 R.y0 + (int)t.y * kWavesPerBlock, z0, min(z0 + chunk, R.z1))};
+  double acc[1] = {heat3_chain_column_v<V, AXPY, kHeatRows>(lds, c, R, alpha_u, a, b, rx, ry, rz,
+                                                            R.x0 + (int)t.x * V * kWave, R.y0 + (int)t.y * kHeatRows,
+                                                            z0, min(z0 + chunk, R.z1))};
   reduce_block_store<double, OPS_INC, 1>(acc, ctl);
 }
 
@@ -69,13 +73,9 @@ int cg_chain_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
     if (!vec16_ok(d, R.x0)) return 0;
   if (U && !vec16_ok(U, R.x0)) ax = nullptr, U = nullptr;
   if (!use_vec() || cw.ctl.slot == nullptr) return 0;
-  const int nx = R.x1 - R.x0, ny = R.y1 - R.y0, nz = R.z1 - R.z0;
-  const int chunk = heat_zchunk_default();
   constexpr int V = 2;
-  const unsigned gx = (nx + V * kWave - 1) / (V * kWave), gy = (ny + kWavesPerBlock - 1) / kWavesPerBlock;
-  const unsigned gz = (nz + chunk - 1) / chunk;
-  // (the calc_w record sized the reduction partials for at least this many
-  // workgroups: its own grid is the same 128-wide one or the 64-wide one)
+  const ColGrid g = heat_column_grid(R, V * kWave, kHeatRows);
+  ensure_partials((size_t)g.blocks() * 8);
   ChainPtrs c;
   DatView pv = view_of(P);
   c.pnew = Ptr<double>{(double *)spare_base(P, R), pv.sy, pv.sz};
@@ -91,9 +91,9 @@ int cg_chain_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
   const RedCtl ctl = with_scratch(cw.ctl);
   const double bytes = (ax ? ax->bytes : 0.0) + bp.bytes + cw.bytes;
   run_loop(ax ? "cg_chain(axpy+axpby+calc_w)" : "cg_chain(axpby+calc_w)", bytes, [&](hipStream_t st) {
-    hipLaunchKernelGGL(ax ? k_cg_chain<true> : k_cg_chain<false>, dim3(gx, gy, gz), dim3(kWave, kWavesPerBlock), 0, st, c, R, alpha_u, a, b, rx, ry, rz, chunk,
-                       ctl);
-    launch_red_finish<double, OPS_INC, 1>(ctl, gx * gy * gz, st);
+    hipLaunchKernelGGL(ax ? k_cg_chain<true> : k_cg_chain<false>, dim3(g.gx, g.gy, g.gz), dim3(kWave, kHeatRows), 0,
+                       st, c, R, alpha_u, a, b, rx, ry, rz, g.chunk, ctl);
+    launch_red_finish<double, OPS_INC, 1>(ctl, g.blocks(), st);
   });
   (void)s;
   swap_spare(P, R);

@@ -214,19 +214,21 @@ __global__ __launch_bounds__(256) void k_heat3_march(Ptr<double> w, Ptr<double> 
   if constexpr (!INIT) reduce_block_store<double, OPS_INC, 1>(acc, ctl);
 }
 
-// Same with two points per lane (16-byte loads, 128 x 4 columns) and
-// non-temporal stores of w (and r): heat3_column_v.
+// Same with two points per lane (16-byte loads), 128 x kHeatRows columns,
+// loads two planes ahead and non-temporal stores of w (and r):
+// heat3_column_v.
 template <bool INIT>
-__global__ __launch_bounds__(256) void k_heat3_march_v(Ptr<double> w, Ptr<double> r, Ptr<const double> Kx,
-                                                       Ptr<const double> Ky, Ptr<const double> Kz,
-                                                       Ptr<const double> p, double rx, double ry, double rz, Range3 rg,
-                                                       int chunk, RedCtl ctl) {
-  __shared__ HeatLdsV<2> lds;
+__global__ __launch_bounds__(kWave * kHeatRows) void k_heat3_march_v(Ptr<double> w, Ptr<double> r,
+                                                                     Ptr<const double> Kx, Ptr<const double> Ky,
+                                                                     Ptr<const double> Kz, Ptr<const double> p,
+                                                                     double rx, double ry, double rz, Range3 rg,
+                                                                     int chunk, RedCtl ctl) {
+  __shared__ HeatLdsV<2, kHeatRows> lds;
   const TileId t = tile_id();
   const int z0 = rg.z0 + (int)t.z * chunk;
-  double acc[1] = {heat3_column_v<2, INIT, true>(lds, w, r, Kx, Ky, Kz, p, rx, ry, rz, rg.x0 + (int)t.x * 2 * kWave,
-                                                 rg.y0 + (int)t.y * kWavesPerBlock, rg.x1, rg.y1, z0,
-                                                 min(z0 + chunk, rg.z1))};
+  double acc[1] = {heat3_column_v<2, INIT, true, kHeatRows, 2>(
+      lds, w, r, Kx, Ky, Kz, p, rx, ry, rz, rg.x0 + (int)t.x * 2 * kWave, rg.y0 + (int)t.y * kHeatRows, rg.x1, rg.y1,
+      z0, min(z0 + chunk, rg.z1))};
   if constexpr (!INIT) reduce_block_store<double, OPS_INC, 1>(acc, ctl);
 }
 
@@ -251,25 +253,29 @@ LoopRec make_heat3_loop(const char *name, const ops_arg *args, int nargs, const 
     rec.launch = [](hipStream_t) {};
     return rec;
   }
-  const int nx = r.x1 - r.x0, ny = r.y1 - r.y0, nz = r.z1 - r.z0;
   bool vec = use_vec();
   for (ops_dat d : {w.dat, Kx.dat, Ky.dat, Kz.dat, p.dat}) vec = vec && vec16_ok(d, r.x0);
   if (INIT) vec = vec && vec16_ok(rr.dat, r.x0);
-  const int tile_x = vec ? 2 * kWave : kWave;
-  const unsigned gx = (nx + tile_x - 1) / tile_x, gy = (ny + kWavesPerBlock - 1) / kWavesPerBlock;
-  const int chunk = heat_zchunk();
-  const unsigned gz = (nz + chunk - 1) / chunk;
+  ColGrid g;
+  if (vec) {
+    g = heat_column_grid(r, 2 * kWave, kHeatRows);
+  } else {
+    g.chunk = heat_zchunk();
+    g.gx = (unsigned)((r.x1 - r.x0 + kWave - 1) / kWave);
+    g.gy = (unsigned)((r.y1 - r.y0 + kWavesPerBlock - 1) / kWavesPerBlock);
+    g.gz = (unsigned)((r.z1 - r.z0 + g.chunk - 1) / g.chunk);
+  }
   RedCtl ctl{};
-  if (!INIT) ctl = red_prepare(*red, 1, DT_F64, gx * gy * gz);
+  if (!INIT) ctl = red_prepare(*red, 1, DT_F64, g.blocks());
   rec.ctl = ctl;
   rec.launch = [=](hipStream_t s) {
     const RedCtl c = with_scratch(ctl);
     Ptr<double> rp = INIT ? ptr_of<double>(rr.dat) : Ptr<double>{nullptr, 0, 0};
-    hipLaunchKernelGGL(vec ? k_heat3_march_v<INIT> : k_heat3_march<INIT>, dim3(gx, gy, gz),
-                       dim3(kWave, kWavesPerBlock), 0, s, ptr_of<double>(w.dat), rp, ptr_of<const double>(Kx.dat),
-                       ptr_of<const double>(Ky.dat), ptr_of<const double>(Kz.dat), ptr_of<const double>(p.dat), rx,
-                       ry, rz, r, chunk, c);
-    if (!INIT) launch_red_finish<double, OPS_INC, 1>(c, gx * gy * gz, s);
+    hipLaunchKernelGGL(vec ? k_heat3_march_v<INIT> : k_heat3_march<INIT>, dim3(g.gx, g.gy, g.gz),
+                       dim3(kWave, vec ? kHeatRows : kWavesPerBlock), 0, s, ptr_of<double>(w.dat), rp,
+                       ptr_of<const double>(Kx.dat), ptr_of<const double>(Ky.dat), ptr_of<const double>(Kz.dat),
+                       ptr_of<const double>(p.dat), rx, ry, rz, r, g.chunk, c);
+    if (!INIT) launch_red_finish<double, OPS_INC, 1>(c, g.blocks(), s);
   };
   return rec;
 }

@@ -1,5 +1,6 @@
 /* ORACLE — test infrastructure only (see ops_oracle.h).  Never linked into
  * or called by the product library. */
+#include <chrono>
 #include "ops_oracle.h"
 
 #include <float.h>
@@ -209,6 +210,10 @@ double orc_heat_apply(int ndim, int nx, int ny, int nz, const double *Kx, const 
   });
 }
 
+/* wall time of the iteration loop of the last orc_heat_cg call (set-up and
+ * output copies excluded): the CPU baseline's timer */
+static double g_loop_seconds = 0.0;
+
 int orc_heat_cg(int ndim, int nx, int ny, int nz, const double *K, const double *rhs, double rx, double ry,
                 double rz, int max_iters, double tol, int nt, double *u_int, double *pw_h, double *rrn_h,
                 double *alpha_h, double *beta_h) {
@@ -234,6 +239,7 @@ int orc_heat_cg(int ndim, int nx, int ny, int nz, const double *K, const double 
   if (rrn_h) rrn_h[0] = rro;
   const double stop = tol > 0 ? tol * sqrt(rro) : -1.0;
   int n = 0;
+  const auto t0 = std::chrono::steady_clock::now();
   while (n < max_iters) {
     double pw = sweep_sum(nt, 0, nx, 0, ny, 0, NZ, [&](int i, int j, int k, double &s) {
       long q = g.at(i, j, k);
@@ -264,9 +270,12 @@ int orc_heat_cg(int ndim, int nx, int ny, int nz, const double *K, const double 
     n++;
     if (sqrt(rrn) <= stop) break;
   }
+  g_loop_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (u_int) sweep(nt, 0, nx, 0, ny, 0, NZ, [&](int i, int j, int k) { u_int[g.lin(i, j, k)] = u[g.at(i, j, k)]; });
   return n;
 }
+
+double orc_last_loop_seconds(void) { return g_loop_seconds; }
 
 /* ------------------------------------------------------------- Poisson CG */
 template <class T>

@@ -55,6 +55,8 @@ float orc_laplace_run_f32(float *A_ext, float *Anew_ext, int nx, int ny, int ite
 int orc_heat_cg(int ndim, int nx, int ny, int nz, const double *K_ext, const double *rhs_int, double rx,
                 double ry, double rz, int max_iters, double tol, int nthreads, double *u_int, double *pw_hist,
                 double *rrn_hist, double *alpha_hist, double *beta_hist);
+/* seconds spent in the iteration loop of the last orc_heat_cg call */
+double orc_last_loop_seconds(void);
 /* face coefficients only (tea_leaf_common_init_Kx_Ky_kernel + zeroed
  * boundary faces), interior-sized outputs */
 void orc_heat_coefs(int ndim, int nx, int ny, int nz, const double *K_ext, double *Kx_ext, double *Ky_ext,

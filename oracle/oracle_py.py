@@ -45,6 +45,7 @@ def lib():
             "orc_laplace_run": (cd, [_d, _d, ci, ci, ci, ci, _d]),
             "orc_laplace_run_f32": (C.c_float, [_f, _f, ci, ci, ci, ci, _f]),
             "orc_heat_cg": (ci, [ci, ci, ci, ci, _d, _d, cd, cd, cd, ci, cd, ci, _d, _d, _d, _d, _d]),
+            "orc_last_loop_seconds": (cd, []),
             "orc_heat_coefs": (None, [ci, ci, ci, ci, _d, _d, _d, _d]),
             "orc_heat_apply": (cd, [ci, ci, ci, ci, _d, _d, _d, _d, cd, cd, cd, _d, ci]),
             "orc_poisson_cg": (ci, [ci, ci, ci, ci, _d, ci, cd, ci, _d, _d, _d, _d]),
@@ -158,6 +159,11 @@ def heat_cg(ndim, nx, ny, nz, K_ext, rhs, rx=1.0, ry=1.0, rz=1.0, max_iters=100,
     n = lib().orc_heat_cg(ndim, nx, ny, nz, P(np.ascontiguousarray(K_ext)), P(np.ascontiguousarray(rhs)),
                           rx, ry, rz, max_iters, tol, nthreads, P(u), P(pw), P(rrn), P(al), P(be))
     return dict(iters=n, u=u, pw=pw[:n], rrn=rrn[: n + 1], alpha=al[:n], beta=be[:n])
+
+
+def last_loop_seconds():
+    """Wall time of the iteration loop of the last heat_cg call."""
+    return lib().orc_last_loop_seconds()
 
 
 def heat_coefs(ndim, nx, ny, nz, K_ext):
