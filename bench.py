@@ -9,9 +9,10 @@ reductions, driven by the C++ app over the C ABI.
   READ/WRITE, 2x for RW/INC, 112 B/point, counted per loop as written even
   when the lazy queue fuses the axpy+axpby pair into one 40 B/point pass) /
   measured time, decimal GB/s.
-  roofline = the kernel with the largest share of the step
-  (tea_leaf_cg_calc_w_reduce_kernel, 40 B/point compulsory traffic) timed
-  with HIP events recorded on the library's stream around every launch.
+  roofline = the kernel with the largest share of the step (the fused
+  axpy + axpby + calc_w chain, 72 B/point compulsory traffic; calc_w alone,
+  40 B/point, with OPSMI_FUSION=1) timed with HIP events recorded on the
+  library's stream around every launch.
   cpu_baseline = the oracle's OpenMP restatement of the reference CPU path on
   the same 512^3 problem, a few iterations on the host cores.
 
@@ -139,14 +140,14 @@ def bench_heat(a, ops, apps):
     kernels = kernel_table(ops, BYTES_PER_PT, pts)
     rr_rel = (h.rr() / h.rr(initial=True)) ** 0.5
     h.close()
+    rl = roofline(kernels, BYTES_PER_PT, pts)
+    if n == 512:
+        rl["traffic"] = profiled_traffic("heat3d_cg_512_hbm_traffic.md", rl["kernel"])
     out = line(value, "GB/s", ms, a, "f64",
                {"workload": f"heat3d_cg_{n}", "grid": [n, n, n], "operator": "7-point variable coefficient",
                 "step": "one CG iteration (4 ops_par_loops, 2 FP64 reductions)", "bytes_per_point": 112,
                 "relres_after_steps": rr_rel},
-               roofline=roofline(kernels, BYTES_PER_PT, pts,
-                                 profiled_traffic("heat3d_cg_512_hbm_traffic.md",
-                                                  "tea_leaf_cg_calc_w_reduce_kernel") if n == 512 else None),
-               kernels=kernels)
+               roofline=rl, kernels=kernels)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_heat(n, a.seed)
     return out

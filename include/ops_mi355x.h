@@ -388,14 +388,17 @@ void *ops_mi_stream(void);
 void ops_mi_sync(void);
 /* per-loop statistics gathered when diags > 1 (the reference's
  * ops_timing_output table, ops/c/src/core/ops_lib_core.cpp:1468-1624):
- * launches, summed kernel milliseconds (HIP events on the library stream) and
- * algorithmic bytes by ops_compute_transfer's rule (:1687-1697).  Returns 0 if
- * the name is unknown. */
+ * launches made under this name, their summed kernel milliseconds (HIP events
+ * on the library stream) and algorithmic bytes by ops_compute_transfer's rule
+ * (:1687-1697; a loop the queue fused into another kernel adds its bytes here
+ * but its launch and time to the fused kernel's name).  Returns 0 if the name
+ * is unknown. */
 int ops_mi_loop_stats(const char *name, long *count, double *kernel_ms, double *bytes);
 void ops_mi_reset_stats(void);
-/* loop-chain fusion in the queue: 0 off, 1 the default rules (CG axpy+axpby
- * pair), 2 also the three-loop CG chain (axpy+axpby+calc_w, measured slower
- * on MI355X at 512^3).  Initial level from OPSMI_FUSION (default 1). */
+/* loop-chain fusion in the queue: 0 off, 1 the pointwise CG axpy+axpby pair,
+ * 2 also the three-loop CG chain (axpy+axpby+calc_w in one column pass).
+ * Every level gives the same bits.  Initial level from OPSMI_FUSION
+ * (default 2). */
 void ops_mi_set_fusion(int level);
 
 #ifdef __cplusplus

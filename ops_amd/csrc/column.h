@@ -289,106 +289,128 @@ struct ChainPtrs {
   Ptr<const double> pold, r, Kx, Ky, Kz;
 };
 
-template <int V, bool AXPY, int WY = kWavesPerBlock>
-__device__ __forceinline__ double heat3_chain_column_v(HeatLdsV<V, WY> &L, const ChainPtrs &c, const Range3 &R,
-                                                       double alpha_u, double a, double b, double rx, double ry,
-                                                       double rz, int x0, int y0, int z0, int z1) {
-  double acc = 0.0;
-  const int lx = (int)threadIdx.x, ly = (int)threadIdx.y;
-  const ColLane cl = col_lane<V, WY>(x0, y0, R.x1, R.y1);
-  const int n = cl.n;
-  const bool act = n > 0;
-  const int x = x0 + V * lx, y = y0 + ly;
-  const int o = V + V * lx;
-  // rims inside R take the axpby value
-  const bool in_xl = x0 - 1 >= R.x0, in_xh = x0 + cl.cnt < R.x1;
-  const bool in_yl = y - 1 >= R.y0, in_yh = y + 1 < R.y1;
-  auto pnew1 = [&](int xx, int yy, int zz, bool in) -> double {
-    const double po = c.pold(xx, yy, zz);
-    return in ? a * po + b * c.r(xx, yy, zz) : po;
-  };
-  // p_new of V points of row yy, plane zz into o_ (p_old into po)
-  auto pnewv = [&](int yy, int zz, bool in, double (&o_)[V], double (&po)[V]) {
-    double rv[V];
-    ldv<double, V>(c.pold.at(x, yy, zz), n, po);
-    if (in) {
-      ldv<double, V>(c.r.at(x, yy, zz), n, rv);
-#pragma unroll
-      for (int j = 0; j < V; j++) o_[j] = a * po[j] + b * rv[j];
-    } else {
-#pragma unroll
-      for (int j = 0; j < V; j++) o_[j] = po[j];
+// one plane of a chain column as a lane loads it, raw: p_old and r of its
+// points (p_new is formed when the plane is used), Kz, Kx, Ky, u and the
+// rims' p_old, r (and Kx, Ky) values
+template <int V>
+struct ChainPlane {
+  double po[V], rr[V], kz[V], kx[V], ky[V], u[V];
+  double yl_po[V], yl_r[V], yh_po[V], yh_r[V], yh_ky[V];
+  double xl_po, xl_r, xh_po, xh_r, xh_kx;
+};
+
+// The chain over one column tile.  Loads run one plane ahead (the three
+// planes in registers form a ring, the z loop is unrolled by three), every
+// step issues the same loads, and p_new is formed from the raw values only
+// when the plane is used, so loads stay in flight across the barrier.  r is
+// read only inside R (elsewhere from a clamped in-range point whose value is
+// not used), Kx, Ky, u and the rims only up to plane z1 - 1.
+template <int V, bool AXPY, int WY, bool FULL>
+struct ChainColumn {
+  static constexpr int RING = 3;
+  HeatLdsV<V, WY> &L;
+  const ChainPtrs &c;
+  const Range3 &R;
+  double alpha_u, a, b, rx, ry, rz;
+  int x0, z0, z1;
+  ColLane cl;
+  int x, y, o, lx, ly;
+  bool in_xl, in_xh, in_yl, in_yh;
+  int xl_r, xh_r, yl_r, yh_r;  // where the rims read r (inside R)
+  double acc;
+  double pm[V], pc[V];
+  ChainPlane<V> P[RING];
+
+  __device__ __forceinline__ ChainColumn(HeatLdsV<V, WY> &L_, const ChainPtrs &c_, const Range3 &R_, double alpha_u_,
+                                         double a_, double b_, double rx_, double ry_, double rz_, int x0_, int y0,
+                                         int z0_, int z1_)
+      : L(L_), c(c_), R(R_), alpha_u(alpha_u_), a(a_), b(b_), rx(rx_), ry(ry_), rz(rz_), x0(x0_), z0(z0_), z1(z1_) {
+    lx = (int)threadIdx.x;
+    ly = (int)threadIdx.y;
+    cl = col_lane<V, WY>(x0, y0, R.x1, R.y1);
+    if (FULL) cl.n = V;
+    x = x0 + V * lx;
+    y = y0 + ly;
+    o = V + V * lx;
+    in_xl = x0 - 1 >= R.x0;
+    in_xh = x0 + cl.cnt < R.x1;
+    in_yl = y - 1 >= R.y0;
+    in_yh = y + 1 < R.y1;
+    xl_r = in_xl ? x0 - 1 : x0;
+    xh_r = in_xh ? x0 + cl.cnt : x0 + cl.cnt - 1;
+    yl_r = in_yl ? y - 1 : y;
+    yh_r = in_yh ? y + 1 : y;
+    acc = 0.0;
+  }
+  __device__ __forceinline__ int n() const { return FULL ? V : cl.n; }
+  __device__ __forceinline__ bool act() const { return FULL || cl.n > 0; }
+  // p_new of one value: the axpby inside R, p_old outside
+  __device__ __forceinline__ double pnew(double po, double r, bool in) const { return in ? a * po + b * r : po; }
+
+  __device__ __forceinline__ void load(int q, ChainPlane<V> &Pl) const {
+    const int qk = min(q, z1 - 1);
+    const int qr = min(max(q, R.z0), R.z1 - 1);
+    const int nn = n();
+    ldv<double, V>(c.pold.at(x, y, q), nn, Pl.po);
+    ldv<double, V>(c.r.at(x, y, qr), nn, Pl.rr);
+    ldv<double, V>(c.Kz.at(x, y, q), nn, Pl.kz);
+    ldv<double, V>(c.Kx.at(x, y, qk), nn, Pl.kx);
+    ldv<double, V>(c.Ky.at(x, y, qk), nn, Pl.ky);
+    if (AXPY) ldv<double, V>(c.u.at(x, y, qk), nn, Pl.u);
+    if (cl.hxl) {
+      Pl.xl_po = c.pold(x0 - 1, y, qk);
+      Pl.xl_r = c.r(xl_r, y, qk);
     }
-  };
-  double pm[V], pc[V], pn[V], poc[V], pon[V], kzc[V], kzn[V], kxc[V], kyc[V], uc[V];
-  double ryl[V], ryh[V], rky[V], scratch[V];
-  double rxl = 0, rxh = 0, rkx = 0;
-#pragma unroll
-  for (int j = 0; j < V; j++)
-    pm[j] = pc[j] = pn[j] = poc[j] = pon[j] = kzc[j] = kzn[j] = kxc[j] = kyc[j] = uc[j] = ryl[j] = ryh[j] =
-        rky[j] = 0;
-  if (act) {
-    pnewv(y, z0 - 1, z0 - 1 >= R.z0, pm, scratch);
-    pnewv(y, z0, true, pc, poc);
-    pnewv(y, z0 + 1, z0 + 1 < R.z1, pn, pon);
-    ldv<double, V>(c.Kz.at(x, y, z0), n, kzc);
-    ldv<double, V>(c.Kz.at(x, y, z0 + 1), n, kzn);
-    ldv<double, V>(c.Kx.at(x, y, z0), n, kxc);
-    ldv<double, V>(c.Ky.at(x, y, z0), n, kyc);
-    if (AXPY) ldv<double, V>(c.u.at(x, y, z0), n, uc);
-    if (cl.hxl) rxl = pnew1(x0 - 1, y, z0, in_xl);
     if (cl.hxh) {
-      rxh = pnew1(x0 + cl.cnt, y, z0, in_xh);
-      rkx = c.Kx(x0 + cl.cnt, y, z0);
+      Pl.xh_po = c.pold(x0 + cl.cnt, y, qk);
+      Pl.xh_r = c.r(xh_r, y, qk);
+      Pl.xh_kx = c.Kx(x0 + cl.cnt, y, qk);
     }
-    if (cl.hyl) pnewv(y - 1, z0, in_yl, ryl, scratch);
+    if (cl.hyl) {
+      ldv<double, V>(c.pold.at(x, y - 1, qk), nn, Pl.yl_po);
+      ldv<double, V>(c.r.at(x, yl_r, qk), nn, Pl.yl_r);
+    }
     if (cl.hyh) {
-      pnewv(y + 1, z0, in_yh, ryh, scratch);
-      ldv<double, V>(c.Ky.at(x, y + 1, z0), n, rky);
+      ldv<double, V>(c.pold.at(x, y + 1, qk), nn, Pl.yh_po);
+      ldv<double, V>(c.r.at(x, yh_r, qk), nn, Pl.yh_r);
+      ldv<double, V>(c.Ky.at(x, y + 1, qk), nn, Pl.yh_ky);
     }
   }
-  for (int z = z0; z < z1; z++) {
-    const int bb = (z - z0) & 1;
-    if (act) {
-      lds_st<double, V>(&L.sp[bb][ly + 1][o], n, pc);
-      lds_st<double, V>(&L.skx[bb][ly][o], n, kxc);
-      lds_st<double, V>(&L.sky[bb][ly][V * lx], n, kyc);
-      if (cl.hxl) L.sp[bb][ly + 1][V - 1] = rxl;
+
+  template <int K>
+  __device__ __forceinline__ void step(int z) {
+    const ChainPlane<V> &C = P[K % RING], &N = P[(K + 1) % RING];
+    const int bb = (z - z0) & 1, nn = n();
+    double pn[V];
+    if (act()) {
+      lds_st<double, V>(&L.sp[bb][ly + 1][o], nn, pc);
+      lds_st<double, V>(&L.skx[bb][ly][o], nn, C.kx);
+      lds_st<double, V>(&L.sky[bb][ly][V * lx], nn, C.ky);
+      if (cl.hxl) L.sp[bb][ly + 1][V - 1] = pnew(C.xl_po, C.xl_r, in_xl);
       if (cl.hxh) {
-        L.sp[bb][ly + 1][V + cl.cnt] = rxh;
-        L.skx[bb][ly][V + cl.cnt] = rkx;
+        L.sp[bb][ly + 1][V + cl.cnt] = pnew(C.xh_po, C.xh_r, in_xh);
+        L.skx[bb][ly][V + cl.cnt] = C.xh_kx;
       }
-      if (cl.hyl) lds_st<double, V>(&L.sp[bb][0][o], n, ryl);
-      if (cl.hyh) {
-        lds_st<double, V>(&L.sp[bb][ly + 2][o], n, ryh);
-        lds_st<double, V>(&L.sky[bb][ly + 1][V * lx], n, rky);
-      }
-    }
-    // next plane: p_new and p_old of z + 2 for the register window, u(z + 1)
-    // for the axpy, coefficients and rims of plane z + 1
-    double pnn[V], ponn[V], kznn[V], kxn_c[V], kyn_c[V], un[V];
+      if (cl.hyl) {
+        double t[V];
 #pragma unroll
-    for (int j = 0; j < V; j++) pnn[j] = ponn[j] = kznn[j] = kxn_c[j] = kyn_c[j] = un[j] = 0;
-    if (act && z + 1 < z1) {
-      const int zn = z + 1;
-      pnewv(y, zn + 1, zn + 1 < R.z1, pnn, ponn);
-      ldv<double, V>(c.Kz.at(x, y, zn + 1), n, kznn);
-      ldv<double, V>(c.Kx.at(x, y, zn), n, kxn_c);
-      ldv<double, V>(c.Ky.at(x, y, zn), n, kyn_c);
-      if (AXPY) ldv<double, V>(c.u.at(x, y, zn), n, un);
-      if (cl.hxl) rxl = pnew1(x0 - 1, y, zn, in_xl);
-      if (cl.hxh) {
-        rxh = pnew1(x0 + cl.cnt, y, zn, in_xh);
-        rkx = c.Kx(x0 + cl.cnt, y, zn);
+        for (int j = 0; j < V; j++) t[j] = pnew(C.yl_po[j], C.yl_r[j], in_yl);
+        lds_st<double, V>(&L.sp[bb][0][o], nn, t);
       }
-      if (cl.hyl) pnewv(y - 1, zn, in_yl, ryl, scratch);
       if (cl.hyh) {
-        pnewv(y + 1, zn, in_yh, ryh, scratch);
-        ldv<double, V>(c.Ky.at(x, y + 1, zn), n, rky);
+        double t[V];
+#pragma unroll
+        for (int j = 0; j < V; j++) t[j] = pnew(C.yh_po[j], C.yh_r[j], in_yh);
+        lds_st<double, V>(&L.sp[bb][ly + 2][o], nn, t);
+        lds_st<double, V>(&L.sky[bb][ly + 1][V * lx], nn, C.yh_ky);
       }
+      load(min(z + 2, z1), P[(K + 2) % RING]);
     }
     __syncthreads();
-    if (act) {
+    const bool in_n = z + 1 < R.z1;
+#pragma unroll
+    for (int j = 0; j < V; j++) pn[j] = pnew(N.po[j], N.rr[j], in_n);
+    if (act()) {
       double pym[V], pyp[V], kyn[V], v[V], unew[V];
       lds_ld<double, V>(&L.sp[bb][ly][o], pym);
       lds_ld<double, V>(&L.sp[bb][ly + 2][o], pyp);
@@ -397,33 +419,80 @@ __device__ __forceinline__ double heat3_chain_column_v(HeatLdsV<V, WY> &L, const
 #pragma unroll
       for (int j = 0; j < V; j++) {
         const double pxm = j == 0 ? left : pc[j == 0 ? 0 : j - 1];
-        const double pxp = j + 1 < n ? pc[j + 1 < V ? j + 1 : j] : L.sp[bb][ly + 1][o + j + 1];
-        const double kxn = j + 1 < n ? kxc[j + 1 < V ? j + 1 : j] : L.skx[bb][ly][o + j + 1];
-        v[j] = (1.0 + ry * (kyn[j] + kyc[j]) + rx * (kxn + kxc[j]) + rz * (kzn[j] + kzc[j])) * pc[j] -
-               ry * (kyn[j] * pyp[j] + kyc[j] * pym[j]) - rx * (kxn * pxp + kxc[j] * pxm) -
-               rz * (kzn[j] * pn[j] + kzc[j] * pm[j]);
-        unew[j] = uc[j] + alpha_u * poc[j];
-        if (j < n) acc = acc + v[j] * pc[j];
+        const double pxp = j + 1 < nn ? pc[j + 1 < V ? j + 1 : j] : L.sp[bb][ly + 1][o + j + 1];
+        const double kxn = j + 1 < nn ? C.kx[j + 1 < V ? j + 1 : j] : L.skx[bb][ly][o + j + 1];
+        v[j] = (1.0 + ry * (kyn[j] + C.ky[j]) + rx * (kxn + C.kx[j]) + rz * (N.kz[j] + C.kz[j])) * pc[j] -
+               ry * (kyn[j] * pyp[j] + C.ky[j] * pym[j]) - rx * (kxn * pxp + C.kx[j] * pxm) -
+               rz * (N.kz[j] * pn[j] + C.kz[j] * pm[j]);
+        unew[j] = C.u[j] + alpha_u * C.po[j];
+        if (j < nn) acc = acc + v[j] * pc[j];
       }
-      stv<double, V, true>(c.w.at(x, y, z), n, v);
-      stv<double, V, true>(c.pnew.at(x, y, z), n, pc);
-      if (AXPY) stv<double, V, true>(c.u.at(x, y, z), n, unew);
+      stv<double, V, true>(c.w.at(x, y, z), nn, v);
+      stv<double, V, true>(c.pnew.at(x, y, z), nn, pc);
+      if (AXPY) stv<double, V, true>(c.u.at(x, y, z), nn, unew);
     }
 #pragma unroll
     for (int j = 0; j < V; j++) {
       pm[j] = pc[j];
       pc[j] = pn[j];
-      pn[j] = pnn[j];
-      poc[j] = pon[j];
-      pon[j] = ponn[j];
-      kzc[j] = kzn[j];
-      kzn[j] = kznn[j];
-      kxc[j] = kxn_c[j];
-      kyc[j] = kyn_c[j];
-      uc[j] = un[j];
     }
   }
-  return acc;
+  template <int K>
+  __device__ __forceinline__ void steps_all(int z) {
+    if constexpr (K < RING) {
+      step<K>(z);
+      steps_all<K + 1>(z + 1);
+    }
+  }
+  template <int K>
+  __device__ __forceinline__ void steps_tail(int z) {
+    if constexpr (K < RING - 1) {
+      if (z < z1) {
+        step<K>(z);
+        steps_tail<K + 1>(z + 1);
+      }
+    }
+  }
+
+  __device__ __forceinline__ double run() {
+#pragma unroll
+    for (int i = 0; i < RING; i++) {
+#pragma unroll
+      for (int j = 0; j < V; j++)
+        P[i].po[j] = P[i].rr[j] = P[i].kz[j] = P[i].kx[j] = P[i].ky[j] = P[i].u[j] = P[i].yl_po[j] = P[i].yl_r[j] =
+            P[i].yh_po[j] = P[i].yh_r[j] = P[i].yh_ky[j] = 0.0;
+      P[i].xl_po = P[i].xl_r = P[i].xh_po = P[i].xh_r = P[i].xh_kx = 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < V; j++) pm[j] = pc[j] = 0.0;
+    if (act()) {
+      // p_new of plane z0 - 1 (inside R when z0 > R.z0)
+      const int qm = z0 - 1;
+      double po[V], rv[V];
+      ldv<double, V>(c.pold.at(x, y, qm), n(), po);
+      ldv<double, V>(c.r.at(x, y, max(qm, R.z0)), n(), rv);
+      load(z0, P[0]);
+      load(min(z0 + 1, z1), P[1]);
+#pragma unroll
+      for (int j = 0; j < V; j++) {
+        pm[j] = pnew(po[j], rv[j], qm >= R.z0);
+        pc[j] = pnew(P[0].po[j], P[0].rr[j], true);
+      }
+    }
+    int z = z0;
+    for (; z + RING <= z1; z += RING) steps_all<0>(z);
+    steps_tail<0>(z);
+    return acc;
+  }
+};
+
+template <int V, bool AXPY, int WY = kWavesPerBlock>
+__device__ __forceinline__ double heat3_chain_column_v(HeatLdsV<V, WY> &L, const ChainPtrs &c, const Range3 &R,
+                                                       double alpha_u, double a, double b, double rx, double ry,
+                                                       double rz, int x0, int y0, int z0, int z1) {
+  if (x0 + kWave * V <= R.x1 && y0 + WY <= R.y1)
+    return ChainColumn<V, AXPY, WY, true>(L, c, R, alpha_u, a, b, rx, ry, rz, x0, y0, z0, z1).run();
+  return ChainColumn<V, AXPY, WY, false>(L, c, R, alpha_u, a, b, rx, ry, rz, x0, y0, z0, z1).run();
 }
 
 // ------------------------------------------------------------ Poisson (7-point, constant coefficient)

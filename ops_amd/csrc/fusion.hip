@@ -12,7 +12,7 @@
 // changes) until the axpby that overwrites p needs it gone, so the three
 // loops meet here.  One z-marching column kernel (heat3_chain_column_v)
 // reads u, p_old, r, Kx, Ky, Kz and writes u, p_new, w: 72 bytes per point
-// instead of 40 (the fused pair below) + 40; opt-in (cg_chain_rule_opt_in).
+// instead of 40 (the fused pair below) + 40 (fusion level 2, the default).
 // p_new is needed at stencil neighbours before every column has written it,
 // so each column forms it
 // from p_old and r for its planes and rims and writes it into p's spare
@@ -145,18 +145,18 @@ int axpy_axpby_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
   return 2;
 }
 
-// The three-loop chain moves 72 instead of 80 bytes per point but needs 154
-// VGPRs (3 waves per SIMD; capping at 128 spills): at 512^3 it measured
-// 2093 us against 1116 + 955 us for calc_w and the pair, and 2569 us with one
-// point per lane.  So it runs only at fusion level 2 (ops_mi_set_fusion(2) or
-// OPSMI_FUSION=2); the pair rule takes the axpy + axpby otherwise.
-int cg_chain_rule_opt_in(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
+// The three-loop chain moves 72 instead of 80 bytes per point: at 512^3 it
+// measured 1853 us against 1025 + 934 us for calc_w and the pair (2093 us
+// when it formed p_new as each load arrived, which made every load wait).
+// It runs at fusion level 2 (the default); at level 1 the pair rule takes the
+// axpy + axpby and calc_w runs alone.
+int cg_chain_rule_level2(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
   return inst().fusion >= 2 ? cg_chain_rule(run, at, s) : 0;
 }
 
 struct Register {
   Register() {
-    register_fusion(cg_chain_rule_opt_in);
+    register_fusion(cg_chain_rule_level2);
     register_fusion(axpy_axpby_rule);
   }
 } g_register;

@@ -350,6 +350,7 @@ void run_loop(const char *name, double bytes, const std::function<void(hipStream
   Instance &I = inst();
   LoopStat &s = I.stats[name ? name : "?"];
   s.count++;
+  s.launches++;
   s.bytes += bytes;
   if (I.diags > 1) {
     hipEvent_t a = get_event(), b = get_event();
@@ -540,7 +541,7 @@ void ops_init(const int argc, const char *const argv[], const int diags_level) {
   OPSMI_HIP(hipGetDeviceProperties(&prop, dev));
   I.num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   const char *fz = getenv("OPSMI_FUSION");
-  I.fusion = fz ? atoi(fz) : 1;
+  I.fusion = fz ? atoi(fz) : 2;
   OPSMI_HIP(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
   ensure_partials((size_t)1 << 20);
   OPSMI_HIP(hipMalloc(&I.d_partials2, (size_t)1 << 20));
@@ -1216,7 +1217,7 @@ int ops_mi_loop_stats(const char *name, long *count, double *kernel_ms, double *
   drain_events();
   auto it = inst().stats.find(name);
   if (it == inst().stats.end()) return 0;
-  if (count) *count = it->second.count;
+  if (count) *count = it->second.launches;
   if (kernel_ms) *kernel_ms = it->second.ms;
   if (bytes) *bytes = it->second.bytes;
   return 1;

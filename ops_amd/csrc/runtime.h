@@ -80,7 +80,8 @@ struct DatImpl {
 };
 
 struct LoopStat {
-  long count = 0;
+  long count = 0;     // calls of the loop (fused ones included)
+  long launches = 0;  // launches under this name (ms is their kernel time)
   double ms = 0.0;
   double bytes = 0.0;
 };
@@ -99,7 +100,7 @@ struct Instance {
   int error_mode = 0;
   int last_error = 0;
   std::string last_msg;
-  int fusion = 1;  // 0 off, 1 default rules, 2 also the opt-in ones
+  int fusion = 2;  // 0 off, 1 the pointwise pair only, 2 also the three-loop chain
   int block_x = 64, block_y = 4, block_z = 1;  // OPS_BLOCK_SIZE_* (x must be 64)
 
   std::vector<ops_block> blocks;

@@ -87,7 +87,7 @@ def test_fused_cg_chain_bit_identical(gpu_ops, n):
             u = h.u()
             h.close()
         finally:
-            ops.lib().ops_mi_set_fusion(1)
+            ops.lib().ops_mi_set_fusion(2)
         runs.append((hist, u, [s[0] if s else 0 for s in st]))
     (h0, u0, st0), (h1, u1, st1), (h2, u2, st2) = runs
     # each iteration's axpy waits in the queue past the r.r result until the
