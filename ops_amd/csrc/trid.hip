@@ -113,6 +113,172 @@ __global__ __launch_bounds__(256) void k_thomas(const T *__restrict__ a, const T
   }
 }
 
+// Solves along x (element stride 1).  One thread per system would make each
+// load of a wave touch 64 rows (64 cache lines); here a one-wave workgroup
+// takes 64 systems and moves them through LDS in chunks of kChunk steps: the
+// wave loads a chunk of a, b, c, d with row-contiguous V-wide loads (kChunk/V
+// lanes per system), stores it transposed ([step][system]) in LDS, and each
+// lane runs the same recurrence as k_thomas on its system from LDS, writing
+// c' and d' to the step-major scratch.  The next chunk's loads are issued
+// before the recurrence runs.  The backward pass reads c', d' per step
+// (coalesced), puts a chunk of x in LDS and writes it (or adds it to u) with
+// row-contiguous stores.  Same operations in the same order as k_thomas.
+constexpr int kChunk = 16;
+
+template <class T, int V>
+struct XChunk {
+  static constexpr int kLanesPerSys = kChunk / V;      // lanes covering one system's chunk
+  static constexpr int kSysPerLoad = 64 / kLanesPerSys;  // systems per wave-wide load
+  static constexpr int kLoads = 64 / kSysPerLoad;        // loads per array per chunk
+};
+
+template <class T, int V, bool INC>
+__global__ __launch_bounds__(64) void k_thomas_x(const T *__restrict__ a, const T *__restrict__ b,
+                                                 const T *__restrict__ c, T *__restrict__ d, T *__restrict__ u,
+                                                 T *__restrict__ cs, T *__restrict__ ds, LineGeo g) {
+  using XC = XChunk<T, V>;
+  __shared__ T LA[kChunk][65], LB[kChunk][65], LC[kChunk][65], LD[kChunk][65];
+  const int lane = (int)threadIdx.x;
+  const long first = (long)blockIdx.x * 64;
+  const long id = first + lane;  // this lane's system in the recurrence
+  const long L = g.count, n = g.n;
+  // the systems and chunk positions this lane covers in row-contiguous loads
+  const int sub = lane % XC::kLanesPerSys, sys0 = lane / XC::kLanesPerSys;
+  long rowoff[XC::kLoads];
+  bool rowok[XC::kLoads];
+#pragma unroll
+  for (int k = 0; k < XC::kLoads; k++) {
+    const long sid = first + sys0 + k * XC::kSysPerLoad;
+    rowok[k] = sid < L;
+    rowoff[k] = rowok[k] ? line_offset(g, sid) : 0;
+  }
+  T ra[XC::kLoads][V], rb[XC::kLoads][V], rc[XC::kLoads][V], rd[XC::kLoads][V];
+  auto load_chunk = [&](long i0) {
+#pragma unroll
+    for (int k = 0; k < XC::kLoads; k++) {
+      const long i = i0 + sub * V;
+      if (rowok[k] && i + V <= n) {
+        using VT = vec_t<T, V>;
+        const VT va = *(const VT *)(a + rowoff[k] + i), vb = *(const VT *)(b + rowoff[k] + i);
+        const VT vc = *(const VT *)(c + rowoff[k] + i), vd = *(const VT *)(d + rowoff[k] + i);
+#pragma unroll
+        for (int j = 0; j < V; j++) {
+          ra[k][j] = va[j];
+          rb[k][j] = vb[j];
+          rc[k][j] = vc[j];
+          rd[k][j] = vd[j];
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < V; j++) {
+          const bool ok = rowok[k] && i + j < n;
+          ra[k][j] = ok ? a[rowoff[k] + i + j] : (T)0;
+          rb[k][j] = ok ? b[rowoff[k] + i + j] : (T)1;
+          rc[k][j] = ok ? c[rowoff[k] + i + j] : (T)0;
+          rd[k][j] = ok ? d[rowoff[k] + i + j] : (T)0;
+        }
+      }
+    }
+  };
+  auto stage_chunk = [&]() {
+#pragma unroll
+    for (int k = 0; k < XC::kLoads; k++) {
+      const int sy = sys0 + k * XC::kSysPerLoad;
+#pragma unroll
+      for (int j = 0; j < V; j++) {
+        LA[sub * V + j][sy] = ra[k][j];
+        LB[sub * V + j][sy] = rb[k][j];
+        LC[sub * V + j][sy] = rc[k][j];
+        LD[sub * V + j][sy] = rd[k][j];
+      }
+    }
+  };
+  const bool live = id < L;
+  T cprev = 0, dprev = 0;
+  load_chunk(0);
+  for (long i0 = 0; i0 < n; i0 += kChunk) {
+    stage_chunk();
+    __syncthreads();
+    if (i0 + kChunk < n) load_chunk(i0 + kChunk);
+    if (live) {
+#pragma unroll
+      for (int k = 0; k < kChunk; k++) {
+        const long i = i0 + k;
+        if (i < n) {
+          const T av = LA[k][lane], bv = LB[k][lane], cv = LC[k][lane], dv = LD[k][lane];
+          if (i == 0) {
+            const T bb = (T)1.0 / bv;
+            cprev = bb * cv;
+            dprev = bb * dv;
+          } else {
+            const T bb = (T)1.0 / (bv - av * cprev);
+            dprev = bb * (dv - av * dprev);
+            cprev = bb * cv;
+          }
+          cs[i * L + id] = cprev;
+          ds[i * L + id] = dprev;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // backward: chunks from the last one down; x of a chunk goes to LA
+  T x = dprev;
+  const long last0 = (n - 1) / kChunk * kChunk;
+  for (long i0 = last0; i0 >= 0; i0 -= kChunk) {
+    if (live) {
+      T cv[kChunk], dv[kChunk];
+#pragma unroll
+      for (int k = 0; k < kChunk; k++) {
+        const long i = i0 + k;
+        if (i < n - 1) {
+          cv[k] = cs[i * L + id];
+          dv[k] = ds[i * L + id];
+        }
+      }
+#pragma unroll
+      for (int k = kChunk - 1; k >= 0; k--) {
+        const long i = i0 + k;
+        if (i == n - 1) LA[k][lane] = x;
+        else if (i < n - 1) {
+          x = dv[k] - cv[k] * x;
+          LA[k][lane] = x;
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < XC::kLoads; k++) {
+      const long i = i0 + sub * V;
+      const int sy = sys0 + k * XC::kSysPerLoad;
+      if (!rowok[k]) continue;
+      if (i + V <= n) {
+        using VT = vec_t<T, V>;
+        VT v;
+#pragma unroll
+        for (int j = 0; j < V; j++) v[j] = LA[sub * V + j][sy];
+        if (INC) {
+          const VT uo = *(const VT *)(u + rowoff[k] + i);
+#pragma unroll
+          for (int j = 0; j < V; j++) v[j] = uo[j] + v[j];
+          *(VT *)(u + rowoff[k] + i) = v;
+        } else {
+          *(VT *)(d + rowoff[k] + i) = v;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < V; j++)
+          if (i + j < n) {
+            const T xv = LA[sub * V + j][sy];
+            if (INC) u[rowoff[k] + i + j] = u[rowoff[k] + i + j] + xv;
+            else d[rowoff[k] + i + j] = xv;
+          }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 template <class T, bool INC>
 __global__ __launch_bounds__(1024) void k_pcr(const T *__restrict__ a, const T *__restrict__ b,
                                               const T *__restrict__ c, T *__restrict__ d, T *__restrict__ u,
@@ -257,6 +423,36 @@ void trid_run(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat 
     size_t sb = (size_t)g.n * g.count * es;
     char *scr = (char *)trid_scratch(2 * sb);
     int nblk = (int)((g.count + 255) / 256);
+    if (solvedim == 0) {
+      // x solves: LDS-staged chunks, 16-byte row loads when every array's
+      // rows start 16-byte aligned
+      bool v16 = g.s0 % 2 == 0 && g.s1 % 2 == 0;
+      for (char *p : {pa, pb, pc, pd, pu})
+        if (p && (uintptr_t)p % 16 != 0) v16 = false;
+      if (dt == DT_F32) v16 = false;
+      const int nw = (int)((g.count + 63) / 64);
+      run_loop(nm, bytes, [&](hipStream_t s) {
+        if (dt == DT_F64) {
+          double *A = (double *)pa, *B = (double *)pb, *C = (double *)pc, *D = (double *)pd, *U = (double *)pu;
+          double *CS = (double *)scr, *DS = (double *)(scr + sb);
+          if (v16) {
+            if (u) hipLaunchKernelGGL((k_thomas_x<double, 2, true>), dim3(nw), dim3(64), 0, s, A, B, C, D, U, CS, DS, g);
+            else hipLaunchKernelGGL((k_thomas_x<double, 2, false>), dim3(nw), dim3(64), 0, s, A, B, C, D, U, CS, DS, g);
+          } else {
+            if (u) hipLaunchKernelGGL((k_thomas_x<double, 1, true>), dim3(nw), dim3(64), 0, s, A, B, C, D, U, CS, DS, g);
+            else hipLaunchKernelGGL((k_thomas_x<double, 1, false>), dim3(nw), dim3(64), 0, s, A, B, C, D, U, CS, DS, g);
+          }
+        } else {
+          float *A = (float *)pa, *B = (float *)pb, *C = (float *)pc, *D = (float *)pd, *U = (float *)pu;
+          float *CS = (float *)scr, *DS = (float *)(scr + sb);
+          if (u) hipLaunchKernelGGL((k_thomas_x<float, 1, true>), dim3(nw), dim3(64), 0, s, A, B, C, D, U, CS, DS, g);
+          else hipLaunchKernelGGL((k_thomas_x<float, 1, false>), dim3(nw), dim3(64), 0, s, A, B, C, D, U, CS, DS, g);
+        }
+      });
+      mark_device_dirty(u ? u : d);
+      note_external_write(u ? u : d);
+      return;
+    }
     run_loop(nm, bytes, [&](hipStream_t s) {
       if (dt == DT_F64) {
         if (u) hipLaunchKernelGGL((k_thomas<double, true>), dim3(nblk), dim3(256), 0, s, (double *)pa, (double *)pb, (double *)pc, (double *)pd, (double *)pu, (double *)scr, (double *)(scr + sb), g);

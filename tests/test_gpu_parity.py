@@ -469,6 +469,30 @@ def test_trid_directions_bit_exact(apps, strategy):
     T.close()
 
 
+@pytest.mark.parametrize("n", [(41, 13, 7), (64, 65, 3), (17, 64, 2), (130, 5, 4)])
+def test_trid_x_solves_bit_exact(apps, n):
+    """Solves along x (staged through LDS in 16-step chunks): odd row pitches
+    (one element per lane) and even ones (two), lengths that are not a
+    multiple of the chunk, system counts that are not a multiple of 64, and
+    the _Inc form."""
+    nx, ny, nz = n
+    T = apps.Trid(nx, ny, nz, seed=29, strategy=0)
+    d_ref = O.trid_rhs(29, nx, ny, nz)
+    a, b, c = O.trid_inputs(29, 0, nx, ny, nz)
+    T.solve(0)
+    O.thomas_batch(3, 0, [nx, ny, nz], [nx, ny, nz], a, b, c, d_ref, nthreads=NT)
+    d, _ = T.fetch()
+    assert np.array_equal(d, d_ref)
+    u_ref = np.zeros_like(d_ref)
+    T.solve(0, inc=True)
+    T.solve(0, inc=True)
+    O.thomas_batch(3, 0, [nx, ny, nz], [nx, ny, nz], a, b, c, d_ref, u_ref, nthreads=NT)
+    O.thomas_batch(3, 0, [nx, ny, nz], [nx, ny, nz], a, b, c, d_ref, u_ref, nthreads=NT)
+    d, u = T.fetch()
+    assert np.array_equal(d, d_ref) and np.array_equal(u, u_ref)
+    T.close()
+
+
 def test_trid_full_size_thomas_and_pcr(apps):
     # the 256^3 ADI workload: Thomas bit-exact vs the CPU path, PCR within 1e-13
     n = 256
