@@ -753,6 +753,9 @@ opsmi_server opsmi_serve_create(size_t hbm_budget, int max_slots, int iters_per_
   S->max_slots = max_slots;
   S->iters_per_launch = iters_per_launch;
   S->use_graph = use_graph ? 1 : 0;
+  // OPSMI_SERVE_GRAPH=0 forces direct launches (same kernels; for profilers)
+  if (const char *e = getenv("OPSMI_SERVE_GRAPH"))
+    if (e[0] == '0') S->use_graph = 0;
   S->grid = inst().num_cus * 8;
   // smallest tile: a 2D tile of 64 x 8 points (2 rows per wave) over 4 fields
   S->max_tiles = std::min<size_t>(S->budget / (size_t)(kWave * kWavesPerBlock * 2 * 4 * 8) + (size_t)max_slots * 8,
