@@ -196,10 +196,13 @@ __global__ __launch_bounds__(256) void k_red_final(const T *__restrict__ in, uns
   block_reduce<T, OP, D>(v, sh);
   if (tid == 0) {
     T *slot = (T *)c.slot;
+    T *mirror = (T *)c.mirror;
 #pragma unroll
     for (int q = 0; q < D; q++) {
       const T base = c.use_init ? init_value<T>(c, q) : slot[q];
-      slot[q] = RedOp<T, OP>::combine(base, v[q]);
+      const T r = RedOp<T, OP>::combine(base, v[q]);
+      slot[q] = r;
+      mirror[q] = r;
     }
   }
 }

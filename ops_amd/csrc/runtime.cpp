@@ -12,6 +12,7 @@
 #include <time.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 
 namespace opsmi {
@@ -95,6 +96,22 @@ void ensure_host(ops_dat d) {
     OPSMI_HIP(hipMemcpyAsync(d->data, d->data_d, d->mem, hipMemcpyDeviceToHost, I.stream));
     OPSMI_HIP(hipStreamSynchronize(I.stream));
     d->dirty_hd = 0;
+  }
+}
+
+void wait_stream(hipStream_t s) {
+  Instance &I = inst();
+  if (!I.ev_wait) OPSMI_HIP(hipEventCreateWithFlags(&I.ev_wait, hipEventDisableTiming));
+  OPSMI_HIP(hipEventRecord(I.ev_wait, s));
+  const auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    const hipError_t e = hipEventQuery(I.ev_wait);
+    if (e == hipSuccess) return;
+    if (e != hipErrorNotReady) OPSMI_HIP(e);
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) {
+      OPSMI_HIP(hipEventSynchronize(I.ev_wait));
+      return;
+    }
   }
 }
 
@@ -288,6 +305,7 @@ RedCtl red_prepare(const ops_arg &a, int dim, DType dt, int nblocks) {
   ensure_partials((size_t)nblocks * dim * 8);
   RedCtl c{};
   c.slot = ri->d_slot;
+  c.mirror = ri->d_mirror;
   c.partials = I.d_partials;
   c.partials2 = I.d_partials2;
   c.use_init = ri->dev_valid ? 0 : 1;
@@ -564,6 +582,7 @@ void ops_exit(void) {
   for (ops_reduction r : I.reductions) {
     RedImpl *ri = (RedImpl *)r->impl;
     (void)hipFree(ri->d_slot);
+    (void)hipHostFree(ri->h_mirror);
     delete ri;
     free(r->data);
     free(r->name);
@@ -603,6 +622,8 @@ void ops_exit(void) {
   (void)hipFree(I.d_partials2);
   I.d_partials2 = nullptr;
   (void)hipHostFree(I.h_pinned);
+  if (I.ev_wait) (void)hipEventDestroy(I.ev_wait);
+  I.ev_wait = nullptr;
   (void)hipStreamDestroy(I.stream);
   I.consts.clear();
   I.stats.clear();
@@ -773,6 +794,9 @@ ops_reduction ops_decl_reduction_handle(int size, const char *type, const char *
   if (!strcmp(type, "float") || !strcmp(type, "int") || !strcmp(type, "real") || !strcmp(type, "integer")) ts = 4;
   ri->dt = dtype_of(type, ts);
   OPSMI_HIP(hipMalloc(&ri->d_slot, std::max(size, 64)));
+  // the result is read from here after the stream completes: no copy kernel
+  OPSMI_HIP(hipHostMalloc(&ri->h_mirror, std::max(size, 64), hipHostMallocMapped | hipHostMallocCoherent));
+  OPSMI_HIP(hipHostGetDevicePointer(&ri->d_mirror, ri->h_mirror, 0));
   r->impl = ri;
   inst().reductions.push_back(r);
   return r;
@@ -935,9 +959,8 @@ void ops_reduction_result_char(ops_reduction handle, int type_size, char *ptr) {
   flush_for(handle);
   RedImpl *ri = (RedImpl *)handle->impl;
   if (ri->dev_valid) {
-    OPSMI_HIP(hipMemcpyAsync(I.h_pinned, ri->d_slot, handle->size, hipMemcpyDeviceToHost, I.stream));
-    OPSMI_HIP(hipStreamSynchronize(I.stream));
-    memcpy(handle->data, I.h_pinned, handle->size);
+    wait_stream(I.stream);
+    memcpy(handle->data, ri->h_mirror, handle->size);
   }
   memcpy(ptr, handle->data, handle->size);
   handle->initialized = 0;

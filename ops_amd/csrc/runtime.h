@@ -64,8 +64,10 @@ struct DatView {
 };
 
 struct RedImpl {
-  void *d_slot = nullptr;  // device value of the handle (size bytes)
-  bool dev_valid = false;  // device slot holds the running value since re-arm
+  void *d_slot = nullptr;    // device value of the handle (size bytes)
+  void *h_mirror = nullptr;  // pinned, device-mapped host copy the final fold also writes
+  void *d_mirror = nullptr;  // its device address
+  bool dev_valid = false;    // device slot holds the running value since re-arm
   DType dt = DT_OTHER;
 };
 
@@ -97,6 +99,7 @@ struct Instance {
   int device = 0;
   int num_cus = 256;
   hipStream_t stream = nullptr;
+  hipEvent_t ev_wait = nullptr;  // wait_stream's completion event
   int error_mode = 0;
   int last_error = 0;
   std::string last_msg;
@@ -142,6 +145,10 @@ DatView view_of(ops_dat d);
 void ensure_host(ops_dat d);      // host mirror allocated and current
 void mark_device_dirty(ops_dat d);  // device now holds the newest data
 void sync_to_device(ops_dat d);   // host newer -> copy up
+// Waits until the work queued on s has completed, polling an event (a
+// blocking wait adds tens of microseconds of wake-up latency to every
+// reduction result); after 50 ms of polling it blocks instead.
+void wait_stream(hipStream_t s);
 
 // ---------------------------------------------------------------- args
 // Per-argument contract of a hand-written kernel.
@@ -191,6 +198,7 @@ int const_as_int(const char *name);
 // Device-side control block passed to reducing kernels.
 struct RedCtl {
   void *slot;          // handle's device value
+  void *mirror;        // host-mapped copy of it (written by the final fold)
   void *partials;      // per-block partials of the loop kernel (dim values per block)
   void *partials2;     // second-stage partials
   int use_init;        // 1: start from init[] (first use since re-arm)
