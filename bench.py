@@ -154,10 +154,14 @@ def bench_heat(a, ops, apps):
 
 
 # ------------------------------------------------------ mixed-precision Poisson
-MIXED_BPP = {"poisson_calc_w_reduce_kernel": 8, "mp_update_kernel": 24, "mp_p_kernel": 12,
+# compulsory bytes per point; the fused chains read u/e, p_old, r/rr and
+# write u/e, p_new, w (48 B in FP64, 24 B in FP32)
+MIXED_BPP = {"poisson_calc_w_reduce_kernel": 8, "mp_update_kernel": 24, "mp_rr_update_kernel": 12,
+             "mp_e_update_kernel": 12, "mp_p_kernel": 12, "mp_chain(e+p+calc_w)": 24, "mp_chain(p+calc_w)": 16,
              "poisson_residual_kernel": 24, "mp_init_kernel": 20, "mp_correct_kernel": 20}
 FP64_BPP = {"poisson_calc_w_reduce_kernel": 16, "tea_leaf_axpy_kernel": 24,
-            "tea_leaf_cg_calc_ur_r_reduce_kernel": 24, "tea_leaf_axpby_kernel": 24, "cg_pair(axpy+axpby)": 40}
+            "tea_leaf_cg_calc_ur_r_reduce_kernel": 24, "tea_leaf_axpby_kernel": 24, "cg_pair(axpy+axpby)": 40,
+            "poisson_chain(axpy+axpby+calc_w)": 48}
 
 
 def bench_mixed(a, ops, apps):

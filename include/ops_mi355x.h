@@ -324,6 +324,11 @@ void ops_par_loop_poisson_calc_w_reduce_kernel(const char *name, ops_block block
  *                            dots[0] += rr*(Mi*rr), dots[1] += rr*rr
  *   mp_update_kernel         e += a*p; rr -= a*w; same two dots (a, Mi
  *                            float gbls; e, rr RW; p, w READ)
+ *   mp_rr_update_kernel      rr -= a*w; the two dots (the rr half of
+ *                            mp_update_kernel)
+ *   mp_e_update_kernel       e += a*p (the e half; the lazy queue holds it
+ *                            past the dots result and fuses it with the next
+ *                            mp_p_kernel and operator sweep)
  *   mp_p_kernel              p = Mi*rr + b*p (Mi, b float gbls)
  *   mp_correct_kernel        x = x + (double)e
  * The FP32 operator sweep is poisson_calc_w_reduce_kernel on float dats. */
@@ -333,6 +338,10 @@ void ops_par_loop_mp_init_kernel(const char *name, ops_block block, int dim, int
                                  ops_arg p, ops_arg r, ops_arg mi, ops_arg dots);
 void ops_par_loop_mp_update_kernel(const char *name, ops_block block, int dim, int *range, ops_arg e, ops_arg rr,
                                    ops_arg p, ops_arg w, ops_arg a, ops_arg mi, ops_arg dots);
+void ops_par_loop_mp_rr_update_kernel(const char *name, ops_block block, int dim, int *range, ops_arg rr,
+                                      ops_arg w, ops_arg a, ops_arg mi, ops_arg dots);
+void ops_par_loop_mp_e_update_kernel(const char *name, ops_block block, int dim, int *range, ops_arg e, ops_arg p,
+                                     ops_arg a);
 void ops_par_loop_mp_p_kernel(const char *name, ops_block block, int dim, int *range, ops_arg p, ops_arg rr,
                               ops_arg mi, ops_arg b);
 void ops_par_loop_mp_correct_kernel(const char *name, ops_block block, int dim, int *range, ops_arg x,

@@ -425,14 +425,19 @@ int opsapp_pmixed_solve(void *ctx, double tol, int max_outer, int inner_iters, d
                                                 ops_arg_reduce(M->red, 1, "double", OPS_INC));
       ops_reduction_result_char(M->red, sizeof(double), (char *)&pw);
       float a = (float)(rz / pw);
-      ops_par_loop_mp_update_kernel("mp_update_kernel", M->blk, d, g.inner,
-                                    ops_arg_dat(M->e, 1, M->s0, "float", OPS_RW),
-                                    ops_arg_dat(M->rr, 1, M->s0, "float", OPS_RW),
-                                    ops_arg_dat(M->p, 1, M->s0, "float", OPS_READ),
-                                    ops_arg_dat(M->w, 1, M->s0, "float", OPS_READ),
-                                    ops_arg_gbl_char((char *)&a, 1, sizeof(float), OPS_READ),
-                                    ops_arg_gbl_char((char *)&mi, 1, sizeof(float), OPS_READ),
-                                    ops_arg_reduce(M->dots, 2, "double", OPS_INC));
+      // rr -= a w with its dots, then e += a p: the dots result does not need
+      // the e update, so the queue holds it until the next p update and
+      // operator sweep and runs the three as one fused pass
+      ops_par_loop_mp_rr_update_kernel("mp_rr_update_kernel", M->blk, d, g.inner,
+                                       ops_arg_dat(M->rr, 1, M->s0, "float", OPS_RW),
+                                       ops_arg_dat(M->w, 1, M->s0, "float", OPS_READ),
+                                       ops_arg_gbl_char((char *)&a, 1, sizeof(float), OPS_READ),
+                                       ops_arg_gbl_char((char *)&mi, 1, sizeof(float), OPS_READ),
+                                       ops_arg_reduce(M->dots, 2, "double", OPS_INC));
+      ops_par_loop_mp_e_update_kernel("mp_e_update_kernel", M->blk, d, g.inner,
+                                      ops_arg_dat(M->e, 1, M->s0, "float", OPS_RW),
+                                      ops_arg_dat(M->p, 1, M->s0, "float", OPS_READ),
+                                      ops_arg_gbl_char((char *)&a, 1, sizeof(float), OPS_READ));
       ops_reduction_result_char(M->dots, 2 * sizeof(double), (char *)dots);
       total++;
       if (sqrt(dots[1]) <= istop) break;

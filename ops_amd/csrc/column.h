@@ -496,79 +496,336 @@ __device__ __forceinline__ double heat3_chain_column_v(HeatLdsV<V, WY> &L, const
 }
 
 // ------------------------------------------------------------ Poisson (7-point, constant coefficient)
-template <class T, int V>
+template <class T, int V, int WY = kWavesPerBlock>
 struct PoisLdsV {
-  __attribute__((aligned(16))) T sp[2][kWavesPerBlock + 2][kWave * V + 2 * V];
+  __attribute__((aligned(16))) T sp[2][WY + 2][kWave * V + 2 * V];
+};
+
+// one plane of a Poisson column as a lane loads it: its V points of p and
+// the rims it owns
+template <class T, int V>
+struct PoisPlane {
+  T p[V], ryl[V], ryh[V];
+  T rxl, rxh;
 };
 
 // w = 6p - (p(-1,0,0) + p(1,0,0)) - (p(0,-1,0) + p(0,1,0)) - (p(0,0,-1) +
-// p(0,0,1)) in T; returns this thread's sum of (double) w (double) p.
-template <class T, int V, bool NT>
-__device__ __forceinline__ double poisson3_column_v(PoisLdsV<T, V> &L, const Ptr<T> &w, const Ptr<const T> &p, int x0,
-                                                    int y0, int xe, int ye, int z0, int z1) {
-  double acc = 0.0;
-  const int lx = (int)threadIdx.x, ly = (int)threadIdx.y;
-  const ColLane c = col_lane<V>(x0, y0, xe, ye);
-  const int n = c.n;
-  const bool act = n > 0;
-  const int x = x0 + V * lx, y = y0 + ly;
-  const int o = V + V * lx;
-  T pm[V], pc[V], pn[V], ryl[V], ryh[V];
-  T rxl = 0, rxh = 0;
-#pragma unroll
-  for (int j = 0; j < V; j++) pm[j] = pc[j] = pn[j] = ryl[j] = ryh[j] = 0;
-  if (act) {
-    ldv<T, V>(p.at(x, y, z0 - 1), n, pm);
-    ldv<T, V>(p.at(x, y, z0), n, pc);
-    ldv<T, V>(p.at(x, y, z0 + 1), n, pn);
-    if (c.hxl) rxl = p(x0 - 1, y, z0);
-    if (c.hxh) rxh = p(x0 + c.cnt, y, z0);
-    if (c.hyl) ldv<T, V>(p.at(x, y - 1, z0), n, ryl);
-    if (c.hyh) ldv<T, V>(p.at(x, y + 1, z0), n, ryh);
+// p(0,0,1)) in T over one column tile; run() returns this thread's sum of
+// (double) w (double) p.  Same structure as HeatColumn: PF planes of loads
+// ahead in an unrolled register ring, the same loads every step (the rims of
+// plane z1, never used, come from plane z1 - 1), and a branch-free form for
+// tiles inside the range.
+template <class T, int V, bool NT, int WY, int PF, bool FULL>
+struct PoisColumn {
+  static constexpr int RING = PF + 2;
+  PoisLdsV<T, V, WY> &L;
+  const Ptr<T> &w;
+  const Ptr<const T> &p;
+  int x0, z0, z1;
+  ColLane c;
+  int x, y, o, lx, ly;
+  double acc;
+  T pm[V];
+  PoisPlane<T, V> R[RING];
+
+  __device__ __forceinline__ PoisColumn(PoisLdsV<T, V, WY> &L_, const Ptr<T> &w_, const Ptr<const T> &p_, int x0_,
+                                        int y0, int xe, int ye, int z0_, int z1_)
+      : L(L_), w(w_), p(p_), x0(x0_), z0(z0_), z1(z1_) {
+    lx = (int)threadIdx.x;
+    ly = (int)threadIdx.y;
+    c = col_lane<V, WY>(x0, y0, xe, ye);
+    if (FULL) c.n = V;
+    x = x0 + V * lx;
+    y = y0 + ly;
+    o = V + V * lx;
+    acc = 0.0;
   }
-  for (int z = z0; z < z1; z++) {
-    const int b = (z - z0) & 1;
-    if (act) {
-      lds_st<T, V>(&L.sp[b][ly + 1][o], n, pc);
-      if (c.hxl) L.sp[b][ly + 1][V - 1] = rxl;
-      if (c.hxh) L.sp[b][ly + 1][V + c.cnt] = rxh;
-      if (c.hyl) lds_st<T, V>(&L.sp[b][0][o], n, ryl);
-      if (c.hyh) lds_st<T, V>(&L.sp[b][ly + 2][o], n, ryh);
-    }
-    T pnn[V];
-#pragma unroll
-    for (int j = 0; j < V; j++) pnn[j] = 0;
-    if (act && z + 1 < z1) {
-      const int zn = z + 1;
-      ldv<T, V>(p.at(x, y, zn + 1), n, pnn);
-      if (c.hxl) rxl = p(x0 - 1, y, zn);
-      if (c.hxh) rxh = p(x0 + c.cnt, y, zn);
-      if (c.hyl) ldv<T, V>(p.at(x, y - 1, zn), n, ryl);
-      if (c.hyh) ldv<T, V>(p.at(x, y + 1, zn), n, ryh);
+  __device__ __forceinline__ int n() const { return FULL ? V : c.n; }
+  __device__ __forceinline__ bool act() const { return FULL || c.n > 0; }
+
+  __device__ __forceinline__ void load(int q, PoisPlane<T, V> &P) const {
+    const int qk = min(q, z1 - 1);
+    ldv<T, V>(p.at(x, y, q), n(), P.p);
+    if (c.hxl) P.rxl = p(x0 - 1, y, qk);
+    if (c.hxh) P.rxh = p(x0 + c.cnt, y, qk);
+    if (c.hyl) ldv<T, V>(p.at(x, y - 1, qk), n(), P.ryl);
+    if (c.hyh) ldv<T, V>(p.at(x, y + 1, qk), n(), P.ryh);
+  }
+
+  template <int K>
+  __device__ __forceinline__ void step(int z) {
+    const PoisPlane<T, V> &C = R[K % RING], &N = R[(K + 1) % RING];
+    const int b = (z - z0) & 1, nn = n();
+    if (act()) {
+      lds_st<T, V>(&L.sp[b][ly + 1][o], nn, C.p);
+      if (c.hxl) L.sp[b][ly + 1][V - 1] = C.rxl;
+      if (c.hxh) L.sp[b][ly + 1][V + c.cnt] = C.rxh;
+      if (c.hyl) lds_st<T, V>(&L.sp[b][0][o], nn, C.ryl);
+      if (c.hyh) lds_st<T, V>(&L.sp[b][ly + 2][o], nn, C.ryh);
+      load(min(z + PF + 1, z1), R[(K + PF + 1) % RING]);
     }
     __syncthreads();
-    if (act) {
+    if (act()) {
       T pym[V], pyp[V], v[V];
       lds_ld<T, V>(&L.sp[b][ly][o], pym);
       lds_ld<T, V>(&L.sp[b][ly + 2][o], pyp);
       const T left = L.sp[b][ly + 1][o - 1];
 #pragma unroll
       for (int j = 0; j < V; j++) {
-        const T pxm = j == 0 ? left : pc[j == 0 ? 0 : j - 1];
-        const T pxp = j + 1 < n ? pc[j + 1 < V ? j + 1 : j] : L.sp[b][ly + 1][o + j + 1];
-        v[j] = (T)6.0 * pc[j] - (pxm + pxp) - (pym[j] + pyp[j]) - (pm[j] + pn[j]);
-        if (j < n) acc = acc + (double)v[j] * (double)pc[j];
+        const T pxm = j == 0 ? left : C.p[j == 0 ? 0 : j - 1];
+        const T pxp = j + 1 < nn ? C.p[j + 1 < V ? j + 1 : j] : L.sp[b][ly + 1][o + j + 1];
+        v[j] = (T)6.0 * C.p[j] - (pxm + pxp) - (pym[j] + pyp[j]) - (pm[j] + N.p[j]);
+        if (j < nn) acc = acc + (double)v[j] * (double)C.p[j];
       }
-      stv<T, V, NT>(w.at(x, y, z), n, v);
+      stv<T, V, NT>(w.at(x, y, z), nn, v);
+    }
+#pragma unroll
+    for (int j = 0; j < V; j++) pm[j] = C.p[j];
+  }
+  template <int K>
+  __device__ __forceinline__ void steps_all(int z) {
+    if constexpr (K < RING) {
+      step<K>(z);
+      steps_all<K + 1>(z + 1);
+    }
+  }
+  template <int K>
+  __device__ __forceinline__ void steps_tail(int z) {
+    if constexpr (K < RING - 1) {
+      if (z < z1) {
+        step<K>(z);
+        steps_tail<K + 1>(z + 1);
+      }
+    }
+  }
+
+  __device__ __forceinline__ double run() {
+#pragma unroll
+    for (int i = 0; i < RING; i++) {
+#pragma unroll
+      for (int j = 0; j < V; j++) R[i].p[j] = R[i].ryl[j] = R[i].ryh[j] = (T)0;
+      R[i].rxl = R[i].rxh = (T)0;
+    }
+#pragma unroll
+    for (int j = 0; j < V; j++) pm[j] = (T)0;
+    if (act()) {
+      ldv<T, V>(p.at(x, y, z0 - 1), n(), pm);
+#pragma unroll
+      for (int i = 0; i < PF + 1; i++) load(min(z0 + i, z1), R[i]);
+    }
+    int z = z0;
+    for (; z + RING <= z1; z += RING) steps_all<0>(z);
+    steps_tail<0>(z);
+    return acc;
+  }
+};
+
+template <class T, int V, bool NT, int WY = kWavesPerBlock, int PF = 1>
+__device__ __forceinline__ double poisson3_column_v(PoisLdsV<T, V, WY> &L, const Ptr<T> &w, const Ptr<const T> &p,
+                                                    int x0, int y0, int xe, int ye, int z0, int z1) {
+  if (x0 + kWave * V <= xe && y0 + WY <= ye)
+    return PoisColumn<T, V, NT, WY, PF, true>(L, w, p, x0, y0, xe, ye, z0, z1).run();
+  return PoisColumn<T, V, NT, WY, PF, false>(L, w, p, x0, y0, xe, ye, z0, z1).run();
+}
+
+// ------------------------------------------------------------ fused Poisson CG chains
+// The constant-coefficient operator's CG loops between two results, as one
+// column pass over the calc_w column's tiles (so the sums of w p are the
+// same bits):
+//   MODE 0 (FP64 CG, tea_leaf kernels):
+//     [AXPY] u = u + alpha_u p_old ; p_new = a p_old + b r ; w = A p_new
+//   MODE 1 (FP32 inner iterations of the mixed solver):
+//     [AXPY] e = e + alpha_u p_old ; p_new = mi rr + b p_old ; w = A p_new
+// with the separate loops' operations in their order; p_new is p_old outside
+// the loop range R.  Same ring structure as ChainColumn.
+template <class T>
+struct PoisChainPtrs {
+  Ptr<T> u, pnew, w;
+  Ptr<const T> pold, r;
+};
+
+template <class T, int V>
+struct PoisChainPlane {
+  T po[V], rr[V], u[V];
+  T yl_po[V], yl_r[V], yh_po[V], yh_r[V];
+  T xl_po, xl_r, xh_po, xh_r;
+};
+
+template <class T, int V, bool AXPY, int MODE, int WY, bool FULL>
+struct PoisChainColumn {
+  static constexpr int RING = 3;
+  PoisLdsV<T, V, WY> &L;
+  Ptr<T> u, pnew, w;
+  Ptr<const T> pold, r;
+  const Range3 &R;
+  T alpha_u, a, b;
+  int x0, z0, z1;
+  ColLane cl;
+  int x, y, o, lx, ly;
+  bool in_xl, in_xh, in_yl, in_yh;
+  int xl_r, xh_r, yl_r, yh_r;
+  double acc;
+  T pm[V], pc[V];
+  PoisChainPlane<T, V> P[RING];
+
+  __device__ __forceinline__ PoisChainColumn(PoisLdsV<T, V, WY> &L_, const PoisChainPtrs<T> &c, const Range3 &R_,
+                                             T alpha_u_, T a_, T b_, int x0_, int y0, int z0_, int z1_)
+      : L(L_), u(c.u), pnew(c.pnew), w(c.w), pold(c.pold), r(c.r), R(R_), alpha_u(alpha_u_), a(a_), b(b_), x0(x0_),
+        z0(z0_), z1(z1_) {
+    lx = (int)threadIdx.x;
+    ly = (int)threadIdx.y;
+    cl = col_lane<V, WY>(x0, y0, R.x1, R.y1);
+    if (FULL) cl.n = V;
+    x = x0 + V * lx;
+    y = y0 + ly;
+    o = V + V * lx;
+    in_xl = x0 - 1 >= R.x0;
+    in_xh = x0 + cl.cnt < R.x1;
+    in_yl = y - 1 >= R.y0;
+    in_yh = y + 1 < R.y1;
+    xl_r = in_xl ? x0 - 1 : x0;
+    xh_r = in_xh ? x0 + cl.cnt : x0 + cl.cnt - 1;
+    yl_r = in_yl ? y - 1 : y;
+    yh_r = in_yh ? y + 1 : y;
+    acc = 0.0;
+  }
+  __device__ __forceinline__ int n() const { return FULL ? V : cl.n; }
+  __device__ __forceinline__ bool act() const { return FULL || cl.n > 0; }
+  __device__ __forceinline__ T pnewv(T po, T rv, bool in) const {
+    if (!in) return po;
+    if constexpr (MODE == 0) return a * po + b * rv;
+    else return a * rv + b * po;  // a = mi
+  }
+
+  __device__ __forceinline__ void load(int q, PoisChainPlane<T, V> &Pl) const {
+    const int qk = min(q, z1 - 1);
+    const int qr = min(max(q, R.z0), R.z1 - 1);
+    const int nn = n();
+    ldv<T, V>(pold.at(x, y, q), nn, Pl.po);
+    ldv<T, V>(r.at(x, y, qr), nn, Pl.rr);
+    if (AXPY) ldv<T, V>(u.at(x, y, qk), nn, Pl.u);
+    if (cl.hxl) {
+      Pl.xl_po = pold(x0 - 1, y, qk);
+      Pl.xl_r = r(xl_r, y, qk);
+    }
+    if (cl.hxh) {
+      Pl.xh_po = pold(x0 + cl.cnt, y, qk);
+      Pl.xh_r = r(xh_r, y, qk);
+    }
+    if (cl.hyl) {
+      ldv<T, V>(pold.at(x, y - 1, qk), nn, Pl.yl_po);
+      ldv<T, V>(r.at(x, yl_r, qk), nn, Pl.yl_r);
+    }
+    if (cl.hyh) {
+      ldv<T, V>(pold.at(x, y + 1, qk), nn, Pl.yh_po);
+      ldv<T, V>(r.at(x, yh_r, qk), nn, Pl.yh_r);
+    }
+  }
+
+  template <int K>
+  __device__ __forceinline__ void step(int z) {
+    const PoisChainPlane<T, V> &C = P[K % RING], &N = P[(K + 1) % RING];
+    const int bb = (z - z0) & 1, nn = n();
+    T pn[V];
+    if (act()) {
+      lds_st<T, V>(&L.sp[bb][ly + 1][o], nn, pc);
+      if (cl.hxl) L.sp[bb][ly + 1][V - 1] = pnewv(C.xl_po, C.xl_r, in_xl);
+      if (cl.hxh) L.sp[bb][ly + 1][V + cl.cnt] = pnewv(C.xh_po, C.xh_r, in_xh);
+      if (cl.hyl) {
+        T t[V];
+#pragma unroll
+        for (int j = 0; j < V; j++) t[j] = pnewv(C.yl_po[j], C.yl_r[j], in_yl);
+        lds_st<T, V>(&L.sp[bb][0][o], nn, t);
+      }
+      if (cl.hyh) {
+        T t[V];
+#pragma unroll
+        for (int j = 0; j < V; j++) t[j] = pnewv(C.yh_po[j], C.yh_r[j], in_yh);
+        lds_st<T, V>(&L.sp[bb][ly + 2][o], nn, t);
+      }
+      load(min(z + 2, z1), P[(K + 2) % RING]);
+    }
+    __syncthreads();
+    const bool in_n = z + 1 < R.z1;
+#pragma unroll
+    for (int j = 0; j < V; j++) pn[j] = pnewv(N.po[j], N.rr[j], in_n);
+    if (act()) {
+      T pym[V], pyp[V], v[V], unew[V];
+      lds_ld<T, V>(&L.sp[bb][ly][o], pym);
+      lds_ld<T, V>(&L.sp[bb][ly + 2][o], pyp);
+      const T left = L.sp[bb][ly + 1][o - 1];
+#pragma unroll
+      for (int j = 0; j < V; j++) {
+        const T pxm = j == 0 ? left : pc[j == 0 ? 0 : j - 1];
+        const T pxp = j + 1 < nn ? pc[j + 1 < V ? j + 1 : j] : L.sp[bb][ly + 1][o + j + 1];
+        v[j] = (T)6.0 * pc[j] - (pxm + pxp) - (pym[j] + pyp[j]) - (pm[j] + pn[j]);
+        unew[j] = C.u[j] + alpha_u * C.po[j];
+        if (j < nn) acc = acc + (double)v[j] * (double)pc[j];
+      }
+      stv<T, V, true>(w.at(x, y, z), nn, v);
+      stv<T, V, true>(pnew.at(x, y, z), nn, pc);
+      if (AXPY) stv<T, V, true>(u.at(x, y, z), nn, unew);
     }
 #pragma unroll
     for (int j = 0; j < V; j++) {
       pm[j] = pc[j];
       pc[j] = pn[j];
-      pn[j] = pnn[j];
     }
   }
-  return acc;
+  template <int K>
+  __device__ __forceinline__ void steps_all(int z) {
+    if constexpr (K < RING) {
+      step<K>(z);
+      steps_all<K + 1>(z + 1);
+    }
+  }
+  template <int K>
+  __device__ __forceinline__ void steps_tail(int z) {
+    if constexpr (K < RING - 1) {
+      if (z < z1) {
+        step<K>(z);
+        steps_tail<K + 1>(z + 1);
+      }
+    }
+  }
+
+  __device__ __forceinline__ double run() {
+#pragma unroll
+    for (int i = 0; i < RING; i++) {
+#pragma unroll
+      for (int j = 0; j < V; j++)
+        P[i].po[j] = P[i].rr[j] = P[i].u[j] = P[i].yl_po[j] = P[i].yl_r[j] = P[i].yh_po[j] = P[i].yh_r[j] = (T)0;
+      P[i].xl_po = P[i].xl_r = P[i].xh_po = P[i].xh_r = (T)0;
+    }
+#pragma unroll
+    for (int j = 0; j < V; j++) pm[j] = pc[j] = (T)0;
+    if (act()) {
+      const int qm = z0 - 1;
+      T po[V], rv[V];
+      ldv<T, V>(pold.at(x, y, qm), n(), po);
+      ldv<T, V>(r.at(x, y, max(qm, R.z0)), n(), rv);
+      load(z0, P[0]);
+      load(min(z0 + 1, z1), P[1]);
+#pragma unroll
+      for (int j = 0; j < V; j++) {
+        pm[j] = pnewv(po[j], rv[j], qm >= R.z0);
+        pc[j] = pnewv(P[0].po[j], P[0].rr[j], true);
+      }
+    }
+    int z = z0;
+    for (; z + RING <= z1; z += RING) steps_all<0>(z);
+    steps_tail<0>(z);
+    return acc;
+  }
+};
+
+template <class T, int V, bool AXPY, int MODE, int WY>
+__device__ __forceinline__ double poisson3_chain_column_v(PoisLdsV<T, V, WY> &L, const PoisChainPtrs<T> &c,
+                                                          const Range3 &R, T alpha_u, T a, T b, int x0, int y0, int z0,
+                                                          int z1) {
+  if (x0 + kWave * V <= R.x1 && y0 + WY <= R.y1)
+    return PoisChainColumn<T, V, AXPY, MODE, WY, true>(L, c, R, alpha_u, a, b, x0, y0, z0, z1).run();
+  return PoisChainColumn<T, V, AXPY, MODE, WY, false>(L, c, R, alpha_u, a, b, x0, y0, z0, z1).run();
 }
 
 }  // namespace opsmi

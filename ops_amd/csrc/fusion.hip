@@ -154,9 +154,91 @@ int cg_chain_rule_level2(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
   return inst().fusion >= 2 ? cg_chain_rule(run, at, s) : 0;
 }
 
+// Constant-coefficient Poisson chains (poisson3_chain_column_v), on the
+// tiles of the Poisson calc_w column (k_poisson3_march) so that the sums of
+// w p are the same bits:
+//   MODE 0, FP64 CG:   [tea_leaf_axpy] + tea_leaf_axpby + poisson_calc_w
+//                      (48 instead of 24 + 40 - ... = 56 bytes per point
+//                      for the pair and the sweep)
+//   MODE 1, FP32 inner iterations of the mixed solver:
+//                      [mp_e_update] + mp_p + poisson_calc_w (float)
+//                      (24 instead of 12 + 12 + 8 bytes per point)
+template <class T, bool AXPY, int MODE>
+__global__ __launch_bounds__(kWave * kHeatRows) void k_pois_chain(PoisChainPtrs<T> c, Range3 R, T alpha_u, T a, T b,
+                                                                  int chunk, RedCtl ctl) {
+  constexpr int V = 16 / (int)sizeof(T);
+  __shared__ PoisLdsV<T, V, kHeatRows> lds;
+  const TileId t = tile_id();
+  const int z0 = R.z0 + (int)t.z * chunk;
+  double acc[1] = {poisson3_chain_column_v<T, V, AXPY, MODE, kHeatRows>(
+      lds, c, R, alpha_u, a, b, R.x0 + (int)t.x * V * kWave, R.y0 + (int)t.y * kHeatRows, z0, min(z0 + chunk, R.z1))};
+  reduce_block_store<double, OPS_INC, 1>(acc, ctl);
+}
+
+template <class T, int MODE>
+int pois_chain_rule_t(std::vector<LoopRec> &run, size_t at) {
+  const LoopKind ka = MODE == 0 ? LK_AXPY : LK_MP_E, kb = MODE == 0 ? LK_AXPBY : LK_MP_P;
+  size_t i = at;
+  const LoopRec *ax = nullptr;
+  if (i < run.size() && run[i].kind == ka) ax = &run[i++];
+  if (i + 1 >= run.size()) return 0;
+  const LoopRec &bp = run[i], &cw = run[i + 1];
+  if (bp.kind != kb || cw.kind != LK_POISSON_W3 || cw.g[0] != (double)sizeof(T) || bp.dims != 3) return 0;
+  ops_dat P = bp.d[0], Rr = bp.d[1], W = cw.d[0];
+  if (cw.d[1] != P || P == Rr || W == P || W == Rr) return 0;
+  if (!same_range(bp.range, cw.range) || cw.range.npts() <= 0) return 0;
+  ops_dat U = nullptr;
+  if (ax) {
+    U = ax->d[0];
+    if (ax->d[1] != P || !same_range(ax->range, cw.range) || U == P || U == Rr || U == W || ax->dims != 3)
+      ax = nullptr, U = nullptr;
+  }
+  const Range3 R = cw.range;
+  for (ops_dat d : {P, Rr, W})
+    if (!vec16_ok(d, R.x0)) return 0;
+  if (U && !vec16_ok(U, R.x0)) ax = nullptr, U = nullptr;
+  if (!use_vec() || cw.ctl.slot == nullptr) return 0;
+  constexpr int V = 16 / (int)sizeof(T);
+  const ColGrid g = heat_column_grid(R, V * kWave, kHeatRows);
+  ensure_partials((size_t)g.blocks() * 8);
+  PoisChainPtrs<T> c;
+  DatView pv = view_of(P);
+  c.pnew = Ptr<T>{(T *)spare_base(P, R), pv.sy, pv.sz};
+  c.pold = ptr_of<const T>(P);
+  c.r = ptr_of<const T>(Rr);
+  c.w = ptr_of<T>(W);
+  c.u = U ? ptr_of<T>(U) : Ptr<T>{nullptr, 0, 0};
+  const T alpha_u = ax ? (T)ax->g[0] : (T)0, a = (T)bp.g[0], b = (T)bp.g[1];
+  const RedCtl ctl = with_scratch(cw.ctl);
+  const double bytes = (ax ? ax->bytes : 0.0) + bp.bytes + cw.bytes;
+  const char *nm = MODE == 0 ? (ax ? "poisson_chain(axpy+axpby+calc_w)" : "poisson_chain(axpby+calc_w)")
+                             : (ax ? "mp_chain(e+p+calc_w)" : "mp_chain(p+calc_w)");
+  run_loop(nm, bytes, [&](hipStream_t st) {
+    auto kern = ax ? k_pois_chain<T, true, MODE> : k_pois_chain<T, false, MODE>;
+    hipLaunchKernelGGL(kern, dim3(g.gx, g.gy, g.gz), dim3(kWave, kHeatRows), 0, st, c, R, alpha_u, a, b, g.chunk,
+                       ctl);
+    launch_red_finish<double, OPS_INC, 1>(ctl, g.blocks(), st);
+  });
+  swap_spare(P, R);
+  if (U) note_external_write(U);
+  note_external_write(W);
+  if (ax) note_fused(ax->name.c_str(), ax->bytes);
+  note_fused(bp.name.c_str(), bp.bytes);
+  note_fused(cw.name.c_str(), cw.bytes);
+  return (int)(i + 2 - at);
+}
+
+int pois_chain_rule_level2(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
+  (void)s;
+  if (inst().fusion < 2) return 0;
+  if (int k = pois_chain_rule_t<double, 0>(run, at)) return k;
+  return pois_chain_rule_t<float, 1>(run, at);
+}
+
 struct Register {
   Register() {
     register_fusion(cg_chain_rule_level2);
+    register_fusion(pois_chain_rule_level2);
     register_fusion(axpy_axpby_rule);
   }
 } g_register;

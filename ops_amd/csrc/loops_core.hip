@@ -333,16 +333,23 @@ struct PoissonW {
 };
 
 // 3D form as z-marching columns with 16 bytes per lane (2 doubles or 4
-// floats), p planes in LDS: poisson3_column_v, same arithmetic.
+// floats), (64 V) x kHeatRows tiles, p planes in LDS: poisson3_column_v, same
+// arithmetic.  Planes of loads in flight: at 768^3 FP64 1464 / 1478 / 1476 /
+// 1432 us and FP32 759 / 769 / 782 / 767 us for 1 / 2 / 3 / 4.
 template <class T>
-__global__ __launch_bounds__(256) void k_poisson3_march(Ptr<T> w, Ptr<const T> p, Range3 rg, int chunk, RedCtl ctl) {
+constexpr int pois_pf() {
+  return sizeof(T) == 8 ? 4 : 1;
+}
+template <class T, int kPoisPF>
+__global__ __launch_bounds__(kWave * kHeatRows) void k_poisson3_march(Ptr<T> w, Ptr<const T> p, Range3 rg, int chunk,
+                                                                      RedCtl ctl) {
   constexpr int V = 16 / (int)sizeof(T);
-  __shared__ PoisLdsV<T, V> lds;
+  __shared__ PoisLdsV<T, V, kHeatRows> lds;
   const TileId t = tile_id();
   const int z0 = rg.z0 + (int)t.z * chunk;
-  double acc[1] = {poisson3_column_v<T, V, true>(lds, w, p, rg.x0 + (int)t.x * kWave * V,
-                                                 rg.y0 + (int)t.y * kWavesPerBlock, rg.x1, rg.y1, z0,
-                                                 min(z0 + chunk, rg.z1))};
+  double acc[1] = {poisson3_column_v<T, V, true, kHeatRows, kPoisPF>(lds, w, p, rg.x0 + (int)t.x * kWave * V,
+                                                                     rg.y0 + (int)t.y * kHeatRows, rg.x1, rg.y1, z0,
+                                                                     min(z0 + chunk, rg.z1))};
   reduce_block_store<double, OPS_INC, 1>(acc, ctl);
 }
 
@@ -360,17 +367,14 @@ LoopRec make_poisson3_loop(const char *name, const ops_arg *args, int nargs, con
     return rec;
   }
   constexpr int V = 16 / (int)sizeof(T);
-  const int nx = r.x1 - r.x0, ny = r.y1 - r.y0, nz = r.z1 - r.z0;
-  const int chunk = heat_zchunk();
-  const unsigned gx = (nx + kWave * V - 1) / (kWave * V), gy = (ny + kWavesPerBlock - 1) / kWavesPerBlock;
-  const unsigned gz = (nz + chunk - 1) / chunk;
-  const RedCtl ctl = red_prepare(*red, 1, DT_F64, gx * gy * gz);
+  const ColGrid g = heat_column_grid(r, kWave * V, kHeatRows);
+  const RedCtl ctl = red_prepare(*red, 1, DT_F64, g.blocks());
   rec.ctl = ctl;
   rec.launch = [=](hipStream_t s) {
     const RedCtl c = with_scratch(ctl);
-    hipLaunchKernelGGL(k_poisson3_march<T>, dim3(gx, gy, gz), dim3(kWave, kWavesPerBlock), 0, s, ptr_of<T>(w.dat),
-                       ptr_of<const T>(p.dat), r, chunk, c);
-    launch_red_finish<double, OPS_INC, 1>(c, gx * gy * gz, s);
+    hipLaunchKernelGGL((k_poisson3_march<T, pois_pf<T>()>), dim3(g.gx, g.gy, g.gz), dim3(kWave, kHeatRows), 0, s,
+                       ptr_of<T>(w.dat), ptr_of<const T>(p.dat), r, g.chunk, c);
+    launch_red_finish<double, OPS_INC, 1>(c, g.blocks(), s);
   };
   return rec;
 }
@@ -783,8 +787,14 @@ void ops_par_loop_poisson_calc_w_reduce_kernel(const char *name, ops_block block
   if (dim < 2) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": needs a 2D or 3D block"));
   const bool col = dim == 3 && use_march() && use_vec() && vec16_ok(w.dat, r.x0) && vec16_ok(p.dat, r.x0);
   if (col) {
-    if (dt == DT_F64) enqueue(make_poisson3_loop<double>(name, args, nargs, r, w, p, &pw));
-    else enqueue(make_poisson3_loop<float>(name, args, nargs, r, w, p, &pw));
+    LoopRec rec = dt == DT_F64 ? make_poisson3_loop<double>(name, args, nargs, r, w, p, &pw)
+                               : make_poisson3_loop<float>(name, args, nargs, r, w, p, &pw);
+    rec.kind = LK_POISSON_W3;
+    rec.dims = 3;
+    rec.d[0] = w.dat;
+    rec.d[1] = p.dat;
+    rec.g[0] = dt == DT_F64 ? 8 : 4;
+    enqueue(std::move(rec));
   } else if (dt == DT_F64) {
     if (dim == 2) LAUNCH_RED(r, pw, PoissonW<double, 2>{PD(w), PCD(p)});
     else LAUNCH_RED(r, pw, PoissonW<double, 3>{PD(w), PCD(p)});

@@ -120,6 +120,59 @@ struct MpUpdate {
   }
 };
 
+// the rr half of MpUpdate (same arithmetic and dots)
+struct MpRrUpdate {
+  using red_t = double;
+  static constexpr int red_op = OPS_INC;
+  static constexpr int red_dim = 2;
+  static constexpr int kVec = 4;
+  Ptr<float> rr;
+  Ptr<const float> w;
+  float a, mi;
+  __device__ void operator()(int i, int j, int k, double (&s)[2]) const {
+    const float q = rr(i, j, k) - a * w(i, j, k);
+    rr(i, j, k) = q;
+    const float z = mi * q;
+    s[0] = s[0] + (double)q * (double)z;
+    s[1] = s[1] + (double)q * (double)q;
+  }
+  template <int V>
+  __device__ void vop(int i, int j, int k, int n, double (&s)[2]) const {
+    float rv[V], wv[V];
+    ldv<float, V>(rr.at(i, j, k), n, rv);
+    ldv<float, V>(w.at(i, j, k), n, wv);
+#pragma unroll
+    for (int c = 0; c < V; c++) {
+      rv[c] = rv[c] - a * wv[c];
+      const float z = mi * rv[c];
+      if (c < n) {
+        s[0] = s[0] + (double)rv[c] * (double)z;
+        s[1] = s[1] + (double)rv[c] * (double)rv[c];
+      }
+    }
+    stv<float, V, true>(rr.at(i, j, k), n, rv);
+  }
+};
+
+// the e half of MpUpdate
+struct MpEUpdate {
+  OPSMI_NO_RED
+  static constexpr int kVec = 4;
+  Ptr<float> e;
+  Ptr<const float> p;
+  float a;
+  __device__ void operator()(int i, int j, int k, double (&)[1]) const { e(i, j, k) = e(i, j, k) + a * p(i, j, k); }
+  template <int V>
+  __device__ void vop(int i, int j, int k, int n, double (&)[1]) const {
+    float ev[V], pv[V];
+    ldv<float, V>(e.at(i, j, k), n, ev);
+    ldv<float, V>(p.at(i, j, k), n, pv);
+#pragma unroll
+    for (int c = 0; c < V; c++) ev[c] = ev[c] + a * pv[c];
+    stv<float, V, true>(e.at(i, j, k), n, ev);
+  }
+};
+
 struct MpP {
   OPSMI_NO_RED
   static constexpr int kVec = 4;
@@ -222,6 +275,40 @@ void ops_par_loop_mp_update_kernel(const char *name, ops_block block, int dim, i
   OPSMI_API_END()
 }
 
+void ops_par_loop_mp_rr_update_kernel(const char *name, ops_block block, int dim, int *range, ops_arg rr,
+                                      ops_arg w, ops_arg a, ops_arg mi, ops_arg dots) {
+  OPSMI_API_BEGIN
+  ARGS(rr, w, a, mi, dots)
+  Range3 rg = check_loop(name, block, dim, range, args,
+                         {dat_w(DT_F32), dat_r(pts_point(), DT_F32), gbl_r(DT_F32), gbl_r(DT_F32),
+                          red_a(OPS_INC, DT_F64, 2)},
+                         nullptr);
+  const float av = gbl<float>(a), m = gbl<float>(mi);
+  launch_loop(name, args, nargs, rg,
+              [=] { return MpRrUpdate{ptr_of<float>(rr.dat), ptr_of<const float>(w.dat), av, m}; }, &dots);
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+
+void ops_par_loop_mp_e_update_kernel(const char *name, ops_block block, int dim, int *range, ops_arg e, ops_arg p,
+                                     ops_arg a) {
+  OPSMI_API_BEGIN
+  ARGS(e, p, a)
+  Range3 rg = check_loop(name, block, dim, range, args, {dat_w(DT_F32), dat_r(pts_point(), DT_F32), gbl_r(DT_F32)},
+                         nullptr);
+  const float av = gbl<float>(a);
+  LoopRec rec =
+      make_loop(name, args, nargs, rg, [=] { return MpEUpdate{ptr_of<float>(e.dat), ptr_of<const float>(p.dat), av}; });
+  rec.kind = LK_MP_E;
+  rec.dims = dim;
+  rec.d[0] = e.dat;
+  rec.d[1] = p.dat;
+  rec.g[0] = av;
+  enqueue(std::move(rec));
+  finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+
 void ops_par_loop_mp_p_kernel(const char *name, ops_block block, int dim, int *range, ops_arg p, ops_arg rr,
                               ops_arg mi, ops_arg b) {
   OPSMI_API_BEGIN
@@ -229,8 +316,15 @@ void ops_par_loop_mp_p_kernel(const char *name, ops_block block, int dim, int *r
   Range3 rg = check_loop(name, block, dim, range, args,
                          {dat_w(DT_F32), dat_r(pts_point(), DT_F32), gbl_r(DT_F32), gbl_r(DT_F32)}, nullptr);
   const float m = gbl<float>(mi), bv = gbl<float>(b);
-  launch_loop(name, args, nargs, rg,
-              [=] { return MpP{ptr_of<float>(p.dat), ptr_of<const float>(rr.dat), m, bv}; });
+  LoopRec rec =
+      make_loop(name, args, nargs, rg, [=] { return MpP{ptr_of<float>(p.dat), ptr_of<const float>(rr.dat), m, bv}; });
+  rec.kind = LK_MP_P;
+  rec.dims = dim;
+  rec.d[0] = p.dat;
+  rec.d[1] = rr.dat;
+  rec.g[0] = m;
+  rec.g[1] = bv;
+  enqueue(std::move(rec));
   finish_loop(args, nargs);
   OPSMI_API_END()
 }

@@ -232,7 +232,9 @@ enum LoopKind {
   LK_AXPY,       // tea_leaf_axpy_kernel: u = u + a p
   LK_AXPBY,      // tea_leaf_axpby_kernel: u = a u + b p
   LK_HEAT_W3,    // tea_leaf_cg_calc_w_reduce_kernel (3D): w = A p, pw
-  LK_POISSON_W,  // poisson_calc_w_reduce_kernel
+  LK_POISSON_W3, // poisson_calc_w_reduce_kernel (3D column form): w = A p, pw; g[0] = element bytes
+  LK_MP_E,       // mp_e_update_kernel: e = e + a p (float)
+  LK_MP_P,       // mp_p_kernel: p = mi rr + b p (float)
 };
 struct LoopRec {
   std::string name;

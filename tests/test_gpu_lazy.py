@@ -104,3 +104,40 @@ def test_fused_cg_chain_bit_identical(gpu_ops, n):
         for key in ("pw", "alpha", "beta", "rrn"):
             assert np.array_equal(h0[key], hist[key]), key
         assert np.array_equal(u0, u)
+
+
+@pytest.mark.parametrize("n", [(48, 40, 33), (130, 66, 40), (260, 37, 21)])
+def test_fused_poisson_chains_bit_identical(gpu_ops, n):
+    """The constant-coefficient chains: FP64 CG (axpy + axpby + Poisson
+    calc_w) and the mixed solver's FP32 inner iterations (e update + p update
+    + calc_w) give the same bits at fusion levels 0, 1 and 2."""
+    from ops_amd import apps
+    ops = gpu_ops
+    f = O.poisson_rhs(31, 3, *n)
+    runs = []
+    for fusion in (0, 1, 2):
+        ops.lib().ops_mi_set_fusion(fusion)
+        try:
+            ops.ops_mi_reset_stats()
+            P = apps.Poisson(3, *n, f=f)
+            k, hist = P.iterate(25, history=True)
+            x = P.x()
+            P.close()
+            M = apps.PoissonMixed(3, *n, f=f)
+            res = M.solve(tol=1e-9)
+            xm = M.x()
+            M.close()
+            st = [ops.ops_mi_loop_stats(nm) for nm in ("poisson_chain(axpy+axpby+calc_w)", "mp_chain(e+p+calc_w)")]
+        finally:
+            ops.lib().ops_mi_set_fusion(2)
+        runs.append((hist, x, res, xm, [s[0] if s else 0 for s in st]))
+    (h0, x0, r0, m0, s0), _, (h2, x2, r2, m2, s2) = runs
+    assert s0 == [0, 0] and runs[1][4] == [0, 0]
+    # every array takes 16-byte lanes (rows of at least 1 KiB are padded)
+    if n[0] + 2 >= 128:
+        assert s2[0] >= 23 and s2[1] > 0, s2
+    for _, x, r, xm, _ in runs[1:]:
+        assert np.array_equal(x0, x) and np.array_equal(m0, xm)
+        assert r["inner_total"] == r0["inner_total"] and np.array_equal(r["relres"], r0["relres"])
+    for key in ("rrn", "alpha", "beta"):
+        assert np.array_equal(h0[key], h2[key]), key
