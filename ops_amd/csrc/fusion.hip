@@ -235,10 +235,127 @@ int pois_chain_rule_level2(std::vector<LoopRec> &run, size_t at, hipStream_t s) 
   return pois_chain_rule_t<float, 1>(run, at);
 }
 
+// Jacobi step of the Laplace tutorial (apps/c/laplace2d_tutorial/step7/
+// laplace2d.cpp: apply_stencil, its error result, copy).  copy(k), A = Anew,
+// is still queued when apply_stencil(k + 1) arrives, and the two run as one
+// pass over R:
+//   A = Anew_old                                    (the copy)
+//   Anew' = 0.25 (A'(x+1) + A'(x-1) + A'(y-1) + A'(y+1)),  err = max(err, |Anew' - A'|)
+// where A' is A after the copy (Anew_old inside R, A's own values outside).
+// Anew' goes to Anew's spare buffer (neighbouring waves still read
+// Anew_old), which then becomes Anew's buffer.  24 bytes per point instead of
+// 16 + 16, and one launch.  Each wave marches kJacRows rows of a (64 V)-wide
+// strip keeping three rows in registers; x neighbours come from the adjacent
+// lanes (shuffles) and, at the strip ends, one scalar load.
+constexpr int kJacRows = 16;
+
+template <class T>
+struct JacPtrs {
+  Ptr<T> A, out;
+  Ptr<const T> An;
+};
+
+template <class T, int V>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_jacobi_chain(JacPtrs<T> c, Range3 R, RedCtl ctl) {
+  T err[1] = {RedOp<T, OPS_MAX>::identity()};
+  const int lx = (int)threadIdx.x, wv = (int)threadIdx.y;
+  const TileId t = tile_id();
+  const int x0 = R.x0 + (int)t.x * kWave * V;
+  const int ya = R.y0 + ((int)t.y * kWavesPerBlock + wv) * kJacRows;
+  const int yb = min(ya + kJacRows, R.y1);
+  const int z = R.z0;
+  const int cnt = min(kWave * V, R.x1 - x0);
+  const int x = x0 + V * lx;
+  const int n = max(0, min(V, cnt - V * lx));
+  const bool hxh = lx == (cnt - 1) / V;
+  const bool in_xl = x0 - 1 >= R.x0, in_xh = x0 + cnt < R.x1;
+  auto row = [&](int yy, T (&v)[V]) {
+    const bool in = yy >= R.y0 && yy < R.y1;
+    ldv<T, V>(in ? c.An.at(x, yy, z) : (const T *)c.A.at(x, yy, z), n, v);
+  };
+  auto edge = [&](int xx, int yy, bool inx) -> T {
+    return inx && yy >= R.y0 && yy < R.y1 ? c.An(xx, yy, z) : c.A(xx, yy, z);
+  };
+  if (ya < yb) {
+    T up[V], cur[V], dn[V];
+    row(ya - 1, up);
+    row(ya, cur);
+    T cl = lx == 0 ? edge(x0 - 1, ya, in_xl) : (T)0, cr = hxh ? edge(x0 + cnt, ya, in_xh) : (T)0;
+    for (int y = ya; y < yb; y++) {
+      row(y + 1, dn);
+      T ncl = 0, ncr = 0;
+      if (y + 1 < yb) {
+        if (lx == 0) ncl = edge(x0 - 1, y + 1, in_xl);
+        if (hxh) ncr = edge(x0 + cnt, y + 1, in_xh);
+      }
+      const T fromleft = __shfl_up(cur[V - 1], 1, kWave), fromright = __shfl_down(cur[0], 1, kWave);
+      T v[V];
+#pragma unroll
+      for (int j = 0; j < V; j++) {
+        const T xm = j > 0 ? cur[j > 0 ? j - 1 : 0] : (lx == 0 ? cl : fromleft);
+        const T xp = j + 1 < n ? cur[j + 1 < V ? j + 1 : j] : (hxh ? cr : fromright);
+        v[j] = (T)0.25f * (xp + xm + up[j] + dn[j]);
+        if (j < n) err[0] = RedOp<T, OPS_MAX>::combine(err[0], fabs(v[j] - cur[j]));
+      }
+      if (n > 0) {
+        stv<T, V, false>(c.A.at(x, y, z), n, cur);
+        stv<T, V, false>(c.out.at(x, y, z), n, v);
+      }
+#pragma unroll
+      for (int j = 0; j < V; j++) {
+        up[j] = cur[j];
+        cur[j] = dn[j];
+      }
+      cl = ncl;
+      cr = ncr;
+    }
+  }
+  reduce_block_store<T, OPS_MAX, 1>(err, ctl);
+}
+
+template <class T>
+void launch_jacobi_chain(const LoopRec &cp, const LoopRec &jac, const Range3 &R) {
+  constexpr int V = 16 / (int)sizeof(T);
+  ops_dat A = jac.d[0], An = jac.d[1];
+  const unsigned gx = (unsigned)((R.x1 - R.x0 + kWave * V - 1) / (kWave * V));
+  const unsigned gy = (unsigned)((R.y1 - R.y0 + kWavesPerBlock * kJacRows - 1) / (kWavesPerBlock * kJacRows));
+  ensure_partials((size_t)gx * gy * sizeof(T));
+  JacPtrs<T> c;
+  DatView v = view_of(An);
+  c.out = Ptr<T>{(T *)spare_base(An, R), v.sy, v.sz};
+  c.A = ptr_of<T>(A);
+  c.An = ptr_of<const T>(An);
+  const RedCtl ctl = with_scratch(jac.ctl);
+  run_loop("jacobi_chain(copy+apply_stencil)", cp.bytes + jac.bytes, [&](hipStream_t st) {
+    hipLaunchKernelGGL((k_jacobi_chain<T, V>), dim3(gx, gy, 1), dim3(kWave, kWavesPerBlock), 0, st, c, R, ctl);
+    launch_red_finish<T, OPS_MAX, 1>(ctl, gx * gy, st);
+  });
+  swap_spare(An, R);
+  note_external_write(A);
+}
+
+int jacobi_chain_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
+  (void)s;
+  if (inst().fusion < 2 || at + 1 >= run.size()) return 0;
+  const LoopRec &cp = run[at], &jac = run[at + 1];
+  if (cp.kind != LK_COPY || jac.kind != LK_JACOBI5 || cp.dims != 2 || jac.dims != 2) return 0;
+  ops_dat A = jac.d[0], An = jac.d[1];
+  if (cp.d[0] != A || cp.d[1] != An || A == An || cp.g[0] != jac.g[0]) return 0;
+  if (!same_range(cp.range, jac.range) || jac.range.npts() <= 0 || jac.ctl.slot == nullptr) return 0;
+  const Range3 R = jac.range;
+  if (!use_vec() || !vec16_ok(A, R.x0) || !vec16_ok(An, R.x0)) return 0;
+  if (jac.g[0] == 8) launch_jacobi_chain<double>(cp, jac, R);
+  else launch_jacobi_chain<float>(cp, jac, R);
+  note_fused(cp.name.c_str(), cp.bytes);
+  note_fused(jac.name.c_str(), jac.bytes);
+  return 2;
+}
+
 struct Register {
   Register() {
     register_fusion(cg_chain_rule_level2);
     register_fusion(pois_chain_rule_level2);
+    register_fusion(jacobi_chain_rule);
     register_fusion(axpy_axpby_rule);
   }
 } g_register;

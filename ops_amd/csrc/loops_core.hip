@@ -515,8 +515,14 @@ void ops_par_loop_copy(const char *name, ops_block block, int dim, int *range, o
   ARGS(a0, a1)
   DType dt;
   Range3 r = check_loop(name, block, dim, range, args, {dat_w(), dat_r()}, &dt);
-  if (dt == DT_F64) LAUNCH(r, Copy<double>{PD(a0), PCD(a1)});
-  else LAUNCH(r, Copy<float>{PF(a0), PCF(a1)});
+  LoopRec rec = dt == DT_F64 ? make_loop(name, args, nargs, r, [=] { return Copy<double>{PD(a0), PCD(a1)}; })
+                             : make_loop(name, args, nargs, r, [=] { return Copy<float>{PF(a0), PCF(a1)}; });
+  rec.kind = LK_COPY;
+  rec.dims = dim;
+  rec.d[0] = a0.dat;
+  rec.d[1] = a1.dat;
+  rec.g[0] = dt == DT_F64 ? 8 : 4;
+  enqueue(std::move(rec));
   finish_loop(args, nargs);
   OPSMI_API_END()
 }
@@ -556,8 +562,16 @@ void ops_par_loop_apply_stencil(const char *name, ops_block block, int dim, int 
   DType dt;
   Range3 r = check_loop(name, block, dim, range, args, {dat_r(pts_5pt()), dat_w(), red_a(OPS_MAX)}, &dt);
   need_dim(name, dim, 2);
-  if (dt == DT_F64) LAUNCH_RED(r, a2, ApplyStencil<double>{PCD(a0), PD(a1)});
-  else LAUNCH_RED(r, a2, ApplyStencil<float>{PCF(a0), PF(a1)});
+  LoopRec rec =
+      dt == DT_F64
+          ? make_loop(name, args, nargs, r, [=] { return ApplyStencil<double>{PCD(a0), PD(a1)}; }, &a2)
+          : make_loop(name, args, nargs, r, [=] { return ApplyStencil<float>{PCF(a0), PF(a1)}; }, &a2);
+  rec.kind = LK_JACOBI5;
+  rec.dims = dim;
+  rec.d[0] = a0.dat;
+  rec.d[1] = a1.dat;
+  rec.g[0] = dt == DT_F64 ? 8 : 4;
+  enqueue(std::move(rec));
   finish_loop(args, nargs);
   OPSMI_API_END()
 }

@@ -235,6 +235,8 @@ enum LoopKind {
   LK_POISSON_W3, // poisson_calc_w_reduce_kernel (3D column form): w = A p, pw; g[0] = element bytes
   LK_MP_E,       // mp_e_update_kernel: e = e + a p (float)
   LK_MP_P,       // mp_p_kernel: p = mi rr + b p (float)
+  LK_COPY,       // copy: d[0] = d[1]; g[0] = element bytes
+  LK_JACOBI5,    // apply_stencil: d[1] = 5-point average of d[0], MAX |change|; g[0] = element bytes
 };
 struct LoopRec {
   std::string name;

@@ -141,3 +141,31 @@ def test_fused_poisson_chains_bit_identical(gpu_ops, n):
         assert r["inner_total"] == r0["inner_total"] and np.array_equal(r["relres"], r0["relres"])
     for key in ("rrn", "alpha", "beta"):
         assert np.array_equal(h0[key], h2[key]), key
+
+
+@pytest.mark.parametrize("n", [(130, 90), (61, 47), (256, 129), (2048, 70)])
+def test_fused_jacobi_step_bit_identical(gpu_ops, n):
+    """The Laplace tutorial's copy and next apply_stencil run as one pass at
+    fusion level 2 (rows that allow 16-byte lanes); field and per-iteration
+    max-change history are the same bits at levels 0, 1 and 2."""
+    from ops_amd import apps
+    ops = gpu_ops
+    nx, ny = n
+    runs = []
+    for fusion in (0, 1, 2):
+        ops.lib().ops_mi_set_fusion(fusion)
+        try:
+            ops.ops_mi_reset_stats()
+            err, hist, A = apps.laplace2d(nx, ny, 30, 0, O.laplace_init(nx, ny, 0))
+            st = ops.ops_mi_loop_stats("jacobi_chain(copy+apply_stencil)")
+        finally:
+            ops.lib().ops_mi_set_fusion(2)
+        runs.append((hist, A, st[0] if st else 0))
+    (h0, A0, s0), (h1, A1, s1), (h2, A2, s2) = runs
+    assert s0 == 0 and s1 == 0
+    if ((nx + 2) * 8) % 16 == 0 or (nx + 2) * 8 >= 1024:
+        assert s2 == 29
+    for h, A in ((h1, A1), (h2, A2)):
+        assert np.array_equal(h0, h) and np.array_equal(A0, A)
+    ref_A, ref_h = O.laplace_run(O.laplace_init(nx, ny, 0), 30)
+    assert np.array_equal(h2, ref_h) and np.array_equal(A2, O.interior(ref_A))
