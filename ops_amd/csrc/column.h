@@ -828,4 +828,133 @@ __device__ __forceinline__ double poisson3_chain_column_v(PoisLdsV<T, V, WY> &L,
   return PoisChainColumn<T, V, AXPY, MODE, WY, false>(L, c, R, alpha_u, a, b, x0, y0, z0, z1).run();
 }
 
+// ------------------------------------------------------------ 27-point (weights by distance)
+// out = sum over dz, dy, dx in -1..1 (that order, dx fastest) of
+// W[|dx| + |dy| + |dz|] u(x + dx, y + dy, z + dz), W = 1/8, 1/16, 1/32, 1/64.
+// Every neighbour comes from LDS: a (128 + 2) x (WY + 2) plane with its
+// corners per z, five plane buffers in rotation so one barrier per plane
+// suffices (the buffer written at step z last held plane z - 3, read at step
+// z - 2).  A lane reads each (dy, dz) row of its two points as one 16-byte
+// and two 8-byte LDS loads; the plane three steps ahead is loaded into
+// registers while the current one is computed.
+constexpr double kW27[4] = {0.125, 0.0625, 0.03125, 0.015625};
+constexpr int kS27Bufs = 5;
+
+template <int WY>
+struct S27Lds {
+  __attribute__((aligned(16))) double sp[kS27Bufs][WY + 2][kWave * 2 + 4];
+};
+
+template <int WY, bool FULL>
+struct S27Column {
+  static constexpr int V = 2;
+  S27Lds<WY> &L;
+  const Ptr<double> &out;
+  const Ptr<const double> &u;
+  int x0, z0, z1;
+  ColLane c;
+  int x, y, o, lx, ly;
+  // the plane in flight: own row, x rims, rim rows with their corners
+  double p[V], rl[V], rh[V];
+  double xl, xh, rl_xl, rl_xh, rh_xl, rh_xh;
+
+  __device__ __forceinline__ S27Column(S27Lds<WY> &L_, const Ptr<double> &out_, const Ptr<const double> &u_, int x0_,
+                                       int y0, int xe, int ye, int z0_, int z1_)
+      : L(L_), out(out_), u(u_), x0(x0_), z0(z0_), z1(z1_) {
+    lx = (int)threadIdx.x;
+    ly = (int)threadIdx.y;
+    c = col_lane<V, WY>(x0, y0, xe, ye);
+    if (FULL) c.n = V;
+    x = x0 + V * lx;
+    y = y0 + ly;
+    o = V + V * lx;
+  }
+  __device__ __forceinline__ int n() const { return FULL ? V : c.n; }
+  __device__ __forceinline__ bool act() const { return FULL || c.n > 0; }
+  __device__ __forceinline__ int buf(int q) const { return (q - z0 + kS27Bufs) % kS27Bufs; }
+
+  __device__ __forceinline__ void load(int q) {
+    const int nn = n();
+    ldv<double, V>(u.at(x, y, q), nn, p);
+    if (c.hxl) xl = u(x0 - 1, y, q);
+    if (c.hxh) xh = u(x0 + c.cnt, y, q);
+    if (c.hyl) {
+      ldv<double, V>(u.at(x, y - 1, q), nn, rl);
+      if (c.hxl) rl_xl = u(x0 - 1, y - 1, q);
+      if (c.hxh) rl_xh = u(x0 + c.cnt, y - 1, q);
+    }
+    if (c.hyh) {
+      ldv<double, V>(u.at(x, y + 1, q), nn, rh);
+      if (c.hxl) rh_xl = u(x0 - 1, y + 1, q);
+      if (c.hxh) rh_xh = u(x0 + c.cnt, y + 1, q);
+    }
+  }
+  __device__ __forceinline__ void stage(int q) {
+    const int b = buf(q), nn = n();
+    lds_st<double, V>(&L.sp[b][ly + 1][o], nn, p);
+    if (c.hxl) L.sp[b][ly + 1][V - 1] = xl;
+    if (c.hxh) L.sp[b][ly + 1][V + c.cnt] = xh;
+    if (c.hyl) {
+      lds_st<double, V>(&L.sp[b][0][o], nn, rl);
+      if (c.hxl) L.sp[b][0][V - 1] = rl_xl;
+      if (c.hxh) L.sp[b][0][V + c.cnt] = rl_xh;
+    }
+    if (c.hyh) {
+      lds_st<double, V>(&L.sp[b][ly + 2][o], nn, rh);
+      if (c.hxl) L.sp[b][ly + 2][V - 1] = rh_xl;
+      if (c.hxh) L.sp[b][ly + 2][V + c.cnt] = rh_xh;
+    }
+  }
+
+  __device__ __forceinline__ void run() {
+    p[0] = p[1] = rl[0] = rl[1] = rh[0] = rh[1] = 0.0;
+    xl = xh = rl_xl = rl_xh = rh_xl = rh_xh = 0.0;
+    if (act()) {
+      // planes z0 - 1 .. z0 + 1 into LDS, z0 + 2 into registers
+      for (int q = z0 - 1; q <= z0 + 1 && q <= z1; q++) {
+        load(q);
+        stage(q);
+      }
+      if (z0 + 2 <= z1) load(z0 + 2);
+    }
+    for (int z = z0; z < z1; z++) {
+      if (act()) {
+        if (z + 2 <= z1) {
+          stage(z + 2);
+          load(min(z + 3, z1));
+        }
+      }
+      __syncthreads();
+      if (act()) {
+        double acc[V] = {0.0, 0.0};
+#pragma unroll
+        for (int dz = -1; dz <= 1; dz++) {
+          const int b = buf(z + dz);
+#pragma unroll
+          for (int dy = -1; dy <= 1; dy++) {
+            const double *row = &L.sp[b][ly + 1 + dy][0];
+            const vec_t<double, 2> mid = *reinterpret_cast<const vec_t<double, 2> *>(row + o);
+            const double lft = row[o - 1], rgt = row[o + 2];
+            const int ad = (dy < 0 ? -dy : dy) + (dz < 0 ? -dz : dz);
+            acc[0] = acc[0] + kW27[ad + 1] * lft;
+            acc[0] = acc[0] + kW27[ad] * mid[0];
+            acc[0] = acc[0] + kW27[ad + 1] * mid[1];
+            acc[1] = acc[1] + kW27[ad + 1] * mid[0];
+            acc[1] = acc[1] + kW27[ad] * mid[1];
+            acc[1] = acc[1] + kW27[ad + 1] * rgt;
+          }
+        }
+        stv<double, V, true>(out.at(x, y, z), n(), acc);
+      }
+    }
+  }
+};
+
+template <int WY>
+__device__ __forceinline__ void stencil27_column(S27Lds<WY> &L, const Ptr<double> &out, const Ptr<const double> &u,
+                                                 int x0, int y0, int xe, int ye, int z0, int z1) {
+  if (x0 + kWave * 2 <= xe && y0 + WY <= ye) S27Column<WY, true>(L, out, u, x0, y0, xe, ye, z0, z1).run();
+  else S27Column<WY, false>(L, out, u, x0, y0, xe, ye, z0, z1).run();
+}
+
 }  // namespace opsmi

@@ -426,6 +426,37 @@ struct Stencil27 {
   }
 };
 
+// 3D form as an LDS z-column (stencil27_column): 128 x kHeatRows tiles, the
+// same sum in the same order.
+__global__ __launch_bounds__(kWave * kHeatRows) void k_stencil27_march(Ptr<double> out, Ptr<const double> u,
+                                                                       Range3 rg, int chunk) {
+  __shared__ S27Lds<kHeatRows> lds;
+  const TileId t = tile_id();
+  const int z0 = rg.z0 + (int)t.z * chunk;
+  stencil27_column<kHeatRows>(lds, out, u, rg.x0 + (int)t.x * 2 * kWave, rg.y0 + (int)t.y * kHeatRows, rg.x1, rg.y1,
+                              z0, min(z0 + chunk, rg.z1));
+}
+
+LoopRec make_stencil27_loop(const char *name, const ops_arg *args, int nargs, const Range3 &r, ops_arg out,
+                            ops_arg u) {
+  LoopRec rec;
+  rec.name = name;
+  rec.range = r;
+  rec.bytes = transfer_bytes(args, nargs, r);
+  deps_from_args(rec, args, nargs);
+  if (r.npts() <= 0) {
+    rec.bytes = 0.0;
+    rec.launch = [](hipStream_t) {};
+    return rec;
+  }
+  const ColGrid g = heat_column_grid(r, 2 * kWave, kHeatRows);
+  rec.launch = [=](hipStream_t s) {
+    hipLaunchKernelGGL(k_stencil27_march, dim3(g.gx, g.gy, g.gz), dim3(kWave, kHeatRows), 0, s,
+                       ptr_of<double>(out.dat), ptr_of<const double>(u.dat), r, g.chunk);
+  };
+  return rec;
+}
+
 // ------------------------------------------------------------ small reductions
 template <int KIND, int OP>
 struct RedX {
@@ -844,7 +875,10 @@ void ops_par_loop_stencil27_kernel(const char *name, ops_block block, int dim, i
   ARGS(out, u)
   Range3 r = check_loop(name, block, dim, range, args, {dat_w(DT_F64), dat_r(pts_27pt(), DT_F64)}, nullptr);
   need_dim(name, dim, 3);
-  LAUNCH(r, Stencil27{PD(out), PCD(u)});
+  if (use_march() && use_vec() && vec16_ok(out.dat, r.x0) && vec16_ok(u.dat, r.x0))
+    enqueue(make_stencil27_loop(name, args, nargs, r, out, u));
+  else
+    LAUNCH(r, Stencil27{PD(out), PCD(u)});
   finish_loop(args, nargs);
   OPSMI_API_END()
 }

@@ -557,3 +557,25 @@ def test_validation_errors(gpu_ops):
     ops.ops_arg_reduce(h, 1, "double", ops.OPS_MAX)
     with pytest.raises(ops.OPSError, match="different access type"):
         ops.ops_arg_reduce(h, 1, "double", ops.OPS_MIN)
+
+
+@pytest.mark.parametrize("n", [(130, 37, 21), (256, 64, 40), (61, 17, 9), (128, 9, 3)])
+def test_stencil27_column_bit_exact(gpu_ops, n):
+    """The 27-point sweep (LDS z-column where rows allow 16-byte lanes, the
+    generic engine otherwise) against the oracle on random data, halos
+    included: full and partial tiles, short z chunks."""
+    ops = gpu_ops
+    nx, ny, nz = n
+    blk, s000 = _block3(ops, n, f"s27_{nx}")
+    pts = [c for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1) for c in (dx, dy, dz)]
+    s27 = ops.ops_decl_stencil(3, 27, pts, "27pt")
+    u = ops.ops_decl_dat(blk, [nx, ny, nz], name="u27")
+    o = ops.ops_decl_dat(blk, [nx, ny, nz], name="o27")
+    ue = O.u01(41, nx, (nx + 2) * (ny + 2) * (nz + 2)).reshape(nz + 2, ny + 2, nx + 2) - 0.5
+    ops.set_ext(u, ue)
+    ops.ops_par_loop("stencil27_kernel", "stencil27", blk, 3, [0, nx, 0, ny, 0, nz],
+                     ops.ops_arg_dat(o, 1, s000, "double", ops.OPS_WRITE),
+                     ops.ops_arg_dat(u, 1, s27, "double", ops.OPS_READ))
+    assert np.array_equal(ops.ops_dat_fetch_data(o), O.stencil27(nx, ny, nz, ue))
+    ops.ops_free_dat(u)
+    ops.ops_free_dat(o)
