@@ -405,9 +405,10 @@ void ops_mi_sync(void);
 int ops_mi_loop_stats(const char *name, long *count, double *kernel_ms, double *bytes);
 void ops_mi_reset_stats(void);
 /* loop-chain fusion in the queue: 0 off, 1 the pointwise CG axpy+axpby pair,
- * 2 also the three-loop CG chain (axpy+axpby+calc_w in one column pass).
- * Every level gives the same bits.  Initial level from OPSMI_FUSION
- * (default 2). */
+ * 2 also the column chains (TeaLeaf and Poisson CG axpy+axpby+calc_w, the
+ * mixed solver's e update + p update + FP32 sweep, the Laplace tutorial's
+ * copy + next apply_stencil), each one pass.  Every level gives the same
+ * bits.  Initial level from OPSMI_FUSION (default 2). */
 void ops_mi_set_fusion(int level);
 
 #ifdef __cplusplus

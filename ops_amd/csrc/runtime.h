@@ -103,7 +103,7 @@ struct Instance {
   int error_mode = 0;
   int last_error = 0;
   std::string last_msg;
-  int fusion = 2;  // 0 off, 1 the pointwise pair only, 2 also the three-loop chain
+  int fusion = 2;  // 0 off, 1 the pointwise pair only, 2 also the column chains
   int block_x = 64, block_y = 4, block_z = 1;  // OPS_BLOCK_SIZE_* (x must be 64)
 
   std::vector<ops_block> blocks;
