@@ -83,8 +83,8 @@ int opsmi_serve_queued(opsmi_server s);
 /* Tile-shape autotuner (server idle): for each kind and size class (3D
  * heat n < 64, < 128, >= 128; 2D Poisson n < 256, < 1024, >= 1024) times
  * `iters` CG iterations of a representative solve with every candidate
- * (3D: 4, 8, 16, 32 z planes per 64 x 4 column; 2D: 2, 4, 8 rows per wave of
- * a 64-wide tile) and keeps the fastest.  Results of later solves depend on
+ * (3D: 4, 8, 16, 32 z planes per 128 x 4 column; 2D: 1, 2, 4, 8 rows per
+ * wave of a 128-wide tile) and keeps the fastest.  Results of later solves depend on
  * the chosen shape only through the order of their partial sums.
  * tile_param returns the parameter a solve of that kind and side gets. */
 int opsmi_serve_autotune(opsmi_server s, int iters);

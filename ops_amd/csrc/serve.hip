@@ -15,8 +15,9 @@
 // Solves that are done skip all work, so a graph of K iterations stops each
 // solve exactly where it converges.  Partials are folded per solve, in a
 // fixed order, by one workgroup: a solve's numbers do not depend on the rest
-// of the batch.  3D heat tiles are 64 x 4 x chunk z-marching columns
-// (heat3_column); 2D Poisson tiles are 64 x (4 chunk) rows.
+// of the batch.  3D heat tiles are 128 x 4 x chunk z-marching columns
+// (heat3_column_v, two points per lane); 2D Poisson tiles are 128 x (4 chunk)
+// rows, two points per lane.
 //
 // Host side.  Fields live in one arena of the HBM budget (first-fit, 256 B
 // granules, coalescing).  A step admits queued solves in FIFO order while a
@@ -31,7 +32,7 @@
 #include "../../include/ops_mi355x_serve.h"
 #include "../apps/problems.h"
 #include "engine.h"
-#include "heat_tile.h"
+#include "column.h"
 
 using namespace opsmi;
 
@@ -74,10 +75,15 @@ struct TileBox {
   int x0, y0, z0, z1;
 };
 
+// Points per lane along x: every field's x = 0 is 256-byte aligned and its
+// row pitch a multiple of 256 bytes, so a lane's kSV points are one 16-byte
+// (FP64) or 8-byte (FP32) load.  A tile is (64 kSV) points wide.
+constexpr int kSV = 2;
+
 __device__ __forceinline__ TileBox tile_box(const SolveDev &d, int li) {
   TileBox b;
   const int tx = li % d.tx, rest = li / d.tx;
-  b.x0 = tx * kWave;
+  b.x0 = tx * kWave * kSV;
   if (d.kind == OPSMI_SOLVE_HEAT3D) {
     b.y0 = (rest % d.ty) * kWavesPerBlock;
     b.z0 = (rest / d.ty) * d.chunk;
@@ -90,19 +96,21 @@ __device__ __forceinline__ TileBox tile_box(const SolveDev &d, int li) {
   return b;
 }
 
-// f(x, y, z) over this thread's points of the tile
+// f(x, y, z, n) over this thread's groups of the tile: kSV consecutive points
+// from x, n (1..kSV) of them inside the block
 template <class F>
-__device__ __forceinline__ void tile_points(const SolveDev &d, const TileBox &b, F f) {
-  const int x = b.x0 + (int)threadIdx.x;
-  if (x >= d.nx) return;
+__device__ __forceinline__ void tile_groups(const SolveDev &d, const TileBox &b, F f) {
+  const int x = b.x0 + kSV * (int)threadIdx.x;
+  const int n = min(kSV, d.nx - x);
+  if (n <= 0) return;
   if (d.kind == OPSMI_SOLVE_HEAT3D) {
     const int y = b.y0 + (int)threadIdx.y;
     if (y >= d.ny) return;
-    for (int z = b.z0; z < b.z1; z++) f(x, y, z);
+    for (int z = b.z0; z < b.z1; z++) f(x, y, z, n);
   } else {
     for (int i = 0; i < d.chunk; i++) {
       const int y = b.y0 + (int)threadIdx.y + kWavesPerBlock * i;
-      if (y < d.ny) f(x, y, 0);
+      if (y < d.ny) f(x, y, 0, n);
     }
   }
 }
@@ -126,11 +134,37 @@ __device__ __forceinline__ void store_partials(double (&acc)[D], double *dst, do
 }
 
 __device__ __forceinline__ Ptr<float> PF32(float *a, const SolveDev &d) { return Ptr<float>{a, d.sy32, 0}; }
+__device__ __forceinline__ Ptr<const float> PF32C(const float *a, const SolveDev &d) {
+  return Ptr<const float>{a, d.sy32, 0};
+}
+
+// the 2D 5-point operator over a group: the group's row and the rows above
+// and below, and the neighbours just left and right of the group
+template <class T>
+struct Row5 {
+  T c[kSV], up[kSV], dn[kSV];
+  T l, r;
+};
+template <class T>
+__device__ __forceinline__ void load5(const Ptr<const T> &p, int x, int y, int n, Row5<T> &R) {
+  ldv<T, kSV>(p.at(x, y, 0), n, R.c);
+  ldv<T, kSV>(p.at(x, y - 1, 0), n, R.up);
+  ldv<T, kSV>(p.at(x, y + 1, 0), n, R.dn);
+  R.l = p(x - 1, y, 0);
+  R.r = p(x + n, y, 0);
+}
+// 4 p - (p(x-1) + p(x+1)) - (p(y-1) + p(y+1)) at point j of the group
+template <class T>
+__device__ __forceinline__ T op5(const Row5<T> &R, int j, int n) {
+  const T xm = j == 0 ? R.l : R.c[j > 0 ? j - 1 : 0];
+  const T xp = j + 1 < n ? R.c[j + 1 < kSV ? j + 1 : j] : R.r;
+  return (T)4.0 * R.c[j] - (xm + xp) - (R.up[j] + R.dn[j]);
+}
 
 // ------------------------------------------------------------ iteration
 __global__ __launch_bounds__(256) void sk_calc_w(const SolveDev *S, const SolveStat *st, const int2 *tiles,
                                                  const int *ntiles, double *part) {
-  __shared__ HeatLds lds;
+  __shared__ HeatLdsV<kSV, kWavesPerBlock> lds;
   __shared__ double sh[16][1];
   const int nt = *ntiles;
   for (int t = (int)xcd_tile(blockIdx.x, gridDim.x); t < nt; t += (int)gridDim.x) {
@@ -144,31 +178,54 @@ __global__ __launch_bounds__(256) void sk_calc_w(const SolveDev *S, const SolveS
         // the correction x = x + (double) e (pointwise; the residual follows
         // in sk_update once every tile has applied it)
         const Ptr<double> x = P(d.u, d);
-        const Ptr<float> ev = PF32(d.e32, d);
-        tile_points(d, b, [&](int i, int j, int k) { x(i, j, k) = x(i, j, k) + (double)ev(i, j, k); });
+        const Ptr<const float> ev = PF32C(d.e32, d);
+        tile_groups(d, b, [&](int i, int j, int k, int n) {
+          double xv[kSV];
+          float evv[kSV];
+          ldv<double, kSV>(x.at(i, j, k), n, xv);
+          ldv<float, kSV>(ev.at(i, j, k), n, evv);
+#pragma unroll
+          for (int q = 0; q < kSV; q++) xv[q] = xv[q] + (double)evv[q];
+          stv<double, kSV, false>(x.at(i, j, k), n, xv);
+        });
         continue;
       }
       // inner sweep in FP32: w = 4p - (p(-1,0) + p(1,0)) - (p(0,-1) + p(0,1)), p.w in FP64
-      const Ptr<float> p = PF32(d.p32, d), w = PF32(d.w32, d);
-      tile_points(d, b, [&](int i, int j, int k) {
-        const float v = 4.0f * p(i, j, k) - (p(i - 1, j, k) + p(i + 1, j, k)) - (p(i, j - 1, k) + p(i, j + 1, k));
-        w(i, j, k) = v;
-        acc = acc + (double)v * (double)p(i, j, k);
+      const Ptr<const float> p = PF32C(d.p32, d);
+      const Ptr<float> w = PF32(d.w32, d);
+      tile_groups(d, b, [&](int i, int j, int k, int n) {
+        Row5<float> R;
+        load5<float>(p, i, j, n, R);
+        float v[kSV];
+#pragma unroll
+        for (int q = 0; q < kSV; q++) {
+          v[q] = op5<float>(R, q, n);
+          if (q < n) acc = acc + (double)v[q] * (double)R.c[q];
+        }
+        stv<float, kSV, false>(w.at(i, j, k), n, v);
       });
       store_partial(acc, part + (size_t)t * kPart, sh);
       continue;
     }
     if (d.kind == OPSMI_SOLVE_HEAT3D) {
       __syncthreads();  // LDS of the previous tile fully read
-      acc = heat3_column<false>(lds, P(d.w, d), Ptr<double>{nullptr, 0, 0}, PC(d.kx, d), PC(d.ky, d), PC(d.kz, d),
-                                PC(d.p, d), d.rx, d.ry, d.rz, b.x0, b.y0, d.nx, d.ny, b.z0, b.z1);
+      acc = heat3_column_v<kSV, false, true, kWavesPerBlock, 1>(lds, P(d.w, d), Ptr<double>{nullptr, 0, 0},
+                                                                 PC(d.kx, d), PC(d.ky, d), PC(d.kz, d), PC(d.p, d),
+                                                                 d.rx, d.ry, d.rz, b.x0, b.y0, d.nx, d.ny, b.z0,
+                                                                 b.z1);
     } else {
       const Ptr<const double> p = PC(d.p, d);
       const Ptr<double> w = P(d.w, d);
-      tile_points(d, b, [&](int x, int y, int z) {
-        const double v = 4.0 * p(x, y, z) - (p(x - 1, y, z) + p(x + 1, y, z)) - (p(x, y - 1, z) + p(x, y + 1, z));
-        w(x, y, z) = v;
-        acc = acc + v * p(x, y, z);
+      tile_groups(d, b, [&](int x, int y, int z, int n) {
+        Row5<double> R;
+        load5<double>(p, x, y, n, R);
+        double v[kSV];
+#pragma unroll
+        for (int q = 0; q < kSV; q++) {
+          v[q] = op5<double>(R, q, n);
+          if (q < n) acc = acc + v[q] * R.c[q];
+        }
+        stv<double, kSV, false>(w.at(x, y, z), n, v);
       });
     }
     store_partial(acc, part + (size_t)t * kPart, sh);
@@ -191,29 +248,53 @@ __global__ __launch_bounds__(256) void sk_update(const SolveDev *S, const SolveS
       if (st[e.x].phase == PH_INNER) {
         // e += a p ; rr -= a w ; (rr . Mi rr, rr . rr) in FP64
         const float a = (float)st[e.x].alpha;
-        tile_points(d, tile_box(d, e.y), [&](int i, int j, int k) {
-          ev(i, j, k) = ev(i, j, k) + a * p(i, j, k);
-          const float q = rr(i, j, k) - a * w(i, j, k);
-          rr(i, j, k) = q;
-          const float z = mi * q;
-          acc[0] = acc[0] + (double)q * (double)z;
-          acc[1] = acc[1] + (double)q * (double)q;
+        tile_groups(d, tile_box(d, e.y), [&](int i, int j, int k, int n) {
+          float evv[kSV], rv[kSV], pv[kSV], wv[kSV];
+          ldv<float, kSV>(ev.at(i, j, k), n, evv);
+          ldv<float, kSV>(rr.at(i, j, k), n, rv);
+          ldv<float, kSV>(p.at(i, j, k), n, pv);
+          ldv<float, kSV>(w.at(i, j, k), n, wv);
+#pragma unroll
+          for (int q = 0; q < kSV; q++) {
+            evv[q] = evv[q] + a * pv[q];
+            rv[q] = rv[q] - a * wv[q];
+            const float z = mi * rv[q];
+            if (q < n) {
+              acc[0] = acc[0] + (double)rv[q] * (double)z;
+              acc[1] = acc[1] + (double)rv[q] * (double)rv[q];
+            }
+          }
+          stv<float, kSV, false>(ev.at(i, j, k), n, evv);
+          stv<float, kSV, false>(rr.at(i, j, k), n, rv);
         });
       } else {
         // FP64 residual r = f - A x, then the inner start from it:
         // rr = (float) r, e = 0, p = Mi rr; (r.r, rr . Mi rr, rr . rr)
         const Ptr<const double> x = PC(d.u, d), f = PC(d.f, d);
-        tile_points(d, tile_box(d, e.y), [&](int i, int j, int k) {
-          const double ax = 4.0 * x(i, j, k) - (x(i - 1, j, k) + x(i + 1, j, k)) - (x(i, j - 1, k) + x(i, j + 1, k));
-          const double v = f(i, j, k) - ax;
-          acc[0] = acc[0] + v * v;
-          const float q = (float)v;
-          const float z = mi * q;
-          ev(i, j, k) = 0.0f;
-          rr(i, j, k) = q;
-          p(i, j, k) = z;
-          acc[1] = acc[1] + (double)q * (double)z;
-          acc[2] = acc[2] + (double)q * (double)q;
+        tile_groups(d, tile_box(d, e.y), [&](int i, int j, int k, int n) {
+          Row5<double> R;
+          load5<double>(x, i, j, n, R);
+          double fv[kSV];
+          ldv<double, kSV>(f.at(i, j, k), n, fv);
+          float q32[kSV], z32[kSV], zero[kSV];
+#pragma unroll
+          for (int q = 0; q < kSV; q++) {
+            const double ax = op5<double>(R, q, n);
+            const double v = fv[q] - ax;
+            const float qv = (float)v;
+            const float z = mi * qv;
+            q32[q] = qv;
+            z32[q] = z;
+            zero[q] = 0.0f;
+            if (q < n) {
+              acc[0] = acc[0] + v * v;
+              acc[1] = acc[1] + (double)qv * (double)z;
+              acc[2] = acc[2] + (double)qv * (double)qv;
+            }
+          }
+          stv<float, kSV, false>(ev.at(i, j, k), n, zero);
+          stv<float, kSV, false>(rr.at(i, j, k), n, q32);
+          stv<float, kSV, false>(p.at(i, j, k), n, z32);
         });
       }
       store_partials<3>(acc, part + (size_t)t * kPart, sh3);
@@ -223,11 +304,20 @@ __global__ __launch_bounds__(256) void sk_update(const SolveDev *S, const SolveS
     const Ptr<double> u = P(d.u, d), r = P(d.r, d);
     const Ptr<const double> p = PC(d.p, d), w = PC(d.w, d);
     double acc = 0.0;
-    tile_points(d, tile_box(d, e.y), [&](int x, int y, int z) {
-      u(x, y, z) = u(x, y, z) + alpha * p(x, y, z);
-      const double v = r(x, y, z) - alpha * w(x, y, z);
-      r(x, y, z) = v;
-      acc = acc + v * v;
+    tile_groups(d, tile_box(d, e.y), [&](int x, int y, int z, int n) {
+      double uv[kSV], rv[kSV], pv[kSV], wv[kSV];
+      ldv<double, kSV>(u.at(x, y, z), n, uv);
+      ldv<double, kSV>(r.at(x, y, z), n, rv);
+      ldv<double, kSV>(p.at(x, y, z), n, pv);
+      ldv<double, kSV>(w.at(x, y, z), n, wv);
+#pragma unroll
+      for (int q = 0; q < kSV; q++) {
+        uv[q] = uv[q] + alpha * pv[q];
+        rv[q] = rv[q] - alpha * wv[q];
+        if (q < n) acc = acc + rv[q] * rv[q];
+      }
+      stv<double, kSV, false>(u.at(x, y, z), n, uv);
+      stv<double, kSV, false>(r.at(x, y, z), n, rv);
     });
     store_partial(acc, part + (size_t)t * kPart, sh);
   }
@@ -245,13 +335,27 @@ __global__ __launch_bounds__(256) void sk_axpby(const SolveDev *S, const SolveSt
       if (!st[e.x].axpby) continue;
       const float mi = 0.25f, bf = (float)st[e.x].beta;
       const Ptr<float> p = PF32(d.p32, d), rr = PF32(d.r32, d);
-      tile_points(d, tile_box(d, e.y), [&](int i, int j, int k) { p(i, j, k) = mi * rr(i, j, k) + bf * p(i, j, k); });
+      tile_groups(d, tile_box(d, e.y), [&](int i, int j, int k, int n) {
+        float pv[kSV], rv[kSV];
+        ldv<float, kSV>(p.at(i, j, k), n, pv);
+        ldv<float, kSV>(rr.at(i, j, k), n, rv);
+#pragma unroll
+        for (int q = 0; q < kSV; q++) pv[q] = mi * rv[q] + bf * pv[q];
+        stv<float, kSV, false>(p.at(i, j, k), n, pv);
+      });
       continue;
     }
     const double beta = st[e.x].beta;
     const Ptr<double> p = P(d.p, d);
     const Ptr<const double> r = PC(d.r, d);
-    tile_points(d, tile_box(d, e.y), [&](int x, int y, int z) { p(x, y, z) = beta * p(x, y, z) + 1.0 * r(x, y, z); });
+    tile_groups(d, tile_box(d, e.y), [&](int x, int y, int z, int n) {
+      double pv[kSV], rv[kSV];
+      ldv<double, kSV>(p.at(x, y, z), n, pv);
+      ldv<double, kSV>(r.at(x, y, z), n, rv);
+#pragma unroll
+      for (int q = 0; q < kSV; q++) pv[q] = beta * pv[q] + 1.0 * rv[q];
+      stv<double, kSV, false>(p.at(x, y, z), n, pv);
+    });
   }
 }
 
@@ -374,21 +478,26 @@ __global__ __launch_bounds__(256) void sk_heat_coefs(SolveDev d) {
 
 // first residual, p = r, partial r.p per tile (grid = the solve's tiles)
 __global__ __launch_bounds__(256) void sk_first_residual(SolveDev d, uint64_t seed, double *part) {
-  __shared__ HeatLds lds;
+  __shared__ HeatLdsV<kSV, kWavesPerBlock> lds;
   __shared__ double sh[16][1];
   const int li = (int)blockIdx.x;
   const TileBox b = tile_box(d, li);
   if (d.kind == OPSMI_SOLVE_HEAT3D)
-    heat3_column<true>(lds, P(d.w, d), P(d.r, d), PC(d.kx, d), PC(d.ky, d), PC(d.kz, d), PC(d.u, d), d.rx, d.ry,
-                       d.rz, b.x0, b.y0, d.nx, d.ny, b.z0, b.z1);
+    heat3_column_v<kSV, true, false, kWavesPerBlock, 1>(lds, P(d.w, d), P(d.r, d), PC(d.kx, d), PC(d.ky, d),
+                                                         PC(d.kz, d), PC(d.u, d), d.rx, d.ry, d.rz, b.x0, b.y0, d.nx,
+                                                         d.ny, b.z0, b.z1);
   const Ptr<double> r = P(d.r, d), p = P(d.p, d);
   const bool poisson = d.kind == OPSMI_SOLVE_POISSON2D;
   double acc = 0.0;
-  tile_points(d, b, [&](int x, int y, int z) {
-    if (poisson) r(x, y, z) = opsapp::poisson_rhs(seed, x, y, 0, d.nx, d.ny);
-    const double v = r(x, y, z);
-    p(x, y, z) = v;
-    acc = acc + v * v;
+  tile_groups(d, b, [&](int x, int y, int z, int n) {
+#pragma unroll
+    for (int q = 0; q < kSV; q++) {
+      if (q >= n) continue;
+      if (poisson) r(x + q, y, z) = opsapp::poisson_rhs(seed, x + q, y, 0, d.nx, d.ny);
+      const double v = r(x + q, y, z);
+      p(x + q, y, z) = v;
+      acc = acc + v * v;
+    }
   });
   store_partial(acc, part + li, sh);
 }
@@ -566,7 +675,7 @@ int tile_param(opsmi_server S, int kind, int n) {
 
 long tiles_of(const opsmi_solve_req &q, int param, int *tx, int *ty) {
   const Geometry g = geometry(q);
-  *tx = (g.nx + kWave - 1) / kWave;
+  *tx = (g.nx + kWave * kSV - 1) / (kWave * kSV);
   if (q.kind == OPSMI_SOLVE_HEAT3D) {
     *ty = (g.ny + kWavesPerBlock - 1) / kWavesPerBlock;
     return (long)*tx * *ty * ((g.nz + param - 1) / param);
@@ -889,7 +998,7 @@ int opsmi_serve_autotune(opsmi_server S, int iters) {
   if (S->n_active || !S->queue.empty()) fail(OPS_INVALID_ARGUMENT, "opsmi_serve_autotune: the server must be idle");
   if (iters < 1 || iters > 1000) fail(OPS_INVALID_ARGUMENT, "opsmi_serve_autotune: iters in [1, 1000]");
   static const int rep[2][3] = {{160, 512, 1536}, {48, 96, 192}};
-  static const int cand2[] = {2, 4, 8}, cand3[] = {4, 8, 16, 32};
+  static const int cand2[] = {1, 2, 4, 8}, cand3[] = {4, 8, 16, 32};
   hipEvent_t e0, e1;
   OPSMI_HIP(hipEventCreate(&e0));
   OPSMI_HIP(hipEventCreate(&e1));
@@ -899,7 +1008,7 @@ int opsmi_serve_autotune(opsmi_server S, int iters) {
       if (geometry(q).bytes > S->budget) continue;
       int *slot_param = kind == OPSMI_SOLVE_HEAT3D ? &S->chunk3d[cls] : &S->rows2d[cls];
       const int *cand = kind == OPSMI_SOLVE_HEAT3D ? cand3 : cand2;
-      const int nc = kind == OPSMI_SOLVE_HEAT3D ? 4 : 3;
+      const int nc = 4;
       int best = *slot_param;
       float best_ms = 1e30f;
       for (int c = 0; c < nc; c++) {

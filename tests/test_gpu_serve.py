@@ -151,7 +151,7 @@ def test_autotune_picks_candidates_and_solves_stay_exact(serve):
     for n in (40, 100, 200):
         assert srv.tile_param(S.HEAT3D, n) in (4, 8, 16, 32)
     for n in (100, 600, 1500):
-        assert srv.tile_param(S.POISSON2D, n) in (2, 4, 8)
+        assert srv.tile_param(S.POISSON2D, n) in (1, 2, 4, 8)
     assert srv.bytes_in_use() == 0 and srv.active() == 0
     ids = [srv.submit(k, n, seed=s) for k, n, s in MIX[:5]]
     out = srv.run()
