@@ -275,16 +275,28 @@ inline bool use_vec() {
   return !(e && e[0] == '0');
 }
 
+// NTL: loads carry the non-temporal hint.  The pointwise engine uses it for
+// loops whose arrays are larger than the Infinity Cache: a pointwise 4-read
+// 1-write sweep over 1 GiB arrays measured 6.24 TB/s with it against
+// 5.77 TB/s without (6 reads 3 writes: 5.96 against 5.25).  The stencil
+// columns load without it (their rims are re-read by neighbouring
+// workgroups from L2), as do loops over arrays that stay in the cache.
+template <class T, bool NTL>
+__device__ __forceinline__ T ldg(const T *p) {
+  if constexpr (NTL) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
 // n <= V consecutive values from p (one vector load when all V are valid)
-template <class T, int V>
+template <class T, int V, bool NTL = false>
 __device__ __forceinline__ void ldv(const T *p, int n, T (&o)[V]) {
   if (n >= V) {
-    const vec_t<T, V> q = *reinterpret_cast<const vec_t<T, V> *>(p);
+    const vec_t<T, V> q = ldg<vec_t<T, V>, NTL>(reinterpret_cast<const vec_t<T, V> *>(p));
 #pragma unroll
     for (int j = 0; j < V; j++) o[j] = q[j];
   } else {
 #pragma unroll
-    for (int j = 0; j < V; j++) o[j] = j < n ? p[j] : T(0);
+    for (int j = 0; j < V; j++) o[j] = j < n ? ldg<T, NTL>(p + j) : T(0);
   }
 }
 
@@ -310,8 +322,9 @@ __device__ __forceinline__ void stv(T *p, int n, const T (&v)[V]) {
 // F provides: using red_t; static constexpr int red_op, red_dim;
 //   __device__ void operator()(int x, int y, int z, red_t (&acc)[red_dim or 1]) const
 // and optionally, for the vector engine (k_loop_v), kVec > 1 and
-//   template <int V> __device__ void vop(int x, int y, int z, int n, red_t (&acc)[...]) const
-// which handles the n <= V consecutive points x .. x + n - 1.
+//   template <int V, bool NTL> __device__ void vop(int x, int y, int z, int n, red_t (&acc)[...]) const
+// which handles the n <= V consecutive points x .. x + n - 1 (NTL: its loads
+// take the non-temporal hint).
 template <class F>
 __global__ __launch_bounds__(256) void k_loop(Range3 r, F f, RedCtl ctl, int chunk, int three_d) {
   using R = typename F::red_t;
@@ -342,7 +355,7 @@ __global__ __launch_bounds__(256) void k_loop(Range3 r, F f, RedCtl ctl, int chu
 // Vector form of k_loop: each lane takes V consecutive x points (one 16-byte
 // load per array when the functor's element is 16 / V bytes), so a wave
 // covers 64 V points of a row.
-template <class F, int V>
+template <class F, int V, bool NTL>
 __global__ __launch_bounds__(256) void k_loop_v(Range3 r, F f, RedCtl ctl, int chunk, int three_d) {
   using R = typename F::red_t;
   constexpr int D = F::red_dim > 0 ? F::red_dim : 1;
@@ -359,16 +372,20 @@ __global__ __launch_bounds__(256) void k_loop_v(Range3 r, F f, RedCtl ctl, int c
       if (y < r.y1) {
         const int z0 = r.z0 + (int)t.z * chunk;
         const int z1 = min(z0 + chunk, r.z1);
-        for (int z = z0; z < z1; z++) f.template vop<V>(x, y, z, n, acc);
+        for (int z = z0; z < z1; z++) f.template vop<V, NTL>(x, y, z, n, acc);
       }
     } else {
       const int y0 = r.y0 + (int)t.y * kWavesPerBlock * chunk;
       const int y1 = min(y0 + kWavesPerBlock * chunk, r.y1);
-      for (int y = y0 + (int)threadIdx.y; y < y1; y += kWavesPerBlock) f.template vop<V>(x, y, r.z0, n, acc);
+      for (int y = y0 + (int)threadIdx.y; y < y1; y += kWavesPerBlock) f.template vop<V, NTL>(x, y, r.z0, n, acc);
     }
   }
   if constexpr (F::red_dim > 0) reduce_block_store<R, OP, D>(acc, ctl);
 }
+
+// bytes (OPS transfer rule) from which a pointwise loop loads non-temporally:
+// the 256 MB Infinity Cache
+constexpr double kNtLoadBytes = 256.0 * (1 << 20);
 
 template <class F, class = void>
 struct vec_width {
@@ -420,6 +437,8 @@ LoopRec make_loop(const char *name, const ops_arg *args, int nargs, const Range3
   for (int i = 0; i < nargs && vec; i++)
     if (args[i].argtype == OPS_ARG_DAT) vec = vec16_ok(args[i].dat, r.x0);
   if (vec) t.grid.x = (unsigned)((r.x1 - r.x0 + kWave * V - 1) / (kWave * V));
+  // non-temporal loads once the loop's arrays outgrow the Infinity Cache
+  const bool ntl = rec.bytes >= kNtLoadBytes;
   RedCtl ctl{};
   if constexpr (F::red_dim > 0) ctl = red_prepare(*red_arg, F::red_dim, dtype_v<typename F::red_t>(), t.blocks());
   rec.ctl = ctl;
@@ -427,9 +446,10 @@ LoopRec make_loop(const char *name, const ops_arg *args, int nargs, const Range3
     const F f = make();
     const RedCtl c = with_scratch(ctl);
     if (vec) {
-      if constexpr (V > 1)
-        hipLaunchKernelGGL((k_loop_v<F, V>), t.grid, dim3(kWave, kWavesPerBlock), 0, s, r, f, c, t.chunk,
-                           t.three_d ? 1 : 0);
+      if constexpr (V > 1) {
+        auto kern = ntl ? k_loop_v<F, V, true> : k_loop_v<F, V, false>;
+        hipLaunchKernelGGL(kern, t.grid, dim3(kWave, kWavesPerBlock), 0, s, r, f, c, t.chunk, t.three_d ? 1 : 0);
+      }
     } else {
       hipLaunchKernelGGL(k_loop<F>, t.grid, dim3(kWave, kWavesPerBlock), 0, s, r, f, c, t.chunk, t.three_d ? 1 : 0);
     }

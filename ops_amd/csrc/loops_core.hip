@@ -27,10 +27,10 @@ struct Copy {
   Ptr<T> a;
   Ptr<const T> b;
   __device__ void operator()(int x, int y, int z, double (&)[1]) const { a(x, y, z) = b(x, y, z); }
-  template <int V>
+  template <int V, bool NTL = false>
   __device__ void vop(int x, int y, int z, int n, double (&)[1]) const {
     T v[V];
-    ldv<T, V>(b.at(x, y, z), n, v);
+    ldv<T, V, NTL>(b.at(x, y, z), n, v);
     stv<T, V, true>(a.at(x, y, z), n, v);
   }
 };
@@ -107,11 +107,11 @@ struct Axpy {
   Ptr<const double> p;
   double alpha;
   __device__ void operator()(int x, int y, int z, double (&)[1]) const { u(x, y, z) = u(x, y, z) + alpha * p(x, y, z); }
-  template <int V>
+  template <int V, bool NTL = false>
   __device__ void vop(int x, int y, int z, int n, double (&)[1]) const {
     double a[V], b[V];
-    ldv<double, V>(u.at(x, y, z), n, a);
-    ldv<double, V>(p.at(x, y, z), n, b);
+    ldv<double, V, NTL>(u.at(x, y, z), n, a);
+    ldv<double, V, NTL>(p.at(x, y, z), n, b);
 #pragma unroll
     for (int j = 0; j < V; j++) a[j] = a[j] + alpha * b[j];
     stv<double, V, true>(u.at(x, y, z), n, a);
@@ -126,11 +126,11 @@ struct Axpby {
   __device__ void operator()(int x, int y, int z, double (&)[1]) const {
     u(x, y, z) = alpha * u(x, y, z) + beta * p(x, y, z);
   }
-  template <int V>
+  template <int V, bool NTL = false>
   __device__ void vop(int x, int y, int z, int n, double (&)[1]) const {
     double a[V], b[V];
-    ldv<double, V>(u.at(x, y, z), n, a);
-    ldv<double, V>(p.at(x, y, z), n, b);
+    ldv<double, V, NTL>(u.at(x, y, z), n, a);
+    ldv<double, V, NTL>(p.at(x, y, z), n, b);
 #pragma unroll
     for (int j = 0; j < V; j++) a[j] = alpha * a[j] + beta * b[j];
     stv<double, V, true>(u.at(x, y, z), n, a);
@@ -298,11 +298,11 @@ struct CalcUrR {
     r(x, y, z) = v;
     s[0] = s[0] + v * v;
   }
-  template <int V>
+  template <int V, bool NTL = false>
   __device__ void vop(int x, int y, int z, int n, double (&s)[1]) const {
     double a[V], b[V];
-    ldv<double, V>(r.at(x, y, z), n, a);
-    ldv<double, V>(w.at(x, y, z), n, b);
+    ldv<double, V, NTL>(r.at(x, y, z), n, a);
+    ldv<double, V, NTL>(w.at(x, y, z), n, b);
 #pragma unroll
     for (int j = 0; j < V; j++) {
       a[j] = a[j] - alpha * b[j];

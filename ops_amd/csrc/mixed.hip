@@ -61,11 +61,11 @@ struct MpInit {
     s[0] = s[0] + (double)q * (double)z;
     s[1] = s[1] + (double)q * (double)q;
   }
-  template <int V>
+  template <int V, bool NTL = false>
   __device__ void vop(int i, int j, int k, int n, double (&s)[2]) const {
     double rv[V];
     float q[V], z[V], zero[V];
-    ldv<double, V>(r.at(i, j, k), n, rv);
+    ldv<double, V, NTL>(r.at(i, j, k), n, rv);
 #pragma unroll
     for (int c = 0; c < V; c++) {
       q[c] = (float)rv[c];
@@ -98,13 +98,13 @@ struct MpUpdate {
     s[0] = s[0] + (double)q * (double)z;
     s[1] = s[1] + (double)q * (double)q;
   }
-  template <int V>
+  template <int V, bool NTL = false>
   __device__ void vop(int i, int j, int k, int n, double (&s)[2]) const {
     float ev[V], rv[V], pv[V], wv[V];
-    ldv<float, V>(e.at(i, j, k), n, ev);
-    ldv<float, V>(rr.at(i, j, k), n, rv);
-    ldv<float, V>(p.at(i, j, k), n, pv);
-    ldv<float, V>(w.at(i, j, k), n, wv);
+    ldv<float, V, NTL>(e.at(i, j, k), n, ev);
+    ldv<float, V, NTL>(rr.at(i, j, k), n, rv);
+    ldv<float, V, NTL>(p.at(i, j, k), n, pv);
+    ldv<float, V, NTL>(w.at(i, j, k), n, wv);
 #pragma unroll
     for (int c = 0; c < V; c++) {
       ev[c] = ev[c] + a * pv[c];
@@ -136,11 +136,11 @@ struct MpRrUpdate {
     s[0] = s[0] + (double)q * (double)z;
     s[1] = s[1] + (double)q * (double)q;
   }
-  template <int V>
+  template <int V, bool NTL = false>
   __device__ void vop(int i, int j, int k, int n, double (&s)[2]) const {
     float rv[V], wv[V];
-    ldv<float, V>(rr.at(i, j, k), n, rv);
-    ldv<float, V>(w.at(i, j, k), n, wv);
+    ldv<float, V, NTL>(rr.at(i, j, k), n, rv);
+    ldv<float, V, NTL>(w.at(i, j, k), n, wv);
 #pragma unroll
     for (int c = 0; c < V; c++) {
       rv[c] = rv[c] - a * wv[c];
@@ -162,11 +162,11 @@ struct MpEUpdate {
   Ptr<const float> p;
   float a;
   __device__ void operator()(int i, int j, int k, double (&)[1]) const { e(i, j, k) = e(i, j, k) + a * p(i, j, k); }
-  template <int V>
+  template <int V, bool NTL = false>
   __device__ void vop(int i, int j, int k, int n, double (&)[1]) const {
     float ev[V], pv[V];
-    ldv<float, V>(e.at(i, j, k), n, ev);
-    ldv<float, V>(p.at(i, j, k), n, pv);
+    ldv<float, V, NTL>(e.at(i, j, k), n, ev);
+    ldv<float, V, NTL>(p.at(i, j, k), n, pv);
 #pragma unroll
     for (int c = 0; c < V; c++) ev[c] = ev[c] + a * pv[c];
     stv<float, V, true>(e.at(i, j, k), n, ev);
@@ -182,11 +182,11 @@ struct MpP {
   __device__ void operator()(int i, int j, int k, double (&)[1]) const {
     p(i, j, k) = mi * rr(i, j, k) + b * p(i, j, k);
   }
-  template <int V>
+  template <int V, bool NTL = false>
   __device__ void vop(int i, int j, int k, int n, double (&)[1]) const {
     float pv[V], rv[V];
-    ldv<float, V>(p.at(i, j, k), n, pv);
-    ldv<float, V>(rr.at(i, j, k), n, rv);
+    ldv<float, V, NTL>(p.at(i, j, k), n, pv);
+    ldv<float, V, NTL>(rr.at(i, j, k), n, rv);
 #pragma unroll
     for (int c = 0; c < V; c++) pv[c] = mi * rv[c] + b * pv[c];
     stv<float, V, true>(p.at(i, j, k), n, pv);
@@ -201,12 +201,12 @@ struct MpCorrect {
   __device__ void operator()(int i, int j, int k, double (&)[1]) const {
     x(i, j, k) = x(i, j, k) + (double)e(i, j, k);
   }
-  template <int V>
+  template <int V, bool NTL = false>
   __device__ void vop(int i, int j, int k, int n, double (&)[1]) const {
     double xv[V];
     float ev[V];
-    ldv<double, V>(x.at(i, j, k), n, xv);
-    ldv<float, V>(e.at(i, j, k), n, ev);
+    ldv<double, V, NTL>(x.at(i, j, k), n, xv);
+    ldv<float, V, NTL>(e.at(i, j, k), n, ev);
 #pragma unroll
     for (int c = 0; c < V; c++) xv[c] = xv[c] + (double)ev[c];
     stv<double, V, true>(x.at(i, j, k), n, xv);

@@ -44,6 +44,7 @@ __device__ __forceinline__ long line_offset(const LineGeo &g, long id) {
 // (the arithmetic and its order are unchanged).  Without _Inc, d' is written
 // in place into d (the backward pass overwrites it with x), so only c' goes
 // through the scratch; _Inc leaves d untouched and keeps d' in scratch too.
+// The forward pass reads a, b, c, d once, with the non-temporal hint.
 constexpr int kPre = 8;
 
 template <class T, bool INC>
@@ -68,10 +69,10 @@ __global__ __launch_bounds__(256) void k_thomas(const T *__restrict__ a, const T
     for (int k = 0; k < kPre; k++) {
       const long i = i0 + k;
       if (i < n) {
-        av[k] = ap[i * s];
-        bv[k] = bp[i * s];
-        cv[k] = cp[i * s];
-        dv[k] = dp[i * s];
+        av[k] = ldg<T, true>(ap + i * s);
+        bv[k] = ldg<T, true>(bp + i * s);
+        cv[k] = ldg<T, true>(cp + i * s);
+        dv[k] = ldg<T, true>(dp + i * s);
       }
     }
 #pragma unroll
@@ -159,8 +160,10 @@ __global__ __launch_bounds__(64) void k_thomas_x(const T *__restrict__ a, const 
       const long i = i0 + sub * V;
       if (rowok[k] && i + V <= n) {
         using VT = vec_t<T, V>;
-        const VT va = *(const VT *)(a + rowoff[k] + i), vb = *(const VT *)(b + rowoff[k] + i);
-        const VT vc = *(const VT *)(c + rowoff[k] + i), vd = *(const VT *)(d + rowoff[k] + i);
+        const VT va = ldg<VT, true>((const VT *)(a + rowoff[k] + i));
+        const VT vb = ldg<VT, true>((const VT *)(b + rowoff[k] + i));
+        const VT vc = ldg<VT, true>((const VT *)(c + rowoff[k] + i));
+        const VT vd = ldg<VT, true>((const VT *)(d + rowoff[k] + i));
 #pragma unroll
         for (int j = 0; j < V; j++) {
           ra[k][j] = va[j];
