@@ -1,0 +1,203 @@
+// Device side of translator-generated loops (MI355X OPS backend).
+//
+// A generated kernel runs a 64 x 4 workgroup per tile, one grid point per
+// thread with x along the lanes (each wave reads contiguous rows); tiles are
+// renumbered so each XCD works through a contiguous share of them (its L2
+// then holds the rows its stencils share).  3D loops: a tile is 64 x 4
+// points over `chunk` z planes; 2D/1D loops: 64 points over `chunk` groups of
+// 4 rows.  Reductions keep one accumulator per thread and component,
+// initialised as the reference's generated code does (0, +-DBL_MAX,
+// +-FLT_MAX, +-INT_MAX: ops_translator/resources/templates/cpp/cuda/
+// loop_host.cpp.j2:116-127, ops/c/include/ops_internal2.h:68-80), fold
+// them across the workgroup with wave64 shuffles, and store one partial per
+// workgroup; the library folds the partials in a fixed order.
+//
+// ACC<T> (ops/c/include/ops_lib_core.h:1356-2038, AoS layout): A(dx[, dy[,
+// dz]]) is the point at that offset from the current one, A(d, dx[, ...])
+// component d of it; combine_max/min/inc update a point atomically.
+#pragma once
+
+#include <float.h>
+#include <hip/hip_runtime.h>
+#include <limits.h>
+
+#include <type_traits>
+
+#include "ops_mi355x_gen.h"
+
+namespace opsmi_gen {
+
+constexpr int kWave = 64;
+constexpr int kRows = 4;
+
+struct Loop {
+  int x0, x1, y0, y1, z0, z1;
+  int chunk, three_d;
+};
+
+inline Loop loop_of(const opsmi_gen_launch *L) {
+  Loop l;
+  l.x0 = L->range[0];
+  l.x1 = L->range[1];
+  l.y0 = L->range[2];
+  l.y1 = L->range[3];
+  l.z0 = L->range[4];
+  l.z1 = L->range[5];
+  l.chunk = L->chunk;
+  l.three_d = L->three_d;
+  return l;
+}
+
+// NDIM: block dimensions; MULTI: the dat may have more than one component
+// (otherwise the component count is the constant 1)
+template <class T, int NDIM, bool MULTI>
+class Acc {
+  T *p_;
+  long md_, sy_, sz_;
+
+  __device__ __forceinline__ long pt(int dx, int dy, int dz) const {
+    long o = (long)dx * (MULTI ? md_ : 1);
+    if (NDIM > 1) o += (long)dy * sy_;
+    if (NDIM > 2) o += (long)dz * sz_;
+    return o;
+  }
+
+ public:
+  __device__ __forceinline__ Acc(const opsmi_gen_dat &d, int x, int y, int z)
+      : md_(d.md), sy_(d.sy), sz_(d.sz) {
+    p_ = (T *)d.base + pt(x, y, z);
+  }
+  // NDIM indices: a point; NDIM + 1: component, then the point
+  template <class... I>
+  __device__ __forceinline__ T &operator()(I... i) const {
+    constexpr int n = sizeof...(I);
+    static_assert(n == NDIM || n == NDIM + 1, "ACC: give NDIM offsets, or a component and NDIM offsets");
+    const int v[n + 3] = {int(i)..., 0, 0, 0};
+    if constexpr (n == NDIM) return p_[pt(v[0], v[1], v[2])];
+    else return p_[v[0] + pt(v[1], v[2], v[3])];
+  }
+  __device__ void combine_max(int dx, T val) const { amax(&p_[pt(dx, 0, 0)], val); }
+  __device__ void combine_max(int dx, int dy, T val) const { amax(&p_[pt(dx, dy, 0)], val); }
+  __device__ void combine_max(int dx, int dy, int dz, T val) const { amax(&p_[pt(dx, dy, dz)], val); }
+  __device__ void combine_min(int dx, T val) const { amin(&p_[pt(dx, 0, 0)], val); }
+  __device__ void combine_min(int dx, int dy, T val) const { amin(&p_[pt(dx, dy, 0)], val); }
+  __device__ void combine_min(int dx, int dy, int dz, T val) const { amin(&p_[pt(dx, dy, dz)], val); }
+  __device__ void combine_inc(int dx, T val) const { atomicAdd(&p_[pt(dx, 0, 0)], val); }
+  __device__ void combine_inc(int dx, int dy, T val) const { atomicAdd(&p_[pt(dx, dy, 0)], val); }
+  __device__ void combine_inc(int dx, int dy, int dz, T val) const { atomicAdd(&p_[pt(dx, dy, dz)], val); }
+
+ private:
+  // compare-and-swap loops on the value's bits (integers use the native
+  // atomics)
+  template <bool MAX>
+  __device__ static void acas(T *a, T val) {
+    if constexpr (sizeof(T) == 8) {
+      unsigned long long *u = (unsigned long long *)a;
+      unsigned long long old = *u;
+      while (true) {
+        const T cur = __longlong_as_double(old);
+        if (!(MAX ? val > cur : val < cur)) break;
+        const unsigned long long prev = atomicCAS(u, old, (unsigned long long)__double_as_longlong(val));
+        if (prev == old) break;
+        old = prev;
+      }
+    } else {
+      unsigned *u = (unsigned *)a;
+      unsigned old = *u;
+      while (true) {
+        const T cur = __uint_as_float(old);
+        if (!(MAX ? val > cur : val < cur)) break;
+        const unsigned prev = atomicCAS(u, old, __float_as_uint(val));
+        if (prev == old) break;
+        old = prev;
+      }
+    }
+  }
+  __device__ static void amax(T *a, T val) {
+    if constexpr (std::is_integral_v<T>) atomicMax(a, val);
+    else acas<true>(a, val);
+  }
+  __device__ static void amin(T *a, T val) {
+    if constexpr (std::is_integral_v<T>) atomicMin(a, val);
+    else acas<false>(a, val);
+  }
+};
+
+// ------------------------------------------------------------ reductions
+template <class T, int OP>
+__device__ __forceinline__ T ident() {
+  if constexpr (OP == OPS_INC) return T(0);
+  else if constexpr (OP == OPS_MIN) {
+    if constexpr (std::is_integral_v<T>) return (T)INT_MAX;
+    else if constexpr (sizeof(T) == 8) return (T)DBL_MAX;
+    else return (T)FLT_MAX;
+  } else {
+    if constexpr (std::is_integral_v<T>) return (T)-INT_MAX;
+    else if constexpr (sizeof(T) == 8) return (T)-DBL_MAX;
+    else return (T)-FLT_MAX;
+  }
+}
+
+template <class T, int OP>
+__device__ __forceinline__ T combine(T a, T b) {
+  if constexpr (OP == OPS_INC) return a + b;
+  else if constexpr (!std::is_integral_v<T>) return OP == OPS_MIN ? (T)fmin(a, b) : (T)fmax(a, b);
+  else return OP == OPS_MIN ? (a < b ? a : b) : (a > b ? a : b);
+}
+
+// folds v over the workgroup and stores part[workgroup * D + q]; every
+// thread of the workgroup calls it
+template <class T, int OP, int D>
+__device__ __forceinline__ void store_partials(T (&v)[D], T *part) {
+  __shared__ T sh[kRows][D];
+  const int lane = threadIdx.x, w = threadIdx.y;
+#pragma unroll
+  for (int q = 0; q < D; q++) {
+    T s = v[q];
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) s = combine<T, OP>(s, __shfl_xor(s, off, kWave));
+    if (lane == 0) sh[w][q] = s;
+  }
+  __syncthreads();
+  if (lane == 0 && w == 0) {
+    const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+#pragma unroll
+    for (int q = 0; q < D; q++) {
+      T s = sh[0][q];
+      for (int k = 1; k < kRows; k++) s = combine<T, OP>(s, sh[k][q]);
+      part[(size_t)b * D + q] = s;
+    }
+  }
+}
+
+// ------------------------------------------------------------ point mapping
+// workgroup b runs on XCD b % 8 and takes logical tile start(b % 8) + b / 8,
+// XCD k owning a contiguous share of the nb tiles (a bijection on [0, nb))
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
+  const unsigned q8 = nb >> 3, r8 = nb & 7u, k = b & 7u;
+  return k * q8 + (k < r8 ? k : r8) + (b >> 3);
+}
+
+// calls f(x, y, z) for this thread's points
+template <class F>
+__device__ __forceinline__ void for_each_point(const Loop &L, F &&f) {
+  const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned l = xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nb);
+  const unsigned tx = l % gridDim.x, rest = l / gridDim.x;
+  const unsigned ty = rest % gridDim.y, tz = rest / gridDim.y;
+  const int x = L.x0 + (int)tx * kWave + (int)threadIdx.x;
+  if (x >= L.x1) return;
+  if (L.three_d) {
+    const int y = L.y0 + (int)ty * kRows + (int)threadIdx.y;
+    if (y >= L.y1) return;
+    const int z0 = L.z0 + (int)tz * L.chunk;
+    const int z1 = min(z0 + L.chunk, L.z1);
+    for (int z = z0; z < z1; z++) f(x, y, z);
+  } else {
+    const int y0 = L.y0 + (int)ty * kRows * L.chunk;
+    const int y1 = min(y0 + kRows * L.chunk, L.y1);
+    for (int y = y0 + (int)threadIdx.y; y < y1; y += kRows) f(x, y, L.z0);
+  }
+}
+
+}  // namespace opsmi_gen

@@ -1,0 +1,85 @@
+/* Run-time support for translator-generated loops (MI355X OPS backend).
+ *
+ * ops_amd/translator/opsmi_translate.py turns an application's
+ * ops_par_loop(kernel, name, block, dim, range, args...) calls into calls of
+ * generated host stubs ops_par_loop_<kernel>(name, block, dim, range,
+ * args...), the shape the reference translator emits
+ * (ops_translator/resources/templates/cpp/loop_host.cpp.j2:12-33), and
+ * writes one HIP kernel per user kernel (device side:
+ * include/ops_mi355x_device.h).  A stub packs what the loop needs by value
+ * (ops_arg_gbl values and ops_decl_const constants, taken when the loop is
+ * called, as the reference's lazy queue copies them,
+ * ops/c/src/core/ops_lazy.cpp:1433-1438) and hands the loop to
+ * ops_mi_gen_par_loop, which validates it, queues it with the hand-written
+ * loops (same dependency tracking and reduction handling) and calls the
+ * stub's launch function when the loop runs.
+ *
+ * This is the backend's counterpart of the reference HIP/CUDA run-time
+ * support used by generated code (ops/c/include/ops_hip_rt_support.h,
+ * ops/c/src/hip/ops_hip_common.cpp): plain C, no HIP types.
+ */
+#ifndef OPS_MI355X_GEN_H
+#define OPS_MI355X_GEN_H
+
+#include <stddef.h>
+
+#include "ops_mi355x.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OPSMI_GEN_MAX_ARGS 32
+
+/* element type codes of a generated loop's arguments */
+enum { OPSMI_GEN_F64 = 0, OPSMI_GEN_F32 = 1, OPSMI_GEN_I32 = 2, OPSMI_GEN_NONE = 3 };
+
+/* one dat argument as the kernel sees it: base = device address of
+ * component 0 of point (0, 0, 0) of the dat's current buffer; strides in
+ * elements (md = components per point, the x stride of an AoS dat) */
+typedef struct opsmi_gen_dat {
+  void *base;
+  long md, sy, sz;
+} opsmi_gen_dat;
+
+/* what a launch function receives when its loop runs */
+typedef struct opsmi_gen_launch {
+  int range[6];      /* x0, x1, y0, y1, z0, z1 (1, 1 for missing dimensions) */
+  int chunk;         /* 3D: z planes per workgroup; 2D/1D: groups of 4 rows */
+  int three_d;       /* 1: grid[2] runs over z chunks */
+  unsigned grid[3];  /* workgroups of 64 x 4 threads */
+  opsmi_gen_dat dat[OPSMI_GEN_MAX_ARGS];    /* dat arguments, by argument index */
+  void *partials[OPSMI_GEN_MAX_ARGS];       /* reduction arguments: dim values per workgroup */
+} opsmi_gen_launch;
+
+/* launches the loop's kernel on `stream` (a hipStream_t); blob is the
+ * stub's by-value data */
+typedef void (*opsmi_gen_launch_fn)(const opsmi_gen_launch *L, const void *blob, void *stream);
+
+/* Queues a generated loop.  types[i]: element type the kernel declares for
+ * argument i (dat, gbl or reduce; OPSMI_GEN_NONE for ops_arg_idx).
+ * npts[i]: number of points the kernel reads or writes through dat argument
+ * i, listed in pts as (component or -1, dx, dy, dz) quadruples, argument
+ * after argument; -1 when the translator could not list them (non-literal
+ * indices), 0 for non-dat arguments.  Checks, as the hand-written entry
+ * points do (ops/c/src/core/ops_lib_core.cpp:1704-1836 under OPS_DEBUG):
+ * argument kinds and access modes, element types against the declared ones,
+ * component counts, that each listed point is in the argument's stencil and
+ * each listed component below the dat's count, and that range plus the
+ * stencil's reach stays inside the dat.  Reductions: up to 4 components
+ * each; the kernel stores one partial per workgroup and component, which
+ * the library folds in a fixed order into the handle. */
+void ops_mi_gen_par_loop(const char *name, ops_block block, int dim, const int *range, const ops_arg *args,
+                         int nargs, const int *types, const int *npts, const int *pts, opsmi_gen_launch_fn fn,
+                         const void *blob, size_t blob_bytes);
+
+/* Copies the current value of constant `name` (declared with
+ * ops_decl_const) into dst; an OPS error if it is missing or its size is
+ * not dim values of `type`. */
+void ops_mi_gen_const(const char *name, const char *type, int dim, void *dst);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* OPS_MI355X_GEN_H */

@@ -253,6 +253,109 @@ Range3 check_loop(const char *name, ops_block block, int dim, const int *range, 
   return r;
 }
 
+Range3 check_gen_loop(const char *name, ops_block block, int dim, const int *range, const ops_arg *args, int nargs,
+                      const int *types, const int *npts, const int *pts) {
+  require_init();
+  if (!block) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": NULL block"));
+  if (dim != block->dims || dim < 1 || dim > 3)
+    fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": dim ", dim, " does not match block ", block->name, " (",
+                                   block->dims, "D); this backend supports 1-3D"));
+  if (!range) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": NULL range"));
+  if (nargs < 0 || nargs > OPSMI_GEN_MAX_ARGS || (nargs > 0 && (!args || !types)))
+    fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": ", nargs, " arguments (at most ", OPSMI_GEN_MAX_ARGS, ")"));
+  Range3 r{0, 1, 0, 1, 0, 1};
+  r.x0 = range[0];
+  r.x1 = range[1];
+  if (dim > 1) { r.y0 = range[2]; r.y1 = range[3]; }
+  if (dim > 2) { r.z0 = range[4]; r.z1 = range[5]; }
+  if (nargs > 0 && (!npts || !pts)) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": no point lists"));
+  const int *pt = pts;
+  for (int i = 0; i < nargs; i++) {
+    const ops_arg &a = args[i];
+    const DType want = (DType)types[i];
+    const int es = want == DT_F64 ? 8 : 4;
+    const int np = npts[i];
+    const int *mine = pt;
+    if (np > 0) pt += 4 * np;
+    if (a.argtype == OPS_ARG_IDX) continue;
+    if (want != DT_F64 && want != DT_F32 && want != DT_I32)
+      fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": arg ", i, " has no element type (only double, float, int)"));
+    if (a.argtype == OPS_ARG_GBL) {
+      if (a.acc == OPS_READ) {
+        if (!a.data || a.dim < 1 || a.elem_size != es)
+          fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": gbl arg ", i, " must hold ", dt_name(want), " values"));
+      } else if (a.acc == OPS_INC || a.acc == OPS_MIN || a.acc == OPS_MAX) {
+        if (!a.data) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": arg ", i, " has no reduction handle"));
+        ops_reduction h = (ops_reduction)a.data;
+        RedImpl *ri = (RedImpl *)h->impl;
+        if (ri->dt != want)
+          fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": reduction handle ", h->name, " has type ", dt_name(ri->dt),
+                                         ", kernel reduces ", dt_name(want)));
+        if (a.dim < 1 || a.dim > 4)
+          fail(OPS_NOT_IMPLEMENTED, cat("loop ", name, ": reduction ", h->name, " has ", a.dim,
+                                        " components (this backend folds 1 to 4)"));
+        if (h->size < a.dim * es)
+          fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": reduction handle ", h->name, " must hold ", a.dim, " values"));
+      } else {
+        fail(OPS_NOT_IMPLEMENTED, cat("loop ", name, ": gbl arg ", i, " must be OPS_READ or a reduction"));
+      }
+      continue;
+    }
+    if (a.argtype != OPS_ARG_DAT) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": arg ", i, " has unknown kind"));
+    if (!a.dat) {
+      if (a.opt) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": dat arg ", i, " is NULL"));
+      continue;  // ops_arg_dat_opt switched off: the kernel must not touch it
+    }
+    ops_dat d = a.dat;
+    if (d->block != block)
+      fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": dat ", d->name, " is not declared on block ", block->name));
+    if (d->locked_hd)
+      fail(OPS_RUNTIME_ERROR, cat("loop ", name, ": dat ", d->name, " is locked by ops_dat_get_raw_pointer (release it first)"));
+    if (impl(d)->dt != want)
+      fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": dat ", d->name, " has type ", dt_name(impl(d)->dt),
+                                     ", the kernel declares ", dt_name(want)));
+    if (a.dim != d->dim)
+      fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": dat ", d->name, " has ", d->dim, " components, the argument says ",
+                                     a.dim));
+    ops_stencil s = a.stencil;
+    if (!s) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": dat ", d->name, " has no stencil"));
+    if (s->mgrid_stride)
+      fail(OPS_NOT_IMPLEMENTED, cat("loop ", name, ": stencil ", s->name, " is a multigrid (restrict/prolong) stencil"));
+    if (s->stride)
+      for (int q = 0; q < s->dims; q++)
+        if (s->stride[q] != 1)
+          fail(OPS_NOT_IMPLEMENTED, cat("loop ", name, ": stencil ", s->name, " is strided (this backend takes unit strides)"));
+    for (int k = 0; k < np; k++) {
+      const int *q4 = mine + 4 * k;
+      if (q4[0] >= d->dim)
+        fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": the kernel reads component ", q4[0], " of dat ", d->name,
+                                       ", which has ", d->dim));
+      if (!stencil_has(s, q4 + 1))
+        fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": stencil ", s->name, " of dat ", d->name, " lacks point (",
+                                       q4[1], ",", q4[2], ",", q4[3], ") read by the kernel"));
+    }
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int p = 0; p < s->points; p++)
+      for (int q = 0; q < 3 && q < s->dims; q++) {
+        lo[q] = std::min(lo[q], s->stencil[p * s->dims + q]);
+        hi[q] = std::max(hi[q], s->stencil[p * s->dims + q]);
+      }
+    int rl[3] = {r.x0, r.y0, r.z0}, rh[3] = {r.x1, r.y1, r.z1};
+    if (r.npts() > 0)
+      for (int q = 0; q < dim; q++) {
+        int first = d->d_m[q] + d->base[q];
+        int last = d->size[q] + d->d_m[q] + d->base[q];
+        if (rl[q] + lo[q] < first || rh[q] - 1 + hi[q] >= last)
+          fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": range [", rl[q], ",", rh[q], ") in dimension ", q,
+                                         " with stencil reach [", lo[q], ",", hi[q], "] leaves dat ", d->name,
+                                         " (valid indices [", first, ",", last, "))"));
+      }
+  }
+  for (int i = 0; i < nargs; i++)
+    if (args[i].argtype == OPS_ARG_DAT && args[i].dat) sync_to_device(args[i].dat);
+  return r;
+}
+
 const Const &get_const(const char *name) {
   auto it = inst().consts.find(name);
   if (it == inst().consts.end()) fail(OPS_RUNTIME_CONFIGURATION_ERROR, cat("constant ", name, " was not declared with ops_decl_const"));
@@ -399,7 +502,8 @@ void deps_from_args(LoopRec &rec, const ops_arg *args, int n) {
       if (a.acc != OPS_WRITE) rec.reads.push_back(a.dat);
       if (a.acc != OPS_READ) rec.writes.push_back(a.dat);
     } else if (a.argtype == OPS_ARG_GBL && a.data && (a.acc == OPS_INC || a.acc == OPS_MIN || a.acc == OPS_MAX)) {
-      rec.red = (ops_reduction)a.data;
+      if (!rec.red) rec.red = (ops_reduction)a.data;
+      else rec.more_reds.push_back((ops_reduction)a.data);
     }
   }
 }
@@ -410,10 +514,22 @@ static bool meet(const std::vector<ops_dat> &x, const std::vector<ops_dat> &y) {
       if (p == q) return true;
   return false;
 }
+static bool reduces_into(const LoopRec &L, ops_reduction h) {
+  if (!h) return false;
+  if (L.red == h) return true;
+  for (ops_reduction q : L.more_reds)
+    if (q == h) return true;
+  return false;
+}
+static bool share_red(const LoopRec &a, const LoopRec &b) {
+  if (reduces_into(b, a.red)) return true;
+  for (ops_reduction q : a.more_reds)
+    if (reduces_into(b, q)) return true;
+  return false;
+}
 // a (earlier) must run before b
 static bool conflict(const LoopRec &a, const LoopRec &b) {
-  return meet(a.writes, b.reads) || meet(a.writes, b.writes) || meet(a.reads, b.writes) ||
-         (a.red && a.red == b.red);
+  return meet(a.writes, b.reads) || meet(a.writes, b.writes) || meet(a.reads, b.writes) || share_red(a, b);
 }
 
 static void invalidate_spare(ops_dat d) {
@@ -460,7 +576,7 @@ void flush_for(ops_reduction h) {
   std::vector<char> need(n, 0);
   bool any = false;
   for (long i = n - 1; i >= 0; i--) {
-    if (g_queue[i].red == h) {
+    if (reduces_into(g_queue[i], h)) {
       need[i] = 1;
       any = true;
       continue;

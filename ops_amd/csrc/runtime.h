@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/ops_mi355x.h"
+#include "../../include/ops_mi355x_gen.h"
 
 namespace opsmi {
 
@@ -183,6 +184,15 @@ struct Range3 {
 Range3 check_loop(const char *name, ops_block block, int dim, const int *range, const ops_arg *args,
                   const std::vector<ArgSpec> &specs, DType *dat_dt);
 
+// Validates the args of a translator-generated loop against the element
+// types its kernel declares (types[i], DType codes; DT_OTHER for idx args):
+// kinds, access modes, types, component counts, unit-stride stencils, the
+// points the kernel touches (npts/pts as in ops_mi_gen_par_loop) inside
+// each stencil, and range plus stencil reach inside every dat.  Returns the
+// loop range.
+Range3 check_gen_loop(const char *name, ops_block block, int dim, const int *range, const ops_arg *args, int nargs,
+                      const int *types, const int *npts, const int *pts);
+
 // value of a READ gbl arg
 template <class T>
 T gbl(const ops_arg &a) {
@@ -245,6 +255,7 @@ struct LoopRec {
   int dims = 0;
   std::vector<ops_dat> reads, writes;  // dats read (READ/RW/INC), written (WRITE/RW/INC)
   ops_reduction red = nullptr;         // handle this loop reduces into
+  std::vector<ops_reduction> more_reds;  // further handles (loops with several reduction args)
   ops_dat d[8] = {};                   // kind-specific dat operands
   double g[4] = {0, 0, 0, 0};          // kind-specific scalar operands
   RedCtl ctl{};                        // reduction control of a reducing loop

@@ -396,14 +396,16 @@ def ops_mi_reset_stats():
 
 
 SERVE_HEADER = os.path.join(ROOT, "include", "ops_mi355x_serve.h")
+GEN_HEADER = os.path.join(ROOT, "include", "ops_mi355x_gen.h")
 
 
 def declared_symbols():
-    """C entry points declared in include/ops_mi355x.h (the extern "C" part)
-    and include/ops_mi355x_serve.h, all exported by libopsmi.so."""
+    """C entry points declared in include/ops_mi355x.h (the extern "C" part),
+    include/ops_mi355x_serve.h and include/ops_mi355x_gen.h, all exported by
+    libopsmi.so."""
     import re
     text = open(HEADER).read()
-    c_part = text.split("} /* extern \"C\" */")[0] + open(SERVE_HEADER).read()
+    c_part = text.split("} /* extern \"C\" */")[0] + open(SERVE_HEADER).read() + open(GEN_HEADER).read()
     c_part = re.sub(r"/\*.*?\*/", "", c_part, flags=re.S)
     names = re.findall(r"\b([a-z_][a-zA-Z0-9_]*)\s*\(", c_part)
     skip = {"if", "for", "while", "sizeof", "return"}

@@ -1,0 +1,561 @@
+"""Source-to-source translator for OPS applications (MI355X backend).
+
+Reads C++ application files written against the OPS dev API
+(ops_par_loop(kernel, name, block, dim, range, args...), user kernels taking
+ACC<T> references, ops/c/include/ops_seq_v2.h:327-335) and writes, per
+application file app.cpp:
+
+  app_ops.cpp      the application with every ops_par_loop(kernel, ...) call
+                   rewritten to ops_par_loop_<kernel>(...) (the reference
+                   translator's rewrite, ops_translator/ops-translator
+                   "loop_host" stubs: resources/templates/cpp/loop_host.cpp.j2:12-33),
+                   its OPS include pointed at ops_mi355x.h and the kernel
+                   definitions and kernel-header includes dropped;
+  app_opsmi_kernels.h    declarations of the generated stubs;
+  app_opsmi_kernels.hip  one HIP kernel per user kernel plus its host stub,
+                         built on include/ops_mi355x_device.h and
+                         include/ops_mi355x_gen.h.
+
+The user kernel's body is emitted unchanged as the call operator of a
+struct whose members are the ops_decl_const constants the body names, so the
+body's arithmetic is the user's own (compile with -ffp-contract=off for
+the reference's sequential rounding).  Requirements, as for the reference
+translator: the kernel is a literal identifier, the loop's dim is an integer
+literal, every ops_arg_* call spells its type as a string literal and its
+access as an OPS_* name, ops_arg_gbl/ops_arg_reduce dims are integer
+literals, and the kernel is defined in the application file or a header it
+includes with quotes.
+
+Usage: python -m ops_amd.translator.opsmi_translate app.cpp [...] -o outdir [-I dir]
+"""
+import argparse
+import os
+import re
+import sys
+
+TYPES = {"double": ("OPSMI_GEN_F64", 8), "float": ("OPSMI_GEN_F32", 4), "int": ("OPSMI_GEN_I32", 4)}
+ACCESS = {"OPS_READ", "OPS_WRITE", "OPS_RW", "OPS_INC", "OPS_MIN", "OPS_MAX"}
+
+
+class TranslateError(Exception):
+    pass
+
+
+# ------------------------------------------------------------------ lexing
+def mask(src):
+    """src with comments and the insides of string/char literals replaced by
+    spaces (newlines kept), so offsets match the original text."""
+    out = list(src)
+    i, n = 0, len(src)
+    while i < n:
+        c = src[i]
+        if src.startswith("//", i):
+            j = src.find("\n", i)
+            j = n if j < 0 else j
+            for k in range(i, j):
+                out[k] = " "
+            i = j
+        elif src.startswith("/*", i):
+            j = src.find("*/", i + 2)
+            j = n if j < 0 else j + 2
+            for k in range(i, j):
+                if out[k] != "\n":
+                    out[k] = " "
+            i = j
+        elif c in "\"'":
+            j = i + 1
+            while j < n and src[j] != c:
+                j += 2 if src[j] == "\\" else 1
+            for k in range(i + 1, min(j, n)):
+                if out[k] != "\n":
+                    out[k] = " "
+            i = j + 1
+        else:
+            i += 1
+    return "".join(out)
+
+
+def match_close(m, i):
+    """index of the bracket closing the one at m[i]"""
+    pairs = {"(": ")", "[": "]", "{": "}"}
+    stack = []
+    for j in range(i, len(m)):
+        c = m[j]
+        if c in pairs:
+            stack.append(pairs[c])
+        elif c in ")]}":
+            if not stack or stack.pop() != c:
+                raise TranslateError(f"unbalanced '{c}' at offset {j}")
+            if not stack:
+                return j
+    raise TranslateError(f"no closing bracket for offset {i}")
+
+
+def split_top(m, a, b):
+    """(start, end) spans of the comma-separated items of m[a:b]"""
+    spans, depth, s = [], 0, a
+    for j in range(a, b):
+        c = m[j]
+        if c in "([{":
+            depth += 1
+        elif c in ")]}":
+            depth -= 1
+        elif c == "," and depth == 0:
+            spans.append((s, j))
+            s = j + 1
+    if m[s:b].strip():
+        spans.append((s, b))
+    return spans
+
+
+def strip_span(src, span):
+    a, b = span
+    while a < b and src[a].isspace():
+        a += 1
+    while b > a and src[b - 1].isspace():
+        b -= 1
+    return a, b
+
+
+def string_literal(text, what):
+    t = text.strip()
+    if not (len(t) >= 2 and t[0] == '"' and t[-1] == '"'):
+        raise TranslateError(f"{what} must be a string literal, got {t!r}")
+    return t[1:-1]
+
+
+def int_literal(text):
+    t = text.strip()
+    return int(t) if re.fullmatch(r"\d+", t) else None
+
+
+# ------------------------------------------------------------------ model
+class Arg:
+    def __init__(self, kind, typ=None, acc=None, dim=None, multi=False):
+        self.kind = kind      # dat | gbl | reduce | idx
+        self.typ = typ        # double | float | int
+        self.acc = acc        # OPS_* access
+        self.dim = dim        # gbl/reduce component count
+        self.multi = multi    # dat: may have more than one component
+
+
+class Loop:
+    def __init__(self, kernel, ndim, args, span, name_span):
+        self.kernel, self.ndim, self.args = kernel, ndim, args
+        self.span = span            # (start of 'ops_par_loop', end of the first argument's comma)
+        self.name_span = name_span
+
+
+class Kernel:
+    def __init__(self, name, params, body, path, span):
+        self.name, self.params, self.body, self.path, self.span = name, params, body, path, span
+
+
+def parse_arg(src, m, span, where):
+    a, b = strip_span(src, span)
+    text_m = m[a:b]
+    mm = re.match(r"(ops_arg_dat_opt|ops_arg_dat|ops_arg_gbl|ops_arg_reduce|ops_arg_idx)\s*(<[^>]*>)?\s*\(", text_m)
+    if not mm:
+        raise TranslateError(f"{where}: argument {src[a:b]!r} is not an ops_arg_* call")
+    fn = mm.group(1)
+    op = a + mm.end() - 1
+    cl = match_close(m, op)
+    items = [src[s:e].strip() for s, e in split_top(m, op + 1, cl)]
+    if fn == "ops_arg_idx":
+        return Arg("idx")
+    if fn in ("ops_arg_dat", "ops_arg_dat_opt"):
+        if len(items) < 5:
+            raise TranslateError(f"{where}: {fn} needs 5 arguments")
+        typ = string_literal(items[3], f"{where}: {fn} type")
+        acc = items[4]
+        d = int_literal(items[1])
+        return Arg("dat", typ, acc, d, multi=(d != 1))
+    if len(items) != 4:
+        raise TranslateError(f"{where}: {fn} needs 4 arguments")
+    typ = string_literal(items[2], f"{where}: {fn} type")
+    acc = items[3]
+    d = int_literal(items[1])
+    if d is None:
+        raise TranslateError(f"{where}: {fn} dim must be an integer literal")
+    kind = "gbl" if fn == "ops_arg_gbl" else "reduce"
+    if kind == "gbl" and acc != "OPS_READ":
+        kind = "reduce"
+    return Arg(kind, typ, acc, d)
+
+
+def find_loops(src, m, path):
+    loops = []
+    for mo in re.finditer(r"\bops_par_loop\s*\(", m):
+        op = mo.end() - 1
+        cl = match_close(m, op)
+        spans = split_top(m, op + 1, cl)
+        where = f"{path}:{src.count(chr(10), 0, mo.start()) + 1}"
+        if len(spans) < 5:
+            raise TranslateError(f"{where}: ops_par_loop needs kernel, name, block, dim, range")
+        kernel = src[slice(*strip_span(src, spans[0]))]
+        if not re.fullmatch(r"[A-Za-z_]\w*", kernel):
+            raise TranslateError(f"{where}: kernel {kernel!r} is not an identifier")
+        nd = int_literal(src[slice(*spans[3])])
+        if nd not in (1, 2, 3):
+            raise TranslateError(f"{where}: loop dim must be the integer literal 1, 2 or 3")
+        args = [parse_arg(src, m, sp, where) for sp in spans[5:]]
+        for x in args:
+            if x.kind != "idx":
+                if x.typ not in TYPES:
+                    raise TranslateError(f"{where}: type {x.typ!r} (only double, float, int)")
+                if x.acc not in ACCESS:
+                    raise TranslateError(f"{where}: access {x.acc!r} is not an OPS_* access")
+        loops.append(Loop(kernel, nd, args, (mo.start(), spans[0][1] + 1), spans[1]))
+    return loops
+
+
+def find_consts(src, m):
+    """ops_decl_const("name", dim, "type", ptr) calls: name -> (dim, type)"""
+    out = {}
+    for mo in re.finditer(r"\bops_decl_const\s*(<[^>]*>)?\s*\(", m):
+        op = mo.end() - 1
+        cl = match_close(m, op)
+        items = [src[s:e].strip() for s, e in split_top(m, op + 1, cl)]
+        if len(items) != 4:
+            continue
+        try:
+            name = string_literal(items[0], "constant name")
+            typ = string_literal(items[2], "constant type")
+        except TranslateError:
+            continue
+        d = int_literal(items[1])
+        if d is None or typ not in TYPES:
+            raise TranslateError(f"ops_decl_const({items[0]}): dim must be an integer literal, type double/float/int")
+        out[name] = (d, typ)
+    return out
+
+
+def find_kernel(name, files):
+    """(Kernel) definition `void name(params) { body }` in one of files"""
+    for path, src, m in files:
+        for mo in re.finditer(r"\bvoid\s+" + re.escape(name) + r"\s*\(", m):
+            op = mo.end() - 1
+            cl = match_close(m, op)
+            j = cl + 1
+            while j < len(m) and m[j].isspace():
+                j += 1
+            if j >= len(m) or m[j] != "{":
+                continue  # a declaration
+            bcl = match_close(m, j)
+            params = []
+            for s, e in split_top(m, op + 1, cl):
+                text = src[s:e].strip()
+                pm = re.search(r"([A-Za-z_]\w*)\s*$", m[s:e].strip())
+                if not pm:
+                    raise TranslateError(f"{path}: kernel {name}: cannot read parameter {text!r}")
+                params.append((text, pm.group(1)))
+            start = mo.start()
+            # a 'static' or 'inline' in front belongs to the definition
+            pre = re.search(r"((?:static|inline)\s+)+$", m[max(0, start - 32):start])
+            if pre:
+                start -= len(pre.group(0))
+            return Kernel(name, params, src[j:bcl + 1], path, (start, bcl + 1))
+    raise TranslateError(f"kernel {name} is not defined in {', '.join(p for p, _, _ in files)}")
+
+
+def local_includes(path, src, m, inc_dirs):
+    out = []
+    for mo in re.finditer(r'#\s*include\s*"([^"]+)"', src):
+        if m[mo.start()] != "#":
+            continue  # inside a comment
+        for d in [os.path.dirname(os.path.abspath(path))] + inc_dirs:
+            p = os.path.join(d, mo.group(1))
+            if os.path.isfile(p):
+                out.append((mo.start(), mo.end(), os.path.abspath(p), mo.group(1)))
+                break
+    return out
+
+
+# ------------------------------------------------------------------ emit
+def accessed_points(body, pname, ndim):
+    """(component, dx, dy, dz) of every ACC call pname(...) in body whose
+    indices are integer literals; None when one is not (then the stencil is
+    not checked against the kernel)."""
+    bm = mask(body)
+    pts = set()
+    for mo in re.finditer(r"\b" + re.escape(pname) + r"\s*\(", bm):
+        # skip member calls (pname.combine_max(...) has no '(' right after the name)
+        op = mo.end() - 1
+        cl = match_close(bm, op)
+        items = [bm[s:e].strip() for s, e in split_top(bm, op + 1, cl)]
+        vals = []
+        for it in items:
+            if not re.fullmatch(r"[+-]?\s*\d+", it):
+                return None
+            vals.append(int(it.replace(" ", "")))
+        if len(vals) == ndim:
+            c, off = -1, vals
+        elif len(vals) == ndim + 1:
+            c, off = vals[0], vals[1:]
+        else:
+            return None
+        off = off + [0] * (3 - len(off))
+        pts.add((c, off[0], off[1], off[2]))
+    if re.search(r"\b" + re.escape(pname) + r"\s*\.\s*combine_", bm):
+        return None
+    return sorted(pts)
+
+
+def kernel_code(loop, kern, consts):
+    K = loop.kernel
+    n = len(loop.args)
+    if len(kern.params) != n:
+        raise TranslateError(f"kernel {K} takes {len(kern.params)} parameters, the loop passes {n} arguments")
+    used = [c for c in sorted(consts) if re.search(r"\b" + re.escape(c) + r"\b", mask(kern.body))]
+    points = [accessed_points(kern.body, pname, loop.ndim) if a.kind == "dat" else []
+              for (_, pname), a in zip(kern.params, loop.args)]
+    L = []
+    L.append(f"// ---------------------------------------------------------------- {K}")
+    L.append(f"struct opsk_{K} {{")
+    for c in used:
+        d, t = consts[c]
+        L.append(f"  {t} {c}{'[' + str(d) + ']' if d > 1 else ''};")
+    ps = []
+    for (text, pname), a in zip(kern.params, loop.args):
+        if a.kind == "dat":
+            multi = "true" if a.multi else "false"
+            new, cnt = re.subn(r"\bACC\s*<\s*([^>]+?)\s*>", rf"::opsmi_gen::Acc<\1, {loop.ndim}, {multi}>", text)
+            if cnt != 1:
+                raise TranslateError(f"kernel {K}: parameter {text!r} for an ops_arg_dat must be an ACC<T> reference")
+            ps.append(new)
+        else:
+            ps.append(text)
+    L.append(f"  __device__ __forceinline__ void operator()({', '.join(ps)}) const {kern.body}")
+    L.append("};")
+    # by-value data: constants and gbl values
+    L.append(f"struct opsb_{K} {{")
+    L.append(f"  opsk_{K} k;")
+    for i, a in enumerate(loop.args):
+        if a.kind == "gbl":
+            L.append(f"  {a.typ} g{i}[{a.dim}];")
+    L.append("};")
+    L.append(f"struct opsa_{K} {{")
+    L.append("  ::opsmi_gen::Loop L;")
+    for i, a in enumerate(loop.args):
+        if a.kind == "dat":
+            L.append(f"  opsmi_gen_dat d{i};")
+        elif a.kind == "reduce":
+            L.append(f"  {a.typ} *r{i};")
+    L.append(f"  opsb_{K} b;")
+    L.append("};")
+    L.append(f"__global__ __launch_bounds__(256) void opsg_{K}(const opsa_{K} A) {{")
+    for i, a in enumerate(loop.args):
+        if a.kind == "reduce":
+            L.append(f"  {a.typ} red{i}[{a.dim}];")
+            L.append(f"  for (int q = 0; q < {a.dim}; q++) red{i}[q] = ::opsmi_gen::ident<{a.typ}, {a.acc}>();")
+    L.append("  ::opsmi_gen::for_each_point(A.L, [&](int x, int y, int z) {")
+    call = []
+    for i, a in enumerate(loop.args):
+        if a.kind == "dat":
+            L.append(f"    ::opsmi_gen::Acc<{a.typ}, {loop.ndim}, {'true' if a.multi else 'false'}> a{i}(A.d{i}, x, y, z);")
+            call.append(f"a{i}")
+        elif a.kind == "idx":
+            L.append(f"    int idx{i}[3] = {{x, y, z}};")
+            call.append(f"idx{i}")
+        elif a.kind == "gbl":
+            call.append(f"A.b.g{i}")
+        else:
+            call.append(f"red{i}")
+    L.append(f"    A.b.k({', '.join(call)});")
+    L.append("  });")
+    for i, a in enumerate(loop.args):
+        if a.kind == "reduce":
+            L.append(f"  ::opsmi_gen::store_partials<{a.typ}, {a.acc}, {a.dim}>(red{i}, A.r{i});")
+    L.append("}")
+    L.append(f"void opsl_{K}(const opsmi_gen_launch *L, const void *blob, void *stream) {{")
+    L.append(f"  opsa_{K} A;")
+    L.append("  A.L = ::opsmi_gen::loop_of(L);")
+    for i, a in enumerate(loop.args):
+        if a.kind == "dat":
+            L.append(f"  A.d{i} = L->dat[{i}];")
+        elif a.kind == "reduce":
+            L.append(f"  A.r{i} = ({a.typ} *)L->partials[{i}];")
+    L.append("  memcpy(&A.b, blob, sizeof(A.b));")
+    L.append(f"  hipLaunchKernelGGL(opsg_{K}, dim3(L->grid[0], L->grid[1], L->grid[2]), "
+             f"dim3(::opsmi_gen::kWave, ::opsmi_gen::kRows), 0, (hipStream_t)stream, A);")
+    L.append("}")
+    return "\n".join(L), used, points
+
+
+def stub_code(loop, used, consts, points):
+    K = loop.kernel
+    n = len(loop.args)
+    params = ", ".join(f"ops_arg arg{i}" for i in range(n))
+    sig = f"void ops_par_loop_{K}(const char *name, ops_block block, int dim, int *range{', ' if n else ''}{params})"
+    L = [f"extern \"C\" OPSMI_GEN_STUB {sig} {{"]
+    if n:
+        L.append(f"  const ops_arg args[{n}] = {{{', '.join(f'arg{i}' for i in range(n))}}};")
+        types = ", ".join("OPSMI_GEN_NONE" if a.kind == "idx" else TYPES[a.typ][0] for a in loop.args)
+        L.append(f"  static const int types[{n}] = {{{types}}};")
+        counts = [-1 if p is None else len(p) for p in points]
+        flat = [v for p in points if p for pt in p for v in pt]
+        L.append(f"  static const int npts[{n}] = {{{', '.join(map(str, counts))}}};")
+        L.append(f"  static const int pts[{max(1, len(flat))}] = {{{', '.join(map(str, flat)) or '0'}}};")
+    else:
+        L.append("  const ops_arg *args = nullptr;")
+        L.append("  const int *types = nullptr, *npts = nullptr, *pts = nullptr;")
+    L.append(f"  opsb_{K} b;")
+    L.append("  memset(&b, 0, sizeof(b));")
+    for c in used:
+        d, t = consts[c]
+        L.append(f"  ops_mi_gen_const(\"{c}\", \"{t}\", {d}, &b.k.{c});")
+    for i, a in enumerate(loop.args):
+        if a.kind == "gbl":
+            L.append(f"  if (arg{i}.data && arg{i}.dim >= {a.dim} && arg{i}.elem_size == {TYPES[a.typ][1]})")
+            L.append(f"    memcpy(b.g{i}, arg{i}.data, sizeof(b.g{i}));")
+    L.append(f"  ops_mi_gen_par_loop(name, block, dim, range, args, {n}, types, npts, pts, &opsl_{K}, &b, sizeof(b));")
+    L.append("}")
+    return "\n".join(L), sig
+
+
+def translate(app, out_dir, inc_dirs=()):
+    inc_dirs = list(inc_dirs)
+    src = open(app).read()
+    m = mask(src)
+    base = os.path.splitext(os.path.basename(app))[0]
+    loops = find_loops(src, m, app)
+    incs = local_includes(app, src, m, inc_dirs)
+    files = [(app, src, m)]
+    for _, _, p, _ in incs:
+        s = open(p).read()
+        files.append((p, s, mask(s)))
+    consts = {}
+    for _, s, mm in files:
+        consts.update(find_consts(s, mm))
+    kernels, order = {}, []
+    for lp in loops:
+        if lp.kernel in kernels:
+            prev = kernels[lp.kernel][0]
+            if [(a.kind, a.typ, a.acc, a.dim, a.multi) for a in prev.args] != \
+               [(a.kind, a.typ, a.acc, a.dim, a.multi) for a in lp.args] or prev.ndim != lp.ndim:
+                raise TranslateError(f"kernel {lp.kernel} is called with different argument lists")
+            continue
+        kernels[lp.kernel] = (lp, find_kernel(lp.kernel, files))
+        order.append(lp.kernel)
+
+    # kernels file
+    H = [f"// Generated by ops_amd/translator/opsmi_translate.py from {os.path.basename(app)}; do not edit.",
+         "#include <string.h>", "", "#include \"ops_mi355x_device.h\"", "",
+         "#define OPSMI_GEN_STUB __attribute__((visibility(\"hidden\")))", "", "namespace {"]
+    stubs, sigs = [], []
+    for k in order:
+        lp, kern = kernels[k]
+        code, used, points = kernel_code(lp, kern, consts)
+        H.append(code)
+        s, sig = stub_code(lp, used, consts, points)
+        stubs.append(s)
+        sigs.append(sig)
+    H.append("}  // namespace")
+    H.append("")
+    H.extend(stubs)
+    os.makedirs(out_dir, exist_ok=True)
+    with open(os.path.join(out_dir, base + "_opsmi_kernels.hip"), "w") as f:
+        f.write("\n".join(H) + "\n")
+    guard = re.sub(r"\W", "_", base).upper() + "_OPSMI_KERNELS_H"
+    D = [f"// Generated by ops_amd/translator/opsmi_translate.py from {os.path.basename(app)}; do not edit.",
+         f"#ifndef {guard}", f"#define {guard}", "#include \"ops_mi355x.h\"", "#ifdef __cplusplus", "extern \"C\" {",
+         "#endif"]
+    for sig in sigs:
+        D.append(f"__attribute__((visibility(\"hidden\"))) {sig};")
+    D += ["#ifdef __cplusplus", "}", "#endif", "#endif"]
+    with open(os.path.join(out_dir, base + "_opsmi_kernels.h"), "w") as f:
+        f.write("\n".join(D) + "\n")
+
+    # rewritten application: edits from the back so offsets stay valid
+    edits = []
+    for lp in loops:
+        edits.append((lp.span[0], lp.span[1], f"ops_par_loop_{lp.kernel}("))
+    kernel_files = {kern.path for _, kern in kernels.values()}
+    for k in order:
+        kern = kernels[k][1]
+        if kern.path == app:
+            edits.append((kern.span[0], kern.span[1], f"/* kernel {k}: generated into {base}_opsmi_kernels.hip */"))
+    for a, b, p, name in incs:
+        if p in kernel_files:
+            edits.append((a, b, f"/* #include \"{name}\": kernels generated into {base}_opsmi_kernels.hip */"))
+    ops_inc = re.compile(r'#\s*include\s*[<"](ops_seq_v2\.h|ops_seq\.h|ops_lib_core\.h)[>"]')
+    first_ops = None
+    for mo in ops_inc.finditer(src):
+        if m[mo.start()] != "#":
+            continue
+        edits.append((mo.start(), mo.end(), "#include \"ops_mi355x.h\""))
+        first_ops = mo.end() if first_ops is None else first_ops
+    out = src
+    for a, b, t in sorted(edits, key=lambda e: -e[0]):
+        out = out[:a] + t + out[b:]
+    decl = f"#include \"{base}_opsmi_kernels.h\"\n"
+    if first_ops is not None:
+        # after the (rewritten) OPS include: find it again in the output
+        mo = re.search(r'#include "ops_mi355x\.h"\n?', out)
+        out = out[:mo.end()] + ("" if out[mo.end() - 1] == "\n" else "\n") + decl + out[mo.end():]
+    else:
+        out = "#include \"ops_mi355x.h\"\n" + decl + out
+    with open(os.path.join(out_dir, base + "_ops.cpp"), "w") as f:
+        f.write(out)
+    return order
+
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def build_shared(apps, out_so, gen_dir, inc_dirs=(), hipcc="/opt/rocm/bin/hipcc", jobs=8):
+    """Translates apps into gen_dir and links them with their generated
+    kernels into the shared object out_so (linked against ops_amd/libopsmi.so
+    with an rpath to it)."""
+    import subprocess
+    from concurrent.futures import ThreadPoolExecutor
+    inc = [os.path.join(ROOT, "include"), gen_dir] + [os.path.dirname(os.path.abspath(a)) for a in apps] + list(inc_dirs)
+    iflags = [f"-I{d}" for d in inc]
+    cmds, objs = [], []
+    for app in apps:
+        translate(app, gen_dir, inc_dirs)
+        base = os.path.splitext(os.path.basename(app))[0]
+        for src, obj, lang in ((f"{base}_opsmi_kernels.hip", f"{base}_k.o", ["-x", "hip"]),
+                               (f"{base}_ops.cpp", f"{base}_a.o", [])):
+            o = os.path.join(gen_dir, obj)
+            cmds.append([hipcc] + HIP_FLAGS + iflags + lang + ["-c", os.path.join(gen_dir, src), "-o", o])
+            objs.append(o)
+    with ThreadPoolExecutor(max(1, jobs)) as ex:
+        for r in ex.map(lambda c: subprocess.run(c, capture_output=True, text=True), cmds):
+            if r.returncode != 0:
+                raise TranslateError(f"compile failed: {' '.join(r.args)}\n{r.stderr}")
+    libdir = os.path.join(ROOT, "ops_amd")
+    rpath = os.path.relpath(libdir, os.path.dirname(os.path.abspath(out_so)))
+    link = [hipcc] + HIP_FLAGS + ["-shared"] + objs + [f"-L{libdir}", "-lopsmi",
+                                                       f"-Wl,-rpath,$ORIGIN/{rpath}", "-o", out_so]
+    r = subprocess.run(link, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise TranslateError(f"link failed: {' '.join(link)}\n{r.stderr}")
+    return out_so
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("apps", nargs="+")
+    ap.add_argument("-o", "--out", default=".")
+    ap.add_argument("-I", dest="inc", action="append", default=[])
+    ap.add_argument("--shared", metavar="LIB.so", help="also compile and link everything into LIB.so")
+    a = ap.parse_args(argv)
+    try:
+        if a.shared:
+            build_shared(a.apps, a.shared, a.out, a.inc)
+            print(f"{len(a.apps)} applications -> {a.shared}")
+        else:
+            for app in a.apps:
+                ks = translate(app, a.out, a.inc)
+                print(f"{app}: {len(ks)} kernels -> {a.out}")
+    except TranslateError as e:
+        print(f"opsmi_translate: error: {e}", file=sys.stderr)
+        return 1
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

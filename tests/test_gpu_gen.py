@@ -1,0 +1,145 @@
+"""Translator-generated loops on the GPU.
+
+tests/genapps/*.cpp are applications written in the OPS dev API
+(ops_par_loop(kernel, ...) with ACC<T> kernels); __graft_entry__.build()
+runs them through ops_amd/translator/opsmi_translate.py into
+tests/genapps/libgenapps.so.  Their results must match the oracle (Laplace:
+bit-exact), closed forms computed with numpy in the kernels' own operation
+order (fields), and the hand-written entry points of the same kernels
+(TeaLeaf calc_w: field bit-exact, p.w within 1e-12 relative)."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+GEN_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "genapps", "libgenapps.so")
+vp, ci, cd = C.c_void_p, C.c_int, C.c_double
+dp = C.POINTER(C.c_double)
+
+
+@pytest.fixture(scope="module")
+def gen(gpu_ops):
+    if not os.path.exists(GEN_LIB):
+        raise RuntimeError(f"{GEN_LIB} is not built; run __graft_entry__.build() first")
+    L = C.CDLL(GEN_LIB)
+    L.gen_laplace.argtypes = [vp, vp, vp, ci, ci, ci, dp]
+    L.gen_laplace_bad_stencil.argtypes = [vp, vp, vp, ci, ci]
+    L.gen_fields.argtypes = [vp, vp, vp, vp, ci, ci, cd, cd, cd, ci, dp, dp]
+    L.gen_heat_step.argtypes = [vp] * 7 + [ci, ci, ci, cd, cd, cd, cd, dp]
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(dp)
+
+
+@pytest.mark.parametrize("n", [(64, 48), (130, 70), (257, 33)])
+def test_generated_laplace_bit_exact(gpu_ops, gen, n):
+    ops = gpu_ops
+    nx, ny = n
+    it = 25
+    init = O.laplace_init(nx, ny, 0)
+    blk = ops.ops_decl_block(2, "genlap")
+    A = ops.ops_decl_dat(blk, [nx, ny], name="A")
+    An = ops.ops_decl_dat(blk, [nx, ny], name="Anew")
+    ops.set_ext(A, init)
+    ops.set_ext(An, init)
+    hist = np.zeros(it)
+    ops.ops_mi_reset_stats()
+    gen.gen_laplace(blk.ptr, A.ptr, An.ptr, nx, ny, it, _p(hist))
+    ops._check()
+    ref_A, ref_h = O.laplace_run(init, it)
+    assert np.array_equal(hist, ref_h)
+    assert np.array_equal(ops.ops_dat_fetch_data(A), O.interior(ref_A))
+    st = ops.ops_mi_loop_stats("gen apply_stencil")
+    assert st and st[0] == it
+
+
+def test_generated_loop_checks_its_stencil(gpu_ops, gen):
+    ops = gpu_ops
+    blk = ops.ops_decl_block(2, "genbad")
+    A = ops.ops_decl_dat(blk, [40, 30], name="A")
+    An = ops.ops_decl_dat(blk, [40, 30], name="An")
+    gen.gen_laplace_bad_stencil(blk.ptr, A.ptr, An.ptr, 40, 30)
+    with pytest.raises(ops.OPSError, match="lacks point"):
+        ops._check()
+    # element type against the kernel's declaration (checked before the stencil)
+    F = ops.ops_decl_dat(blk, [40, 30], dtype=np.float32, name="F")
+    gen.gen_laplace_bad_stencil(blk.ptr, F.ptr, An.ptr, 40, 30)
+    with pytest.raises(ops.OPSError, match="declares double"):
+        ops._check()
+
+
+@pytest.mark.parametrize("n", [(37, 29), (200, 61)])
+def test_generated_fields_all_argument_kinds(gpu_ops, gen, n):
+    ops = gpu_ops
+    nx, ny = n
+    blk = ops.ops_decl_block(2, "genfields")
+    v = ops.ops_decl_dat(blk, [nx, ny], name="v", dim=2)
+    m = ops.ops_decl_dat(blk, [nx, ny], dtype=np.int32, name="m")
+    w = ops.ops_decl_dat(blk, [nx, ny], name="w")
+    sx, sy, sh, mu = 0.5, 0.25, 1.5, 3
+    wts = np.array([0.5, -0.25, 2.0])
+    out = np.zeros(5)
+    gen.gen_fields(blk.ptr, v.ptr, m.ptr, w.ptr, nx, ny, sx, sy, sh, mu, _p(wts), _p(out))
+    ops._check()
+    # closed forms over the whole array (halo included), index [y, x]
+    X, Y = np.meshgrid(np.arange(-1, nx + 1), np.arange(-1, ny + 1))
+    v0 = sx * X + sh
+    v1 = sy * Y - 0.5 * X
+    mm = (mu * X - Y).astype(np.int32)
+    V = ops.get_whole(v)
+    assert np.array_equal(V[..., 0], v0) and np.array_equal(V[..., 1], v1)
+    assert np.array_equal(ops.get_whole(m), mm)
+    i = (slice(1, -1), slice(1, -1))
+    wref = (wts[0] * v0[1:-1, 2:] + wts[1] * v1[:-2, 1:-1]) + wts[2] * (v0[2:, 1:-1] - v1[1:-1, :-2])
+    W = ops.ops_dat_fetch_data(w)
+    assert np.array_equal(W, wref)
+    assert out[0] == v0[i].sum()  # multiples of 1/4: exact in any order
+    assert abs(out[1] - np.sum(wref * wref)) <= 1e-12 * np.sum(wref * wref)
+    assert out[2] == wref.min() and out[3] == v1[i].max()
+    assert out[4] == int(mm[i].sum())
+
+
+@pytest.mark.parametrize("n", [(40, 33, 20), (130, 17, 9)])
+def test_generated_heat_matches_hand_written(gpu_ops, gen, n):
+    ops = gpu_ops
+    nx, ny, nz = n
+    blk = ops.ops_decl_block(3, "genheat")
+    names = ["w", "Kx", "Ky", "Kz", "p", "r", "w2"]
+    D = {k: ops.ops_decl_dat(blk, [nx, ny, nz], name=k) for k in names}
+    ext = (nz + 2, ny + 2, nx + 2)
+    for s, k in enumerate(["Kx", "Ky", "Kz", "p", "r"]):
+        ops.set_whole(D[k], O.u01(91, s, int(np.prod(ext))).reshape(ext) + (1.0 if k[0] == "K" else 0.0))
+    rx, ry, rz, alpha = 0.3, 0.7, 1.1, 0.37
+    r0 = ops.ops_dat_fetch_data(D["r"])
+    out = np.zeros(2)
+    gen.gen_heat_step(blk.ptr, *[D[k].ptr for k in ["w", "Kx", "Ky", "Kz", "p", "r"]], nx, ny, nz, rx, ry, rz,
+                      alpha, _p(out))
+    ops._check()
+    # the hand-written entry point of the same kernel
+    R, W = ops.OPS_READ, ops.OPS_WRITE
+    s0 = ops.ops_decl_stencil(3, 1, [0, 0, 0], "g0")
+    s7 = ops.ops_decl_stencil(3, 7, [0, 0, 0, 1, 0, 0, -1, 0, 0, 0, 1, 0, 0, -1, 0, 0, 0, 1, 0, 0, -1], "g7")
+    sxs = ops.ops_decl_stencil(3, 2, [0, 0, 0, 1, 0, 0], "gx")
+    sys_ = ops.ops_decl_stencil(3, 2, [0, 0, 0, 0, 1, 0], "gy")
+    szs = ops.ops_decl_stencil(3, 2, [0, 0, 0, 0, 0, 1], "gz")
+    h = ops.ops_decl_reduction_handle(8, "double", "pw_hw")
+    ops.ops_par_loop("tea_leaf_cg_calc_w_reduce_kernel_3d", "hw calc_w", blk, 3, [0, nx, 0, ny, 0, nz],
+                     ops.ops_arg_dat(D["w2"], 1, s0, "double", W), ops.ops_arg_dat(D["Kx"], 1, sxs, "double", R),
+                     ops.ops_arg_dat(D["Ky"], 1, sys_, "double", R), ops.ops_arg_dat(D["Kz"], 1, szs, "double", R),
+                     ops.ops_arg_dat(D["p"], 1, s7, "double", R), ops.ops_arg_gbl(rx, 1, "double"),
+                     ops.ops_arg_gbl(ry, 1, "double"), ops.ops_arg_gbl(rz, 1, "double"),
+                     ops.ops_arg_reduce(h, 1, "double", ops.OPS_INC))
+    pw = ops.ops_reduction_result(h)
+    w_gen = ops.ops_dat_fetch_data(D["w"])
+    assert np.array_equal(w_gen, ops.ops_dat_fetch_data(D["w2"]))
+    assert abs(out[0] - pw) <= 1e-12 * abs(pw)
+    r1 = ops.ops_dat_fetch_data(D["r"])
+    assert np.array_equal(r1, r0 - alpha * w_gen)
+    assert abs(out[1] - np.sum(r1 * r1)) <= 1e-12 * np.sum(r1 * r1)

@@ -1,0 +1,91 @@
+"""The source-to-source translator (CPU only): what it extracts from an
+application and its kernels, what it writes, and what it refuses."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from ops_amd.translator import opsmi_translate as T
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+APPS = [os.path.join(HERE, "genapps", a) for a in ("laplace_gen.cpp", "fields_gen.cpp", "heat3d_gen.cpp")]
+
+
+def test_mask_blanks_comments_and_strings():
+    src = 'a(/* x(y) */ "b(c", \'(\') // d(\ne'
+    m = T.mask(src)
+    assert len(m) == len(src)
+    assert m.count("(") == 1 and m.count(")") == 1
+    assert m.endswith("\ne") and "d(" not in m and "x" not in m
+
+
+def test_translates_dev_api_applications(tmp_path):
+    out = str(tmp_path)
+    ks = [T.translate(a, out) for a in APPS]
+    assert ks == [["apply_stencil", "copy"], ["fill_fields", "mix_fields", "summary"], ["heat_calc_w", "heat_ur_r"]]
+    lap = open(os.path.join(out, "laplace_gen_ops.cpp")).read()
+    assert not re.search(r"\bops_par_loop\s*\(", T.mask(lap))
+    assert lap.count("ops_par_loop_apply_stencil(") == 2 and "ops_par_loop_copy(" in lap
+    assert '#include "ops_mi355x.h"\n#include "laplace_gen_opsmi_kernels.h"' in lap
+    assert '/* #include "laplace_gen_kernels.h"' in lap and '\n#include "laplace_gen_kernels.h"' not in lap
+    fields = open(os.path.join(out, "fields_gen_ops.cpp")).read()
+    assert "void summary(" not in fields and "/* kernel summary: generated" in fields
+    hip = open(os.path.join(out, "fields_gen_opsmi_kernels.hip")).read()
+    # constants the body names become members; the two-component dat is MULTI
+    assert re.search(r"struct opsk_mix_fields \{\s*double weights\[3\];", hip)
+    assert re.search(r"struct opsk_fill_fields \{\s*int mult;\s*double shift;", hip)
+    assert "::opsmi_gen::Acc<double, 2, true> &v" in hip and "::opsmi_gen::Acc<int, 2, false> &m" in hip
+    assert "store_partials<int, OPS_INC, 1>" in hip and "store_partials<double, OPS_INC, 2>" in hip
+    decl = open(os.path.join(out, "heat3d_gen_opsmi_kernels.h")).read()
+    assert 'visibility("hidden")' in decl and "ops_par_loop_heat_calc_w(" in decl
+
+
+def test_point_lists_follow_the_kernel_body():
+    body = "{ w(0,0) = v(0, 1, 0) + v(1,0,-1) - v(0,0,1); }"
+    assert T.accessed_points(body, "v", 2) == [(0, 0, 1, 0), (0, 1, 0, 0), (1, 0, -1, 0)]
+    assert T.accessed_points(body, "w", 2) == [(-1, 0, 0, 0)]
+    assert T.accessed_points("{ p(0, 0, -1) = p(1, 0, 0); }", "p", 3) == [(-1, 0, 0, -1), (-1, 1, 0, 0)]
+    # a computed index or an atomic combine: no list (the stencil is not checked)
+    assert T.accessed_points("{ a(i, 0) = 1; }", "a", 2) is None
+    assert T.accessed_points("{ a.combine_max(0, 0, 1.0); }", "a", 2) is None
+    # comments do not count
+    assert T.accessed_points("{ a(0, 0) = 1; // a(5, 5)\n }", "a", 2) == [(-1, 0, 0, 0)]
+
+
+def _app(tmp_path, loop, kernel="void k(ACC<double> &a) { a(0, 0) = 1.0; }"):
+    p = tmp_path / "app.cpp"
+    p.write_text(f'#include "ops_seq_v2.h"\n{kernel}\nvoid run(ops_block b, ops_dat d, ops_stencil s, int *r) {{\n'
+                 f'  {loop};\n}}\n')
+    return str(p)
+
+
+@pytest.mark.parametrize("loop,kernel,msg", [
+    ('ops_par_loop(k, "k", b, nd, r, ops_arg_dat(d, 1, s, "double", OPS_WRITE))', None, "integer literal"),
+    ('ops_par_loop(k, "k", b, 2, r, ops_arg_dat(d, 1, s, dtype, OPS_WRITE))', None, "string literal"),
+    ('ops_par_loop(k, "k", b, 2, r, ops_arg_dat(d, 1, s, "double", OPS_WRITE))', "void other(ACC<double> &a) {}",
+     "not defined"),
+    ('ops_par_loop(k, "k", b, 2, r, ops_arg_dat(d, 1, s, "double", OPS_WRITE), ops_arg_idx())', None,
+     "takes 1 parameters"),
+    ('ops_par_loop(k, "k", b, 2, r, ops_arg_dat(d, 1, s, "double", OPS_WRITE))', "void k(double *a) { *a = 1; }",
+     "ACC<T> reference"),
+    ('ops_par_loop(k, "k", b, 2, r, ops_arg_gbl(&x, n, "double", OPS_READ))', "void k(const double *x) {}",
+     "integer literal"),
+    ('ops_par_loop(k, "k", b, 2, r, ops_arg_dat(d, 1, s, "half", OPS_WRITE))', None, "only double, float, int"),
+])
+def test_refuses_what_it_cannot_translate(tmp_path, loop, kernel, msg):
+    app = _app(tmp_path, loop) if kernel is None else _app(tmp_path, loop, kernel)
+    with pytest.raises(T.TranslateError, match=msg):
+        T.translate(app, str(tmp_path / "out"))
+
+
+def test_generated_library_built():
+    lib = os.path.join(HERE, "genapps", "libgenapps.so")
+    assert os.path.exists(lib), "run __graft_entry__.build() first"
+    dyn = subprocess.run(["nm", "-D", lib], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r" T (\w+)", dyn))
+    assert {"gen_laplace", "gen_fields", "gen_heat_step", "gen_laplace_bad_stencil"} <= exported
+    # the generated stubs stay inside the library (no clash with the
+    # hand-written ops_par_loop_apply_stencil of libopsmi.so)
+    assert not any(n.startswith("ops_par_loop_") for n in exported)
+    assert re.search(r" U ops_mi_gen_par_loop\b", dyn)
