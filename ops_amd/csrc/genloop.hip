@@ -8,6 +8,7 @@
 // partial buffer per reduction argument, and the library then folds each
 // argument's partials into its handle in a fixed order (the same two-stage
 // fold as the hand-written loops).
+#include <stdlib.h>
 #include <string.h>
 
 #include "engine.h"
@@ -34,6 +35,16 @@ void finish(DType dt, const RedCtl &c, int op, int dim, unsigned parts, hipStrea
   if (dt == DT_F64) finish_op<double>(c, op, dim, parts, s);
   else if (dt == DT_F32) finish_op<float>(c, op, dim, parts, s);
   else finish_op<int>(c, op, dim, parts, s);
+}
+
+// z planes per workgroup of a generated 3D loop (OPSMI_GEN_ZCHUNK)
+int gen_zchunk() {
+  static int v = [] {
+    const char *e = getenv("OPSMI_GEN_ZCHUNK");
+    const int n = e ? atoi(e) : 1;
+    return n >= 1 && n <= 256 ? n : 1;
+  }();
+  return v;
 }
 
 struct GenRed {
@@ -70,7 +81,7 @@ void ops_mi_gen_par_loop(const char *name, ops_block block, int dim, const int *
   } else {
     // the hand-written engine's tiling: 3D one plane of 64 x 4 per
     // workgroup, 2D four groups of four rows
-    const Tiling t = make_tiling(r, r.z1 - r.z0 > 1 ? 1 : 4);
+    const Tiling t = make_tiling(r, r.z1 - r.z0 > 1 ? gen_zchunk() : 4);
     const unsigned parts = t.blocks();
     std::vector<GenRed> reds;
     size_t off = 0;
