@@ -225,26 +225,37 @@ __global__ __launch_bounds__(64) void k_thomas_x(const T *__restrict__ a, const 
     }
     __syncthreads();
   }
-  // backward: chunks from the last one down; x of a chunk goes to LA
+  // backward: chunks from the last one down; x of a chunk goes to LA.  The
+  // c', d' of the next chunk are loaded while this one is solved (two
+  // register sets used in turn).
   T x = dprev;
   const long last0 = (n - 1) / kChunk * kChunk;
-  for (long i0 = last0; i0 >= 0; i0 -= kChunk) {
-    if (live) {
-      T cv[kChunk], dv[kChunk];
+  T cv[2][kChunk], dv[2][kChunk];
+  auto load_back = [&](int q, long i0) {
+    if (!live || i0 < 0) return;
 #pragma unroll
-      for (int k = 0; k < kChunk; k++) {
-        const long i = i0 + k;
-        if (i < n - 1) {
-          cv[k] = cs[i * L + id];
-          dv[k] = ds[i * L + id];
-        }
+    for (int k = 0; k < kChunk; k++) {
+      const long i = i0 + k;
+      if (i < n - 1) {
+        cv[q][k] = cs[i * L + id];
+        dv[q][k] = ds[i * L + id];
       }
+    }
+  };
+  load_back(0, last0);
+  int q = 0;
+  for (long i0 = last0; i0 >= 0; i0 -= kChunk, q ^= 1) {
+    if (q == 0) load_back(1, i0 - kChunk);
+    else load_back(0, i0 - kChunk);
+    if (live) {
 #pragma unroll
       for (int k = kChunk - 1; k >= 0; k--) {
         const long i = i0 + k;
+        const T cq = q == 0 ? cv[0][k] : cv[1][k];
+        const T dq = q == 0 ? dv[0][k] : dv[1][k];
         if (i == n - 1) LA[k][lane] = x;
         else if (i < n - 1) {
-          x = dv[k] - cv[k] * x;
+          x = dq - cq * x;
           LA[k][lane] = x;
         }
       }
