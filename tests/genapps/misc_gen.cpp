@@ -1,0 +1,96 @@
+// More dev-API loops for the translator: a 1D block with float dats and a
+// float MAX reduction, a 3D three-component dat with a three-component INC
+// and an int MIN reduction, an optional dat (ops_arg_dat_opt), and a 2D
+// fill whose result the caller reads through a hand-written loop (the two
+// kinds of loop share one queue).
+#include "ops_seq_v2.h"
+
+void ramp1d(ACC<float> &f, const int *idx, const float *c) {
+  f(0) = c[0] * idx[0] + c[1];
+}
+
+void smooth1d(const ACC<float> &f, ACC<float> &g, float *mx) {
+  g(0) = 0.25f * f(-1) + 0.5f * f(0) + 0.25f * f(1);
+  *mx = fmaxf(*mx, g(0));
+}
+
+void fill_vec3(ACC<double> &v, const int *idx) {
+  v(0, 0, 0, 0) = idx[0];
+  v(1, 0, 0, 0) = 2.0 * idx[1];
+  v(2, 0, 0, 0) = 3.0 * idx[2] + 1.0;
+}
+
+void div_vec3(const ACC<double> &v, ACC<double> &d, double *sums, int *imin) {
+  d(0, 0, 0) = (v(0, 1, 0, 0) - v(0, -1, 0, 0)) + (v(1, 0, 1, 0) - v(1, 0, -1, 0)) + (v(2, 0, 0, 1) - v(2, 0, 0, -1));
+  sums[0] = sums[0] + v(0, 0, 0, 0);
+  sums[1] = sums[1] + v(1, 0, 0, 0);
+  sums[2] = sums[2] + v(2, 0, 0, 0);
+  const int q = (int)d(0, 0, 0) - (int)v(0, 0, 0, 0);
+  *imin = q < *imin ? q : *imin;
+}
+
+void opt_add(ACC<double> &d, const ACC<double> &e, const int *use) {
+  if (*use) d(0, 0, 0) = d(0, 0, 0) + e(0, 0, 0);
+  else d(0, 0, 0) = 2.0 * d(0, 0, 0);
+}
+
+void fill2d(ACC<double> &a, const int *idx, const double *val) {
+  a(0, 0) = *val + idx[0] - 0.5 * idx[1];
+}
+
+// f, g: 1D float dats of n points with a one-point halo
+extern "C" int gen_ramp1d(ops_block block, ops_dat f, ops_dat g, int n, float c0, float c1, float *mx_out) {
+  int s0[] = {0};
+  ops_stencil S0 = ops_decl_stencil(1, 1, s0, "m0");
+  int s3[] = {0, -1, 1};
+  ops_stencil S3 = ops_decl_stencil(1, 3, s3, "m3");
+  ops_reduction h = ops_decl_reduction_handle(sizeof(float), "float", "mx");
+  int whole[] = {-1, n + 1};
+  int inner[] = {0, n};
+  float c[2] = {c0, c1};
+  ops_par_loop(ramp1d, "ramp1d", block, 1, whole, ops_arg_dat(f, 1, S0, "float", OPS_WRITE), ops_arg_idx(),
+               ops_arg_gbl(c, 2, "float", OPS_READ));
+  ops_par_loop(smooth1d, "smooth1d", block, 1, inner, ops_arg_dat(f, 1, S3, "float", OPS_READ),
+               ops_arg_dat(g, 1, S0, "float", OPS_WRITE), ops_arg_reduce(h, 1, "float", OPS_MAX));
+  ops_reduction_result(h, mx_out);
+  return 0;
+}
+
+// v: 3D three-component dat, d: one component; both with a one-point halo
+extern "C" int gen_vec3(ops_block block, ops_dat v, ops_dat d, int nx, int ny, int nz, double *sums, int *imin) {
+  int s0[] = {0, 0, 0};
+  ops_stencil S0 = ops_decl_stencil(3, 1, s0, "v0");
+  int s7[] = {0, 0, 0, 1, 0, 0, -1, 0, 0, 0, 1, 0, 0, -1, 0, 0, 0, 1, 0, 0, -1};
+  ops_stencil S7 = ops_decl_stencil(3, 7, s7, "v7");
+  ops_reduction hs = ops_decl_reduction_handle(3 * sizeof(double), "double", "sums3");
+  ops_reduction hm = ops_decl_reduction_handle(sizeof(int), "int", "imin");
+  int whole[] = {-1, nx + 1, -1, ny + 1, -1, nz + 1};
+  int inner[] = {0, nx, 0, ny, 0, nz};
+  ops_par_loop(fill_vec3, "fill_vec3", block, 3, whole, ops_arg_dat(v, 3, S0, "double", OPS_WRITE), ops_arg_idx());
+  ops_par_loop(div_vec3, "div_vec3", block, 3, inner, ops_arg_dat(v, 3, S7, "double", OPS_READ),
+               ops_arg_dat(d, 1, S0, "double", OPS_WRITE), ops_arg_reduce(hs, 3, "double", OPS_INC),
+               ops_arg_reduce(hm, 1, "int", OPS_MIN));
+  ops_reduction_result(hs, sums);
+  ops_reduction_result(hm, imin);
+  return 0;
+}
+
+// d += e where use != 0 (e passed through ops_arg_dat_opt), else d *= 2
+extern "C" int gen_opt_add(ops_block block, ops_dat d, ops_dat e, int nx, int ny, int nz, int use) {
+  int s0[] = {0, 0, 0};
+  ops_stencil S0 = ops_decl_stencil(3, 1, s0, "o0");
+  int inner[] = {0, nx, 0, ny, 0, nz};
+  ops_par_loop(opt_add, "opt_add", block, 3, inner, ops_arg_dat(d, 1, S0, "double", OPS_RW),
+               ops_arg_dat_opt(e, 1, S0, "double", OPS_READ, use), ops_arg_gbl(&use, 1, "int", OPS_READ));
+  return 0;
+}
+
+// queues a fill of a over the interior and returns without a result
+extern "C" int gen_fill2d(ops_block block, ops_dat a, int nx, int ny, double val) {
+  int s0[] = {0, 0};
+  ops_stencil S0 = ops_decl_stencil(2, 1, s0, "f0");
+  int inner[] = {0, nx, 0, ny};
+  ops_par_loop(fill2d, "fill2d", block, 2, inner, ops_arg_dat(a, 1, S0, "double", OPS_WRITE), ops_arg_idx(),
+               ops_arg_gbl(&val, 1, "double", OPS_READ));
+  return 0;
+}

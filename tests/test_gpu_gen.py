@@ -31,6 +31,10 @@ def gen(gpu_ops):
     L.gen_laplace_bad_stencil.argtypes = [vp, vp, vp, ci, ci]
     L.gen_fields.argtypes = [vp, vp, vp, vp, ci, ci, cd, cd, cd, ci, dp, dp]
     L.gen_heat_step.argtypes = [vp] * 7 + [ci, ci, ci, cd, cd, cd, cd, dp]
+    L.gen_ramp1d.argtypes = [vp, vp, vp, ci, C.c_float, C.c_float, C.POINTER(C.c_float)]
+    L.gen_vec3.argtypes = [vp, vp, vp, ci, ci, ci, dp, C.POINTER(C.c_int)]
+    L.gen_opt_add.argtypes = [vp, vp, vp, ci, ci, ci, ci]
+    L.gen_fill2d.argtypes = [vp, vp, ci, ci, cd]
     return L
 
 
@@ -143,3 +147,97 @@ def test_generated_heat_matches_hand_written(gpu_ops, gen, n):
     r1 = ops.ops_dat_fetch_data(D["r"])
     assert np.array_equal(r1, r0 - alpha * w_gen)
     assert abs(out[1] - np.sum(r1 * r1)) <= 1e-12 * np.sum(r1 * r1)
+
+
+@pytest.mark.parametrize("n", [2, 63, 1000])  # (a size of 1 declares an OPS edge dat, without halo)
+def test_generated_1d_float_loops(gpu_ops, gen, n):
+    ops = gpu_ops
+    blk = ops.ops_decl_block(1, "gen1d")
+    f = ops.ops_decl_dat(blk, [n], d_m=[-1], d_p=[1], dtype=np.float32, name="f")
+    g = ops.ops_decl_dat(blk, [n], d_m=[-1], d_p=[1], dtype=np.float32, name="g")
+    c0, c1 = np.float32(0.3), np.float32(-1.7)
+    mx = C.c_float(0)
+    gen.gen_ramp1d(blk.ptr, f.ptr, g.ptr, n, c0, c1, C.byref(mx))
+    ops._check()
+    X = np.arange(-1, n + 1).astype(np.float32)
+    fr = c0 * X + c1
+    assert np.array_equal(ops.get_whole(f), fr)
+    gr = (np.float32(0.25) * fr[:-2] + np.float32(0.5) * fr[1:-1]) + np.float32(0.25) * fr[2:]
+    assert np.array_equal(ops.ops_dat_fetch_data(g), gr)
+    assert np.float32(mx.value) == gr.max()
+
+
+@pytest.mark.parametrize("n", [(9, 7, 5), (70, 12, 6)])
+def test_generated_3d_three_components(gpu_ops, gen, n):
+    ops = gpu_ops
+    nx, ny, nz = n
+    blk = ops.ops_decl_block(3, "genv3")
+    v = ops.ops_decl_dat(blk, [nx, ny, nz], name="v", dim=3)
+    d = ops.ops_decl_dat(blk, [nx, ny, nz], name="d")
+    sums = np.zeros(3)
+    imin = C.c_int(0)
+    gen.gen_vec3(blk.ptr, v.ptr, d.ptr, nx, ny, nz, _p(sums), C.byref(imin))
+    ops._check()
+    Z, Y, X = np.meshgrid(np.arange(-1, nz + 1), np.arange(-1, ny + 1), np.arange(-1, nx + 1), indexing="ij")
+    V = ops.get_whole(v)
+    assert np.array_equal(V[..., 0], X.astype(float))
+    assert np.array_equal(V[..., 1], 2.0 * Y) and np.array_equal(V[..., 2], 3.0 * Z + 1.0)
+    assert np.all(ops.ops_dat_fetch_data(d) == 12.0)
+    i = (slice(1, -1),) * 3
+    assert list(sums) == [X[i].sum(), (2.0 * Y[i]).sum(), (3.0 * Z[i] + 1.0).sum()]
+    assert imin.value == 12 - (nx - 1)
+
+
+def test_generated_optional_dat(gpu_ops, gen):
+    ops = gpu_ops
+    nx, ny, nz = 20, 9, 4
+    blk = ops.ops_decl_block(3, "genopt")
+    d = ops.ops_decl_dat(blk, [nx, ny, nz], name="d")
+    e = ops.ops_decl_dat(blk, [nx, ny, nz], name="e")
+    d0 = O.u01(5, 1, nx * ny * nz).reshape(nz, ny, nx)
+    e0 = O.u01(5, 2, nx * ny * nz).reshape(nz, ny, nx)
+    ops.ops_dat_set_data(d, d0)
+    ops.ops_dat_set_data(e, e0)
+    gen.gen_opt_add(blk.ptr, d.ptr, e.ptr, nx, ny, nz, 1)
+    ops._check()
+    assert np.array_equal(ops.ops_dat_fetch_data(d), d0 + e0)
+    gen.gen_opt_add(blk.ptr, d.ptr, e.ptr, nx, ny, nz, 0)
+    ops._check()
+    assert np.array_equal(ops.ops_dat_fetch_data(d), 2.0 * (d0 + e0))
+
+
+def test_generated_and_hand_written_loops_share_the_queue(gpu_ops, gen):
+    """A generated fill and hand-written loops in one lazy queue: a result
+    that does not need the fill is right, a result that reads the filled dat
+    sees it, and a later hand-written write to it and a later generated fill
+    keep program order."""
+    ops = gpu_ops
+    nx, ny = 50, 21
+    blk = ops.ops_decl_block(2, "genq")
+    a = ops.ops_decl_dat(blk, [nx, ny], name="a")
+    b = ops.ops_decl_dat(blk, [nx, ny], name="b")
+    ops.ops_dat_set_data(b, np.ones((ny, nx)))
+    s00 = ops.ops_decl_stencil(2, 1, [0, 0], "q00")
+    rng = [0, nx, 0, ny]
+    R, W = ops.OPS_READ, ops.OPS_WRITE
+
+    def total(d, tag):
+        h = ops.ops_decl_reduction_handle(8, "double", tag)
+        ops.ops_par_loop("red_sum", tag, blk, 2, rng, ops.ops_arg_dat(d, 1, s00, "double", R),
+                         ops.ops_arg_reduce(h, 1, "double", ops.OPS_INC))
+        return ops.ops_reduction_result(h)
+
+    Y, X = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    gen.gen_fill2d(blk.ptr, a.ptr, nx, ny, 1.25)
+    ops._check()
+    assert total(b, "sb") == nx * ny
+    assert total(a, "sa") == ((1.25 + X) - 0.5 * Y).sum()  # quarter multiples: exact
+    # generated fill, then hand-written copy over it, then a generated fill of b
+    gen.gen_fill2d(blk.ptr, a.ptr, nx, ny, 7.0)
+    ops.ops_par_loop("copy", "copy b to a", blk, 2, rng, ops.ops_arg_dat(a, 1, s00, "double", W),
+                     ops.ops_arg_dat(b, 1, s00, "double", R))
+    gen.gen_fill2d(blk.ptr, b.ptr, nx, ny, -2.0)
+    ops._check()
+    assert total(a, "sa2") == nx * ny
+    assert np.array_equal(ops.ops_dat_fetch_data(b), (-2.0 + X) - 0.5 * Y)
+    assert ops.ops_mi_loop_stats("fill2d")[0] == 3
