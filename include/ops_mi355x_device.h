@@ -1,11 +1,11 @@
 // Device side of translator-generated loops (MI355X OPS backend).
 //
-// A generated kernel runs a 64 x 4 workgroup per tile, one grid point per
-// thread with x along the lanes (each wave reads contiguous rows); tiles are
+// A generated kernel runs a 64 x 4 workgroup per tile, OPSMI_GEN_XP grid
+// points per thread along x (each wave reads contiguous rows); tiles are
 // renumbered so each XCD works through a contiguous share of them (its L2
-// then holds the rows its stencils share).  3D loops: a tile is 64 x 4
-// points over `chunk` z planes; 2D/1D loops: 64 points over `chunk` groups of
-// 4 rows.  Reductions keep one accumulator per thread and component,
+// then holds the rows its stencils share).  3D loops: a tile is
+// (64 OPSMI_GEN_XP) x 4 points over `chunk` z planes; 2D/1D loops: 64
+// OPSMI_GEN_XP points over `chunk` groups of 4 rows.  Reductions keep one accumulator per thread and component,
 // initialised as the reference's generated code does (0, +-DBL_MAX,
 // +-FLT_MAX, +-INT_MAX: ops_translator/resources/templates/cpp/cuda/
 // loop_host.cpp.j2:116-127, ops/c/include/ops_internal2.h:68-80), fold
@@ -66,6 +66,11 @@ class Acc {
   __device__ __forceinline__ Acc(const opsmi_gen_dat &d, int x, int y, int z)
       : md_(d.md), sy_(d.sy), sz_(d.sz) {
     p_ = (T *)d.base + pt(x, y, z);
+  }
+  // base: the dat's (restrict-qualified) kernel argument, d.base's value
+  __device__ __forceinline__ Acc(const T *base, const opsmi_gen_dat &d, int x, int y, int z)
+      : md_(d.md), sy_(d.sy), sz_(d.sz) {
+    p_ = const_cast<T *>(base) + pt(x, y, z);
   }
   // NDIM indices: a point; NDIM + 1: component, then the point
   template <class... I>
@@ -178,25 +183,34 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
   return k * q8 + (k < r8 ? k : r8) + (b >> 3);
 }
 
-// calls f(x, y, z) for this thread's points
+// calls f(x, y, z) for this thread's points: OPSMI_GEN_XP points along x,
+// 64 apart, per row (a loop writes a dat only at the point itself, so the
+// points are independent and, with restrict-qualified dat arguments, the
+// compiler overlaps their loads)
 template <class F>
 __device__ __forceinline__ void for_each_point(const Loop &L, F &&f) {
+  constexpr int XP = OPSMI_GEN_XP;
   const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
   const unsigned l = xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nb);
   const unsigned tx = l % gridDim.x, rest = l / gridDim.x;
   const unsigned ty = rest % gridDim.y, tz = rest / gridDim.y;
-  const int x = L.x0 + (int)tx * kWave + (int)threadIdx.x;
+  const int x = L.x0 + (int)tx * kWave * XP + (int)threadIdx.x;
   if (x >= L.x1) return;
+  auto row = [&](int y, int z) {
+#pragma unroll
+    for (int k = 0; k < XP; k++)
+      if (x + k * kWave < L.x1) f(x + k * kWave, y, z);
+  };
   if (L.three_d) {
     const int y = L.y0 + (int)ty * kRows + (int)threadIdx.y;
     if (y >= L.y1) return;
     const int z0 = L.z0 + (int)tz * L.chunk;
     const int z1 = min(z0 + L.chunk, L.z1);
-    for (int z = z0; z < z1; z++) f(x, y, z);
+    for (int z = z0; z < z1; z++) row(y, z);
   } else {
     const int y0 = L.y0 + (int)ty * kRows * L.chunk;
     const int y1 = min(y0 + kRows * L.chunk, L.y1);
-    for (int y = y0 + (int)threadIdx.y; y < y1; y += kRows) f(x, y, L.z0);
+    for (int y = y0 + (int)threadIdx.y; y < y1; y += kRows) row(y, L.z0);
   }
 }
 

@@ -30,6 +30,9 @@ extern "C" {
 #endif
 
 #define OPSMI_GEN_MAX_ARGS 32
+/* grid points per lane along x: a workgroup row covers 64 * OPSMI_GEN_XP
+ * consecutive points, lane l taking points l, l + 64, ... */
+#define OPSMI_GEN_XP 2
 
 /* element type codes of a generated loop's arguments */
 enum { OPSMI_GEN_F64 = 0, OPSMI_GEN_F32 = 1, OPSMI_GEN_I32 = 2, OPSMI_GEN_NONE = 3 };
@@ -48,6 +51,7 @@ typedef struct opsmi_gen_launch {
   int chunk;         /* 3D: z planes per workgroup; 2D/1D: groups of 4 rows */
   int three_d;       /* 1: grid[2] runs over z chunks */
   unsigned grid[3];  /* workgroups of 64 x 4 threads */
+  int alias;         /* 1: one dat is passed in two arguments (no restrict-qualified kernel) */
   opsmi_gen_dat dat[OPSMI_GEN_MAX_ARGS];    /* dat arguments, by argument index */
   void *partials[OPSMI_GEN_MAX_ARGS];       /* reduction arguments: dim values per workgroup */
 } opsmi_gen_launch;

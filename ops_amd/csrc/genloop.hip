@@ -79,9 +79,11 @@ void ops_mi_gen_par_loop(const char *name, ops_block block, int dim, const int *
     rec.bytes = 0.0;
     rec.launch = [](hipStream_t) {};
   } else {
-    // the hand-written engine's tiling: 3D one plane of 64 x 4 per
-    // workgroup, 2D four groups of four rows
-    const Tiling t = make_tiling(r, r.z1 - r.z0 > 1 ? gen_zchunk() : 4);
+    // the hand-written engine's tiling with rows of 64 * OPSMI_GEN_XP
+    // points: 3D one plane of 4 rows per workgroup, 2D four groups of four
+    // rows
+    Tiling t = make_tiling(r, r.z1 - r.z0 > 1 ? gen_zchunk() : 4);
+    t.grid.x = (unsigned)(((long)r.x1 - r.x0 + kWave * OPSMI_GEN_XP - 1) / (kWave * OPSMI_GEN_XP));
     const unsigned parts = t.blocks();
     std::vector<GenRed> reds;
     size_t off = 0;
@@ -105,6 +107,10 @@ void ops_mi_gen_par_loop(const char *name, ops_block block, int dim, const int *
     std::vector<ops_dat> dats((size_t)nargs, nullptr);
     for (int i = 0; i < nargs; i++)
       if (args[i].argtype == OPS_ARG_DAT) dats[i] = args[i].dat;
+    int alias = 0;
+    for (int i = 0; i < nargs; i++)
+      for (int j = i + 1; j < nargs; j++)
+        if (dats[i] && dats[i] == dats[j]) alias = 1;
     rec.launch = [=](hipStream_t s) {
       opsmi_gen_launch L;
       memset(&L, 0, sizeof(L));
@@ -115,6 +121,7 @@ void ops_mi_gen_par_loop(const char *name, ops_block block, int dim, const int *
       L.grid[0] = t.grid.x;
       L.grid[1] = t.grid.y;
       L.grid[2] = t.grid.z;
+      L.alias = alias;
       for (size_t i = 0; i < dats.size(); i++)
         if (dats[i]) {
           const DatView v = view_of(dats[i]);

@@ -343,30 +343,38 @@ def kernel_code(loop, kern, consts):
             L.append(f"  {a.typ} *r{i};")
     L.append(f"  opsb_{K} b;")
     L.append("};")
-    L.append(f"__global__ __launch_bounds__(256) void opsg_{K}(const opsa_{K} A) {{")
-    for i, a in enumerate(loop.args):
-        if a.kind == "reduce":
-            L.append(f"  {a.typ} red{i}[{a.dim}];")
-            L.append(f"  for (int q = 0; q < {a.dim}; q++) red{i}[q] = ::opsmi_gen::ident<{a.typ}, {a.acc}>();")
-    L.append("  ::opsmi_gen::for_each_point(A.L, [&](int x, int y, int z) {")
-    call = []
-    for i, a in enumerate(loop.args):
-        if a.kind == "dat":
-            L.append(f"    ::opsmi_gen::Acc<{a.typ}, {loop.ndim}, {'true' if a.multi else 'false'}> a{i}(A.d{i}, x, y, z);")
-            call.append(f"a{i}")
-        elif a.kind == "idx":
-            L.append(f"    int idx{i}[3] = {{x, y, z}};")
-            call.append(f"idx{i}")
-        elif a.kind == "gbl":
-            call.append(f"A.b.g{i}")
-        else:
-            call.append(f"red{i}")
-    L.append(f"    A.b.k({', '.join(call)});")
-    L.append("  });")
-    for i, a in enumerate(loop.args):
-        if a.kind == "reduce":
-            L.append(f"  ::opsmi_gen::store_partials<{a.typ}, {a.acc}, {a.dim}>(red{i}, A.r{i});")
-    L.append("}")
+    # two kernels: dat pointers restrict-qualified (OPS writes a dat only at
+    # the point itself, so a thread's points are independent), and plain for
+    # loops that pass one dat twice
+    dats = [i for i, a in enumerate(loop.args) if a.kind == "dat"]
+    for suffix, qual in (("", "__restrict__ "), ("_alias", "")):
+        ptrs = "".join(f", {'const ' if loop.args[i].acc == 'OPS_READ' else ''}{loop.args[i].typ} *{qual}p{i}"
+                       for i in dats)
+        L.append(f"__global__ __launch_bounds__(256) void opsg_{K}{suffix}(const opsa_{K} A{ptrs}) {{")
+        for i, a in enumerate(loop.args):
+            if a.kind == "reduce":
+                L.append(f"  {a.typ} red{i}[{a.dim}];")
+                L.append(f"  for (int q = 0; q < {a.dim}; q++) red{i}[q] = ::opsmi_gen::ident<{a.typ}, {a.acc}>();")
+        L.append("  ::opsmi_gen::for_each_point(A.L, [&](int x, int y, int z) {")
+        call = []
+        for i, a in enumerate(loop.args):
+            if a.kind == "dat":
+                L.append(f"    ::opsmi_gen::Acc<{a.typ}, {loop.ndim}, {'true' if a.multi else 'false'}> "
+                         f"a{i}(p{i}, A.d{i}, x, y, z);")
+                call.append(f"a{i}")
+            elif a.kind == "idx":
+                L.append(f"    int idx{i}[3] = {{x, y, z}};")
+                call.append(f"idx{i}")
+            elif a.kind == "gbl":
+                call.append(f"A.b.g{i}")
+            else:
+                call.append(f"red{i}")
+        L.append(f"    A.b.k({', '.join(call)});")
+        L.append("  });")
+        for i, a in enumerate(loop.args):
+            if a.kind == "reduce":
+                L.append(f"  ::opsmi_gen::store_partials<{a.typ}, {a.acc}, {a.dim}>(red{i}, A.r{i});")
+        L.append("}")
     L.append(f"void opsl_{K}(const opsmi_gen_launch *L, const void *blob, void *stream) {{")
     L.append(f"  opsa_{K} A;")
     L.append("  A.L = ::opsmi_gen::loop_of(L);")
@@ -376,8 +384,10 @@ def kernel_code(loop, kern, consts):
         elif a.kind == "reduce":
             L.append(f"  A.r{i} = ({a.typ} *)L->partials[{i}];")
     L.append("  memcpy(&A.b, blob, sizeof(A.b));")
-    L.append(f"  hipLaunchKernelGGL(opsg_{K}, dim3(L->grid[0], L->grid[1], L->grid[2]), "
-             f"dim3(::opsmi_gen::kWave, ::opsmi_gen::kRows), 0, (hipStream_t)stream, A);")
+    ptrs = "".join(f", ({loop.args[i].typ} *)L->dat[{i}].base" for i in dats)
+    L.append(f"  auto kern = L->alias ? opsg_{K}_alias : opsg_{K};")
+    L.append(f"  hipLaunchKernelGGL(kern, dim3(L->grid[0], L->grid[1], L->grid[2]), "
+             f"dim3(::opsmi_gen::kWave, ::opsmi_gen::kRows), 0, (hipStream_t)stream, A{ptrs});")
     L.append("}")
     return "\n".join(L), used, points
 
