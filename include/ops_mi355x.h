@@ -460,6 +460,18 @@ static inline void ops_tridMultiDimBatch_Inc(int ndim, int solvedim, int *range,
                                              ops_dat c, ops_dat d, ops_dat u, ops_tridsolver_params *ctx) {
   ops_tridMultiDimBatch_Inc_c(ndim, solvedim, range, a, b, c, d, u, ctx->tridsolver_params);
 }
+/* the reference's C++ signatures over the single instance
+ * (ops/c/include/ops_lib_core.h: ops_execute(OPS_instance *),
+ * ops_timing_output(std::ostream &)); checkpointing is not part of this
+ * backend, so ops_checkpointing_init returns false, as the reference does
+ * when checkpointing is not enabled */
+#include <iostream>
+static inline void ops_execute(void *instance) {
+  (void)instance;
+  ops_execute();
+}
+static inline void ops_timing_output(std::ostream &) { ops_timing_output_stdout(); }
+static inline bool ops_checkpointing_init(const char *, double, int) { return false; }
 #endif
 
 #endif /* OPS_MI355X_H */

@@ -13,11 +13,8 @@
  *       the accessor the app's own kernel definitions are written with
  *       (ops/c/include/ops_lib_core.h:1356-2038: x fastest), so those
  *       definitions compile; they are host code the GPU loops replace;
- *   ops_execute(instance), ops_timing_output(std::ostream &)
- *       the reference's C++ signatures over the single instance;
- *   ops_checkpointing_init(...)
- *       checkpointing is not part of this backend: returns false, as the
- *       reference does when checkpointing is not enabled.
+ * (ops_execute(instance), ops_timing_output(std::ostream &) and
+ * ops_checkpointing_init come from ops_mi355x.h's C++ part.)
  */
 #ifndef OPS_MI355X_SEQ_V2_H
 #define OPS_MI355X_SEQ_V2_H
@@ -38,12 +35,5 @@ class ACC {
   T *p_;
   long sx_, sy_;
 };
-
-static inline void ops_execute(void *instance) {
-  (void)instance;
-  ops_execute();
-}
-static inline void ops_timing_output(std::ostream &) { ops_timing_output_stdout(); }
-static inline bool ops_checkpointing_init(const char *, double, int) { return false; }
 
 #endif

@@ -26,7 +26,14 @@ access as an OPS_* name, ops_arg_gbl/ops_arg_reduce dims are integer
 literals, and the kernel is defined in the application file or a header it
 includes with quotes.
 
+Helper functions the kernels call live in the application's own headers,
+declared with the OPS_FUN_PREFIX macro as for the reference's generated code
+(e.g. apps/c/poisson/user_types.h); the generated kernel file defines it as
+__host__ __device__ and includes the application's local headers other than
+the kernel headers.
+
 Usage: python -m ops_amd.translator.opsmi_translate app.cpp [...] -o outdir [-I dir]
+                                                   [--shared LIB.so | --exe PROGRAM]
 """
 import argparse
 import os
@@ -448,10 +455,17 @@ def translate(app, out_dir, inc_dirs=()):
         kernels[lp.kernel] = (lp, find_kernel(lp.kernel, files))
         order.append(lp.kernel)
 
-    # kernels file
+    # kernels file: the application's other local headers (user types and
+    # helper functions) come first, with OPS_FUN_PREFIX making the helpers
+    # device functions, as the reference's generated code does
+    kernel_files = {kern.path for _, kern in kernels.values()}
+    ops_names = {"ops_seq_v2.h", "ops_seq.h", "ops_lib_core.h"}
+    user_incs = [name for _, _, p, name in incs if p not in kernel_files and os.path.basename(name) not in ops_names]
     H = [f"// Generated by ops_amd/translator/opsmi_translate.py from {os.path.basename(app)}; do not edit.",
-         "#include <string.h>", "", "#include \"ops_mi355x_device.h\"", "",
-         "#define OPSMI_GEN_STUB __attribute__((visibility(\"hidden\")))", "", "namespace {"]
+         "#include <math.h>", "#include <string.h>", "", "#include \"ops_mi355x_device.h\"", "",
+         "#ifndef OPS_FUN_PREFIX", "#define OPS_FUN_PREFIX __host__ __device__", "#endif"]
+    H += [f"#include \"{name}\"" for name in user_incs]
+    H += ["", "#define OPSMI_GEN_STUB __attribute__((visibility(\"hidden\")))", "", "namespace {"]
     stubs, sigs = [], []
     for k in order:
         lp, kern = kernels[k]
@@ -480,7 +494,6 @@ def translate(app, out_dir, inc_dirs=()):
     edits = []
     for lp in loops:
         edits.append((lp.span[0], lp.span[1], f"ops_par_loop_{lp.kernel}("))
-    kernel_files = {kern.path for _, kern in kernels.values()}
     for k in order:
         kern = kernels[k][1]
         if kern.path == app:
@@ -515,10 +528,11 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contra
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def build_shared(apps, out_so, gen_dir, inc_dirs=(), hipcc="/opt/rocm/bin/hipcc", jobs=8):
+def build_shared(apps, out_so, gen_dir, inc_dirs=(), hipcc="/opt/rocm/bin/hipcc", jobs=8, executable=False):
     """Translates apps into gen_dir and links them with their generated
-    kernels into the shared object out_so (linked against ops_amd/libopsmi.so
-    with an rpath to it)."""
+    kernels into the shared object out_so (or, with executable=True, the
+    program out_so), linked against ops_amd/libopsmi.so with an rpath to
+    it."""
     import subprocess
     from concurrent.futures import ThreadPoolExecutor
     inc = [os.path.join(ROOT, "include"), gen_dir] + [os.path.dirname(os.path.abspath(a)) for a in apps] + list(inc_dirs)
@@ -538,7 +552,7 @@ def build_shared(apps, out_so, gen_dir, inc_dirs=(), hipcc="/opt/rocm/bin/hipcc"
                 raise TranslateError(f"compile failed: {' '.join(r.args)}\n{r.stderr}")
     libdir = os.path.join(ROOT, "ops_amd")
     rpath = os.path.relpath(libdir, os.path.dirname(os.path.abspath(out_so)))
-    link = [hipcc] + HIP_FLAGS + ["-shared"] + objs + [f"-L{libdir}", "-lopsmi",
+    link = [hipcc] + HIP_FLAGS + ([] if executable else ["-shared"]) + objs + [f"-L{libdir}", "-lopsmi",
                                                        f"-Wl,-rpath,$ORIGIN/{rpath}", "-o", out_so]
     r = subprocess.run(link, capture_output=True, text=True)
     if r.returncode != 0:
@@ -552,11 +566,13 @@ def main(argv=None):
     ap.add_argument("-o", "--out", default=".")
     ap.add_argument("-I", dest="inc", action="append", default=[])
     ap.add_argument("--shared", metavar="LIB.so", help="also compile and link everything into LIB.so")
+    ap.add_argument("--exe", metavar="PROGRAM", help="also compile and link everything into PROGRAM")
     a = ap.parse_args(argv)
     try:
-        if a.shared:
-            build_shared(a.apps, a.shared, a.out, a.inc)
-            print(f"{len(a.apps)} applications -> {a.shared}")
+        if a.shared or a.exe:
+            out = a.exe or a.shared
+            build_shared(a.apps, out, a.out, a.inc, executable=bool(a.exe))
+            print(f"{len(a.apps)} applications -> {out}")
         else:
             for app in a.apps:
                 ks = translate(app, a.out, a.inc)

@@ -46,3 +46,27 @@ def test_multiblock_poisson_app_other_sizes():
         out = _run("app_poisson_opsmi", f"-sizex={nx}", f"-sizey={ny}", f"-iters={it}")
         err = float(re.search(r"Total error: ([0-9.eE+-]+)", out).group(1))
         assert abs(err - ref) <= 1e-12 * ref, (nx, ny, it)
+
+
+# The same two apps through this project's translator: their own kernel
+# headers become generated HIP kernels (oracle/Makefile.ref, APPS_GEN).
+def test_laplace_tutorial_app_through_the_translator():
+    out = _run("app_laplace2d_gen")
+    assert "This run is considered PASSED" in out, out[-2000:]
+    pct = float(re.search(r"within ([0-9.eE+-]+) %", out).group(1))
+    assert pct < 1e-9
+    hw = _run("app_laplace2d_opsmi")
+    assert re.search(r"within ([0-9.eE+-]+) %", out).group(1) == re.search(r"within ([0-9.eE+-]+) %", hw).group(1)
+
+
+def test_multiblock_poisson_app_through_the_translator():
+    from oracle import oracle_py as O
+    out = _run("app_poisson_gen")
+    assert "This run is considered PASSED" in out, out[-2000:]
+    err = float(re.search(r"Total error: ([0-9.eE+-]+)", out).group(1))
+    assert abs(err - 16.5173570176708) <= 1e-12 * 16.5173570176708
+    g, m = O.load_golden()
+    for (nx, ny, it), ref in zip(m["mb_poisson"]["configs"], g["mb_poisson_err"]):
+        out = _run("app_poisson_gen", f"-sizex={nx}", f"-sizey={ny}", f"-iters={it}")
+        err = float(re.search(r"Total error: ([0-9.eE+-]+)", out).group(1))
+        assert abs(err - ref) <= 1e-12 * ref, (nx, ny, it)
