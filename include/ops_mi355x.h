@@ -416,7 +416,31 @@ void ops_mi_set_fusion(int level);
 
 /* ---- C++ conveniences with the reference's template signatures
  *      (ops/c/include/ops_lib_core.h:647-694,793-796,922-934,947-975) ---- */
+#include <math.h>
+#include <stdlib.h>
 #include <string.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+/* The accessor type user kernels are written with
+ * (ops/c/include/ops_lib_core.h:1356-2038), so kernel definitions an
+ * application includes compile as host code.  They are never run here: the
+ * loops run as hand-written or translator-generated GPU kernels, whose
+ * device accessor is opsmi_gen::Acc (include/ops_mi355x_device.h). */
+template <class T>
+class ACC {
+ public:
+  ACC(T *p = nullptr, long sx = 0, long sy = 0) : p_(p), sx_(sx), sy_(sy) {}
+  T &operator()(int x, int y = 0, int z = 0, int w = 0) const {
+    return p_[x + sx_ * (y + sy_ * ((long)z + w))];
+  }
+
+ private:
+  T *p_;
+  long sx_, sy_;
+};
 template <class T>
 static inline ops_dat ops_decl_dat(ops_block block, int data_size, int *block_size, int *base, int *d_m,
                                    int *d_p, T *data, const char *type, const char *name) {

@@ -65,7 +65,8 @@ typedef void (*opsmi_gen_launch_fn)(const opsmi_gen_launch *L, const void *blob,
  * npts[i]: number of points the kernel reads or writes through dat argument
  * i, listed in pts as (component or -1, dx, dy, dz) quadruples, argument
  * after argument; -1 when the translator could not list them (non-literal
- * indices), 0 for non-dat arguments.  Checks, as the hand-written entry
+ * indices); for an OPS_READ ops_arg_gbl the number of values the kernel's
+ * copy holds (a call passing more is refused); 0 for other arguments.  Checks, as the hand-written entry
  * points do (ops/c/src/core/ops_lib_core.cpp:1704-1836 under OPS_DEBUG):
  * argument kinds and access modes, element types against the declared ones,
  * component counts, that each listed point is in the argument's stencil and

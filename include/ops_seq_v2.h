@@ -9,11 +9,8 @@
  *       kernel name — the rewrite ops_translator applies to an app
  *       (ops_translator/resources/templates/cpp/loop_host.cpp.j2:12-21), so
  *       an app may use the kernels this backend provides;
- *   ACC<T>
- *       the accessor the app's own kernel definitions are written with
- *       (ops/c/include/ops_lib_core.h:1356-2038: x fastest), so those
- *       definitions compile; they are host code the GPU loops replace;
- * (ops_execute(instance), ops_timing_output(std::ostream &) and
+ * (The host ACC<T> the app's own kernel definitions compile against,
+ * ops_execute(instance), ops_timing_output(std::ostream &) and
  * ops_checkpointing_init come from ops_mi355x.h's C++ part.)
  */
 #ifndef OPS_MI355X_SEQ_V2_H
@@ -24,16 +21,5 @@
 #include "ops_mi355x.h"
 
 #define ops_par_loop(kernel, ...) ops_par_loop_##kernel(__VA_ARGS__)
-
-template <class T>
-class ACC {
- public:
-  ACC(T *p = nullptr, long sx = 0, long sy = 0) : p_(p), sx_(sx), sy_(sy) {}
-  T &operator()(int x, int y = 0, int z = 0) const { return p_[x + sx_ * (y + sy_ * (long)z)]; }
-
- private:
-  T *p_;
-  long sx_, sy_;
-};
 
 #endif

@@ -276,7 +276,7 @@ Range3 check_gen_loop(const char *name, ops_block block, int dim, const int *ran
     const int es = want == DT_F64 ? 8 : 4;
     const int np = npts[i];
     const int *mine = pt;
-    if (np > 0) pt += 4 * np;
+    if (np > 0 && a.argtype == OPS_ARG_DAT) pt += 4 * np;
     if (a.argtype == OPS_ARG_IDX) continue;
     if (want != DT_F64 && want != DT_F32 && want != DT_I32)
       fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": arg ", i, " has no element type (only double, float, int)"));
@@ -284,6 +284,9 @@ Range3 check_gen_loop(const char *name, ops_block block, int dim, const int *ran
       if (a.acc == OPS_READ) {
         if (!a.data || a.dim < 1 || a.elem_size != es)
           fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": gbl arg ", i, " must hold ", dt_name(want), " values"));
+        if (np > 0 && a.dim > np)
+          fail(OPS_NOT_IMPLEMENTED, cat("loop ", name, ": gbl arg ", i, " passes ", a.dim,
+                                        " values, the generated kernel holds ", np));
       } else if (a.acc == OPS_INC || a.acc == OPS_MIN || a.acc == OPS_MAX) {
         if (!a.data) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": arg ", i, " has no reduction handle"));
         ops_reduction h = (ops_reduction)a.data;

@@ -22,9 +22,10 @@ body's arithmetic is the user's own (compile with -ffp-contract=off for
 the reference's sequential rounding).  Requirements, as for the reference
 translator: the kernel is a literal identifier, the loop's dim is an integer
 literal, every ops_arg_* call spells its type as a string literal and its
-access as an OPS_* name, ops_arg_gbl/ops_arg_reduce dims are integer
-literals, and the kernel is defined in the application file or a header it
-includes with quotes.
+access as an OPS_* name, ops_arg_reduce dims are integer literals (an
+ops_arg_gbl whose dim is not a literal gets a copy of up to 64 values, and
+a call passing more is refused at run time), and the kernel is defined in
+the application file or a header it includes with quotes.
 
 Helper functions the kernels call live in the application's own headers,
 declared with the OPS_FUN_PREFIX macro as for the reference's generated code
@@ -41,6 +42,7 @@ import re
 import sys
 
 TYPES = {"double": ("OPSMI_GEN_F64", 8), "float": ("OPSMI_GEN_F32", 4), "int": ("OPSMI_GEN_I32", 4)}
+GBL_CAP = 64  # values a generated kernel holds of an ops_arg_gbl whose dim is not a literal
 ACCESS = {"OPS_READ", "OPS_WRITE", "OPS_RW", "OPS_INC", "OPS_MIN", "OPS_MAX"}
 
 
@@ -182,11 +184,13 @@ def parse_arg(src, m, span, where):
     typ = string_literal(items[2], f"{where}: {fn} type")
     acc = items[3]
     d = int_literal(items[1])
-    if d is None:
-        raise TranslateError(f"{where}: {fn} dim must be an integer literal")
     kind = "gbl" if fn == "ops_arg_gbl" else "reduce"
     if kind == "gbl" and acc != "OPS_READ":
         kind = "reduce"
+    if d is None:
+        if kind == "reduce":
+            raise TranslateError(f"{where}: {fn} dim must be an integer literal for a reduction")
+        d = GBL_CAP  # a run-time count: the kernel's copy holds up to GBL_CAP values
     return Arg(kind, typ, acc, d)
 
 
@@ -409,7 +413,7 @@ def stub_code(loop, used, consts, points):
         L.append(f"  const ops_arg args[{n}] = {{{', '.join(f'arg{i}' for i in range(n))}}};")
         types = ", ".join("OPSMI_GEN_NONE" if a.kind == "idx" else TYPES[a.typ][0] for a in loop.args)
         L.append(f"  static const int types[{n}] = {{{types}}};")
-        counts = [-1 if p is None else len(p) for p in points]
+        counts = [a.dim if a.kind == "gbl" else (-1 if p is None else len(p)) for p, a in zip(points, loop.args)]
         flat = [v for p in points if p for pt in p for v in pt]
         L.append(f"  static const int npts[{n}] = {{{', '.join(map(str, counts))}}};")
         L.append(f"  static const int pts[{max(1, len(flat))}] = {{{', '.join(map(str, flat)) or '0'}}};")
@@ -423,8 +427,9 @@ def stub_code(loop, used, consts, points):
         L.append(f"  ops_mi_gen_const(\"{c}\", \"{t}\", {d}, &b.k.{c});")
     for i, a in enumerate(loop.args):
         if a.kind == "gbl":
-            L.append(f"  if (arg{i}.data && arg{i}.dim >= {a.dim} && arg{i}.elem_size == {TYPES[a.typ][1]})")
-            L.append(f"    memcpy(b.g{i}, arg{i}.data, sizeof(b.g{i}));")
+            L.append(f"  if (arg{i}.data && arg{i}.elem_size == {TYPES[a.typ][1]} && arg{i}.dim > 0)")
+            L.append(f"    memcpy(b.g{i}, arg{i}.data, (size_t)(arg{i}.dim < {a.dim} ? arg{i}.dim : {a.dim}) * "
+                     f"{TYPES[a.typ][1]});")
     L.append(f"  ops_mi_gen_par_loop(name, block, dim, range, args, {n}, types, npts, pts, &opsl_{K}, &b, sizeof(b));")
     L.append("}")
     return "\n".join(L), sig

@@ -69,8 +69,8 @@ def _app(tmp_path, loop, kernel="void k(ACC<double> &a) { a(0, 0) = 1.0; }"):
      "takes 1 parameters"),
     ('ops_par_loop(k, "k", b, 2, r, ops_arg_dat(d, 1, s, "double", OPS_WRITE))', "void k(double *a) { *a = 1; }",
      "ACC<T> reference"),
-    ('ops_par_loop(k, "k", b, 2, r, ops_arg_gbl(&x, n, "double", OPS_READ))', "void k(const double *x) {}",
-     "integer literal"),
+    ('ops_par_loop(k, "k", b, 2, r, ops_arg_reduce(h, n, "double", OPS_INC))', "void k(double *x) {}",
+     "integer literal for a reduction"),
     ('ops_par_loop(k, "k", b, 2, r, ops_arg_dat(d, 1, s, "half", OPS_WRITE))', None, "only double, float, int"),
 ])
 def test_refuses_what_it_cannot_translate(tmp_path, loop, kernel, msg):
@@ -90,3 +90,14 @@ def test_generated_library_built():
     # hand-written ops_par_loop_apply_stencil of libopsmi.so)
     assert not any(n.startswith("ops_par_loop_") for n in exported)
     assert re.search(r" U ops_mi_gen_par_loop\b", dyn)
+
+
+def test_gbl_with_a_run_time_dim_gets_a_bounded_copy(tmp_path):
+    app = _app(tmp_path, 'ops_par_loop(k, "k", b, 2, r, ops_arg_dat(d, 1, s, "double", OPS_WRITE), '
+                         'ops_arg_gbl(xs, n, "double", OPS_READ))',
+               "void k(ACC<double> &a, const double *xs) { a(0, 0) = xs[1]; }")
+    T.translate(app, str(tmp_path / "out"))
+    hip = open(tmp_path / "out" / "app_opsmi_kernels.hip").read()
+    assert f"double g1[{T.GBL_CAP}];" in hip
+    # the runtime learns the capacity (npts of a gbl argument) and refuses more
+    assert f"static const int npts[2] = {{1, {T.GBL_CAP}}};" in hip
