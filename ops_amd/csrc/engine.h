@@ -204,6 +204,11 @@ __global__ __launch_bounds__(256) void k_red_final(const T *__restrict__ in, uns
       slot[q] = r;
       mirror[q] = r;
     }
+    // the values reach host memory before the sequence word the host polls
+    if (c.seq) {
+      __threadfence_system();
+      *(volatile unsigned long long *)c.seq = c.seq_val;
+    }
   }
 }
 

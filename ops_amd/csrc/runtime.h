@@ -68,6 +68,11 @@ struct RedImpl {
   void *d_slot = nullptr;    // device value of the handle (size bytes)
   void *h_mirror = nullptr;  // pinned, device-mapped host copy the final fold also writes
   void *d_mirror = nullptr;  // its device address
+  // sequence word in the same pinned allocation: the final fold of the
+  // n-th launch reducing into the handle stores n after its values
+  volatile unsigned long long *h_seq = nullptr;
+  unsigned long long *d_seq = nullptr;
+  unsigned long long seq_issued = 0;  // launches prepared so far
   bool dev_valid = false;    // device slot holds the running value since re-arm
   DType dt = DT_OTHER;
 };
@@ -214,6 +219,8 @@ struct RedCtl {
   int use_init;        // 1: start from init[] (first use since re-arm)
   double init[4];      // initial values, stored as the reduction type
   int init_i[4];
+  unsigned long long *seq;     // host-mapped sequence word (null: none)
+  unsigned long long seq_val;  // stored there after the values
 };
 // Prepares a handle for a launch (device slot, init values).  Marks the
 // handle's device value as live; the result is copied back at result time.
