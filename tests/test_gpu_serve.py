@@ -121,7 +121,8 @@ def test_mixed_poisson_matches_oracle(serve, n):
     done, x = srv.run()[i]
     srv.close()
     f = O.poisson_rhs(SEED, 2, n, n)
-    ref = O.poisson_mixed(2, n, n, 1, f, tol=1e-10, nthreads=8)
+    # the engine's inner FP32 solves run to 1e-6 of their initial residual
+    ref = O.poisson_mixed(2, n, n, 1, f, tol=1e-10, inner_tol=1e-6, nthreads=8)
     assert done["status"] == 1
     assert done["outer"] == ref["outer"] and done["iters"] == ref["inner_total"]
     assert np.max(np.abs(x - ref["x"])) <= 1e-12 * np.max(np.abs(ref["x"]))
