@@ -378,8 +378,24 @@ __global__ __launch_bounds__(256) void sk_fold(const SolveDev *S, SolveStat *st,
   const int t0 = S[s].tile0, n = S[s].ntiles;
   double acc[3] = {0.0, 0.0, 0.0};
   const int nq = mixed && PHASE == 1 ? 3 : 1;
-  for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x)
-    for (int k = 0; k < nq; k++) acc[k] = acc[k] + part[(size_t)(t0 + i) * kPart + k];
+  // thread i adds partials i, i + 256, ... in that order; eight are loaded
+  // before they are added (eight loads in flight, the same sums)
+  const int nb = (int)blockDim.x;
+  for (int i0 = (int)threadIdx.x; i0 < n; i0 += 8 * nb) {
+    double t8[8][3];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int i = i0 + u * nb;
+#pragma unroll
+      for (int k = 0; k < 3; k++) t8[u][k] = i < n && k < nq ? part[(size_t)(t0 + i) * kPart + k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (i0 + u * nb < n)
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+          if (k < nq) acc[k] = acc[k] + t8[u][k];
+  }
   block_reduce<double, OPS_INC, 3>(acc, sh);
   if (threadIdx.x != 0) return;
   SolveStat &q = st[s];
