@@ -893,20 +893,66 @@ struct S27Column {
       if (c.hxh) rh_xh = u(x0 + c.cnt, y + 1, q);
     }
   }
-  __device__ __forceinline__ void stage(int q) {
-    const int b = buf(q), nn = n();
-    lds_st<double, V>(&L.sp[b][ly + 1][o], nn, p);
-    if (c.hxl) L.sp[b][ly + 1][V - 1] = xl;
-    if (c.hxh) L.sp[b][ly + 1][V + c.cnt] = xh;
+  template <int B>
+  __device__ __forceinline__ void stage_b() {
+    const int nn = n();
+    lds_st<double, V>(&L.sp[B][ly + 1][o], nn, p);
+    if (c.hxl) L.sp[B][ly + 1][V - 1] = xl;
+    if (c.hxh) L.sp[B][ly + 1][V + c.cnt] = xh;
     if (c.hyl) {
-      lds_st<double, V>(&L.sp[b][0][o], nn, rl);
-      if (c.hxl) L.sp[b][0][V - 1] = rl_xl;
-      if (c.hxh) L.sp[b][0][V + c.cnt] = rl_xh;
+      lds_st<double, V>(&L.sp[B][0][o], nn, rl);
+      if (c.hxl) L.sp[B][0][V - 1] = rl_xl;
+      if (c.hxh) L.sp[B][0][V + c.cnt] = rl_xh;
     }
     if (c.hyh) {
-      lds_st<double, V>(&L.sp[b][ly + 2][o], nn, rh);
-      if (c.hxl) L.sp[b][ly + 2][V - 1] = rh_xl;
-      if (c.hxh) L.sp[b][ly + 2][V + c.cnt] = rh_xh;
+      lds_st<double, V>(&L.sp[B][ly + 2][o], nn, rh);
+      if (c.hxl) L.sp[B][ly + 2][V - 1] = rh_xl;
+      if (c.hxh) L.sp[B][ly + 2][V + c.cnt] = rh_xh;
+    }
+  }
+  __device__ __forceinline__ void stage(int q) {
+    switch (buf(q)) {
+      case 0: stage_b<0>(); break;
+      case 1: stage_b<1>(); break;
+      case 2: stage_b<2>(); break;
+      case 3: stage_b<3>(); break;
+      default: stage_b<4>(); break;
+    }
+  }
+
+  // plane z = z0 + s with s % kS27Bufs == K: planes z - 1, z, z + 1 sit in
+  // buffers (K + 4) % 5, K, (K + 1) % 5 and plane z + 2 is staged into
+  // (K + 2) % 5 (the z loop is unrolled by the ring length, so every buffer
+  // index is a compile-time constant)
+  template <int K>
+  __device__ __forceinline__ void step(int z) {
+    if (act()) {
+      if (z + 2 <= z1) {
+        stage_b<(K + 2) % kS27Bufs>();
+        load(min(z + 3, z1));
+      }
+    }
+    __syncthreads();
+    if (act()) {
+      double acc[V] = {0.0, 0.0};
+#pragma unroll
+      for (int dz = -1; dz <= 1; dz++) {
+        const int b = (K + dz + kS27Bufs) % kS27Bufs;
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++) {
+          const double *row = &L.sp[b][ly + 1 + dy][0];
+          const vec_t<double, 2> mid = *reinterpret_cast<const vec_t<double, 2> *>(row + o);
+          const double lft = row[o - 1], rgt = row[o + 2];
+          const int ad = (dy < 0 ? -dy : dy) + (dz < 0 ? -dz : dz);
+          acc[0] = acc[0] + kW27[ad + 1] * lft;
+          acc[0] = acc[0] + kW27[ad] * mid[0];
+          acc[0] = acc[0] + kW27[ad + 1] * mid[1];
+          acc[1] = acc[1] + kW27[ad + 1] * mid[0];
+          acc[1] = acc[1] + kW27[ad] * mid[1];
+          acc[1] = acc[1] + kW27[ad + 1] * rgt;
+        }
+      }
+      stv<double, V, true>(out.at(x, y, z), n(), acc);
     }
   }
 
@@ -921,36 +967,19 @@ struct S27Column {
       }
       if (z0 + 2 <= z1) load(z0 + 2);
     }
-    for (int z = z0; z < z1; z++) {
-      if (act()) {
-        if (z + 2 <= z1) {
-          stage(z + 2);
-          load(min(z + 3, z1));
-        }
-      }
-      __syncthreads();
-      if (act()) {
-        double acc[V] = {0.0, 0.0};
-#pragma unroll
-        for (int dz = -1; dz <= 1; dz++) {
-          const int b = buf(z + dz);
-#pragma unroll
-          for (int dy = -1; dy <= 1; dy++) {
-            const double *row = &L.sp[b][ly + 1 + dy][0];
-            const vec_t<double, 2> mid = *reinterpret_cast<const vec_t<double, 2> *>(row + o);
-            const double lft = row[o - 1], rgt = row[o + 2];
-            const int ad = (dy < 0 ? -dy : dy) + (dz < 0 ? -dz : dz);
-            acc[0] = acc[0] + kW27[ad + 1] * lft;
-            acc[0] = acc[0] + kW27[ad] * mid[0];
-            acc[0] = acc[0] + kW27[ad + 1] * mid[1];
-            acc[1] = acc[1] + kW27[ad + 1] * mid[0];
-            acc[1] = acc[1] + kW27[ad] * mid[1];
-            acc[1] = acc[1] + kW27[ad + 1] * rgt;
-          }
-        }
-        stv<double, V, true>(out.at(x, y, z), n(), acc);
-      }
+    int z = z0;
+    for (; z + kS27Bufs <= z1; z += kS27Bufs) {
+      step<0>(z);
+      step<1>(z + 1);
+      step<2>(z + 2);
+      step<3>(z + 3);
+      step<4>(z + 4);
     }
+    const int rem = z1 - z;
+    if (rem > 0) step<0>(z);
+    if (rem > 1) step<1>(z + 1);
+    if (rem > 2) step<2>(z + 2);
+    if (rem > 3) step<3>(z + 3);
   }
 };
 
