@@ -94,3 +94,18 @@ extern "C" int gen_fill2d(ops_block block, ops_dat a, int nx, int ny, double val
                ops_arg_gbl(&val, 1, "double", OPS_READ));
   return 0;
 }
+
+void double_it(ACC<double> &d, const ACC<double> &s) {
+  d(0, 0, 0) = 2.0 * s(0, 0, 0);
+}
+
+// d = 2 d with d passed as both arguments (the kernel without restrict-
+// qualified dat pointers runs)
+extern "C" int gen_double_in_place(ops_block block, ops_dat d, int nx, int ny, int nz) {
+  int s0[] = {0, 0, 0};
+  ops_stencil S0 = ops_decl_stencil(3, 1, s0, "d0");
+  int inner[] = {0, nx, 0, ny, 0, nz};
+  ops_par_loop(double_it, "double_it", block, 3, inner, ops_arg_dat(d, 1, S0, "double", OPS_WRITE),
+               ops_arg_dat(d, 1, S0, "double", OPS_READ));
+  return 0;
+}

@@ -35,6 +35,7 @@ def gen(gpu_ops):
     L.gen_vec3.argtypes = [vp, vp, vp, ci, ci, ci, dp, C.POINTER(C.c_int)]
     L.gen_opt_add.argtypes = [vp, vp, vp, ci, ci, ci, ci]
     L.gen_fill2d.argtypes = [vp, vp, ci, ci, cd]
+    L.gen_double_in_place.argtypes = [vp, vp, ci, ci, ci]
     return L
 
 
@@ -204,6 +205,18 @@ def test_generated_optional_dat(gpu_ops, gen):
     gen.gen_opt_add(blk.ptr, d.ptr, e.ptr, nx, ny, nz, 0)
     ops._check()
     assert np.array_equal(ops.ops_dat_fetch_data(d), 2.0 * (d0 + e0))
+
+
+def test_generated_loop_with_one_dat_twice(gpu_ops, gen):
+    ops = gpu_ops
+    nx, ny, nz = 130, 7, 5
+    blk = ops.ops_decl_block(3, "gentwice")
+    d = ops.ops_decl_dat(blk, [nx, ny, nz], name="d")
+    d0 = O.u01(6, 1, nx * ny * nz).reshape(nz, ny, nx)
+    ops.ops_dat_set_data(d, d0)
+    gen.gen_double_in_place(blk.ptr, d.ptr, nx, ny, nz)
+    ops._check()
+    assert np.array_equal(ops.ops_dat_fetch_data(d), 2.0 * d0)
 
 
 def test_generated_and_hand_written_loops_share_the_queue(gpu_ops, gen):

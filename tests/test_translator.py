@@ -85,7 +85,7 @@ def test_generated_library_built():
     dyn = subprocess.run(["nm", "-D", lib], capture_output=True, text=True, check=True).stdout
     exported = set(re.findall(r" T (\w+)", dyn))
     assert {"gen_laplace", "gen_fields", "gen_heat_step", "gen_laplace_bad_stencil", "gen_ramp1d", "gen_vec3",
-            "gen_opt_add", "gen_fill2d"} <= exported
+            "gen_opt_add", "gen_fill2d", "gen_double_in_place"} <= exported
     # the generated stubs stay inside the library (no clash with the
     # hand-written ops_par_loop_apply_stencil of libopsmi.so)
     assert not any(n.startswith("ops_par_loop_") for n in exported)
