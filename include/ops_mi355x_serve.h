@@ -85,8 +85,9 @@ int opsmi_serve_queued(opsmi_server s);
  * `iters` CG iterations of a representative solve with every candidate
  * (3D: 4, 8, 16, 32 z planes per 128 x 4 column; 2D: 1, 2, 4, 8 rows per
  * wave of a 128-wide tile) and keeps the fastest.  Results of later solves depend on
- * the chosen shape only through the order of their partial sums.
- * tile_param returns the parameter a solve of that kind and side gets. */
+ * the chosen shape only through the order of their partial sums.  2D
+ * FP32-inner (mixed) solves are tuned separately from FP64 ones.
+ * tile_param returns the parameter an FP64 solve of that kind and side gets. */
 int opsmi_serve_autotune(opsmi_server s, int iters);
 int opsmi_serve_tile_param(opsmi_server s, int kind, int n);
 /* bytes a request needs (fields with halos and row padding) */

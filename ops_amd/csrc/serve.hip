@@ -664,6 +664,7 @@ struct opsmi_server_s {
   // tile parameter per size class (defaults; opsmi_serve_autotune measures)
   int chunk3d[3] = {kChunk3D, kChunk3D, kChunk3D};
   int rows2d[3] = {kRows2D, kRows2D, kRows2D};
+  int rows2d_mixed[3] = {kRows2D, kRows2D, kRows2D};  // FP32-inner solves
 };
 
 namespace {
@@ -685,8 +686,9 @@ int size_class(int kind, int n) {
   if (kind == OPSMI_SOLVE_HEAT3D) return n < 64 ? 0 : (n < 128 ? 1 : 2);
   return n < 256 ? 0 : (n < 1024 ? 1 : 2);
 }
-int tile_param(opsmi_server S, int kind, int n) {
-  return kind == OPSMI_SOLVE_HEAT3D ? S->chunk3d[size_class(kind, n)] : S->rows2d[size_class(kind, n)];
+int tile_param(opsmi_server S, int kind, int n, int mixed) {
+  if (kind == OPSMI_SOLVE_HEAT3D) return S->chunk3d[size_class(kind, n)];
+  return mixed ? S->rows2d_mixed[size_class(kind, n)] : S->rows2d[size_class(kind, n)];
 }
 
 long tiles_of(const opsmi_solve_req &q, int param, int *tx, int *ty) {
@@ -729,7 +731,7 @@ void admit(opsmi_server S) {
     size_t off;
     if (!S->arena.alloc(g.bytes, &off)) return;
     int tx, ty;
-    const int param = tile_param(S, R.q.kind, R.q.n);
+    const int param = tile_param(S, R.q.kind, R.q.n, R.q.mixed && R.q.kind == OPSMI_SOLVE_POISSON2D);
     const long nt = tiles_of(R.q, param, &tx, &ty);
     if ((size_t)(S->tiles_active + nt) > S->max_tiles) {
       S->arena.release(off, g.bytes);
@@ -919,7 +921,8 @@ int opsmi_serve_submit(opsmi_server S, const opsmi_solve_req *req) {
   if (g.bytes > S->budget)
     fail(OPS_INVALID_ARGUMENT, cat("opsmi_serve_submit: solve needs ", g.bytes, " bytes, budget is ", S->budget));
   int tx, ty;
-  if ((size_t)tiles_of(*req, tile_param(S, req->kind, req->n), &tx, &ty) > S->max_tiles)
+  if ((size_t)tiles_of(*req, tile_param(S, req->kind, req->n, req->mixed && req->kind == OPSMI_SOLVE_POISSON2D),
+                       &tx, &ty) > S->max_tiles)
     fail(OPS_INVALID_ARGUMENT, "opsmi_serve_submit: solve has more tiles than the tile table holds");
   const int id = S->next_id++;
   Req R;
@@ -1026,11 +1029,14 @@ int opsmi_serve_autotune(opsmi_server S, int iters) {
   hipEvent_t e0, e1;
   OPSMI_HIP(hipEventCreate(&e0));
   OPSMI_HIP(hipEventCreate(&e1));
-  for (int kind = 0; kind < 2; kind++)
+  // 2D Poisson: FP64 and FP32-inner solves get their own tables (the FP32
+  // kernels move half the bytes per point and may prefer other rows)
+  for (int km = 0; km < 3; km++)
     for (int cls = 0; cls < 3; cls++) {
-      opsmi_solve_req q{kind, rep[kind][cls], 12345, 1e-300, 1 << 30, 0};
+      const int kind = km == 1 ? OPSMI_SOLVE_HEAT3D : OPSMI_SOLVE_POISSON2D, mixed = km == 2;
+      opsmi_solve_req q{kind, rep[kind][cls], 12345, 1e-300, 1 << 30, mixed};
       if (geometry(q).bytes > S->budget) continue;
-      int *slot_param = kind == OPSMI_SOLVE_HEAT3D ? &S->chunk3d[cls] : &S->rows2d[cls];
+      int *slot_param = kind == OPSMI_SOLVE_HEAT3D ? &S->chunk3d[cls] : (mixed ? &S->rows2d_mixed[cls] : &S->rows2d[cls]);
       const int *cand = kind == OPSMI_SOLVE_HEAT3D ? cand3 : cand2;
       const int nc = 4;
       int best = *slot_param;
@@ -1083,7 +1089,7 @@ int opsmi_serve_tile_param(opsmi_server S, int kind, int n) {
   if (!S) fail(OPS_INVALID_ARGUMENT, "opsmi_serve_tile_param: NULL server");
   if (kind != OPSMI_SOLVE_POISSON2D && kind != OPSMI_SOLVE_HEAT3D)
     fail(OPS_INVALID_ARGUMENT, cat("opsmi_serve_tile_param: unknown solve kind ", kind));
-  return tile_param(S, kind, n);
+  return tile_param(S, kind, n, 0);
   OPSMI_API_END(-1)
 }
 
