@@ -98,19 +98,28 @@ __device__ __forceinline__ TileBox tile_box(const SolveDev &d, int li) {
 
 // f(x, y, z, n) over this thread's groups of the tile: kSV consecutive points
 // from x, n (1..kSV) of them inside the block
+//
+// A lane whose group is whole (n == kSV: every lane but those at a solve's
+// right edge) calls f with the literal kSV, so the element-count branches in
+// f's vector loads and stores fold away: a branch join makes the compiler
+// wait for every load in flight, which would serialise a group's loads.
 template <class F>
 __device__ __forceinline__ void tile_groups(const SolveDev &d, const TileBox &b, F f) {
   const int x = b.x0 + kSV * (int)threadIdx.x;
   const int n = min(kSV, d.nx - x);
   if (n <= 0) return;
+  auto g = [&](int y, int z) {
+    if (n == kSV) f(x, y, z, kSV);
+    else f(x, y, z, n);
+  };
   if (d.kind == OPSMI_SOLVE_HEAT3D) {
     const int y = b.y0 + (int)threadIdx.y;
     if (y >= d.ny) return;
-    for (int z = b.z0; z < b.z1; z++) f(x, y, z, n);
+    for (int z = b.z0; z < b.z1; z++) g(y, z);
   } else {
     for (int i = 0; i < d.chunk; i++) {
       const int y = b.y0 + (int)threadIdx.y + kWavesPerBlock * i;
-      if (y < d.ny) f(x, y, 0, n);
+      if (y < d.ny) g(y, 0);
     }
   }
 }
