@@ -371,18 +371,25 @@ __global__ __launch_bounds__(256) void k_loop_v(Range3 r, F f, RedCtl ctl, int c
   const TileId t = tile_id();
   const int x = r.x0 + (int)t.x * kWave * V + V * (int)threadIdx.x;
   const int n = min(V, r.x1 - x);
+  // a lane with all V points calls vop with the literal V, so the
+  // element-count branches of its loads and stores fold away (a branch join
+  // makes the compiler wait for every load in flight)
+  auto pt = [&](int y, int z) {
+    if (n == V) f.template vop<V, NTL>(x, y, z, V, acc);
+    else f.template vop<V, NTL>(x, y, z, n, acc);
+  };
   if (n > 0) {
     if (three_d) {
       const int y = r.y0 + (int)t.y * kWavesPerBlock + (int)threadIdx.y;
       if (y < r.y1) {
         const int z0 = r.z0 + (int)t.z * chunk;
         const int z1 = min(z0 + chunk, r.z1);
-        for (int z = z0; z < z1; z++) f.template vop<V, NTL>(x, y, z, n, acc);
+        for (int z = z0; z < z1; z++) pt(y, z);
       }
     } else {
       const int y0 = r.y0 + (int)t.y * kWavesPerBlock * chunk;
       const int y1 = min(y0 + kWavesPerBlock * chunk, r.y1);
-      for (int y = y0 + (int)threadIdx.y; y < y1; y += kWavesPerBlock) f.template vop<V, NTL>(x, y, r.z0, n, acc);
+      for (int y = y0 + (int)threadIdx.y; y < y1; y += kWavesPerBlock) pt(y, r.z0);
     }
   }
   if constexpr (F::red_dim > 0) reduce_block_store<R, OP, D>(acc, ctl);
