@@ -255,7 +255,12 @@ struct JacPtrs {
   Ptr<const T> An;
 };
 
-template <class T, int V>
+// FULL: every tile is 64 V points wide inside the range (every lane has V
+// points): the element counts are literals and the edge values are loaded
+// by every lane (one broadcast address) and kept by the edge lane, so a row
+// has no lane-divergent branches (a branch join makes the compiler wait for
+// every load in flight)
+template <class T, int V, bool FULL>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_jacobi_chain(JacPtrs<T> c, Range3 R, RedCtl ctl) {
   T err[1] = {RedOp<T, OPS_MAX>::identity()};
   const int lx = (int)threadIdx.x, wv = (int)threadIdx.y;
@@ -264,9 +269,9 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_jacobi_chain(JacPtrs
   const int ya = R.y0 + ((int)t.y * kWavesPerBlock + wv) * kJacRows;
   const int yb = min(ya + kJacRows, R.y1);
   const int z = R.z0;
-  const int cnt = min(kWave * V, R.x1 - x0);
+  const int cnt = FULL ? kWave * V : min(kWave * V, R.x1 - x0);
   const int x = x0 + V * lx;
-  const int n = max(0, min(V, cnt - V * lx));
+  const int n = FULL ? V : max(0, min(V, cnt - V * lx));
   const bool hxh = lx == (cnt - 1) / V;
   const bool in_xl = x0 - 1 >= R.x0, in_xh = x0 + cnt < R.x1;
   auto row = [&](int yy, T (&v)[V]) {
@@ -280,13 +285,25 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_jacobi_chain(JacPtrs
     T up[V], cur[V], dn[V];
     row(ya - 1, up);
     row(ya, cur);
-    T cl = lx == 0 ? edge(x0 - 1, ya, in_xl) : (T)0, cr = hxh ? edge(x0 + cnt, ya, in_xh) : (T)0;
+    T cl, cr;
+    if (FULL) {
+      cl = edge(x0 - 1, ya, in_xl);
+      cr = edge(x0 + cnt, ya, in_xh);
+    } else {
+      cl = lx == 0 ? edge(x0 - 1, ya, in_xl) : (T)0;
+      cr = hxh ? edge(x0 + cnt, ya, in_xh) : (T)0;
+    }
     for (int y = ya; y < yb; y++) {
       row(y + 1, dn);
       T ncl = 0, ncr = 0;
       if (y + 1 < yb) {
-        if (lx == 0) ncl = edge(x0 - 1, y + 1, in_xl);
-        if (hxh) ncr = edge(x0 + cnt, y + 1, in_xh);
+        if (FULL) {
+          ncl = edge(x0 - 1, y + 1, in_xl);
+          ncr = edge(x0 + cnt, y + 1, in_xh);
+        } else {
+          if (lx == 0) ncl = edge(x0 - 1, y + 1, in_xl);
+          if (hxh) ncr = edge(x0 + cnt, y + 1, in_xh);
+        }
       }
       const T fromleft = __shfl_up(cur[V - 1], 1, kWave), fromright = __shfl_down(cur[0], 1, kWave);
       T v[V];
@@ -297,7 +314,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_jacobi_chain(JacPtrs
         v[j] = (T)0.25f * (xp + xm + up[j] + dn[j]);
         if (j < n) err[0] = RedOp<T, OPS_MAX>::combine(err[0], fabs(v[j] - cur[j]));
       }
-      if (n > 0) {
+      if (FULL || n > 0) {
         stv<T, V, false>(c.A.at(x, y, z), n, cur);
         stv<T, V, false>(c.out.at(x, y, z), n, v);
       }
@@ -327,7 +344,9 @@ void launch_jacobi_chain(const LoopRec &cp, const LoopRec &jac, const Range3 &R)
   c.An = ptr_of<const T>(An);
   const RedCtl ctl = with_scratch(jac.ctl);
   run_loop("jacobi_chain(copy+apply_stencil)", cp.bytes + jac.bytes, [&](hipStream_t st) {
-    hipLaunchKernelGGL((k_jacobi_chain<T, V>), dim3(gx, gy, 1), dim3(kWave, kWavesPerBlock), 0, st, c, R, ctl);
+    // every tile whole: the branch-free form
+    auto kern = (R.x1 - R.x0) % (kWave * V) == 0 ? k_jacobi_chain<T, V, true> : k_jacobi_chain<T, V, false>;
+    hipLaunchKernelGGL(kern, dim3(gx, gy, 1), dim3(kWave, kWavesPerBlock), 0, st, c, R, ctl);
     launch_red_finish<T, OPS_MAX, 1>(ctl, gx * gy, st);
   });
   swap_spare(An, R);
