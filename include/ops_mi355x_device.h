@@ -196,10 +196,19 @@ __device__ __forceinline__ void for_each_point(const Loop &L, F &&f) {
   const unsigned ty = rest % gridDim.y, tz = rest / gridDim.y;
   const int x = L.x0 + (int)tx * kWave * XP + (int)threadIdx.x;
   if (x >= L.x1) return;
+  // a lane whose points all lie in the range calls f for them with no branch
+  // in between (a branch join makes the compiler wait for every load in
+  // flight, which would serialise the points)
+  const bool whole = x + (XP - 1) * kWave < L.x1;
   auto row = [&](int y, int z) {
+    if (whole) {
 #pragma unroll
-    for (int k = 0; k < XP; k++)
-      if (x + k * kWave < L.x1) f(x + k * kWave, y, z);
+      for (int k = 0; k < XP; k++) f(x + k * kWave, y, z);
+    } else {
+#pragma unroll
+      for (int k = 0; k < XP; k++)
+        if (x + k * kWave < L.x1) f(x + k * kWave, y, z);
+    }
   };
   if (L.three_d) {
     const int y = L.y0 + (int)ty * kRows + (int)threadIdx.y;
