@@ -256,10 +256,10 @@ struct JacPtrs {
 };
 
 // FULL: every tile is 64 V points wide inside the range (every lane has V
-// points): the element counts are literals and the edge values are loaded
-// by every lane (one broadcast address) and kept by the edge lane, so a row
-// has no lane-divergent branches (a branch join makes the compiler wait for
-// every load in flight)
+// points): the element counts are literals, the edge values are loaded by
+// every lane (one broadcast address) and kept by the edge lane, so a row has
+// no lane-divergent branches (a branch join makes the compiler wait for
+// every load in flight), and each row is loaded one row ahead of its use
 template <class T, int V, bool FULL>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_jacobi_chain(JacPtrs<T> c, Range3 R, RedCtl ctl) {
   T err[1] = {RedOp<T, OPS_MAX>::identity()};
@@ -281,29 +281,53 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_jacobi_chain(JacPtrs
   auto edge = [&](int xx, int yy, bool inx) -> T {
     return inx && yy >= R.y0 && yy < R.y1 ? c.An(xx, yy, z) : c.A(xx, yy, z);
   };
-  if (ya < yb) {
+  if (FULL && ya < yb) {
+    // rows y + 1 and y + 2 are in registers while row y is computed (the
+    // load of row y + 2 is issued one row ahead; rows past yb read row yb,
+    // a row of the range or its halo)
+    T up[V], cur[V], dn[V], dn2[V];
+    row(ya - 1, up);
+    row(ya, cur);
+    row(ya + 1, dn);
+    T cl = edge(x0 - 1, ya, in_xl), cr = edge(x0 + cnt, ya, in_xh);
+    for (int y = ya; y < yb; y++) {
+      row(min(y + 2, yb), dn2);
+      T ncl = 0, ncr = 0;
+      if (y + 1 < yb) {
+        ncl = edge(x0 - 1, y + 1, in_xl);
+        ncr = edge(x0 + cnt, y + 1, in_xh);
+      }
+      const T fromleft = __shfl_up(cur[V - 1], 1, kWave), fromright = __shfl_down(cur[0], 1, kWave);
+      T v[V];
+#pragma unroll
+      for (int j = 0; j < V; j++) {
+        const T xm = j > 0 ? cur[j > 0 ? j - 1 : 0] : (lx == 0 ? cl : fromleft);
+        const T xp = j + 1 < V ? cur[j + 1 < V ? j + 1 : j] : (hxh ? cr : fromright);
+        v[j] = (T)0.25f * (xp + xm + up[j] + dn[j]);
+        err[0] = RedOp<T, OPS_MAX>::combine(err[0], fabs(v[j] - cur[j]));
+      }
+      stv<T, V, false>(c.A.at(x, y, z), V, cur);
+      stv<T, V, false>(c.out.at(x, y, z), V, v);
+#pragma unroll
+      for (int j = 0; j < V; j++) {
+        up[j] = cur[j];
+        cur[j] = dn[j];
+        dn[j] = dn2[j];
+      }
+      cl = ncl;
+      cr = ncr;
+    }
+  } else if (ya < yb) {
     T up[V], cur[V], dn[V];
     row(ya - 1, up);
     row(ya, cur);
-    T cl, cr;
-    if (FULL) {
-      cl = edge(x0 - 1, ya, in_xl);
-      cr = edge(x0 + cnt, ya, in_xh);
-    } else {
-      cl = lx == 0 ? edge(x0 - 1, ya, in_xl) : (T)0;
-      cr = hxh ? edge(x0 + cnt, ya, in_xh) : (T)0;
-    }
+    T cl = lx == 0 ? edge(x0 - 1, ya, in_xl) : (T)0, cr = hxh ? edge(x0 + cnt, ya, in_xh) : (T)0;
     for (int y = ya; y < yb; y++) {
       row(y + 1, dn);
       T ncl = 0, ncr = 0;
       if (y + 1 < yb) {
-        if (FULL) {
-          ncl = edge(x0 - 1, y + 1, in_xl);
-          ncr = edge(x0 + cnt, y + 1, in_xh);
-        } else {
-          if (lx == 0) ncl = edge(x0 - 1, y + 1, in_xl);
-          if (hxh) ncr = edge(x0 + cnt, y + 1, in_xh);
-        }
+        if (lx == 0) ncl = edge(x0 - 1, y + 1, in_xl);
+        if (hxh) ncr = edge(x0 + cnt, y + 1, in_xh);
       }
       const T fromleft = __shfl_up(cur[V - 1], 1, kWave), fromright = __shfl_down(cur[0], 1, kWave);
       T v[V];
@@ -314,7 +338,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_jacobi_chain(JacPtrs
         v[j] = (T)0.25f * (xp + xm + up[j] + dn[j]);
         if (j < n) err[0] = RedOp<T, OPS_MAX>::combine(err[0], fabs(v[j] - cur[j]));
       }
-      if (FULL || n > 0) {
+      if (n > 0) {
         stv<T, V, false>(c.A.at(x, y, z), n, cur);
         stv<T, V, false>(c.out.at(x, y, z), n, v);
       }
