@@ -83,6 +83,25 @@ void ops_mi_gen_par_loop(const char *name, ops_block block, int dim, const int *
  * not dim values of `type`. */
 void ops_mi_gen_const(const char *name, const char *type, int dim, void *dst);
 
+/* launches one kernel that runs, at each point, the first loop's kernel and
+ * then the second's (L1, L2 share range and tiling) */
+typedef void (*opsmi_gen_fused_fn)(const opsmi_gen_launch *L1, const void *blob1, const opsmi_gen_launch *L2,
+                                   const void *blob2, void *stream);
+
+/* Registers `fused` as the one-pass form of a loop launched by `first`
+ * directly followed by a loop launched by `second` (no counterpart in the
+ * reference: its lazy queue, ops/c/src/core/ops_lazy.cpp:380-1187, tiles
+ * loop chains but launches each loop separately).  The translator registers
+ * one per pair of ops_par_loop calls that follow each other in an
+ * application's source.  When the queue runs two such loops back to back it
+ * launches the fused form if running them point by point gives what running
+ * them one after the other gives: the same block, range and dimension, no
+ * shared reduction handle, every dat that either loop writes accessed by
+ * both only at the point itself (listed points, offset 0), and fusion
+ * enabled (ops_mi_set_fusion level >= 1).  Reductions and results are the
+ * same bits either way. */
+void ops_mi_gen_fused(opsmi_gen_launch_fn first, opsmi_gen_launch_fn second, opsmi_gen_fused_fn fused);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

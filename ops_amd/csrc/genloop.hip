@@ -11,6 +11,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <map>
+#include <memory>
+
 #include "engine.h"
 
 namespace opsmi {
@@ -51,9 +54,132 @@ struct GenRed {
   int arg;
   DType dt;
   int op, dim;
-  size_t offset;  // bytes into the partial buffer
+  size_t offset;  // bytes into the loop's share of the partial buffer
   RedCtl ctl;
 };
+
+}  // namespace
+
+// A generated loop as queued: what its launch function is given when it
+// runs, alone or as the second half of a fused pair.
+struct GenLoop {
+  opsmi_gen_launch_fn fn = nullptr;
+  ops_block block = nullptr;
+  Range3 r{};
+  int dims = 0;
+  Tiling t{};
+  int alias = 0;
+  std::vector<char> data;
+  std::vector<ops_dat> dats;  // by argument (nullptr for other kinds)
+  std::vector<int> acc;       // access mode by argument
+  std::vector<char> at_point; // dat argument touched only at offset 0 (listed points)
+  std::vector<GenRed> reds;
+  size_t pbytes = 0;          // partial buffer bytes the reductions take
+
+  unsigned parts() const { return t.blocks(); }
+  // the launch descriptor with reductions at pbase
+  void fill(opsmi_gen_launch &L, char *pbase) const {
+    memset(&L, 0, sizeof(L));
+    const int rr[6] = {r.x0, r.x1, r.y0, r.y1, r.z0, r.z1};
+    memcpy(L.range, rr, sizeof(rr));
+    L.chunk = t.chunk;
+    L.three_d = t.three_d ? 1 : 0;
+    L.grid[0] = t.grid.x;
+    L.grid[1] = t.grid.y;
+    L.grid[2] = t.grid.z;
+    L.alias = alias;
+    for (size_t i = 0; i < dats.size(); i++)
+      if (dats[i]) {
+        const DatView v = view_of(dats[i]);
+        L.dat[i].base = v.base;
+        L.dat[i].md = dats[i]->dim;
+        L.dat[i].sy = v.sy;
+        L.dat[i].sz = v.sz;
+      }
+    for (const GenRed &g : reds) L.partials[g.arg] = pbase + g.offset;
+  }
+  const void *blob() const { return data.empty() ? nullptr : data.data(); }
+  // folds each reduction argument's partials into its handle
+  void finish_reds(char *pbase, hipStream_t s) const {
+    for (const GenRed &g : reds) {
+      RedCtl c = with_scratch(g.ctl);
+      c.partials = pbase + g.offset;
+      finish(g.dt, c, g.op, g.dim, parts(), s);
+    }
+  }
+  bool writes(ops_dat d) const {
+    for (size_t i = 0; i < dats.size(); i++)
+      if (dats[i] == d && acc[i] != OPS_READ) return true;
+    return false;
+  }
+  // every argument on d touches it only at the point itself
+  bool only_at_point(ops_dat d) const {
+    for (size_t i = 0; i < dats.size(); i++)
+      if (dats[i] == d && !at_point[i]) return false;
+    return true;
+  }
+};
+
+namespace {
+
+std::map<std::pair<opsmi_gen_launch_fn, opsmi_gen_launch_fn>, opsmi_gen_fused_fn> &fused_table() {
+  static std::map<std::pair<opsmi_gen_launch_fn, opsmi_gen_launch_fn>, opsmi_gen_fused_fn> m;
+  return m;
+}
+
+bool same_range(const Range3 &a, const Range3 &b) {
+  return a.x0 == b.x0 && a.x1 == b.x1 && a.y0 == b.y0 && a.y1 == b.y1 && a.z0 == b.z0 && a.z1 == b.z1;
+}
+
+bool share_handle(const GenLoop &a, const GenLoop &b) {
+  for (const GenRed &x : a.reds)
+    for (const GenRed &y : b.reds)
+      if (x.ctl.slot == y.ctl.slot) return true;
+  return false;
+}
+
+// Two generated loops back to back become one pass when running them point
+// by point equals running them in turn: every dat either writes is touched
+// by both only at the point itself, so no point reads what another point's
+// second loop writes, or reads before another point's first loop writes.
+int gen_pair_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
+  if (at + 1 >= run.size() || !run[at].gen || !run[at + 1].gen) return 0;
+  const GenLoop &A = *run[at].gen, &B = *run[at + 1].gen;
+  const auto it = fused_table().find({A.fn, B.fn});
+  if (it == fused_table().end()) return 0;
+  if (A.block != B.block || A.dims != B.dims || !same_range(A.r, B.r) || A.r.npts() <= 0) return 0;
+  if (share_handle(A, B)) return 0;
+  for (const GenLoop *W : {&A, &B})
+    for (size_t i = 0; i < W->dats.size(); i++) {
+      const ops_dat d = W->dats[i];
+      if (d && W->acc[i] != OPS_READ && !(A.only_at_point(d) && B.only_at_point(d))) return 0;
+    }
+  const opsmi_gen_fused_fn fused = it->second;
+  const size_t off_b = A.pbytes;
+  ensure_partials(off_b + B.pbytes);
+  LoopRec &ra = run[at], &rb = run[at + 1];
+  const std::string name = "fused(" + ra.name + "+" + rb.name + ")";
+  run_loop(name.c_str(), ra.bytes + rb.bytes, [&](hipStream_t st) {
+    char *pbase = (char *)inst().d_partials;
+    opsmi_gen_launch La, Lb;
+    A.fill(La, pbase);
+    B.fill(Lb, pbase + off_b);
+    fused(&La, A.blob(), &Lb, B.blob(), (void *)st);
+    OPSMI_HIP(hipGetLastError());
+    A.finish_reds(pbase, st);
+    B.finish_reds(pbase + off_b, st);
+  });
+  (void)s;
+  note_fused(ra.name.c_str(), ra.bytes);
+  note_fused(rb.name.c_str(), rb.bytes);
+  for (const LoopRec *L : {&ra, &rb})
+    for (ops_dat d : L->writes) note_external_write(d);
+  return 2;
+}
+
+struct RegisterGen {
+  RegisterGen() { register_fusion(gen_pair_rule); }
+} g_register_gen;
 
 }  // namespace
 }  // namespace opsmi
@@ -79,13 +205,17 @@ void ops_mi_gen_par_loop(const char *name, ops_block block, int dim, const int *
     rec.bytes = 0.0;
     rec.launch = [](hipStream_t) {};
   } else {
+    auto G = std::make_shared<GenLoop>();
+    G->fn = fn;
+    G->block = block;
+    G->r = r;
+    G->dims = dim;
     // the hand-written engine's tiling with rows of 64 * OPSMI_GEN_XP
     // points: 3D one plane of 4 rows per workgroup, 2D four groups of four
     // rows
-    Tiling t = make_tiling(r, r.z1 - r.z0 > 1 ? gen_zchunk() : 4);
-    t.grid.x = (unsigned)(((long)r.x1 - r.x0 + kWave * OPSMI_GEN_XP - 1) / (kWave * OPSMI_GEN_XP));
-    const unsigned parts = t.blocks();
-    std::vector<GenRed> reds;
+    G->t = make_tiling(r, r.z1 - r.z0 > 1 ? gen_zchunk() : 4);
+    G->t.grid.x = (unsigned)(((long)r.x1 - r.x0 + kWave * OPSMI_GEN_XP - 1) / (kWave * OPSMI_GEN_XP));
+    const unsigned parts = G->parts();
     size_t off = 0;
     for (int i = 0; i < nargs; i++) {
       const ops_arg &a = args[i];
@@ -99,50 +229,51 @@ void ops_mi_gen_par_loop(const char *name, ops_block block, int dim, const int *
       // handles are armed in argument order, when the loop is called
       g.ctl = red_prepare(a, a.dim, g.dt, 0);
       off += ((size_t)parts * a.dim * 8 + 255) / 256 * 256;
-      reds.push_back(g);
+      G->reds.push_back(g);
     }
+    G->pbytes = off;
     ensure_partials(off);
-    if (!reds.empty()) rec.ctl = reds[0].ctl;
-    std::vector<char> data((const char *)blob, (const char *)blob + blob_bytes);
-    std::vector<ops_dat> dats((size_t)nargs, nullptr);
-    for (int i = 0; i < nargs; i++)
-      if (args[i].argtype == OPS_ARG_DAT) dats[i] = args[i].dat;
-    int alias = 0;
+    if (!G->reds.empty()) rec.ctl = G->reds[0].ctl;
+    G->data.assign((const char *)blob, (const char *)blob + blob_bytes);
+    G->dats.assign((size_t)nargs, nullptr);
+    G->acc.assign((size_t)nargs, 0);
+    G->at_point.assign((size_t)nargs, 0);
+    const int *pt = pts;
+    for (int i = 0; i < nargs; i++) {
+      const int np = npts[i];
+      if (args[i].argtype == OPS_ARG_DAT) {
+        G->dats[i] = args[i].dat;
+        G->acc[i] = args[i].acc;
+        bool zero = np >= 0;
+        for (int q = 0; q < np; q++)
+          if (pt[4 * q + 1] != 0 || pt[4 * q + 2] != 0 || pt[4 * q + 3] != 0) zero = false;
+        G->at_point[i] = zero;
+        if (np > 0) pt += 4 * np;
+      }
+    }
     for (int i = 0; i < nargs; i++)
       for (int j = i + 1; j < nargs; j++)
-        if (dats[i] && dats[i] == dats[j]) alias = 1;
-    rec.launch = [=](hipStream_t s) {
-      opsmi_gen_launch L;
-      memset(&L, 0, sizeof(L));
-      const int rr[6] = {r.x0, r.x1, r.y0, r.y1, r.z0, r.z1};
-      memcpy(L.range, rr, sizeof(rr));
-      L.chunk = t.chunk;
-      L.three_d = t.three_d ? 1 : 0;
-      L.grid[0] = t.grid.x;
-      L.grid[1] = t.grid.y;
-      L.grid[2] = t.grid.z;
-      L.alias = alias;
-      for (size_t i = 0; i < dats.size(); i++)
-        if (dats[i]) {
-          const DatView v = view_of(dats[i]);
-          L.dat[i].base = v.base;
-          L.dat[i].md = dats[i]->dim;
-          L.dat[i].sy = v.sy;
-          L.dat[i].sz = v.sz;
-        }
+        if (G->dats[i] && G->dats[i] == G->dats[j]) G->alias = 1;
+    rec.gen = G;
+    const GenLoop *g = G.get();
+    rec.launch = [G, g](hipStream_t s) {
       char *pbase = (char *)inst().d_partials;
-      for (const GenRed &g : reds) L.partials[g.arg] = pbase + g.offset;
-      fn(&L, data.empty() ? nullptr : data.data(), (void *)s);
+      opsmi_gen_launch L;
+      g->fill(L, pbase);
+      g->fn(&L, g->blob(), (void *)s);
       OPSMI_HIP(hipGetLastError());
-      for (const GenRed &g : reds) {
-        RedCtl c = with_scratch(g.ctl);
-        c.partials = pbase + g.offset;
-        finish(g.dt, c, g.op, g.dim, parts, s);
-      }
+      g->finish_reds(pbase, s);
     };
   }
   enqueue(std::move(rec));
   finish_loop(args, nargs);
+  OPSMI_API_END()
+}
+
+void ops_mi_gen_fused(opsmi_gen_launch_fn first, opsmi_gen_launch_fn second, opsmi_gen_fused_fn fused) {
+  OPSMI_API_BEGIN
+  if (!first || !second || !fused) fail(OPS_INVALID_ARGUMENT, "ops_mi_gen_fused: NULL function");
+  fused_table()[{first, second}] = fused;
   OPSMI_API_END()
 }
 

@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <functional>
+#include <memory>
 #include <map>
 #include <sstream>
 #include <string>
@@ -255,8 +256,10 @@ enum LoopKind {
   LK_COPY,       // copy: d[0] = d[1]; g[0] = element bytes
   LK_JACOBI5,    // apply_stencil: d[1] = 5-point average of d[0], MAX |change|; g[0] = element bytes
 };
+struct GenLoop;  // a translator-generated loop's launch data (genloop.hip)
 struct LoopRec {
   std::string name;
+  std::shared_ptr<const GenLoop> gen;  // set for a generated loop
   LoopKind kind = LK_OPAQUE;
   Range3 range{};
   int dims = 0;

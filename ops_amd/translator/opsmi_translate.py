@@ -362,32 +362,13 @@ def kernel_code(loop, kern, consts):
         ptrs = "".join(f", {'const ' if loop.args[i].acc == 'OPS_READ' else ''}{loop.args[i].typ} *{qual}p{i}"
                        for i in dats)
         L.append(f"__global__ __launch_bounds__(256) void opsg_{K}{suffix}(const opsa_{K} A{ptrs}) {{")
-        for i, a in enumerate(loop.args):
-            if a.kind == "reduce":
-                L.append(f"  {a.typ} red{i}[{a.dim}];")
-                L.append(f"  for (int q = 0; q < {a.dim}; q++) red{i}[q] = ::opsmi_gen::ident<{a.typ}, {a.acc}>();")
+        L.extend(red_init_lines(loop, ""))
         L.append("  ::opsmi_gen::for_each_point(A.L, [&](int x, int y, int z) {")
-        call = []
-        for i, a in enumerate(loop.args):
-            if a.kind == "dat":
-                L.append(f"    ::opsmi_gen::Acc<{a.typ}, {loop.ndim}, {'true' if a.multi else 'false'}> "
-                         f"a{i}(p{i}, A.d{i}, x, y, z);")
-                call.append(f"a{i}")
-            elif a.kind == "idx":
-                L.append(f"    int idx{i}[3] = {{x, y, z}};")
-                call.append(f"idx{i}")
-            elif a.kind == "gbl":
-                call.append(f"A.b.g{i}")
-            else:
-                call.append(f"red{i}")
-        L.append(f"    A.b.k({', '.join(call)});")
+        L.extend(point_lines(loop, "A", ""))
         L.append("  });")
-        for i, a in enumerate(loop.args):
-            if a.kind == "reduce":
-                L.append(f"  ::opsmi_gen::store_partials<{a.typ}, {a.acc}, {a.dim}>(red{i}, A.r{i});")
+        L.extend(red_store_lines(loop, "A", ""))
         L.append("}")
-    L.append(f"void opsl_{K}(const opsmi_gen_launch *L, const void *blob, void *stream) {{")
-    L.append(f"  opsa_{K} A;")
+    L.append(f"void opsfill_{K}(opsa_{K} &A, const opsmi_gen_launch *L, const void *blob) {{")
     L.append("  A.L = ::opsmi_gen::loop_of(L);")
     for i, a in enumerate(loop.args):
         if a.kind == "dat":
@@ -395,12 +376,98 @@ def kernel_code(loop, kern, consts):
         elif a.kind == "reduce":
             L.append(f"  A.r{i} = ({a.typ} *)L->partials[{i}];")
     L.append("  memcpy(&A.b, blob, sizeof(A.b));")
-    ptrs = "".join(f", ({loop.args[i].typ} *)L->dat[{i}].base" for i in dats)
+    L.append("}")
+    L.append(f"void opsl_{K}(const opsmi_gen_launch *L, const void *blob, void *stream) {{")
+    L.append(f"  opsa_{K} A;")
+    L.append(f"  opsfill_{K}(A, L, blob);")
+    ptrs = launch_ptrs(loop, "L")
     L.append(f"  auto kern = L->alias ? opsg_{K}_alias : opsg_{K};")
     L.append(f"  hipLaunchKernelGGL(kern, dim3(L->grid[0], L->grid[1], L->grid[2]), "
              f"dim3(::opsmi_gen::kWave, ::opsmi_gen::kRows), 0, (hipStream_t)stream, A{ptrs});")
     L.append("}")
     return "\n".join(L), used, points
+
+
+def red_init_lines(loop, pfx):
+    L = []
+    for i, a in enumerate(loop.args):
+        if a.kind == "reduce":
+            L.append(f"  {a.typ} {pfx}red{i}[{a.dim}];")
+            L.append(f"  for (int q = 0; q < {a.dim}; q++) {pfx}red{i}[q] = ::opsmi_gen::ident<{a.typ}, {a.acc}>();")
+    return L
+
+
+def point_lines(loop, S, pfx):
+    """Accessors of loop's arguments at point (x, y, z) over pointers
+    {pfx}p<i>, then the call of its kernel (member k of {S}.b)."""
+    L, call = [], []
+    for i, a in enumerate(loop.args):
+        if a.kind == "dat":
+            L.append(f"    ::opsmi_gen::Acc<{a.typ}, {loop.ndim}, {'true' if a.multi else 'false'}> "
+                     f"{pfx}a{i}({pfx}p{i}, {S}.d{i}, x, y, z);")
+            call.append(f"{pfx}a{i}")
+        elif a.kind == "idx":
+            L.append(f"    int {pfx}idx{i}[3] = {{x, y, z}};")
+            call.append(f"{pfx}idx{i}")
+        elif a.kind == "gbl":
+            call.append(f"{S}.b.g{i}")
+        else:
+            call.append(f"{pfx}red{i}")
+    L.append(f"    {S}.b.k({', '.join(call)});")
+    return L
+
+
+def red_store_lines(loop, S, pfx):
+    return [f"  ::opsmi_gen::store_partials<{a.typ}, {a.acc}, {a.dim}>({pfx}red{i}, {S}.r{i});"
+            for i, a in enumerate(loop.args) if a.kind == "reduce"]
+
+
+def launch_ptrs(loop, Lname):
+    return "".join(f", ({a.typ} *){Lname}->dat[{i}].base" for i, a in enumerate(loop.args) if a.kind == "dat")
+
+
+def fused_code(l1, l2):
+    """One kernel running l1's kernel and then l2's at every point, and its
+    launch function (registered for the pair; the library decides per call
+    whether the two queued loops may run this way)."""
+    K1, K2 = l1.kernel, l2.kernel
+    F = f"{K1}__{K2}"
+    ptrs = []
+    for pfx, lp in (("f", l1), ("s", l2)):
+        ptrs += [f"{'const ' if a.acc == 'OPS_READ' else ''}{a.typ} *{pfx}p{i}"
+                 for i, a in enumerate(lp.args) if a.kind == "dat"]
+    L = [f"// ---------------------------------------------------------------- {K1} then {K2}",
+         "// (plain pointers: the two loops may share dats)",
+         f"__global__ __launch_bounds__(256) void opsg_fuse_{F}(const opsa_{K1} A, const opsa_{K2} B"
+         f"{''.join(', ' + p for p in ptrs)}) {{"]
+    L += red_init_lines(l1, "f") + red_init_lines(l2, "s")
+    L.append("  ::opsmi_gen::for_each_point(A.L, [&](int x, int y, int z) {")
+    L += point_lines(l1, "A", "f") + point_lines(l2, "B", "s")
+    L.append("  });")
+    L += red_store_lines(l1, "A", "f") + red_store_lines(l2, "B", "s")
+    L.append("}")
+    L.append(f"void opsf_{F}(const opsmi_gen_launch *L1, const void *blob1, const opsmi_gen_launch *L2, "
+             "const void *blob2, void *stream) {")
+    L.append(f"  opsa_{K1} A;")
+    L.append(f"  opsfill_{K1}(A, L1, blob1);")
+    L.append(f"  opsa_{K2} B;")
+    L.append(f"  opsfill_{K2}(B, L2, blob2);")
+    L.append(f"  hipLaunchKernelGGL(opsg_fuse_{F}, dim3(L1->grid[0], L1->grid[1], L1->grid[2]), "
+             f"dim3(::opsmi_gen::kWave, ::opsmi_gen::kRows), 0, (hipStream_t)stream, A, B"
+             f"{launch_ptrs(l1, 'L1')}{launch_ptrs(l2, 'L2')});")
+    L.append("}")
+    return "\n".join(L), f"ops_mi_gen_fused(&opsl_{K1}, &opsl_{K2}, &opsf_{F});"
+
+
+def adjacent_pairs(loops):
+    """(first, second) loops of each pair of ops_par_loop calls that follow
+    each other in the source, once per kernel pair, same dimension only."""
+    seen, out = set(), []
+    for a, b in zip(loops, loops[1:]):
+        if a.ndim == b.ndim and (a.kernel, b.kernel) not in seen:
+            seen.add((a.kernel, b.kernel))
+            out.append((a, b))
+    return out
 
 
 def stub_code(loop, used, consts, points):
@@ -479,6 +546,17 @@ def translate(app, out_dir, inc_dirs=()):
         s, sig = stub_code(lp, used, consts, points)
         stubs.append(s)
         sigs.append(sig)
+    regs = []
+    for a, b in adjacent_pairs(loops):
+        code, reg = fused_code(kernels[a.kernel][0], kernels[b.kernel][0])
+        H.append(code)
+        regs.append(reg)
+    if regs:
+        H.append("struct opsr_fused_pairs {")
+        H.append("  opsr_fused_pairs() {")
+        H += [f"    {r}" for r in regs]
+        H.append("  }")
+        H.append("} opsr_fused_pairs_;")
     H.append("}  // namespace")
     H.append("")
     H.extend(stubs)

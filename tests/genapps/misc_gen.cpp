@@ -109,3 +109,32 @@ extern "C" int gen_double_in_place(ops_block block, ops_dat d, int nx, int ny, i
                ops_arg_dat(d, 1, S0, "double", OPS_READ));
   return 0;
 }
+
+void cg_axpy(ACC<double> &u, const ACC<double> &p, const double *a) {
+  u(0, 0, 0) = u(0, 0, 0) + (*a) * p(0, 0, 0);
+}
+
+void cg_axpby_dot(ACC<double> &p, const ACC<double> &r, const double *b, double *pp) {
+  p(0, 0, 0) = r(0, 0, 0) + (*b) * p(0, 0, 0);
+  *pp = *pp + p(0, 0, 0) * p(0, 0, 0);
+}
+
+// The tail of a CG iteration, `iters` times over the interior: u = u + a p,
+// then p = r + b p with p.p into pp[it] (two loops on shared dats that touch
+// them only at the point itself, so the queue may run them as one pass)
+extern "C" int gen_cg_tail(ops_block block, ops_dat u, ops_dat p, ops_dat r, int nx, int ny, int nz, double a,
+                           double b, int iters, double *pp) {
+  int s0[] = {0, 0, 0};
+  ops_stencil S0 = ops_decl_stencil(3, 1, s0, "t0");
+  ops_reduction h = ops_decl_reduction_handle(sizeof(double), "double", "pp");
+  int inner[] = {0, nx, 0, ny, 0, nz};
+  for (int it = 0; it < iters; it++) {
+    ops_par_loop(cg_axpy, "cg_axpy", block, 3, inner, ops_arg_dat(u, 1, S0, "double", OPS_RW),
+                 ops_arg_dat(p, 1, S0, "double", OPS_READ), ops_arg_gbl(&a, 1, "double", OPS_READ));
+    ops_par_loop(cg_axpby_dot, "cg_axpby_dot", block, 3, inner, ops_arg_dat(p, 1, S0, "double", OPS_RW),
+                 ops_arg_dat(r, 1, S0, "double", OPS_READ), ops_arg_gbl(&b, 1, "double", OPS_READ),
+                 ops_arg_reduce(h, 1, "double", OPS_INC));
+    ops_reduction_result(h, &pp[it]);
+  }
+  return 0;
+}

@@ -36,6 +36,7 @@ def gen(gpu_ops):
     L.gen_opt_add.argtypes = [vp, vp, vp, ci, ci, ci, ci]
     L.gen_fill2d.argtypes = [vp, vp, ci, ci, cd]
     L.gen_double_in_place.argtypes = [vp, vp, ci, ci, ci]
+    L.gen_cg_tail.argtypes = [vp, vp, vp, vp, ci, ci, ci, cd, cd, ci, dp]
     return L
 
 
@@ -254,3 +255,73 @@ def test_generated_and_hand_written_loops_share_the_queue(gpu_ops, gen):
     assert total(a, "sa2") == nx * ny
     assert np.array_equal(ops.ops_dat_fetch_data(b), (-2.0 + X) - 0.5 * Y)
     assert ops.ops_mi_loop_stats("fill2d")[0] == 3
+
+
+def _cg_tail(ops, gen, n, iters, fusion):
+    nx, ny, nz = n
+    blk = ops.ops_decl_block(3, "gentail")
+    ext = (nz + 2, ny + 2, nx + 2)
+    D = {}
+    for s, k in enumerate(["u", "p", "r"]):
+        D[k] = ops.ops_decl_dat(blk, [nx, ny, nz], name=k)
+        ops.set_whole(D[k], O.u01(57, s, int(np.prod(ext))).reshape(ext) - 0.5)
+    start = {k: ops.get_whole(D[k]) for k in D}
+    pp = np.zeros(iters)
+    ops.lib().ops_mi_set_fusion(fusion)
+    ops.ops_mi_reset_stats()
+    try:
+        gen.gen_cg_tail(blk.ptr, D["u"].ptr, D["p"].ptr, D["r"].ptr, nx, ny, nz, 0.37, -0.61, iters, _p(pp))
+        ops._check()
+        fused = ops.ops_mi_loop_stats("fused(cg_axpy+cg_axpby_dot)")
+    finally:
+        ops.lib().ops_mi_set_fusion(2)
+    return start, {k: ops.get_whole(D[k]) for k in D}, pp, fused
+
+
+@pytest.mark.parametrize("n", [(64, 20, 9), (131, 7, 5)])
+def test_generated_pair_runs_fused_and_bit_exact(gpu_ops, gen, n):
+    """axpy then axpby + dot: one fused pass per iteration, the same bits as
+    the two loops run one after the other, and numpy's values (fields
+    bit-exact, p.p within 1e-12 relative)."""
+    iters = 3
+    s0, fused, pp_f, st = _cg_tail(gpu_ops, gen, n, iters, 2)
+    _, apart, pp_a, st_a = _cg_tail(gpu_ops, gen, n, iters, 0)
+    assert st is not None and st[0] == iters
+    assert st_a is None or st_a[0] == 0
+    for k in ("u", "p", "r"):
+        assert np.array_equal(fused[k], apart[k]), k
+    assert np.array_equal(pp_f, pp_a)
+    u, p, r = (s0[k].copy() for k in ("u", "p", "r"))
+    i = (slice(1, -1),) * 3
+    for it in range(iters):
+        u[i] = u[i] + 0.37 * p[i]
+        p[i] = r[i] + -0.61 * p[i]
+        ref = np.sum(p[i] * p[i])
+        assert abs(pp_f[it] - ref) <= 1e-12 * abs(ref)
+    assert np.array_equal(fused["u"], u) and np.array_equal(fused["p"], p)
+    assert np.array_equal(fused["r"], s0["r"])
+
+
+def test_generated_pair_with_a_neighbour_read_is_not_fused(gpu_ops, gen):
+    """copy then apply_stencil: the second loop reads neighbours of what the
+    first writes, so the two run as separate passes (the results are the
+    Laplace test's, bit-exact against the oracle)."""
+    ops = gpu_ops
+    nx, ny = 64, 48
+    init = O.laplace_init(nx, ny, 0)
+    blk = ops.ops_decl_block(2, "genlapnf")
+    A = ops.ops_decl_dat(blk, [nx, ny], name="A")
+    An = ops.ops_decl_dat(blk, [nx, ny], name="Anew")
+    ops.set_ext(A, init)
+    ops.set_ext(An, init)
+    hist = np.zeros(4)
+    ops.ops_mi_reset_stats()
+    gen.gen_laplace(blk.ptr, A.ptr, An.ptr, nx, ny, 4, _p(hist))
+    ops._check()
+    for name in ("fused(gen copy+gen apply_stencil)", "fused(gen apply_stencil+gen copy)"):
+        st = ops.ops_mi_loop_stats(name)
+        assert st is None or st[0] == 0, name
+    assert ops.ops_mi_loop_stats("gen copy")[0] == 4 and ops.ops_mi_loop_stats("gen apply_stencil")[0] == 4
+    ref_A, ref_h = O.laplace_run(init, 4)
+    assert np.array_equal(hist, ref_h)
+    assert np.array_equal(ops.ops_dat_fetch_data(A), O.interior(ref_A))

@@ -37,6 +37,11 @@ def test_translates_dev_api_applications(tmp_path):
     assert re.search(r"struct opsk_fill_fields \{\s*int mult;\s*double shift;", hip)
     assert "::opsmi_gen::Acc<double, 2, true> &v" in hip and "::opsmi_gen::Acc<int, 2, false> &m" in hip
     assert "store_partials<int, OPS_INC, 1>" in hip and "store_partials<double, OPS_INC, 2>" in hip
+    # one fused kernel per pair of loops that follow each other in the source
+    assert "opsg_fuse_mix_fields__summary(const opsa_mix_fields A, const opsa_summary B" in hip
+    assert "ops_mi_gen_fused(&opsl_fill_fields, &opsl_mix_fields, &opsf_fill_fields__mix_fields);" in hip
+    assert "ops_mi_gen_fused(&opsl_mix_fields, &opsl_summary, &opsf_mix_fields__summary);" in hip
+    assert "opsf_summary__" not in hip
     decl = open(os.path.join(out, "heat3d_gen_opsmi_kernels.h")).read()
     assert 'visibility("hidden")' in decl and "ops_par_loop_heat_calc_w(" in decl
 
@@ -85,7 +90,7 @@ def test_generated_library_built():
     dyn = subprocess.run(["nm", "-D", lib], capture_output=True, text=True, check=True).stdout
     exported = set(re.findall(r" T (\w+)", dyn))
     assert {"gen_laplace", "gen_fields", "gen_heat_step", "gen_laplace_bad_stencil", "gen_ramp1d", "gen_vec3",
-            "gen_opt_add", "gen_fill2d", "gen_double_in_place"} <= exported
+            "gen_opt_add", "gen_fill2d", "gen_double_in_place", "gen_cg_tail"} <= exported
     # the generated stubs stay inside the library (no clash with the
     # hand-written ops_par_loop_apply_stencil of libopsmi.so)
     assert not any(n.startswith("ops_par_loop_") for n in exported)
