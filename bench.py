@@ -26,6 +26,9 @@ Other workloads (--workload):
                      sizes) on the batched engine: solves/s with all
                      requests queued, p50/p99 latency under Poisson arrivals,
                      against the per-solve FP64 path one solve at a time
+  gen_cg_tail_512    512^3 CG tail u = u + a p, then p = r + b p with p.p,
+                     written as two dev-API loops (tests/genapps/misc_gen.cpp)
+                     and run as the translator's fused pair (one pass, 40 B/pt)
 
 usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload heat3d_cg_512]
 """
@@ -323,8 +326,54 @@ def bench_serve(a, ops, apps):
                 iterations={"batched_total": int(iters.sum()), "per_solve_total": int(sum(it_base))})
 
 
+# ------------------------------------------------------ generated loops
+def bench_gen_tail(a, ops, apps):
+    """One step = one iteration of the CG tail through translator-generated
+    loops (libgenapps.so, built by __graft_entry__.build()): the queue runs
+    cg_axpy and cg_axpby_dot as one fused pass, then the p.p result is read."""
+    import ctypes as C
+
+    import numpy as np
+    lib = os.path.join(ROOT, "tests", "genapps", "libgenapps.so")
+    if not os.path.exists(lib):
+        raise RuntimeError(f"{lib} is not built; run __graft_entry__.build() first")
+    G = C.CDLL(lib)
+    dp = C.POINTER(C.c_double)
+    G.gen_cg_tail.argtypes = [C.c_void_p] * 4 + [C.c_int] * 3 + [C.c_double] * 2 + [C.c_int, dp]
+    n = a.size or 512
+    ops.ops_init(diags=2)
+    blk = ops.ops_decl_block(3, "gen_tail")
+    rng = np.random.default_rng(a.seed)
+    D = []
+    for k in ("u", "p", "r"):
+        d = ops.ops_decl_dat(blk, [n, n, n], name=k)
+        ops.set_whole(d, rng.random((n + 2, n + 2, n + 2)) - 0.5)
+        D.append(d)
+    pp = np.zeros(max(a.steps, a.warmup, 1))
+    args = [blk.ptr] + [d.ptr for d in D] + [n, n, n, 0.37, -0.61]
+    G.gen_cg_tail(*args, a.warmup, pp.ctypes.data_as(dp))
+    ops.ops_mi_sync()
+    ops.ops_mi_reset_stats()
+    t0 = time.perf_counter()
+    G.gen_cg_tail(*args, a.steps, pp.ctypes.data_as(dp))
+    ops.ops_mi_sync()
+    t1 = time.perf_counter()
+    ops._check()
+    ms = (t1 - t0) / a.steps * 1e3
+    pts = float(n) ** 3
+    # fused: reads u, p, r, writes u, p; apart: 24 B/pt each
+    bpp = {"fused(cg_axpy+cg_axpby_dot)": 40, "cg_axpy": 24, "cg_axpby_dot": 24}
+    kernels = kernel_table(ops, bpp, pts)
+    value = 40.0 * pts / (ms * 1e-3) / 1e9
+    return line(value, "GB/s", ms, a, "f64",
+                {"workload": f"gen_cg_tail_{n}", "grid": [n, n, n],
+                 "loops": "cg_axpy + cg_axpby_dot (translator-generated, fused pair)", "bytes_per_point": 40,
+                 "last_pp": float(pp[a.steps - 1])},
+                roofline=roofline(kernels, bpp, pts), kernels=kernels, data="synthetic (numpy PCG64, seeded)")
+
+
 WORKLOADS = {"heat3d_cg_512": bench_heat, "poisson_mixed_768": bench_mixed, "laplace2d_2048": bench_laplace,
-             "adi_trid_256": bench_trid, "serve_stream": bench_serve}
+             "adi_trid_256": bench_trid, "serve_stream": bench_serve, "gen_cg_tail_512": bench_gen_tail}
 
 
 def main():
