@@ -15,8 +15,12 @@ reductions, driven by the C++ app over the C ABI.
   library's stream around every launch.
   cpu_baseline = the oracle's OpenMP restatement of the reference CPU path on
   the same 512^3 problem, a few iterations on the host cores.
+  configs = the other benchmark configurations, measured in the same run
+  (--no-configs skips them): laplace2d_2048, adi_trid_256,
+  poisson_mixed_768 and serve_stream (60 requests), each with its own
+  roofline or speed-up figures.
 
-Other workloads (--workload):
+Workloads on their own (--workload):
   poisson_mixed_768  768^3 7-point Poisson to 1e-10 relative residual: FP64
                      CG against FP64 iterative refinement around FP32 PCG
                      (solves/s, iteration counts, error against FP64 CG)
@@ -153,6 +157,14 @@ def bench_heat(a, ops, apps):
                roofline=rl, kernels=kernels)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_heat(n, a.seed)
+    if not a.no_configs:
+        # the other configurations, measured in the same run (one GPU)
+        out["configs"] = {
+            "laplace2d_2048": measure_laplace(ops, apps, 2048),
+            "adi_trid_256": measure_trid(ops, apps, 256, a.seed),
+            "poisson_mixed_768": measure_mixed(ops, apps, 768, a.seed),
+            "serve_stream": measure_serve(ops, apps, 60, a.seed),
+        }
     return out
 
 
@@ -167,15 +179,16 @@ FP64_BPP = {"poisson_calc_w_reduce_kernel": 16, "tea_leaf_axpy_kernel": 24,
             "poisson_chain(axpy+axpby+calc_w)": 48}
 
 
-def bench_mixed(a, ops, apps):
+def measure_mixed(ops, apps, n, seed):
+    """768^3 Poisson to 1e-10: FP64 CG, then FP64 refinement around FP32 PCG,
+    each timed from resident inputs to the converged solution."""
     import numpy as np
 
-    n = a.size or 768
     tol = 1e-10
     pts = float(n) ** 3
-    ops.ops_init(diags=2)
+    ops.ops_mi_set_diags(2)
     # FP64 CG (x0 = 0, stop at |r|/|f| <= tol on the recursive residual)
-    P = apps.Poisson(3, n, n, n, seed=a.seed)
+    P = apps.Poisson(3, n, n, n, seed=seed)
     ops.ops_mi_sync()
     ops.ops_mi_reset_stats()
     t0 = time.perf_counter()
@@ -185,8 +198,8 @@ def bench_mixed(a, ops, apps):
     k64 = kernel_table(ops, FP64_BPP, pts)
     x64 = P.x()
     P.close()
-    print(f"fp64 CG: {it64} iterations, {t64:.2f} s", file=sys.stderr, flush=True)
-    M = apps.PoissonMixed(3, n, n, n, seed=a.seed)
+    print(f"poisson {n}^3 fp64 CG: {it64} iterations, {t64:.2f} s", file=sys.stderr, flush=True)
+    M = apps.PoissonMixed(3, n, n, n, seed=seed)
     ops.ops_mi_sync()
     ops.ops_mi_reset_stats()
     t0 = time.perf_counter()
@@ -196,25 +209,38 @@ def bench_mixed(a, ops, apps):
     kmp = kernel_table(ops, MIXED_BPP, pts)
     xm = M.x()
     M.close()
-    print(f"mixed: {res['outer']} outer, {res['inner_total']} inner, {tmp:.2f} s", file=sys.stderr, flush=True)
+    print(f"poisson {n}^3 mixed: {res['outer']} outer, {res['inner_total']} inner, {tmp:.2f} s", file=sys.stderr,
+          flush=True)
     err = float(np.max(np.abs(xm - x64)) / np.max(np.abs(x64)))
     del xm, x64
-    value = 1.0 / tmp
-    return line(value, "solves/s", tmp * 1e3, a, "f32 sweeps, f64 reductions/residual/correction",
-                {"workload": f"poisson_mixed_{n}", "grid": [n, n, n], "operator": "7-point constant coefficient",
-                 "tol": tol},
-                fp64_cg={"iterations": it64, "seconds": round(t64, 3), "kernels": k64},
-                mixed={"outer": res["outer"], "inner_total": res["inner_total"],
-                       "relres": [float(v) for v in res["relres"]], "seconds": round(tmp, 3), "kernels": kmp},
-                speedup_vs_fp64=round(t64 / tmp, 3), max_rel_diff_vs_fp64_cg=err,
-                roofline=roofline(kmp, MIXED_BPP, pts))
+    return {"workload": f"poisson_mixed_{n}", "grid": [n, n, n], "tol": tol,
+            "fp64_cg": {"iterations": it64, "seconds": round(t64, 3), "solves_per_s": round(1.0 / t64, 4),
+                        "roofline": roofline(k64, FP64_BPP, pts), "kernels": k64},
+            "mixed": {"outer": res["outer"], "inner_total": res["inner_total"],
+                      "relres": [float(v) for v in res["relres"]], "seconds": round(tmp, 3),
+                      "solves_per_s": round(1.0 / tmp, 4), "roofline": roofline(kmp, MIXED_BPP, pts),
+                      "kernels": kmp},
+            "speedup_vs_fp64": round(t64 / tmp, 3), "max_rel_diff_vs_fp64_cg": err}
+
+
+def bench_mixed(a, ops, apps):
+    n = a.size or 768
+    ops.ops_init(diags=2)
+    m = measure_mixed(ops, apps, n, a.seed)
+    tmp = m["mixed"]["seconds"]
+    return line(1.0 / tmp, "solves/s", tmp * 1e3, a, "f32 sweeps, f64 reductions/residual/correction",
+                {"workload": m["workload"], "grid": m["grid"], "operator": "7-point constant coefficient",
+                 "tol": m["tol"]},
+                fp64_cg=m["fp64_cg"], mixed=m["mixed"], speedup_vs_fp64=m["speedup_vs_fp64"],
+                max_rel_diff_vs_fp64_cg=m["max_rel_diff_vs_fp64_cg"], roofline=m["mixed"]["roofline"])
 
 
 # ------------------------------------------------------------------ Laplace
-def bench_laplace(a, ops, apps):
-    n = a.size or 2048
-    iters = 1000
-    ops.ops_init(diags=2)
+def measure_laplace(ops, apps, n, iters=1000):
+    """2048^2 Jacobi, 1000 iterations, each with its max-abs-diff result
+    (the tutorial's loop); per-iteration time from the difference of a
+    (10 + iters)- and a 10-iteration run, so set-up cancels."""
+    ops.ops_mi_set_diags(2)
     apps.laplace2d(n, n, 10, want_field=False)
     ops.ops_mi_sync()
     t0 = time.perf_counter()
@@ -230,33 +256,55 @@ def bench_laplace(a, ops, apps):
     # (reads Anew, writes A and the new Anew: 24 B/pt)
     bpp = {"apply stencil": 16, "copy": 16, "jacobi_chain(copy+apply_stencil)": 24}
     kernels = kernel_table(ops, bpp, pts)
-    value = 32.0 * pts / (ms * 1e-3) / 1e9
-    return line(value, "GB/s", ms, a, "f64",
-                {"workload": f"laplace2d_{n}", "grid": [n, n], "iterations": iters, "bytes_per_point": 32,
-                 "final_max_abs_diff": err},
-                roofline=roofline(kernels, bpp, pts), kernels=kernels)
+    return {"workload": f"laplace2d_{n}", "grid": [n, n], "iterations": iters, "bytes_per_point": 32,
+            "ms_per_iteration": round(ms, 5), "GB_s": round(32.0 * pts / (ms * 1e-3) / 1e9, 1),
+            "final_max_abs_diff": err, "roofline": roofline(kernels, bpp, pts), "kernels": kernels}
+
+
+def bench_laplace(a, ops, apps):
+    n = a.size or 2048
+    ops.ops_init(diags=2)
+    m = measure_laplace(ops, apps, n)
+    return line(m["GB_s"], "GB/s", m["ms_per_iteration"], a, "f64",
+                {k: m[k] for k in ("workload", "grid", "iterations", "bytes_per_point", "final_max_abs_diff")},
+                roofline=m["roofline"], kernels=m["kernels"])
 
 
 # --------------------------------------------------------------- tridiagonal
-def bench_trid(a, ops, apps):
-    n = a.size or 256
-    ops.ops_init(diags=2)
-    T = apps.Trid(n, n, n, seed=a.seed, strategy=0)
-    for _ in range(a.warmup):
+# algorithmic bytes per point: R a, b, c and RW d (x, y); R a, b, c, d and RW u (z, _Inc)
+TRID_BPP = {"ops_tridMultiDimBatch x": 40, "ops_tridMultiDimBatch y": 40, "ops_tridMultiDimBatch z": 40,
+            "ops_tridMultiDimBatch_Inc x": 48, "ops_tridMultiDimBatch_Inc y": 48, "ops_tridMultiDimBatch_Inc z": 48}
+
+
+def measure_trid(ops, apps, n, seed, sweeps=20, warmup=3):
+    """256^3 Thomas solves along x, y, then z with _Inc (the ADI sweep)."""
+    ops.ops_mi_set_diags(2)
+    T = apps.Trid(n, n, n, seed=seed, strategy=0)
+    for _ in range(warmup):
         T.sweep(inc=True)
     ops.ops_mi_sync()
     ops.ops_mi_reset_stats()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
+    for _ in range(sweeps):
         T.sweep(inc=True)
     ops.ops_mi_sync()
-    ms = (time.perf_counter() - t0) / a.steps * 1e3
+    ms = (time.perf_counter() - t0) / sweeps * 1e3
     T.close()
     pts = float(n) ** 3
-    value = (40.0 + 40.0 + 48.0) * pts / (ms * 1e-3) / 1e9
-    return line(value, "GB/s", ms, a, "f64",
-                {"workload": f"adi_trid_{n}", "grid": [n, n, n], "step": "Thomas solves along x, y, z (z with _Inc)",
-                 "bytes_per_point": 128})
+    kernels = kernel_table(ops, TRID_BPP, pts)
+    return {"workload": f"adi_trid_{n}", "grid": [n, n, n], "step": "Thomas solves along x, y, z (z with _Inc)",
+            "bytes_per_point": 128, "ms_per_sweep": round(ms, 4),
+            "GB_s": round(128.0 * pts / (ms * 1e-3) / 1e9, 1), "roofline": roofline(kernels, TRID_BPP, pts),
+            "kernels": kernels}
+
+
+def bench_trid(a, ops, apps):
+    n = a.size or 256
+    ops.ops_init(diags=2)
+    m = measure_trid(ops, apps, n, a.seed, a.steps, a.warmup)
+    return line(m["GB_s"], "GB/s", m["ms_per_sweep"], a, "f64",
+                {k: m[k] for k in ("workload", "grid", "step", "bytes_per_point")}, roofline=m["roofline"],
+                kernels=m["kernels"])
 
 
 # ------------------------------------------------------------------ serving
@@ -274,30 +322,35 @@ def serve_requests(nreq, seed):
     return kinds, sizes, seeds, g
 
 
-def bench_serve(a, ops, apps):
+def measure_serve(ops, apps, nreq, seed):
+    """The request stream on the batched engine (FP64, then FP32-inner
+    refinement for the 2D Poisson requests), all queued at t = 0 and under
+    Poisson arrivals, against the per-solve FP64 path (the library apps, one
+    solve after another).  Launches are not timed with events here (diags 1),
+    so neither path pays for them."""
     import numpy as np
 
-    nreq, tol, budget = a.requests, 1e-10, 200 * 10 ** 9
-    kinds, sizes, seeds, g = serve_requests(nreq, a.seed)
-    ops.ops_init(diags=1)
+    tol, budget = 1e-10, 200 * 10 ** 9
+    kinds, sizes, seeds, g = serve_requests(nreq, seed)
+    ops.ops_mi_set_diags(1)
     # saturation: every request queued at t = 0
     span, done, iters = apps.serve_stream(kinds, sizes, seeds, np.zeros(nreq), tol, budget, autotune=True)
     assert np.all(iters > 0), "a solve stopped without converging"
     sat = nreq / span
-    print(f"batched, all queued at t=0: {nreq} solves in {span:.2f} s", file=sys.stderr, flush=True)
-    # the same stream with FP32-inner refinement for the 2D Poisson requests
+    print(f"serve: batched, all queued at t=0: {nreq} solves in {span:.2f} s", file=sys.stderr, flush=True)
     span_m, _, iters_m = apps.serve_stream(kinds, sizes, seeds, np.zeros(nreq), tol, budget, autotune=True,
                                            mixed=True)
     assert np.all(iters_m > 0), "a mixed solve stopped without converging"
     sat_m = nreq / span_m
-    print(f"batched FP32-inner, all queued at t=0: {nreq} solves in {span_m:.2f} s", file=sys.stderr, flush=True)
+    print(f"serve: batched FP32-inner, all queued at t=0: {nreq} solves in {span_m:.2f} s", file=sys.stderr,
+          flush=True)
     # latency under Poisson arrivals at 70 % of the saturated rate
     rate = 0.7 * sat
     arr = np.cumsum(g.exponential(1.0 / rate, nreq))
     arr -= arr[0]
     span2, done2, iters2 = apps.serve_stream(kinds, sizes, seeds, arr, tol, budget, autotune=True)
     lat = done2 - arr
-    print(f"batched, Poisson arrivals at {rate:.1f}/s: span {span2:.2f} s", file=sys.stderr, flush=True)
+    print(f"serve: batched, Poisson arrivals at {rate:.1f}/s: span {span2:.2f} s", file=sys.stderr, flush=True)
     # per-solve FP64 path: the library apps, one solve after another
     t_base, it_base = 0.0, []
     for j, (k, n, s) in enumerate(zip(kinds, sizes, seeds)):
@@ -310,20 +363,31 @@ def bench_serve(a, ops, apps):
         t_base += time.perf_counter() - t0
         P.close()
         if j % 20 == 19:
-            print(f"per-solve path: {j + 1} solves, {t_base:.2f} s", file=sys.stderr, flush=True)
+            print(f"serve: per-solve path: {j + 1} solves, {t_base:.2f} s", file=sys.stderr, flush=True)
     base = nreq / t_base
-    return line(sat, "solves/s", span / nreq * 1e3, a, "f64",
-                {"workload": f"serve_stream_{nreq}", "requests": nreq, "tol": tol, "budget_bytes": budget,
-                 "mix": "60% 2D 5-pt Poisson side log-U[64,2048], 40% 3D 7-pt heat side log-U[32,256]"},
-                batched_saturated={"solves_per_s": round(sat, 3), "makespan_s": round(span, 3)},
-                batched_fp32_inner_saturated={"solves_per_s": round(sat_m, 3), "makespan_s": round(span_m, 3),
-                                              "speedup_vs_per_solve_fp64": round(sat_m / base, 3)},
-                batched_poisson_arrivals={"rate_per_s": round(rate, 3), "p50_latency_s": float(np.percentile(lat, 50)),
-                                          "p99_latency_s": float(np.percentile(lat, 99)),
-                                          "solves_per_s": round(nreq / span2, 3)},
-                per_solve_fp64={"solves_per_s": round(base, 3), "solve_seconds_total": round(t_base, 3)},
-                speedup_vs_per_solve=round(sat / base, 3),
-                iterations={"batched_total": int(iters.sum()), "per_solve_total": int(sum(it_base))})
+    ops.ops_mi_set_diags(2)
+    return {"workload": f"serve_stream_{nreq}", "requests": nreq, "tol": tol, "budget_bytes": budget,
+            "mix": "60% 2D 5-pt Poisson side log-U[64,2048], 40% 3D 7-pt heat side log-U[32,256]",
+            "batched_saturated": {"solves_per_s": round(sat, 3), "makespan_s": round(span, 3)},
+            "batched_fp32_inner_saturated": {"solves_per_s": round(sat_m, 3), "makespan_s": round(span_m, 3)},
+            "batched_poisson_arrivals": {"rate_per_s": round(rate, 3),
+                                         "p50_latency_s": float(np.percentile(lat, 50)),
+                                         "p99_latency_s": float(np.percentile(lat, 99)),
+                                         "solves_per_s": round(nreq / span2, 3)},
+            "per_solve_fp64": {"solves_per_s": round(base, 3), "solve_seconds_total": round(t_base, 3)},
+            "speedup_batched_fp64_vs_per_solve": round(sat / base, 3),
+            "speedup_batched_fp32_inner_vs_per_solve": round(sat_m / base, 3),
+            "iterations": {"batched_total": int(iters.sum()), "batched_fp32_inner_total": int(iters_m.sum()),
+                           "per_solve_total": int(sum(it_base))}}
+
+
+def bench_serve(a, ops, apps):
+    ops.ops_init(diags=1)
+    m = measure_serve(ops, apps, a.requests, a.seed)
+    sat = m["batched_saturated"]["solves_per_s"]
+    return line(sat, "solves/s", 1e3 / sat, a, "f64",
+                {k: m[k] for k in ("workload", "requests", "tol", "budget_bytes", "mix")},
+                **{k: m[k] for k in m if k not in ("workload", "requests", "tol", "budget_bytes", "mix")})
 
 
 # ------------------------------------------------------ generated loops
@@ -385,6 +449,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20240611)
     ap.add_argument("--workload", default="heat3d_cg_512", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="heat3d_cg_512: skip the other configurations")
     ap.add_argument("--requests", type=int, default=100, help="serve_stream: number of requests")
     a = ap.parse_args()
     if a.gpus != 1:

@@ -158,8 +158,23 @@ typedef struct ops_arg {
 } ops_arg;
 
 /* ---- lifecycle ---- */
-/* ops/c/include/ops_lib_core.h:601-608; argv tokens OPS_DIAGS=, OPS_BLOCK_SIZE_{X,Y,Z}=,
- * OPS_TILING, OPS_DEVICE= are honoured as in ops/c/src/core/ops_lib_core.cpp:100-249 */
+/* ops/c/include/ops_lib_core.h:601-608.  argv tokens (as parsed by
+ * ops/c/src/core/ops_lib_core.cpp:100-249):
+ *   OPS_DIAGS=n, -OPS_DIAGS=n   diagnostics level
+ *   OPS_DEVICE=n                HIP device
+ *   OPS_TILING                  run-time loop-chain fusion on (fusion level 2,
+ *                               the default unless OPSMI_FUSION lowers it)
+ *   OPS_TILING_MAXDEPTH=n       recorded; it bounds the halo depth tiles
+ *                               exchange between MPI processes, and one GPU
+ *                               exchanges none
+ *   OPS_BLOCK_SIZE_Y=n          translator-generated 2D loops: rows per
+ *                               workgroup (rounded up to a multiple of 4, the
+ *                               workgroup's waves)
+ *   OPS_BLOCK_SIZE_Z=n          translator-generated 3D loops: z planes per
+ *                               workgroup
+ *   OPS_BLOCK_SIZE_X=n          recorded; a workgroup is one 64-lane wave
+ *                               wide in x on this backend
+ * Hand-written kernels keep their tuned tile shapes. */
 void ops_init(const int argc, const char *const argv[], const int diags_level);
 void ops_exit(void);
 /* ops/c/include/ops_lib_core.h:1076-1077 (single GPU: no decomposition) */
@@ -184,8 +199,9 @@ ops_reduction ops_decl_reduction_handle(int size, const char *type, const char *
 ops_halo ops_decl_halo(ops_dat from, ops_dat to, int *iter_size, int *from_base, int *to_base, int *from_dir,
                        int *to_dir);
 ops_halo_group ops_decl_halo_group(int nhalos, ops_halo *halos);
-/* ops/c/include/ops_internal1.h:99 (instance argument dropped: one instance) */
-void ops_decl_const_char(int dim, const char *type, int type_size, char *data, const char *name);
+/* ops/c/include/ops_internal1.h:99; instance may be NULL (the one
+ * instance this backend keeps) */
+void ops_decl_const_char(void *instance, int dim, const char *type, int type_size, char *data, const char *name);
 
 /* ---- loop arguments: ops/c/include/ops_lib_core.h:732-796 ---- */
 ops_arg ops_arg_dat(ops_dat dat, int dim, ops_stencil stencil, const char *type, ops_access acc);
@@ -238,19 +254,41 @@ void ops_diagnostic_output(void);
 int ops_is_root(void);
 
 /* ---- tridiagonal: ops/c/include/ops_tridiag.h:45-83.  The reference's
- * ops_tridsolver_params is a C++ class; in C it is an opaque handle made by
- * ops_tridsolver_params_create (the C++ class below wraps it). ---- */
+ * ops_tridsolver_params is a C++ class whose one data member,
+ * tridsolver_params, points at the solver's settings; the C++ class below
+ * keeps that layout, and C code fills the struct itself from
+ * ops_tridsolver_params_create. ---- */
 typedef struct ops_tridsolver_params_c ops_tridsolver_params_c;
 /* strategy: the reference SolveStrategy values; PCR (3) selects parallel
  * cyclic reduction, every other value the Thomas algorithm */
 ops_tridsolver_params_c *ops_tridsolver_params_create(ops_block block, int strategy);
 void ops_tridsolver_params_destroy(ops_tridsolver_params_c *p);
-/* ops/c/include/ops_tridiag.h:73-76, ops/c/src/tridiag/ops_tridiag.cpp:78-160 */
+#ifdef __cplusplus
+class ops_tridsolver_params;
+#else
+typedef struct ops_tridsolver_params {
+  void *tridsolver_params; /* an ops_tridsolver_params_c */
+} ops_tridsolver_params;
+#endif
+/* ops/c/include/ops_tridiag.h:73-76 (OPS_FTN_INTEROP: C linkage),
+ * ops/c/src/tridiag/ops_tridiag.cpp:78-160 */
+void ops_tridMultiDimBatch(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c, ops_dat d,
+                           ops_tridsolver_params *tridsolver_ctx);
+/* ops/c/include/ops_tridiag.h:78-81, ops/c/src/tridiag/ops_tridiag.cpp:168-249 */
+void ops_tridMultiDimBatch_Inc(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c, ops_dat d,
+                               ops_dat u, ops_tridsolver_params *tridsolver_ctx);
+/* the same solves over the settings object itself */
 void ops_tridMultiDimBatch_c(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c, ops_dat d,
                              ops_tridsolver_params_c *ctx);
-/* ops/c/include/ops_tridiag.h:78-81, ops/c/src/tridiag/ops_tridiag.cpp:168-249 */
 void ops_tridMultiDimBatch_Inc_c(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c,
                                  ops_dat d, ops_dat u, ops_tridsolver_params_c *ctx);
+
+/* ---- HDF5 output: ops/c/include/ops_hdf5.h:130-166.  This backend has no
+ * HDF5 library; each call reports OPS_NOT_IMPLEMENTED (error modes below) ---- */
+void ops_fetch_block_hdf5_file(ops_block block, const char *file_name);
+void ops_fetch_dat_hdf5_file(ops_dat dat, const char *file_name);
+void ops_fetch_stencil_hdf5_file(ops_stencil stencil, const char *file_name);
+void ops_fetch_halo_hdf5_file(ops_halo halo, const char *file_name);
 
 /* ==== hand-written per-loop entry points ============================== */
 
@@ -387,8 +425,13 @@ void ops_par_loop_red_scaled(const char *name, ops_block block, int dim, int *ra
                              ops_arg s);
 
 /* ==== backend extensions (no reference counterpart) =================== */
-/* 0: print and abort on error (default); 1: record and return */
+/* 0: print and abort on error (default); 1: record and return; 2: print,
+ * record and return.  Initial mode from OPSMI_ERROR_MODE. */
 void ops_mi_set_error_mode(int mode);
+/* diagnostics level after ops_init (the argument of ops_init, OPS_DIAGS):
+ * above 1 every launch is timed with HIP events for ops_mi_loop_stats and
+ * the timing table; queued loops run first */
+void ops_mi_set_diags(int level);
 /* last error code (0 = none) and message; reading clears it */
 int ops_mi_last_error(char *msg, int msg_len);
 /* the library's HIP stream (hipStream_t) */
@@ -432,14 +475,14 @@ void ops_mi_set_fusion(int level);
 template <class T>
 class ACC {
  public:
-  ACC(T *p = nullptr, long sx = 0, long sy = 0) : p_(p), sx_(sx), sy_(sy) {}
+  ACC(T *p = nullptr, long sx = 0, long sy = 0, long sz = 0) : p_(p), sx_(sx), sy_(sy), sz_(sz) {}
   T &operator()(int x, int y = 0, int z = 0, int w = 0) const {
-    return p_[x + sx_ * (y + sy_ * ((long)z + w))];
+    return p_[x + sx_ * (y + sy_ * ((long)z + sz_ * (long)w))];
   }
 
  private:
   T *p_;
-  long sx_, sy_;
+  long sx_, sy_, sz_;
 };
 template <class T>
 static inline ops_dat ops_decl_dat(ops_block block, int data_size, int *block_size, int *base, int *d_m,
@@ -459,9 +502,9 @@ static inline void ops_reduction_result(ops_reduction handle, T *ptr) {
 }
 template <class T>
 static inline void ops_decl_const(const char *name, int dim, const char *type, T *data) {
-  ops_decl_const_char(dim, type, sizeof(T), (char *)data, name);
+  ops_decl_const_char(nullptr, dim, type, sizeof(T), (char *)data, name);
 }
-/* ops/c/include/ops_tridiag.h:45-71 */
+/* ops/c/include/ops_tridiag.h:45-71: same single data member */
 class ops_tridsolver_params {
  public:
   enum SolveStrategy { GATHER_SCATTER = 0, ALLGATHER, JACOBI, PCR, LATENCY_HIDING_INTERLEAVED,
@@ -469,21 +512,16 @@ class ops_tridsolver_params {
   explicit ops_tridsolver_params(ops_block block) : tridsolver_params(ops_tridsolver_params_create(block, 0)) {}
   ops_tridsolver_params(ops_block block, SolveStrategy s)
       : tridsolver_params(ops_tridsolver_params_create(block, (int)s)) {}
-  ~ops_tridsolver_params() { ops_tridsolver_params_destroy(tridsolver_params); }
+  ~ops_tridsolver_params() { ops_tridsolver_params_destroy((ops_tridsolver_params_c *)tridsolver_params); }
+  ops_tridsolver_params(const ops_tridsolver_params &) = delete;
+  ops_tridsolver_params &operator=(const ops_tridsolver_params &) = delete;
+  /* settings of the distributed and CUDA solvers: no effect on one GPU */
   void set_jacobi_params(double, double, int) {}
   void set_batch_size(int) {}
   void set_cuda_opts(int, int, int) {}
   void set_cuda_sync(int) {}
-  ops_tridsolver_params_c *tridsolver_params;
+  void *tridsolver_params;
 };
-static inline void ops_tridMultiDimBatch(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c,
-                                         ops_dat d, ops_tridsolver_params *ctx) {
-  ops_tridMultiDimBatch_c(ndim, solvedim, range, a, b, c, d, ctx->tridsolver_params);
-}
-static inline void ops_tridMultiDimBatch_Inc(int ndim, int solvedim, int *range, ops_dat a, ops_dat b,
-                                             ops_dat c, ops_dat d, ops_dat u, ops_tridsolver_params *ctx) {
-  ops_tridMultiDimBatch_Inc_c(ndim, solvedim, range, a, b, c, d, u, ctx->tridsolver_params);
-}
 /* the reference's C++ signatures over the single instance
  * (ops/c/include/ops_lib_core.h: ops_execute(OPS_instance *),
  * ops_timing_output(std::ostream &)); checkpointing is not part of this

@@ -49,14 +49,18 @@ inline Loop loop_of(const opsmi_gen_launch *L) {
 }
 
 // NDIM: block dimensions; MULTI: the dat may have more than one component
-// (otherwise the component count is the constant 1)
-template <class T, int NDIM, bool MULTI>
+// (otherwise the component count is the constant 1); GEN: the x step comes
+// from the launch (0 when the argument's stencil does not advance in x, a
+// strided stencil over a dat of size 1 in x) instead of being the constant
+// component count.  Strides along y and z always come from the launch (0
+// for a dimension the stencil does not advance in).
+template <class T, int NDIM, bool MULTI, bool GEN = false>
 class Acc {
   T *p_;
   long md_, sy_, sz_;
 
   __device__ __forceinline__ long pt(int dx, int dy, int dz) const {
-    long o = (long)dx * (MULTI ? md_ : 1);
+    long o = (long)dx * ((MULTI || GEN) ? md_ : 1);
     if (NDIM > 1) o += (long)dy * sy_;
     if (NDIM > 2) o += (long)dz * sz_;
     return o;

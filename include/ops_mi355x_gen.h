@@ -39,7 +39,10 @@ enum { OPSMI_GEN_F64 = 0, OPSMI_GEN_F32 = 1, OPSMI_GEN_I32 = 2, OPSMI_GEN_NONE =
 
 /* one dat argument as the kernel sees it: base = device address of
  * component 0 of point (0, 0, 0) of the dat's current buffer; strides in
- * elements (md = components per point, the x stride of an AoS dat) */
+ * elements (md = components per point, the x stride of an AoS dat).  A
+ * dimension the argument's stencil does not advance in (stride 0, a dat of
+ * size 1 there: ops/c/include/ops_lib_core.h:824-852) has stride 0 here
+ * (md = 0 for x). */
 typedef struct opsmi_gen_dat {
   void *base;
   long md, sy, sz;
@@ -51,7 +54,8 @@ typedef struct opsmi_gen_launch {
   int chunk;         /* 3D: z planes per workgroup; 2D/1D: groups of 4 rows */
   int three_d;       /* 1: grid[2] runs over z chunks */
   unsigned grid[3];  /* workgroups of 64 x 4 threads */
-  int alias;         /* 1: one dat is passed in two arguments (no restrict-qualified kernel) */
+  int alias;         /* 1: the general kernel (plain pointers, x steps from md): one dat is
+                        passed in two arguments, or a stencil does not advance in a dimension */
   opsmi_gen_dat dat[OPSMI_GEN_MAX_ARGS];    /* dat arguments, by argument index */
   void *partials[OPSMI_GEN_MAX_ARGS];       /* reduction arguments: dim values per workgroup */
 } opsmi_gen_launch;

@@ -474,9 +474,9 @@ double opsapp_laplace2d(int nx, int ny, int iters, int mode, const double *init_
   ops_stencil s5 = ops_decl_stencil(2, 5, S5, "5pt");
   ops_reduction h_err = ops_decl_reduction_handle(sizeof(double), "double", "error");
   double pi = 2.0 * asin(1.0);
-  ops_decl_const_char(1, "int", sizeof(int), (char *)&nx, "imax");
-  ops_decl_const_char(1, "int", sizeof(int), (char *)&ny, "jmax");
-  ops_decl_const_char(1, "double", sizeof(double), (char *)&pi, "pi");
+  ops_decl_const_char(nullptr, 1, "int", sizeof(int), (char *)&nx, "imax");
+  ops_decl_const_char(nullptr, 1, "int", sizeof(int), (char *)&ny, "jmax");
+  ops_decl_const_char(nullptr, 1, "double", sizeof(double), (char *)&pi, "pi");
   ops_dat both[2] = {A, An};
   if (init_ext) {
     for (ops_dat q : both) ops_dat_set_data_slab_memspace(q, 0, (char *)init_ext, g.full, OPS_HOST);
@@ -598,8 +598,8 @@ void opsapp_trid_destroy(void *ctx) {
 // be NULL) gets the whole interior, x fastest.
 double opsapp_mb_poisson(int nx, int ny, int ngx, int ngy, int iters, double *u_out) {
   double dx = 0.01, dy = 0.01;
-  ops_decl_const_char(1, "double", sizeof(double), (char *)&dx, "dx");
-  ops_decl_const_char(1, "double", sizeof(double), (char *)&dy, "dy");
+  ops_decl_const_char(nullptr, 1, "double", sizeof(double), (char *)&dx, "dx");
+  ops_decl_const_char(nullptr, 1, "double", sizeof(double), (char *)&dy, "dy");
   const int nb = ngx * ngy;
   std::vector<ops_block> blocks(nb);
   std::vector<ops_dat> u(nb), u2(nb), f(nb), ref(nb);

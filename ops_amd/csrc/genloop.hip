@@ -8,9 +8,12 @@
 // partial buffer per reduction argument, and the library then folds each
 // argument's partials into its handle in a fixed order (the same two-stage
 // fold as the hand-written loops).
+#include <float.h>
+#include <limits.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <memory>
 
@@ -40,8 +43,9 @@ void finish(DType dt, const RedCtl &c, int op, int dim, unsigned parts, hipStrea
   else finish_op<int>(c, op, dim, parts, s);
 }
 
-// z planes per workgroup of a generated 3D loop (OPSMI_GEN_ZCHUNK)
-int gen_zchunk() {
+// z planes per workgroup of a generated 3D loop without OPS_BLOCK_SIZE_Z
+// (OPSMI_GEN_ZCHUNK, else 1: more planes per workgroup measured slower)
+int gen_zchunk_env() {
   static int v = [] {
     const char *e = getenv("OPSMI_GEN_ZCHUNK");
     const int n = e ? atoi(e) : 1;
@@ -60,6 +64,11 @@ struct GenRed {
 
 }  // namespace
 
+int Instance::gen_rows_2d() const {
+  return block_y > 0 ? (block_y + kWavesPerBlock - 1) / kWavesPerBlock * kWavesPerBlock : 4 * kWavesPerBlock;
+}
+int Instance::gen_planes_3d() const { return block_z > 0 ? block_z : gen_zchunk_env(); }
+
 // A generated loop as queued: what its launch function is given when it
 // runs, alone or as the second half of a fused pair.
 struct GenLoop {
@@ -73,6 +82,10 @@ struct GenLoop {
   std::vector<ops_dat> dats;  // by argument (nullptr for other kinds)
   std::vector<int> acc;       // access mode by argument
   std::vector<char> at_point; // dat argument touched only at offset 0 (listed points)
+  std::vector<unsigned char> adv;  // dat argument: bit q set when its stencil advances in dimension q
+  bool strided = false;       // some stencil does not advance in a dimension (stride 0)
+  // edge dats reset to the identity of their accumulation before the kernel
+  std::vector<std::pair<ops_dat, int>> resets;
   std::vector<GenRed> reds;
   size_t pbytes = 0;          // partial buffer bytes the reductions take
 
@@ -87,18 +100,20 @@ struct GenLoop {
     L.grid[0] = t.grid.x;
     L.grid[1] = t.grid.y;
     L.grid[2] = t.grid.z;
-    L.alias = alias;
+    L.alias = alias || strided;
     for (size_t i = 0; i < dats.size(); i++)
       if (dats[i]) {
         const DatView v = view_of(dats[i]);
         L.dat[i].base = v.base;
-        L.dat[i].md = dats[i]->dim;
-        L.dat[i].sy = v.sy;
-        L.dat[i].sz = v.sz;
+        L.dat[i].md = (adv[i] & 1) ? dats[i]->dim : 0;
+        L.dat[i].sy = (adv[i] & 2) ? v.sy : 0;
+        L.dat[i].sz = (adv[i] & 4) ? v.sz : 0;
       }
     for (const GenRed &g : reds) L.partials[g.arg] = pbase + g.offset;
   }
   const void *blob() const { return data.empty() ? nullptr : data.data(); }
+  // the identity fills of edge dats this loop accumulates into afresh
+  void run_resets(hipStream_t s) const;
   // folds each reduction argument's partials into its handle
   void finish_reds(char *pbase, hipStream_t s) const {
     for (const GenRed &g : reds) {
@@ -119,6 +134,45 @@ struct GenLoop {
     return true;
   }
 };
+
+namespace {
+
+template <class T>
+__global__ void k_fill_value(T *p, size_t n, T v) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+template <class T>
+void fill_value(ops_dat d, T v, hipStream_t s) {
+  const size_t n = d->mem / sizeof(T);
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+  if (n) hipLaunchKernelGGL(k_fill_value<T>, dim3(blocks), dim3(256), 0, s, (T *)d->data_d, n, v);
+}
+
+}  // namespace
+
+// identity of an accumulation into an edge dat: 0, the type's largest value
+// (MIN) or lowest value (MAX), as ops/c/src/core/ops_dummy_singlenode.cpp:
+// 252-378 fills them
+void GenLoop::run_resets(hipStream_t s) const {
+  for (const auto &rs : resets) {
+    const ops_dat d = rs.first;
+    const int op = rs.second;
+    switch (((DatImpl *)d->impl)->dt) {
+      case DT_F64:
+        fill_value<double>(d, op == OPS_INC ? 0.0 : (op == OPS_MIN ? DBL_MAX : -DBL_MAX), s);
+        break;
+      case DT_F32:
+        fill_value<float>(d, op == OPS_INC ? 0.0f : (op == OPS_MIN ? FLT_MAX : -FLT_MAX), s);
+        break;
+      default:
+        fill_value<int>(d, op == OPS_INC ? 0 : (op == OPS_MIN ? INT_MAX : INT_MIN), s);
+        break;
+    }
+    OPSMI_HIP(hipGetLastError());
+  }
+}
 
 namespace {
 
@@ -149,6 +203,8 @@ int gen_pair_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
   if (it == fused_table().end()) return 0;
   if (A.block != B.block || A.dims != B.dims || !same_range(A.r, B.r) || A.r.npts() <= 0) return 0;
   if (share_handle(A, B)) return 0;
+  // the fused kernel takes the restrict-qualified point mapping only
+  if (A.alias || B.alias || A.strided || B.strided || !A.resets.empty() || !B.resets.empty()) return 0;
   for (const GenLoop *W : {&A, &B})
     for (size_t i = 0; i < W->dats.size(); i++) {
       const ops_dat d = W->dats[i];
@@ -201,9 +257,33 @@ void ops_mi_gen_par_loop(const char *name, ops_block block, int dim, const int *
   rec.dims = dim;
   rec.bytes = transfer_bytes(args, nargs, r);
   deps_from_args(rec, args, nargs);
+  // edge dats (ops/c/src/core/ops_dummy_singlenode.cpp:204-391): an
+  // accumulation after a different access starts from the identity; the
+  // access of every write is remembered, in program order
+  std::vector<std::pair<ops_dat, int>> resets;
+  for (int i = 0; i < nargs; i++) {
+    const ops_arg &a = args[i];
+    if (a.argtype != OPS_ARG_DAT || !a.dat || !a.dat->e_dat) continue;
+    DatImpl *m = (DatImpl *)a.dat->impl;
+    if ((a.acc == OPS_INC || a.acc == OPS_MIN || a.acc == OPS_MAX) && m->edat_prev_acc != a.acc) {
+      m->edat_prev_acc = a.acc;
+      resets.push_back({a.dat, a.acc});
+    }
+  }
+  for (int i = 0; i < nargs; i++) {
+    const ops_arg &a = args[i];
+    if (a.argtype == OPS_ARG_DAT && a.dat && a.dat->e_dat && a.acc != OPS_READ)
+      ((DatImpl *)a.dat->impl)->edat_prev_acc = a.acc;
+  }
   if (r.npts() <= 0) {
     rec.bytes = 0.0;
-    rec.launch = [](hipStream_t) {};
+    if (resets.empty()) {
+      rec.launch = [](hipStream_t) {};
+    } else {
+      auto G = std::make_shared<GenLoop>();
+      G->resets = resets;
+      rec.launch = [G](hipStream_t s) { G->run_resets(s); };
+    }
   } else {
     auto G = std::make_shared<GenLoop>();
     G->fn = fn;
@@ -213,7 +293,7 @@ void ops_mi_gen_par_loop(const char *name, ops_block block, int dim, const int *
     // the hand-written engine's tiling with rows of 64 * OPSMI_GEN_XP
     // points: 3D one plane of 4 rows per workgroup, 2D four groups of four
     // rows
-    G->t = make_tiling(r, r.z1 - r.z0 > 1 ? gen_zchunk() : 4);
+    G->t = make_tiling(r, r.z1 - r.z0 > 1 ? inst().gen_planes_3d() : inst().gen_rows_2d() / kWavesPerBlock);
     G->t.grid.x = (unsigned)(((long)r.x1 - r.x0 + kWave * OPSMI_GEN_XP - 1) / (kWave * OPSMI_GEN_XP));
     const unsigned parts = G->parts();
     size_t off = 0;
@@ -238,19 +318,31 @@ void ops_mi_gen_par_loop(const char *name, ops_block block, int dim, const int *
     G->dats.assign((size_t)nargs, nullptr);
     G->acc.assign((size_t)nargs, 0);
     G->at_point.assign((size_t)nargs, 0);
+    G->adv.assign((size_t)nargs, 7);
     const int *pt = pts;
     for (int i = 0; i < nargs; i++) {
       const int np = npts[i];
       if (args[i].argtype == OPS_ARG_DAT) {
         G->dats[i] = args[i].dat;
         G->acc[i] = args[i].acc;
-        bool zero = np >= 0;
+        // only listed points count: a kernel that reaches the dat another
+        // way (np < 0) or not at all (np == 0) is not known to stay at the point
+        bool zero = np > 0;
         for (int q = 0; q < np; q++)
           if (pt[4 * q + 1] != 0 || pt[4 * q + 2] != 0 || pt[4 * q + 3] != 0) zero = false;
         G->at_point[i] = zero;
         if (np > 0) pt += 4 * np;
+        const ops_stencil st = args[i].stencil;
+        if (args[i].dat && st && st->stride) {
+          unsigned char m = 0;
+          for (int q = 0; q < 3; q++)
+            if (q >= st->dims || st->stride[q] != 0) m |= (unsigned char)(1u << q);
+          G->adv[i] = m;
+          if (m != 7) G->strided = true;
+        }
       }
     }
+    G->resets = resets;
     for (int i = 0; i < nargs; i++)
       for (int j = i + 1; j < nargs; j++)
         if (G->dats[i] && G->dats[i] == G->dats[j]) G->alias = 1;
@@ -258,6 +350,7 @@ void ops_mi_gen_par_loop(const char *name, ops_block block, int dim, const int *
     const GenLoop *g = G.get();
     rec.launch = [G, g](hipStream_t s) {
       char *pbase = (char *)inst().d_partials;
+      g->run_resets(s);
       opsmi_gen_launch L;
       g->fill(L, pbase);
       g->fn(&L, g->blob(), (void *)s);

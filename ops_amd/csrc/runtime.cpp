@@ -24,14 +24,22 @@ void fail(int code, const std::string &msg) { throw Error{code, msg}; }
 
 void report(const Error &e) {
   Instance &I = inst();
-  if (I.error_mode == 0) {
+  if (I.error_mode != 1) {
     fprintf(stderr, "OPS error %d: %s\n", e.code, e.msg.c_str());
     fflush(stderr);
-    abort();
+    if (I.error_mode == 0) abort();
   }
   I.last_error = e.code;
   I.last_msg = e.msg;
 }
+
+// initial error mode from OPSMI_ERROR_MODE (read when the library loads)
+static int g_error_mode_env = [] {
+  const char *e = getenv("OPSMI_ERROR_MODE");
+  const int m = e ? atoi(e) : 0;
+  inst().error_mode = (m >= 0 && m <= 2) ? m : 0;
+  return m;
+}();
 
 void require_init() {
   if (!inst().initialised) fail(OPS_RUNTIME_CONFIGURATION_ERROR, "ops_init has not been called");
@@ -324,10 +332,16 @@ Range3 check_gen_loop(const char *name, ops_block block, int dim, const int *ran
     if (!s) fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": dat ", d->name, " has no stencil"));
     if (s->mgrid_stride)
       fail(OPS_NOT_IMPLEMENTED, cat("loop ", name, ": stencil ", s->name, " is a multigrid (restrict/prolong) stencil"));
+    // strides: 1, or 0 for a dimension the stencil does not advance in
+    // (the point's index there is 0: a dat of size 1, lowdim)
+    int adv[3] = {1, 1, 1};
     if (s->stride)
-      for (int q = 0; q < s->dims; q++)
-        if (s->stride[q] != 1)
-          fail(OPS_NOT_IMPLEMENTED, cat("loop ", name, ": stencil ", s->name, " is strided (this backend takes unit strides)"));
+      for (int q = 0; q < s->dims && q < 3; q++) {
+        if (s->stride[q] != 0 && s->stride[q] != 1)
+          fail(OPS_NOT_IMPLEMENTED, cat("loop ", name, ": stencil ", s->name, " has stride ", s->stride[q],
+                                        " in dimension ", q, " (this backend takes strides 0 and 1)"));
+        adv[q] = s->stride[q];
+      }
     for (int k = 0; k < np; k++) {
       const int *q4 = mine + 4 * k;
       if (q4[0] >= d->dim)
@@ -346,12 +360,22 @@ Range3 check_gen_loop(const char *name, ops_block block, int dim, const int *ran
     int rl[3] = {r.x0, r.y0, r.z0}, rh[3] = {r.x1, r.y1, r.z1};
     if (r.npts() > 0)
       for (int q = 0; q < dim; q++) {
-        int first = d->d_m[q] + d->base[q];
-        int last = d->size[q] + d->d_m[q] + d->base[q];
-        if (rl[q] + lo[q] < first || rh[q] - 1 + hi[q] >= last)
+        // a dimension of size 1 (an edge dat's) holds its one point at
+        // index 0 (ops/c/src/core/ops_lib_core.cpp:466-478)
+        int first = d->size[q] == 1 ? 0 : d->d_m[q] + d->base[q];
+        int last = d->size[q] == 1 ? 1 : d->size[q] + d->d_m[q] + d->base[q];
+        const int a0 = adv[q] ? rl[q] : 0, a1 = adv[q] ? rh[q] - 1 : 0;
+        if (a0 + lo[q] < first || a1 + hi[q] >= last)
           fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": range [", rl[q], ",", rh[q], ") in dimension ", q,
-                                         " with stencil reach [", lo[q], ",", hi[q], "] leaves dat ", d->name,
-                                         " (valid indices [", first, ",", last, "))"));
+                                         adv[q] ? "" : " (stride 0)", " with stencil reach [", lo[q], ",", hi[q],
+                                         "] leaves dat ", d->name, " (valid indices [", first, ",", last, "))"));
+        // several points on one element: only OPS_INC/MIN/MAX access (the
+        // kernel's combine_* calls, atomic here) may write it
+        if (!adv[q] && rh[q] - rl[q] > 1 && (a.acc == OPS_WRITE || a.acc == OPS_RW))
+          fail(OPS_NOT_IMPLEMENTED, cat("loop ", name, ": dat ", d->name, " is written with ",
+                                        a.acc == OPS_WRITE ? "OPS_WRITE" : "OPS_RW", " through stencil ", s->name,
+                                        ", which maps ", rh[q] - rl[q], " points of dimension ", q,
+                                        " to one element (use OPS_INC, OPS_MIN or OPS_MAX)"));
       }
   }
   for (int i = 0; i < nargs; i++)
@@ -571,6 +595,8 @@ void enqueue(LoopRec &&rec) {
   if (g_queue.size() > 64) flush_all();
 }
 
+bool queue_empty() { return g_queue.empty(); }
+
 void flush_all() {
   if (g_queue.empty()) return;
   std::vector<LoopRec> run;
@@ -665,14 +691,22 @@ void ops_init(const int argc, const char *const argv[], const int diags_level) {
   if (I.initialised) return;
   I.diags = diags_level;
   int dev = 0;
+  bool tiling = false;
   for (int n = 1; n < argc && argv; n++) {
     const char *a = argv[n];
     if (!a) continue;
     const char *p;
     if ((p = strstr(a, "OPS_DIAGS="))) I.diags = atoi(p + 10);
     if ((p = strstr(a, "OPS_DEVICE="))) dev = atoi(p + 11);
-    if ((p = strstr(a, "OPS_BLOCK_SIZE_Y="))) I.block_y = std::max(1, std::min(16, atoi(p + 17)));
+    if ((p = strstr(a, "OPS_BLOCK_SIZE_X="))) I.block_x = atoi(p + 17);
+    if ((p = strstr(a, "OPS_BLOCK_SIZE_Y="))) I.block_y = atoi(p + 17);
+    if ((p = strstr(a, "OPS_BLOCK_SIZE_Z="))) I.block_z = atoi(p + 17);
+    if (strstr(a, "OPS_TILING")) tiling = true;
+    if ((p = strstr(a, "OPS_TILING_MAXDEPTH="))) I.tiling_maxdepth = atoi(p + 20);
   }
+  if (I.block_x < 0 || I.block_y < 0 || I.block_z < 0 || I.block_y > 1024 || I.block_z > 1024)
+    fail(OPS_RUNTIME_CONFIGURATION_ERROR, cat("OPS_BLOCK_SIZE_{X,Y,Z}: ", I.block_x, ", ", I.block_y, ", ",
+                                              I.block_z, " out of range"));
   int ndev = 0;
   OPSMI_HIP(hipGetDeviceCount(&ndev));
   if (ndev < 1) fail(OPS_RUNTIME_CONFIGURATION_ERROR, "no HIP device available");
@@ -684,6 +718,10 @@ void ops_init(const int argc, const char *const argv[], const int diags_level) {
   I.num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   const char *fz = getenv("OPSMI_FUSION");
   I.fusion = fz ? atoi(fz) : 2;
+  if (tiling) I.fusion = 2;
+  if (I.diags > 0 && (tiling || I.block_y_set() || I.block_z_set()))
+    printf(" OPS (MI355X): loop-chain fusion level %d, generated-loop workgroups %d rows (2D) / %d planes (3D)\n",
+           I.fusion, I.gen_rows_2d(), I.gen_planes_3d());
   OPSMI_HIP(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
   ensure_partials((size_t)1 << 20);
   OPSMI_HIP(hipMalloc(&I.d_partials2, (size_t)1 << 20));
@@ -804,15 +842,18 @@ ops_dat ops_decl_dat_char(ops_block block, int dim, int *dsize, int *base, int *
       fail(OPS_INVALID_ARGUMENT, cat("Error: ops_decl_dat -- Non negative d_m during declaration of: ", name));
     if (d_p[n] < 0)
       fail(OPS_INVALID_ARGUMENT, cat("Error: ops_decl_dat -- Non positive d_p during declaration of: ", name));
-    if (dsize[n] != 1) {
+    // a dimension of size 1 is an edge (lowdim) dimension: one point at
+    // index 0, no halo
+    const bool edge = dsize[n] == 1;
+    if (!edge) {
       d->size[n] = dsize[n] - d_m[n] + d_p[n];
     } else {
       d->size[n] = 1;
       d->e_dat = 1;
     }
     d->base[n] = d->size[n] == 1 ? 0 : base[n];
-    d->d_m[n] = d_m[n];
-    d->d_p[n] = d_p[n];
+    d->d_m[n] = edge ? 0 : d_m[n];
+    d->d_p[n] = edge ? 0 : d_p[n];
     if (stride) d->stride[n] = stride[n];
   }
   // x padding so each row starts on a 256-byte boundary (rows at least 1 KiB)
@@ -973,9 +1014,11 @@ ops_halo_group ops_decl_halo_group(int nhalos, ops_halo *halos) {
   OPSMI_API_END(nullptr)
 }
 
-void ops_decl_const_char(int dim, const char *type, int type_size, char *data, const char *name) {
+void ops_decl_const_char(void *instance, int dim, const char *type, int type_size, char *data, const char *name) {
   OPSMI_API_BEGIN
   require_init();
+  if (instance && instance != (void *)&inst()) fail(OPS_INVALID_ARGUMENT, "ops_decl_const: unknown OPS instance");
+  if (!name || dim < 0 || type_size <= 0 || (dim > 0 && !data)) fail(OPS_INVALID_ARGUMENT, "ops_decl_const: bad arguments");
   Const c;
   c.type = type ? type : "";
   c.bytes.assign(data, data + (size_t)dim * type_size);
@@ -1325,11 +1368,44 @@ void ops_fprintf(FILE *stream, const char *format, ...) {
 
 int ops_is_root(void) { return 1; }
 
+// HDF5 output (ops/c/include/ops_hdf5.h:130-166): this backend has no HDF5
+// library; the calls are recognised and refused
+static void no_hdf5(const char *what, const char *file) {
+  fail(OPS_NOT_IMPLEMENTED, cat(what, "(\"", file ? file : "", "\"): HDF5 output is not available in this backend"));
+}
+void ops_fetch_block_hdf5_file(ops_block block, const char *file_name) {
+  OPSMI_API_BEGIN
+  (void)block;
+  no_hdf5("ops_fetch_block_hdf5_file", file_name);
+  OPSMI_API_END()
+}
+void ops_fetch_dat_hdf5_file(ops_dat dat, const char *file_name) {
+  OPSMI_API_BEGIN
+  (void)dat;
+  no_hdf5("ops_fetch_dat_hdf5_file", file_name);
+  OPSMI_API_END()
+}
+void ops_fetch_stencil_hdf5_file(ops_stencil stencil, const char *file_name) {
+  OPSMI_API_BEGIN
+  (void)stencil;
+  no_hdf5("ops_fetch_stencil_hdf5_file", file_name);
+  OPSMI_API_END()
+}
+void ops_fetch_halo_hdf5_file(ops_halo halo, const char *file_name) {
+  OPSMI_API_BEGIN
+  (void)halo;
+  no_hdf5("ops_fetch_halo_hdf5_file", file_name);
+  OPSMI_API_END()
+}
+
 void ops_diagnostic_output(void) {
   Instance &I = inst();
   printf("\n OPS diagnostic output (MI355X backend, device %d, %d CUs)\n", I.device, I.num_cus);
   printf(" blocks %zu, dats %zu, stencils %zu, reductions %zu\n", I.blocks.size(), I.dats.size(),
          I.stencils.size(), I.reductions.size());
+  printf(" fusion level %d, tiling max depth %d, OPS_BLOCK_SIZE %d x %d x %d (generated loops: %d rows 2D, "
+         "%d planes 3D per workgroup)\n", I.fusion, I.tiling_maxdepth, I.block_x, I.block_y, I.block_z,
+         I.gen_rows_2d(), I.gen_planes_3d());
   for (ops_dat d : I.dats)
     if (d)
       printf("   %-16s block %-10s dim %d type %-6s size [%d %d %d] mem %.3f MB\n", d->name, d->block->name, d->dim,
@@ -1353,6 +1429,14 @@ void ops_timing_output_stdout(void) {
 
 // ------------------------------------------------------------------ extensions
 void ops_mi_set_error_mode(int mode) { inst().error_mode = mode; }
+
+void ops_mi_set_diags(int level) {
+  OPSMI_API_BEGIN
+  flush_all();
+  drain_events();
+  inst().diags = level;
+  OPSMI_API_END()
+}
 
 int ops_mi_last_error(char *msg, int msg_len) {
   Instance &I = inst();

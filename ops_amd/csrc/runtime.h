@@ -86,6 +86,10 @@ struct DatImpl {
   bool spare_synced = false;  // spare equals the current buffer outside spare_range
   bool spare_ranged = false;  // spare_range is set (else: equal everywhere)
   int spare_range[6] = {0, 0, 0, 0, 0, 0};
+  // edge dats: access of the last loop that wrote the dat, in program
+  // order; an OPS_INC/MIN/MAX loop after a different access first resets the
+  // dat to the identity (ops/c/src/core/ops_dummy_singlenode.cpp:204-391)
+  int edat_prev_acc = OPS_READ;
 };
 
 struct LoopStat {
@@ -111,7 +115,16 @@ struct Instance {
   int last_error = 0;
   std::string last_msg;
   int fusion = 2;  // 0 off, 1 the pointwise pair only, 2 also the column chains
-  int block_x = 64, block_y = 4, block_z = 1;  // OPS_BLOCK_SIZE_* (x must be 64)
+  // OPS_BLOCK_SIZE_{X,Y,Z} (0: not given) and OPS_TILING_MAXDEPTH
+  int block_x = 0, block_y = 0, block_z = 0;
+  int tiling_maxdepth = 0;
+  bool block_y_set() const { return block_y > 0; }
+  bool block_z_set() const { return block_z > 0; }
+  // rows of a generated 2D loop's workgroup: OPS_BLOCK_SIZE_Y rounded up to
+  // a multiple of its 4 waves (default 16); planes of a generated 3D loop's
+  // workgroup: OPS_BLOCK_SIZE_Z (default OPSMI_GEN_ZCHUNK, else 1)
+  int gen_rows_2d() const;
+  int gen_planes_3d() const;
 
   std::vector<ops_block> blocks;
   std::vector<ops_dat> dats;  // index -> dat (nullptr after free)
@@ -278,6 +291,7 @@ typedef int (*FusionRule)(std::vector<LoopRec> &run, size_t at, hipStream_t s);
 void register_fusion(FusionRule rule);
 void enqueue(LoopRec &&rec);
 void flush_all();
+bool queue_empty();
 void flush_for(ops_reduction h);
 void deps_from_args(LoopRec &rec, const ops_arg *args, int n);
 // Double buffering for fused chains that rewrite a dat its own stencil reads

@@ -38,16 +38,31 @@ __device__ __forceinline__ long line_offset(const LineGeo &g, long id) {
   return i0 * g.s0 + i1 * g.s1;
 }
 
-// The recurrence is serial per system but its loads are not: each pass loads
-// the coefficients of kPre steps at once, then runs those steps, so a thread
-// keeps 4 kPre loads in flight instead of waiting on memory at every step
-// (the arithmetic and its order are unchanged).  Without _Inc, d' is written
-// in place into d (the backward pass overwrites it with x), so only c' goes
-// through the scratch; _Inc leaves d untouched and keeps d' in scratch too.
-// The forward pass reads a, b, c, d once, with the non-temporal hint.
+// The recurrence is serial per system but its loads are not: the passes
+// run in blocks of P steps, and while a block's recurrence runs the next
+// block's loads are already in flight (two register sets, the loop unrolled
+// by two so each set has fixed registers), so a thread waits on memory once
+// per pass rather than once per block.  Loads of the block past the end are
+// clamped to the last step (in bounds, never used), so the loads and the
+// recurrence of full blocks carry no branches (a branch join would make the
+// compiler wait for every load in flight); the last, partial block runs with
+// per-step conditions.  The arithmetic and its order are those of the plain
+// loop.  Without _Inc, d' is written in place into d (the backward pass
+// overwrites it with x), so only c' goes through the scratch; _Inc leaves d
+// untouched and keeps d' in scratch too.  The forward pass reads a, b, c, d
+// once, with the non-temporal hint.
 constexpr int kPre = 8;
 
-template <class T, bool INC>
+template <class T, int P>
+struct FwdBuf {
+  T a[P], b[P], c[P], d[P];
+};
+template <class T, int P>
+struct BwdBuf {
+  T c[P], d[P], u[P];
+};
+
+template <class T, bool INC, int P>
 __global__ __launch_bounds__(256) void k_thomas(const T *__restrict__ a, const T *__restrict__ b,
                                                 const T *__restrict__ c, T *__restrict__ d, T *__restrict__ u,
                                                 T *__restrict__ cs, T *__restrict__ ds, LineGeo g) {
@@ -63,53 +78,105 @@ __global__ __launch_bounds__(256) void k_thomas(const T *__restrict__ a, const T
   cs[id] = cprev;
   if (INC) ds[id] = dprev;
   else dp[0] = dprev;
-  for (long i0 = 1; i0 < n; i0 += kPre) {
-    T av[kPre], bv[kPre], cv[kPre], dv[kPre];
+  // ---- forward: steps 1 .. n-1
+  auto fload = [&](long i0, FwdBuf<T, P> &f) {
 #pragma unroll
-    for (int k = 0; k < kPre; k++) {
-      const long i = i0 + k;
-      if (i < n) {
-        av[k] = ldg<T, true>(ap + i * s);
-        bv[k] = ldg<T, true>(bp + i * s);
-        cv[k] = ldg<T, true>(cp + i * s);
-        dv[k] = ldg<T, true>(dp + i * s);
-      }
+    for (int k = 0; k < P; k++) {
+      const long i = min(i0 + k, n - 1);
+      f.a[k] = ldg<T, true>(ap + i * s);
+      f.b[k] = ldg<T, true>(bp + i * s);
+      f.c[k] = ldg<T, true>(cp + i * s);
+      f.d[k] = ldg<T, true>(dp + i * s);
     }
+  };
+  auto fstep = [&](const FwdBuf<T, P> &f, int k, long i) {
+    bb = (T)1.0 / (f.b[k] - f.a[k] * cprev);
+    dprev = bb * (f.d[k] - f.a[k] * dprev);
+    cprev = bb * f.c[k];
+    cs[i * L + id] = cprev;
+    if (INC) ds[i * L + id] = dprev;
+    else dp[i * s] = dprev;
+  };
+  auto ffull = [&](const FwdBuf<T, P> &f, long i0) {
 #pragma unroll
-    for (int k = 0; k < kPre; k++) {
-      const long i = i0 + k;
-      if (i < n) {
-        bb = (T)1.0 / (bv[k] - av[k] * cprev);
-        dprev = bb * (dv[k] - av[k] * dprev);
-        cprev = bb * cv[k];
-        cs[i * L + id] = cprev;
-        if (INC) ds[i * L + id] = dprev;
-        else dp[i * s] = dprev;
+    for (int k = 0; k < P; k++) fstep(f, k, i0 + k);
+  };
+  auto ftail = [&](const FwdBuf<T, P> &f, long i0) {
+#pragma unroll
+    for (int k = 0; k < P; k++)
+      if (i0 + k < n) fstep(f, k, i0 + k);
+  };
+  if (n > 1) {
+    FwdBuf<T, P> X, Y;
+    long i0 = 1;
+    fload(i0, X);
+    while (true) {
+      if (i0 + P > n) {
+        ftail(X, i0);
+        break;
       }
+      fload(i0 + P, Y);
+      ffull(X, i0);
+      i0 += P;
+      if (i0 >= n) break;
+      if (i0 + P > n) {
+        ftail(Y, i0);
+        break;
+      }
+      fload(i0 + P, X);
+      ffull(Y, i0);
+      i0 += P;
+      if (i0 >= n) break;
     }
   }
+  // ---- backward: x_{n-1} = d'_{n-1}, then steps n-2 .. 0
   T x = dprev;
   if (INC) u[o + (n - 1) * s] = u[o + (n - 1) * s] + x;
   else dp[(n - 1) * s] = x;
-  for (long i1 = n - 2; i1 >= 0; i1 -= kPre) {
-    T cv[kPre], dv[kPre], uv[kPre];
+  auto bload = [&](long i1, BwdBuf<T, P> &w) {
 #pragma unroll
-    for (int k = 0; k < kPre; k++) {
-      const long i = i1 - k;
-      if (i >= 0) {
-        cv[k] = cs[i * L + id];
-        dv[k] = INC ? ds[i * L + id] : dp[i * s];
-        if (INC) uv[k] = u[o + i * s];
-      }
+    for (int k = 0; k < P; k++) {
+      const long i = max(i1 - k, 0L);
+      w.c[k] = cs[i * L + id];
+      w.d[k] = INC ? ds[i * L + id] : dp[i * s];
+      if (INC) w.u[k] = u[o + i * s];
     }
+  };
+  auto bstep = [&](const BwdBuf<T, P> &w, int k, long i) {
+    x = w.d[k] - w.c[k] * x;
+    if (INC) u[o + i * s] = w.u[k] + x;
+    else dp[i * s] = x;
+  };
+  auto bfull = [&](const BwdBuf<T, P> &w, long i1) {
 #pragma unroll
-    for (int k = 0; k < kPre; k++) {
-      const long i = i1 - k;
-      if (i >= 0) {
-        x = dv[k] - cv[k] * x;
-        if (INC) u[o + i * s] = uv[k] + x;
-        else dp[i * s] = x;
+    for (int k = 0; k < P; k++) bstep(w, k, i1 - k);
+  };
+  auto btail = [&](const BwdBuf<T, P> &w, long i1) {
+#pragma unroll
+    for (int k = 0; k < P; k++)
+      if (i1 - k >= 0) bstep(w, k, i1 - k);
+  };
+  if (n > 1) {
+    BwdBuf<T, P> X, Y;
+    long i1 = n - 2;
+    bload(i1, X);
+    while (true) {
+      if (i1 - P + 1 < 0) {
+        btail(X, i1);
+        break;
       }
+      bload(i1 - P, Y);
+      bfull(X, i1);
+      i1 -= P;
+      if (i1 < 0) break;
+      if (i1 - P + 1 < 0) {
+        btail(Y, i1);
+        break;
+      }
+      bload(i1 - P, X);
+      bfull(Y, i1);
+      i1 -= P;
+      if (i1 < 0) break;
     }
   }
 }
@@ -404,7 +471,7 @@ void trid_run(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat 
               ops_tridsolver_params_c *ctx) {
   require_init();
   flush_all();
-  if (!a || !b || !c || !d || !range) fail(OPS_INVALID_ARGUMENT, "Tridsolver error: NULL argument");
+  if (!a || !b || !c || !d || !range || !ctx) fail(OPS_INVALID_ARGUMENT, "Tridsolver error: NULL argument");
   if (ndim < 1 || ndim > 3 || ndim != a->block->dims)
     fail(OPS_INVALID_ARGUMENT, "Tridsolver error: ndim must match the block (1-3D)");
   if (solvedim < 0 || solvedim >= ndim) fail(OPS_INVALID_ARGUMENT, "Tridsolver error: solvedim out of range");
@@ -446,7 +513,11 @@ void trid_run(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat 
   g.s0 = st[od[0]];
   g.s1 = st[od[1]];
   g.count = (long)dims[od[0]] * dims[od[1]];
-  const char *nm = u ? "ops_tridMultiDimBatch_Inc" : "ops_tridMultiDimBatch";
+  // statistics per solve direction (the timing table's loop name)
+  static const char *names[2][3] = {{"ops_tridMultiDimBatch x", "ops_tridMultiDimBatch y", "ops_tridMultiDimBatch z"},
+                                    {"ops_tridMultiDimBatch_Inc x", "ops_tridMultiDimBatch_Inc y",
+                                     "ops_tridMultiDimBatch_Inc z"}};
+  const char *nm = names[u ? 1 : 0][solvedim];
   if (g.n <= 0 || g.count <= 0) {
     run_loop(nm, 0.0, [](hipStream_t) {});
     return;
@@ -506,11 +577,11 @@ void trid_run(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat 
     }
     run_loop(nm, bytes, [&](hipStream_t s) {
       if (dt == DT_F64) {
-        if (u) hipLaunchKernelGGL((k_thomas<double, true>), dim3(nblk), dim3(256), 0, s, (double *)pa, (double *)pb, (double *)pc, (double *)pd, (double *)pu, (double *)scr, (double *)(scr + sb), g);
-        else hipLaunchKernelGGL((k_thomas<double, false>), dim3(nblk), dim3(256), 0, s, (double *)pa, (double *)pb, (double *)pc, (double *)pd, (double *)pu, (double *)scr, (double *)(scr + sb), g);
+        if (u) hipLaunchKernelGGL((k_thomas<double, true, kPre>), dim3(nblk), dim3(256), 0, s, (double *)pa, (double *)pb, (double *)pc, (double *)pd, (double *)pu, (double *)scr, (double *)(scr + sb), g);
+        else hipLaunchKernelGGL((k_thomas<double, false, kPre>), dim3(nblk), dim3(256), 0, s, (double *)pa, (double *)pb, (double *)pc, (double *)pd, (double *)pu, (double *)scr, (double *)(scr + sb), g);
       } else {
-        if (u) hipLaunchKernelGGL((k_thomas<float, true>), dim3(nblk), dim3(256), 0, s, (float *)pa, (float *)pb, (float *)pc, (float *)pd, (float *)pu, (float *)scr, (float *)(scr + sb), g);
-        else hipLaunchKernelGGL((k_thomas<float, false>), dim3(nblk), dim3(256), 0, s, (float *)pa, (float *)pb, (float *)pc, (float *)pd, (float *)pu, (float *)scr, (float *)(scr + sb), g);
+        if (u) hipLaunchKernelGGL((k_thomas<float, true, kPre>), dim3(nblk), dim3(256), 0, s, (float *)pa, (float *)pb, (float *)pc, (float *)pd, (float *)pu, (float *)scr, (float *)(scr + sb), g);
+        else hipLaunchKernelGGL((k_thomas<float, false, kPre>), dim3(nblk), dim3(256), 0, s, (float *)pa, (float *)pb, (float *)pc, (float *)pd, (float *)pu, (float *)scr, (float *)(scr + sb), g);
       }
     });
   }
@@ -543,6 +614,25 @@ void ops_tridMultiDimBatch_Inc_c(int ndim, int solvedim, int *range, ops_dat a, 
   OPSMI_API_BEGIN
   if (!u) fail(OPS_INVALID_ARGUMENT, "Tridsolver error: NULL u");
   trid_run(ndim, solvedim, range, a, b, c, d, u, ctx);
+  OPSMI_API_END()
+}
+
+// The reference's C-linkage entry points: the params object's one data
+// member points at the solver settings.
+void ops_tridMultiDimBatch(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c, ops_dat d,
+                           ops_tridsolver_params *tridsolver_ctx) {
+  OPSMI_API_BEGIN
+  if (!tridsolver_ctx) fail(OPS_INVALID_ARGUMENT, "Tridsolver error: NULL ops_tridsolver_params");
+  trid_run(ndim, solvedim, range, a, b, c, d, nullptr, *(ops_tridsolver_params_c **)tridsolver_ctx);
+  OPSMI_API_END()
+}
+
+void ops_tridMultiDimBatch_Inc(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c, ops_dat d,
+                               ops_dat u, ops_tridsolver_params *tridsolver_ctx) {
+  OPSMI_API_BEGIN
+  if (!tridsolver_ctx) fail(OPS_INVALID_ARGUMENT, "Tridsolver error: NULL ops_tridsolver_params");
+  if (!u) fail(OPS_INVALID_ARGUMENT, "Tridsolver error: NULL u");
+  trid_run(ndim, solvedim, range, a, b, c, d, u, *(ops_tridsolver_params_c **)tridsolver_ctx);
   OPSMI_API_END()
 }
 

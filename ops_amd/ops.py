@@ -75,13 +75,14 @@ def lib():
         sig = {
             "ops_init": (None, [ci, C.POINTER(cs), ci]),
             "ops_exit": (None, []),
+            "ops_execute": (None, []),
             "ops_decl_block": (vp, [ci, cs]),
             "ops_decl_dat_char": (vp, [vp, ci, ip, ip, ip, ip, ip, vp, ci, cs, cs]),
             "ops_free_dat": (None, [vp]),
             "ops_decl_stencil": (vp, [ci, ci, ip, cs]),
             "ops_decl_strided_stencil": (vp, [ci, ci, ip, ip, cs]),
             "ops_decl_reduction_handle": (vp, [ci, cs, cs]),
-            "ops_decl_const_char": (None, [ci, cs, ci, vp, cs]),
+            "ops_decl_const_char": (None, [vp, ci, cs, ci, vp, cs]),
             "ops_decl_halo": (vp, [vp, vp, ip, ip, ip, ip, ip]),
             "ops_decl_halo_group": (vp, [ci, C.POINTER(vp)]),
             "ops_halo_transfer": (None, [vp]),
@@ -105,6 +106,8 @@ def lib():
             "ops_tridsolver_params_destroy": (None, [vp]),
             "ops_tridMultiDimBatch_c": (None, [ci, ci, ip, vp, vp, vp, vp, vp]),
             "ops_tridMultiDimBatch_Inc_c": (None, [ci, ci, ip, vp, vp, vp, vp, vp, vp]),
+            "ops_tridMultiDimBatch": (None, [ci, ci, ip, vp, vp, vp, vp, vp]),
+            "ops_tridMultiDimBatch_Inc": (None, [ci, ci, ip, vp, vp, vp, vp, vp, vp]),
             "ops_mi_set_error_mode": (None, [ci]),
             "ops_mi_last_error": (ci, [cs, ci]),
             "ops_mi_stream": (vp, []),
@@ -112,6 +115,7 @@ def lib():
             "ops_mi_loop_stats": (ci, [cs, C.POINTER(C.c_long), C.POINTER(cd), C.POINTER(cd)]),
             "ops_mi_reset_stats": (None, []),
             "ops_mi_set_fusion": (None, [ci]),
+            "ops_mi_set_diags": (None, [ci]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -187,6 +191,10 @@ def ops_exit():
     _call("ops_exit")
 
 
+def ops_execute():
+    _call("ops_execute")
+
+
 def ops_decl_block(dims, name):
     return Block(_call("ops_decl_block", dims, _b(name)), dims, name)
 
@@ -230,7 +238,7 @@ def ops_decl_reduction_handle(size, type_, name):
 def ops_decl_const(name, dim, type_, value):
     dt = {"double": np.float64, "float": np.float32, "int": np.int32}[type_]
     arr = np.ascontiguousarray(np.atleast_1d(np.asarray(value, dtype=dt)))
-    _call("ops_decl_const_char", dim, _b(type_), arr.itemsize, arr.ctypes.data, _b(name))
+    _call("ops_decl_const_char", None, dim, _b(type_), arr.itemsize, arr.ctypes.data, _b(name))
 
 
 def ops_arg_dat(dat, dim, stencil, type_, acc):
@@ -356,10 +364,15 @@ def ops_fill_random_normal(dat):
 
 
 class TridParams:
+    """ops_tridsolver_params: ptr is the settings object, ref the address of
+    a struct laid out as the reference class (one void* member pointing at
+    it), which the C-linkage ops_tridMultiDimBatch[_Inc] take."""
     GATHER_SCATTER, ALLGATHER, JACOBI, PCR = 0, 1, 2, 3
 
     def __init__(self, block, strategy=0):
         self.ptr = _call("ops_tridsolver_params_create", block.ptr, strategy)
+        self._obj = C.c_void_p(self.ptr)
+        self.ref = C.addressof(self._obj)
 
     def __del__(self):
         try:
@@ -370,11 +383,11 @@ class TridParams:
 
 
 def ops_tridMultiDimBatch(ndim, solvedim, rng, a, b, c, d, ctx):
-    _call("ops_tridMultiDimBatch_c", ndim, solvedim, _ints(rng), a.ptr, b.ptr, c.ptr, d.ptr, ctx.ptr)
+    _call("ops_tridMultiDimBatch", ndim, solvedim, _ints(rng), a.ptr, b.ptr, c.ptr, d.ptr, ctx.ref)
 
 
 def ops_tridMultiDimBatch_Inc(ndim, solvedim, rng, a, b, c, d, u, ctx):
-    _call("ops_tridMultiDimBatch_Inc_c", ndim, solvedim, _ints(rng), a.ptr, b.ptr, c.ptr, d.ptr, u.ptr, ctx.ptr)
+    _call("ops_tridMultiDimBatch_Inc", ndim, solvedim, _ints(rng), a.ptr, b.ptr, c.ptr, d.ptr, u.ptr, ctx.ref)
 
 
 def ops_mi_sync():

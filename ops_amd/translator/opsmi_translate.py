@@ -285,11 +285,19 @@ def local_includes(path, src, m, inc_dirs):
 # ------------------------------------------------------------------ emit
 def accessed_points(body, pname, ndim):
     """(component, dx, dy, dz) of every ACC call pname(...) in body whose
-    indices are integer literals; None when one is not (then the stencil is
-    not checked against the kernel)."""
+    indices are integer literals; None when the kernel touches the accessor
+    any other way (non-literal indices, combine_* calls, passing it to a
+    helper, binding another name to it) or never names it, since then the
+    points it reaches are unknown (the stencil is not checked against the
+    kernel, and the loop is not fused on the assumption that the dat is
+    touched only at the point)."""
     bm = mask(body)
     pts = set()
-    for mo in re.finditer(r"\b" + re.escape(pname) + r"\s*\(", bm):
+    names = list(re.finditer(r"\b" + re.escape(pname) + r"\b", bm))
+    calls = list(re.finditer(r"\b" + re.escape(pname) + r"\s*\(", bm))
+    if not names or len(names) != len(calls):
+        return None
+    for mo in calls:
         # skip member calls (pname.combine_max(...) has no '(' right after the name)
         op = mo.end() - 1
         cl = match_close(bm, op)
@@ -307,8 +315,6 @@ def accessed_points(body, pname, ndim):
             return None
         off = off + [0] * (3 - len(off))
         pts.add((c, off[0], off[1], off[2]))
-    if re.search(r"\b" + re.escape(pname) + r"\s*\.\s*combine_", bm):
-        return None
     return sorted(pts)
 
 
@@ -326,17 +332,22 @@ def kernel_code(loop, kern, consts):
     for c in used:
         d, t = consts[c]
         L.append(f"  {t} {c}{'[' + str(d) + ']' if d > 1 else ''};")
+    # the accessor type carries the kernel variant (OPSMI_G: x steps from
+    # the launch), so the call operator is a template over it when the
+    # kernel takes dats
     ps = []
     for (text, pname), a in zip(kern.params, loop.args):
         if a.kind == "dat":
             multi = "true" if a.multi else "false"
-            new, cnt = re.subn(r"\bACC\s*<\s*([^>]+?)\s*>", rf"::opsmi_gen::Acc<\1, {loop.ndim}, {multi}>", text)
+            new, cnt = re.subn(r"\bACC\s*<\s*([^>]+?)\s*>", rf"::opsmi_gen::Acc<\1, {loop.ndim}, {multi}, OPSMI_G>",
+                               text)
             if cnt != 1:
                 raise TranslateError(f"kernel {K}: parameter {text!r} for an ops_arg_dat must be an ACC<T> reference")
             ps.append(new)
         else:
             ps.append(text)
-    L.append(f"  __device__ __forceinline__ void operator()({', '.join(ps)}) const {kern.body}")
+    tmpl = "template <bool OPSMI_G> " if any(a.kind == "dat" for a in loop.args) else ""
+    L.append(f"  {tmpl}__device__ __forceinline__ void operator()({', '.join(ps)}) const {kern.body}")
     L.append("};")
     # by-value data: constants and gbl values
     L.append(f"struct opsb_{K} {{")
@@ -355,16 +366,18 @@ def kernel_code(loop, kern, consts):
     L.append(f"  opsb_{K} b;")
     L.append("};")
     # two kernels: dat pointers restrict-qualified (OPS writes a dat only at
-    # the point itself, so a thread's points are independent), and plain for
-    # loops that pass one dat twice
+    # the point itself, so a thread's points are independent), and the
+    # general one with plain pointers and x steps from the launch, for loops
+    # that pass one dat twice or whose stencils do not advance in a
+    # dimension (dats of size 1 there, several points on one element)
     dats = [i for i, a in enumerate(loop.args) if a.kind == "dat"]
-    for suffix, qual in (("", "__restrict__ "), ("_alias", "")):
+    for suffix, qual, gen in (("", "__restrict__ ", "false"), ("_alias", "", "true")):
         ptrs = "".join(f", {'const ' if loop.args[i].acc == 'OPS_READ' else ''}{loop.args[i].typ} *{qual}p{i}"
                        for i in dats)
         L.append(f"__global__ __launch_bounds__(256) void opsg_{K}{suffix}(const opsa_{K} A{ptrs}) {{")
         L.extend(red_init_lines(loop, ""))
         L.append("  ::opsmi_gen::for_each_point(A.L, [&](int x, int y, int z) {")
-        L.extend(point_lines(loop, "A", ""))
+        L.extend(point_lines(loop, "A", "", gen))
         L.append("  });")
         L.extend(red_store_lines(loop, "A", ""))
         L.append("}")
@@ -397,13 +410,13 @@ def red_init_lines(loop, pfx):
     return L
 
 
-def point_lines(loop, S, pfx):
+def point_lines(loop, S, pfx, gen="false"):
     """Accessors of loop's arguments at point (x, y, z) over pointers
     {pfx}p<i>, then the call of its kernel (member k of {S}.b)."""
     L, call = [], []
     for i, a in enumerate(loop.args):
         if a.kind == "dat":
-            L.append(f"    ::opsmi_gen::Acc<{a.typ}, {loop.ndim}, {'true' if a.multi else 'false'}> "
+            L.append(f"    ::opsmi_gen::Acc<{a.typ}, {loop.ndim}, {'true' if a.multi else 'false'}, {gen}> "
                      f"{pfx}a{i}({pfx}p{i}, {S}.d{i}, x, y, z);")
             call.append(f"{pfx}a{i}")
         elif a.kind == "idx":

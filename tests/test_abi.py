@@ -40,7 +40,8 @@ def test_par_loop_entry_points_cover_reference_kernels():
               "tea_leaf_axpby_kernel", "tea_leaf_dot_kernel", "tea_leaf_common_init_Kx_Ky_kernel",
               "tea_leaf_common_init_kernel", "preproc_kernel"]:
         assert "ops_par_loop_" + k in names, k
-    for k in ["ops_tridMultiDimBatch_c", "ops_tridMultiDimBatch_Inc_c", "ops_reduction_result_char",
+    for k in ["ops_tridMultiDimBatch", "ops_tridMultiDimBatch_Inc", "ops_tridMultiDimBatch_c",
+              "ops_tridMultiDimBatch_Inc_c", "ops_reduction_result_char", "ops_decl_const_char",
               "ops_decl_dat_char", "ops_arg_reduce", "ops_arg_gbl_char", "ops_halo_transfer",
               "ops_dat_fetch_data", "ops_dat_set_data", "ops_dat_get_raw_pointer"]:
         assert k in names, k
@@ -69,3 +70,50 @@ def test_header_compiles_as_c_and_cpp(tmp_path):
                        ' return a.dim;}\n')
     subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(src_cpp)],
                    check=True)
+
+
+def test_reference_tridiagonal_call_compiles_as_c_and_cpp(tmp_path):
+    """The reference's calls compile unchanged: C++ constructs the
+    ops_tridsolver_params class (apps/c/adi/adi.cpp style), C fills the struct
+    with the settings object; both reach the C-linkage symbols, whose
+    mangled-free names the library exports."""
+    inc = os.path.join(ROOT, "include")
+    src_cpp = tmp_path / "trid.cpp"
+    src_cpp.write_text(
+        '#include "ops_mi355x.h"\n'
+        'void solve(ops_block b, ops_dat a, ops_dat bb, ops_dat c, ops_dat d, ops_dat u, int *r) {\n'
+        '  ops_tridsolver_params p(b);\n'
+        '  ops_tridsolver_params q(b, ops_tridsolver_params::PCR);\n'
+        '  ops_tridMultiDimBatch(3, 0, r, a, bb, c, d, &p);\n'
+        '  ops_tridMultiDimBatch_Inc(3, 2, r, a, bb, c, d, u, &q);\n'
+        '}\n'
+        'static_assert(sizeof(ops_tridsolver_params) == sizeof(void *), "one data member");\n')
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", inc, str(src_cpp)], check=True)
+    src_c = tmp_path / "trid.c"
+    src_c.write_text(
+        '#include "ops_mi355x.h"\n'
+        'void solve(ops_block b, ops_dat a, ops_dat bb, ops_dat c, ops_dat d, int *r) {\n'
+        '  ops_tridsolver_params p = {ops_tridsolver_params_create(b, 0)};\n'
+        '  ops_tridMultiDimBatch(3, 1, r, a, bb, c, d, &p);\n'
+        '  ops_tridsolver_params_destroy(p.tridsolver_params);\n'
+        '}\n')
+    subprocess.run(["gcc", "-std=c99", "-fsyntax-only", "-I", inc, str(src_c)], check=True)
+    out = subprocess.run(["nm", "-D", "--defined-only", ops.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r" T (\w+)", out))
+    assert {"ops_tridMultiDimBatch", "ops_tridMultiDimBatch_Inc"} <= exported
+
+
+def test_init_flags_parsed_without_gpu():
+    """OPS_TILING / OPS_BLOCK_SIZE_* tokens: an out-of-range block size is a
+    configuration error raised before the device is touched (child process:
+    this interpreter never initialises HIP)."""
+    code = (
+        "from ops_amd import ops\n"
+        "try:\n"
+        "    ops.ops_init(argv=['app', 'OPS_TILING', 'OPS_BLOCK_SIZE_Z=-3'])\n"
+        "except ops.OPSError as e:\n"
+        "    print('code', e.code, e)\n"
+    )
+    r = subprocess.run(["python", "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert "code 2" in r.stdout and "OPS_BLOCK_SIZE" in r.stdout, r.stdout + r.stderr

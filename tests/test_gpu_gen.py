@@ -37,6 +37,9 @@ def gen(gpu_ops):
     L.gen_fill2d.argtypes = [vp, vp, ci, ci, cd]
     L.gen_double_in_place.argtypes = [vp, vp, ci, ci, ci]
     L.gen_cg_tail.argtypes = [vp, vp, vp, vp, ci, ci, ci, cd, cd, ci, dp]
+    L.gen_lowdim3d.argtypes = [vp, ci, ci, ci, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_float), dp]
+    L.gen_lowdim2d.argtypes = [vp, ci, ci, dp]
+    L.gen_lowdim_bad.argtypes = [vp, ci, ci, ci, ci]
     return L
 
 
@@ -146,6 +149,11 @@ def test_generated_heat_matches_hand_written(gpu_ops, gen, n):
     w_gen = ops.ops_dat_fetch_data(D["w"])
     assert np.array_equal(w_gen, ops.ops_dat_fetch_data(D["w2"]))
     assert abs(out[0] - pw) <= 1e-12 * abs(pw)
+    # and the oracle's operator with the same non-unit rx, ry, rz
+    Kx, Ky, Kz, p_ext = (ops.get_ext(D[k]) for k in ("Kx", "Ky", "Kz", "p"))
+    w_ref, pw_ref = O.heat_apply(3, nx, ny, nz, Kx, Ky, Kz, p_ext, rx, ry, rz)
+    assert np.array_equal(w_gen, O.interior(w_ref))
+    assert abs(out[0] - pw_ref) <= 1e-12 * abs(pw_ref)
     r1 = ops.ops_dat_fetch_data(D["r"])
     assert np.array_equal(r1, r0 - alpha * w_gen)
     assert abs(out[1] - np.sum(r1 * r1)) <= 1e-12 * np.sum(r1 * r1)
@@ -325,3 +333,63 @@ def test_generated_pair_with_a_neighbour_read_is_not_fused(gpu_ops, gen):
     ref_A, ref_h = O.laplace_run(init, 4)
     assert np.array_equal(hist, ref_h)
     assert np.array_equal(ops.ops_dat_fetch_data(A), O.interior(ref_A))
+
+
+# ------------------------------------------------------------------ low-dimensional dats
+@pytest.mark.parametrize("n", [(9, 7, 5), (130, 20, 11), (300, 3, 70)])
+def test_generated_lowdim_accumulation(gpu_ops, gen, n):
+    """Edge dats (size 1 in some dimensions) through strided stencils: MAX,
+    MIN and INC accumulation from a 3D field into 2D and 1D dats (double,
+    float, int) with the reference's reset rule
+    (ops/c/src/core/ops_dummy_singlenode.cpp:204-391): the first INC starts
+    from 0 whatever the dat held, a second INC adds to it, an INC after an
+    RW loop starts from 0 again; then one 3D loop reads every edge dat.
+    Integer-valued data, so every sum is exact in any order."""
+    ops = gpu_ops
+    nx, ny, nz = n
+    blk = ops.ops_decl_block(3, f"lowdim_{nx}")
+    steps = np.zeros((4, ny, nx))
+    rmax, zmin = np.zeros(nx), np.zeros(nz)
+    cnt = np.zeros(ny, np.int32)
+    ymax = np.zeros(ny, np.float32)
+    out = np.zeros((nz, ny, nx))
+    gen.gen_lowdim3d(blk.ptr, nx, ny, nz, _p(steps), _p(rmax), _p(zmin), cnt.ctypes.data_as(C.POINTER(C.c_int)),
+                     ymax.ctypes.data_as(C.POINTER(C.c_float)), _p(out))
+    ops._check()
+    Z, Y, X = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    f = ((X * 7 + Y * 3 + Z * 11) % 17).astype(np.float64) - 5.0
+    col = f.sum(axis=0)
+    assert np.array_equal(steps[0], col)
+    assert np.array_equal(steps[1], 2 * col)
+    assert np.array_equal(steps[2], col)
+    assert np.array_equal(steps[3], col)
+    assert np.array_equal(rmax, f.max(axis=(0, 1)))
+    assert np.array_equal(zmin, f.min(axis=(1, 2)))
+    assert np.array_equal(cnt, np.full(ny, nx * nz, np.int32))
+    assert np.array_equal(ymax, (f.astype(np.float32) + np.float32(0.5)).max(axis=(0, 2)))
+    ref = f + 2.0 * col[None] + 4.0 * rmax[None, None, :] + 8.0 * zmin[:, None, None] + 16.0 * cnt[None, :, None]
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("n", [(33, 17), (200, 64)])
+def test_generated_lowdim_2d_min(gpu_ops, gen, n):
+    ops = gpu_ops
+    nx, ny = n
+    blk = ops.ops_decl_block(2, f"lowdim2_{nx}")
+    m = np.zeros(nx)
+    gen.gen_lowdim2d(blk.ptr, nx, ny, _p(m))
+    ops._check()
+    Y, X = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    g = ((X * 5 + Y * 9) % 13).astype(np.float64) - 6.0
+    assert np.array_equal(m, g.min(axis=0))
+
+
+@pytest.mark.parametrize("mode,msg", [(0, "OPS_INC, OPS_MIN or OPS_MAX"), (1, "stride 2")])
+def test_generated_lowdim_refused(gpu_ops, gen, mode, msg):
+    """A plain write through a stride-0 stencil that puts several points on
+    one element, and a stride other than 0 or 1, are refused before launch."""
+    ops = gpu_ops
+    blk = ops.ops_decl_block(3, f"lowdim_bad{mode}")
+    gen.gen_lowdim_bad(blk.ptr, 8, 6, 4, mode)
+    with pytest.raises(ops.OPSError, match=msg):
+        ops._check()

@@ -3,12 +3,14 @@ reference's own libops_seq fixtures on the same seeded inputs.
 
 Bars: stencil sweeps bit-exact (FP contraction off); MIN/MAX reductions
 bit-exact; sum reductions and CG scalars within 1e-12 relative."""
+import ctypes as C
 import os
 
 import numpy as np
 import pytest
 
 from oracle import oracle_py as O
+from opshelp import heat_setup_py as _heat_setup_py
 
 pytestmark = pytest.mark.gpu
 
@@ -82,17 +84,6 @@ def test_laplace_f32_fixture_bit_exact(gpu_ops, golden):
     ops.ops_free_dat(An)
 
 
-def test_laplace_full_size_bit_exact(apps):
-    # the 2048^2 Laplace workload, 100 sweeps, against the threaded oracle
-    nx = ny = 2048
-    it = 100
-    init = O.laplace_init(nx, ny, 0)
-    err, hist, A = apps.laplace2d(nx, ny, it, 0, init)
-    A_ref, h_ref = O.laplace_run(init, it, nthreads=NT)
-    assert np.array_equal(hist, h_ref)
-    assert np.array_equal(A, O.interior(A_ref))
-
-
 def test_laplace_reference_known_answer(apps, golden):
     # apps/c/laplace2d_tutorial/step7: 4094^2, 100 iterations
     g, m = golden
@@ -121,56 +112,6 @@ def test_heat_cg_fixture_trajectory(apps, golden, ndim, key):
     h.close()
     for name in ("pw", "alpha", "beta"):
         assert relerr(hist[name], g[f"{key}_{name}"][:10]) < 1e-11, name
-
-
-def _heat_setup_py(ops, ndim, n, K_ext, rhs):
-    """TeaLeaf setup through the Python front end (same loops as the C++ app)."""
-    nx, ny, nz = n
-    blk = ops.ops_decl_block(ndim, "tea_lockstep")
-    size = [nx, ny] if ndim == 2 else [nx, ny, nz]
-    D = {q: ops.ops_decl_dat(blk, size, name=q) for q in ("density", "u", "w", "Kx", "Ky", "Kz", "r", "p")}
-    if ndim == 3:
-        S = dict(s0=[0, 0, 0], sp=[0, 0, 0, -1, 0, 0, 1, 0, 0, 0, -1, 0, 0, 1, 0, 0, 0, -1, 0, 0, 1],
-                 sx=[0, 0, 0, 1, 0, 0], sy=[0, 0, 0, 0, 1, 0], sz=[0, 0, 0, 0, 0, 1],
-                 sm=[0, 0, 0, -1, 0, 0, 0, -1, 0, 0, 0, -1])
-    else:
-        S = dict(s0=[0, 0], sp=[0, 0, 1, 0, -1, 0, 0, 1, 0, -1], sx=[0, 0, 1, 0], sy=[0, 0, 0, 1],
-                 sm=[0, 0, -1, 0, 0, -1])
-    st = {k: ops.ops_decl_stencil(ndim, len(v) // ndim, v, k) for k, v in S.items()}
-    red = ops.ops_decl_reduction_handle(8, "double", "red")
-    ops.set_ext(D["density"], K_ext)
-    ops.ops_dat_set_data(D["u"], rhs)
-    full = [-1, nx + 1, -1, ny + 1, -1, nz + 1][: 2 * ndim]
-    inner = [0, nx, 0, ny, 0, nz][: 2 * ndim]
-    ext = [0, nx + 1, 0, ny + 1, 0, nz + 1][: 2 * ndim]
-    A = lambda q, s, acc: ops.ops_arg_dat(D[q], 1, st[s], "double", acc)
-    R, W, RW, INC = ops.OPS_READ, ops.OPS_WRITE, ops.OPS_RW, ops.OPS_INC
-    ops.ops_par_loop("tea_leaf_recip_kernel", "recip", blk, ndim, full, A("w", "s0", W), A("density", "s0", R))
-    if ndim == 3:
-        ops.ops_par_loop("tea_leaf_common_init_Kx_Ky_Kz_kernel", "initK", blk, 3, ext, A("Kx", "s0", W),
-                         A("Ky", "s0", W), A("Kz", "s0", W), A("w", "sm", R))
-    else:
-        ops.ops_par_loop("tea_leaf_common_init_Kx_Ky_kernel", "initK", blk, 2, ext, A("Kx", "s0", W),
-                         A("Ky", "s0", W), A("w", "sm", R))
-    for q, name in enumerate(["Kx", "Ky", "Kz"][:ndim]):
-        lo, hi = list(full), list(full)
-        lo[2 * q:2 * q + 2] = [-1, 1]
-        hi[2 * q:2 * q + 2] = [n[q], n[q] + 1]
-        for rg in (lo, hi):
-            ops.ops_par_loop("tea_leaf_init_zero_kernel", "zero", blk, ndim, rg, A(name, "s0", W))
-    one = ops.ops_arg_gbl(1.0, 1, "double")
-    if ndim == 3:
-        ops.ops_par_loop("tea_leaf_common_init_kernel_3d", "init", blk, 3, inner, A("w", "s0", W), A("r", "s0", W),
-                         A("Kx", "sx", R), A("Ky", "sy", R), A("Kz", "sz", R), A("u", "sp", R), one,
-                         ops.ops_arg_gbl(1.0, 1, "double"), ops.ops_arg_gbl(1.0, 1, "double"))
-    else:
-        ops.ops_par_loop("tea_leaf_common_init_kernel", "init", blk, 2, inner, A("w", "s0", W), A("r", "s0", W),
-                         A("Kx", "sx", R), A("Ky", "sy", R), A("u", "sp", R), one, ops.ops_arg_gbl(1.0, 1, "double"))
-    ops.ops_par_loop("tea_leaf_yeqx_kernel", "yeqx", blk, ndim, inner, A("p", "s0", W), A("r", "s0", R))
-    ops.ops_par_loop("tea_leaf_dot_kernel", "dot", blk, ndim, inner, A("r", "s0", R), A("p", "s0", R),
-                     ops.ops_arg_reduce(red, 1, "double", INC))
-    rro = ops.ops_reduction_result(red)
-    return blk, D, st, red, inner, rro
 
 
 @pytest.mark.parametrize("ndim,n", [(2, (40, 34, 1)), (3, (26, 22, 18))])
@@ -425,6 +366,82 @@ def test_fetch_set_roundtrip_and_padding(gpu_ops):
     ops.ops_free_dat(x)
 
 
+def _whole_view(ops, dat, addr):
+    """numpy view of a dat's whole host or device-copied array (halos and x
+    padding included) from the raw pointer to point (0, 0)"""
+    core = dat.core
+    n = core.size[0] * core.size[1]
+    buf = (C.c_double * n).from_address(addr - core.base_offset)
+    return np.ctypeslib.as_array(buf).reshape(core.size[1], core.size[0])
+
+
+def test_raw_pointer_host_and_device(gpu_ops):
+    """ops_dat_get_raw_pointer / ops_dat_release_raw_data[_memspace]
+    (ops/c/src/core/ops_dummy_singlenode.cpp:662-735): the host pointer sees
+    the device's newest data; a write released with OPS_WRITE leaves the
+    host newer (dirty_hd 1) and the next loop and fetch see it; the device
+    pointer, written through HIP copies, leaves the device newer (2); an
+    OPS_READ release changes nothing; a loop on a locked dat and a release
+    without a matching get are errors."""
+    ops = gpu_ops
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemcpy.restype = C.c_int
+    nx, ny = 150, 9  # rows of 152 doubles >= 1 KiB: x-padded
+    blk = ops.ops_decl_block(2, "raw")
+    x = ops.ops_decl_dat(blk, [nx, ny], name="xraw")
+    s00 = ops.ops_decl_stencil(2, 1, [0, 0], "raw00")
+    h = ops.ops_decl_reduction_handle(8, "double", "rawsum")
+    v = O.u01(8, 8, nx * ny).reshape(ny, nx)
+    ops.ops_dat_set_data(x, v)
+    lib = ops.lib()
+    lib.ops_dat_release_raw_data_memspace.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    ms = C.c_int(ops.OPS_HOST)
+    addr = ops._call("ops_dat_get_raw_pointer", x.ptr, 0, s00, C.byref(ms))
+    assert ms.value == ops.OPS_HOST and x.core.locked_hd == ops.OPS_HOST
+    W = _whole_view(ops, x, addr)
+    assert np.array_equal(W[1:ny + 1, 1:nx + 1], v)
+    # a loop on the locked dat is refused
+    with pytest.raises(ops.OPSError, match="locked"):
+        ops.ops_par_loop("red_sum", "sum", blk, 2, [0, nx, 0, ny], ops.ops_arg_dat(x, 1, s00, "double", ops.OPS_READ),
+                         ops.ops_arg_reduce(h, 1, "double", ops.OPS_INC))
+    ops.ops_reduction_result(h)
+    W[1:ny + 1, 1:nx + 1] *= 2.0
+    ops._call("ops_dat_release_raw_data", x.ptr, 0, ops.OPS_WRITE)
+    assert x.core.dirty_hd == 1 and x.core.locked_hd == 0
+    ops.ops_par_loop("red_sum", "sum", blk, 2, [0, nx, 0, ny], ops.ops_arg_dat(x, 1, s00, "double", ops.OPS_READ),
+                     ops.ops_arg_reduce(h, 1, "double", ops.OPS_INC))
+    assert relerr(ops.ops_reduction_result(h), np.sum(2.0 * v)) < RTOL_RED
+    assert np.array_equal(ops.ops_dat_fetch_data(x), 2.0 * v)
+    # device pointer: overwrite row by row (the device rows carry the x padding)
+    ms = C.c_int(ops.OPS_DEVICE)
+    daddr = ops._call("ops_dat_get_raw_pointer", x.ptr, 0, s00, C.byref(ms))
+    assert ms.value == ops.OPS_DEVICE
+    core = x.core
+    pitch = core.size[0] * 8
+    new = 3.0 * v + 1.0
+    for j in range(ny):
+        row = np.ascontiguousarray(new[j])
+        assert hip.hipMemcpy(daddr + j * pitch, row.ctypes.data, nx * 8, 1) == 0
+    ms2 = C.c_int(ops.OPS_DEVICE)
+    lib.ops_dat_release_raw_data_memspace(x.ptr, 0, ops.OPS_RW, C.byref(ms2))
+    ops._check()
+    assert x.core.dirty_hd == 2
+    assert np.array_equal(ops.ops_dat_fetch_data(x), new)
+    # default memspace: the device holds the newest data
+    ms = C.c_int(0)
+    daddr2 = ops._call("ops_dat_get_raw_pointer", x.ptr, 0, s00, C.byref(ms))
+    assert ms.value == ops.OPS_DEVICE and daddr2 == daddr
+    back = np.zeros(nx)
+    assert hip.hipMemcpy(back.ctypes.data, daddr2 + 4 * pitch, nx * 8, 2) == 0
+    assert np.array_equal(back, new[4])
+    ops._call("ops_dat_release_raw_data", x.ptr, 0, ops.OPS_READ)
+    assert x.core.dirty_hd == 2 and x.core.locked_hd == 0
+    with pytest.raises(ops.OPSError, match="no matching"):
+        ops._call("ops_dat_release_raw_data", x.ptr, 0, ops.OPS_READ)
+    ops.ops_free_dat(x)
+
+
 def test_random_fill_uniform_matches_generator(gpu_ops):
     ops = gpu_ops
     blk = ops.ops_decl_block(2, "rng")
@@ -491,6 +508,53 @@ def test_trid_x_solves_bit_exact(apps, n):
     d, u = T.fetch()
     assert np.array_equal(d, d_ref) and np.array_equal(u, u_ref)
     T.close()
+
+
+@pytest.mark.parametrize("strategy", [0, 3])
+def test_trid_reference_c_entry_points(gpu_ops, strategy):
+    """ops_tridMultiDimBatch / _Inc through their C-linkage symbols with a
+    params object laid out as the reference class (ops/c/include/
+    ops_tridiag.h:45-81), over the whole interior and over a sub-range
+    (ops/c/src/tridiag/ops_tridiag.cpp:103-118: offset and extents from the
+    range), against the oracle."""
+    ops = gpu_ops
+    n = (37, 20, 11)
+    nx, ny, nz = n
+    blk = ops.ops_decl_block(3, f"trid_c{strategy}")
+    names = ("a", "b", "c", "d", "u")
+    D = {k: ops.ops_decl_dat(blk, list(n), name=k) for k in names}
+    fn = O.pcr_batch if strategy == 3 else O.thomas_batch
+    ctx = ops.TridParams(blk, strategy)
+    d_ref = O.trid_rhs(57, nx, ny, nz)
+    u_ref = np.zeros_like(d_ref)
+    ops.ops_dat_set_data(D["d"], d_ref)
+    ops.ops_dat_set_data(D["u"], u_ref)
+    cases = [(0, [0, nx, 0, ny, 0, nz], False), (1, [3, 30, 2, 19, 1, 9], False), (2, [0, nx, 0, ny, 0, nz], True),
+             (2, [5, 11, 0, 7, 2, 11], True), (0, [1, 36, 4, 5, 0, 3], False)]
+    for sd, rg, inc in cases:
+        a, b, c = O.trid_inputs(57 + sd, sd, nx, ny, nz)
+        for k, v in zip("abc", (a, b, c)):
+            ops.ops_dat_set_data(D[k], v)
+        if inc:
+            ops.ops_tridMultiDimBatch_Inc(3, sd, rg, D["a"], D["b"], D["c"], D["d"], D["u"], ctx)
+        else:
+            ops.ops_tridMultiDimBatch(3, sd, rg, D["a"], D["b"], D["c"], D["d"], ctx)
+        off = rg[0] + nx * (rg[2] + ny * rg[4])
+        dims = [rg[1] - rg[0], rg[3] - rg[2], rg[5] - rg[4]]
+        flat = lambda q: q.reshape(-1)[off:]
+        fn(3, sd, dims, [nx, ny, nz], flat(a), flat(b), flat(c), flat(d_ref), flat(u_ref) if inc else None,
+           nthreads=NT)
+        got_d = ops.ops_dat_fetch_data(D["d"])
+        got_u = ops.ops_dat_fetch_data(D["u"])
+        if strategy == 0:
+            assert np.array_equal(got_d, d_ref) and np.array_equal(got_u, u_ref), (sd, rg)
+        else:
+            scale = max(np.max(np.abs(d_ref)), 1.0)
+            assert np.max(np.abs(got_d - d_ref)) <= 1e-13 * scale, (sd, rg)
+            assert np.max(np.abs(got_u - u_ref)) <= 1e-13 * max(np.max(np.abs(u_ref)), 1.0), (sd, rg)
+            d_ref, u_ref = got_d, got_u  # continue from the GPU state
+    for q in D.values():
+        ops.ops_free_dat(q)
 
 
 def test_trid_full_size_thomas_and_pcr(apps):

@@ -35,7 +35,10 @@ def test_translates_dev_api_applications(tmp_path):
     # constants the body names become members; the two-component dat is MULTI
     assert re.search(r"struct opsk_mix_fields \{\s*double weights\[3\];", hip)
     assert re.search(r"struct opsk_fill_fields \{\s*int mult;\s*double shift;", hip)
-    assert "::opsmi_gen::Acc<double, 2, true> &v" in hip and "::opsmi_gen::Acc<int, 2, false> &m" in hip
+    assert "::opsmi_gen::Acc<double, 2, true, OPSMI_G> &v" in hip and "::opsmi_gen::Acc<int, 2, false, OPSMI_G> &m" in hip
+    # the general kernel (plain pointers, x steps from the launch) beside the restrict-qualified one
+    assert "::opsmi_gen::Acc<double, 2, true, true> a0(p0, A.d0, x, y, z);" in hip
+    assert "auto kern = L->alias ? opsg_fill_fields_alias : opsg_fill_fields;" in hip
     assert "store_partials<int, OPS_INC, 1>" in hip and "store_partials<double, OPS_INC, 2>" in hip
     # one fused kernel per pair of loops that follow each other in the source
     assert "opsg_fuse_mix_fields__summary(const opsa_mix_fields A, const opsa_summary B" in hip
@@ -56,6 +59,32 @@ def test_point_lists_follow_the_kernel_body():
     assert T.accessed_points("{ a.combine_max(0, 0, 1.0); }", "a", 2) is None
     # comments do not count
     assert T.accessed_points("{ a(0, 0) = 1; // a(5, 5)\n }", "a", 2) == [(-1, 0, 0, 0)]
+    # an accessor handed to a helper, bound to another name, or never named:
+    # the points it reaches are unknown
+    assert T.accessed_points("{ w(0, 0) = nb(b); }", "b", 2) is None
+    assert T.accessed_points("{ const ACC<double> &q = b; w(0, 0) = q(1, 0); }", "b", 2) is None
+    assert T.accessed_points("{ w(0, 0) = b(0, 0) + nb(b); }", "b", 2) is None
+    assert T.accessed_points("{ w(0, 0) = 1.0; }", "b", 2) is None
+
+
+def test_helper_reading_neighbours_is_not_fused(tmp_path):
+    """Two adjacent loops where the second reads the first's output through a
+    helper: the stub of the second lists no points (-1), so the runtime does
+    not take the dat as touched only at the point and keeps the loops apart."""
+    p = tmp_path / "helper.cpp"
+    p.write_text('#include "ops_seq_v2.h"\n'
+                 'inline double nb(const ACC<double> &b) { return b(1, 0) + b(-1, 0); }\n'
+                 'void k1(ACC<double> &a) { a(0, 0) = 1.0; }\n'
+                 'void k2(const ACC<double> &a, ACC<double> &w) { w(0, 0) = nb(a); }\n'
+                 'void run(ops_block b, ops_dat d, ops_dat e, ops_stencil s0, ops_stencil s3, int *r) {\n'
+                 '  ops_par_loop(k1, "k1", b, 2, r, ops_arg_dat(d, 1, s0, "double", OPS_WRITE));\n'
+                 '  ops_par_loop(k2, "k2", b, 2, r, ops_arg_dat(d, 1, s3, "double", OPS_READ),\n'
+                 '               ops_arg_dat(e, 1, s0, "double", OPS_WRITE));\n'
+                 '}\n')
+    T.translate(str(p), str(tmp_path / "out"))
+    hip = open(tmp_path / "out" / "helper_opsmi_kernels.hip").read()
+    stub = hip[hip.index("void ops_par_loop_k2("):]
+    assert "static const int npts[2] = {-1, 1};" in stub
 
 
 def _app(tmp_path, loop, kernel="void k(ACC<double> &a) { a(0, 0) = 1.0; }"):
@@ -90,7 +119,8 @@ def test_generated_library_built():
     dyn = subprocess.run(["nm", "-D", lib], capture_output=True, text=True, check=True).stdout
     exported = set(re.findall(r" T (\w+)", dyn))
     assert {"gen_laplace", "gen_fields", "gen_heat_step", "gen_laplace_bad_stencil", "gen_ramp1d", "gen_vec3",
-            "gen_opt_add", "gen_fill2d", "gen_double_in_place", "gen_cg_tail"} <= exported
+            "gen_opt_add", "gen_fill2d", "gen_double_in_place", "gen_cg_tail", "gen_lowdim3d", "gen_lowdim2d",
+            "gen_lowdim_bad"} <= exported
     # the generated stubs stay inside the library (no clash with the
     # hand-written ops_par_loop_apply_stencil of libopsmi.so)
     assert not any(n.startswith("ops_par_loop_") for n in exported)
