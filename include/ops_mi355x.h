@@ -62,6 +62,17 @@ typedef int ops_access;
 typedef int ops_arg_type;
 typedef int ops_memspace;
 
+/* IEEE binary16 element type of "half" dats (ops/c/include/ops_lib_core.h:
+ * 98-119): the compiler's _Float16 where it has one (clang and hipcc, as the
+ * translator builds applications; GCC 12 and later), otherwise its bits */
+#include <float.h>
+#if defined(__clang__) || defined(FLT16_MIN) || defined(__STDCPP_FLOAT16_T__)
+typedef _Float16 half;
+#else
+#include <stdint.h>
+typedef uint16_t half;
+#endif
+
 /* ---- object types: ops/c/include/ops_lib_core.h:181-552 ---- */
 typedef struct ops_block_core {
   int index;
@@ -282,6 +293,10 @@ void ops_tridMultiDimBatch_c(int ndim, int solvedim, int *range, ops_dat a, ops_
                              ops_tridsolver_params_c *ctx);
 void ops_tridMultiDimBatch_Inc_c(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c,
                                  ops_dat d, ops_dat u, ops_tridsolver_params_c *ctx);
+
+/* ops/c/include/ops_internal2.h:219: reports the first NaN in the dat
+ * (halos included) and exits with status 2 */
+void ops_NaNcheck(ops_dat dat);
 
 /* ---- HDF5 output: ops/c/include/ops_hdf5.h:130-166.  This backend has no
  * HDF5 library; each call reports OPS_NOT_IMPLEMENTED (error modes below) ---- */

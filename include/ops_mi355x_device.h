@@ -132,6 +132,29 @@ class Acc {
   }
 };
 
+// ------------------------------------------------------------ fused pairs
+// A written argument of a kernel run at a neighbouring point only for the
+// values it would write there (a fused pair whose second loop reads, at
+// stencil offsets, what the first loop writes): the writes go to registers,
+// nothing to memory.  The first loop writes such a dat only at the point
+// itself, so every access names that point; with C > 1 components the first
+// index is the component when NDIM + 1 indices are given.
+template <class T, int NDIM, int C>
+struct Cap {
+  mutable T v[C];
+  template <class... I>
+  __device__ __forceinline__ T &operator()(I... i) const {
+    constexpr int n = sizeof...(I);
+    static_assert(n == NDIM || n == NDIM + 1, "ACC: give NDIM offsets, or a component and NDIM offsets");
+    if constexpr (n == NDIM || C == 1) {
+      return v[0];
+    } else {
+      const int q[n] = {int(i)...};
+      return v[q[0]];
+    }
+  }
+};
+
 // ------------------------------------------------------------ reductions
 template <class T, int OP>
 __device__ __forceinline__ T ident() {

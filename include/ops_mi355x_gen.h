@@ -35,7 +35,7 @@ extern "C" {
 #define OPSMI_GEN_XP 2
 
 /* element type codes of a generated loop's arguments */
-enum { OPSMI_GEN_F64 = 0, OPSMI_GEN_F32 = 1, OPSMI_GEN_I32 = 2, OPSMI_GEN_NONE = 3 };
+enum { OPSMI_GEN_F64 = 0, OPSMI_GEN_F32 = 1, OPSMI_GEN_I32 = 2, OPSMI_GEN_NONE = 3, OPSMI_GEN_F16 = 4 };
 
 /* one dat argument as the kernel sees it: base = device address of
  * component 0 of point (0, 0, 0) of the dat's current buffer; strides in
@@ -105,6 +105,29 @@ typedef void (*opsmi_gen_fused_fn)(const opsmi_gen_launch *L1, const void *blob1
  * enabled (ops_mi_set_fusion level >= 1).  Reductions and results are the
  * same bits either way. */
 void ops_mi_gen_fused(opsmi_gen_launch_fn first, opsmi_gen_launch_fn second, opsmi_gen_fused_fn fused);
+
+/* Registers `fused` as the one-pass form of a loop launched by `first`
+ * directly followed by a loop launched by `second` that reads, at stencil
+ * offsets, dats the first one writes (the GPU form of the reference's
+ * run-time tiling of loop chains, ops/c/src/core/ops_lazy.cpp:380-1187:
+ * the reference skews tiles so a tile of the second loop runs after the
+ * tiles of the first it depends on; here each point of the second loop
+ * gets the first loop's output at its neighbours by running the first
+ * loop's kernel there into registers).  links holds nlinks pairs
+ * (i, j): argument i of the second loop reads what argument j of the first
+ * writes.  The queue launches it at fusion level 2 when: same block, range
+ * and dimension; no shared reduction handle; for each link the two
+ * arguments name the same dat, the first loop writes it with OPS_WRITE and
+ * the second reads it with OPS_READ at listed points; the first loop reads
+ * no dat it writes; the second loop touches every other dat the first
+ * writes only at the point; and no dat is written by the second loop and
+ * linked.  Dats the second loop writes and the first reads get a second
+ * buffer for the pass (the first loop's recomputation still reads the old
+ * values), which then becomes theirs.  Fields are the same bits as the two
+ * loops run one after the other; reductions of the second loop too, those
+ * of the first are accumulated at each point once. */
+void ops_mi_gen_fused_nb(opsmi_gen_launch_fn first, opsmi_gen_launch_fn second, opsmi_gen_fused_fn fused,
+                         const int *links, int nlinks);
 
 #ifdef __cplusplus
 } /* extern "C" */

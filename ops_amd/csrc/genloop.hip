@@ -166,6 +166,9 @@ void GenLoop::run_resets(hipStream_t s) const {
       case DT_F32:
         fill_value<float>(d, op == OPS_INC ? 0.0f : (op == OPS_MIN ? FLT_MAX : -FLT_MAX), s);
         break;
+      case DT_F16:  // 0, +65504, -65504 as binary16 bits
+        fill_value<unsigned short>(d, op == OPS_INC ? 0x0000 : (op == OPS_MIN ? 0x7bff : 0xfbff), s);
+        break;
       default:
         fill_value<int>(d, op == OPS_INC ? 0 : (op == OPS_MIN ? INT_MAX : INT_MIN), s);
         break;
@@ -178,6 +181,15 @@ namespace {
 
 std::map<std::pair<opsmi_gen_launch_fn, opsmi_gen_launch_fn>, opsmi_gen_fused_fn> &fused_table() {
   static std::map<std::pair<opsmi_gen_launch_fn, opsmi_gen_launch_fn>, opsmi_gen_fused_fn> m;
+  return m;
+}
+
+struct NbFused {
+  opsmi_gen_fused_fn fn = nullptr;
+  std::vector<std::pair<int, int>> links;  // (argument of the second loop, argument of the first)
+};
+std::map<std::pair<opsmi_gen_launch_fn, opsmi_gen_launch_fn>, NbFused> &nb_table() {
+  static std::map<std::pair<opsmi_gen_launch_fn, opsmi_gen_launch_fn>, NbFused> m;
   return m;
 }
 
@@ -233,8 +245,86 @@ int gen_pair_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
   return 2;
 }
 
+bool reads(const GenLoop &G, ops_dat d) {
+  for (size_t i = 0; i < G.dats.size(); i++)
+    if (G.dats[i] == d && G.acc[i] != OPS_WRITE) return true;
+  return false;
+}
+
+// The neighbour form (ops_mi_gen_fused_nb): the second loop reads, at
+// stencil offsets, what the first writes.  Conditions in ops_mi355x_gen.h.
+int gen_nb_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
+  if (inst().fusion < 2) return 0;
+  if (at + 1 >= run.size() || !run[at].gen || !run[at + 1].gen) return 0;
+  const GenLoop &A = *run[at].gen, &B = *run[at + 1].gen;
+  const auto it = nb_table().find({A.fn, B.fn});
+  if (it == nb_table().end()) return 0;
+  const NbFused &nb = it->second;
+  if (A.block != B.block || A.dims != B.dims || !same_range(A.r, B.r) || A.r.npts() <= 0) return 0;
+  if (share_handle(A, B)) return 0;
+  if (A.alias || B.alias || A.strided || B.strided || !A.resets.empty() || !B.resets.empty()) return 0;
+  std::vector<char> linked_b(B.dats.size(), 0);
+  std::vector<ops_dat> linked;
+  for (const auto &l : nb.links) {
+    const int i = l.first, j = l.second;
+    if (i < 0 || j < 0 || (size_t)i >= B.dats.size() || (size_t)j >= A.dats.size()) return 0;
+    if (!B.dats[i] || B.dats[i] != A.dats[j] || A.acc[j] != OPS_WRITE || B.acc[i] != OPS_READ) return 0;
+    linked_b[i] = 1;
+    linked.push_back(B.dats[i]);
+  }
+  // the first loop reads no dat it writes (its recomputation at a neighbour
+  // must see the values from before the pair)
+  for (size_t j = 0; j < A.dats.size(); j++)
+    if (A.dats[j] && A.acc[j] != OPS_READ && reads(A, A.dats[j])) return 0;
+  // the second loop: other dats the first writes only at the point; no
+  // write to a linked dat
+  for (size_t i = 0; i < B.dats.size(); i++) {
+    const ops_dat d = B.dats[i];
+    if (!d || linked_b[i]) continue;
+    if (A.writes(d) && !B.at_point[i]) return 0;
+    if (B.acc[i] != OPS_READ && std::find(linked.begin(), linked.end(), d) != linked.end()) return 0;
+  }
+  // dats the second loop writes and the first reads: second buffer
+  std::vector<ops_dat> spare;
+  for (size_t i = 0; i < B.dats.size(); i++) {
+    const ops_dat d = B.dats[i];
+    if (d && B.acc[i] != OPS_READ && reads(A, d) && std::find(spare.begin(), spare.end(), d) == spare.end())
+      spare.push_back(d);
+  }
+  const size_t off_b = A.pbytes;
+  ensure_partials(off_b + B.pbytes);
+  std::vector<char *> spare_base_of(spare.size());
+  for (size_t k = 0; k < spare.size(); k++) spare_base_of[k] = spare_base(spare[k], B.r);
+  LoopRec &ra = run[at], &rb = run[at + 1];
+  const std::string name = "fused(" + ra.name + "+" + rb.name + ")";
+  run_loop(name.c_str(), ra.bytes + rb.bytes, [&](hipStream_t st) {
+    char *pbase = (char *)inst().d_partials;
+    opsmi_gen_launch La, Lb;
+    A.fill(La, pbase);
+    B.fill(Lb, pbase + off_b);
+    for (size_t i = 0; i < B.dats.size(); i++)
+      for (size_t k = 0; k < spare.size(); k++)
+        if (B.dats[i] == spare[k]) Lb.dat[i].base = spare_base_of[k];
+    nb.fn(&La, A.blob(), &Lb, B.blob(), (void *)st);
+    OPSMI_HIP(hipGetLastError());
+    A.finish_reds(pbase, st);
+    B.finish_reds(pbase + off_b, st);
+  });
+  (void)s;
+  for (ops_dat d : spare) swap_spare(d, B.r);
+  note_fused(ra.name.c_str(), ra.bytes);
+  note_fused(rb.name.c_str(), rb.bytes);
+  for (const LoopRec *L : {&ra, &rb})
+    for (ops_dat d : L->writes)
+      if (std::find(spare.begin(), spare.end(), d) == spare.end()) note_external_write(d);
+  return 2;
+}
+
 struct RegisterGen {
-  RegisterGen() { register_fusion(gen_pair_rule); }
+  RegisterGen() {
+    register_fusion(gen_pair_rule);
+    register_fusion(gen_nb_rule);
+  }
 } g_register_gen;
 
 }  // namespace
@@ -367,6 +457,18 @@ void ops_mi_gen_fused(opsmi_gen_launch_fn first, opsmi_gen_launch_fn second, ops
   OPSMI_API_BEGIN
   if (!first || !second || !fused) fail(OPS_INVALID_ARGUMENT, "ops_mi_gen_fused: NULL function");
   fused_table()[{first, second}] = fused;
+  OPSMI_API_END()
+}
+
+void ops_mi_gen_fused_nb(opsmi_gen_launch_fn first, opsmi_gen_launch_fn second, opsmi_gen_fused_fn fused,
+                         const int *links, int nlinks) {
+  OPSMI_API_BEGIN
+  if (!first || !second || !fused || (nlinks > 0 && !links) || nlinks < 1 || nlinks > OPSMI_GEN_MAX_ARGS)
+    fail(OPS_INVALID_ARGUMENT, "ops_mi_gen_fused_nb: bad arguments");
+  NbFused nb;
+  nb.fn = fused;
+  for (int k = 0; k < nlinks; k++) nb.links.push_back({links[2 * k], links[2 * k + 1]});
+  nb_table()[{first, second}] = nb;
   OPSMI_API_END()
 }
 

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 
 namespace opsmi {
@@ -64,6 +65,7 @@ DType dtype_of(const char *type, int type_size) {
   if ((!strcmp(type, "int") || !strcmp(type, "int(4)") || !strcmp(type, "integer") ||
        !strcmp(type, "integer(4)") || !strcmp(type, "integer(kind=4)")) && type_size == 4)
     return DT_I32;
+  if (!strcmp(type, "half") && type_size == 2) return DT_F16;
   return DT_OTHER;
 }
 
@@ -72,6 +74,7 @@ static const char *dt_name(DType d) {
     case DT_F64: return "double";
     case DT_F32: return "float";
     case DT_I32: return "int";
+    case DT_F16: return "half";
     default: return "other";
   }
 }
@@ -281,13 +284,15 @@ Range3 check_gen_loop(const char *name, ops_block block, int dim, const int *ran
   for (int i = 0; i < nargs; i++) {
     const ops_arg &a = args[i];
     const DType want = (DType)types[i];
-    const int es = want == DT_F64 ? 8 : 4;
+    const int es = want == DT_F64 ? 8 : (want == DT_F16 ? 2 : 4);
     const int np = npts[i];
     const int *mine = pt;
     if (np > 0 && a.argtype == OPS_ARG_DAT) pt += 4 * np;
     if (a.argtype == OPS_ARG_IDX) continue;
-    if (want != DT_F64 && want != DT_F32 && want != DT_I32)
-      fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": arg ", i, " has no element type (only double, float, int)"));
+    if (want != DT_F64 && want != DT_F32 && want != DT_I32 && want != DT_F16)
+      fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": arg ", i, " has no element type (only double, float, int, half)"));
+    if (want == DT_F16 && a.argtype != OPS_ARG_DAT)
+      fail(OPS_NOT_IMPLEMENTED, cat("loop ", name, ": arg ", i, ": half values are supported in dats only"));
     if (a.argtype == OPS_ARG_GBL) {
       if (a.acc == OPS_READ) {
         if (!a.data || a.dim < 1 || a.elem_size != es)
@@ -1395,6 +1400,43 @@ void ops_fetch_halo_hdf5_file(ops_halo halo, const char *file_name) {
   OPSMI_API_BEGIN
   (void)halo;
   no_hdf5("ops_fetch_halo_hdf5_file", file_name);
+  OPSMI_API_END()
+}
+
+// ops/c/src/core/ops_dummy_singlenode.cpp:871-878, ops/c/src/core/
+// ops_lib_core.cpp:1845-1960: scans the whole dat (halos included) on the
+// host; the first NaN is reported and the process exits with status 2
+void ops_NaNcheck(ops_dat dat) {
+  OPSMI_API_BEGIN
+  require_init();
+  if (!dat) fail(OPS_INVALID_ARGUMENT, "ops_NaNcheck: NULL dat");
+  flush_all();
+  ensure_host(dat);
+  printf("ops_NaNcheck_core called for %s \n", dat->name);
+  const DType dt = impl(dat)->dt;
+  const int nd = dat->block->dims;
+  long n[3] = {1, 1, 1};
+  for (int q = 0; q < nd && q < 3; q++) n[q] = dat->size[q];
+  for (long k = 0; k < n[2]; k++)
+    for (long j = 0; j < n[1]; j++)
+      for (long i = 0; i < n[0]; i++)
+        for (int d = 0; d < dat->dim; d++) {
+          const size_t off = (size_t)((k * n[1] + j) * n[0] + i) * dat->dim + d;
+          bool nan = false;
+          if (dt == DT_F64) nan = std::isnan(((const double *)dat->data)[off]);
+          else if (dt == DT_F32) nan = std::isnan(((const float *)dat->data)[off]);
+          else if (dt == DT_F16) {
+            const uint16_t h = ((const uint16_t *)dat->data)[off];
+            nan = (h & 0x7c00u) == 0x7c00u && (h & 0x03ffu) != 0;
+          }
+          if (nan) {
+            const long idx[3] = {i + dat->d_m[0], j + (nd > 1 ? dat->d_m[1] : 0), k + (nd > 2 ? dat->d_m[2] : 0)};
+            printf("Error: NaN detected at element dim:%d,index:(%ld", d, idx[0]);
+            for (int q = 1; q < nd && q < 3; q++) printf(",%ld", idx[q]);
+            printf(")\n");
+            exit(2);
+          }
+        }
   OPSMI_API_END()
 }
 

@@ -55,7 +55,7 @@ std::string cat(const A &...a) {
 }
 
 // ---------------------------------------------------------------- types
-enum DType { DT_F64 = 0, DT_F32 = 1, DT_I32 = 2, DT_OTHER = 3 };
+enum DType { DT_F64 = 0, DT_F32 = 1, DT_I32 = 2, DT_OTHER = 3, DT_F16 = 4 };
 DType dtype_of(const char *type, int type_size);
 
 // device-side view of one dat argument: pointer to point (0,0,0) and strides

@@ -41,7 +41,8 @@ import os
 import re
 import sys
 
-TYPES = {"double": ("OPSMI_GEN_F64", 8), "float": ("OPSMI_GEN_F32", 4), "int": ("OPSMI_GEN_I32", 4)}
+TYPES = {"double": ("OPSMI_GEN_F64", 8), "float": ("OPSMI_GEN_F32", 4), "int": ("OPSMI_GEN_I32", 4),
+         "half": ("OPSMI_GEN_F16", 2)}
 GBL_CAP = 64  # values a generated kernel holds of an ops_arg_gbl whose dim is not a literal
 ACCESS = {"OPS_READ", "OPS_WRITE", "OPS_RW", "OPS_INC", "OPS_MIN", "OPS_MAX"}
 
@@ -140,12 +141,13 @@ def int_literal(text):
 
 # ------------------------------------------------------------------ model
 class Arg:
-    def __init__(self, kind, typ=None, acc=None, dim=None, multi=False):
+    def __init__(self, kind, typ=None, acc=None, dim=None, multi=False, expr=None):
         self.kind = kind      # dat | gbl | reduce | idx
         self.typ = typ        # double | float | int
         self.acc = acc        # OPS_* access
         self.dim = dim        # gbl/reduce component count
         self.multi = multi    # dat: may have more than one component
+        self.expr = expr      # dat: the dat expression as written (whitespace removed)
 
 
 class Loop:
@@ -153,6 +155,7 @@ class Loop:
         self.kernel, self.ndim, self.args = kernel, ndim, args
         self.span = span            # (start of 'ops_par_loop', end of the first argument's comma)
         self.name_span = name_span
+        self.path = None            # the file the call is in
 
 
 class Kernel:
@@ -178,7 +181,7 @@ def parse_arg(src, m, span, where):
         typ = string_literal(items[3], f"{where}: {fn} type")
         acc = items[4]
         d = int_literal(items[1])
-        return Arg("dat", typ, acc, d, multi=(d != 1))
+        return Arg("dat", typ, acc, d, multi=(d != 1), expr=re.sub(r"\s+", "", items[0]))
     if len(items) != 4:
         raise TranslateError(f"{where}: {fn} needs 4 arguments")
     typ = string_literal(items[2], f"{where}: {fn} type")
@@ -213,11 +216,35 @@ def find_loops(src, m, path):
         for x in args:
             if x.kind != "idx":
                 if x.typ not in TYPES:
-                    raise TranslateError(f"{where}: type {x.typ!r} (only double, float, int)")
+                    raise TranslateError(f"{where}: type {x.typ!r} (only double, float, int, half)")
                 if x.acc not in ACCESS:
                     raise TranslateError(f"{where}: access {x.acc!r} is not an OPS_* access")
         loops.append(Loop(kernel, nd, args, (mo.start(), spans[0][1] + 1), spans[1]))
+        loops[-1].path = path
     return loops
+
+
+def loop_body_pairs(src, m, loops):
+    """(last, first) ops_par_loop calls of every while/for/do body holding
+    two or more of them: the pair that meets in the queue when the body
+    runs again (the last loop of one pass, then the first of the next)."""
+    out = []
+    for mo in re.finditer(r"\b(while|for)\s*\(|\bdo\s*\{", m):
+        if mo.group(0).startswith("do"):
+            ob = mo.end() - 1
+        else:
+            cl = match_close(m, mo.end() - 1)
+            j = cl + 1
+            while j < len(m) and m[j].isspace():
+                j += 1
+            if j >= len(m) or m[j] != "{":
+                continue
+            ob = j
+        cb = match_close(m, ob)
+        inside = [lp for lp in loops if ob < lp.span[0] < cb]
+        if len(inside) >= 2:
+            out.append((inside[-1], inside[0]))
+    return out
 
 
 def find_consts(src, m):
@@ -332,21 +359,21 @@ def kernel_code(loop, kern, consts):
     for c in used:
         d, t = consts[c]
         L.append(f"  {t} {c}{'[' + str(d) + ']' if d > 1 else ''};")
-    # the accessor type carries the kernel variant (OPSMI_G: x steps from
-    # the launch), so the call operator is a template over it when the
-    # kernel takes dats
-    ps = []
-    for (text, pname), a in zip(kern.params, loop.args):
+    # every ACC<T> parameter becomes a template type: the plain kernels pass
+    # ::opsmi_gen::Acc accessors (restrict-qualified or general), fused
+    # kernels also pass accessors that capture a write or hold values
+    # another loop computed
+    ps, tps = [], []
+    for i, ((text, pname), a) in enumerate(zip(kern.params, loop.args)):
         if a.kind == "dat":
-            multi = "true" if a.multi else "false"
-            new, cnt = re.subn(r"\bACC\s*<\s*([^>]+?)\s*>", rf"::opsmi_gen::Acc<\1, {loop.ndim}, {multi}, OPSMI_G>",
-                               text)
+            new, cnt = re.subn(r"\bACC\s*<\s*([^>]+?)\s*>", f"OPSMI_A{i}", text)
             if cnt != 1:
                 raise TranslateError(f"kernel {K}: parameter {text!r} for an ops_arg_dat must be an ACC<T> reference")
             ps.append(new)
+            tps.append(f"class OPSMI_A{i}")
         else:
             ps.append(text)
-    tmpl = "template <bool OPSMI_G> " if any(a.kind == "dat" for a in loop.args) else ""
+    tmpl = f"template <{', '.join(tps)}> " if tps else ""
     L.append(f"  {tmpl}__device__ __forceinline__ void operator()({', '.join(ps)}) const {kern.body}")
     L.append("};")
     # by-value data: constants and gbl values
@@ -472,6 +499,164 @@ def fused_code(l1, l2):
     return "\n".join(L), f"ops_mi_gen_fused(&opsl_{K1}, &opsl_{K2}, &opsf_{F});"
 
 
+def nb_links(l1, l2, pts2):
+    """(i, j): argument i of l2 reads, at stencil offsets other than the
+    point itself, the dat l1 writes through argument j (same dat
+    expression, same type and component count; l1 writes it with OPS_WRITE,
+    l2 reads it with OPS_READ at literal offsets).  Empty when no argument
+    of l2 reads l1's output at a neighbour."""
+    links, nonzero = [], False
+    for i, b in enumerate(l2.args):
+        if b.kind != "dat" or b.acc != "OPS_READ" or pts2[i] is None:
+            continue
+        for j, a in enumerate(l1.args):
+            if a.kind == "dat" and a.acc == "OPS_WRITE" and a.expr == b.expr and a.typ == b.typ and \
+               a.multi == b.multi and (not a.multi or a.dim is not None):
+                links.append((i, j))
+                nonzero = nonzero or any(q[1:] != (0, 0, 0) for q in pts2[i])
+                break
+    return links if nonzero else []
+
+
+def nb_fused_code(l1, l2, pts1, pts2, links):
+    """One kernel running, at each point p, the first loop's kernel and then
+    the second's, where the second reads at neighbours p + o what the first
+    writes (the linked arguments): the first loop's output at p + o is
+    recomputed there into registers (its kernel run at p + o with its written
+    arguments captured, its reductions dropped) when p + o lies in the range,
+    and read from memory (not written by the first loop) when it does not.
+    The library launches it only when that gives the results of the two
+    loops run one after the other (genloop.hip)."""
+    K1, K2 = l1.kernel, l2.kernel
+    F = f"{K1}__{K2}"
+    nd = l1.ndim
+    lk = dict(links)
+    ptrs = []
+    for pfx, lp in (("f", l1), ("s", l2)):
+        ptrs += [f"{'const ' if a.acc == 'OPS_READ' else ''}{a.typ} *{pfx}p{i}"
+                 for i, a in enumerate(lp.args) if a.kind == "dat"]
+    L = [f"// ---------------------------------------------------------------- {K2} reading {K1}'s output at neighbours"]
+    # the held values of each linked argument: one slot per listed (component, offset)
+    slots = {}
+    for i, j in links:
+        b = l2.args[i]
+        sl = pts2[i]
+        slots[i] = sl
+        L.append(f"struct opssh_{F}_{i} {{")
+        L.append(f"  {b.typ} v[{len(sl)}];")
+        if b.multi:
+            sig = ", ".join(["int c"] + [f"int d{q}" for q in range(nd)])
+        else:
+            sig = ", ".join(f"int d{q}" for q in range(nd))
+        L.append(f"  __device__ __forceinline__ const {b.typ} &operator()({sig}) const {{")
+        expr = f"v[{len(sl) - 1}]"
+        for k in range(len(sl) - 2, -1, -1):
+            c, dx, dy, dz = sl[k]
+            conds = [f"d0 == {dx}"] + ([f"d1 == {dy}"] if nd > 1 else []) + ([f"d2 == {dz}"] if nd > 2 else [])
+            if b.multi:
+                conds = [f"c == {max(c, 0)}"] + conds
+            expr = f"({' && '.join(conds)}) ? v[{k}] : {expr}"
+        L.append(f"    return {expr};")
+        L.append("  }")
+        L.append("};")
+    L.append(f"__global__ __launch_bounds__(256) void opsg_fusenb_{F}(const opsa_{K1} A, const opsa_{K2} B"
+             f"{''.join(', ' + p for p in ptrs)}) {{")
+    L += red_init_lines(l1, "f") + red_init_lines(l2, "s")
+    L.append("  ::opsmi_gen::for_each_point(A.L, [&](int x, int y, int z) {")
+    L.append("    // the first loop at the point")
+    L += point_lines(l1, "A", "f")
+    # distinct offsets over the linked arguments' slots
+    offs = sorted({q[1:] for i in slots for q in slots[i]})
+    for i in slots:
+        L.append(f"    opssh_{F}_{i} sh{i};")
+    for o in offs:
+        dx, dy, dz = o
+        L.append(f"    {{  // offset ({dx}, {dy}, {dz})")
+        fills = [(i, k, lk[i], q[0]) for i in slots for k, q in enumerate(slots[i]) if q[1:] == o]
+
+        def val(src_acc, j, c):
+            a = l1.args[j]
+            idx = [str(dx)] + ([str(dy)] if nd > 1 else []) + ([str(dz)] if nd > 2 else [])
+            if src_acc.startswith("fa"):
+                if a.multi:
+                    return f"{src_acc}({max(c, 0)}, {', '.join(idx)})"
+                return f"{src_acc}({', '.join(idx)})"
+            return f"{src_acc}.v[{max(c, 0)}]"
+        if o == (0, 0, 0):
+            for i, k, j, c in fills:
+                L.append(f"      sh{i}.v[{k}] = {val(f'fa{j}', j, c)};")
+        else:
+            conds = [f"x + ({dx}) >= A.L.x0 && x + ({dx}) < A.L.x1"]
+            if nd > 1:
+                conds.append(f"y + ({dy}) >= A.L.y0 && y + ({dy}) < A.L.y1")
+            if nd > 2:
+                conds.append(f"z + ({dz}) >= A.L.z0 && z + ({dz}) < A.L.z1")
+            L.append(f"      if ({' && '.join(conds)}) {{")
+            L.append(f"        const int xo = x + ({dx}), yo = y + ({dy}), zo = z + ({dz});")
+            call = []
+            for j, a in enumerate(l1.args):
+                if a.kind == "dat":
+                    if a.acc == "OPS_READ":
+                        L.append(f"        ::opsmi_gen::Acc<{a.typ}, {nd}, {'true' if a.multi else 'false'}, false> "
+                                 f"ra{j}(fp{j}, A.d{j}, xo, yo, zo);")
+                        call.append(f"ra{j}")
+                    else:
+                        L.append(f"        ::opsmi_gen::Cap<{a.typ}, {nd}, {a.dim if a.multi else 1}> cp{j}{{}};")
+                        call.append(f"cp{j}")
+                elif a.kind == "idx":
+                    L.append(f"        int ridx{j}[3] = {{xo, yo, zo}};")
+                    call.append(f"ridx{j}")
+                elif a.kind == "gbl":
+                    call.append(f"A.b.g{j}")
+                else:
+                    L.append(f"        {a.typ} rred{j}[{a.dim}];")
+                    L.append(f"        for (int q = 0; q < {a.dim}; q++) rred{j}[q] = ::opsmi_gen::ident<{a.typ}, {a.acc}>();")
+                    call.append(f"rred{j}")
+            L.append(f"        A.b.k({', '.join(call)});")
+            for i, k, j, c in fills:
+                L.append(f"        sh{i}.v[{k}] = {val(f'cp{j}', j, c)};")
+            L.append("      } else {")
+            for i, k, j, c in fills:
+                L.append(f"        sh{i}.v[{k}] = {val(f'fa{j}', j, c)};")
+            L.append("      }")
+        L.append("    }")
+    L.append("    // the second loop, its linked arguments read from the held values")
+    call = []
+    for i, a in enumerate(l2.args):
+        if a.kind == "dat":
+            if i in lk:
+                call.append(f"sh{i}")
+            else:
+                L.append(f"    ::opsmi_gen::Acc<{a.typ}, {nd}, {'true' if a.multi else 'false'}, false> "
+                         f"sa{i}(sp{i}, B.d{i}, x, y, z);")
+                call.append(f"sa{i}")
+        elif a.kind == "idx":
+            L.append(f"    int sidx{i}[3] = {{x, y, z}};")
+            call.append(f"sidx{i}")
+        elif a.kind == "gbl":
+            call.append(f"B.b.g{i}")
+        else:
+            call.append(f"sred{i}")
+    L.append(f"    B.b.k({', '.join(call)});")
+    L.append("  });")
+    L += red_store_lines(l1, "A", "f") + red_store_lines(l2, "B", "s")
+    L.append("}")
+    L.append(f"void opsfnb_{F}(const opsmi_gen_launch *L1, const void *blob1, const opsmi_gen_launch *L2, "
+             "const void *blob2, void *stream) {")
+    L.append(f"  opsa_{K1} A;")
+    L.append(f"  opsfill_{K1}(A, L1, blob1);")
+    L.append(f"  opsa_{K2} B;")
+    L.append(f"  opsfill_{K2}(B, L2, blob2);")
+    L.append(f"  hipLaunchKernelGGL(opsg_fusenb_{F}, dim3(L1->grid[0], L1->grid[1], L1->grid[2]), "
+             f"dim3(::opsmi_gen::kWave, ::opsmi_gen::kRows), 0, (hipStream_t)stream, A, B"
+             f"{launch_ptrs(l1, 'L1')}{launch_ptrs(l2, 'L2')});")
+    L.append("}")
+    flat = ", ".join(f"{i}, {j}" for i, j in links)
+    reg = (f"{{ static const int lk[{2 * len(links)}] = {{{flat}}}; "
+           f"ops_mi_gen_fused_nb(&opsl_{K1}, &opsl_{K2}, &opsfnb_{F}, lk, {len(links)}); }}")
+    return "\n".join(L), reg
+
+
 def adjacent_pairs(loops):
     """(first, second) loops of each pair of ops_par_loop calls that follow
     each other in the source, once per kernel pair, same dimension only."""
@@ -551,10 +736,11 @@ def translate(app, out_dir, inc_dirs=()):
          "#ifndef OPS_FUN_PREFIX", "#define OPS_FUN_PREFIX __host__ __device__", "#endif"]
     H += [f"#include \"{name}\"" for name in user_incs]
     H += ["", "#define OPSMI_GEN_STUB __attribute__((visibility(\"hidden\")))", "", "namespace {"]
-    stubs, sigs = [], []
+    stubs, sigs, kpoints = [], [], {}
     for k in order:
         lp, kern = kernels[k]
         code, used, points = kernel_code(lp, kern, consts)
+        kpoints[k] = points
         H.append(code)
         s, sig = stub_code(lp, used, consts, points)
         stubs.append(s)
@@ -562,6 +748,23 @@ def translate(app, out_dir, inc_dirs=()):
     regs = []
     for a, b in adjacent_pairs(loops):
         code, reg = fused_code(kernels[a.kernel][0], kernels[b.kernel][0])
+        H.append(code)
+        regs.append(reg)
+    # pairs whose second loop reads the first's output at neighbours:
+    # adjacent calls and the last and first call of a loop body, matched by
+    # the dat expressions as written (the library checks the dats at run time)
+    seen = set()
+    for a, b in adjacent_pairs(loops) + loop_body_pairs(src, m, loops):
+        if a.ndim != b.ndim or (a.kernel, b.kernel) in seen or a.kernel == b.kernel:
+            continue
+        seen.add((a.kernel, b.kernel))
+        l1, l2 = kernels[a.kernel][0], kernels[b.kernel][0]
+        # match expressions on these calls, the argument lists on the kernels' first calls
+        links = nb_links(a, b, kpoints[b.kernel])
+        if not links or any(p is None for p, x in zip(kpoints[a.kernel], l1.args) if x.kind == "dat"
+                            and x.acc != "OPS_READ"):
+            continue
+        code, reg = nb_fused_code(l1, l2, kpoints[a.kernel], kpoints[b.kernel], links)
         H.append(code)
         regs.append(reg)
     if regs:

@@ -40,6 +40,10 @@ def gen(gpu_ops):
     L.gen_lowdim3d.argtypes = [vp, ci, ci, ci, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_float), dp]
     L.gen_lowdim2d.argtypes = [vp, ci, ci, dp]
     L.gen_lowdim_bad.argtypes = [vp, ci, ci, ci, ci]
+    ip = C.POINTER(C.c_int)
+    L.gen_chain_src_lap.argtypes = [vp, vp, vp, vp, ip, ip, dp]
+    L.gen_chain_mix_avg.argtypes = [vp, vp, vp, vp, ci, ci, ci, dp]
+    L.gen_chain_refused.argtypes = [vp, vp, vp, vp, vp, ci, ci, ci]
     return L
 
 
@@ -393,3 +397,166 @@ def test_generated_lowdim_refused(gpu_ops, gen, mode, msg):
     gen.gen_lowdim_bad(blk.ptr, 8, 6, 4, mode)
     with pytest.raises(ops.OPSError, match=msg):
         ops._check()
+
+
+# ------------------------------------------------------------------ neighbour-form fused pairs
+def _levels(ops, fn):
+    """fn() at fusion levels 0, 1 and 2 (the library's default is 2)"""
+    out = []
+    for lev in (0, 1, 2):
+        ops.lib().ops_mi_set_fusion(lev)
+        try:
+            ops.ops_mi_reset_stats()
+            out.append(fn())
+        finally:
+            ops.lib().ops_mi_set_fusion(2)
+    return out
+
+
+def _fused_count(ops, name):
+    st = ops.ops_mi_loop_stats(name)
+    return st[0] if st else 0
+
+
+@pytest.mark.parametrize("n,shift", [((70, 20, 13), 0), ((130, 9, 40), 0), ((41, 30, 17), 1)])
+def test_generated_nb_fusion_src_lap7(gpu_ops, gen, n, shift):
+    """u = 2a + 1, then w = 7-point operator of u with sum of w^2: one pass at
+    fusion level 2 when both loops share the range (u at neighbours
+    recomputed from a inside the range, read from memory in the halo), two
+    passes otherwise.  Fields bit-identical across levels and to numpy; the
+    sum within 1e-12 and the same bits across levels."""
+    ops = gpu_ops
+    nx, ny, nz = n
+    ext = (nz + 2, ny + 2, nx + 2)
+    u0 = O.u01(71, 1, int(np.prod(ext))).reshape(ext) - 0.5
+    a0 = O.u01(71, 2, int(np.prod(ext))).reshape(ext)
+    r1 = [0, nx, 0, ny, 0, nz]
+    r2 = [0, nx, 0, ny, shift, nz]
+
+    def run():
+        blk = ops.ops_decl_block(3, f"chn_{nx}")
+        u, a, w = (ops.ops_decl_dat(blk, [nx, ny, nz], name=k) for k in ("chu", "cha", "chw"))
+        ops.set_ext(u, u0)
+        ops.set_ext(a, a0)
+        sv = np.zeros(1)
+        gen.gen_chain_src_lap(blk.ptr, u.ptr, a.ptr, w.ptr, (C.c_int * 6)(*r1), (C.c_int * 6)(*r2), _p(sv))
+        ops._check()
+        out = (ops.get_ext(u), ops.get_ext(w), sv[0], _fused_count(ops, "fused(ch_src+ch_lap7)"))
+        for d in (u, a, w):
+            ops.ops_free_dat(d)
+        return out
+
+    runs = _levels(ops, run)
+    ref_u = u0.copy()
+    ref_u[1:-1, 1:-1, 1:-1] = 2.0 * a0[1:-1, 1:-1, 1:-1] + 1.0
+    U = ref_u
+    c = U[1:-1, 1:-1, 1:-1]
+    wint = 6.0 * c - (U[1:-1, 1:-1, :-2] + U[1:-1, 1:-1, 2:]) - (U[1:-1, :-2, 1:-1] + U[1:-1, 2:, 1:-1]) - \
+        (U[:-2, 1:-1, 1:-1] + U[2:, 1:-1, 1:-1])
+    z0 = shift
+    for lev, (uu, ww, sv, cnt) in enumerate(runs):
+        assert np.array_equal(uu, ref_u), lev
+        assert np.array_equal(ww[1 + z0:-1, 1:-1, 1:-1], wint[z0:]), lev
+        ref_s = np.sum(wint[z0:] * wint[z0:])
+        assert abs(sv - ref_s) <= 1e-12 * ref_s, lev
+        assert cnt == (1 if lev == 2 and shift == 0 else 0), (lev, cnt)
+    assert runs[0][2] == runs[2][2] or shift != 0
+
+
+@pytest.mark.parametrize("n", [(50, 37), (300, 41)])
+def test_generated_nb_fusion_double_buffered(gpu_ops, gen, n):
+    """u = a - b/2 then a = average of u's neighbours with max(a), six times:
+    the second loop overwrites a, which the first reads, so the fused pass
+    writes a into a second buffer.  Bit-identical across levels and to numpy."""
+    ops = gpu_ops
+    nx, ny = n
+    it = 6
+    ext = (ny + 2, nx + 2)
+    a0 = O.u01(73, 1, int(np.prod(ext))).reshape(ext)
+    b0 = O.u01(73, 2, int(np.prod(ext))).reshape(ext) - 0.5
+    u0 = O.u01(73, 3, int(np.prod(ext))).reshape(ext)
+
+    def run():
+        blk = ops.ops_decl_block(2, f"chm_{nx}")
+        u, a, b = (ops.ops_decl_dat(blk, [nx, ny], name=k) for k in ("cmu", "cma", "cmb"))
+        ops.set_ext(u, u0)
+        ops.set_ext(a, a0)
+        ops.set_ext(b, b0)
+        mx = np.zeros(it)
+        gen.gen_chain_mix_avg(blk.ptr, u.ptr, a.ptr, b.ptr, nx, ny, it, _p(mx))
+        ops._check()
+        out = (ops.get_ext(u), ops.get_ext(a), mx, _fused_count(ops, "fused(ch_mix+ch_avg)"))
+        for d in (u, a, b):
+            ops.ops_free_dat(d)
+        return out
+
+    runs = _levels(ops, run)
+    U, A, B = u0.copy(), a0.copy(), b0.copy()
+    mxs = []
+    for _ in range(it):
+        U[1:-1, 1:-1] = A[1:-1, 1:-1] - 0.5 * B[1:-1, 1:-1]
+        A[1:-1, 1:-1] = 0.25 * (U[1:-1, 2:] + U[1:-1, :-2] + U[2:, 1:-1] + U[:-2, 1:-1])
+        mxs.append(A[1:-1, 1:-1].max())
+    for lev, (uu, aa, mx, cnt) in enumerate(runs):
+        assert np.array_equal(uu, U) and np.array_equal(aa, A), lev
+        assert np.array_equal(mx, np.array(mxs)), lev
+        assert cnt == (it if lev == 2 else 0), (lev, cnt)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_generated_nb_fusion_refused(gpu_ops, gen, mode):
+    """The dat the source links is not the one the second loop reads (the
+    variable was reassigned between the calls), or the first loop reads the
+    dat it writes: the pair runs as two passes, with their results."""
+    ops = gpu_ops
+    nx, ny = 67, 23
+    ext = (ny + 2, nx + 2)
+    blk = ops.ops_decl_block(2, f"chr_{mode}")
+    D = {k: ops.ops_decl_dat(blk, [nx, ny], name=f"chr{mode}{k}") for k in ("u", "v", "w", "o")}
+    init = {k: O.u01(79, i, int(np.prod(ext))).reshape(ext) for i, k in enumerate(("u", "v", "w", "o"))}
+    for k, d in D.items():
+        ops.set_ext(d, init[k])
+    ops.ops_mi_reset_stats()
+    gen.gen_chain_refused(blk.ptr, D["u"].ptr, D["v"].ptr, D["w"].ptr, D["o"].ptr, nx, ny, mode)
+    ops._check()
+    assert _fused_count(ops, "fused(ch_scale+ch_diff)") == 0
+    U = init["u"].copy()
+    U[1:-1, 1:-1] = 3.0 * (init["v"] if mode == 1 else init["u"])[1:-1, 1:-1]
+    P = init["o"] if mode == 1 else U
+    wref = P[1:-1, 2:] - P[1:-1, :-2] + P[2:, 1:-1] - P[:-2, 1:-1]
+    assert np.array_equal(ops.get_ext(D["u"]), U)
+    assert np.array_equal(ops.ops_dat_fetch_data(D["w"]), wref)
+    for d in D.values():
+        ops.ops_free_dat(d)
+
+
+@pytest.mark.parametrize("n", [(64, 48), (257, 33)])
+def test_generated_laplace_fused_across_levels(gpu_ops, gen, n):
+    """The Laplace loop through the translator: each copy runs with the next
+    apply_stencil as one pass at fusion level 2 (apply_stencil reads A, which
+    copy writes, at its four neighbours); history and field bit-identical at
+    every level and to the oracle."""
+    ops = gpu_ops
+    nx, ny = n
+    it = 20
+    init = O.laplace_init(nx, ny, 0)
+
+    def run():
+        blk = ops.ops_decl_block(2, "genlapf")
+        A = ops.ops_decl_dat(blk, [nx, ny], name="A")
+        An = ops.ops_decl_dat(blk, [nx, ny], name="Anew")
+        ops.set_ext(A, init)
+        ops.set_ext(An, init)
+        hist = np.zeros(it)
+        gen.gen_laplace(blk.ptr, A.ptr, An.ptr, nx, ny, it, _p(hist))
+        ops._check()
+        out = (hist, ops.ops_dat_fetch_data(A), _fused_count(ops, "fused(gen copy+gen apply_stencil)"))
+        ops.ops_free_dat(A)
+        ops.ops_free_dat(An)
+        return out
+
+    runs = _levels(ops, run)
+    ref_A, ref_h = O.laplace_run(init, it)
+    for lev, (h, A, cnt) in enumerate(runs):
+        assert np.array_equal(h, ref_h) and np.array_equal(A, O.interior(ref_A)), lev
+        assert cnt == (it - 1 if lev == 2 else 0), (lev, cnt)

@@ -51,12 +51,23 @@ def test_multiblock_poisson_app_other_sizes():
 # The same two apps through this project's translator: their own kernel
 # headers become generated HIP kernels (oracle/Makefile.ref, APPS_GEN).
 def test_laplace_tutorial_app_through_the_translator():
+    """The tutorial's copy and the next apply_stencil run as one generated
+    pass (apply_stencil reads, at neighbours, the A that copy writes); the
+    printed error is the hand-written entry points' to the last digit, and
+    the same with fusion off (OPSMI_FUSION=0)."""
     out = _run("app_laplace2d_gen")
+    assert re.search(r"fused\(copy\+apply stencil\)\s+(\d+)", out), out[-3000:]
     assert "This run is considered PASSED" in out, out[-2000:]
     pct = float(re.search(r"within ([0-9.eE+-]+) %", out).group(1))
     assert pct < 1e-9
     hw = _run("app_laplace2d_opsmi")
     assert re.search(r"within ([0-9.eE+-]+) %", out).group(1) == re.search(r"within ([0-9.eE+-]+) %", hw).group(1)
+    exe = os.path.join(REF, "app_laplace2d_gen")
+    off = subprocess.run([exe], capture_output=True, text=True, timeout=120, cwd=REF,
+                         env=dict(os.environ, OPSMI_FUSION="0"))
+    assert off.returncode == 0 and "fused(copy+apply stencil)" not in off.stdout
+    assert re.search(r"within ([0-9.eE+-]+) %", off.stdout).group(1) == re.search(r"within ([0-9.eE+-]+) %",
+                                                                                  out).group(1)
 
 
 def test_multiblock_poisson_app_through_the_translator():
@@ -91,3 +102,23 @@ def test_lowdim_app_through_the_translator():
         assert what in r.stdout, (what, r.stdout[-2000:])
     assert "TEST FAILED" not in r.stdout and "errors found" not in r.stdout
     assert r.stderr.count("HDF5 output is not available") == 8, r.stderr[-2000:]
+
+
+def test_half_precision_app_through_the_translator():
+    """The reference's half-precision Laplace app (apps/c/halfprecision/
+    laplace2d.cpp, unchanged): "half" dats with binary16 arithmetic in the
+    generated kernels (native FP16 instructions, one rounding per operation)
+    and a float MAX reduction, 4094^2 for 100 iterations; its own check
+    against the known answer 0.002685546875 (laplace2d.cpp:122).  HDF5
+    output is reported and skipped (OPSMI_ERROR_MODE=2); the app's NaN check
+    of A runs."""
+    exe = os.path.join(REF, "app_halfprec_gen")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    env = dict(os.environ, OPSMI_ERROR_MODE="2")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120, cwd=REF, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "This run is considered PASSED" in r.stdout, r.stdout[-3000:]
+    assert re.search(r"^\s*100, 0\.002686\s*$", r.stdout, re.M), r.stdout[-3000:]
+    assert "ops_NaNcheck_core called for A" in r.stdout and "NaN detected" not in r.stdout
+    assert r.stderr.count("HDF5 output is not available") == 2, r.stderr[-2000:]

@@ -35,7 +35,9 @@ def test_translates_dev_api_applications(tmp_path):
     # constants the body names become members; the two-component dat is MULTI
     assert re.search(r"struct opsk_mix_fields \{\s*double weights\[3\];", hip)
     assert re.search(r"struct opsk_fill_fields \{\s*int mult;\s*double shift;", hip)
-    assert "::opsmi_gen::Acc<double, 2, true, OPSMI_G> &v" in hip and "::opsmi_gen::Acc<int, 2, false, OPSMI_G> &m" in hip
+    assert re.search(r"template <class OPSMI_A0, class OPSMI_A1> __device__ __forceinline__ void operator\(\)"
+                     r"\(const OPSMI_A0 &v, OPSMI_A1 &w", hip)
+    assert "::opsmi_gen::Acc<double, 2, true, false> a0(p0, A.d0, x, y, z);" in hip
     # the general kernel (plain pointers, x steps from the launch) beside the restrict-qualified one
     assert "::opsmi_gen::Acc<double, 2, true, true> a0(p0, A.d0, x, y, z);" in hip
     assert "auto kern = L->alias ? opsg_fill_fields_alias : opsg_fill_fields;" in hip
@@ -105,7 +107,8 @@ def _app(tmp_path, loop, kernel="void k(ACC<double> &a) { a(0, 0) = 1.0; }"):
      "ACC<T> reference"),
     ('ops_par_loop(k, "k", b, 2, r, ops_arg_reduce(h, n, "double", OPS_INC))', "void k(double *x) {}",
      "integer literal for a reduction"),
-    ('ops_par_loop(k, "k", b, 2, r, ops_arg_dat(d, 1, s, "half", OPS_WRITE))', None, "only double, float, int"),
+    ('ops_par_loop(k, "k", b, 2, r, ops_arg_dat(d, 1, s, "complexd", OPS_WRITE))', None,
+     "only double, float, int, half"),
 ])
 def test_refuses_what_it_cannot_translate(tmp_path, loop, kernel, msg):
     app = _app(tmp_path, loop) if kernel is None else _app(tmp_path, loop, kernel)
