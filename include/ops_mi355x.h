@@ -34,10 +34,6 @@
 #include <stddef.h>
 #include <stdio.h>
 
-#ifdef __cplusplus
-extern "C" {
-#endif
-
 /* ---- constants: ops/c/include/ops_lib_core.h:57-92,163-165 ---- */
 #define OPS_MAX_DIM 5
 #define OPS_READ 0
@@ -73,12 +69,24 @@ typedef _Float16 half;
 typedef uint16_t half;
 #endif
 
-/* ---- object types: ops/c/include/ops_lib_core.h:181-552 ---- */
+/* ---- object types: ops/c/include/ops_lib_core.h:181-552.  Under C++ the
+ * structs also carry the reference's member functions (its OPS_CPP_API,
+ * defined after the C declarations below); the data layout is the same. ---- */
+struct ops_dat_core;
+struct ops_stencil_core;
 typedef struct ops_block_core {
   int index;
   int dims;
   const char *name;
   void *instance;
+#ifdef __cplusplus
+  template <class T>
+  struct ops_dat_core *decl_dat(int data_size, int *block_size, int *base, int *d_m, int *d_p, int *stride, T *data,
+                                const char *type, const char *name);
+  template <class T>
+  struct ops_dat_core *decl_dat(int data_size, int *block_size, int *base, int *d_m, int *d_p, T *data,
+                                const char *type, const char *name);
+#endif
 } ops_block_core;
 typedef ops_block_core *ops_block;
 
@@ -107,6 +115,19 @@ typedef struct ops_dat_core {
   size_t base_offset;       /* bytes from data(_d) to point (0,...,0) */
   int stride[OPS_MAX_DIM];
   void *impl;
+#ifdef __cplusplus
+  void get_data();
+  int get_local_npartitions();
+  void get_extents(int part, int *disp, int *size);
+  void get_raw_metadata(int part, int *disp, int *size, int *stride, int *d_m, int *d_p);
+  char *get_raw_pointer(int part, struct ops_stencil_core *stencil, int *memspace);
+  void release_raw_data(int part, int acc);
+  void release_raw_data(int part, int acc, int *memspace);
+  void fetch_data(int part, char *data, int memspace = OPS_HOST);
+  void fetch_data_slab(int part, char *data, int *range, int memspace = OPS_HOST);
+  void set_data(int part, char *data, int memspace = OPS_HOST);
+  void set_data_slab(int part, char *data, int *range, int memspace = OPS_HOST);
+#endif
 } ops_dat_core;
 typedef ops_dat_core *ops_dat;
 
@@ -132,6 +153,10 @@ typedef struct ops_reduction_core {
   char *name;
   void *instance;
   void *impl;
+#ifdef __cplusplus
+  template <class T>
+  void get_result(T *ptr);
+#endif
 } ops_reduction_core;
 typedef ops_reduction_core *ops_reduction;
 
@@ -152,6 +177,9 @@ typedef struct ops_halo_group_core {
   ops_halo *halos;
   int index;
   void *instance;
+#ifdef __cplusplus
+  void halo_transfer();
+#endif
 } ops_halo_group_core;
 typedef ops_halo_group_core *ops_halo_group;
 
@@ -167,6 +195,12 @@ typedef struct ops_arg {
   ops_arg_type argtype;
   int opt;
 } ops_arg;
+
+/* types above, functions below: the structs carry C++ member templates
+ * under C++, which must not have C linkage */
+#ifdef __cplusplus
+extern "C" {
+#endif
 
 /* ---- lifecycle ---- */
 /* ops/c/include/ops_lib_core.h:601-608.  argv tokens (as parsed by
@@ -190,6 +224,10 @@ void ops_init(const int argc, const char *const argv[], const int diags_level);
 void ops_exit(void);
 /* ops/c/include/ops_lib_core.h:1076-1077 (single GPU: no decomposition) */
 void ops_partition(const char *routine);
+/* ops/c/include/ops_lib_core.h:611: multi-component dats declared after
+ * ops_set_soa(1) keep each component in its own array (structure of
+ * arrays); fetch/set buffers keep a point's components together either way */
+void ops_set_soa(const int soa_val);
 
 /* ---- declarations ---- */
 /* ops/c/include/ops_lib_core.h:620-621 */
@@ -549,6 +587,140 @@ static inline void ops_execute(void *instance) {
 }
 static inline void ops_timing_output(std::ostream &) { ops_timing_output_stdout(); }
 static inline bool ops_checkpointing_init(const char *, double, int) { return false; }
+
+/* ---- the reference's C++ object API (OPS_CPP_API: ops/c/include/
+ *      ops_instance.h:62-380, ops/c/include/ops_lib_core.h:183-552,
+ *      ops/c/include/ops_exceptions.h:46-53) over the C entry points.  This
+ *      backend keeps one instance per process: an OPS_instance is it (its
+ *      constructor runs ops_init, its destructor ops_exit), and a second one
+ *      while the first exists is refused. ---- */
+#include <exception>
+#include <sstream>
+
+struct OPSException : public std::exception {
+  int code;
+  std::stringstream msg;
+  mutable std::string text;
+  explicit OPSException(int c) : code(c) {}
+  OPSException(int c, const char *m) : code(c) { msg << m; }
+  OPSException(const OPSException &o) : std::exception(o), code(o.code) { msg << o.msg.str(); }
+  template <class T>
+  OPSException &operator<<(const T &v) {
+    msg << v;
+    return *this;
+  }
+  const char *what() const throw() override {
+    text = msg.str();
+    return text.c_str();
+  }
+};
+
+class OPS_instance {
+ public:
+  OPS_instance(const int argc, const char *const argv[], const int diags_level, std::ostream &os = std::cout)
+      : OPS_soa(0), os_(&os) {
+    if (current()) throw OPSException(OPS_RUNTIME_CONFIGURATION_ERROR, "this backend keeps one OPS instance at a time");
+    ops_init(argc, argv, diags_level);
+    current() = this;
+  }
+  ~OPS_instance() { exit(); }
+  OPS_instance(const OPS_instance &) = delete;
+  OPS_instance &operator=(const OPS_instance &) = delete;
+  /* the live instance, or NULL */
+  static OPS_instance *&current() {
+    static OPS_instance *p = nullptr;
+    return p;
+  }
+  static int numInstances() { return current() ? 1 : 0; }
+  std::ostream &ostream() { return *os_; }
+  void set_ostream(std::ostream &s) { os_ = &s; }
+  void exit() {
+    if (current() == this) {
+      ops_exit();
+      current() = nullptr;
+    }
+  }
+  /* declarations follow the current OPS_soa setting */
+  ops_block decl_block(int dims, const char *name) {
+    ops_set_soa(OPS_soa);
+    return ops_decl_block(dims, name);
+  }
+  ops_reduction decl_reduction_handle(int size, const char *type, const char *name) {
+    return ops_decl_reduction_handle(size, type, name);
+  }
+  ops_stencil decl_stencil(int dims, int points, int *stencil, const char *name) {
+    return ops_decl_stencil(dims, points, stencil, name);
+  }
+  ops_stencil decl_strided_stencil(int dims, int points, int *sten, int *stride, const char *name) {
+    return ops_decl_strided_stencil(dims, points, sten, stride, name);
+  }
+  ops_halo decl_halo(ops_dat from, ops_dat to, int *iter_size, int *from_base, int *to_base, int *from_dir,
+                     int *to_dir) {
+    return ops_decl_halo(from, to, iter_size, from_base, to_base, from_dir, to_dir);
+  }
+  ops_halo_group decl_halo_group(int nhalos, ops_halo *halos) { return ops_decl_halo_group(nhalos, halos); }
+  template <class T>
+  void decl_const(const char *name, int dim, const char *type, T *data) {
+    ops_decl_const_char(nullptr, dim, type, sizeof(T), (char *)data, name);
+  }
+  template <class T>
+  void update_const(const char *name, int dim, const char *type, T *data) {
+    decl_const(name, dim, type, data);
+  }
+  void partition(const char *routine) { ops_partition(routine); }
+  void diagnostic_output() { ops_diagnostic_output(); }
+  void timing_output(std::ostream &) { ops_timing_output_stdout(); }
+  void timing_output_stdout() { ops_timing_output_stdout(); }
+  int is_root() { return 1; }
+  void execute() { ops_execute(); }
+  int OPS_soa; /* 1: multi-component dats declared from now on are SoA */
+
+ private:
+  std::ostream *os_;
+};
+
+template <class T>
+inline ops_dat ops_block_core::decl_dat(int data_size, int *block_size, int *base, int *d_m, int *d_p, int *stride,
+                                        T *data, const char *type, const char *name) {
+  if (OPS_instance::current()) ops_set_soa(OPS_instance::current()->OPS_soa);
+  return ops_decl_dat_char(this, data_size, block_size, base, d_m, d_p, stride, (char *)data, sizeof(T), type, name);
+}
+template <class T>
+inline ops_dat ops_block_core::decl_dat(int data_size, int *block_size, int *base, int *d_m, int *d_p, T *data,
+                                        const char *type, const char *name) {
+  int stride[OPS_MAX_DIM] = {1, 1, 1, 1, 1};
+  return decl_dat(data_size, block_size, base, d_m, d_p, stride, data, type, name);
+}
+template <class T>
+inline void ops_reduction_core::get_result(T *ptr) {
+  ops_reduction_result_char(this, sizeof(T), (char *)ptr);
+}
+inline void ops_halo_group_core::halo_transfer() { ops_halo_transfer(this); }
+inline void ops_dat_core::get_data() { ops_execute(); }
+inline int ops_dat_core::get_local_npartitions() { return ops_dat_get_local_npartitions(this); }
+inline void ops_dat_core::get_extents(int part, int *disp, int *size) { ops_dat_get_extents(this, part, disp, size); }
+inline void ops_dat_core::get_raw_metadata(int part, int *disp, int *size, int *stride, int *d_m, int *d_p) {
+  ops_dat_get_raw_metadata(this, part, disp, size, stride, d_m, d_p);
+}
+inline char *ops_dat_core::get_raw_pointer(int part, ops_stencil_core *stencil, int *memspace) {
+  return ops_dat_get_raw_pointer(this, part, stencil, memspace);
+}
+inline void ops_dat_core::release_raw_data(int part, int acc) { ops_dat_release_raw_data(this, part, acc); }
+inline void ops_dat_core::release_raw_data(int part, int acc, int *memspace) {
+  ops_dat_release_raw_data_memspace(this, part, acc, memspace);
+}
+inline void ops_dat_core::fetch_data(int part, char *data, int memspace) {
+  ops_dat_fetch_data_memspace(this, part, data, memspace);
+}
+inline void ops_dat_core::fetch_data_slab(int part, char *data, int *range, int memspace) {
+  ops_dat_fetch_data_slab_memspace(this, part, data, range, memspace);
+}
+inline void ops_dat_core::set_data(int part, char *data, int memspace) {
+  ops_dat_set_data_memspace(this, part, data, memspace);
+}
+inline void ops_dat_core::set_data_slab(int part, char *data, int *range, int memspace) {
+  ops_dat_set_data_slab_memspace(this, part, data, range, memspace);
+}
 #endif
 
 #endif /* OPS_MI355X_H */

@@ -57,7 +57,7 @@ inline Loop loop_of(const opsmi_gen_launch *L) {
 template <class T, int NDIM, bool MULTI, bool GEN = false>
 class Acc {
   T *p_;
-  long md_, sy_, sz_;
+  long md_, sy_, sz_, cs_;
 
   __device__ __forceinline__ long pt(int dx, int dy, int dz) const {
     long o = (long)dx * ((MULTI || GEN) ? md_ : 1);
@@ -68,12 +68,12 @@ class Acc {
 
  public:
   __device__ __forceinline__ Acc(const opsmi_gen_dat &d, int x, int y, int z)
-      : md_(d.md), sy_(d.sy), sz_(d.sz) {
+      : md_(d.md), sy_(d.sy), sz_(d.sz), cs_(d.cs) {
     p_ = (T *)d.base + pt(x, y, z);
   }
   // base: the dat's (restrict-qualified) kernel argument, d.base's value
   __device__ __forceinline__ Acc(const T *base, const opsmi_gen_dat &d, int x, int y, int z)
-      : md_(d.md), sy_(d.sy), sz_(d.sz) {
+      : md_(d.md), sy_(d.sy), sz_(d.sz), cs_(d.cs) {
     p_ = const_cast<T *>(base) + pt(x, y, z);
   }
   // NDIM indices: a point; NDIM + 1: component, then the point
@@ -83,7 +83,7 @@ class Acc {
     static_assert(n == NDIM || n == NDIM + 1, "ACC: give NDIM offsets, or a component and NDIM offsets");
     const int v[n + 3] = {int(i)..., 0, 0, 0};
     if constexpr (n == NDIM) return p_[pt(v[0], v[1], v[2])];
-    else return p_[v[0] + pt(v[1], v[2], v[3])];
+    else return p_[(MULTI ? (long)v[0] * cs_ : (long)v[0]) + pt(v[1], v[2], v[3])];
   }
   __device__ void combine_max(int dx, T val) const { amax(&p_[pt(dx, 0, 0)], val); }
   __device__ void combine_max(int dx, int dy, T val) const { amax(&p_[pt(dx, dy, 0)], val); }

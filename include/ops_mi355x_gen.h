@@ -46,6 +46,8 @@ enum { OPSMI_GEN_F64 = 0, OPSMI_GEN_F32 = 1, OPSMI_GEN_I32 = 2, OPSMI_GEN_NONE =
 typedef struct opsmi_gen_dat {
   void *base;
   long md, sy, sz;
+  long cs; /* elements between components of a point: 1 (AoS), or the
+              points of the whole array (SoA, ops_set_soa; md is then 1) */
 } opsmi_gen_dat;
 
 /* what a launch function receives when its loop runs */

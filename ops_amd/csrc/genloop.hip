@@ -105,9 +105,10 @@ struct GenLoop {
       if (dats[i]) {
         const DatView v = view_of(dats[i]);
         L.dat[i].base = v.base;
-        L.dat[i].md = (adv[i] & 1) ? dats[i]->dim : 0;
+        L.dat[i].md = (adv[i] & 1) ? v.sx : 0;
         L.dat[i].sy = (adv[i] & 2) ? v.sy : 0;
         L.dat[i].sz = (adv[i] & 4) ? v.sz : 0;
+        L.dat[i].cs = v.cs;
       }
     for (const GenRed &g : reds) L.partials[g.arg] = pbase + g.offset;
   }

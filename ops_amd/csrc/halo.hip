@@ -19,7 +19,9 @@ namespace {
 struct HaloSide {
   char *data;          // start of the dat's array (raw index 0)
   long size0, size1;   // raw extents of dims 0, 1
-  int elem, tsize;     // bytes per point, bytes per component
+  long pstride;        // bytes between neighbouring points of one component
+  long cstride;        // bytes between the components of a point
+  int tsize;           // bytes per component
   int dim;             // components
   int first[3];        // raw index of the first box point per dat dimension
   int q[3];            // buffer dimension walked by each dat dimension
@@ -37,7 +39,7 @@ __device__ __forceinline__ long side_offset(const HaloSide &s, const int *b, con
     const int pos = b[s.q[i]];
     r[i] = s.sgn[i] > 0 ? s.first[i] + pos : s.first[i] + (box.ext[s.q[i]] - 1 - pos);
   }
-  return (r[0] + r[1] * s.size0 + r[2] * s.size0 * s.size1) * s.elem + (long)d * s.tsize;
+  return (r[0] + r[1] * s.size0 + r[2] * s.size0 * s.size1) * s.pstride + (long)d * s.cstride;
 }
 
 // storage: 8 or 4 bytes per buffer value
@@ -72,7 +74,11 @@ HaloSide make_side(ops_dat dat, const int *base, const int *dir, const int *iter
   s.data = dat->data_d;
   s.size0 = dat->size[0];
   s.size1 = dat->size[1];
-  s.elem = dat->elem_size;
+  // AoS: a point's components are adjacent; SoA (ops_set_soa): each
+  // component is one array (ops/c/src/sequential/ops_host_singlenode.cpp:124-131)
+  const DatView v = view_of(dat);
+  s.pstride = v.sx * dat->type_size;
+  s.cstride = v.cs * dat->type_size;
   s.tsize = dat->type_size;
   s.dim = dat->dim;
   const int nd = dat->block->dims;

@@ -83,9 +83,16 @@ static const char *dt_name(DType d) {
 DatView view_of(ops_dat d) {
   DatView v;
   v.base = d->data_d + d->base_offset;
-  long cx = d->dim;  // AoS components
+  const bool soa = ((DatImpl *)d->impl)->soa;
+  const long cx = soa ? 1 : d->dim;  // AoS: components of a point are adjacent
+  v.sx = cx;
   v.sy = cx * d->size[0];
   v.sz = v.sy * d->size[1];
+  v.cs = 1;
+  if (soa) {
+    v.cs = 1;
+    for (int n = 0; n < d->block->dims; n++) v.cs *= d->size[n];
+  }
   return v;
 }
 
@@ -861,13 +868,18 @@ ops_dat ops_decl_dat_char(ops_block block, int dim, int *dsize, int *base, int *
     d->d_p[n] = edge ? 0 : d_p[n];
     if (stride) d->stride[n] = stride[n];
   }
+  // multi-component dats follow the instance's layout (ops_set_soa,
+  // ops/c/src/sequential/ops_host_singlenode.cpp:97): SoA keeps each
+  // component in its own array
+  m->soa = I.soa && dim > 1;
+  const size_t unit = m->soa ? (size_t)d->type_size : (size_t)d->elem_size;  // bytes per point of one array
   // x padding so each row starts on a 256-byte boundary (rows at least 1 KiB)
   const size_t A = 256;
-  size_t row = (size_t)d->size[0] * d->elem_size;
+  size_t row = (size_t)d->size[0] * unit;
   int user_x = d->size[0];
-  if (d->size[0] > 1 && row >= 1024 && A % d->elem_size == 0) {
+  if (d->size[0] > 1 && row >= 1024 && A % unit == 0) {
     size_t padded = (row + A - 1) / A * A;
-    d->x_pad = (int)((padded - row) / d->elem_size);
+    d->x_pad = (int)((padded - row) / unit);
     d->size[0] += d->x_pad;
     d->d_p[0] += d->x_pad;
   }
@@ -877,21 +889,25 @@ ops_dat ops_decl_dat_char(ops_block block, int dim, int *dsize, int *base, int *
   d->base_offset = 0;
   size_t cum = 1;
   for (int n = 0; n < nd; n++) {
-    d->base_offset += (size_t)d->elem_size * cum * (size_t)(-d->base[n] - d->d_m[n]);
+    d->base_offset += unit * cum * (size_t)(-d->base[n] - d->d_m[n]);
     cum *= d->size[n];
   }
   // leading pad so that point x = 0 is 256-byte aligned
-  size_t lead = (A - ((size_t)(-d->d_m[0]) * d->elem_size) % A) % A;
+  size_t lead = (A - ((size_t)(-d->d_m[0]) * unit) % A) % A;
   OPSMI_HIP(hipMalloc(&m->alloc, bytes + lead + A));
   d->data_d = m->alloc + lead;
   OPSMI_HIP(hipMemset(m->alloc, 0, bytes + lead + A));
   if (data) {
-    // user array: same layout without the x padding
-    size_t upitch = (size_t)user_x * d->elem_size;
+    // user array: same layout without the x padding (SoA: one such array
+    // per component, one after the other)
+    size_t upitch = (size_t)user_x * unit;
     size_t rows = 1;
     for (int n = 1; n < nd; n++) rows *= d->size[n];
-    OPSMI_HIP(hipMemcpy2D(d->data_d, row + (size_t)d->x_pad * d->elem_size, data, upitch, upitch, rows,
-                          hipMemcpyHostToDevice));
+    const int ncomp = m->soa ? dim : 1;
+    for (int c = 0; c < ncomp; c++)
+      OPSMI_HIP(hipMemcpy2D(d->data_d + (size_t)c * row * rows + (size_t)c * d->x_pad * unit * rows,
+                            row + (size_t)d->x_pad * unit, data + (size_t)c * upitch * rows, upitch, upitch, rows,
+                            hipMemcpyHostToDevice));
   }
   d->data = nullptr;
   d->user_managed = 0;
@@ -1255,6 +1271,31 @@ static void slab_copy(ops_dat dat, char *buf, const int *lo, const int *hi, bool
   if (w == 0 || rows == 0 || h[2] <= l[2]) return;
   DatView v = view_of(dat);
   size_t pitch = (size_t)v.sy * dat->type_size;
+  if (impl(dat)->soa) {
+    // the buffer holds the points' components together, as the reference's
+    // slab copy presents a SoA dat (ops/c/src/externlib/ops_util.cpp:353-500):
+    // each component's box goes through a component-major staging copy
+    const size_t ts = dat->type_size, nc = dat->dim;
+    const size_t wx = (size_t)(h[0] - l[0]), planes = (size_t)(h[2] - l[2]), np = wx * rows * planes;
+    std::vector<char> tmp(np * es);
+    if (to_dat)
+      for (size_t p = 0; p < np; p++)
+        for (size_t c = 0; c < nc; c++) memcpy(&tmp[(c * np + p) * ts], buf + (p * nc + c) * ts, ts);
+    for (size_t c = 0; c < nc; c++)
+      for (int z = l[2]; z < h[2]; z++) {
+        char *dp = v.base + ((long)c * v.cs + (long)l[0] + (long)l[1] * v.sy + (long)z * v.sz) * (long)ts;
+        char *bp = &tmp[(c * np + (size_t)(z - l[2]) * wx * rows) * ts];
+        if (to_dat)
+          OPSMI_HIP(hipMemcpy2DAsync(dp, pitch, bp, wx * ts, wx * ts, rows, hipMemcpyDefault, I.stream));
+        else
+          OPSMI_HIP(hipMemcpy2DAsync(bp, wx * ts, dp, pitch, wx * ts, rows, hipMemcpyDefault, I.stream));
+      }
+    OPSMI_HIP(hipStreamSynchronize(I.stream));
+    if (!to_dat)
+      for (size_t p = 0; p < np; p++)
+        for (size_t c = 0; c < nc; c++) memcpy(buf + (p * nc + c) * ts, &tmp[(c * np + p) * ts], ts);
+    return;
+  }
   for (int z = l[2]; z < h[2]; z++) {
     char *dp = v.base + ((long)l[0] * dat->dim + (long)l[1] * v.sy + (long)z * v.sz) * dat->type_size;
     char *bp = buf + (size_t)(z - l[2]) * w * rows;
@@ -1471,6 +1512,9 @@ void ops_timing_output_stdout(void) {
 
 // ------------------------------------------------------------------ extensions
 void ops_mi_set_error_mode(int mode) { inst().error_mode = mode; }
+
+// ops/c/src/core/ops_lib_core.cpp:372-374
+void ops_set_soa(const int soa_val) { inst().soa = soa_val ? 1 : 0; }
 
 void ops_mi_set_diags(int level) {
   OPSMI_API_BEGIN

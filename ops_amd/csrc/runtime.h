@@ -58,11 +58,14 @@ std::string cat(const A &...a) {
 enum DType { DT_F64 = 0, DT_F32 = 1, DT_I32 = 2, DT_OTHER = 3, DT_F16 = 4 };
 DType dtype_of(const char *type, int type_size);
 
-// device-side view of one dat argument: pointer to point (0,0,0) and strides
-// in elements (x stride is the component count for AoS dats)
+// device-side view of one dat argument: pointer to component 0 of point
+// (0,0,0) and strides in elements: sx along x (the component count for an
+// AoS dat, 1 for SoA), sy, sz, and cs between components (1 for AoS, the
+// points of the whole array for SoA)
 struct DatView {
   char *base;
   long sy, sz;
+  long sx = 1, cs = 1;
 };
 
 struct RedImpl {
@@ -90,6 +93,10 @@ struct DatImpl {
   // order; an OPS_INC/MIN/MAX loop after a different access first resets the
   // dat to the identity (ops/c/src/core/ops_dummy_singlenode.cpp:204-391)
   int edat_prev_acc = OPS_READ;
+  // structure-of-arrays layout (declared with more than one component while
+  // the instance's SoA flag was set): component d of every point is one
+  // array, d arrays apart
+  bool soa = false;
 };
 
 struct LoopStat {
@@ -115,6 +122,7 @@ struct Instance {
   int last_error = 0;
   std::string last_msg;
   int fusion = 2;  // 0 off, 1 the pointwise pair only, 2 also the column chains
+  int soa = 0;     // ops_set_soa: multi-component dats declared from now on are SoA
   // OPS_BLOCK_SIZE_{X,Y,Z} (0: not given) and OPS_TILING_MAXDEPTH
   int block_x = 0, block_y = 0, block_z = 0;
   int tiling_maxdepth = 0;

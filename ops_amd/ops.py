@@ -116,6 +116,7 @@ def lib():
             "ops_mi_reset_stats": (None, []),
             "ops_mi_set_fusion": (None, [ci]),
             "ops_mi_set_diags": (None, [ci]),
+            "ops_set_soa": (None, [ci]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -193,6 +194,12 @@ def ops_exit():
 
 def ops_execute():
     _call("ops_execute")
+
+
+def ops_set_soa(soa):
+    """multi-component dats declared from now on: 1 structure of arrays, 0
+    array of structures (the default)"""
+    _call("ops_set_soa", 1 if soa else 0)
 
 
 def ops_decl_block(dims, name):

@@ -730,7 +730,11 @@ def translate(app, out_dir, inc_dirs=()):
     # device functions, as the reference's generated code does
     kernel_files = {kern.path for _, kern in kernels.values()}
     ops_names = {"ops_seq_v2.h", "ops_seq.h", "ops_lib_core.h"}
-    user_incs = [name for _, _, p, name in incs if p not in kernel_files and os.path.basename(name) not in ops_names]
+    # headers of kernels no loop calls (they take ACC<T> parameters) stay in
+    # the application only: their host functions would be defined twice
+    acc_headers = {p for _, _, p, _ in incs if re.search(r"\bACC\s*<", mask(open(p).read()))}
+    user_incs = [name for _, _, p, name in incs
+                 if p not in kernel_files and p not in acc_headers and os.path.basename(name) not in ops_names]
     H = [f"// Generated by ops_amd/translator/opsmi_translate.py from {os.path.basename(app)}; do not edit.",
          "#include <math.h>", "#include <string.h>", "", "#include \"ops_mi355x_device.h\"", "",
          "#ifndef OPS_FUN_PREFIX", "#define OPS_FUN_PREFIX __host__ __device__", "#endif"]

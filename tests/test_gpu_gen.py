@@ -182,11 +182,19 @@ def test_generated_1d_float_loops(gpu_ops, gen, n):
 
 
 @pytest.mark.parametrize("n", [(9, 7, 5), (70, 12, 6)])
-def test_generated_3d_three_components(gpu_ops, gen, n):
+@pytest.mark.parametrize("soa", [0, 1])
+def test_generated_3d_three_components(gpu_ops, gen, n, soa):
+    """A three-component dat in either layout (ops_set_soa: structure of
+    arrays keeps each component in its own array); fetches and slabs present
+    a point's components together either way."""
     ops = gpu_ops
     nx, ny, nz = n
     blk = ops.ops_decl_block(3, "genv3")
-    v = ops.ops_decl_dat(blk, [nx, ny, nz], name="v", dim=3)
+    ops.ops_set_soa(soa)
+    try:
+        v = ops.ops_decl_dat(blk, [nx, ny, nz], name="v", dim=3)
+    finally:
+        ops.ops_set_soa(0)
     d = ops.ops_decl_dat(blk, [nx, ny, nz], name="d")
     sums = np.zeros(3)
     imin = C.c_int(0)
@@ -200,6 +208,18 @@ def test_generated_3d_three_components(gpu_ops, gen, n):
     i = (slice(1, -1),) * 3
     assert list(sums) == [X[i].sum(), (2.0 * Y[i]).sum(), (3.0 * Z[i] + 1.0).sum()]
     assert imin.value == 12 - (nx - 1)
+    # the array itself, through the host raw pointer: component-major for SoA
+    core = v.core
+    ms = C.c_int(ops.OPS_HOST)
+    addr = ops._call("ops_dat_get_raw_pointer", v.ptr, 0, None, C.byref(ms))
+    npts = core.size[0] * core.size[1] * core.size[2]
+    raw = np.ctypeslib.as_array((C.c_double * (3 * npts)).from_address(addr - core.base_offset)).copy()
+    ops._call("ops_dat_release_raw_data", v.ptr, 0, ops.OPS_READ)
+    if soa:
+        comp = raw.reshape(3, core.size[2], core.size[1], core.size[0])[:, :, :, :nx + 2]
+    else:
+        comp = np.moveaxis(raw.reshape(core.size[2], core.size[1], core.size[0], 3), 3, 0)[:, :, :, :nx + 2]
+    assert np.array_equal(comp[0], X.astype(float)) and np.array_equal(comp[2], 3.0 * Z + 1.0)
 
 
 def test_generated_optional_dat(gpu_ops, gen):

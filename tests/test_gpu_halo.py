@@ -62,3 +62,27 @@ def test_halo_box_outside_dat_is_an_error(gpu_ops):
     h = ops.ops_decl_halo(a, b, [1, 5], [5, 0], [-1, 0], [1, 3], [1, 2])
     with pytest.raises(ops.OPSError, match="bad direction"):
         ops.ops_halo_transfer(ops.ops_decl_halo_group([h]))
+
+
+def test_halo_transfer_soa_equals_aos(gpu_ops):
+    """Two-component dats in SoA layout (ops_set_soa) exchange the same
+    values as in AoS: a flipped, permuted 3D transfer between two dats."""
+    ops = gpu_ops
+    n = (9, 7, 5)
+    out = []
+    for soa in (0, 1):
+        blk = ops.ops_decl_block(3, f"hsoa{soa}")
+        ops.ops_set_soa(soa)
+        try:
+            a = ops.ops_decl_dat(blk, list(n), d_m=[-2, -1, -1], d_p=[2, 1, 1], name=f"ha{soa}", dim=2)
+            b = ops.ops_decl_dat(blk, list(n), d_m=[-2, -1, -1], d_p=[2, 1, 1], name=f"hb{soa}", dim=2)
+        finally:
+            ops.ops_set_soa(0)
+        whole = (n[2] + 2, n[1] + 2, n[0] + 4, 2)
+        ops.set_whole(a, np.arange(np.prod(whole), dtype=np.float64).reshape(whole))
+        ops.set_whole(b, -np.ones(whole))
+        h = ops.ops_decl_halo(a, b, [3, 4, 2], [1, 0, 2], [-2, 1, 0], [2, -1, 3], [1, 2, -3])
+        ops.ops_halo_transfer(ops.ops_decl_halo_group([h]))
+        out.append(ops.get_whole(b))
+    assert np.array_equal(out[0], out[1])
+    assert np.any(out[0] != -1)

@@ -122,3 +122,27 @@ def test_half_precision_app_through_the_translator():
     assert re.search(r"^\s*100, 0\.002686\s*$", r.stdout, re.M), r.stdout[-3000:]
     assert "ops_NaNcheck_core called for A" in r.stdout and "NaN detected" not in r.stdout
     assert r.stderr.count("HDF5 output is not available") == 2, r.stderr[-2000:]
+
+
+def test_multidim_soa_app_through_the_translator():
+    """The reference's multiDim test (apps/c/multiDim/multidim.cpp, unchanged,
+    the C++ object API): an OPS_instance with OPS_soa = 1, two-component dats
+    in SoA layout, a halo group exchange, a two-component reduction checked
+    against its known answer 24 + 24 (multidim.cpp:171), gbl arrays indexed
+    by the loop index; HDF5 output reported and skipped."""
+    exe = os.path.join(REF, "app_multidim_gen")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    env = dict(os.environ, OPSMI_ERROR_MODE="2")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120, cwd=REF, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "This test is considered PASSED" in r.stdout, r.stdout[-3000:]
+    assert "Reduction result = 24, 24" in r.stdout, r.stdout[-3000:]
+    assert "The test using ops_arg_gbl arguments passed!" in r.stdout
+
+
+def test_mb_trid_app_through_the_translator():
+    """The reference's mb_trid_test (apps/c/mb_trid_test/mb_trid.cpp): an
+    OPS_instance with two blocks and a tridiagonal settings object per block."""
+    out = _run("app_mbtrid_gen")
+    assert "error" not in out.lower()
