@@ -69,8 +69,10 @@ def test_generated_laplace_bit_exact(gpu_ops, gen, n):
     ref_A, ref_h = O.laplace_run(init, it)
     assert np.array_equal(hist, ref_h)
     assert np.array_equal(ops.ops_dat_fetch_data(A), O.interior(ref_A))
-    st = ops.ops_mi_loop_stats("gen apply_stencil")
-    assert st and st[0] == it
+    # every apply_stencil ran: on its own, or with the copy before it as one
+    # pass (fusion level 2: the first iteration's alone, the rest fused)
+    assert _fused_count(ops, "gen apply_stencil") + _fused_count(ops, "fused(gen copy+gen apply_stencil)") == it
+    assert _fused_count(ops, "fused(gen copy+gen apply_stencil)") == it - 1
 
 
 def test_generated_loop_checks_its_stencil(gpu_ops, gen):
@@ -336,8 +338,9 @@ def test_generated_pair_runs_fused_and_bit_exact(gpu_ops, gen, n):
 
 def test_generated_pair_with_a_neighbour_read_is_not_fused(gpu_ops, gen):
     """copy then apply_stencil: the second loop reads neighbours of what the
-    first writes, so the two run as separate passes (the results are the
-    Laplace test's, bit-exact against the oracle)."""
+    first writes, so the point-wise pair rule (fusion level 1) keeps them
+    apart; only the neighbour form (level 2) joins them.  The results are the
+    Laplace test's, bit-exact against the oracle."""
     ops = gpu_ops
     nx, ny = 64, 48
     init = O.laplace_init(nx, ny, 0)
@@ -347,13 +350,16 @@ def test_generated_pair_with_a_neighbour_read_is_not_fused(gpu_ops, gen):
     ops.set_ext(A, init)
     ops.set_ext(An, init)
     hist = np.zeros(4)
-    ops.ops_mi_reset_stats()
-    gen.gen_laplace(blk.ptr, A.ptr, An.ptr, nx, ny, 4, _p(hist))
-    ops._check()
-    for name in ("fused(gen copy+gen apply_stencil)", "fused(gen apply_stencil+gen copy)"):
-        st = ops.ops_mi_loop_stats(name)
-        assert st is None or st[0] == 0, name
-    assert ops.ops_mi_loop_stats("gen copy")[0] == 4 and ops.ops_mi_loop_stats("gen apply_stencil")[0] == 4
+    ops.lib().ops_mi_set_fusion(1)
+    try:
+        ops.ops_mi_reset_stats()
+        gen.gen_laplace(blk.ptr, A.ptr, An.ptr, nx, ny, 4, _p(hist))
+        ops._check()
+        for name in ("fused(gen copy+gen apply_stencil)", "fused(gen apply_stencil+gen copy)"):
+            assert _fused_count(ops, name) == 0, name
+        assert ops.ops_mi_loop_stats("gen copy")[0] == 4 and ops.ops_mi_loop_stats("gen apply_stencil")[0] == 4
+    finally:
+        ops.lib().ops_mi_set_fusion(2)
     ref_A, ref_h = O.laplace_run(init, 4)
     assert np.array_equal(hist, ref_h)
     assert np.array_equal(ops.ops_dat_fetch_data(A), O.interior(ref_A))
@@ -527,7 +533,12 @@ def test_generated_nb_fusion_double_buffered(gpu_ops, gen, n):
 def test_generated_nb_fusion_refused(gpu_ops, gen, mode):
     """The dat the source links is not the one the second loop reads (the
     variable was reassigned between the calls), or the first loop reads the
-    dat it writes: the pair runs as two passes, with their results."""
+    dat it writes: the neighbour form is refused.  In the first case the
+    second loop's neighbour reads are of a dat neither loop writes, so the
+    point-wise pair form runs them as one pass; in the second the pair runs
+    as two passes.  Either way the results are those of the loops in turn
+    (the neighbour form would have given w from 3 v instead of the other
+    dat)."""
     ops = gpu_ops
     nx, ny = 67, 23
     ext = (ny + 2, nx + 2)
@@ -539,7 +550,7 @@ def test_generated_nb_fusion_refused(gpu_ops, gen, mode):
     ops.ops_mi_reset_stats()
     gen.gen_chain_refused(blk.ptr, D["u"].ptr, D["v"].ptr, D["w"].ptr, D["o"].ptr, nx, ny, mode)
     ops._check()
-    assert _fused_count(ops, "fused(ch_scale+ch_diff)") == 0
+    assert _fused_count(ops, "fused(ch_scale+ch_diff)") == (1 if mode == 1 else 0)
     U = init["u"].copy()
     U[1:-1, 1:-1] = 3.0 * (init["v"] if mode == 1 else init["u"])[1:-1, 1:-1]
     P = init["o"] if mode == 1 else U
