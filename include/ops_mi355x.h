@@ -331,6 +331,20 @@ void ops_tridMultiDimBatch_c(int ndim, int solvedim, int *range, ops_dat a, ops_
                              ops_tridsolver_params_c *ctx);
 void ops_tridMultiDimBatch_Inc_c(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c,
                                  ops_dat d, ops_dat u, ops_tridsolver_params_c *ctx);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ops_tridMultiDimBatch (u NULL) or
+ * ops_tridMultiDimBatch_Inc (u given) on the same kernels, with the c'/d'
+ * scratch in a fixed 64 MiB device array of the library instead of
+ * library-allocated memory.  Limit: 2 * rows * systems * sizeof(element)
+ * <= 64 MiB (4,194,304 points in double, 8,388,608 in float); larger
+ * solves are refused with OPS_INVALID_ARGUMENT.  It calls no hipMalloc or
+ * hipFree and records no statistics or timing events; it refuses to run
+ * while loops are queued (call ops_execute() first) or while a dat's host
+ * copy is newer than its device copy.  Calls on one GPU share the scratch:
+ * calls that overlap on one GPU (several host threads) race. */
+void ops_tridMultiDimBatch_noalloc(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c,
+                                   ops_dat d, ops_dat u, ops_tridsolver_params *tridsolver_ctx);
 
 /* ops/c/include/ops_internal2.h:219: reports the first NaN in the dat
  * (halos included) and exits with status 2 */

@@ -735,6 +735,7 @@ void ops_init(const int argc, const char *const argv[], const int diags_level) {
     printf(" OPS (MI355X): loop-chain fusion level %d, generated-loop workgroups %d rows (2D) / %d planes (3D)\n",
            I.fusion, I.gen_rows_2d(), I.gen_planes_3d());
   OPSMI_HIP(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
+  trid_noalloc_setup(dev);
   ensure_partials((size_t)1 << 20);
   OPSMI_HIP(hipMalloc(&I.d_partials2, (size_t)1 << 20));
   OPSMI_HIP(hipHostMalloc(&I.h_pinned, 4096));

@@ -312,6 +312,7 @@ void deps_from_args(LoopRec &rec, const ops_arg *args, int n);
 char *spare_base(ops_dat d, const Range3 &r);
 void swap_spare(ops_dat d, const Range3 &r);
 void note_external_write(ops_dat d);
+void trid_noalloc_setup(int dev);  // trid.hip: per-device address of the no-allocation scratch
 // counts a loop that ran inside a fused kernel (its time is the fused entry's)
 void note_fused(const char *name, double bytes);
 // z planes per workgroup of the z-marching 7-point kernels (16, or

@@ -467,10 +467,28 @@ void *trid_scratch(size_t bytes) {
   return g_scratch.p;
 }
 
+// Scratch of the entry point that allocates nothing
+// (ops_tridMultiDimBatch_noalloc): c' and d' of every system, step-major, as
+// the library scratch above.  Each GPU holds its own copy for the life of the
+// process; host code reaches it only through the address that
+// hipGetSymbolAddress returned on that device (the array's name in host code
+// is not a device address).  Calls on one GPU share it, so calls that overlap
+// on one GPU race.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+__device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+char *g_noalloc_ptr[kMaxDevices];
+
+// trid_run with fixed_scr set: no flush, no library scratch, no timing events
+// or statistics records, so nothing is allocated on the host or the device.
 void trid_run(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c, ops_dat d, ops_dat u,
-              ops_tridsolver_params_c *ctx) {
+              ops_tridsolver_params_c *ctx, char *fixed_scr = nullptr) {
   require_init();
-  flush_all();
+  if (!fixed_scr) flush_all();
+  else if (!queue_empty())
+    fail(OPS_RUNTIME_ERROR, "ops_tridMultiDimBatch_noalloc: loops are queued; call ops_execute() first "
+                            "(running them may allocate)");
   if (!a || !b || !c || !d || !range || !ctx) fail(OPS_INVALID_ARGUMENT, "Tridsolver error: NULL argument");
   if (ndim < 1 || ndim > 3 || ndim != a->block->dims)
     fail(OPS_INVALID_ARGUMENT, "Tridsolver error: ndim must match the block (1-3D)");
@@ -502,7 +520,12 @@ void trid_run(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat 
       fail(OPS_INVALID_ARGUMENT, cat("Tridsolver error: range outside the dats in dimension ", i));
     offset += (long)range[2 * i] * st[i];
   }
-  for (int k = 0; k < nd; k++) sync_to_device(all[k]);
+  for (int k = 0; k < nd; k++) {
+    if (fixed_scr && all[k]->dirty_hd == 1)
+      fail(OPS_RUNTIME_ERROR, cat("ops_tridMultiDimBatch_noalloc: dat ", all[k]->name,
+                                  " has newer data on the host; upload it first (ops_execute, or any loop)"));
+    sync_to_device(all[k]);
+  }
   LineGeo g;
   g.n = dims[solvedim];
   g.stride = st[solvedim];
@@ -518,11 +541,26 @@ void trid_run(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat 
                                     {"ops_tridMultiDimBatch_Inc x", "ops_tridMultiDimBatch_Inc y",
                                      "ops_tridMultiDimBatch_Inc z"}};
   const char *nm = names[u ? 1 : 0][solvedim];
+  double bytes = (double)g.n * g.count * a->type_size * (u ? 6.0 : 5.0);
   if (g.n <= 0 || g.count <= 0) {
-    run_loop(nm, 0.0, [](hipStream_t) {});
+    if (!fixed_scr) run_loop(nm, 0.0, [](hipStream_t) {});
     return;
   }
-  double bytes = (double)g.n * g.count * a->type_size * (u ? 6.0 : 5.0);
+  if (fixed_scr && (double)g.n * g.count * a->type_size * 2.0 > (double)kNoAllocScratchBytes)
+    fail(OPS_INVALID_ARGUMENT,
+         cat("ops_tridMultiDimBatch_noalloc: ", g.count, " systems of ", g.n, " rows need ",
+             2 * g.n * g.count * (long)a->type_size, " bytes of scratch; the maximum is ", kNoAllocScratchBytes,
+             " (", kNoAllocScratchBytes / 16, " points in double, ", kNoAllocScratchBytes / 8, " in float)"));
+  // the launch: through run_loop (statistics, timing events) or, for the
+  // no-allocation entry point, straight onto the library's stream
+  auto launch = [&](const std::function<void(hipStream_t)> &fn) {
+    if (fixed_scr) {
+      fn(inst().stream);
+      OPSMI_HIP(hipGetLastError());
+    } else {
+      run_loop(nm, bytes, fn);
+    }
+  };
   size_t es = a->type_size;
   char *pa = a->data_d + a->base_offset + offset * es, *pb = b->data_d + b->base_offset + offset * es;
   char *pc = c->data_d + c->base_offset + offset * es, *pd = d->data_d + d->base_offset + offset * es;
@@ -532,7 +570,7 @@ void trid_run(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat 
     if (g.n > 1024) fail(OPS_NOT_IMPLEMENTED, "Tridsolver error: PCR supports systems of at most 1024 rows");
     int threads = (int)((g.n + 63) / 64 * 64);
     size_t sm = 8 * g.n * es;
-    run_loop(nm, bytes, [&](hipStream_t s) {
+    launch([&](hipStream_t s) {
       if (dt == DT_F64) {
         if (u) hipLaunchKernelGGL((k_pcr<double, true>), dim3(g.count), dim3(threads), sm, s, (double *)pa, (double *)pb, (double *)pc, (double *)pd, (double *)pu, g);
         else hipLaunchKernelGGL((k_pcr<double, false>), dim3(g.count), dim3(threads), sm, s, (double *)pa, (double *)pb, (double *)pc, (double *)pd, (double *)pu, g);
@@ -543,7 +581,7 @@ void trid_run(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat 
     });
   } else {
     size_t sb = (size_t)g.n * g.count * es;
-    char *scr = (char *)trid_scratch(2 * sb);
+    char *scr = fixed_scr ? fixed_scr : (char *)trid_scratch(2 * sb);
     int nblk = (int)((g.count + 255) / 256);
     if (solvedim == 0) {
       // x solves: LDS-staged chunks, 16-byte row loads when every array's
@@ -553,7 +591,7 @@ void trid_run(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat 
         if (p && (uintptr_t)p % 16 != 0) v16 = false;
       if (dt == DT_F32) v16 = false;
       const int nw = (int)((g.count + 63) / 64);
-      run_loop(nm, bytes, [&](hipStream_t s) {
+      launch([&](hipStream_t s) {
         if (dt == DT_F64) {
           double *A = (double *)pa, *B = (double *)pb, *C = (double *)pc, *D = (double *)pd, *U = (double *)pu;
           double *CS = (double *)scr, *DS = (double *)(scr + sb);
@@ -575,7 +613,7 @@ void trid_run(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat 
       note_external_write(u ? u : d);
       return;
     }
-    run_loop(nm, bytes, [&](hipStream_t s) {
+    launch([&](hipStream_t s) {
       if (dt == DT_F64) {
         if (u) hipLaunchKernelGGL((k_thomas<double, true, kPre>), dim3(nblk), dim3(256), 0, s, (double *)pa, (double *)pb, (double *)pc, (double *)pd, (double *)pu, (double *)scr, (double *)(scr + sb), g);
         else hipLaunchKernelGGL((k_thomas<double, false, kPre>), dim3(nblk), dim3(256), 0, s, (double *)pa, (double *)pb, (double *)pc, (double *)pd, (double *)pu, (double *)scr, (double *)(scr + sb), g);
@@ -590,6 +628,19 @@ void trid_run(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat 
 }
 
 }  // namespace
+
+namespace opsmi {
+// Called by ops_init with the library's device current: the device address
+// of this device's copy of the no-allocation scratch.
+void trid_noalloc_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) fail(OPS_RUNTIME_CONFIGURATION_ERROR, cat("device ", dev, " out of range"));
+  if (g_noalloc_ptr[dev]) return;
+  void *ptr = nullptr;
+  OPSMI_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) fail(OPS_RUNTIME_ERROR, "hipGetSymbolAddress returned no address for the tridiagonal scratch");
+  g_noalloc_ptr[dev] = (char *)ptr;
+}
+}  // namespace opsmi
 
 extern "C" {
 
@@ -633,6 +684,22 @@ void ops_tridMultiDimBatch_Inc(int ndim, int solvedim, int *range, ops_dat a, op
   if (!tridsolver_ctx) fail(OPS_INVALID_ARGUMENT, "Tridsolver error: NULL ops_tridsolver_params");
   if (!u) fail(OPS_INVALID_ARGUMENT, "Tridsolver error: NULL u");
   trid_run(ndim, solvedim, range, a, b, c, d, u, *(ops_tridsolver_params_c **)tridsolver_ctx);
+  OPSMI_API_END()
+}
+
+// ops_tridMultiDimBatch (u NULL) or ops_tridMultiDimBatch_Inc (u given) on
+// the same kernels, with the c'/d' scratch in the library's fixed device
+// array instead of library-allocated memory: at most 64 MiB of scratch,
+// i.e. 2 * rows * systems * sizeof(element) <= 64 MiB.
+void ops_tridMultiDimBatch_noalloc(int ndim, int solvedim, int *range, ops_dat a, ops_dat b, ops_dat c, ops_dat d,
+                                   ops_dat u, ops_tridsolver_params *tridsolver_ctx) {
+  OPSMI_API_BEGIN
+  if (!tridsolver_ctx) fail(OPS_INVALID_ARGUMENT, "Tridsolver error: NULL ops_tridsolver_params");
+  require_init();
+  const int dev = inst().device;
+  char *ptr = dev >= 0 && dev < kMaxDevices ? g_noalloc_ptr[dev] : nullptr;
+  if (!ptr) fail(OPS_RUNTIME_ERROR, "ops_tridMultiDimBatch_noalloc: scratch address not set up (ops_init)");
+  trid_run(ndim, solvedim, range, a, b, c, d, u, *(ops_tridsolver_params_c **)tridsolver_ctx, ptr);
   OPSMI_API_END()
 }
 

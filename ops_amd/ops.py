@@ -108,6 +108,7 @@ def lib():
             "ops_tridMultiDimBatch_Inc_c": (None, [ci, ci, ip, vp, vp, vp, vp, vp, vp]),
             "ops_tridMultiDimBatch": (None, [ci, ci, ip, vp, vp, vp, vp, vp]),
             "ops_tridMultiDimBatch_Inc": (None, [ci, ci, ip, vp, vp, vp, vp, vp, vp]),
+            "ops_tridMultiDimBatch_noalloc": (None, [ci, ci, ip, vp, vp, vp, vp, vp, vp]),
             "ops_mi_set_error_mode": (None, [ci]),
             "ops_mi_last_error": (ci, [cs, ci]),
             "ops_mi_stream": (vp, []),
@@ -395,6 +396,13 @@ def ops_tridMultiDimBatch(ndim, solvedim, rng, a, b, c, d, ctx):
 
 def ops_tridMultiDimBatch_Inc(ndim, solvedim, rng, a, b, c, d, u, ctx):
     _call("ops_tridMultiDimBatch_Inc", ndim, solvedim, _ints(rng), a.ptr, b.ptr, c.ptr, d.ptr, u.ptr, ctx.ref)
+
+
+def ops_tridMultiDimBatch_noalloc(ndim, solvedim, rng, a, b, c, d, u, ctx):
+    """u None: ops_tridMultiDimBatch; u a dat: ops_tridMultiDimBatch_Inc; the
+    c'/d' scratch in the library's fixed device array (no allocation)."""
+    _call("ops_tridMultiDimBatch_noalloc", ndim, solvedim, _ints(rng), a.ptr, b.ptr, c.ptr, d.ptr,
+          u.ptr if u is not None else None, ctx.ref)
 
 
 def ops_mi_sync():
