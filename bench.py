@@ -33,6 +33,9 @@ Workloads on their own (--workload):
   gen_cg_tail_512    512^3 CG tail u = u + a p, then p = r + b p with p.p,
                      written as two dev-API loops (tests/genapps/misc_gen.cpp)
                      and run as the translator's fused pair (one pass, 40 B/pt)
+  stream_mix         pointwise read/write mixes of the path's kernels over
+                     512^3-point FP64 arrays (plain and non-temporal): the
+                     practical HBM ceiling each kernel is measured against
 
 usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload heat3d_cg_512]
 """
@@ -436,8 +439,56 @@ def bench_gen_tail(a, ops, apps):
                 roofline=roofline(kernels, bpp, pts), kernels=kernels, data="synthetic (numpy PCG64, seeded)")
 
 
+# ------------------------------------------------------ stream-mix ceilings
+# (reads, writes) of this path's kernels: the practical HBM ceiling each is
+# measured against (pointwise, 16-byte lanes, no arithmetic or neighbours)
+STREAM_MIXES = {"copy 1R1W": (1, 1), "read-only 4R": (4, 0), "write-only 1W": (0, 1),
+                "calc_ur_r / axpy 2R1W": (2, 1), "jacobi_chain 1R2W": (1, 2), "cg pair 3R2W": (3, 2),
+                "calc_w column 4R1W": (4, 1), "cg chain 6R3W": (6, 3), "thomas fwd (_Inc) 4R2W": (4, 2),
+                "thomas fwd 5R1W": (5, 1)}
+
+
+def measure_stream_mixes(ops, n, reps=10):
+    """Each mix over n^3-point FP64 arrays, plain and non-temporal loads and
+    stores: GB/s of the bytes it moves (decimal)."""
+    import ctypes as C
+    L = ops.lib()
+    L.ops_mi_stream_probe.restype = C.c_int
+    L.ops_mi_stream_probe.argtypes = [C.c_int, C.c_int, C.c_int, C.c_long, C.c_int, C.POINTER(C.c_double)]
+    pts = n ** 3
+    out = {}
+    for name, (r, w) in STREAM_MIXES.items():
+        row = {"reads": r, "writes": w}
+        for nt in (0, 1):
+            us = C.c_double()
+            if L.ops_mi_stream_probe(r, w, nt, pts, reps, C.byref(us)) != 0:
+                ops._check()
+            gbs = (r + w) * 8.0 * pts / (us.value * 1e-6) / 1e9
+            row["nt" if nt else "plain"] = {"avg_us": round(us.value, 2), "GB_s": round(gbs, 1),
+                                            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        out[name] = row
+        print(f"stream {name}: plain {row['plain']['GB_s']} GB/s, nt {row['nt']['GB_s']} GB/s", file=sys.stderr,
+              flush=True)
+    return out
+
+
+def bench_stream(a, ops, apps):
+    n = a.size or 512
+    ops.ops_init(diags=1)
+    m = measure_stream_mixes(ops, n)
+    best = max(v["nt"]["GB_s"] for v in m.values())
+    chain = m["cg chain 6R3W"]["nt"]
+    return line(best, "GB/s", chain["avg_us"] * 1e-3, a, "f64",
+                {"workload": f"stream_mix_{n}", "points_per_array": n ** 3,
+                 "kernel": "pointwise, 2 doubles per lane (16-byte), XCD-aware order"},
+                mixes=m, roofline={"kernel": "stream cg chain 6R3W (nt)", "bound": "hbm", "achieved": chain["GB_s"],
+                                   "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": chain["frac"], "traffic": None},
+                data="zero-filled arrays")
+
+
 WORKLOADS = {"heat3d_cg_512": bench_heat, "poisson_mixed_768": bench_mixed, "laplace2d_2048": bench_laplace,
-             "adi_trid_256": bench_trid, "serve_stream": bench_serve, "gen_cg_tail_512": bench_gen_tail}
+             "adi_trid_256": bench_trid, "serve_stream": bench_serve, "gen_cg_tail_512": bench_gen_tail,
+             "stream_mix": bench_stream}
 
 
 def main():

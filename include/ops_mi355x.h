@@ -514,6 +514,13 @@ void ops_mi_sync(void);
  * is unknown. */
 int ops_mi_loop_stats(const char *name, long *count, double *kernel_ms, double *bytes);
 void ops_mi_reset_stats(void);
+/* memory-stream probe: a pointwise kernel reading nread and writing nwrite
+ * arrays of `points` doubles (16-byte lanes; nt: non-temporal loads and
+ * stores), launched reps times after one untimed launch; *avg_us = average
+ * launch time.  The read/write mixes of this path's kernels: (1,0) (4,0)
+ * (0,1) (1,1) (1,2) (2,1) (3,2) (4,1) (6,3) (4,2) (5,1).  Allocates its
+ * arrays for the call.  Returns 0, or -1 on error. */
+int ops_mi_stream_probe(int nread, int nwrite, int nt, long points, int reps, double *avg_us);
 /* loop-chain fusion in the queue: 0 off, 1 the pointwise CG axpy+axpby pair,
  * 2 also the column chains (TeaLeaf and Poisson CG axpy+axpby+calc_w, the
  * mixed solver's e update + p update + FP32 sweep, the Laplace tutorial's

@@ -117,6 +117,7 @@ def lib():
             "ops_mi_reset_stats": (None, []),
             "ops_mi_set_fusion": (None, [ci]),
             "ops_mi_set_diags": (None, [ci]),
+            "ops_mi_stream_probe": (ci, [ci, ci, ci, C.c_long, ci, C.POINTER(cd)]),
             "ops_set_soa": (None, [ci]),
         }
         for name, (res, args) in sig.items():
@@ -421,6 +422,16 @@ def ops_mi_loop_stats(name):
 
 def ops_mi_reset_stats():
     _call("ops_mi_reset_stats")
+
+
+def ops_mi_set_diags(level):
+    """diagnostics level: above 1 every launch is timed with HIP events"""
+    _call("ops_mi_set_diags", level)
+
+
+def ops_mi_set_fusion(level):
+    """loop-chain fusion level of the queue: 0 off, 1 point-wise pairs, 2 chains"""
+    _call("ops_mi_set_fusion", level)
 
 
 SERVE_HEADER = os.path.join(ROOT, "include", "ops_mi355x_serve.h")
