@@ -502,6 +502,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="heat3d_cg_512: skip the other configurations")
     ap.add_argument("--requests", type=int, default=100, help="serve_stream: number of requests")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run a 1 GiB copy (the calibration kernel of PMC byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -510,6 +512,15 @@ def main():
 
     from ops_amd import apps, ops
 
+    if a.calibrate:
+        # a copy of known size (1 GiB read, 1 GiB written; one untimed and one
+        # timed launch of k_stream<1, 1, false, false>) that scales the PMC
+        # FETCH_SIZE / WRITE_SIZE counters of a profiled run
+        ops.ops_init(diags=2)  # the level the workloads ask for (a second ops_init is a no-op)
+        import ctypes as C
+        us = C.c_double()
+        ops.lib().ops_mi_stream_probe(1, 1, 0, 1 << 27, 1, C.byref(us))
+        ops._check()
     out = WORKLOADS[a.workload](a, ops, apps)
     print(json.dumps(out))
 
