@@ -311,13 +311,13 @@ def bench_trid(a, ops, apps):
 
 
 # ------------------------------------------------------------------ serving
-def serve_requests(nreq, seed):
-    """60 % 2D 5-point Poisson with sides log-uniform in [64, 2048], 40 % 3D
-    7-point heat with sides log-uniform in [32, 256]."""
+def serve_requests(nreq, seed, frac2d=0.6):
+    """60 % (frac2d) 2D 5-point Poisson with sides log-uniform in [64, 2048],
+    the rest 3D 7-point heat with sides log-uniform in [32, 256]."""
     import numpy as np
 
     g = np.random.default_rng(seed)
-    kinds = np.where(g.random(nreq) < 0.6, 0, 1).astype(np.int32)
+    kinds = np.where(g.random(nreq) < frac2d, 0, 1).astype(np.int32)
     s2 = np.exp(g.uniform(np.log(64), np.log(2048), nreq))
     s3 = np.exp(g.uniform(np.log(32), np.log(256), nreq))
     sizes = np.where(kinds == 0, s2, s3).round().astype(np.int32)
@@ -325,7 +325,7 @@ def serve_requests(nreq, seed):
     return kinds, sizes, seeds, g
 
 
-def measure_serve(ops, apps, nreq, seed):
+def measure_serve(ops, apps, nreq, seed, frac2d=0.6):
     """The request stream on the batched engine (FP64, then FP32-inner
     refinement for the 2D Poisson requests), all queued at t = 0 and under
     Poisson arrivals, against the per-solve FP64 path (the library apps, one
@@ -334,7 +334,7 @@ def measure_serve(ops, apps, nreq, seed):
     import numpy as np
 
     tol, budget = 1e-10, 200 * 10 ** 9
-    kinds, sizes, seeds, g = serve_requests(nreq, seed)
+    kinds, sizes, seeds, g = serve_requests(nreq, seed, frac2d)
     ops.ops_mi_set_diags(1)
     # saturation: every request queued at t = 0
     span, done, iters = apps.serve_stream(kinds, sizes, seeds, np.zeros(nreq), tol, budget, autotune=True)
@@ -370,7 +370,8 @@ def measure_serve(ops, apps, nreq, seed):
     base = nreq / t_base
     ops.ops_mi_set_diags(2)
     return {"workload": f"serve_stream_{nreq}", "requests": nreq, "tol": tol, "budget_bytes": budget,
-            "mix": "60% 2D 5-pt Poisson side log-U[64,2048], 40% 3D 7-pt heat side log-U[32,256]",
+            "mix": f"{100 * frac2d:.0f}% 2D 5-pt Poisson side log-U[64,2048], {100 * (1 - frac2d):.0f}% 3D 7-pt heat "
+                   "side log-U[32,256]",
             "batched_saturated": {"solves_per_s": round(sat, 3), "makespan_s": round(span, 3)},
             "batched_fp32_inner_saturated": {"solves_per_s": round(sat_m, 3), "makespan_s": round(span_m, 3)},
             "batched_poisson_arrivals": {"rate_per_s": round(rate, 3),
@@ -386,7 +387,7 @@ def measure_serve(ops, apps, nreq, seed):
 
 def bench_serve(a, ops, apps):
     ops.ops_init(diags=1)
-    m = measure_serve(ops, apps, a.requests, a.seed)
+    m = measure_serve(ops, apps, a.requests, a.seed, a.frac2d)
     sat = m["batched_saturated"]["solves_per_s"]
     return line(sat, "solves/s", 1e3 / sat, a, "f64",
                 {k: m[k] for k in ("workload", "requests", "tol", "budget_bytes", "mix")},
@@ -502,6 +503,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="heat3d_cg_512: skip the other configurations")
     ap.add_argument("--requests", type=int, default=100, help="serve_stream: number of requests")
+    ap.add_argument("--frac2d", type=float, default=0.6, help="serve_stream: share of 2D Poisson requests")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a 1 GiB copy (the calibration kernel of PMC byte counters)")
     a = ap.parse_args()

@@ -76,8 +76,7 @@ double opsapp_mb_poisson(int nx, int ny, int ngx, int ngy, int iters, double *u_
  * done_s[i] gets the time of the scheduler step that reported it converged,
  * iters[i] its iteration count (negative: stopped without converging).
  * autotune != 0 runs the tile-shape autotuner before the clock starts;
- * mixed != 0 solves the 2D Poisson requests in the FP32-inner refinement
- * mode (heat requests stay FP64).
+ * mixed != 0 solves every request in the FP32-inner refinement mode.
  * Returns the time from start to the last completion, or -1 on error. */
 double opsapp_serve_stream(int nreq, const int *kind, const int *n, const uint64_t *seed, const double *arrival_s,
                            double tol, size_t budget, int max_slots, int iters_per_launch, int autotune, int mixed,

@@ -45,10 +45,13 @@ typedef struct {
   uint64_t seed; /* input generator seed */
   double tol;    /* relative residual target */
   int max_iters; /* CG iterations (mixed: inner iterations in total) */
-  int mixed;     /* 1: FP32-inner refinement (2D Poisson only): FP64 x, f
-                    and residual, Jacobi PCG on FP32 fields with FP64 dots,
-                    inner stop max(1e-4, tol / (2 relres)), at most 30
-                    corrections — the algorithm of opsapp_pmixed_solve */
+  int mixed;     /* 1: FP32-inner refinement: FP64 x, f and residual, CG on
+                    FP32 fields with FP64 dots (2D Poisson: Jacobi
+                    preconditioned; 3D heat: unpreconditioned, with the face
+                    coefficients rounded to float), inner stop
+                    max(1e-6, tol / (2 relres)) of the inner start, at most
+                    30 corrections; stop at |r| <= tol |r_0| (the 3D heat
+                    solve starts from x = rhs, as TeaLeaf's) */
 } opsmi_solve_req;
 
 typedef struct {

@@ -202,7 +202,7 @@ def mb_poisson(nx=20, ny=20, ngx=1, ngy=1, iters=10, want_field=True):
 def serve_stream(kinds, sizes, seeds, arrival_s, tol=1e-10, budget=200 * 10 ** 9, max_slots=256,
                  iters_per_launch=8, autotune=False, mixed=False):
     """Runs a request stream on the batched engine (opsapp_serve_stream);
-    mixed solves the 2D Poisson requests with FP32-inner refinement.
+    mixed solves every request with FP32-inner refinement.
     Returns (makespan_s, done_s, iters)."""
     n = len(kinds)
     k = np.ascontiguousarray(kinds, dtype=np.int32)

@@ -732,8 +732,7 @@ double opsapp_serve_stream(int nreq, const int *kind, const int *n, const uint64
   while (done < nreq) {
     double t = wtime() - t0;
     while (next < nreq && arrival_s[next] <= t) {
-      opsmi_solve_req q{kind[next], n[next], seed[next], tol, 1000000,
-                        mixed && kind[next] == OPSMI_SOLVE_POISSON2D ? 1 : 0};
+      opsmi_solve_req q{kind[next], n[next], seed[next], tol, 1000000, mixed ? 1 : 0};
       const int id = opsmi_serve_submit(S, &q);
       if (id < 0) {
         opsmi_serve_destroy(S);

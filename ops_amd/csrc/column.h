@@ -75,11 +75,11 @@ __device__ __forceinline__ ColLane col_lane(int x0, int y0, int xe, int ye) {
 // 512^3 the kernel measured 987 us against 1060 us.
 constexpr int kHeatRows = 8;
 
-template <int V, int WY = kWavesPerBlock>
+template <int V, int WY = kWavesPerBlock, class T = double>
 struct HeatLdsV {
-  __attribute__((aligned(16))) double sp[2][WY + 2][kWave * V + 2 * V];
-  __attribute__((aligned(16))) double skx[2][WY][kWave * V + 2 * V];
-  __attribute__((aligned(16))) double sky[2][WY + 1][kWave * V];
+  __attribute__((aligned(16))) T sp[2][WY + 2][kWave * V + 2 * V];
+  __attribute__((aligned(16))) T skx[2][WY][kWave * V + 2 * V];
+  __attribute__((aligned(16))) T sky[2][WY + 1][kWave * V];
 };
 
 // Launch grid of a heat column kernel over r, tiles of tile_x x rows points.
@@ -109,16 +109,46 @@ inline ColGrid heat_column_grid(const Range3 &r, int tile_x, int rows) {
 
 // one plane of a heat column as a lane loads it: its V points of p, Kz, Kx,
 // Ky and the rims it owns
-template <int V>
+template <int V, class T = double>
 struct HeatPlane {
-  double p[V], kz[V], kx[V], ky[V], ryl[V], ryh[V], rky[V];
-  double rxl, rxh, rkx;
+  T p[V], kz[V], kx[V], ky[V], ryl[V], ryh[V], rky[V];
+  T rxl, rxh, rkx;
+};
+
+// Output stages of the heat column: called per lane and plane with the
+// operator's values v = (A p) at the lane's nn points and the centre values
+// pc of p.
+//   HeatOutW:    w = v; acc = sum of v p (the calc_w reduction)
+//   HeatOutInit: w = v, r = p - v (tea_leaf_common_init's residual)
+template <class T, int V, bool NT>
+struct HeatOutW {
+  const Ptr<T> &w;
+  double acc = 0.0;
+  __device__ __forceinline__ void operator()(int x, int y, int z, int nn, const T (&v)[V], const T (&pc)[V]) {
+#pragma unroll
+    for (int j = 0; j < V; j++)
+      if (j < nn) acc = acc + (double)v[j] * (double)pc[j];
+    stv<T, V, NT>(w.at(x, y, z), nn, v);
+  }
+};
+template <class T, int V, bool NT>
+struct HeatOutInit {
+  const Ptr<T> &w, &r;
+  double acc = 0.0;
+  __device__ __forceinline__ void operator()(int x, int y, int z, int nn, const T (&v)[V], const T (&pc)[V]) {
+    T rv[V];
+#pragma unroll
+    for (int j = 0; j < V; j++) rv[j] = pc[j] - v[j];
+    stv<T, V, NT>(w.at(x, y, z), nn, v);
+    stv<T, V, NT>(r.at(x, y, z), nn, rv);
+  }
 };
 
 // TeaLeaf 3D operator (apps/c/TeaLeaf/tea_leaf_cg_kernels.h:8-21 with the z
 // pair after the x pair), arithmetic in the reference order, over one column
-// tile.  INIT: w = A p, r = p - w; otherwise w = A p and run() returns this
-// thread's sum of w p.  Loads run PF planes ahead of the plane being
+// tile, in element type T (double; float for the FP32 inner sweeps of the
+// batched engine), the result of each plane handed to the output stage OUT
+// (above).  Loads run PF planes ahead of the plane being
 // computed, so a lane has PF planes of loads in flight while it waits at the
 // barrier; the PF + 2 planes held in registers form a ring indexed by the
 // step number modulo PF + 2, and the z loop is unrolled by the ring length
@@ -126,28 +156,26 @@ struct HeatPlane {
 // and Kz are read at plane z1 (the loop's stencils guarantee those, not the
 // rest).  FULL: the tile lies inside the range (every lane has V points, all
 // rows active), which drops the per-lane partial-load paths.
-template <int V, bool INIT, bool NT, int WY, int PF, bool FULL>
+template <class T, int V, int WY, int PF, bool FULL, class OUT>
 struct HeatColumn {
   static constexpr int RING = PF + 2;
-  HeatLdsV<V, WY> &L;
-  const Ptr<double> &w, &r;
-  const Ptr<const double> &Kx, &Ky, &Kz, &p;
-  double rx, ry, rz;
+  HeatLdsV<V, WY, T> &L;
+  OUT &out;
+  const Ptr<const T> &Kx, &Ky, &Kz, &p;
+  T rx, ry, rz;
   int x0, z0, z1;
   ColLane c;
   int x, y, o, lx, ly;
-  double acc;
-  double pm[V];
+  T pm[V];
   // (one rim slot shared by the low and high rims of a full tile saves 10
   // VGPRs but makes the compiler wait for all loads before refilling it)
-  using Plane = HeatPlane<V>;
+  using Plane = HeatPlane<V, T>;
   Plane R[RING];
 
-  __device__ __forceinline__ HeatColumn(HeatLdsV<V, WY> &L_, const Ptr<double> &w_, const Ptr<double> &r_,
-                                        const Ptr<const double> &Kx_, const Ptr<const double> &Ky_,
-                                        const Ptr<const double> &Kz_, const Ptr<const double> &p_, double rx_,
-                                        double ry_, double rz_, int x0_, int y0, int xe, int ye, int z0_, int z1_)
-      : L(L_), w(w_), r(r_), Kx(Kx_), Ky(Ky_), Kz(Kz_), p(p_), rx(rx_), ry(ry_), rz(rz_), x0(x0_), z0(z0_), z1(z1_) {
+  __device__ __forceinline__ HeatColumn(HeatLdsV<V, WY, T> &L_, OUT &out_, const Ptr<const T> &Kx_,
+                                        const Ptr<const T> &Ky_, const Ptr<const T> &Kz_, const Ptr<const T> &p_,
+                                        T rx_, T ry_, T rz_, int x0_, int y0, int xe, int ye, int z0_, int z1_)
+      : L(L_), out(out_), Kx(Kx_), Ky(Ky_), Kz(Kz_), p(p_), rx(rx_), ry(ry_), rz(rz_), x0(x0_), z0(z0_), z1(z1_) {
     lx = (int)threadIdx.x;
     ly = (int)threadIdx.y;
     c = col_lane<V, WY>(x0, y0, xe, ye);
@@ -155,7 +183,6 @@ struct HeatColumn {
     x = x0 + V * lx;
     y = y0 + ly;
     o = V + V * lx;
-    acc = 0.0;
   }
   __device__ __forceinline__ int n() const { return FULL ? V : c.n; }
   __device__ __forceinline__ bool act() const { return FULL || c.n > 0; }
@@ -165,19 +192,19 @@ struct HeatColumn {
   // same loads (the compiler then keeps them in flight across the barrier)
   __device__ __forceinline__ void load(int q, Plane &P) const {
     const int qk = min(q, z1 - 1);
-    ldv<double, V>(p.at(x, y, q), n(), P.p);
-    ldv<double, V, true>(Kz.at(x, y, q), n(), P.kz);
-    ldv<double, V, true>(Kx.at(x, y, qk), n(), P.kx);
-    ldv<double, V, true>(Ky.at(x, y, qk), n(), P.ky);
+    ldv<T, V>(p.at(x, y, q), n(), P.p);
+    ldv<T, V, true>(Kz.at(x, y, q), n(), P.kz);
+    ldv<T, V, true>(Kx.at(x, y, qk), n(), P.kx);
+    ldv<T, V, true>(Ky.at(x, y, qk), n(), P.ky);
     if (c.hxl) P.rxl = p(x0 - 1, y, qk);
     if (c.hxh) {
       P.rxh = p(x0 + c.cnt, y, qk);
       P.rkx = Kx(x0 + c.cnt, y, qk);
     }
-    if (c.hyl) ldv<double, V>(p.at(x, y - 1, qk), n(), P.ryl);
+    if (c.hyl) ldv<T, V>(p.at(x, y - 1, qk), n(), P.ryl);
     if (c.hyh) {
-      ldv<double, V>(p.at(x, y + 1, qk), n(), P.ryh);
-      ldv<double, V>(Ky.at(x, y + 1, qk), n(), P.rky);
+      ldv<T, V>(p.at(x, y + 1, qk), n(), P.ryh);
+      ldv<T, V>(Ky.at(x, y + 1, qk), n(), P.rky);
     }
   }
 
@@ -187,18 +214,18 @@ struct HeatColumn {
     const Plane &C = R[K % RING], &N = R[(K + 1) % RING];
     const int b = (z - z0) & 1, nn = n();
     if (act()) {
-      lds_st<double, V>(&L.sp[b][ly + 1][o], nn, C.p);
-      lds_st<double, V>(&L.skx[b][ly][o], nn, C.kx);
-      lds_st<double, V>(&L.sky[b][ly][V * lx], nn, C.ky);
+      lds_st<T, V>(&L.sp[b][ly + 1][o], nn, C.p);
+      lds_st<T, V>(&L.skx[b][ly][o], nn, C.kx);
+      lds_st<T, V>(&L.sky[b][ly][V * lx], nn, C.ky);
       if (c.hxl) L.sp[b][ly + 1][V - 1] = C.rxl;
       if (c.hxh) {
         L.sp[b][ly + 1][V + c.cnt] = C.rxh;
         L.skx[b][ly][V + c.cnt] = C.rkx;
       }
-      if (c.hyl) lds_st<double, V>(&L.sp[b][0][o], nn, C.ryl);
+      if (c.hyl) lds_st<T, V>(&L.sp[b][0][o], nn, C.ryl);
       if (c.hyh) {
-        lds_st<double, V>(&L.sp[b][ly + 2][o], nn, C.ryh);
-        lds_st<double, V>(&L.sky[b][ly + 1][V * lx], nn, C.rky);
+        lds_st<T, V>(&L.sp[b][ly + 2][o], nn, C.ryh);
+        lds_st<T, V>(&L.sky[b][ly + 1][V * lx], nn, C.rky);
       }
       // the slot takes plane z + PF + 1 (or, near the end, plane z1 again:
       // the slot held plane z - 1, which is no longer needed)
@@ -206,24 +233,21 @@ struct HeatColumn {
     }
     __syncthreads();
     if (act()) {
-      double pym[V], pyp[V], kyn[V], v[V], rv[V];
-      lds_ld<double, V>(&L.sp[b][ly][o], pym);
-      lds_ld<double, V>(&L.sp[b][ly + 2][o], pyp);
-      lds_ld<double, V>(&L.sky[b][ly + 1][V * lx], kyn);
-      const double left = L.sp[b][ly + 1][o - 1];
+      T pym[V], pyp[V], kyn[V], v[V];
+      lds_ld<T, V>(&L.sp[b][ly][o], pym);
+      lds_ld<T, V>(&L.sp[b][ly + 2][o], pyp);
+      lds_ld<T, V>(&L.sky[b][ly + 1][V * lx], kyn);
+      const T left = L.sp[b][ly + 1][o - 1];
 #pragma unroll
       for (int j = 0; j < V; j++) {
-        const double pxm = j == 0 ? left : C.p[j == 0 ? 0 : j - 1];
-        const double pxp = j + 1 < nn ? C.p[j + 1 < V ? j + 1 : j] : L.sp[b][ly + 1][o + j + 1];
-        const double kxn = j + 1 < nn ? C.kx[j + 1 < V ? j + 1 : j] : L.skx[b][ly][o + j + 1];
-        v[j] = (1.0 + ry * (kyn[j] + C.ky[j]) + rx * (kxn + C.kx[j]) + rz * (N.kz[j] + C.kz[j])) * C.p[j] -
+        const T pxm = j == 0 ? left : C.p[j == 0 ? 0 : j - 1];
+        const T pxp = j + 1 < nn ? C.p[j + 1 < V ? j + 1 : j] : L.sp[b][ly + 1][o + j + 1];
+        const T kxn = j + 1 < nn ? C.kx[j + 1 < V ? j + 1 : j] : L.skx[b][ly][o + j + 1];
+        v[j] = ((T)1.0 + ry * (kyn[j] + C.ky[j]) + rx * (kxn + C.kx[j]) + rz * (N.kz[j] + C.kz[j])) * C.p[j] -
                ry * (kyn[j] * pyp[j] + C.ky[j] * pym[j]) - rx * (kxn * pxp + C.kx[j] * pxm) -
                rz * (N.kz[j] * N.p[j] + C.kz[j] * pm[j]);
-        rv[j] = C.p[j] - v[j];
-        if (!INIT && j < nn) acc = acc + v[j] * C.p[j];
       }
-      stv<double, V, NT>(w.at(x, y, z), nn, v);
-      if constexpr (INIT) stv<double, V, NT>(r.at(x, y, z), nn, rv);
+      out(x, y, z, nn, v, C.p);
     }
 #pragma unroll
     for (int j = 0; j < V; j++) pm[j] = C.p[j];
@@ -245,37 +269,56 @@ struct HeatColumn {
     }
   }
 
-  __device__ __forceinline__ double run() {
+  __device__ __forceinline__ void run() {
 #pragma unroll
     for (int i = 0; i < RING; i++) {
 #pragma unroll
       for (int j = 0; j < V; j++)
-        R[i].p[j] = R[i].kz[j] = R[i].kx[j] = R[i].ky[j] = R[i].ryl[j] = R[i].ryh[j] = R[i].rky[j] = 0.0;
-      R[i].rxl = R[i].rxh = R[i].rkx = 0.0;
+        R[i].p[j] = R[i].kz[j] = R[i].kx[j] = R[i].ky[j] = R[i].ryl[j] = R[i].ryh[j] = R[i].rky[j] = (T)0;
+      R[i].rxl = R[i].rxh = R[i].rkx = (T)0;
     }
 #pragma unroll
-    for (int j = 0; j < V; j++) pm[j] = 0.0;
+    for (int j = 0; j < V; j++) pm[j] = (T)0;
     if (act()) {
-      ldv<double, V>(p.at(x, y, z0 - 1), n(), pm);
+      ldv<T, V>(p.at(x, y, z0 - 1), n(), pm);
 #pragma unroll
       for (int i = 0; i < PF + 1; i++) load(min(z0 + i, z1), R[i]);
     }
     int z = z0;
     for (; z + RING <= z1; z += RING) steps_all<0>(z);
     steps_tail<0>(z);
-    return acc;
   }
 };
 
+// the column over one tile with output stage out: the branch-free form when
+// the tile lies inside the range
+template <class T, int V, int WY, int PF, class OUT>
+__device__ __forceinline__ void heat3_column_out(HeatLdsV<V, WY, T> &L, OUT &out, const Ptr<const T> &Kx,
+                                                 const Ptr<const T> &Ky, const Ptr<const T> &Kz, const Ptr<const T> &p,
+                                                 T rx, T ry, T rz, int x0, int y0, int xe, int ye, int z0, int z1) {
+  if (x0 + kWave * V <= xe && y0 + WY <= ye)
+    HeatColumn<T, V, WY, PF, true, OUT>(L, out, Kx, Ky, Kz, p, rx, ry, rz, x0, y0, xe, ye, z0, z1).run();
+  else
+    HeatColumn<T, V, WY, PF, false, OUT>(L, out, Kx, Ky, Kz, p, rx, ry, rz, x0, y0, xe, ye, z0, z1).run();
+}
+
+// INIT: w = A p, r = p - w (returns 0); otherwise w = A p, returns this
+// thread's sum of w p
 template <int V, bool INIT, bool NT, int WY = kWavesPerBlock, int PF = 1>
 __device__ __forceinline__ double heat3_column_v(HeatLdsV<V, WY> &L, const Ptr<double> &w, const Ptr<double> &r,
                                                  const Ptr<const double> &Kx, const Ptr<const double> &Ky,
                                                  const Ptr<const double> &Kz, const Ptr<const double> &p, double rx,
                                                  double ry, double rz, int x0, int y0, int xe, int ye, int z0,
                                                  int z1) {
-  if (x0 + kWave * V <= xe && y0 + WY <= ye)
-    return HeatColumn<V, INIT, NT, WY, PF, true>(L, w, r, Kx, Ky, Kz, p, rx, ry, rz, x0, y0, xe, ye, z0, z1).run();
-  return HeatColumn<V, INIT, NT, WY, PF, false>(L, w, r, Kx, Ky, Kz, p, rx, ry, rz, x0, y0, xe, ye, z0, z1).run();
+  if constexpr (INIT) {
+    HeatOutInit<double, V, NT> out{w, r};
+    heat3_column_out<double, V, WY, PF>(L, out, Kx, Ky, Kz, p, rx, ry, rz, x0, y0, xe, ye, z0, z1);
+    return 0.0;
+  } else {
+    HeatOutW<double, V, NT> out{w};
+    heat3_column_out<double, V, WY, PF>(L, out, Kx, Ky, Kz, p, rx, ry, rz, x0, y0, xe, ye, z0, z1);
+    return out.acc;
+  }
 }
 
 // ------------------------------------------------------------ fused CG chain

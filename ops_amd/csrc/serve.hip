@@ -51,12 +51,14 @@ struct SolveDev {
   int tile0, ntiles;
   int active;
   double rx, ry, rz;
-  // FP32-inner refinement (2D Poisson): u = x and f (FP64), and the FP32
-  // inner fields e, rr, p, w with their own row pitch
+  // FP32-inner refinement: u = x and f (FP64), and the FP32 inner fields
+  // e, rr, p, w (3D heat: also the face coefficients rounded to float) with
+  // their own row and plane pitch
   int mixed;
   double *f;
   float *e32, *r32, *p32, *w32;
-  long sy32;
+  float *kx32, *ky32, *kz32;
+  long sy32, sz32;
 };
 
 // phases of a mixed solve: the next iteration runs an inner FP32 PCG step,
@@ -142,10 +144,45 @@ __device__ __forceinline__ void store_partials(double (&acc)[D], double *dst, do
     for (int q = 0; q < D; q++) dst[q] = acc[q];
 }
 
-__device__ __forceinline__ Ptr<float> PF32(float *a, const SolveDev &d) { return Ptr<float>{a, d.sy32, 0}; }
+__device__ __forceinline__ Ptr<float> PF32(float *a, const SolveDev &d) { return Ptr<float>{a, d.sy32, d.sz32}; }
 __device__ __forceinline__ Ptr<const float> PF32C(const float *a, const SolveDev &d) {
-  return Ptr<const float>{a, d.sy32, 0};
+  return Ptr<const float>{a, d.sy32, d.sz32};
 }
+
+// output stage of the FP64 heat column for a mixed heat solve's residual
+// pass: r = f - A x, the inner start rr = (float) r, e = 0, p = rr (no
+// preconditioner), sums (r.r, rr.rr, rr.rr) in FP64
+struct HeatOutResid {
+  const Ptr<const double> &f;
+  const Ptr<float> &rr, &e, &p;
+  double acc[3] = {0.0, 0.0, 0.0};
+  __device__ __forceinline__ void operator()(int x, int y, int z, int nn, const double (&v)[kSV],
+                                             const double (&pc)[kSV]) {
+    double fv[kSV];
+    ldv<double, kSV>(f.at(x, y, z), nn, fv);
+    float q32[kSV], zero[kSV];
+#pragma unroll
+    for (int j = 0; j < kSV; j++) {
+      const double r = fv[j] - v[j];
+      q32[j] = (float)r;
+      zero[j] = 0.0f;
+      if (j < nn) {
+        acc[0] = acc[0] + r * r;
+        acc[1] = acc[1] + (double)q32[j] * (double)q32[j];
+        acc[2] = acc[2] + (double)q32[j] * (double)q32[j];
+      }
+    }
+    stv<float, kSV, false>(e.at(x, y, z), nn, zero);
+    stv<float, kSV, false>(rr.at(x, y, z), nn, q32);
+    stv<float, kSV, false>(p.at(x, y, z), nn, q32);
+  }
+};
+
+// LDS of the calc_w pass: the FP64 heat column or its FP32 form
+union CalcWLds {
+  HeatLdsV<kSV, kWavesPerBlock> d;
+  HeatLdsV<kSV, kWavesPerBlock, float> f;
+};
 
 // the 2D 5-point operator over a group: the group's row and the rows above
 // and below, and the neighbours just left and right of the group
@@ -173,8 +210,9 @@ __device__ __forceinline__ T op5(const Row5<T> &R, int j, int n) {
 // ------------------------------------------------------------ iteration
 __global__ __launch_bounds__(256) void sk_calc_w(const SolveDev *S, const SolveStat *st, const int2 *tiles,
                                                  const int *ntiles, double *part) {
-  __shared__ HeatLdsV<kSV, kWavesPerBlock> lds;
+  __shared__ CalcWLds lds;
   __shared__ double sh[16][1];
+  __shared__ double sh3[16][3];
   const int nt = *ntiles;
   for (int t = (int)xcd_tile(blockIdx.x, gridDim.x); t < nt; t += (int)gridDim.x) {
     const int2 e = tiles[t];
@@ -182,6 +220,29 @@ __global__ __launch_bounds__(256) void sk_calc_w(const SolveDev *S, const SolveS
     const SolveDev d = S[e.x];
     const TileBox b = tile_box(d, e.y);
     double acc = 0.0;
+    if (d.mixed && d.kind == OPSMI_SOLVE_HEAT3D) {
+      __syncthreads();  // LDS of the previous tile fully read
+      if (st[e.x].phase == PH_RESID) {
+        // FP64 residual of the corrected x (the correction x += e ran in the
+        // previous iteration's last pass) and the inner start
+        const Ptr<const double> f = PC(d.f, d);
+        const Ptr<float> rr = PF32(d.r32, d), ev = PF32(d.e32, d), p = PF32(d.p32, d);
+        HeatOutResid out{f, rr, ev, p};
+        heat3_column_out<double, kSV, kWavesPerBlock, 1>(lds.d, out, PC(d.kx, d), PC(d.ky, d), PC(d.kz, d),
+                                                         PC(d.u, d), d.rx, d.ry, d.rz, b.x0, b.y0, d.nx, d.ny, b.z0,
+                                                         b.z1);
+        store_partials<3>(out.acc, part + (size_t)t * kPart, sh3);
+        continue;
+      }
+      // inner sweep in FP32 with float coefficients: w = A p, p.w in FP64
+      const Ptr<float> w = PF32(d.w32, d);
+      HeatOutW<float, kSV, false> out{w};
+      heat3_column_out<float, kSV, kWavesPerBlock, 1>(lds.f, out, PF32C(d.kx32, d), PF32C(d.ky32, d),
+                                                      PF32C(d.kz32, d), PF32C(d.p32, d), (float)d.rx, (float)d.ry,
+                                                      (float)d.rz, b.x0, b.y0, d.nx, d.ny, b.z0, b.z1);
+      store_partial(out.acc, part + (size_t)t * kPart, sh);
+      continue;
+    }
     if (d.mixed) {
       if (st[e.x].phase == PH_RESID) {
         // the correction x = x + (double) e (pointwise; the residual follows
@@ -218,7 +279,7 @@ __global__ __launch_bounds__(256) void sk_calc_w(const SolveDev *S, const SolveS
     }
     if (d.kind == OPSMI_SOLVE_HEAT3D) {
       __syncthreads();  // LDS of the previous tile fully read
-      acc = heat3_column_v<kSV, false, true, kWavesPerBlock, 1>(lds, P(d.w, d), Ptr<double>{nullptr, 0, 0},
+      acc = heat3_column_v<kSV, false, true, kWavesPerBlock, 1>(lds.d, P(d.w, d), Ptr<double>{nullptr, 0, 0},
                                                                  PC(d.kx, d), PC(d.ky, d), PC(d.kz, d), PC(d.p, d),
                                                                  d.rx, d.ry, d.rz, b.x0, b.y0, d.nx, d.ny, b.z0,
                                                                  b.z1);
@@ -251,7 +312,11 @@ __global__ __launch_bounds__(256) void sk_update(const SolveDev *S, const SolveS
     if (st[e.x].done) continue;
     const SolveDev d = S[e.x];
     if (d.mixed) {
-      const float mi = 0.25f;  // Jacobi: 1 / diag of the 5-point operator
+      // 2D Poisson: Jacobi, 1 / diag of the 5-point operator; 3D heat: none
+      const bool heat = d.kind == OPSMI_SOLVE_HEAT3D;
+      const float mi = heat ? 1.0f : 0.25f;
+      // a heat solve's residual pass ran in calc_w (its sums are in place)
+      if (heat && st[e.x].phase == PH_RESID) continue;
       const Ptr<float> ev = PF32(d.e32, d), rr = PF32(d.r32, d), p = PF32(d.p32, d), w = PF32(d.w32, d);
       double acc[3] = {0.0, 0.0, 0.0};
       if (st[e.x].phase == PH_INNER) {
@@ -340,9 +405,25 @@ __global__ __launch_bounds__(256) void sk_axpby(const SolveDev *S, const SolveSt
     if (st[e.x].done) continue;
     const SolveDev d = S[e.x];
     if (d.mixed) {
-      // p = Mi rr + b p, unless the inner solve just stopped or started
-      if (!st[e.x].axpby) continue;
-      const float mi = 0.25f, bf = (float)st[e.x].beta;
+      // p = Mi rr + b p, unless the inner solve just stopped or started;
+      // a heat solve whose inner solve just stopped applies x += e here
+      const int ax = st[e.x].axpby;
+      if (!ax) continue;
+      if (ax == 2) {
+        const Ptr<double> x = P(d.u, d);
+        const Ptr<const float> ev = PF32C(d.e32, d);
+        tile_groups(d, tile_box(d, e.y), [&](int i, int j, int k, int n) {
+          double xv[kSV];
+          float evv[kSV];
+          ldv<double, kSV>(x.at(i, j, k), n, xv);
+          ldv<float, kSV>(ev.at(i, j, k), n, evv);
+#pragma unroll
+          for (int q = 0; q < kSV; q++) xv[q] = xv[q] + (double)evv[q];
+          stv<double, kSV, false>(x.at(i, j, k), n, xv);
+        });
+        continue;
+      }
+      const float mi = d.kind == OPSMI_SOLVE_HEAT3D ? 1.0f : 0.25f, bf = (float)st[e.x].beta;
       const Ptr<float> p = PF32(d.p32, d), rr = PF32(d.r32, d);
       tile_groups(d, tile_box(d, e.y), [&](int i, int j, int k, int n) {
         float pv[kSV], rv[kSV];
@@ -434,7 +515,9 @@ __global__ __launch_bounds__(256) void sk_fold(const SolveDev *S, SolveStat *st,
     if (sqrt(rrn) <= q.istop) {
       q.phase = PH_RESID;
       q.outer = q.outer + 1;
-      q.axpby = 0;
+      // 2D Poisson: the next calc_w pass applies x += e; 3D heat: this
+      // iteration's axpby pass does, and the next calc_w is the residual
+      q.axpby = S[s].kind == OPSMI_SOLVE_HEAT3D ? 2 : 0;
     } else if (q.iters >= q.max_iters) {
       q.done = 2;
     } else {
@@ -527,6 +610,22 @@ __global__ __launch_bounds__(256) void sk_first_residual(SolveDev d, uint64_t se
   store_partial(acc, part + li, sh);
 }
 
+// mixed 3D heat, after sk_heat_inputs and sk_heat_coefs wrote 1/K into f's
+// storage, x = rhs and the face coefficients: f = rhs inside, and the face
+// coefficients rounded to float over the extended box
+__global__ __launch_bounds__(256) void sk_mixed_heat_finish(SolveDev d) {
+  const long ne = (long)(d.nx + 2) * (d.ny + 2) * (d.nz + 2);
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < ne; q += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(q % (d.nx + 2)) - 1;
+    const long rest = q / (d.nx + 2);
+    const int y = (int)(rest % (d.ny + 2)) - 1, z = (int)(rest / (d.ny + 2)) - 1;
+    if (x >= 0 && x < d.nx && y >= 0 && y < d.ny && z >= 0 && z < d.nz) P(d.f, d)(x, y, z) = P(d.u, d)(x, y, z);
+    PF32(d.kx32, d)(x, y, z) = (float)P(d.kx, d)(x, y, z);
+    PF32(d.ky32, d)(x, y, z) = (float)P(d.ky, d)(x, y, z);
+    PF32(d.kz32, d)(x, y, z) = (float)P(d.kz, d)(x, y, z);
+  }
+}
+
 // mixed 2D Poisson: f = rhs inside (x and the FP32 fields start at zero);
 // the first iteration's residual pass finds r = f
 __global__ __launch_bounds__(256) void sk_mixed_inputs(SolveDev d, uint64_t seed) {
@@ -613,25 +712,29 @@ struct Arena {
 
 struct Geometry {
   int nfields, nx, ny, nz;
-  long pitch, plane, field;  // double fields, elements
-  int nfields32;             // FP32 fields (mixed solves)
-  long pitch32, field32;     // elements
+  long pitch, plane, field;        // double fields, elements
+  int nfields32;                   // FP32 fields (mixed solves)
+  long pitch32, plane32, field32;  // elements
   size_t bytes;
 };
 
 Geometry geometry(const opsmi_solve_req &q) {
   Geometry g;
   const bool heat = q.kind == OPSMI_SOLVE_HEAT3D;
-  g.nfields = heat ? 7 : (q.mixed ? 2 : 4);
+  // FP64 fields: u, p, r, w [, kx, ky, kz]; mixed: x, f [, kx, ky, kz]
+  g.nfields = heat ? (q.mixed ? 5 : 7) : (q.mixed ? 2 : 4);
   g.nx = g.ny = q.n;
   g.nz = heat ? q.n : 1;
   g.pitch = ((long)g.nx + 2 + 31) / 32 * 32;
   g.plane = g.pitch * (g.ny + 2);
   // 32 lead elements put x = 0 on a 256-byte boundary; round to 256 bytes
   g.field = (g.plane * (heat ? g.nz + 2 : 1) + 32 + 31) / 32 * 32;
-  g.nfields32 = q.mixed ? 4 : 0;
+  // FP32 fields of a mixed solve: e, rr, p, w [, kx, ky, kz]
+  g.nfields32 = q.mixed ? (heat ? 7 : 4) : 0;
   g.pitch32 = ((long)g.nx + 2 + 63) / 64 * 64;
-  g.field32 = (g.pitch32 * (g.ny + 2) + 64 + 63) / 64 * 64;
+  g.plane32 = g.pitch32 * (g.ny + 2);
+  // 64 lead elements put x = 0 on a 256-byte boundary
+  g.field32 = (g.plane32 * (heat ? g.nz + 2 : 1) + 64 + 63) / 64 * 64;
   g.bytes = (size_t)g.nfields * g.field * sizeof(double) + (size_t)g.nfields32 * g.field32 * sizeof(float);
   return g;
 }
@@ -672,6 +775,7 @@ struct opsmi_server_s {
   long tiles_active = 0;  // tiles of the admitted solves (the next table)
   // tile parameter per size class (defaults; opsmi_serve_autotune measures)
   int chunk3d[3] = {kChunk3D, kChunk3D, kChunk3D};
+  int chunk3d_mixed[3] = {kChunk3D, kChunk3D, kChunk3D};  // FP32-inner heat solves
   int rows2d[3] = {kRows2D, kRows2D, kRows2D};
   int rows2d_mixed[3] = {kRows2D, kRows2D, kRows2D};  // FP32-inner solves
 };
@@ -685,8 +789,8 @@ void check_req(const opsmi_solve_req &q) {
     fail(OPS_INVALID_ARGUMENT, cat("opsmi_serve_submit: grid side ", q.n, " out of range"));
   if (!(q.tol > 0.0) || q.max_iters < 1)
     fail(OPS_INVALID_ARGUMENT, "opsmi_serve_submit: tol must be > 0 and max_iters >= 1");
-  if (q.mixed != 0 && (q.mixed != 1 || q.kind != OPSMI_SOLVE_POISSON2D))
-    fail(OPS_INVALID_ARGUMENT, "opsmi_serve_submit: the FP32-inner mode (mixed = 1) is for 2D Poisson solves");
+  if (q.mixed != 0 && q.mixed != 1)
+    fail(OPS_INVALID_ARGUMENT, "opsmi_serve_submit: mixed must be 0 (FP64) or 1 (FP32-inner refinement)");
 }
 
 // tile parameter of a solve: z planes per 3D tile, rows per wave of a 2D
@@ -696,7 +800,7 @@ int size_class(int kind, int n) {
   return n < 256 ? 0 : (n < 1024 ? 1 : 2);
 }
 int tile_param(opsmi_server S, int kind, int n, int mixed) {
-  if (kind == OPSMI_SOLVE_HEAT3D) return S->chunk3d[size_class(kind, n)];
+  if (kind == OPSMI_SOLVE_HEAT3D) return mixed ? S->chunk3d_mixed[size_class(kind, n)] : S->chunk3d[size_class(kind, n)];
   return mixed ? S->rows2d_mixed[size_class(kind, n)] : S->rows2d[size_class(kind, n)];
 }
 
@@ -740,7 +844,7 @@ void admit(opsmi_server S) {
     size_t off;
     if (!S->arena.alloc(g.bytes, &off)) return;
     int tx, ty;
-    const int param = tile_param(S, R.q.kind, R.q.n, R.q.mixed && R.q.kind == OPSMI_SOLVE_POISSON2D);
+    const int param = tile_param(S, R.q.kind, R.q.n, R.q.mixed);
     const long nt = tiles_of(R.q, param, &tx, &ty);
     if ((size_t)(S->tiles_active + nt) > S->max_tiles) {
       S->arena.release(off, g.bytes);
@@ -754,23 +858,35 @@ void admit(opsmi_server S) {
     R.bytes = g.bytes;
     S->in_use += g.bytes;
     S->slot_req[slot] = id;
-    // fields: u, p, r, w [, kx, ky, kz]; mixed: x, f, then FP32 e, rr, p, w
+    // fields: u, p, r, w [, kx, ky, kz]; mixed: x, f [, kx, ky, kz], then
+    // FP32 e, rr, p, w [, kx, ky, kz]
     double *base = (double *)(S->arena_base + off);
     double *f[7] = {};
     for (int i = 0; i < g.nfields; i++) f[i] = base + (size_t)i * g.field;
     // element (-1,-1,-1) sits at field + 31, so x = 0 starts a 256-byte line
     const long lead = 32 + g.pitch + (R.q.kind == OPSMI_SOLVE_HEAT3D ? g.plane : 0);
     SolveDev d{};
+    const bool heat = R.q.kind == OPSMI_SOLVE_HEAT3D;
     if (R.q.mixed) {
       float *b32 = (float *)(base + (size_t)g.nfields * g.field);
-      const long lead32 = 64 + g.pitch32;  // float (-1,-1) at +63
+      // float element (-1,-1[,-1]) at +63
+      const long lead32 = 64 + g.pitch32 + (heat ? g.plane32 : 0);
       d.u = f[0] + lead;
       d.f = f[1] + lead;
       d.e32 = b32 + lead32;
       d.r32 = b32 + g.field32 + lead32;
       d.p32 = b32 + 2 * g.field32 + lead32;
       d.w32 = b32 + 3 * g.field32 + lead32;
+      if (heat) {
+        d.kx = f[2] + lead;
+        d.ky = f[3] + lead;
+        d.kz = f[4] + lead;
+        d.kx32 = b32 + 4 * g.field32 + lead32;
+        d.ky32 = b32 + 5 * g.field32 + lead32;
+        d.kz32 = b32 + 6 * g.field32 + lead32;
+      }
       d.sy32 = g.pitch32;
+      d.sz32 = heat ? g.plane32 : 0;
       d.mixed = 1;
     } else {
       d.u = f[0] + lead;
@@ -778,7 +894,7 @@ void admit(opsmi_server S) {
       d.r = f[2] + lead;
       d.w = f[3] + lead;
     }
-    if (R.q.kind == OPSMI_SOLVE_HEAT3D) {
+    if (heat && !R.q.mixed) {
       d.kx = f[4] + lead;
       d.ky = f[5] + lead;
       d.kz = f[6] + lead;
@@ -801,10 +917,22 @@ void admit(opsmi_server S) {
     const size_t nall = g.bytes / sizeof(double);
     hipLaunchKernelGGL(sk_zero, dim3((unsigned)std::min<size_t>((nall + 255) / 256, 4096)), dim3(256), 0, st, base,
                        nall);
-    if (R.q.mixed) {
+    if (R.q.mixed && heat) {
+      // 1/K into f's storage and x = rhs (sk_heat_inputs), the face
+      // coefficients from them, then f = rhs and the float coefficients
+      SolveDev dk = d;
+      dk.w = d.f;
+      const long ne = (long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+      const unsigned gb = (unsigned)std::min<long>((ne + 255) / 256, 4096);
+      hipLaunchKernelGGL(sk_heat_inputs, dim3(gb), dim3(256), 0, st, dk, R.q.seed);
+      hipLaunchKernelGGL(sk_heat_coefs, dim3(gb), dim3(256), 0, st, dk);
+      hipLaunchKernelGGL(sk_mixed_heat_finish, dim3(gb), dim3(256), 0, st, d);
+    } else if (R.q.mixed) {
       const long ni = (long)g.nx * g.ny;
       hipLaunchKernelGGL(sk_mixed_inputs, dim3((unsigned)std::min<long>((ni + 255) / 256, 4096)), dim3(256), 0, st, d,
                          R.q.seed);
+    }
+    if (R.q.mixed) {
       // inner FP32 solves run to 1e-6 of their initial residual (or
       // tol / (2 relres)): on the 60-request serving mix 1e-4 took 100 502
       // iterations, 1e-5 96 018, 1e-6 90 305, 1e-7 91 951, 1e-8 96 738
@@ -930,7 +1058,7 @@ int opsmi_serve_submit(opsmi_server S, const opsmi_solve_req *req) {
   if (g.bytes > S->budget)
     fail(OPS_INVALID_ARGUMENT, cat("opsmi_serve_submit: solve needs ", g.bytes, " bytes, budget is ", S->budget));
   int tx, ty;
-  if ((size_t)tiles_of(*req, tile_param(S, req->kind, req->n, req->mixed && req->kind == OPSMI_SOLVE_POISSON2D),
+  if ((size_t)tiles_of(*req, tile_param(S, req->kind, req->n, req->mixed),
                        &tx, &ty) > S->max_tiles)
     fail(OPS_INVALID_ARGUMENT, "opsmi_serve_submit: solve has more tiles than the tile table holds");
   const int id = S->next_id++;
@@ -1038,14 +1166,16 @@ int opsmi_serve_autotune(opsmi_server S, int iters) {
   hipEvent_t e0, e1;
   OPSMI_HIP(hipEventCreate(&e0));
   OPSMI_HIP(hipEventCreate(&e1));
-  // 2D Poisson: FP64 and FP32-inner solves get their own tables (the FP32
-  // kernels move half the bytes per point and may prefer other rows)
-  for (int km = 0; km < 3; km++)
+  // FP64 and FP32-inner solves get their own tables (the FP32 kernels move
+  // half the bytes per point and may prefer other tiles): km 0 Poisson,
+  // 1 heat, 2 FP32-inner Poisson, 3 FP32-inner heat
+  for (int km = 0; km < 4; km++)
     for (int cls = 0; cls < 3; cls++) {
-      const int kind = km == 1 ? OPSMI_SOLVE_HEAT3D : OPSMI_SOLVE_POISSON2D, mixed = km == 2;
+      const int kind = km % 2 == 1 ? OPSMI_SOLVE_HEAT3D : OPSMI_SOLVE_POISSON2D, mixed = km >= 2;
       opsmi_solve_req q{kind, rep[kind][cls], 12345, 1e-300, 1 << 30, mixed};
       if (geometry(q).bytes > S->budget) continue;
-      int *slot_param = kind == OPSMI_SOLVE_HEAT3D ? &S->chunk3d[cls] : (mixed ? &S->rows2d_mixed[cls] : &S->rows2d[cls]);
+      int *slot_param = kind == OPSMI_SOLVE_HEAT3D ? (mixed ? &S->chunk3d_mixed[cls] : &S->chunk3d[cls])
+                                                   : (mixed ? &S->rows2d_mixed[cls] : &S->rows2d[cls]);
       const int *cand = kind == OPSMI_SOLVE_HEAT3D ? cand3 : cand2;
       const int nc = 4;
       int best = *slot_param;

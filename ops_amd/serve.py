@@ -66,8 +66,8 @@ class Server:
         self.shapes = {}
 
     def submit(self, kind, n, seed=20240611, tol=1e-10, max_iters=100000, mixed=False):
-        """mixed: FP32-inner refinement (2D Poisson)."""
-        r = SolveReq(kind, n, seed, tol, max_iters, 1 if mixed else 0)
+        """mixed: FP32-inner refinement (2D Poisson and 3D heat)."""
+        r = SolveReq(kind, n, seed, tol, max_iters, int(mixed))
         i = lib().opsmi_serve_submit(self.h, C.byref(r))
         ops._check()
         self.shapes[i] = (n, n) if kind == POISSON2D else (n, n, n)

@@ -188,14 +188,16 @@ void orc_heat_coefs(int ndim, int nx, int ny, int nz, const double *K, double *K
       }
 }
 
-/* operator of tea_leaf_cg_kernels.h:8-15 (+ z terms appended in 3D) */
-static inline double heat_op(const Geo &g, const double *Kx, const double *Ky, const double *Kz, const double *p,
-                             long q, double rx, double ry, double rz) {
+/* operator of tea_leaf_cg_kernels.h:8-15 (+ z terms appended in 3D), in
+ * element type T (double; float for the FP32 inner sweeps of the mixed
+ * heat solve) */
+template <class T>
+static inline T heat_op(const Geo &g, const T *Kx, const T *Ky, const T *Kz, const T *p, long q, T rx, T ry, T rz) {
   const long sx = g.sx, sz = g.dz();
   if (g.ndim == 2)
-    return (1.0 + ry * (Ky[q + sx] + Ky[q]) + rx * (Kx[q + 1] + Kx[q])) * p[q] -
+    return ((T)1.0 + ry * (Ky[q + sx] + Ky[q]) + rx * (Kx[q + 1] + Kx[q])) * p[q] -
            ry * (Ky[q + sx] * p[q + sx] + Ky[q] * p[q - sx]) - rx * (Kx[q + 1] * p[q + 1] + Kx[q] * p[q - 1]);
-  return (1.0 + ry * (Ky[q + sx] + Ky[q]) + rx * (Kx[q + 1] + Kx[q]) + rz * (Kz[q + sz] + Kz[q])) * p[q] -
+  return ((T)1.0 + ry * (Ky[q + sx] + Ky[q]) + rx * (Kx[q + 1] + Kx[q]) + rz * (Kz[q + sz] + Kz[q])) * p[q] -
          ry * (Ky[q + sx] * p[q + sx] + Ky[q] * p[q - sx]) - rx * (Kx[q + 1] * p[q + 1] + Kx[q] * p[q - 1]) -
          rz * (Kz[q + sz] * p[q + sz] + Kz[q] * p[q - sz]);
 }
@@ -276,6 +278,92 @@ int orc_heat_cg(int ndim, int nx, int ny, int nz, const double *K, const double 
 }
 
 double orc_last_loop_seconds(void) { return g_loop_seconds; }
+
+/* FP64 iterative refinement around FP32 CG for the 3D heat problem (no
+ * reference counterpart: the batched engine's FP32-inner heat mode).  The
+ * outer loop starts from TeaLeaf's x = u0 = rhs, computes r = f - A x in
+ * FP64 (f = rhs) and stops at |r| <= tol |r_0|.  Each correction solves
+ * A e = r by unpreconditioned CG in FP32 storage and arithmetic, with the
+ * face coefficients rounded to float, every dot product accumulated in
+ * FP64, until |rr| <= max(inner_tol, tol / (2 relres)) |rr_0|; then
+ * x += (double) e.  Returns the outer iterations; relres_h[k] = |r_k|/|r_0|. */
+int orc_heat_mixed(int nx, int ny, int nz, const double *K, const double *rhs, double tol, int max_outer,
+                   int inner_iters, double inner_tol, int nt, double *x_int, double *relres_h, int *inner_total) {
+  Geo g(3, nx, ny, nz);
+  long N = g.next();
+  std::vector<double> Kx(N), Ky(N), Kz(N), x(N, 0.0), f(N, 0.0), r(N, 0.0);
+  orc_heat_coefs(3, nx, ny, nz, K, Kx.data(), Ky.data(), Kz.data());
+  std::vector<float> kx(N), ky(N), kz(N), e(N, 0.0f), rr(N, 0.0f), z(N, 0.0f), p(N, 0.0f), w(N, 0.0f);
+  for (long q = 0; q < N; q++) {
+    kx[q] = (float)Kx[q];
+    ky[q] = (float)Ky[q];
+    kz[q] = (float)Kz[q];
+  }
+  sweep(nt, 0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {
+    const long q = g.at(i, j, k);
+    x[q] = f[q] = rhs[g.lin(i, j, k)];
+  });
+  double fn = 0.0;
+  int outer = 0, total = 0;
+  while (true) {
+    const double rrd = sweep_sum(nt, 0, nx, 0, ny, 0, nz, [&](int i, int j, int k, double &s) {
+      const long q = g.at(i, j, k);
+      const double ax = heat_op<double>(g, Kx.data(), Ky.data(), Kz.data(), x.data(), q, 1.0, 1.0, 1.0);
+      r[q] = f[q] - ax;
+      s = s + r[q] * r[q];
+    });
+    if (outer == 0) fn = sqrt(rrd);
+    const double rel = sqrt(rrd) / fn;
+    if (relres_h) relres_h[outer] = rel;
+    if (rel <= tol || outer >= max_outer) break;
+    const double rz = sweep_sum(nt, 0, nx, 0, ny, 0, nz, [&](int i, int j, int k, double &s) {
+      const long q = g.at(i, j, k);
+      e[q] = 0.0f;
+      rr[q] = (float)r[q];
+      z[q] = rr[q];
+      p[q] = z[q];
+      s = s + (double)rr[q] * (double)z[q];
+    });
+    double rzc = rz;
+    const double istop = fmax(inner_tol, 0.5 * tol / rel) * sqrt(rz);
+    for (int it = 0; it < inner_iters; it++) {
+      const double pw = sweep_sum(nt, 0, nx, 0, ny, 0, nz, [&](int i, int j, int k, double &s) {
+        const long q = g.at(i, j, k);
+        w[q] = heat_op<float>(g, kx.data(), ky.data(), kz.data(), p.data(), q, 1.0f, 1.0f, 1.0f);
+        s = s + (double)w[q] * (double)p[q];
+      });
+      const float a = (float)(rzc / pw);
+      double rzn = 0.0, rrn = 0.0;
+#pragma omp parallel for collapse(2) num_threads(nt) reduction(+ : rzn, rrn) schedule(static)
+      for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+          for (int i = 0; i < nx; i++) {
+            const long q = g.at(i, j, k);
+            e[q] = e[q] + a * p[q];
+            rr[q] = rr[q] - a * w[q];
+            z[q] = rr[q];
+            rzn = rzn + (double)rr[q] * (double)z[q];
+            rrn = rrn + (double)rr[q] * (double)rr[q];
+          }
+      total++;
+      if (sqrt(rrn) <= istop) break;
+      const float b = (float)(rzn / rzc);
+      sweep(nt, 0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {
+        const long q = g.at(i, j, k);
+        p[q] = z[q] + b * p[q];
+      });
+      rzc = rzn;
+    }
+    sweep(nt, 0, nx, 0, ny, 0, nz, [&](int i, int j, int k) {
+      const long q = g.at(i, j, k);
+      x[q] = x[q] + (double)e[q];
+    });
+    outer++;
+  }
+  if (inner_total) *inner_total = total;
+  if (x_int) sweep(nt, 0, nx, 0, ny, 0, nz, [&](int i, int j, int k) { x_int[g.lin(i, j, k)] = x[g.at(i, j, k)]; });
+  return outer;
+}
 
 /* ------------------------------------------------------------- Poisson CG */
 template <class T>

@@ -85,6 +85,16 @@ double orc_poisson_residual(int ndim, int nx, int ny, int nz, const double *f_in
 int orc_poisson_mixed(int ndim, int nx, int ny, int nz, const double *f_int, double tol, int max_outer,
                       int inner_iters, double inner_tol, int nthreads, double *x_int, double *relres_hist,
                       int *inner_total);
+/* ---- FP64 refinement around FP32 CG for the 3D heat problem (the batched
+ *      engine's FP32-inner heat mode; no reference counterpart): x0 = rhs
+ *      (TeaLeaf's start), f = rhs, FP64 residual, stop at |r| <= tol |r_0|;
+ *      inner unpreconditioned CG in FP32 with float face coefficients, FP64
+ *      dots, inner stop at max(inner_tol, tol/(2 relres)) |rr_0|.  Pinned:
+ *      converged solution vs orc_heat_cg's and the FP64 residual recomputed
+ *      independently in tests.  Returns outer iterations. ---- */
+int orc_heat_mixed(int nx, int ny, int nz, const double *K_ext, const double *rhs_int, double tol, int max_outer,
+                   int inner_iters, double inner_tol, int nthreads, double *x_int, double *relres_hist,
+                   int *inner_total);
 
 /* ---- ADI preproc (apps/c/adi/preproc_kernel.h:4-39, lambda) — pinned:
  *      fixture.  Outputs interior-sized. ---- */

@@ -52,6 +52,7 @@ def lib():
             "orc_poisson_apply_f32": (cd, [ci, ci, ci, ci, _f, _f]),
             "orc_poisson_residual": (cd, [ci, ci, ci, ci, _d, _d, _d, ci]),
             "orc_poisson_mixed": (ci, [ci, ci, ci, ci, _d, cd, ci, ci, cd, ci, _d, _d, _i]),
+            "orc_heat_mixed": (ci, [ci, ci, ci, _d, _d, cd, ci, ci, cd, ci, _d, _d, _i]),
             "orc_adi_preproc": (None, [ci, ci, ci, cd, _d, _d, _d, _d, _d]),
             "orc_stencil27": (None, [ci, ci, ci, _d, _d]),
             "orc_mb_poisson": (cd, [ci, ci, ci, _d]),
@@ -207,6 +208,16 @@ def poisson_mixed(ndim, nx, ny, nz, f, tol=1e-10, max_outer=30, inner_iters=1000
     tot = np.zeros(1, np.int32)
     n = lib().orc_poisson_mixed(ndim, nx, ny, nz, P(np.ascontiguousarray(f)), tol, max_outer, inner_iters,
                                 inner_tol, nthreads, P(x), P(rel), P(tot))
+    return dict(outer=n, x=x, relres=rel[: n + 1], inner_total=int(tot[0]))
+
+
+def heat_mixed(nx, ny, nz, K_ext, rhs, tol=1e-10, max_outer=30, inner_iters=100000, inner_tol=1e-6, nthreads=1):
+    """3D heat problem by FP64 refinement around FP32 CG (orc_heat_mixed)."""
+    x = np.zeros(int_shape(3, nx, ny, nz))
+    rel = np.zeros(max_outer + 1)
+    tot = np.zeros(1, np.int32)
+    n = lib().orc_heat_mixed(nx, ny, nz, P(np.ascontiguousarray(K_ext)), P(np.ascontiguousarray(rhs)), tol,
+                             max_outer, inner_iters, inner_tol, nthreads, P(x), P(rel), P(tot))
     return dict(outer=n, x=x, relres=rel[: n + 1], inner_total=int(tot[0]))
 
 

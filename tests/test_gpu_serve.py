@@ -129,9 +129,57 @@ def test_mixed_poisson_matches_oracle(serve, n):
     assert np.sqrt(done["rr"]) <= 1e-10 * np.sqrt(done["rr0"])
 
 
+@pytest.mark.parametrize("n", [24, 45])
+def test_mixed_heat_matches_oracle(serve, n):
+    # FP32-inner refinement of the 3D heat problem in the batched engine
+    # against the oracle's (orc_heat_mixed): same corrections, inner
+    # iterations within a few (the device generates the conductivity with
+    # its own exp/log/cos, last-bit differences from the host's), solution
+    # within 1e-9 of the largest value, and the FP64 residual of the GPU's
+    # solution, recomputed on the host, at the target
+    S = serve
+    srv = S.Server(budget_bytes=1 << 28, max_slots=4, iters_per_launch=8)
+    i = srv.submit(S.HEAT3D, n, seed=SEED, mixed=True)
+    done, x = srv.run()[i]
+    srv.close()
+    K, rhs = O.heat_inputs(SEED, 3, n, n, n)
+    ref = O.heat_mixed(n, n, n, K, rhs, tol=1e-10, inner_tol=1e-6, nthreads=8)
+    assert done["status"] == 1
+    assert done["outer"] == ref["outer"]
+    assert abs(done["iters"] - ref["inner_total"]) <= 2 * ref["outer"]
+    assert np.max(np.abs(x - ref["x"])) <= 1e-9 * np.max(np.abs(ref["x"]))
+    Kx, Ky, Kz = O.heat_coefs(3, n, n, n, K)
+    x_ext = np.zeros((n + 2, n + 2, n + 2))
+    x_ext[1:-1, 1:-1, 1:-1] = x
+    u0 = np.zeros_like(x_ext)
+    u0[1:-1, 1:-1, 1:-1] = rhs
+    ax, _ = O.heat_apply(3, n, n, n, Kx, Ky, Kz, x_ext)
+    a0, _ = O.heat_apply(3, n, n, n, Kx, Ky, Kz, u0)
+    r = rhs - ax[1:-1, 1:-1, 1:-1]
+    r0 = rhs - a0[1:-1, 1:-1, 1:-1]
+    assert np.linalg.norm(r) <= 1.05e-10 * np.linalg.norm(r0)
+    # the engine's own FP64 residual at the end
+    assert np.sqrt(done["rr"]) <= 1e-10 * np.sqrt(done["rr0"])
+
+
+def test_mixed_heat_takes_fp64_solution(serve):
+    # FP64 CG and FP32-inner refinement of the same heat problem reach the
+    # same solution (both at 1e-10 relative residual)
+    S = serve
+    srv = S.Server(budget_bytes=1 << 28, max_slots=4, iters_per_launch=8)
+    a = srv.submit(S.HEAT3D, 30, seed=SEED + 1)
+    b = srv.submit(S.HEAT3D, 30, seed=SEED + 1, mixed=True)
+    out = srv.run()
+    srv.close()
+    x64, xm = out[a][1], out[b][1]
+    assert out[a][0]["status"] == 1 and out[b][0]["status"] == 1
+    assert np.max(np.abs(xm - x64)) <= 1e-8 * np.max(np.abs(x64))
+
+
 def test_mixed_and_fp64_in_one_batch(serve):
     S = serve
-    reqs = [(0, 70, 31, True), (1, 21, 32, False), (0, 130, 33, True), (0, 90, 34, False)]
+    reqs = [(0, 70, 31, True), (1, 21, 32, False), (0, 130, 33, True), (0, 90, 34, False), (1, 26, 35, True),
+            (1, 19, 36, True)]
     srv = S.Server(budget_bytes=1 << 28, max_slots=8, iters_per_launch=8)
     ids = [srv.submit(k, n, seed=s, mixed=m) for k, n, s, m in reqs]
     out = srv.run()
@@ -177,7 +225,7 @@ def test_max_iters_and_errors(serve):
     with pytest.raises(ops.OPSError):
         srv.submit(7, 10)
     with pytest.raises(ops.OPSError):
-        srv.submit(S.HEAT3D, 10, mixed=True)  # FP32-inner mode is for 2D Poisson
+        srv.submit(S.HEAT3D, 10, mixed=2)  # mixed is 0 or 1
     with pytest.raises(ops.OPSError):
         srv.submit(S.HEAT3D, 600)  # larger than the budget
     with pytest.raises(ops.OPSError):

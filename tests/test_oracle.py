@@ -301,6 +301,27 @@ def test_mixed_precision_reaches_fp64_solution():
     assert np.sqrt(rr) / np.linalg.norm(f) == pytest.approx(mp["relres"][-1], rel=1e-6)
 
 
+def test_mixed_heat_reaches_fp64_solution():
+    # FP64 refinement around FP32 CG on the 3D heat problem: reaches the
+    # FP64 CG solution, and its reported residual is the FP64 residual of x
+    # recomputed here (relative to the residual of TeaLeaf's start x = rhs)
+    nx, ny, nz = 22, 19, 17
+    K, rhs = O.heat_inputs(9, 3, nx, ny, nz)
+    mp = O.heat_mixed(nx, ny, nz, K, rhs, tol=1e-10, inner_tol=1e-6, nthreads=4)
+    assert mp["relres"][-1] <= 1e-10 and mp["outer"] >= 2  # FP32 inner solves cannot reach 1e-10 at once
+    cg = O.heat_cg(3, nx, ny, nz, K, rhs, max_iters=5000, tol=1e-13, nthreads=4)
+    assert np.linalg.norm(mp["x"] - cg["u"]) / np.linalg.norm(cg["u"]) < 1e-8
+    Kx, Ky, Kz = O.heat_coefs(3, nx, ny, nz, K)
+    x = np.zeros(O.ext_shape(3, nx, ny, nz))
+    u0 = np.zeros_like(x)
+    O.interior(x)[:] = mp["x"]
+    O.interior(u0)[:] = rhs
+    ax, _ = O.heat_apply(3, nx, ny, nz, Kx, Ky, Kz, x)
+    a0, _ = O.heat_apply(3, nx, ny, nz, Kx, Ky, Kz, u0)
+    rel = np.linalg.norm(rhs - O.interior(ax)) / np.linalg.norm(rhs - O.interior(a0))
+    assert rel == pytest.approx(mp["relres"][-1], rel=1e-6)
+
+
 def test_mb_poisson_matches_reference_app(golden):
     # the reference's multi-block Poisson app, built from its sources and run
     # unchanged (its printed total error has 15 significant digits)
