@@ -100,8 +100,8 @@ def test_laplace_reference_known_answer(apps, golden):
 def test_heat_cg_fixture_trajectory(apps, golden, ndim, key):
     # CG amplifies last-bit differences of the dot products (their order
     # differs from the serial reference), so whole trajectories are compared
-    # only over the first iterations; per-loop parity at every iteration is
-    # test_heat_cg_lockstep
+    # only over the first iterations (measured: at most 3.2e-14 relative over
+    # these 10); per-loop parity at every iteration is test_heat_cg_lockstep
     g, m = golden
     c = m[key]
     nz = c.get("nz", 1)
@@ -111,7 +111,7 @@ def test_heat_cg_fixture_trajectory(apps, golden, ndim, key):
     k, hist = h.iterate(10, history=True)
     h.close()
     for name in ("pw", "alpha", "beta"):
-        assert relerr(hist[name], g[f"{key}_{name}"][:10]) < 1e-11, name
+        assert relerr(hist[name], g[f"{key}_{name}"][:10]) < RTOL_RED, name
 
 
 @pytest.mark.parametrize("ndim,n", [(2, (40, 34, 1)), (3, (26, 22, 18))])
@@ -203,7 +203,7 @@ def test_poisson_cg_fixture(apps, golden, ndim, key):
     # trajectory stays within the reduction tolerance
     for name in ("alpha", "beta"):
         assert relerr(hist[name], g[f"{key}_{name}"]) < RTOL_RED, name
-    assert relerr(hist["rrn"], g[key + "_rrn"][1:]) < 1e-10
+    assert relerr(hist["rrn"], g[key + "_rrn"][1:]) < RTOL_RED  # measured: at most 3.2e-14
     assert np.linalg.norm(x.ravel() - g[key + "_x"]) / np.linalg.norm(g[key + "_x"]) < 1e-12
 
 
