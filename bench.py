@@ -157,7 +157,19 @@ def bench_heat(a, ops, apps):
                {"workload": f"heat3d_cg_{n}", "grid": [n, n, n], "operator": "7-point variable coefficient",
                 "step": "one CG iteration (4 ops_par_loops, 2 FP64 reductions)", "bytes_per_point": 112,
                 "relres_after_steps": rr_rel},
-               roofline=rl, kernels=kernels)
+               roofline=rl, kernels=kernels,
+               value_basis="OPS ops_compute_transfer rule: 112 B/pt for the four loops as written, "
+                           "also when the queue fuses them (96 B/pt compulsory after fusion)")
+    # HBM traffic actually moved per step, from the PMC profile of the two
+    # kernels that make up the step (reduction folds add under 0.1 %)
+    if n == 512:
+        t_chain = profiled_traffic("heat3d_cg_512_hbm_traffic.md", "cg_chain(axpy+axpby+calc_w)")
+        t_urr = profiled_traffic("heat3d_cg_512_hbm_traffic.md", "tea_leaf_cg_calc_ur_r_reduce_kernel")
+        if t_chain and t_urr:
+            out["hbm_traffic_per_step"] = {"bytes": t_chain + t_urr,
+                                           "GB_s": round((t_chain + t_urr) / (ms * 1e-3) / 1e9, 1),
+                                           "frac_of_peak": round((t_chain + t_urr) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                           "source": "profiles/heat3d_cg_512_hbm_traffic.md (PMC)"}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_heat(n, a.seed)
     if not a.no_configs:
@@ -242,18 +254,23 @@ def bench_mixed(a, ops, apps):
 def measure_laplace(ops, apps, n, iters=1000):
     """2048^2 Jacobi, 1000 iterations, each with its max-abs-diff result
     (the tutorial's loop); per-iteration time from the difference of a
-    (10 + iters)- and a 10-iteration run, so set-up cancels."""
-    ops.ops_mi_set_diags(2)
+    (10 + iters)- and a 10-iteration run, so set-up cancels.  The timed runs
+    record no timing events (each costs about 1.8 us of a 27 us iteration);
+    the per-kernel table comes from a third run with events."""
+    ops.ops_mi_set_diags(1)
     apps.laplace2d(n, n, 10, want_field=False)
     ops.ops_mi_sync()
     t0 = time.perf_counter()
     apps.laplace2d(n, n, 10, want_field=False)
     t1 = time.perf_counter()
-    ops.ops_mi_reset_stats()
     t2 = time.perf_counter()
     err, _, _ = apps.laplace2d(n, n, 10 + iters, want_field=False)
     t3 = time.perf_counter()
     ms = ((t3 - t2) - (t1 - t0)) / iters * 1e3
+    ops.ops_mi_set_diags(2)
+    ops.ops_mi_reset_stats()
+    apps.laplace2d(n, n, 10 + iters, want_field=False)
+    ops.ops_mi_sync()
     pts = float(n) ** 2
     # the queue runs each copy with the next apply_stencil as one pass
     # (reads Anew, writes A and the new Anew: 24 B/pt)
