@@ -372,7 +372,7 @@ def measure_serve(ops, apps, nreq, seed, frac2d=0.6):
     lat = done2 - arr
     print(f"serve: batched, Poisson arrivals at {rate:.1f}/s: span {span2:.2f} s", file=sys.stderr, flush=True)
     # per-solve FP64 path: the library apps, one solve after another
-    t_base, it_base = 0.0, []
+    t_base, it_base, t_each = 0.0, [], []
     for j, (k, n, s) in enumerate(zip(kinds, sizes, seeds)):
         n = int(n)
         P = apps.Poisson(2, n, n, 1, seed=int(s)) if k == 0 else apps.Heat(3, n, n, n, seed=int(s))
@@ -380,7 +380,8 @@ def measure_serve(ops, apps, nreq, seed, frac2d=0.6):
         t0 = time.perf_counter()
         it_base.append(P.iterate(1000000, tol=tol))
         ops.ops_mi_sync()
-        t_base += time.perf_counter() - t0
+        t_each.append(time.perf_counter() - t0)
+        t_base += t_each[-1]
         P.close()
         if j % 20 == 19:
             print(f"serve: per-solve path: {j + 1} solves, {t_base:.2f} s", file=sys.stderr, flush=True)
@@ -399,7 +400,14 @@ def measure_serve(ops, apps, nreq, seed, frac2d=0.6):
             "speedup_batched_fp64_vs_per_solve": round(sat / base, 3),
             "speedup_batched_fp32_inner_vs_per_solve": round(sat_m / base, 3),
             "iterations": {"batched_total": int(iters.sum()), "batched_fp32_inner_total": int(iters_m.sum()),
-                           "per_solve_total": int(sum(it_base))}}
+                           "per_solve_total": int(sum(it_base))},
+            # the requests behind the tail latency: kind, side, CG iterations,
+            # latency in the arrival stream and the same solve's time alone on
+            # the per-solve path
+            "slowest_requests": [{"kind": "poisson2d" if kinds[i] == 0 else "heat3d", "side": int(sizes[i]),
+                                  "iterations": int(iters2[i]), "latency_s": round(float(lat[i]), 4),
+                                  "alone_per_solve_s": round(t_each[i], 4)}
+                                 for i in np.argsort(lat)[::-1][:3]]}
 
 
 def bench_serve(a, ops, apps):

@@ -48,6 +48,7 @@ struct SolveDev {
   int kind, nx, ny, nz;
   int tx, ty;   // tiles along x and y
   int chunk;    // 3D: z planes per tile; 2D: rows per wave
+  int sv;       // points per lane along x (kSV; kSV32 for FP32-inner solves)
   int tile0, ntiles;
   int active;
   double rx, ry, rz;
@@ -78,14 +79,21 @@ struct TileBox {
 };
 
 // Points per lane along x: every field's x = 0 is 256-byte aligned and its
-// row pitch a multiple of 256 bytes, so a lane's kSV points are one 16-byte
-// (FP64) or 8-byte (FP32) load.  A tile is (64 kSV) points wide.
+// row pitch a multiple of 256 bytes, so a lane's points are one aligned
+// load: kSV doubles (16 bytes) for FP64 solves and FP32-inner 3D heat
+// solves (8-byte FP32 loads; their 3D grids, sides 32-256, would leave most
+// lanes of 256-point tiles idle: the wider form measured 0.73 s against
+// 0.59 s on a 30-request heat stream), kSV32 floats (16 bytes) for the FP32
+// fields of an FP32-inner 2D Poisson solve, whose FP64 fields a lane then
+// reads as two 16-byte loads (40-request Poisson stream: 1.20 s against
+// 1.35 s with 8-byte lanes).  A tile is (64 sv) points wide.
 constexpr int kSV = 2;
+constexpr int kSV32 = 4;
 
 __device__ __forceinline__ TileBox tile_box(const SolveDev &d, int li) {
   TileBox b;
   const int tx = li % d.tx, rest = li / d.tx;
-  b.x0 = tx * kWave * kSV;
+  b.x0 = tx * kWave * d.sv;
   if (d.kind == OPSMI_SOLVE_HEAT3D) {
     b.y0 = (rest % d.ty) * kWavesPerBlock;
     b.z0 = (rest / d.ty) * d.chunk;
@@ -98,20 +106,20 @@ __device__ __forceinline__ TileBox tile_box(const SolveDev &d, int li) {
   return b;
 }
 
-// f(x, y, z, n) over this thread's groups of the tile: kSV consecutive points
-// from x, n (1..kSV) of them inside the block
+// f(x, y, z, n) over this thread's groups of the tile: V consecutive points
+// from x (V = the solve's points per lane), n (1..V) of them inside the block
 //
-// A lane whose group is whole (n == kSV: every lane but those at a solve's
-// right edge) calls f with the literal kSV, so the element-count branches in
+// A lane whose group is whole (n == V: every lane but those at a solve's
+// right edge) calls f with the literal V, so the element-count branches in
 // f's vector loads and stores fold away: a branch join makes the compiler
 // wait for every load in flight, which would serialise a group's loads.
-template <class F>
+template <int V, class F>
 __device__ __forceinline__ void tile_groups(const SolveDev &d, const TileBox &b, F f) {
-  const int x = b.x0 + kSV * (int)threadIdx.x;
-  const int n = min(kSV, d.nx - x);
+  const int x = b.x0 + V * (int)threadIdx.x;
+  const int n = min(V, d.nx - x);
   if (n <= 0) return;
   auto g = [&](int y, int z) {
-    if (n == kSV) f(x, y, z, kSV);
+    if (n == V) f(x, y, z, V);
     else f(x, y, z, n);
   };
   if (d.kind == OPSMI_SOLVE_HEAT3D) {
@@ -178,7 +186,8 @@ struct HeatOutResid {
   }
 };
 
-// LDS of the calc_w pass: the FP64 heat column or its FP32 form
+// LDS of the calc_w pass: the FP64 heat column (2 doubles per lane) or its
+// FP32 form (4 floats per lane); both are 31 KB
 union CalcWLds {
   HeatLdsV<kSV, kWavesPerBlock> d;
   HeatLdsV<kSV, kWavesPerBlock, float> f;
@@ -186,24 +195,24 @@ union CalcWLds {
 
 // the 2D 5-point operator over a group: the group's row and the rows above
 // and below, and the neighbours just left and right of the group
-template <class T>
+template <class T, int V>
 struct Row5 {
-  T c[kSV], up[kSV], dn[kSV];
+  T c[V], up[V], dn[V];
   T l, r;
 };
-template <class T>
-__device__ __forceinline__ void load5(const Ptr<const T> &p, int x, int y, int n, Row5<T> &R) {
-  ldv<T, kSV>(p.at(x, y, 0), n, R.c);
-  ldv<T, kSV>(p.at(x, y - 1, 0), n, R.up);
-  ldv<T, kSV>(p.at(x, y + 1, 0), n, R.dn);
+template <class T, int V>
+__device__ __forceinline__ void load5(const Ptr<const T> &p, int x, int y, int n, Row5<T, V> &R) {
+  ldv<T, V>(p.at(x, y, 0), n, R.c);
+  ldv<T, V>(p.at(x, y - 1, 0), n, R.up);
+  ldv<T, V>(p.at(x, y + 1, 0), n, R.dn);
   R.l = p(x - 1, y, 0);
   R.r = p(x + n, y, 0);
 }
 // 4 p - (p(x-1) + p(x+1)) - (p(y-1) + p(y+1)) at point j of the group
-template <class T>
-__device__ __forceinline__ T op5(const Row5<T> &R, int j, int n) {
+template <class T, int V>
+__device__ __forceinline__ T op5(const Row5<T, V> &R, int j, int n) {
   const T xm = j == 0 ? R.l : R.c[j > 0 ? j - 1 : 0];
-  const T xp = j + 1 < n ? R.c[j + 1 < kSV ? j + 1 : j] : R.r;
+  const T xp = j + 1 < n ? R.c[j + 1 < V ? j + 1 : j] : R.r;
   return (T)4.0 * R.c[j] - (xm + xp) - (R.up[j] + R.dn[j]);
 }
 
@@ -227,10 +236,16 @@ __global__ __launch_bounds__(256) void sk_calc_w(const SolveDev *S, const SolveS
         // previous iteration's last pass) and the inner start
         const Ptr<const double> f = PC(d.f, d);
         const Ptr<float> rr = PF32(d.r32, d), ev = PF32(d.e32, d), p = PF32(d.p32, d);
+        // (the FP64 column covers 64 kSV points; a wider tile in parts)
         HeatOutResid out{f, rr, ev, p};
-        heat3_column_out<double, kSV, kWavesPerBlock, 1>(lds.d, out, PC(d.kx, d), PC(d.ky, d), PC(d.kz, d),
-                                                         PC(d.u, d), d.rx, d.ry, d.rz, b.x0, b.y0, d.nx, d.ny, b.z0,
-                                                         b.z1);
+        for (int h = 0; h < d.sv / kSV; h++) {
+          const int x0 = b.x0 + h * kWave * kSV;
+          if (x0 >= d.nx) break;  // (uniform over the workgroup)
+          if (h > 0) __syncthreads();  // LDS of the previous half fully read
+          heat3_column_out<double, kSV, kWavesPerBlock, 1>(lds.d, out, PC(d.kx, d), PC(d.ky, d), PC(d.kz, d),
+                                                           PC(d.u, d), d.rx, d.ry, d.rz, x0, b.y0, d.nx, d.ny, b.z0,
+                                                           b.z1);
+        }
         store_partials<3>(out.acc, part + (size_t)t * kPart, sh3);
         continue;
       }
@@ -249,30 +264,30 @@ __global__ __launch_bounds__(256) void sk_calc_w(const SolveDev *S, const SolveS
         // in sk_update once every tile has applied it)
         const Ptr<double> x = P(d.u, d);
         const Ptr<const float> ev = PF32C(d.e32, d);
-        tile_groups(d, b, [&](int i, int j, int k, int n) {
-          double xv[kSV];
-          float evv[kSV];
-          ldv<double, kSV>(x.at(i, j, k), n, xv);
-          ldv<float, kSV>(ev.at(i, j, k), n, evv);
+        tile_groups<kSV32>(d, b, [&](int i, int j, int k, int n) {
+          double xv[kSV32];
+          float evv[kSV32];
+          ldv<double, kSV32>(x.at(i, j, k), n, xv);
+          ldv<float, kSV32>(ev.at(i, j, k), n, evv);
 #pragma unroll
-          for (int q = 0; q < kSV; q++) xv[q] = xv[q] + (double)evv[q];
-          stv<double, kSV, false>(x.at(i, j, k), n, xv);
+          for (int q = 0; q < kSV32; q++) xv[q] = xv[q] + (double)evv[q];
+          stv<double, kSV32, false>(x.at(i, j, k), n, xv);
         });
         continue;
       }
       // inner sweep in FP32: w = 4p - (p(-1,0) + p(1,0)) - (p(0,-1) + p(0,1)), p.w in FP64
       const Ptr<const float> p = PF32C(d.p32, d);
       const Ptr<float> w = PF32(d.w32, d);
-      tile_groups(d, b, [&](int i, int j, int k, int n) {
-        Row5<float> R;
-        load5<float>(p, i, j, n, R);
-        float v[kSV];
+      tile_groups<kSV32>(d, b, [&](int i, int j, int k, int n) {
+        Row5<float, kSV32> R;
+        load5<float, kSV32>(p, i, j, n, R);
+        float v[kSV32];
 #pragma unroll
-        for (int q = 0; q < kSV; q++) {
-          v[q] = op5<float>(R, q, n);
+        for (int q = 0; q < kSV32; q++) {
+          v[q] = op5<float, kSV32>(R, q, n);
           if (q < n) acc = acc + (double)v[q] * (double)R.c[q];
         }
-        stv<float, kSV, false>(w.at(i, j, k), n, v);
+        stv<float, kSV32, false>(w.at(i, j, k), n, v);
       });
       store_partial(acc, part + (size_t)t * kPart, sh);
       continue;
@@ -286,13 +301,13 @@ __global__ __launch_bounds__(256) void sk_calc_w(const SolveDev *S, const SolveS
     } else {
       const Ptr<const double> p = PC(d.p, d);
       const Ptr<double> w = P(d.w, d);
-      tile_groups(d, b, [&](int x, int y, int z, int n) {
-        Row5<double> R;
-        load5<double>(p, x, y, n, R);
+      tile_groups<kSV>(d, b, [&](int x, int y, int z, int n) {
+        Row5<double, kSV> R;
+        load5<double, kSV>(p, x, y, n, R);
         double v[kSV];
 #pragma unroll
         for (int q = 0; q < kSV; q++) {
-          v[q] = op5<double>(R, q, n);
+          v[q] = op5<double, kSV>(R, q, n);
           if (q < n) acc = acc + v[q] * R.c[q];
         }
         stv<double, kSV, false>(w.at(x, y, z), n, v);
@@ -322,38 +337,48 @@ __global__ __launch_bounds__(256) void sk_update(const SolveDev *S, const SolveS
       if (st[e.x].phase == PH_INNER) {
         // e += a p ; rr -= a w ; (rr . Mi rr, rr . rr) in FP64
         const float a = (float)st[e.x].alpha;
-        tile_groups(d, tile_box(d, e.y), [&](int i, int j, int k, int n) {
-          float evv[kSV], rv[kSV], pv[kSV], wv[kSV];
-          ldv<float, kSV>(ev.at(i, j, k), n, evv);
-          ldv<float, kSV>(rr.at(i, j, k), n, rv);
-          ldv<float, kSV>(p.at(i, j, k), n, pv);
-          ldv<float, kSV>(w.at(i, j, k), n, wv);
+        // V: the solve's points per lane (kSV32 for 2D Poisson, kSV for 3D heat)
+        auto inner = [&](auto vt) {
+          constexpr int V = decltype(vt)::value;
+          tile_groups<V>(d, tile_box(d, e.y), [&](int i, int j, int k, int n) {
+            float evv[V], rv[V], pv[V], wv[V];
+            ldv<float, V>(ev.at(i, j, k), n, evv);
+            ldv<float, V>(rr.at(i, j, k), n, rv);
+            ldv<float, V>(p.at(i, j, k), n, pv);
+            ldv<float, V>(w.at(i, j, k), n, wv);
 #pragma unroll
-          for (int q = 0; q < kSV; q++) {
-            evv[q] = evv[q] + a * pv[q];
-            rv[q] = rv[q] - a * wv[q];
-            const float z = mi * rv[q];
-            if (q < n) {
-              acc[0] = acc[0] + (double)rv[q] * (double)z;
-              acc[1] = acc[1] + (double)rv[q] * (double)rv[q];
+            for (int q = 0; q < V; q++) {
+              evv[q] = evv[q] + a * pv[q];
+              rv[q] = rv[q] - a * wv[q];
+              const float z = mi * rv[q];
+              if (q < n) {
+                acc[0] = acc[0] + (double)rv[q] * (double)z;
+                acc[1] = acc[1] + (double)rv[q] * (double)rv[q];
+              }
             }
-          }
-          stv<float, kSV, false>(ev.at(i, j, k), n, evv);
-          stv<float, kSV, false>(rr.at(i, j, k), n, rv);
-        });
+            stv<float, V, false>(ev.at(i, j, k), n, evv);
+            stv<float, V, false>(rr.at(i, j, k), n, rv);
+          });
+        };
+        if (d.sv == kSV32) inner(std::integral_constant<int, kSV32>{});
+        else inner(std::integral_constant<int, kSV>{});
       } else {
         // FP64 residual r = f - A x, then the inner start from it:
-        // rr = (float) r, e = 0, p = Mi rr; (r.r, rr . Mi rr, rr . rr)
+        // rr = (float) r, e = 0, p = Mi rr; (r.r, rr . Mi rr, rr . rr).
+        // Runs once per correction: the tile is covered in halves of 64 kSV
+        // points (two doubles per lane, fewer registers for the kernel).
         const Ptr<const double> x = PC(d.u, d), f = PC(d.f, d);
-        tile_groups(d, tile_box(d, e.y), [&](int i, int j, int k, int n) {
-          Row5<double> R;
-          load5<double>(x, i, j, n, R);
+        TileBox bh = tile_box(d, e.y);
+        for (int h = 0; h < kSV32 / kSV; h++, bh.x0 += kWave * kSV)
+        tile_groups<kSV>(d, bh, [&](int i, int j, int k, int n) {
+          Row5<double, kSV> R;
+          load5<double, kSV>(x, i, j, n, R);
           double fv[kSV];
           ldv<double, kSV>(f.at(i, j, k), n, fv);
           float q32[kSV], z32[kSV], zero[kSV];
 #pragma unroll
           for (int q = 0; q < kSV; q++) {
-            const double ax = op5<double>(R, q, n);
+            const double ax = op5<double, kSV>(R, q, n);
             const double v = fv[q] - ax;
             const float qv = (float)v;
             const float z = mi * qv;
@@ -378,7 +403,7 @@ __global__ __launch_bounds__(256) void sk_update(const SolveDev *S, const SolveS
     const Ptr<double> u = P(d.u, d), r = P(d.r, d);
     const Ptr<const double> p = PC(d.p, d), w = PC(d.w, d);
     double acc = 0.0;
-    tile_groups(d, tile_box(d, e.y), [&](int x, int y, int z, int n) {
+    tile_groups<kSV>(d, tile_box(d, e.y), [&](int x, int y, int z, int n) {
       double uv[kSV], rv[kSV], pv[kSV], wv[kSV];
       ldv<double, kSV>(u.at(x, y, z), n, uv);
       ldv<double, kSV>(r.at(x, y, z), n, rv);
@@ -412,7 +437,8 @@ __global__ __launch_bounds__(256) void sk_axpby(const SolveDev *S, const SolveSt
       if (ax == 2) {
         const Ptr<double> x = P(d.u, d);
         const Ptr<const float> ev = PF32C(d.e32, d);
-        tile_groups(d, tile_box(d, e.y), [&](int i, int j, int k, int n) {
+        // (3D heat solves only: kSV points per lane)
+        tile_groups<kSV>(d, tile_box(d, e.y), [&](int i, int j, int k, int n) {
           double xv[kSV];
           float evv[kSV];
           ldv<double, kSV>(x.at(i, j, k), n, xv);
@@ -425,20 +451,25 @@ __global__ __launch_bounds__(256) void sk_axpby(const SolveDev *S, const SolveSt
       }
       const float mi = d.kind == OPSMI_SOLVE_HEAT3D ? 1.0f : 0.25f, bf = (float)st[e.x].beta;
       const Ptr<float> p = PF32(d.p32, d), rr = PF32(d.r32, d);
-      tile_groups(d, tile_box(d, e.y), [&](int i, int j, int k, int n) {
-        float pv[kSV], rv[kSV];
-        ldv<float, kSV>(p.at(i, j, k), n, pv);
-        ldv<float, kSV>(rr.at(i, j, k), n, rv);
+      auto pupd = [&](auto vt) {
+        constexpr int V = decltype(vt)::value;
+        tile_groups<V>(d, tile_box(d, e.y), [&](int i, int j, int k, int n) {
+          float pv[V], rv[V];
+          ldv<float, V>(p.at(i, j, k), n, pv);
+          ldv<float, V>(rr.at(i, j, k), n, rv);
 #pragma unroll
-        for (int q = 0; q < kSV; q++) pv[q] = mi * rv[q] + bf * pv[q];
-        stv<float, kSV, false>(p.at(i, j, k), n, pv);
-      });
+          for (int q = 0; q < V; q++) pv[q] = mi * rv[q] + bf * pv[q];
+          stv<float, V, false>(p.at(i, j, k), n, pv);
+        });
+      };
+      if (d.sv == kSV32) pupd(std::integral_constant<int, kSV32>{});
+      else pupd(std::integral_constant<int, kSV>{});
       continue;
     }
     const double beta = st[e.x].beta;
     const Ptr<double> p = P(d.p, d);
     const Ptr<const double> r = PC(d.r, d);
-    tile_groups(d, tile_box(d, e.y), [&](int x, int y, int z, int n) {
+    tile_groups<kSV>(d, tile_box(d, e.y), [&](int x, int y, int z, int n) {
       double pv[kSV], rv[kSV];
       ldv<double, kSV>(p.at(x, y, z), n, pv);
       ldv<double, kSV>(r.at(x, y, z), n, rv);
@@ -597,7 +628,7 @@ __global__ __launch_bounds__(256) void sk_first_residual(SolveDev d, uint64_t se
   const Ptr<double> r = P(d.r, d), p = P(d.p, d);
   const bool poisson = d.kind == OPSMI_SOLVE_POISSON2D;
   double acc = 0.0;
-  tile_groups(d, b, [&](int x, int y, int z, int n) {
+  tile_groups<kSV>(d, b, [&](int x, int y, int z, int n) {
 #pragma unroll
     for (int q = 0; q < kSV; q++) {
       if (q >= n) continue;
@@ -806,7 +837,8 @@ int tile_param(opsmi_server S, int kind, int n, int mixed) {
 
 long tiles_of(const opsmi_solve_req &q, int param, int *tx, int *ty) {
   const Geometry g = geometry(q);
-  *tx = (g.nx + kWave * kSV - 1) / (kWave * kSV);
+  const int w = kWave * (q.mixed && q.kind == OPSMI_SOLVE_POISSON2D ? kSV32 : kSV);  // tile width in points
+  *tx = (g.nx + w - 1) / w;
   if (q.kind == OPSMI_SOLVE_HEAT3D) {
     *ty = (g.ny + kWavesPerBlock - 1) / kWavesPerBlock;
     return (long)*tx * *ty * ((g.nz + param - 1) / param);
@@ -908,6 +940,7 @@ void admit(opsmi_server S) {
     d.tx = tx;
     d.ty = ty;
     d.chunk = param;
+    d.sv = R.q.mixed && R.q.kind == OPSMI_SOLVE_POISSON2D ? kSV32 : kSV;
     d.tile0 = 0;
     d.ntiles = (int)nt;
     d.active = 1;
