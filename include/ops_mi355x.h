@@ -34,6 +34,52 @@
 #include <stddef.h>
 #include <stdio.h>
 
+/* ---- utility macros applications use (ops/c/include/ops_macros.h:43-57) ---- */
+#ifndef MIN
+#define MIN(a, b) ((a < b) ? (a) : (b))
+#endif
+#ifndef MIN3
+#define MIN3(a, b, c) MIN(a, MIN(b, c))
+#endif
+#ifndef MAX
+#define MAX(a, b) ((a > b) ? (a) : (b))
+#endif
+#ifndef MAX3
+#define MAX3(a, b, c) MAX(a, MAX(b, c))
+#endif
+#ifndef SIGN
+#define SIGN(a, b) ((b < 0.0) ? (a * (-1)) : (a))
+#endif
+
+/* The legacy index macros OPS_ACCn / OPS_ACC_MDn (ops/c/include/
+ * ops_macros.h:723-1133) mean something only inside a kernel that the
+ * translator generates a loop from (include/ops_mi355x_device.h defines them
+ * there); a legacy kernel run any other way stops with an error. */
+#include <stdlib.h>
+static inline long ops_mi_legacy_acc_host(int arg) {
+  fprintf(stderr, "OPS error %d: OPS_ACC%d indexing outside a translator-generated loop\n", 1, arg);
+  abort();
+  return 0;
+}
+#ifndef OPSMI_DEVICE_LEGACY_ACC
+#define OPS_ACC0(...) ops_mi_legacy_acc_host(0)
+#define OPS_ACC1(...) ops_mi_legacy_acc_host(1)
+#define OPS_ACC2(...) ops_mi_legacy_acc_host(2)
+#define OPS_ACC3(...) ops_mi_legacy_acc_host(3)
+#define OPS_ACC4(...) ops_mi_legacy_acc_host(4)
+#define OPS_ACC5(...) ops_mi_legacy_acc_host(5)
+#define OPS_ACC6(...) ops_mi_legacy_acc_host(6)
+#define OPS_ACC7(...) ops_mi_legacy_acc_host(7)
+#define OPS_ACC8(...) ops_mi_legacy_acc_host(8)
+#define OPS_ACC9(...) ops_mi_legacy_acc_host(9)
+#define OPS_ACC_MD0(...) ops_mi_legacy_acc_host(0)
+#define OPS_ACC_MD1(...) ops_mi_legacy_acc_host(1)
+#define OPS_ACC_MD2(...) ops_mi_legacy_acc_host(2)
+#define OPS_ACC_MD3(...) ops_mi_legacy_acc_host(3)
+#define OPS_ACC_MD4(...) ops_mi_legacy_acc_host(4)
+#define OPS_ACC_MD5(...) ops_mi_legacy_acc_host(5)
+#endif
+
 /* ---- constants: ops/c/include/ops_lib_core.h:57-92,163-165 ---- */
 #define OPS_MAX_DIM 5
 #define OPS_READ 0
@@ -350,6 +396,32 @@ void ops_tridMultiDimBatch_noalloc(int ndim, int solvedim, int *range, ops_dat a
  * (halos included) and exits with status 2 */
 void ops_NaNcheck(ops_dat dat);
 
+/* ---- utilities applications call: ops/c/include/ops_util.h:41-57 (the
+ * allocators report an OPS error instead of returning NULL) ---- */
+void *xmalloc(size_t size);
+void *xrealloc(void *ptr, size_t size);
+void *xcalloc(size_t number, size_t size);
+int file_exist(const char *filename);
+
+/* ---- checkpointing: ops/c/include/ops_checkpointing.h:59-180.  This
+ * backend has no HDF5, so it behaves as the reference built without
+ * checkpointing (ops/c/src/externlib/ops_checkpointing.cpp:1734-1779):
+ * ops_checkpointing_init returns false (no restart), the rest do nothing
+ * and return false. ---- */
+#define OPS_CHECKPOINT_INITPHASE 1
+#define OPS_CHECKPOINT_MANUAL_DATLIST 2
+#define OPS_CHECKPOINT_FASTFW 4
+#define OPS_CHECKPOINT_MANUAL 8
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+bool ops_checkpointing_init(const char *filename, double interval, int options);
+void ops_checkpointing_initphase_done(void);
+void ops_checkpointing_manual_datlist(int ndats, ops_dat *datlist);
+bool ops_checkpointing_fastfw(int nbytes, char *payload);
+bool ops_checkpointing_manual_datlist_fastfw(int ndats, ops_dat *datlist, int nbytes, char *payload);
+bool ops_checkpointing_manual_datlist_fastfw_trigger(int ndats, ops_dat *datlist, int nbytes, char *payload);
+
 /* ---- HDF5 output: ops/c/include/ops_hdf5.h:130-166.  This backend has no
  * HDF5 library; each call reports OPS_NOT_IMPLEMENTED (error modes below) ---- */
 void ops_fetch_block_hdf5_file(ops_block block, const char *file_name);
@@ -598,16 +670,13 @@ class ops_tridsolver_params {
 };
 /* the reference's C++ signatures over the single instance
  * (ops/c/include/ops_lib_core.h: ops_execute(OPS_instance *),
- * ops_timing_output(std::ostream &)); checkpointing is not part of this
- * backend, so ops_checkpointing_init returns false, as the reference does
- * when checkpointing is not enabled */
+ * ops_timing_output(std::ostream &)) */
 #include <iostream>
 static inline void ops_execute(void *instance) {
   (void)instance;
   ops_execute();
 }
 static inline void ops_timing_output(std::ostream &) { ops_timing_output_stdout(); }
-static inline bool ops_checkpointing_init(const char *, double, int) { return false; }
 
 /* ---- the reference's C++ object API (OPS_CPP_API: ops/c/include/
  *      ops_instance.h:62-380, ops/c/include/ops_lib_core.h:183-552,

@@ -23,6 +23,9 @@
 
 #include <type_traits>
 
+// the legacy index macros are this header's (below), not the host
+// fallbacks of ops_mi355x.h
+#define OPSMI_DEVICE_LEGACY_ACC
 #include "ops_mi355x_gen.h"
 
 namespace opsmi_gen {
@@ -47,6 +50,20 @@ inline Loop loop_of(const opsmi_gen_launch *L) {
   l.three_d = L->three_d;
   return l;
 }
+
+// Legacy kernels (the OPS_ACC macro API, ops/c/include/ops_macros.h:
+// 723-1133, 1345-1656) take T * parameters and index them with
+// OPS_ACCn(dx[, dy[, dz]]) or OPS_ACC_MDn(d, dx[, dy[, dz]]), where n is the
+// argument's position and the macro turns the offsets into an element index
+// with that argument's sizes.  Here the parameters are accessors and the
+// macros build the offsets themselves, which each accessor resolves with its
+// own strides (operator[] below), so n plays no part.
+struct Off {
+  int x = 0, y = 0, z = 0;
+};
+struct OffMD {
+  int d = 0, x = 0, y = 0, z = 0;
+};
 
 // NDIM: block dimensions; MULTI: the dat may have more than one component
 // (otherwise the component count is the constant 1); GEN: the x step comes
@@ -84,6 +101,11 @@ class Acc {
     const int v[n + 3] = {int(i)..., 0, 0, 0};
     if constexpr (n == NDIM) return p_[pt(v[0], v[1], v[2])];
     else return p_[(MULTI ? (long)v[0] * cs_ : (long)v[0]) + pt(v[1], v[2], v[3])];
+  }
+  // legacy indexing: p[OPS_ACCn(dx, dy, dz)], p[OPS_ACC_MDn(d, dx, dy, dz)]
+  __device__ __forceinline__ T &operator[](Off o) const { return p_[pt(o.x, o.y, o.z)]; }
+  __device__ __forceinline__ T &operator[](OffMD o) const {
+    return p_[(MULTI ? (long)o.d * cs_ : (long)o.d) + pt(o.x, o.y, o.z)];
   }
   __device__ void combine_max(int dx, T val) const { amax(&p_[pt(dx, 0, 0)], val); }
   __device__ void combine_max(int dx, int dy, T val) const { amax(&p_[pt(dx, dy, 0)], val); }
@@ -142,6 +164,9 @@ class Acc {
 template <class T, int NDIM, int C>
 struct Cap {
   mutable T v[C];
+  // legacy (OPS_ACCn) kernels: the point itself, or component o.d of it
+  __device__ __forceinline__ T &operator[](Off) const { return v[0]; }
+  __device__ __forceinline__ T &operator[](OffMD o) const { return v[C == 1 ? 0 : o.d]; }
   template <class... I>
   __device__ __forceinline__ T &operator()(I... i) const {
     constexpr int n = sizeof...(I);
@@ -251,3 +276,69 @@ __device__ __forceinline__ void for_each_point(const Loop &L, F &&f) {
 }
 
 }  // namespace opsmi_gen
+
+// the legacy index macros (argument positions 0-31): offsets for the accessors
+#define OPS_ACC0(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD0(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC1(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD1(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC2(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD2(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC3(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD3(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC4(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD4(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC5(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD5(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC6(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD6(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC7(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD7(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC8(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD8(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC9(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD9(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC10(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD10(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC11(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD11(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC12(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD12(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC13(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD13(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC14(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD14(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC15(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD15(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC16(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD16(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC17(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD17(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC18(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD18(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC19(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD19(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC20(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD20(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC21(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD21(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC22(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD22(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC23(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD23(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC24(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD24(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC25(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD25(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC26(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD26(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC27(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD27(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC28(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD28(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC29(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD29(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC30(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD30(...) (::opsmi_gen::OffMD{__VA_ARGS__})
+#define OPS_ACC31(...) (::opsmi_gen::Off{__VA_ARGS__})
+#define OPS_ACC_MD31(...) (::opsmi_gen::OffMD{__VA_ARGS__})

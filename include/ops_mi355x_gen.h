@@ -88,6 +88,16 @@ void ops_mi_gen_par_loop(const char *name, ops_block block, int dim, const int *
  * ops_decl_const) into dst; an OPS error if it is missing or its size is
  * not dim values of `type`. */
 void ops_mi_gen_const(const char *name, const char *type, int dim, void *dst);
+/* The same for a constant of any type (user structs included) or of a count
+ * that is not a literal: elem_size is sizeof the type; with exact != 0 the
+ * constant must hold max_dim values, otherwise at most max_dim (the kernel's
+ * copy holds OPSMI_CONST_CAP(T) of them, OPSMI_CONST_BYTES bytes); an OPS
+ * error if the type name differs from the declaration's or the count does
+ * not fit. */
+void ops_mi_gen_const_sized(const char *name, const char *type, size_t elem_size, int max_dim, int exact,
+                            void *dst);
+#define OPSMI_CONST_BYTES 2048
+#define OPSMI_CONST_CAP(T) ((OPSMI_CONST_BYTES + sizeof(T) - 1) / sizeof(T))
 
 /* launches one kernel that runs, at each point, the first loop's kernel and
  * then the second's (L1, L2 share range and tiling) */

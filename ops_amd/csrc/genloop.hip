@@ -487,4 +487,20 @@ void ops_mi_gen_const(const char *name, const char *type, int dim, void *dst) {
   OPSMI_API_END()
 }
 
+void ops_mi_gen_const_sized(const char *name, const char *type, size_t elem_size, int max_dim, int exact,
+                            void *dst) {
+  OPSMI_API_BEGIN
+  require_init();
+  if (!name || !type || !dst || elem_size == 0 || max_dim < 1)
+    fail(OPS_INVALID_ARGUMENT, "ops_mi_gen_const_sized: bad arguments");
+  const Const &c = get_const(name);
+  const size_t n = c.bytes.size() / elem_size;
+  if (c.type != type || c.bytes.size() % elem_size != 0 || (exact ? n != (size_t)max_dim : n > (size_t)max_dim))
+    fail(OPS_INVALID_ARGUMENT, cat("constant ", name, " was declared as ", c.bytes.size(), " bytes of ", c.type,
+                                   "; the kernel holds ", exact ? "exactly " : "at most ", max_dim, " ", type,
+                                   " of ", elem_size, " bytes"));
+  memcpy(dst, c.bytes.data(), c.bytes.size());
+  OPSMI_API_END()
+}
+
 }  // extern "C"

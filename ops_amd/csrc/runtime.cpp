@@ -381,12 +381,15 @@ Range3 check_gen_loop(const char *name, ops_block block, int dim, const int *ran
           fail(OPS_INVALID_ARGUMENT, cat("loop ", name, ": range [", rl[q], ",", rh[q], ") in dimension ", q,
                                          adv[q] ? "" : " (stride 0)", " with stencil reach [", lo[q], ",", hi[q],
                                          "] leaves dat ", d->name, " (valid indices [", first, ",", last, "))"));
-        // several points on one element: only OPS_INC/MIN/MAX access (the
-        // kernel's combine_* calls, atomic here) may write it
-        if (!adv[q] && rh[q] - rl[q] > 1 && (a.acc == OPS_WRITE || a.acc == OPS_RW))
-          fail(OPS_NOT_IMPLEMENTED, cat("loop ", name, ": dat ", d->name, " is written with ",
-                                        a.acc == OPS_WRITE ? "OPS_WRITE" : "OPS_RW", " through stencil ", s->name,
-                                        ", which maps ", rh[q] - rl[q], " points of dimension ", q,
+        // several points on one element: OPS_INC/MIN/MAX combine into it
+        // (atomics here); OPS_WRITE lets every point store into it, as the
+        // reference's GPU backends do (the reference's apps store one value
+        // from every point, e.g. TeaLeaf's initialise_chunk_kernel_zero_x;
+        // points storing different values leave any one of them); OPS_RW
+        // would be a read-modify-write race and is refused
+        if (!adv[q] && rh[q] - rl[q] > 1 && a.acc == OPS_RW)
+          fail(OPS_NOT_IMPLEMENTED, cat("loop ", name, ": dat ", d->name, " is read and written with OPS_RW through stencil ",
+                                        s->name, ", which maps ", rh[q] - rl[q], " points of dimension ", q,
                                         " to one element (use OPS_INC, OPS_MIN or OPS_MAX)"));
       }
   }
@@ -1448,6 +1451,70 @@ void ops_fetch_halo_hdf5_file(ops_halo halo, const char *file_name) {
 // ops/c/src/core/ops_dummy_singlenode.cpp:871-878, ops/c/src/core/
 // ops_lib_core.cpp:1845-1960: scans the whole dat (halos included) on the
 // host; the first NaN is reported and the process exits with status 2
+// allocation helpers applications use (ops/c/include/ops_util.h:41-45)
+void *xmalloc(size_t size) {
+  void *p = malloc(size);
+  if (!p && size) {
+    OPSMI_API_BEGIN
+    fail(OPS_RUNTIME_ERROR, cat("xmalloc: out of host memory (", size, " bytes)"));
+    OPSMI_API_END(nullptr)
+  }
+  return p;
+}
+void *xrealloc(void *ptr, size_t size) {
+  void *p = realloc(ptr, size);
+  if (!p && size) {
+    OPSMI_API_BEGIN
+    fail(OPS_RUNTIME_ERROR, cat("xrealloc: out of host memory (", size, " bytes)"));
+    OPSMI_API_END(nullptr)
+  }
+  return p;
+}
+void *xcalloc(size_t number, size_t size) {
+  void *p = calloc(number, size);
+  if (!p && number && size) {
+    OPSMI_API_BEGIN
+    fail(OPS_RUNTIME_ERROR, cat("xcalloc: out of host memory (", number, " x ", size, " bytes)"));
+    OPSMI_API_END(nullptr)
+  }
+  return p;
+}
+int file_exist(const char *filename) {
+  FILE *f = filename ? fopen(filename, "r") : nullptr;
+  if (!f) return 0;
+  fclose(f);
+  return 1;
+}
+
+// checkpointing as the reference built without it
+// (ops/c/src/externlib/ops_checkpointing.cpp:1734-1779)
+bool ops_checkpointing_init(const char *filename, double interval, int options) {
+  (void)filename;
+  (void)interval;
+  (void)options;
+  return false;
+}
+void ops_checkpointing_initphase_done(void) {}
+void ops_checkpointing_manual_datlist(int ndats, ops_dat *datlist) {
+  (void)ndats;
+  (void)datlist;
+}
+bool ops_checkpointing_fastfw(int nbytes, char *payload) {
+  (void)nbytes;
+  (void)payload;
+  return false;
+}
+bool ops_checkpointing_manual_datlist_fastfw(int ndats, ops_dat *datlist, int nbytes, char *payload) {
+  (void)ndats;
+  (void)datlist;
+  (void)nbytes;
+  (void)payload;
+  return false;
+}
+bool ops_checkpointing_manual_datlist_fastfw_trigger(int ndats, ops_dat *datlist, int nbytes, char *payload) {
+  return ops_checkpointing_manual_datlist_fastfw(ndats, datlist, nbytes, payload);
+}
+
 void ops_NaNcheck(ops_dat dat) {
   OPSMI_API_BEGIN
   require_init();

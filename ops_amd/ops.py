@@ -448,4 +448,6 @@ def declared_symbols():
     c_part = re.sub(r"/\*.*?\*/", "", c_part, flags=re.S)
     names = re.findall(r"\b([a-z_][a-zA-Z0-9_]*)\s*\(", c_part)
     skip = {"if", "for", "while", "sizeof", "return"}
+    # header-only helpers are not library symbols
+    skip |= set(re.findall(r"static\s+inline\s+[\w\s\*]+?\b([a-z_]\w*)\s*\(", c_part))
     return sorted({n for n in names if (n.startswith("ops_") or n.startswith("opsmi_")) and n not in skip})

@@ -248,7 +248,10 @@ def loop_body_pairs(src, m, loops):
 
 
 def find_consts(src, m):
-    """ops_decl_const("name", dim, "type", ptr) calls: name -> (dim, type)"""
+    """ops_decl_const("name", dim, "type", ptr) calls: name -> (dim, type);
+    dim is None when it is not an integer literal (the kernels then hold up
+    to OPSMI_CONST_BYTES bytes of it), type a basic type or the name of a
+    user type (a struct the application's headers define)."""
     out = {}
     for mo in re.finditer(r"\bops_decl_const\s*(<[^>]*>)?\s*\(", m):
         op = mo.end() - 1
@@ -262,10 +265,86 @@ def find_consts(src, m):
         except TranslateError:
             continue
         d = int_literal(items[1])
-        if d is None or typ not in TYPES:
-            raise TranslateError(f"ops_decl_const({items[0]}): dim must be an integer literal, type double/float/int")
+        if not re.fullmatch(r"[A-Za-z_]\w*", typ) or (d is not None and d < 1):
+            raise TranslateError(f"ops_decl_const({items[0]}): type must be a type name and dim a positive count")
         out[name] = (d, typ)
     return out
+
+
+# (static, const and constexpr variables have internal linkage: no clash)
+GLOBAL_DEF = re.compile(r"^\s*(?!extern\b|typedef\b|using\b|return\b|namespace\b|template\b|static\b|const\b"
+                        r"|constexpr\b)(?:unsigned\s+|signed\s+|long\s+|short\s+)*"
+                        r"[A-Za-z_][\w:]*(?:\s*<[^;()]*>)?[\s\*&]+[A-Za-z_]\w*\s*(?:\[[^\]]*\]\s*)*"
+                        r"(?:=[^;]*)?(?:,\s*\**\s*[A-Za-z_]\w*\s*(?:\[[^\]]*\]\s*)*(?:=[^;]*)?)*;")
+
+
+def defines_globals(src):
+    """True when a header defines variables at file scope (not extern
+    declarations): the application's main file includes such a header once,
+    and a generated kernels file must not define them again."""
+    m = mask(src)
+    depth, start = 0, 0
+    for i, ch in enumerate(m):
+        if ch == "{":
+            depth += 1
+        elif ch == "}":
+            depth -= 1
+            start = i + 1
+        elif ch == ";" and depth == 0:
+            stmt = m[start:i + 1]
+            start = i + 1
+            # drop preprocessor lines
+            stmt = "\n".join(ln for ln in stmt.splitlines() if not ln.lstrip().startswith("#"))
+            if "(" not in stmt and GLOBAL_DEF.match(stmt.strip().replace("\n", " ")):
+                return True
+        elif ch == "\n" and depth == 0 and m[start:i].lstrip().startswith("#"):
+            start = i + 1
+    return False
+
+
+def const_member(c, d, t):
+    """the member holding constant c in a kernel's functor"""
+    if d is None:
+        return f"{t} {c}[OPSMI_CONST_CAP({t})];"
+    return f"{t} {c}{'[' + str(d) + ']' if d > 1 else ''};"
+
+
+def const_fetch(c, d, t):
+    """the stub's copy of constant c's current value into the functor"""
+    if d is not None and t in TYPES:
+        return f"ops_mi_gen_const(\"{c}\", \"{t}\", {d}, &b.k.{c});"
+    cap = f"(int)OPSMI_CONST_CAP({t})" if d is None else str(d)
+    return f"ops_mi_gen_const_sized(\"{c}\", \"{t}\", sizeof({t}), {cap}, {'1' if d is not None else '0'}, &b.k.{c});"
+
+
+class Program:
+    """What the source files of one program share (the reference translator
+    sees the whole program too): constants declared in any file, kernels
+    defined in any file or local header, and the headers that define the
+    kernels the program's loops call (dropped from every translated file)."""
+
+    def __init__(self, apps, inc_dirs=()):
+        self.files, seen = [], set()
+        self.consts, names = {}, set()
+        for app in apps:
+            src = open(app).read()
+            m = mask(src)
+            for p, s2, m2 in [(app, src, m)] + [(p, None, None) for _, _, p, _ in
+                                                  local_includes(app, src, m, list(inc_dirs))]:
+                if p in seen:
+                    continue
+                seen.add(p)
+                if s2 is None:
+                    s2 = open(p).read()
+                    m2 = mask(s2)
+                self.files.append((p, s2, m2))
+                self.consts.update(find_consts(s2, m2))
+            names |= {lp.kernel for lp in find_loops(src, m, app)}
+        self.kernel_headers = set()
+        for k in names:
+            path = find_kernel(k, self.files).path
+            if path not in apps:
+                self.kernel_headers.add(path)
 
 
 def find_kernel(name, files):
@@ -325,6 +404,8 @@ def accessed_points(body, pname, ndim):
     if not names or len(names) != len(calls):
         return None
     for mo in calls:
+        if address_taken(bm, mo.start()):
+            return None
         # skip member calls (pname.combine_max(...) has no '(' right after the name)
         op = mo.end() - 1
         cl = match_close(bm, op)
@@ -345,20 +426,92 @@ def accessed_points(body, pname, ndim):
     return sorted(pts)
 
 
+def address_taken(bm, pos):
+    """True when the name at pos is the operand of a unary & (its address
+    taken: pointer arithmetic from it reaches points no index lists)"""
+    j = pos - 1
+    while j >= 0 and bm[j].isspace():
+        j -= 1
+    return j >= 0 and bm[j] == "&" and (j == 0 or bm[j - 1] != "&")
+
+
+def legacy_points(body, pname, ndim):
+    """accessed_points for a legacy kernel: every use of pname must be an
+    index pname[OPS_ACCn(dx, ...)] or pname[OPS_ACC_MDn(d, dx, ...)] with
+    integer literals; None otherwise."""
+    bm = mask(body)
+    pts = set()
+    names = list(re.finditer(r"\b" + re.escape(pname) + r"\b", bm))
+    uses = list(re.finditer(r"\b" + re.escape(pname) + r"\s*\[\s*OPS_ACC(_MD)?\d+\s*\(", bm))
+    if not names or len(names) != len(uses):
+        return None
+    for mo in uses:
+        if address_taken(bm, mo.start()):
+            return None
+        op = mo.end() - 1
+        cl = match_close(bm, op)
+        rest = bm[cl + 1:].lstrip()
+        if not rest.startswith("]"):
+            return None
+        items = [bm[a:b].strip() for a, b in split_top(bm, op + 1, cl)]
+        vals = []
+        for it in items:
+            if not re.fullmatch(r"[+-]?\s*\d+", it):
+                return None
+            vals.append(int(it.replace(" ", "")))
+        if mo.group(1):
+            if len(vals) < 2:
+                return None
+            c, off = vals[0], vals[1:]
+        else:
+            c, off = -1, vals
+        if len(off) > ndim:
+            return None
+        off = off + [0] * (3 - len(off))
+        pts.add((c, off[0], off[1], off[2]))
+    return sorted(pts)
+
+
+LEGACY_PARAM = re.compile(r"^(?:const\s+)?([A-Za-z_][\w:]*)\s*(?:const\s*)?\*\s*(?:(?:__restrict__|restrict|__restrict)\s+)?"
+                          r"([A-Za-z_]\w*)$")
+
+
+def is_legacy(kern, loop):
+    """True when the kernel takes its dat arguments as T * pointers indexed
+    with the OPS_ACCn / OPS_ACC_MDn macros (the legacy API,
+    ops/c/include/ops_macros.h), False when they are ACC<T> references."""
+    forms = set()
+    for (text, pname), a in zip(kern.params, loop.args):
+        if a.kind != "dat":
+            continue
+        if re.search(r"\bACC\s*<", text):
+            forms.add("acc")
+        elif LEGACY_PARAM.match(text.strip()) and re.search(r"\bOPS_ACC(?:_MD)?\d+\s*\(", kern.body):
+            forms.add("legacy")
+        else:
+            raise TranslateError(f"kernel {kern.name}: parameter {text!r} for an ops_arg_dat must be an ACC<T> "
+                                 "reference (or a T * pointer indexed with OPS_ACCn)")
+    if len(forms) > 1:
+        raise TranslateError(f"kernel {kern.name}: mixes ACC<T> and pointer (OPS_ACCn) dat parameters")
+    return forms == {"legacy"}
+
+
 def kernel_code(loop, kern, consts):
     K = loop.kernel
     n = len(loop.args)
     if len(kern.params) != n:
         raise TranslateError(f"kernel {K} takes {len(kern.params)} parameters, the loop passes {n} arguments")
     used = [c for c in sorted(consts) if re.search(r"\b" + re.escape(c) + r"\b", mask(kern.body))]
-    points = [accessed_points(kern.body, pname, loop.ndim) if a.kind == "dat" else []
+    legacy = is_legacy(kern, loop)
+    listing = legacy_points if legacy else accessed_points
+    points = [listing(kern.body, pname, loop.ndim) if a.kind == "dat" else []
               for (_, pname), a in zip(kern.params, loop.args)]
     L = []
     L.append(f"// ---------------------------------------------------------------- {K}")
     L.append(f"struct opsk_{K} {{")
     for c in used:
         d, t = consts[c]
-        L.append(f"  {t} {c}{'[' + str(d) + ']' if d > 1 else ''};")
+        L.append("  " + const_member(c, d, t))
     # every ACC<T> parameter becomes a template type: the plain kernels pass
     # ::opsmi_gen::Acc accessors (restrict-qualified or general), fused
     # kernels also pass accessors that capture a write or hold values
@@ -366,7 +519,11 @@ def kernel_code(loop, kern, consts):
     ps, tps = [], []
     for i, ((text, pname), a) in enumerate(zip(kern.params, loop.args)):
         if a.kind == "dat":
-            new, cnt = re.subn(r"\bACC\s*<\s*([^>]+?)\s*>", f"OPSMI_A{i}", text)
+            if legacy:
+                # T *p indexed with OPS_ACCn(...): an accessor the offsets resolve against
+                new, cnt = f"const OPSMI_A{i} &{pname}", 1
+            else:
+                new, cnt = re.subn(r"\bACC\s*<\s*([^>]+?)\s*>", f"OPSMI_A{i}", text)
             if cnt != 1:
                 raise TranslateError(f"kernel {K}: parameter {text!r} for an ops_arg_dat must be an ACC<T> reference")
             ps.append(new)
@@ -558,6 +715,14 @@ def nb_fused_code(l1, l2, pts1, pts2, links):
             expr = f"({' && '.join(conds)}) ? v[{k}] : {expr}"
         L.append(f"    return {expr};")
         L.append("  }")
+        # legacy (OPS_ACCn) kernels index with offset objects
+        args_off = ", ".join(["o.x", "o.y", "o.z"][:nd])
+        if b.multi:
+            L.append(f"  __device__ __forceinline__ const {b.typ} &operator[](::opsmi_gen::OffMD o) const "
+                     f"{{ return (*this)(o.d, {args_off}); }}")
+        else:
+            L.append(f"  __device__ __forceinline__ const {b.typ} &operator[](::opsmi_gen::Off o) const "
+                     f"{{ return (*this)({args_off}); }}")
         L.append("};")
     L.append(f"__global__ __launch_bounds__(256) void opsg_fusenb_{F}(const opsa_{K1} A, const opsa_{K2} B"
              f"{''.join(', ' + p for p in ptrs)}) {{")
@@ -689,7 +854,7 @@ def stub_code(loop, used, consts, points):
     L.append("  memset(&b, 0, sizeof(b));")
     for c in used:
         d, t = consts[c]
-        L.append(f"  ops_mi_gen_const(\"{c}\", \"{t}\", {d}, &b.k.{c});")
+        L.append("  " + const_fetch(c, d, t))
     for i, a in enumerate(loop.args):
         if a.kind == "gbl":
             L.append(f"  if (arg{i}.data && arg{i}.elem_size == {TYPES[a.typ][1]} && arg{i}.dim > 0)")
@@ -700,7 +865,9 @@ def stub_code(loop, used, consts, points):
     return "\n".join(L), sig
 
 
-def translate(app, out_dir, inc_dirs=()):
+def translate(app, out_dir, inc_dirs=(), program=None):
+    """Translates one source file; program: what the other files of the same
+    program provide (Program), or None for a file on its own."""
     inc_dirs = list(inc_dirs)
     src = open(app).read()
     m = mask(src)
@@ -711,9 +878,11 @@ def translate(app, out_dir, inc_dirs=()):
     for _, _, p, _ in incs:
         s = open(p).read()
         files.append((p, s, mask(s)))
-    consts = {}
+    consts = dict(program.consts) if program else {}
     for _, s, mm in files:
         consts.update(find_consts(s, mm))
+    # kernels: this file and its headers first, then the rest of the program
+    kfiles = files + ([f for f in program.files if f[0] not in {q[0] for q in files}] if program else [])
     kernels, order = {}, []
     for lp in loops:
         if lp.kernel in kernels:
@@ -722,24 +891,27 @@ def translate(app, out_dir, inc_dirs=()):
                [(a.kind, a.typ, a.acc, a.dim, a.multi) for a in lp.args] or prev.ndim != lp.ndim:
                 raise TranslateError(f"kernel {lp.kernel} is called with different argument lists")
             continue
-        kernels[lp.kernel] = (lp, find_kernel(lp.kernel, files))
+        kernels[lp.kernel] = (lp, find_kernel(lp.kernel, kfiles))
         order.append(lp.kernel)
 
     # kernels file: the application's other local headers (user types and
     # helper functions) come first, with OPS_FUN_PREFIX making the helpers
     # device functions, as the reference's generated code does
-    kernel_files = {kern.path for _, kern in kernels.values()}
+    kernel_files = {kern.path for _, kern in kernels.values()} | (program.kernel_headers if program else set())
     ops_names = {"ops_seq_v2.h", "ops_seq.h", "ops_lib_core.h"}
     # headers of kernels no loop calls (they take ACC<T> parameters) stay in
     # the application only: their host functions would be defined twice
     acc_headers = {p for _, _, p, _ in incs if re.search(r"\bACC\s*<", mask(open(p).read()))}
     user_incs = [name for _, _, p, name in incs
-                 if p not in kernel_files and p not in acc_headers and os.path.basename(name) not in ops_names]
+                 if p not in kernel_files and p not in acc_headers and os.path.basename(name) not in ops_names
+                 and not defines_globals(open(p).read())]
     H = [f"// Generated by ops_amd/translator/opsmi_translate.py from {os.path.basename(app)}; do not edit.",
          "#include <math.h>", "#include <string.h>", "", "#include \"ops_mi355x_device.h\"", "",
          "#ifndef OPS_FUN_PREFIX", "#define OPS_FUN_PREFIX __host__ __device__", "#endif"]
     H += [f"#include \"{name}\"" for name in user_incs]
-    H += ["", "#define OPSMI_GEN_STUB __attribute__((visibility(\"hidden\")))", "", "namespace {"]
+    # stubs are weak: an application whose source files each call the same
+    # kernel links one of the (identical) stubs
+    H += ["", "#define OPSMI_GEN_STUB __attribute__((visibility(\"hidden\"), weak))", "", "namespace {"]
     stubs, sigs, kpoints = [], [], {}
     for k in order:
         lp, kern = kernels[k]
@@ -841,8 +1013,11 @@ def build_shared(apps, out_so, gen_dir, inc_dirs=(), hipcc="/opt/rocm/bin/hipcc"
     inc = [os.path.join(ROOT, "include"), gen_dir] + [os.path.dirname(os.path.abspath(a)) for a in apps] + list(inc_dirs)
     iflags = [f"-I{d}" for d in inc]
     cmds, objs = [], []
+    # a program's files share its constants; separate applications linked
+    # into one library keep their own
+    prog = Program(apps, inc_dirs) if executable else None
     for app in apps:
-        translate(app, gen_dir, inc_dirs)
+        translate(app, gen_dir, inc_dirs, prog)
         base = os.path.splitext(os.path.basename(app))[0]
         for src, obj, lang in ((f"{base}_opsmi_kernels.hip", f"{base}_k.o", ["-x", "hip"]),
                                (f"{base}_ops.cpp", f"{base}_a.o", [])):

@@ -146,3 +146,45 @@ def test_mb_trid_app_through_the_translator():
     OPS_instance with two blocks and a tridiagonal settings object per block."""
     out = _run("app_mbtrid_gen")
     assert "error" not in out.lower()
+
+
+# The reference's TeaLeaf (apps/c/TeaLeaf, 18 files, legacy OPS_ACCn
+# kernels) through the translator as one program, on its own test input
+# (tea.in: 10 x 10 cells, 10 steps, tl_eps 1e-15, test_problem 1) with each of
+# its four solvers; field_summary.cpp:82-103 compares the final temperature
+# with 157.55084183279294 and prints PASSED within 1e-3 %.
+TEA_IN = """*tea
+ state 1 density=100.0 energy=0.0001
+ state 2 density=0.1 energy=25.0 geometry=rectangle xmin=0.0 xmax=1.0 ymin=1.0 ymax=2.0
+ state 3 density=0.1 energy=0.1 geometry=rectangle xmin=1.0 xmax=6.0 ymin=1.0 ymax=2.0
+ state 4 density=0.1 energy=0.1 geometry=rectangle xmin=5.0 xmax=6.0 ymin=1.0 ymax=8.0
+ state 5 density=0.1 energy=0.1 geometry=rectangle xmin=5.0 xmax=10.0 ymin=7.0 ymax=8.0
+ x_cells=10
+ y_cells=10
+ xmin=0.0
+ ymin=0.0
+ xmax=10.0
+ ymax=10.0
+ initial_timestep=0.004
+ end_step=10
+ tl_max_iters=1000
+ {solver}
+ tl_eps=1e-15
+ test_problem 1
+*endtea
+"""
+
+
+@pytest.mark.parametrize("solver", ["tl_use_jacobi", "tl_use_cg", "tl_use_chebyshev", "tl_use_ppcg"])
+def test_tealeaf_app_through_the_translator(tmp_path, solver):
+    exe = os.path.join(REF, "app_tealeaf_gen")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    (tmp_path / "tea.in").write_text(TEA_IN.format(solver=solver))
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    txt = out.stdout + (tmp_path / "tea.out").read_text()
+    assert "This test is considered PASSED" in txt, txt[-3000:]
+    pct = float(re.search(r"within\s+([0-9.eE+-]+)%", txt).group(1))
+    assert pct < 1e-3, pct
+

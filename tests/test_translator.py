@@ -139,3 +139,63 @@ def test_gbl_with_a_run_time_dim_gets_a_bounded_copy(tmp_path):
     assert f"double g1[{T.GBL_CAP}];" in hip
     # the runtime learns the capacity (npts of a gbl argument) and refuses more
     assert f"static const int npts[2] = {{1, {T.GBL_CAP}}};" in hip
+
+
+def test_legacy_pointer_kernels_translate(tmp_path):
+    """Kernels in the legacy OPS_ACCn form (T * parameters indexed with
+    OPS_ACCn(dx, dy) / OPS_ACC_MDn(d, dx, dy), ops/c/include/ops_macros.h)
+    become accessor parameters; the literal offsets are listed as points
+    (stencil check, fusion) like ACC<T> calls, and a pointer used any other
+    way lists none."""
+    p = tmp_path / "legacy.cpp"
+    p.write_text('#include "ops_seq.h"\n'
+                 'void lk(const double *a, double * restrict b, double *v) {\n'
+                 '  b[OPS_ACC1(0,0)] = a[OPS_ACC0(1,0)] + a[OPS_ACC0(-1,0)]; *v = *v + b[OPS_ACC1(0,0)]; }\n'
+                 'void lk2(const double *a, double *b) { b[OPS_ACC_MD1(1,0,0)] = a[OPS_ACC0(0,0)]; }\n'
+                 'void run(ops_block bl, ops_dat d, ops_dat e, ops_stencil s0, ops_stencil s3, ops_reduction h, int *r) {\n'
+                 '  ops_par_loop(lk, "lk", bl, 2, r, ops_arg_dat(d, 1, s3, "double", OPS_READ),\n'
+                 '               ops_arg_dat(e, 1, s0, "double", OPS_WRITE), ops_arg_reduce(h, 1, "double", OPS_INC));\n'
+                 '  ops_par_loop(lk2, "lk2", bl, 2, r, ops_arg_dat(e, 1, s0, "double", OPS_READ),\n'
+                 '               ops_arg_dat(d, 2, s0, "double", OPS_WRITE));\n'
+                 '}\n')
+    T.translate(str(p), str(tmp_path / "out"))
+    hip = open(tmp_path / "out" / "legacy_opsmi_kernels.hip").read()
+    assert "const OPSMI_A0 &a, const OPSMI_A1 &b, double *v" in hip
+    assert "b[OPS_ACC1(0,0)] = a[OPS_ACC0(1,0)]" in hip  # the body is kept as written
+    stub = hip[hip.index("void ops_par_loop_lk("):]
+    assert "static const int npts[3] = {2, 1, 0};" in stub
+    assert "static const int pts[12] = {-1, -1, 0, 0, -1, 1, 0, 0, -1, 0, 0, 0};" in stub
+    stub2 = hip[hip.index("void ops_par_loop_lk2("):]
+    assert "static const int npts[2] = {1, 1};" in stub2 and "1, 0, 0, 0" in stub2  # component 1 of d
+    assert "ops_mi_gen_fused(" in hip
+    assert T.legacy_points("{ b[OPS_ACC0(0,0)] = *(&a[OPS_ACC1(0,0)] + 1); }", "a", 2) is None
+    assert T.accessed_points("{ w(0, 0) = *(&p(0, 0) + 1); }", "p", 2) is None
+    assert T.legacy_points("{ b[OPS_ACC0(0,0)] = (a[OPS_ACC1(0,0)] > 0) && (a[OPS_ACC1(1,0)] > 0); }", "a", 2) \
+        == [(-1, 0, 0, 0), (-1, 1, 0, 0)]
+    assert T.legacy_points("{ double *q = a; b[OPS_ACC0(0,0)] = q[0]; }", "a", 2) is None
+    assert T.legacy_points("{ b[OPS_ACC0(0,0)] = a[OPS_ACC1(i,0)]; }", "a", 2) is None
+    app = open(tmp_path / "out" / "legacy_ops.cpp").read()
+    assert "ops_par_loop_lk(" in app and '#include "ops_mi355x.h"' in app
+
+
+def test_program_constants_and_globals(tmp_path):
+    """Files of one program share constants (a user struct of run-time
+    count declared in another file); headers that define globals stay out of
+    the generated kernel files."""
+    (tmp_path / "types.h").write_text("typedef struct { double a; int g; } st_t;\n")
+    (tmp_path / "vars.h").write_text('#include "types.h"\nst_t *sts;\nint nst;\n')
+    (tmp_path / "main.cpp").write_text('#include "ops_seq.h"\n#include "vars.h"\n'
+                                       'void setup() { ops_decl_const("sts", nst, "st_t", sts); }\n')
+    (tmp_path / "k.cpp").write_text('#include "ops_seq.h"\n#include "types.h"\n'
+                                    'void kk(double *u) { u[OPS_ACC0(0,0)] = sts[1].a; }\n'
+                                    'void run(ops_block bl, ops_dat d, ops_stencil s0, int *r) {\n'
+                                    '  ops_par_loop(kk, "kk", bl, 2, r, ops_arg_dat(d, 1, s0, "double", OPS_WRITE));\n}\n')
+    prog = T.Program([str(tmp_path / "main.cpp"), str(tmp_path / "k.cpp")])
+    assert prog.consts["sts"] == (None, "st_t")
+    T.translate(str(tmp_path / "k.cpp"), str(tmp_path / "out"), program=prog)
+    hip = open(tmp_path / "out" / "k_opsmi_kernels.hip").read()
+    assert "st_t sts[OPSMI_CONST_CAP(st_t)];" in hip
+    assert 'ops_mi_gen_const_sized("sts", "st_t", sizeof(st_t), (int)OPSMI_CONST_CAP(st_t), 0, &b.k.sts);' in hip
+    assert T.defines_globals(open(tmp_path / "vars.h").read())
+    assert not T.defines_globals(open(tmp_path / "types.h").read())
+    assert not T.defines_globals("extern int q;\nstatic const int k = 3;\nint f(int x);\n")
