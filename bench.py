@@ -126,10 +126,15 @@ def cpu_baseline_heat(n, seed, target_s=10.0):
     O.heat_cg(3, n, n, n, K, rhs, max_iters=iters, nthreads=nt)
     dt = max(O.last_loop_seconds(), 1e-9) / iters
     gbs = 112.0 * n ** 3 / dt / 1e9
+    # one thread, as the reference's sequential (dev_seq) build runs
+    O.heat_cg(3, n, n, n, K, rhs, max_iters=2, nthreads=1)
+    dt1 = max(O.last_loop_seconds(), 1e-9) / 2
     del K, rhs
     return {"value": round(gbs, 2), "unit": "GB/s", "cores": nt, "kind": "port",
             "sample": f"{iters} CG iterations of the same {n}^3 heat problem (oracle OpenMP, "
-                      f"{dt * 1e3:.1f} ms/iteration)"}
+                      f"{dt * 1e3:.1f} ms/iteration)",
+            "one_thread": {"value": round(112.0 * n ** 3 / dt1 / 1e9, 2), "unit": "GB/s",
+                           "ms_per_iteration": round(dt1 * 1e3, 1), "sample": "2 CG iterations, 1 thread"}}
 
 
 def bench_heat(a, ops, apps):

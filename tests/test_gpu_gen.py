@@ -44,6 +44,8 @@ def gen(gpu_ops):
     L.gen_chain_src_lap.argtypes = [vp, vp, vp, vp, ip, ip, dp]
     L.gen_chain_mix_avg.argtypes = [vp, vp, vp, vp, ci, ci, ci, dp]
     L.gen_chain_refused.argtypes = [vp, vp, vp, vp, vp, ci, ci, ci]
+    L.gen_legacy_lap2d.argtypes = [vp, vp, vp, ci, ci, ci]
+    L.gen_legacy_md3d.argtypes = [vp, vp, vp, ci, ci, ci, ci, dp]
     return L
 
 
@@ -596,3 +598,77 @@ def test_generated_laplace_fused_across_levels(gpu_ops, gen, n):
     for lev, (h, A, cnt) in enumerate(runs):
         assert np.array_equal(h, ref_h) and np.array_equal(A, O.interior(ref_A)), lev
         assert cnt == (it - 1 if lev == 2 else 0), (lev, cnt)
+
+
+@pytest.mark.parametrize("n", [(64, 40), (131, 23)])
+def test_legacy_pointer_kernels_2d(gpu_ops, gen, n):
+    """Kernels in the legacy form (T * parameters indexed with OPS_ACCn) give
+    the bits of the same kernels written with ACC<T>, and of numpy in the
+    kernels' operation order, at every fusion level; at level 2 the legacy
+    copy runs with the next 5-point average as one pass."""
+    ops = gpu_ops
+    nx, ny = n
+    a0 = O.u01(11, 1, (nx + 2) * (ny + 2)).reshape(ny + 2, nx + 2)
+
+    def lap(a):
+        b = a.copy()
+        b[1:-1, 1:-1] = 0.25 * (((a[1:-1, 2:] + a[1:-1, :-2]) + a[2:, 1:-1]) + a[:-2, 1:-1])
+        return b
+
+    b1 = lap(a0)
+    a1 = a0.copy()
+    a1[1:-1, 1:-1] = b1[1:-1, 1:-1]
+    want_b = lap(a1)[1:-1, 1:-1]
+
+    def run(mode):
+        blk = ops.ops_decl_block(2, "genleg2")
+        a = ops.ops_decl_dat(blk, [nx, ny], name="a")
+        b = ops.ops_decl_dat(blk, [nx, ny], name="b")
+        ops.set_ext(a, a0)
+        assert gen.gen_legacy_lap2d(blk.ptr, a.ptr, b.ptr, nx, ny, mode) == 0
+        ops._check()
+        out = (ops.ops_dat_fetch_data(a), ops.ops_dat_fetch_data(b),
+               _fused_count(ops, "fused(leg_copy+leg_lap)" if mode == 0 else "fused(acc_copy+acc_lap)"))
+        ops.ops_free_dat(a)
+        ops.ops_free_dat(b)
+        return out
+
+    for lev, (leg, acc) in enumerate(zip(_levels(ops, lambda: run(0)), _levels(ops, lambda: run(1)))):
+        assert np.array_equal(leg[0], acc[0]) and np.array_equal(leg[1], acc[1]), lev
+        assert np.array_equal(leg[0], a1[1:-1, 1:-1]) and np.array_equal(leg[1], want_b), lev
+        assert leg[2] == acc[2] == (1 if lev == 2 else 0), (lev, leg[2], acc[2])
+
+
+@pytest.mark.parametrize("n", [(20, 9, 6), (67, 13, 5)])
+@pytest.mark.parametrize("soa", [0, 1])
+def test_legacy_multi_component_kernels_3d(gpu_ops, gen, n, soa):
+    """OPS_ACC_MDn on a two-component 3D dat (either layout) at neighbours,
+    with an index-weighted sum: field bit-identical to the ACC<T> form and to
+    numpy in the kernel's order; the sum bit-identical between the forms and
+    within 1e-12 of numpy's."""
+    ops = gpu_ops
+    nx, ny, nz = n
+    v0 = O.u01(12, 1, (nz + 2) * (ny + 2) * (nx + 2) * 2).reshape(nz + 2, ny + 2, nx + 2, 2)
+    w_ref = (v0[1:-1, 1:-1, 2:, 0] - v0[:-2, 1:-1, 1:-1, 1]) + 0.5 * (v0[1:-1, 2:, 1:-1, 1] - v0[1:-1, :-2, 1:-1, 0])
+    Z, Y, X = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    s_ref = float((w_ref * (X + 2 * Y + 3 * Z)).sum())
+    out = []
+    for mode in (0, 1):
+        blk = ops.ops_decl_block(3, "genleg3")
+        ops.ops_set_soa(soa)
+        try:
+            v = ops.ops_decl_dat(blk, [nx, ny, nz], name="v", dim=2)
+        finally:
+            ops.ops_set_soa(0)
+        w = ops.ops_decl_dat(blk, [nx, ny, nz], name="w")
+        ops.set_ext(v, v0)
+        s = np.zeros(1)
+        assert gen.gen_legacy_md3d(blk.ptr, v.ptr, w.ptr, nx, ny, nz, mode, _p(s)) == 0
+        ops._check()
+        out.append((ops.ops_dat_fetch_data(w), s[0]))
+        ops.ops_free_dat(v)
+        ops.ops_free_dat(w)
+    (w_leg, s_leg), (w_acc, s_acc) = out
+    assert np.array_equal(w_leg, w_acc) and s_leg == s_acc
+    assert np.array_equal(w_leg, w_ref)
+    assert abs(s_leg - s_ref) <= 1e-12 * max(1.0, np.abs(w_ref * (X + 2 * Y + 3 * Z)).sum())
