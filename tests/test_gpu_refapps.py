@@ -188,3 +188,65 @@ def test_tealeaf_app_through_the_translator(tmp_path, solver):
     pct = float(re.search(r"within\s+([0-9.eE+-]+)%", txt).group(1))
     assert pct < 1e-3, pct
 
+
+
+def _adi_oracle(nx, ny, nz, iters, pcr):
+    """The ADI app's computation restated on the oracle: init_kernel (1 on
+    the faces of the range, 0 inside, apps/c/adi/init_kernel.h), then per
+    iteration preproc_kernel (lambda = 1) and the x, y solves in place on du
+    and the z solve added to u (adi.cpp:385-421); the app's a, b, c are the
+    same for the three directions.  Returns u with its one-point halo."""
+    import numpy as np
+
+    from oracle import oracle_py as O
+
+    u = np.zeros((nz + 2, ny + 2, nx + 2))
+    core = u[1:-1, 1:-1, 1:-1]
+    core[...] = 1.0
+    core[1:-1, 1:-1, 1:-1] = 0.0
+    solve = O.pcr_batch if pcr else O.thomas_batch
+    dims = [nx, ny, nz]
+    for _ in range(iters):
+        du, a, b, c = O.adi_preproc(nx, ny, nz, u, 1.0)
+        solve(3, 0, dims, dims, a, b, c, du)
+        solve(3, 1, dims, dims, a, b, c, du)
+        ui = np.ascontiguousarray(u[1:-1, 1:-1, 1:-1])
+        solve(3, 2, dims, dims, a, b, c, du, ui)
+        u[1:-1, 1:-1, 1:-1] = ui
+    return u
+
+
+# The reference's ADI app (apps/c/adi/adi.cpp, legacy OPS_ACCn kernels,
+# ops_tridMultiDimBatch along x and y and _Inc along z) through the
+# translator.  Its HDF5 dumps are refused by this backend (no HDF5 library)
+# and reported, so the run sets OPSMI_ERROR_MODE=2 (print and continue); it
+# then checks its halo ("Halo Test") and writes the first nx*ny*nz doubles of
+# u's host array to <argv[0]>.dat (adi.cpp:88-118,491), which the test
+# compares with the oracle's u in the same layout.  -m 0 selects Thomas
+# (bit-exact), -m 5 PCR (within 1e-12 of the oracle's PCR).
+@pytest.mark.parametrize("m", [0, 5])
+def test_adi_app_through_the_translator(tmp_path, m):
+    import numpy as np
+
+    exe = os.path.join(REF, "app_adi_gen")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    nx, ny, nz, iters = 37, 20, 13, 3
+    env = dict(os.environ, OPSMI_ERROR_MODE="2")
+    out = subprocess.run(["adi_out", "-nx", str(nx), "-ny", str(ny), "-nz", str(nz), "-iter", str(iters),
+                          "-m", str(m)], executable=exe, capture_output=True, text=True, timeout=120,
+                         cwd=tmp_path, env=env)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    assert "Halo Test: PASSED" in out.stdout, out.stdout[-2000:]
+    assert "Successful exit from OPS!" in out.stdout
+    assert "HDF5 output is not available" in out.stderr
+    sx, sy, sz = (int(v) for v in re.search(r"Local dimensions: (\d+) x (\d+) x (\d+)", out.stdout).groups())
+    got = np.fromfile(tmp_path / "adi_out.dat", dtype=np.float64)
+    assert got.size == nx * ny * nz
+    full = np.zeros((sz, sy, sx))
+    full[: nz + 2, : ny + 2, : nx + 2] = _adi_oracle(nx, ny, nz, iters, pcr=m == 5)
+    want = full.ravel()[: nx * ny * nz]
+    if m == 0:
+        assert np.array_equal(got, want)
+    else:
+        assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
