@@ -630,10 +630,13 @@ class ACC {
   T *p_;
   long sx_, sy_, sz_;
 };
+/* applies the OPS_soa flag of the instance object, if there is one (below) */
+static inline void opsmi_instance_layout(void);
 template <class T>
 static inline ops_dat ops_decl_dat(ops_block block, int data_size, int *block_size, int *base, int *d_m,
                                    int *d_p, T *data, const char *type, const char *name) {
   int stride[OPS_MAX_DIM] = {1, 1, 1, 1, 1};
+  opsmi_instance_layout();
   return ops_decl_dat_char(block, data_size, block_size, base, d_m, d_p, stride, (char *)data, sizeof(T), type,
                            name);
 }
@@ -721,6 +724,22 @@ class OPS_instance {
     static OPS_instance *p = nullptr;
     return p;
   }
+  /* ops/c/include/ops_instance.h:67: the instance of the C API.  After
+   * ops_init there is no instance object; this returns one that stands for
+   * the library's single instance (created on first use, never calling
+   * ops_init or ops_exit), whose OPS_soa the C-API declarations follow. */
+  static OPS_instance *getOPSInstance() {
+    if (current()) return current();
+    if (!view()) {
+      static OPS_instance v{ViewTag{}};
+      view() = &v;
+    }
+    return view();
+  }
+  static OPS_instance *&view() {
+    static OPS_instance *p = nullptr;
+    return p;
+  }
   static int numInstances() { return current() ? 1 : 0; }
   std::ostream &ostream() { return *os_; }
   void set_ostream(std::ostream &s) { os_ = &s; }
@@ -766,13 +785,20 @@ class OPS_instance {
   int OPS_soa; /* 1: multi-component dats declared from now on are SoA */
 
  private:
+  struct ViewTag {};
+  explicit OPS_instance(ViewTag) : OPS_soa(0), os_(&std::cout) {}
   std::ostream *os_;
 };
+
+static inline void opsmi_instance_layout(void) {
+  OPS_instance *i = OPS_instance::current() ? OPS_instance::current() : OPS_instance::view();
+  if (i) ops_set_soa(i->OPS_soa);
+}
 
 template <class T>
 inline ops_dat ops_block_core::decl_dat(int data_size, int *block_size, int *base, int *d_m, int *d_p, int *stride,
                                         T *data, const char *type, const char *name) {
-  if (OPS_instance::current()) ops_set_soa(OPS_instance::current()->OPS_soa);
+  opsmi_instance_layout();
   return ops_decl_dat_char(this, data_size, block_size, base, d_m, d_p, stride, (char *)data, sizeof(T), type, name);
 }
 template <class T>
