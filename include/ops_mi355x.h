@@ -163,6 +163,7 @@ typedef struct ops_dat_core {
   void *impl;
 #ifdef __cplusplus
   void get_data();
+  void print_to_txtfile(const char *file_name);
   int get_local_npartitions();
   void get_extents(int part, int *disp, int *size);
   void get_raw_metadata(int part, int *disp, int *size, int *stride, int *d_m, int *d_p);
@@ -395,6 +396,10 @@ void ops_tridMultiDimBatch_noalloc(int ndim, int solvedim, int *range, ops_dat a
 /* ops/c/include/ops_internal2.h:219: reports the first NaN in the dat
  * (halos included) and exits with status 2 */
 void ops_NaNcheck(ops_dat dat);
+/* ops/c/include/ops_lib_core.h:1050 (ops_print_dat_to_txtfile,
+ * ops/c/src/core/ops_lib_core.cpp:1277-1462): appends the dat's whole array,
+ * halos included, to a text file in the reference's format */
+void ops_print_dat_to_txtfile(ops_dat dat, const char *file_name);
 
 /* ---- utilities applications call: ops/c/include/ops_util.h:41-57 (the
  * allocators report an OPS error instead of returning NULL) ---- */
@@ -813,6 +818,7 @@ inline void ops_reduction_core::get_result(T *ptr) {
 }
 inline void ops_halo_group_core::halo_transfer() { ops_halo_transfer(this); }
 inline void ops_dat_core::get_data() { ops_execute(); }
+inline void ops_dat_core::print_to_txtfile(const char *file_name) { ops_print_dat_to_txtfile(this, file_name); }
 inline int ops_dat_core::get_local_npartitions() { return ops_dat_get_local_npartitions(this); }
 inline void ops_dat_core::get_extents(int part, int *disp, int *size) { ops_dat_get_extents(this, part, disp, size); }
 inline void ops_dat_core::get_raw_metadata(int part, int *disp, int *size, int *stride, int *d_m, int *d_p) {

@@ -1549,6 +1549,61 @@ void ops_NaNcheck(ops_dat dat) {
   OPSMI_API_END()
 }
 
+// ops/c/src/core/ops_lib_core.cpp:1277-1462 (ops_print_dat_to_txtfile_core):
+// appends the dat's whole array (halos and padding included) to the file:
+// a header, then for each row every point's components, rows ending in a
+// newline, planes in a blank line
+void ops_print_dat_to_txtfile(ops_dat dat, const char *file_name) {
+  OPSMI_API_BEGIN
+  require_init();
+  if (!dat || !file_name) fail(OPS_INVALID_ARGUMENT, "ops_print_dat_to_txtfile: NULL argument");
+  const DType dt = impl(dat)->dt;
+  const bool cplx = strcmp(dat->type, "complexd") == 0;
+  if (dt != DT_F64 && dt != DT_F32 && dt != DT_I32 && !cplx)
+    fail(OPS_NOT_IMPLEMENTED, cat("Error: Unknown type ", dat->type, " cannot be written to file ", file_name));
+  flush_all();
+  ensure_host(dat);
+  FILE *fp = fopen(file_name, "a");
+  if (!fp) fail(OPS_RUNTIME_ERROR, cat("Error: can't open file ", file_name));
+  const int nd = dat->block->dims;
+  long n[3] = {1, 1, 1};
+  for (int q = 0; q < nd && q < 3; q++) n[q] = dat->size[q];
+  fprintf(fp, "ops_dat:  %s \n", dat->name);
+  fprintf(fp, "ops_dat dim:  %d \n", dat->dim);
+  fprintf(fp, "block Dims : %d ", nd);
+  for (int q = 0; q < nd; q++) fprintf(fp, "[%d]", dat->size[q]);
+  fprintf(fp, "\n");
+  fprintf(fp, "elem size %d \n", dat->elem_size);
+  const size_t plane = (size_t)(n[0] * n[1] * n[2]);
+  const bool soa = impl(dat)->soa;
+  for (long k = 0; k < n[2]; k++) {
+    for (long j = 0; j < n[1]; j++) {
+      for (long i = 0; i < n[0]; i++)
+        for (int d = 0; d < dat->dim; d++) {
+          const size_t pt = (size_t)((k * n[1] + j) * n[0] + i);
+          const size_t off = soa ? pt + d * plane : pt * dat->dim + d;
+          if (cplx) {
+            const double *c = (const double *)dat->data;
+            fprintf(fp, "%3.10lf+%3.10lfi ", c[2 * off], c[2 * off + 1]);
+          } else if (dt == DT_F64) {
+            fprintf(fp, " %3.10lf", ((const double *)dat->data)[off]);
+          } else if (dt == DT_F32) {
+            fprintf(fp, "%e ", ((const float *)dat->data)[off]);
+          } else {
+            fprintf(fp, "%d ", ((const int *)dat->data)[off]);
+          }
+        }
+      if (n[0] > 1) fprintf(fp, "\n");
+    }
+    if (n[1] > 1) fprintf(fp, "\n");
+  }
+  if (n[2] > 1) fprintf(fp, "\n");
+  const bool bad = ferror(fp) != 0;
+  fclose(fp);
+  if (bad) fail(OPS_RUNTIME_ERROR, cat("Error: error writing to file ", file_name));
+  OPSMI_API_END()
+}
+
 void ops_diagnostic_output(void) {
   Instance &I = inst();
   printf("\n OPS diagnostic output (MI355X backend, device %d, %d CUs)\n", I.device, I.num_cus);

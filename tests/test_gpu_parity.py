@@ -458,6 +458,26 @@ def test_random_fill_uniform_matches_generator(gpu_ops):
     assert np.max(np.abs(ext - refn)) < 1e-13
 
 
+@pytest.mark.parametrize("soa", [0, 1])
+def test_random_fill_multi_component(gpu_ops, soa):
+    """Component c of point i of a D-component dat takes counter D i + c in
+    either layout: the same values AoS and SoA, halos included."""
+    ops = gpu_ops
+    nx, ny, nz, dim = 37, 6, 5, 3
+    blk = ops.ops_decl_block(3, "rng3")
+    ops.ops_set_soa(soa)
+    try:
+        x = ops.ops_decl_dat(blk, [nx, ny, nz], name="x3", dim=dim)
+    finally:
+        ops.ops_set_soa(0)
+    ops.ops_randomgen_init(5)
+    ops.ops_fill_random_uniform(x)
+    ext = ops.get_ext(x)
+    ref = O.u01(5, 0, (nx + 2) * (ny + 2) * (nz + 2) * dim).reshape(nz + 2, ny + 2, nx + 2, dim)
+    assert np.array_equal(ext, ref)
+    ops.ops_free_dat(x)
+
+
 # ------------------------------------------------------------------ tridiagonal
 def _trid_host(seed, nx, ny, nz, sd):
     a, b, c = O.trid_inputs(seed, sd, nx, ny, nz)

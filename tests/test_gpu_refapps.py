@@ -283,3 +283,39 @@ def test_laplace_tutorial_steps_through_the_translator(step):
     assert "This run is considered PASSED" in r.stdout, r.stdout[-3000:]
     pct = float(re.search(r"within ([0-9.eE+-]+) %", r.stdout).group(1))
     assert pct < 1e-9, pct
+
+
+def test_random_app_through_the_translator(tmp_path):
+    """The reference's random test (apps/c/random/random.cpp, unchanged): two
+    uniform fills (seed 0) of a two-component 4 x 4 dat with halos (-2, -1)
+    and (2, 1), each appended to random.dat with ops_print_dat_to_txtfile.
+    The file holds, per fill, the whole array (halos and padding columns) in
+    the reference's text format; its values are the generator's (fill k =
+    stream k, counter 2 i + c) to the 10 printed decimals, padding 0."""
+    import numpy as np
+
+    from oracle import oracle_py as O
+
+    exe = os.path.join(REF, "app_random_gen")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "This run is considered PASSED" in r.stdout
+    txt = (tmp_path / "random.dat").read_text()
+    blocks = txt.split("ops_dat:  dat0 \n")[1:]
+    assert len(blocks) == 2
+    w, h, dim = 4 + 2 + 2, 4 + 1 + 1, 2
+    for k, b in enumerate(blocks):
+        lines = b.split("\n")
+        assert lines[0] == "ops_dat dim:  2 "
+        sx, sy = (int(v) for v in re.match(r"block Dims : 2 \[(\d+)\]\[(\d+)\]", lines[1]).groups())
+        assert sy == h and sx >= w
+        assert lines[2] == f"elem size {8 * dim} "
+        rows = [ln for ln in lines[3:] if ln.strip()]
+        got = np.array([[float(v) for v in ln.split()] for ln in rows])
+        assert got.shape == (sy, sx * dim)
+        got = got.reshape(sy, sx, dim)
+        want = O.u01(0, k, w * h * dim).reshape(h, w, dim)
+        assert np.max(np.abs(got[:, :w] - want)) <= 5.1e-11
+        assert np.all(got[:, w:] == 0.0)
