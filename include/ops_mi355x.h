@@ -120,11 +120,25 @@ typedef uint16_t half;
  * defined after the C declarations below); the data layout is the same. ---- */
 struct ops_dat_core;
 struct ops_stencil_core;
+#ifdef __cplusplus
+/* the objects' instance member (ops/c/include/ops_lib_core.h: an
+ * OPS_instance *): one pointer wide, as in C, and under C++ it reaches the
+ * library's single instance, e.g. block->instance->OPS_diags */
+class OPS_instance;
+struct opsmi_instance_ref {
+  void *unused_;
+  inline OPS_instance *operator->() const;
+  inline operator OPS_instance *() const;
+};
+#define OPSMI_INSTANCE_MEMBER opsmi_instance_ref instance
+#else
+#define OPSMI_INSTANCE_MEMBER void *instance
+#endif
 typedef struct ops_block_core {
   int index;
   int dims;
   const char *name;
-  void *instance;
+  OPSMI_INSTANCE_MEMBER;
 #ifdef __cplusplus
   template <class T>
   struct ops_dat_core *decl_dat(int data_size, int *block_size, int *base, int *d_m, int *d_p, int *stride, T *data,
@@ -198,7 +212,7 @@ typedef struct ops_reduction_core {
   ops_access acc;
   char *type;
   char *name;
-  void *instance;
+  OPSMI_INSTANCE_MEMBER;
   void *impl;
 #ifdef __cplusplus
   template <class T>
@@ -223,7 +237,7 @@ typedef struct ops_halo_group_core {
   int nhalos;
   ops_halo *halos;
   int index;
-  void *instance;
+  OPSMI_INSTANCE_MEMBER;
 #ifdef __cplusplus
   void halo_transfer();
 #endif
@@ -576,6 +590,8 @@ void ops_mi_set_error_mode(int mode);
  * above 1 every launch is timed with HIP events for ops_mi_loop_stats and
  * the timing table; queued loops run first */
 void ops_mi_set_diags(int level);
+/* the current diagnostics level */
+int ops_mi_get_diags(void);
 /* last error code (0 = none) and message; reading clears it */
 int ops_mi_last_error(char *msg, int msg_len);
 /* the library's HIP stream (hipStream_t) */
@@ -645,6 +661,14 @@ static inline ops_dat ops_decl_dat(ops_block block, int data_size, int *block_si
   return ops_decl_dat_char(block, data_size, block_size, base, d_m, d_p, stride, (char *)data, sizeof(T), type,
                            name);
 }
+/* ops/c/include/ops_lib_core.h:647-660: with the dat's stride per dimension */
+template <class T>
+static inline ops_dat ops_decl_dat(ops_block block, int data_size, int *block_size, int *base, int *d_m,
+                                   int *d_p, int *stride, T *data, const char *type, const char *name) {
+  opsmi_instance_layout();
+  return ops_decl_dat_char(block, data_size, block_size, base, d_m, d_p, stride, (char *)data, sizeof(T), type,
+                           name);
+}
 template <class T>
 static inline ops_arg ops_arg_gbl(T *data, int dim, const char *type, ops_access acc) {
   (void)type;
@@ -656,6 +680,11 @@ static inline void ops_reduction_result(ops_reduction handle, T *ptr) {
 }
 template <class T>
 static inline void ops_decl_const(const char *name, int dim, const char *type, T *data) {
+  ops_decl_const_char(nullptr, dim, type, sizeof(T), (char *)data, name);
+}
+/* ops/c/include/ops_lib_core.h:946-956: a new value for a declared constant */
+template <class T>
+static inline void ops_update_const(const char *name, int dim, const char *type, T *data) {
   ops_decl_const_char(nullptr, dim, type, sizeof(T), (char *)data, name);
 }
 /* ops/c/include/ops_tridiag.h:45-71: same single data member */
@@ -788,12 +817,24 @@ class OPS_instance {
   int is_root() { return 1; }
   void execute() { ops_execute(); }
   int OPS_soa; /* 1: multi-component dats declared from now on are SoA */
+  /* ops/c/include/ops_instance.h: OPS_diags, the diagnostics level (read
+   * from the library) */
+  struct DiagsLevel {
+    operator int() const { return ops_mi_get_diags(); }
+  } OPS_diags;
+  /* ops/c/include/ops_instance.h:246: restarts the energy counters the
+   * reference reads through rocm_smi/NVML when built with them; no counters
+   * here */
+  void reset_power_counters() {}
 
  private:
   struct ViewTag {};
   explicit OPS_instance(ViewTag) : OPS_soa(0), os_(&std::cout) {}
   std::ostream *os_;
 };
+
+inline OPS_instance *opsmi_instance_ref::operator->() const { return OPS_instance::getOPSInstance(); }
+inline opsmi_instance_ref::operator OPS_instance *() const { return OPS_instance::getOPSInstance(); }
 
 static inline void opsmi_instance_layout(void) {
   OPS_instance *i = OPS_instance::current() ? OPS_instance::current() : OPS_instance::view();

@@ -821,7 +821,7 @@ ops_block ops_decl_block(int dims, const char *name) {
   b->index = (int)inst().blocks.size();
   b->dims = dims;
   b->name = copy_str(name);
-  b->instance = &inst();
+  b->instance.unused_ = &inst();
   inst().blocks.push_back(b);
   return b;
   OPSMI_API_END(nullptr)
@@ -978,7 +978,7 @@ ops_reduction ops_decl_reduction_handle(int size, const char *type, const char *
   r->index = (int)inst().reductions.size();
   r->name = copy_str(name);
   r->type = copy_str(type);
-  r->instance = &inst();
+  r->instance.unused_ = &inst();
   RedImpl *ri = new RedImpl;
   int ts = 8;
   if (!strcmp(type, "float") || !strcmp(type, "int") || !strcmp(type, "real") || !strcmp(type, "integer")) ts = 4;
@@ -1033,7 +1033,7 @@ ops_halo_group ops_decl_halo_group(int nhalos, ops_halo *halos) {
   g->halos = (ops_halo *)malloc(sizeof(ops_halo) * std::max(nhalos, 1));
   for (int i = 0; i < nhalos; i++) g->halos[i] = halos[i];
   g->index = (int)inst().groups.size();
-  g->instance = &inst();
+  g->instance.unused_ = &inst();
   inst().groups.push_back(g);
   return g;
   OPSMI_API_END(nullptr)
@@ -1638,6 +1638,8 @@ void ops_mi_set_error_mode(int mode) { inst().error_mode = mode; }
 
 // ops/c/src/core/ops_lib_core.cpp:372-374
 void ops_set_soa(const int soa_val) { inst().soa = soa_val ? 1 : 0; }
+
+int ops_mi_get_diags(void) { return inst().diags; }
 
 void ops_mi_set_diags(int level) {
   OPSMI_API_BEGIN

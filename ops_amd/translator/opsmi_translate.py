@@ -302,6 +302,33 @@ def defines_globals(src):
     return False
 
 
+def defines_functions(src):
+    """True when a file defines functions with external linkage at file
+    scope (not inline, static, constexpr or templates), as an implementation
+    file the application includes (CloverLeaf's profile.cpp) does: a
+    generated kernels file that included it too would define them twice."""
+    m = mask(src)
+    depth, start = 0, 0
+    for i, ch in enumerate(m):
+        if ch == "{":
+            if depth == 0:
+                head = "\n".join(ln for ln in m[start:i].splitlines() if not ln.lstrip().startswith("#")).strip()
+                if re.search(r"\)\s*(?:const\s*)?(?:noexcept\s*)?$", head) and \
+                        not re.search(r"\b(?:inline|static|constexpr|template|struct|class|namespace|extern\s*\"C\")\b",
+                                      head) and not re.match(r"^(?:if|for|while|switch)\b", head):
+                    return True
+            depth += 1
+        elif ch == "}":
+            depth -= 1
+            if depth == 0:
+                start = i + 1
+        elif ch == ";" and depth == 0:
+            start = i + 1
+        elif ch == "\n" and depth == 0 and m[start:i].lstrip().startswith("#"):
+            start = i + 1
+    return False
+
+
 def const_member(c, d, t):
     """the member holding constant c in a kernel's functor"""
     if d is None:
@@ -904,7 +931,7 @@ def translate(app, out_dir, inc_dirs=(), program=None):
     acc_headers = {p for _, _, p, _ in incs if re.search(r"\bACC\s*<", mask(open(p).read()))}
     user_incs = [name for _, _, p, name in incs
                  if p not in kernel_files and p not in acc_headers and os.path.basename(name) not in ops_names
-                 and not defines_globals(open(p).read())]
+                 and not defines_globals(open(p).read()) and not defines_functions(open(p).read())]
     H = [f"// Generated by ops_amd/translator/opsmi_translate.py from {os.path.basename(app)}; do not edit.",
          "#include <math.h>", "#include <string.h>", "", "#include \"ops_mi355x_device.h\"", "",
          "#ifndef OPS_FUN_PREFIX", "#define OPS_FUN_PREFIX __host__ __device__", "#endif"]

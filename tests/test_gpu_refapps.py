@@ -319,3 +319,43 @@ def test_random_app_through_the_translator(tmp_path):
         want = O.u01(0, k, w * h * dim).reshape(h, w, dim)
         assert np.max(np.abs(got[:, :w] - want)) <= 5.1e-11
         assert np.all(got[:, w:] == 0.0)
+
+
+# The reference's CloverLeaf 2D (apps/c/CloverLeaf: 21 files, legacy OPS_ACCn
+# kernels, ops_update_const, the C API's instance object) through the
+# translator as one program.  field_summary.cpp:89-103 compares the final
+# kinetic energy with the known answer of each test problem and prints
+# PASSED within 1e-3 %: problem 1 is clover.in_default (10 x 2 cells, to
+# t = 3), problem 2 clover_bm_short.in (960^2 cells, 87 steps).
+CLOVER_IN = """*clover
+ state 1 density=0.2 energy=1.0
+ state 2 density=1.0 energy=2.5 geometry=rectangle xmin=0.0 xmax=5.0 ymin=0.0 ymax=2.0
+ x_cells={n}
+ y_cells={m}
+ xmin=0.0
+ ymin=0.0
+ xmax=10.0
+ ymax={ymax}
+ initial_timestep=0.04
+ timestep_rise=1.5
+ max_timestep=0.04
+ {stop}
+ test_problem {tp}
+*endclover
+"""
+
+
+@pytest.mark.parametrize("tp", [1, 2])
+def test_cloverleaf_app_through_the_translator(tmp_path, tp):
+    exe = os.path.join(REF, "app_cloverleaf_gen")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    cfg = dict(n=10, m=2, ymax="2.0", stop="end_time=3.0", tp=1) if tp == 1 else \
+        dict(n=960, m=960, ymax="10.0", stop="end_step=87", tp=2)
+    (tmp_path / "clover.in").write_text(CLOVER_IN.format(**cfg))
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    txt = out.stdout + (tmp_path / "clover.out").read_text()
+    assert "This test is considered PASSED" in txt, txt[-3000:]
+    pct = float(re.search(r"within\s+([0-9.eE+-]+)\s*%", txt).group(1))
+    assert pct < 1e-3, pct
