@@ -388,7 +388,8 @@ def find_kernel(name, files):
             bcl = match_close(m, j)
             params = []
             for s, e in split_top(m, op + 1, cl):
-                text = src[s:e].strip()
+                # the parameter as written, without its comments
+                text = " ".join(re.sub(r"//[^\n]*|/\*.*?\*/", " ", src[s:e], flags=re.S).split())
                 pm = re.search(r"([A-Za-z_]\w*)\s*$", m[s:e].strip())
                 if not pm:
                     raise TranslateError(f"{path}: kernel {name}: cannot read parameter {text!r}")

@@ -359,3 +359,45 @@ def test_cloverleaf_app_through_the_translator(tmp_path, tp):
     assert "This test is considered PASSED" in txt, txt[-3000:]
     pct = float(re.search(r"within\s+([0-9.eE+-]+)\s*%", txt).group(1))
     assert pct < 1e-3, pct
+
+
+CLOVER3_IN = """*clover
+ state 1 density=0.2 energy=1.0
+ state 2 density=1.0 energy=2.5 geometry=cuboid xmin=0.0 xmax=5.0 ymin=0.0 ymax=2.0 zmin=0.0 zmax={zmax2}
+ x_cells={n}
+ y_cells={m}
+ z_cells={m}
+ xmin=0.0
+ ymin=0.0
+ zmin=0.0
+ xmax=10.0
+ ymax={ymax}
+ zmax={ymax}
+ initial_timestep={dt}
+ max_timestep={dt}
+ end_step={steps}
+ test_problem {tp}
+ checkpoint_frequency={ckpt}
+*endclover
+"""
+
+
+@pytest.mark.parametrize("tp", [1, 2])
+def test_cloverleaf3d_app_through_the_translator(tmp_path, tp):
+    """CloverLeaf 3D (apps/c/CloverLeaf_3D) the same way: test problem 1
+    (clover.in_default: 10 x 2 x 2 cells, 75 steps) and 2
+    (clover_bm_short.in: 96^3 cells, 87 steps), field_summary.cpp:93-107.
+    The input sets checkpoint_frequency as the reference's inputs do: the
+    main loop takes step % checkpoint_frequency (clover_leaf.cpp:209)."""
+    exe = os.path.join(REF, "app_cloverleaf3d_gen")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    cfg = dict(n=10, m=2, ymax="2.0", zmax2="2.0", dt="0.04", steps=75, tp=1, ckpt=100) if tp == 1 else \
+        dict(n=96, m=96, ymax="10.0", zmax2="4.0", dt="0.1", steps=87, tp=2, ckpt=10)
+    (tmp_path / "clover.in").write_text(CLOVER3_IN.format(**cfg))
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    txt = out.stdout + (tmp_path / "clover.out").read_text()
+    assert "This test is considered PASSED" in txt, txt[-3000:]
+    pct = float(re.search(r"within\s+([0-9.eE+-]+)\s*%", txt).group(1))
+    assert pct < 1e-3, pct
