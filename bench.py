@@ -470,6 +470,58 @@ def bench_gen_tail(a, ops, apps):
                 roofline=roofline(kernels, bpp, pts), kernels=kernels, data="synthetic (numpy PCG64, seeded)")
 
 
+def bench_gen_heat(a, ops, apps):
+    """One step = the TeaLeaf CG operator (w = A p with p.w, 7-point variable
+    coefficient, 40 B/pt) and the residual update (r -= alpha w with r.r,
+    24 B/pt) at 512^3 through translator-generated loops
+    (tests/genapps/heat3d_gen.cpp): at the default fusion level they run as
+    one pass (56 B/pt, w is written and read at the point), with
+    OPSMI_FUSION=0 as two.  Both results are read each step."""
+    import ctypes as C
+
+    import numpy as np
+    lib = os.path.join(ROOT, "tests", "genapps", "libgenapps.so")
+    if not os.path.exists(lib):
+        raise RuntimeError(f"{lib} is not built; run __graft_entry__.build() first")
+    G = C.CDLL(lib)
+    dp = C.POINTER(C.c_double)
+    G.gen_heat_step.argtypes = [C.c_void_p] * 7 + [C.c_int] * 3 + [C.c_double] * 4 + [dp]
+    n = a.size or 512
+    ops.ops_init(diags=2)
+    blk = ops.ops_decl_block(3, "gen_heat")
+    rng = np.random.default_rng(a.seed)
+    D = {}
+    for k in ("w", "Kx", "Ky", "Kz", "p", "r"):
+        d = ops.ops_decl_dat(blk, [n, n, n], name=k)
+        v = rng.random((n + 2, n + 2, n + 2))
+        ops.set_whole(d, v + 0.5 if k.startswith("K") else v - 0.5)
+        D[k] = d
+    out = np.zeros(2)
+    args = [blk.ptr] + [D[k].ptr for k in ("w", "Kx", "Ky", "Kz", "p", "r")] + [n, n, n, 0.3, 0.3, 0.3, 1e-3,
+                                                                                 out.ctypes.data_as(dp)]
+    for _ in range(a.warmup):
+        G.gen_heat_step(*args)
+    ops.ops_mi_sync()
+    ops.ops_mi_reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        G.gen_heat_step(*args)
+    ops.ops_mi_sync()
+    t1 = time.perf_counter()
+    ops._check()
+    ms = (t1 - t0) / a.steps * 1e3
+    pts = float(n) ** 3
+    bpp = {"fused(gen heat_calc_w+gen heat_ur_r)": 56, "gen heat_calc_w": 40, "gen heat_ur_r": 24}
+    kernels = kernel_table(ops, bpp, pts)
+    value = 64.0 * pts / (ms * 1e-3) / 1e9
+    return line(value, "GB/s", ms, a, "f64",
+                {"workload": f"gen_heat_step_{n}", "grid": [n, n, n],
+                 "loops": "TeaLeaf 3D calc_w + ur_r (translator-generated)", "bytes_per_point": 64,
+                 "pw": float(out[0]), "rr": float(out[1])},
+                roofline=roofline(kernels, bpp, pts), kernels=kernels, data="synthetic (numpy PCG64, seeded)",
+                value_basis="64 B/pt: the two loops as written (40 + 24)")
+
+
 # ------------------------------------------------------ stream-mix ceilings
 # (reads, writes) of this path's kernels: the practical HBM ceiling each is
 # measured against (pointwise, 16-byte lanes, no arithmetic or neighbours)
@@ -519,6 +571,7 @@ def bench_stream(a, ops, apps):
 
 WORKLOADS = {"heat3d_cg_512": bench_heat, "poisson_mixed_768": bench_mixed, "laplace2d_2048": bench_laplace,
              "adi_trid_256": bench_trid, "serve_stream": bench_serve, "gen_cg_tail_512": bench_gen_tail,
+             "gen_heat_step_512": bench_gen_heat,
              "stream_mix": bench_stream}
 
 
