@@ -834,6 +834,12 @@ ops_dat ops_decl_dat_char(ops_block block, int dim, int *dsize, int *base, int *
   if (!block) fail(OPS_INVALID_ARGUMENT, "ops_decl_dat: NULL block");
   if (dim <= 0)
     fail(OPS_INVALID_ARGUMENT, cat("Error: ops_decl_dat -- negative/zero number of items per grid point in dataset: ", name));
+  // a stride other than 1 makes a multigrid dat, read through restrict /
+  // prolong stencils, which this backend does not run
+  for (int n = 0; stride && n < block->dims; n++)
+    if (stride[n] != 1)
+      fail(OPS_NOT_IMPLEMENTED, cat("ops_decl_dat: dat ", name ? name : "?", " has stride ", stride[n],
+                                    " in dimension ", n, " (multigrid dats are not supported)"));
   Instance &I = inst();
   ops_dat d = new ops_dat_core;
   memset(d, 0, sizeof(*d));

@@ -641,6 +641,12 @@ def test_validation_errors(gpu_ops):
     ops.ops_arg_reduce(h, 1, "double", ops.OPS_MAX)
     with pytest.raises(ops.OPSError, match="different access type"):
         ops.ops_arg_reduce(h, 1, "double", ops.OPS_MIN)
+    # a multigrid dat (stride 2) is refused at declaration, nothing allocated
+    import ctypes as C
+    ints = lambda v: (C.c_int * len(v))(*v)
+    with pytest.raises(ops.OPSError, match="multigrid"):
+        ops._call("ops_decl_dat_char", blk.ptr, 1, ints([10, 10]), ints([0, 0]), ints([-1, -1]), ints([1, 1]),
+                  ints([2, 1]), None, 8, b"double", b"coarse")
 
 
 @pytest.mark.parametrize("n", [(130, 37, 21), (256, 64, 40), (61, 17, 9), (128, 9, 3)])
