@@ -199,3 +199,31 @@ def test_program_constants_and_globals(tmp_path):
     assert T.defines_globals(open(tmp_path / "vars.h").read())
     assert not T.defines_globals(open(tmp_path / "types.h").read())
     assert not T.defines_globals("extern int q;\nstatic const int k = 3;\nint f(int x);\n")
+
+
+def test_implementation_files_and_commented_parameters(tmp_path):
+    """An included file that defines functions with external linkage (an
+    implementation file the program includes, as CloverLeaf's profile.cpp)
+    stays out of the generated kernel file, which would define them again;
+    inline, static and template definitions and declarations do not count.
+    Comments inside a kernel's parameter list are dropped from the
+    parameter text."""
+    assert T.defines_functions("void process(int n) { n++; }\n")
+    assert T.defines_functions("OPS_FUN_PREFIX double g(double x) { return x; }\n")
+    assert not T.defines_functions("inline double g(double x) { return x; }\nstatic int h() { return 1; }\n"
+                                   "template <class T> T k(T a) { return a; }\nstruct S { int a; void f() {} };\n"
+                                   "double decl(double);\n#define M(x) { x }\n")
+    (tmp_path / "prof.cpp").write_text("#include <stdio.h>\nvoid report() { printf(\"x\\n\"); }\n")
+    (tmp_path / "helpers.h").write_text("static inline OPS_FUN_PREFIX double twice(double v) { return 2.0 * v; }\n")
+    p = tmp_path / "app.cpp"
+    p.write_text('#include "ops_seq.h"\n#include "helpers.h"\n#include "prof.cpp"\n'
+                 'void kc(const double *a, // input\n'
+                 '        double *b /* output */) {\n'
+                 '  b[OPS_ACC1(0,0)] = twice(a[OPS_ACC0(0,0)]); }\n'
+                 'void run(ops_block bl, ops_dat d, ops_dat e, ops_stencil s0, int *r) {\n'
+                 '  ops_par_loop(kc, "kc", bl, 2, r, ops_arg_dat(d, 1, s0, "double", OPS_READ),\n'
+                 '               ops_arg_dat(e, 1, s0, "double", OPS_WRITE));\n  report();\n}\n')
+    T.translate(str(p), str(tmp_path / "out"))
+    hip = open(tmp_path / "out" / "app_opsmi_kernels.hip").read()
+    assert '#include "helpers.h"' in hip and '#include "prof.cpp"' not in hip
+    assert "const OPSMI_A0 &a, const OPSMI_A1 &b) const {" in hip
