@@ -401,3 +401,24 @@ def test_cloverleaf3d_app_through_the_translator(tmp_path, tp):
     assert "This test is considered PASSED" in txt, txt[-3000:]
     pct = float(re.search(r"within\s+([0-9.eE+-]+)\s*%", txt).group(1))
     assert pct < 1e-3, pct
+
+
+def test_shsgc_apps_through_the_translator(tmp_path):
+    """The reference's shock-capturing test apps, unchanged: shsgc
+    (apps/c/shsgc/shsgc.cpp:409-420: solution RMS against 0.233688543536201
+    within 1e-3 %) and its multi-block form
+    (apps/c/mb_shsgc/Max_datatransfer/shsgc.cpp:500-511: post-shock error
+    against 0.0003206 for 200 points)."""
+    exe = os.path.join(REF, "app_shsgc_gen")
+    exe2 = os.path.join(REF, "app_mbshsgc_gen")
+    if not (os.path.exists(exe) and os.path.exists(exe2)):
+        pytest.skip("shsgc apps not built")
+    env = dict(os.environ, OPSMI_ERROR_MODE="2")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120, cwd=tmp_path, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "This run is considered PASSED" in r.stdout, r.stdout[-3000:]
+    pct = float(re.search(r"within ([0-9.eE+-]+) %", r.stdout).group(1))
+    assert pct < 1e-3, pct
+    r = subprocess.run([exe2], capture_output=True, text=True, timeout=120, cwd=tmp_path, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "PASSED" in r.stdout, r.stdout[-3000:]
