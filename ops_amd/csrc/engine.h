@@ -212,9 +212,11 @@ __global__ __launch_bounds__(256) void k_red_final(const T *__restrict__ in, uns
   }
 }
 
-// launches the fold of `parts` stage-1 partials into the handle
+// launches the fold of `parts` stage-1 partials into the handle (none when
+// the host folds them, with_host_fold)
 template <class T, int OP, int D>
 void launch_red_finish(const RedCtl &c, unsigned parts, hipStream_t s) {
+  if (c.hostfold) return;
   if (parts <= 4u * kSpan) {
     hipLaunchKernelGGL((k_red_final<T, OP, D>), dim3(1), dim3(256), 0, s, (const T *)c.partials, parts, c);
   } else {
@@ -417,6 +419,29 @@ constexpr DType dtype_v() {
 inline RedCtl with_scratch(RedCtl c) {
   c.partials = inst().d_partials;
   c.partials2 = inst().d_partials2;
+  c.hostfold = 0;
+  return c;
+}
+
+// with_scratch for a kernel of nblocks workgroups that ends in
+// reduce_block_store<T, OP, dim> in every workgroup: when the handle's result
+// is read right after this flush and the launch qualifies (FP64, at most
+// kHostFoldMax partials), its partials go to the pinned host buffer instead,
+// preset to kHostFoldEmpty, and launch_red_finish launches nothing; the
+// reader folds them (host_fold_result in runtime.cpp)
+inline RedCtl with_host_fold(RedCtl c, unsigned nblocks, int dim, DType dt) {
+  c = with_scratch(c);
+  Instance &I = inst();
+  if (I.hf_slot && !I.hf_used && c.slot == I.hf_slot && dt == DT_F64 && dim >= 1 && dim <= 4 &&
+      (size_t)nblocks * dim <= kHostFoldMax) {
+    unsigned long long *h = (unsigned long long *)I.h_hfold;
+    for (size_t i = 0; i < (size_t)nblocks * dim; i++) h[i] = kHostFoldEmpty;
+    c.partials = I.d_hfold;
+    c.hostfold = 1;
+    I.hf_used = true;
+    I.hf_n = nblocks;
+    I.hf_dim = dim;
+  }
   return c;
 }
 
@@ -456,7 +481,8 @@ LoopRec make_loop(const char *name, const ops_arg *args, int nargs, const Range3
   rec.ctl = ctl;
   rec.launch = [=](hipStream_t s) {
     const F f = make();
-    const RedCtl c = with_scratch(ctl);
+    const RedCtl c = F::red_dim > 0 ? with_host_fold(ctl, t.blocks(), F::red_dim, dtype_v<typename F::red_t>())
+                                    : with_scratch(ctl);
     if (vec) {
       if constexpr (V > 1) {
         auto kern = ntl ? k_loop_v<F, V, true> : k_loop_v<F, V, false>;

@@ -88,7 +88,7 @@ int cg_chain_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
   c.u = U ? ptr_of<double>(U) : Ptr<double>{nullptr, 0, 0};
   const double alpha_u = ax ? ax->g[0] : 0.0, a = bp.g[0], b = bp.g[1];
   const double rx = cw.g[0], ry = cw.g[1], rz = cw.g[2];
-  const RedCtl ctl = with_scratch(cw.ctl);
+  const RedCtl ctl = with_host_fold(cw.ctl, g.blocks(), 1, DT_F64);
   const double bytes = (ax ? ax->bytes : 0.0) + bp.bytes + cw.bytes;
   run_loop(ax ? "cg_chain(axpy+axpby+calc_w)" : "cg_chain(axpby+calc_w)", bytes, [&](hipStream_t st) {
     hipLaunchKernelGGL(ax ? k_cg_chain<true> : k_cg_chain<false>, dim3(g.gx, g.gy, g.gz), dim3(kWave, kHeatRows), 0,
@@ -209,7 +209,7 @@ int pois_chain_rule_t(std::vector<LoopRec> &run, size_t at) {
   c.w = ptr_of<T>(W);
   c.u = U ? ptr_of<T>(U) : Ptr<T>{nullptr, 0, 0};
   const T alpha_u = ax ? (T)ax->g[0] : (T)0, a = (T)bp.g[0], b = (T)bp.g[1];
-  const RedCtl ctl = with_scratch(cw.ctl);
+  const RedCtl ctl = with_host_fold(cw.ctl, g.blocks(), 1, DT_F64);
   const double bytes = (ax ? ax->bytes : 0.0) + bp.bytes + cw.bytes;
   const char *nm = MODE == 0 ? (ax ? "poisson_chain(axpy+axpby+calc_w)" : "poisson_chain(axpby+calc_w)")
                              : (ax ? "mp_chain(e+p+calc_w)" : "mp_chain(p+calc_w)");
@@ -366,7 +366,7 @@ void launch_jacobi_chain(const LoopRec &cp, const LoopRec &jac, const Range3 &R)
   c.out = Ptr<T>{(T *)spare_base(An, R), v.sy, v.sz};
   c.A = ptr_of<T>(A);
   c.An = ptr_of<const T>(An);
-  const RedCtl ctl = with_scratch(jac.ctl);
+  const RedCtl ctl = with_host_fold(jac.ctl, gx * gy, 1, sizeof(T) == 8 ? DT_F64 : DT_F32);
   run_loop("jacobi_chain(copy+apply_stencil)", cp.bytes + jac.bytes, [&](hipStream_t st) {
     // every tile whole: the branch-free form
     auto kern = (R.x1 - R.x0) % (kWave * V) == 0 ? k_jacobi_chain<T, V, true> : k_jacobi_chain<T, V, false>;

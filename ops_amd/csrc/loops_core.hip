@@ -269,7 +269,7 @@ LoopRec make_heat3_loop(const char *name, const ops_arg *args, int nargs, const 
   if (!INIT) ctl = red_prepare(*red, 1, DT_F64, g.blocks());
   rec.ctl = ctl;
   rec.launch = [=](hipStream_t s) {
-    const RedCtl c = with_scratch(ctl);
+    const RedCtl c = INIT ? with_scratch(ctl) : with_host_fold(ctl, g.blocks(), 1, DT_F64);
     Ptr<double> rp = INIT ? ptr_of<double>(rr.dat) : Ptr<double>{nullptr, 0, 0};
     hipLaunchKernelGGL(vec ? k_heat3_march_v<INIT> : k_heat3_march<INIT>, dim3(g.gx, g.gy, g.gz),
                        dim3(kWave, vec ? kHeatRows : kWavesPerBlock), 0, s, ptr_of<double>(w.dat), rp,
@@ -371,7 +371,7 @@ LoopRec make_poisson3_loop(const char *name, const ops_arg *args, int nargs, con
   const RedCtl ctl = red_prepare(*red, 1, DT_F64, g.blocks());
   rec.ctl = ctl;
   rec.launch = [=](hipStream_t s) {
-    const RedCtl c = with_scratch(ctl);
+    const RedCtl c = with_host_fold(ctl, g.blocks(), 1, DT_F64);
     hipLaunchKernelGGL((k_poisson3_march<T, pois_pf<T>()>), dim3(g.gx, g.gy, g.gz), dim3(kWave, kHeatRows), 0, s,
                        ptr_of<T>(w.dat), ptr_of<const T>(p.dat), r, g.chunk, c);
     launch_red_finish<double, OPS_INC, 1>(c, g.blocks(), s);

@@ -12,6 +12,7 @@
 #include <time.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -640,7 +641,28 @@ void flush_for(ops_reduction h) {
   std::vector<LoopRec> run, rest;
   for (long i = 0; i < n; i++) (need[i] ? run : rest).push_back(std::move(g_queue[i]));
   g_queue.swap(rest);
+  // the handle's result is read right after this flush: when one loop of
+  // the run reduces into it, with no other handle, starting from the OPS
+  // initial value, its partials may be folded by the host
+  // (with_host_fold, host_fold_result)
+  Instance &I = inst();
+  const LoopRec *only = nullptr;
+  int count = 0;
+  for (const LoopRec &L : run)
+    if (reduces_into(L, h)) {
+      count++;
+      only = &L;
+    }
+  RedImpl *ri = (RedImpl *)h->impl;
+  I.hf_used = false;
+  I.hf_slot = nullptr;
+  if (I.host_fold && count == 1 && only->red == h && only->more_reds.empty() && only->ctl.use_init && ri->dt == DT_F64 &&
+      only->ctl.slot == ri->d_slot) {
+    I.hf_slot = ri->d_slot;
+    memcpy(I.hf_init, only->ctl.init, sizeof(I.hf_init));
+  }
   execute(run);
+  I.hf_slot = nullptr;
 }
 
 static bool same_range6(const int *a, const Range3 &r) {
@@ -742,6 +764,8 @@ void ops_init(const int argc, const char *const argv[], const int diags_level) {
   ensure_partials((size_t)1 << 20);
   OPSMI_HIP(hipMalloc(&I.d_partials2, (size_t)1 << 20));
   OPSMI_HIP(hipHostMalloc(&I.h_pinned, 4096));
+  OPSMI_HIP(hipHostMalloc((void **)&I.h_hfold, 8 * 4096, hipHostMallocMapped | hipHostMallocCoherent));
+  OPSMI_HIP(hipHostGetDevicePointer(&I.d_hfold, I.h_hfold, 0));
   I.initialised = true;
   OPSMI_API_END()
 }
@@ -800,6 +824,11 @@ void ops_exit(void) {
   (void)hipFree(I.d_partials2);
   I.d_partials2 = nullptr;
   (void)hipHostFree(I.h_pinned);
+  (void)hipHostFree(I.h_hfold);
+  I.h_hfold = nullptr;
+  I.d_hfold = nullptr;
+  I.hf_slot = nullptr;
+  I.hf_used = false;
   if (I.ev_wait) (void)hipEventDestroy(I.ev_wait);
   I.ev_wait = nullptr;
   (void)hipStreamDestroy(I.stream);
@@ -1154,6 +1183,76 @@ void ops_execute(void) {
   OPSMI_API_END()
 }
 
+// The device's final fold (k_red_final over n partials of dim components,
+// engine.h) restated on the host in the same order, so the result has the
+// same bits: for each component, "thread" t < 256 combines partials t,
+// t + 256, ... in that order, each wave of 64 such values folds by xor
+// butterflies (offsets 32 .. 1), the four wave values fold in order, and the
+// result combines with the initial value.  The loops run across the
+// independent threads and lanes, which leaves every value's order of
+// operations as it is.
+extern "C++" {
+template <int OP>
+static inline double fold_op(double a, double b) {
+  return OP == OPS_MIN ? std::fmin(a, b) : (OP == OPS_MAX ? std::fmax(a, b) : a + b);
+}
+template <int OP>
+static double host_fold(const double *in, unsigned n, int dim, int q, double init) {
+  const double ident = OP == OPS_MIN ? HUGE_VAL : (OP == OPS_MAX ? -HUGE_VAL : 0.0);
+  double v[256];
+  for (int t = 0; t < 256; t++) v[t] = ident;
+  unsigned i0 = 0;
+  for (; i0 + 256 <= n; i0 += 256)
+    for (int t = 0; t < 256; t++) v[t] = fold_op<OP>(v[t], in[(size_t)(i0 + t) * dim + q]);
+  for (unsigned t = 0; i0 + t < n; t++) v[t] = fold_op<OP>(v[t], in[(size_t)(i0 + t) * dim + q]);
+  double sh[4];
+  for (int w = 0; w < 4; w++) {
+    double a[64], b[64];
+    for (int l = 0; l < 64; l++) a[l] = v[64 * w + l];
+    for (int off = 32; off > 0; off >>= 1) {
+      for (int l = 0; l < 64; l++) b[l] = fold_op<OP>(a[l], a[l ^ off]);
+      for (int l = 0; l < 64; l++) a[l] = b[l];
+    }
+    sh[w] = a[0];
+  }
+  double sum = sh[0];
+  for (int w = 1; w < 4; w++) sum = fold_op<OP>(sum, sh[w]);
+  return fold_op<OP>(init, sum);
+}
+}  // extern "C++"
+static void host_fold_result(ops_reduction handle) {
+  Instance &I = inst();
+  const unsigned n = I.hf_n;
+  const int dim = I.hf_dim;
+  const size_t total = (size_t)n * dim;
+  // the partials arrive as the workgroups finish; the stream is waited for
+  // after 50 ms, and a partial still missing then is an error
+  const volatile unsigned long long *raw = (const volatile unsigned long long *)I.h_hfold;
+  const auto t0 = std::chrono::steady_clock::now();
+  size_t k = 0;
+  bool waited = false;
+  while (k < total) {
+    if (raw[k] != kHostFoldEmpty) {
+      k++;
+      continue;
+    }
+    if (waited) fail(OPS_RUNTIME_ERROR, cat("reduction ", handle->name, ": a workgroup partial never arrived"));
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) {
+      wait_stream(I.stream);
+      waited = true;
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  const double *in = (const double *)I.h_hfold;
+  double res[4];
+  for (int q = 0; q < dim; q++) {
+    if (handle->acc == OPS_MIN) res[q] = host_fold<OPS_MIN>(in, n, dim, q, I.hf_init[q]);
+    else if (handle->acc == OPS_MAX) res[q] = host_fold<OPS_MAX>(in, n, dim, q, I.hf_init[q]);
+    else res[q] = host_fold<OPS_INC>(in, n, dim, q, I.hf_init[q]);
+  }
+  memcpy(handle->data, res, sizeof(double) * dim);
+}
+
 void ops_reduction_result_char(ops_reduction handle, int type_size, char *ptr) {
   OPSMI_API_BEGIN
   (void)type_size;
@@ -1161,7 +1260,10 @@ void ops_reduction_result_char(ops_reduction handle, int type_size, char *ptr) {
   Instance &I = inst();
   flush_for(handle);
   RedImpl *ri = (RedImpl *)handle->impl;
-  if (ri->dev_valid) {
+  if (I.hf_used) {
+    I.hf_used = false;
+    host_fold_result(handle);
+  } else if (ri->dev_valid) {
     // the last fold into the handle stores its sequence number after the
     // values: poll it (no event round trip); after 50 ms, or if it never
     // comes, wait for the stream instead
@@ -1646,6 +1748,8 @@ void ops_mi_set_error_mode(int mode) { inst().error_mode = mode; }
 void ops_set_soa(const int soa_val) { inst().soa = soa_val ? 1 : 0; }
 
 int ops_mi_get_diags(void) { return inst().diags; }
+
+void ops_mi_set_host_fold(int on) { inst().host_fold = on != 0; }
 
 void ops_mi_set_diags(int level) {
   OPSMI_API_BEGIN

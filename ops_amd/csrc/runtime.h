@@ -152,6 +152,20 @@ struct Instance {
   // host staging for reduction results
   void *h_pinned = nullptr;
 
+  // Host fold of a result read right away: the one loop of a flush that
+  // reduces into the handle being read (FP64, first use since the last
+  // read) writes its workgroup partials into this pinned, device-mapped
+  // buffer (kHostFoldMax values, each preset to a sentinel NaN), and the
+  // host folds them in the device fold's order; no fold kernel runs
+  double *h_hfold = nullptr;
+  void *d_hfold = nullptr;
+  const void *hf_slot = nullptr;  // device slot of the armed handle (null: none armed)
+  bool hf_used = false;           // the armed handle's loop took the host fold
+  unsigned hf_n = 0;              // partials of that launch
+  int hf_dim = 1;
+  double hf_init[4] = {0, 0, 0, 0};  // the handle's initial values
+  bool host_fold = true;             // ops_mi_set_host_fold
+
   // random fills
   uint64_t rng_seed = 0;
   bool rng_init = false;
@@ -233,6 +247,12 @@ int const_as_int(const char *name);
 
 // ---------------------------------------------------------------- reductions
 // Device-side control block passed to reducing kernels.
+// partials one host fold takes (workgroups x components), and the bit
+// pattern the host presets each to (a signalling NaN no arithmetic
+// produces): a partial has arrived when it differs
+constexpr unsigned kHostFoldMax = 4096;
+constexpr unsigned long long kHostFoldEmpty = 0x7ff4dead5eed0001ULL;
+
 struct RedCtl {
   void *slot;          // handle's device value
   void *mirror;        // host-mapped copy of it (written by the final fold)
@@ -243,6 +263,7 @@ struct RedCtl {
   int init_i[4];
   unsigned long long *seq;     // host-mapped sequence word (null: none)
   unsigned long long seq_val;  // stored there after the values
+  int hostfold;                // 1: partials go to host memory and the host folds them
 };
 // Prepares a handle for a launch (device slot, init values).  Marks the
 // handle's device value as live; the result is copied back at result time.

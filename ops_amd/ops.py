@@ -117,6 +117,7 @@ def lib():
             "ops_mi_reset_stats": (None, []),
             "ops_mi_set_fusion": (None, [ci]),
             "ops_mi_set_diags": (None, [ci]),
+            "ops_mi_set_host_fold": (None, [ci]),
             "ops_mi_stream_probe": (ci, [ci, ci, ci, C.c_long, ci, C.POINTER(cd)]),
             "ops_set_soa": (None, [ci]),
         }
@@ -427,6 +428,11 @@ def ops_mi_reset_stats():
 def ops_mi_set_diags(level):
     """diagnostics level: above 1 every launch is timed with HIP events"""
     _call("ops_mi_set_diags", level)
+
+
+def ops_mi_set_host_fold(on):
+    """1: results read right after their one loop are folded by the host (same bits)"""
+    _call("ops_mi_set_host_fold", 1 if on else 0)
 
 
 def ops_mi_set_fusion(level):

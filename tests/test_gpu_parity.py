@@ -219,6 +219,27 @@ def test_heat_cg_app_inputs_match_oracle_generator(apps):
     assert relerr(hist["alpha"], ref["alpha"]) < RTOL_RED
 
 
+@pytest.mark.parametrize("n", [(18, 14, 10), (300, 40, 37), (512, 64, 70)])
+def test_host_fold_keeps_the_bits(gpu_ops, apps, n):
+    """A result read right after its one loop is folded by the host from the
+    workgroup partials in the fold kernel's order: the CG scalars (p.w from
+    the fused chain, r.r from the residual update, at grids of up to 4096
+    partials and beyond) have the same bits with the host fold on and off."""
+    ops = gpu_ops
+    out = []
+    for on in (1, 0):
+        ops.ops_mi_set_host_fold(on)
+        try:
+            h = apps.Heat(3, *n, seed=7)
+            k, hist = h.iterate(6, history=True)
+            h.close()
+        finally:
+            ops.ops_mi_set_host_fold(1)
+        out.append(hist)
+    for name in out[0]:
+        assert np.array_equal(np.asarray(out[0][name]), np.asarray(out[1][name])), name
+
+
 # ------------------------------------------------------------------ stencils
 def _block3(ops, n, name):
     blk = ops.ops_decl_block(3, name)
