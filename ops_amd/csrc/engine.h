@@ -429,18 +429,25 @@ inline RedCtl with_scratch(RedCtl c) {
 // kHostFoldMax partials), its partials go to the pinned host buffer instead,
 // preset to kHostFoldEmpty, and launch_red_finish launches nothing; the
 // reader folds them (host_fold_result in runtime.cpp)
+// claims the host fold for a launch reducing through ctl (true: its
+// partials go to I.d_hfold, preset here)
+inline bool host_fold_claim(const RedCtl &c, unsigned nblocks, int dim, DType dt) {
+  Instance &I = inst();
+  if (!I.hf_slot || I.hf_used || c.slot != I.hf_slot || dt != DT_F64 || dim < 1 || dim > 4 ||
+      (size_t)nblocks * dim > kHostFoldMax)
+    return false;
+  unsigned long long *h = (unsigned long long *)I.h_hfold;
+  for (size_t i = 0; i < (size_t)nblocks * dim; i++) h[i] = kHostFoldEmpty;
+  I.hf_used = true;
+  I.hf_n = nblocks;
+  I.hf_dim = dim;
+  return true;
+}
 inline RedCtl with_host_fold(RedCtl c, unsigned nblocks, int dim, DType dt) {
   c = with_scratch(c);
-  Instance &I = inst();
-  if (I.hf_slot && !I.hf_used && c.slot == I.hf_slot && dt == DT_F64 && dim >= 1 && dim <= 4 &&
-      (size_t)nblocks * dim <= kHostFoldMax) {
-    unsigned long long *h = (unsigned long long *)I.h_hfold;
-    for (size_t i = 0; i < (size_t)nblocks * dim; i++) h[i] = kHostFoldEmpty;
-    c.partials = I.d_hfold;
+  if (host_fold_claim(c, nblocks, dim, dt)) {
+    c.partials = inst().d_hfold;
     c.hostfold = 1;
-    I.hf_used = true;
-    I.hf_n = nblocks;
-    I.hf_dim = dim;
   }
   return c;
 }

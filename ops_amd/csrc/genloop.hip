@@ -111,6 +111,10 @@ struct GenLoop {
         L.dat[i].cs = v.cs;
       }
     for (const GenRed &g : reds) L.partials[g.arg] = pbase + g.offset;
+    // a loop with one reduction whose result is read right after this
+    // flush: the host folds its partials (with_host_fold in engine.h)
+    if (reds.size() == 1 && host_fold_claim(reds[0].ctl, parts(), reds[0].dim, reds[0].dt))
+      L.partials[reds[0].arg] = (char *)inst().d_hfold;
   }
   const void *blob() const { return data.empty() ? nullptr : data.data(); }
   // the identity fills of edge dats this loop accumulates into afresh
@@ -118,6 +122,7 @@ struct GenLoop {
   // folds each reduction argument's partials into its handle
   void finish_reds(char *pbase, hipStream_t s) const {
     for (const GenRed &g : reds) {
+      if (reds.size() == 1 && inst().hf_used && g.ctl.slot == inst().hf_slot) continue;  // host fold
       RedCtl c = with_scratch(g.ctl);
       c.partials = pbase + g.offset;
       finish(g.dt, c, g.op, g.dim, parts(), s);

@@ -672,3 +672,31 @@ def test_legacy_multi_component_kernels_3d(gpu_ops, gen, n, soa):
     assert np.array_equal(w_leg, w_acc) and s_leg == s_acc
     assert np.array_equal(w_leg, w_ref)
     assert abs(s_leg - s_ref) <= 1e-12 * max(1.0, np.abs(w_ref * (X + 2 * Y + 3 * Z)).sum())
+
+
+@pytest.mark.parametrize("n", [(40, 30, 20), (300, 50, 40)])
+def test_generated_host_fold_keeps_the_bits(gpu_ops, gen, n):
+    """The generated calc_w and residual update with each result read after
+    its loop: the host fold of the workgroup partials gives the same bits as
+    the fold kernel (ops_mi_set_host_fold(0))."""
+    ops = gpu_ops
+    nx, ny, nz = n
+    blk = ops.ops_decl_block(3, "genhf")
+    names = ["w", "Kx", "Ky", "Kz", "p", "r"]
+    D = {k: ops.ops_decl_dat(blk, [nx, ny, nz], name=k) for k in names}
+    ext = (nz + 2, ny + 2, nx + 2)
+    res = []
+    for on in (1, 0):
+        for s, k in enumerate(["Kx", "Ky", "Kz", "p", "r"]):
+            ops.set_whole(D[k], O.u01(93, s, int(np.prod(ext))).reshape(ext) + (1.0 if k[0] == "K" else -0.5))
+        ops.ops_mi_set_host_fold(on)
+        try:
+            out = np.zeros(2)
+            gen.gen_heat_step(blk.ptr, *[D[k].ptr for k in names], nx, ny, nz, 0.3, 0.7, 1.1, 0.37, _p(out))
+            ops._check()
+        finally:
+            ops.ops_mi_set_host_fold(1)
+        res.append((out.copy(), ops.ops_dat_fetch_data(D["r"])))
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+    for k in names:
+        ops.ops_free_dat(D[k])
