@@ -595,7 +595,8 @@ int ops_mi_get_diags(void);
 /* 1 (default): a result read right after the one loop reducing into it
  * (FP64, first use since the last read, at most 4096 partials) is folded by
  * the host from partials the loop writes to pinned memory, with the same
- * bits as the fold kernel; 0: always the fold kernel */
+ * bits as the fold kernel; 0: always the fold kernel.  Initial value from
+ * OPSMI_HOST_FOLD (read by ops_init) */
 void ops_mi_set_host_fold(int on);
 /* last error code (0 = none) and message; reading clears it */
 int ops_mi_last_error(char *msg, int msg_len);

@@ -755,6 +755,8 @@ void ops_init(const int argc, const char *const argv[], const int diags_level) {
   I.num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   const char *fz = getenv("OPSMI_FUSION");
   I.fusion = fz ? atoi(fz) : 2;
+  const char *hf = getenv("OPSMI_HOST_FOLD");
+  I.host_fold = hf ? atoi(hf) != 0 : true;
   if (tiling) I.fusion = 2;
   if (I.diags > 0 && (tiling || I.block_y_set() || I.block_z_set()))
     printf(" OPS (MI355X): loop-chain fusion level %d, generated-loop workgroups %d rows (2D) / %d planes (3D)\n",
