@@ -345,13 +345,16 @@ CLOVER_IN = """*clover
 """
 
 
-@pytest.mark.parametrize("tp", [1, 2])
+@pytest.mark.parametrize("tp", [1, 2, 3])
 def test_cloverleaf_app_through_the_translator(tmp_path, tp):
+    """Test problem 3 is the benchmark input clover_bm.in (960^2 cells to
+    t = 15.5 or 2955 steps); the run's wall clock is printed."""
     exe = os.path.join(REF, "app_cloverleaf_gen")
     if not os.path.exists(exe):
         pytest.skip(f"{exe} not built")
-    cfg = dict(n=10, m=2, ymax="2.0", stop="end_time=3.0", tp=1) if tp == 1 else \
-        dict(n=960, m=960, ymax="10.0", stop="end_step=87", tp=2)
+    cfg = {1: dict(n=10, m=2, ymax="2.0", stop="end_time=3.0", tp=1),
+           2: dict(n=960, m=960, ymax="10.0", stop="end_step=87", tp=2),
+           3: dict(n=960, m=960, ymax="10.0", stop="end_time=15.5\n end_step=2955", tp=3)}[tp]
     (tmp_path / "clover.in").write_text(CLOVER_IN.format(**cfg))
     out = subprocess.run([exe], capture_output=True, text=True, timeout=120, cwd=tmp_path)
     assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
@@ -359,6 +362,8 @@ def test_cloverleaf_app_through_the_translator(tmp_path, tp):
     assert "This test is considered PASSED" in txt, txt[-3000:]
     pct = float(re.search(r"within\s+([0-9.eE+-]+)\s*%", txt).group(1))
     assert pct < 1e-3, pct
+    wall = re.findall(r"Total Wall time\s+([0-9.]+)", txt)[-1]
+    print(f"CloverLeaf 2D test problem {tp}: wall clock {wall} s")
 
 
 CLOVER3_IN = """*clover
@@ -376,24 +381,28 @@ CLOVER3_IN = """*clover
  initial_timestep={dt}
  max_timestep={dt}
  end_step={steps}
+ end_time={end_time}
  test_problem {tp}
  checkpoint_frequency={ckpt}
 *endclover
 """
 
 
-@pytest.mark.parametrize("tp", [1, 2])
+@pytest.mark.parametrize("tp", [1, 2, 3])
 def test_cloverleaf3d_app_through_the_translator(tmp_path, tp):
     """CloverLeaf 3D (apps/c/CloverLeaf_3D) the same way: test problem 1
     (clover.in_default: 10 x 2 x 2 cells, 75 steps) and 2
-    (clover_bm_short.in: 96^3 cells, 87 steps), field_summary.cpp:93-107.
+    (clover_bm_short.in: 96^3 cells, 87 steps) and 3 (clover_bm.in: 96^3,
+    2955 steps), field_summary.cpp:93-107; the wall clock is printed.
     The input sets checkpoint_frequency as the reference's inputs do: the
     main loop takes step % checkpoint_frequency (clover_leaf.cpp:209)."""
     exe = os.path.join(REF, "app_cloverleaf3d_gen")
     if not os.path.exists(exe):
         pytest.skip(f"{exe} not built")
-    cfg = dict(n=10, m=2, ymax="2.0", zmax2="2.0", dt="0.04", steps=75, tp=1, ckpt=100) if tp == 1 else \
-        dict(n=96, m=96, ymax="10.0", zmax2="4.0", dt="0.1", steps=87, tp=2, ckpt=10)
+    cfg = {1: dict(n=10, m=2, ymax="2.0", zmax2="2.0", dt="0.04", steps=75, end_time="10.0", tp=1, ckpt=100),
+           2: dict(n=96, m=96, ymax="10.0", zmax2="4.0", dt="0.1", steps=87, end_time="1000.1", tp=2, ckpt=10),
+           3: dict(n=96, m=96, ymax="10.0", zmax2="4.0", dt="0.1", steps=2955, end_time="1000.1", tp=3,
+                   ckpt=100)}[tp]
     (tmp_path / "clover.in").write_text(CLOVER3_IN.format(**cfg))
     out = subprocess.run([exe], capture_output=True, text=True, timeout=120, cwd=tmp_path)
     assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
@@ -401,6 +410,8 @@ def test_cloverleaf3d_app_through_the_translator(tmp_path, tp):
     assert "This test is considered PASSED" in txt, txt[-3000:]
     pct = float(re.search(r"within\s+([0-9.eE+-]+)\s*%", txt).group(1))
     assert pct < 1e-3, pct
+    wall = re.findall(r"Total Wall time\s+([0-9.]+)", txt)[-1]
+    print(f"CloverLeaf 3D test problem {tp}: wall clock {wall} s")
 
 
 def test_shsgc_apps_through_the_translator(tmp_path):
