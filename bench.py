@@ -591,7 +591,7 @@ def main():
                     help="first run a 1 GiB copy (the calibration kernel of PMC byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+        print(f"bench.py: the backend runs on one GPU; --gpus {a.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
 

@@ -1,6 +1,6 @@
 """bench.py's contract, checked without a GPU: every ops.* / apps.* name it
 calls exists in the bindings, every workload is registered, and --gpus N > 1
-exits with a message (this build runs on one GPU)."""
+exits with a message (the backend runs on one GPU)."""
 import ast
 import os
 import subprocess
