@@ -522,6 +522,40 @@ def bench_gen_heat(a, ops, apps):
                 value_basis="64 B/pt: the two loops as written (40 + 24)")
 
 
+def bench_gen_overhead(a, ops, apps):
+    """One step = one small generated loop (a 64 x 64 fill): the per-loop
+    cost of the generated path when the grid is too small to matter (the
+    stub, the queue, the launch), against the kernel's own time from HIP
+    events in a second pass."""
+    import ctypes as C
+    lib = os.path.join(ROOT, "tests", "genapps", "libgenapps.so")
+    if not os.path.exists(lib):
+        raise RuntimeError(f"{lib} is not built; run __graft_entry__.build() first")
+    G = C.CDLL(lib)
+    G.gen_fill2d_many.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int]
+    ops.ops_init(diags=1)
+    blk = ops.ops_decl_block(2, "gen_ovh")
+    d = ops.ops_decl_dat(blk, [64, 64], name="f")
+    G.gen_fill2d_many(blk.ptr, d.ptr, 64, 64, max(a.warmup, 1))
+    ops.ops_mi_sync()
+    reps = max(a.steps, 1) * 1000
+    t0 = time.perf_counter()
+    G.gen_fill2d_many(blk.ptr, d.ptr, 64, 64, reps)
+    ops.ops_mi_sync()
+    t1 = time.perf_counter()
+    us = (t1 - t0) / reps * 1e6
+    ops.ops_mi_set_diags(2)
+    ops.ops_mi_reset_stats()
+    G.gen_fill2d_many(blk.ptr, d.ptr, 64, 64, 1000)
+    ops.ops_mi_sync()
+    st = ops.ops_mi_loop_stats("fill2d many")
+    kern_us = st[1] / st[0] * 1e3 if st and st[0] else None
+    ops._check()
+    return line(1e6 / us, "loops/s", us * 1e-3, a, "f64",
+                {"workload": "gen_loop_overhead", "grid": [64, 64], "loop": "generated 2D fill, 1000 per step"},
+                us_per_loop=round(us, 2), kernel_us_by_events=round(kern_us, 2) if kern_us else None)
+
+
 # ------------------------------------------------------ stream-mix ceilings
 # (reads, writes) of this path's kernels: the practical HBM ceiling each is
 # measured against (pointwise, 16-byte lanes, no arithmetic or neighbours)
@@ -571,7 +605,7 @@ def bench_stream(a, ops, apps):
 
 WORKLOADS = {"heat3d_cg_512": bench_heat, "poisson_mixed_768": bench_mixed, "laplace2d_2048": bench_laplace,
              "adi_trid_256": bench_trid, "serve_stream": bench_serve, "gen_cg_tail_512": bench_gen_tail,
-             "gen_heat_step_512": bench_gen_heat,
+             "gen_heat_step_512": bench_gen_heat, "gen_loop_overhead": bench_gen_overhead,
              "stream_mix": bench_stream}
 
 

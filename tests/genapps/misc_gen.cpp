@@ -95,6 +95,21 @@ extern "C" int gen_fill2d(ops_block block, ops_dat a, int nx, int ny, double val
   return 0;
 }
 
+// reps fills of a in a row, then the queue runs: the per-loop cost of a
+// small generated loop (host work and launch) when the grid is small
+extern "C" int gen_fill2d_many(ops_block block, ops_dat a, int nx, int ny, int reps) {
+  int s0[] = {0, 0};
+  ops_stencil S0 = ops_decl_stencil(2, 1, s0, "fm0");
+  int inner[] = {0, nx, 0, ny};
+  for (int k = 0; k < reps; k++) {
+    double val = k;
+    ops_par_loop(fill2d, "fill2d many", block, 2, inner, ops_arg_dat(a, 1, S0, "double", OPS_WRITE), ops_arg_idx(),
+                 ops_arg_gbl(&val, 1, "double", OPS_READ));
+  }
+  ops_execute();
+  return 0;
+}
+
 void double_it(ACC<double> &d, const ACC<double> &s) {
   d(0, 0, 0) = 2.0 * s(0, 0, 0);
 }
