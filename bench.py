@@ -199,6 +199,15 @@ FP64_BPP = {"poisson_calc_w_reduce_kernel": 16, "tea_leaf_axpy_kernel": 24,
             "poisson_chain(axpy+axpby+calc_w)": 48}
 
 
+def pois_roofline(kernels, bpp, pts, n):
+    """roofline of the dominant chain, with its HBM traffic per launch from
+    the PMC profile at 768^3"""
+    rl = roofline(kernels, bpp, pts)
+    if n == 768:
+        rl["traffic"] = profiled_traffic("poisson_mixed_768_hbm_traffic.md", rl["kernel"])
+    return rl
+
+
 def measure_mixed(ops, apps, n, seed):
     """768^3 Poisson to 1e-10: FP64 CG, then FP64 refinement around FP32 PCG,
     each timed from resident inputs to the converged solution."""
@@ -235,10 +244,10 @@ def measure_mixed(ops, apps, n, seed):
     del xm, x64
     return {"workload": f"poisson_mixed_{n}", "grid": [n, n, n], "tol": tol,
             "fp64_cg": {"iterations": it64, "seconds": round(t64, 3), "solves_per_s": round(1.0 / t64, 4),
-                        "roofline": roofline(k64, FP64_BPP, pts), "kernels": k64},
+                        "roofline": pois_roofline(k64, FP64_BPP, pts, n), "kernels": k64},
             "mixed": {"outer": res["outer"], "inner_total": res["inner_total"],
                       "relres": [float(v) for v in res["relres"]], "seconds": round(tmp, 3),
-                      "solves_per_s": round(1.0 / tmp, 4), "roofline": roofline(kmp, MIXED_BPP, pts),
+                      "solves_per_s": round(1.0 / tmp, 4), "roofline": pois_roofline(kmp, MIXED_BPP, pts, n),
                       "kernels": kmp},
             "speedup_vs_fp64": round(t64 / tmp, 3), "max_rel_diff_vs_fp64_cg": err}
 
