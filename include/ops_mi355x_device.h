@@ -35,7 +35,7 @@ constexpr int kRows = 4;
 
 struct Loop {
   int x0, x1, y0, y1, z0, z1;
-  int chunk, three_d;
+  int chunk, three_d, spread;
 };
 
 inline Loop loop_of(const opsmi_gen_launch *L) {
@@ -48,6 +48,7 @@ inline Loop loop_of(const opsmi_gen_launch *L) {
   l.z1 = L->range[5];
   l.chunk = L->chunk;
   l.three_d = L->three_d;
+  l.spread = L->spread;
   return l;
 }
 
@@ -235,31 +236,48 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
   return k * q8 + (k < r8 ? k : r8) + (b >> 3);
 }
 
-// calls f(x, y, z) for this thread's points: OPSMI_GEN_XP points along x,
-// 64 apart, per row (a loop writes a dat only at the point itself, so the
-// points are independent and, with restrict-qualified dat arguments, the
-// compiler overlaps their loads)
-template <class F>
+// calls f(x, y, z) for this thread's points: OPSMI_GEN_XP points along x
+// per row, 64 apart (ADJ false) or adjacent (ADJ true: with restrict-
+// qualified dat arguments the compiler merges a lane's accesses of one array
+// into one 16-byte access; a loop without reductions uses it, since the
+// order of a reduction's per-thread sums follows the point layout and must
+// not differ from that of the fused pair kernels, which keep the 64-apart
+// form).  A loop writes a dat only at the point itself, so the points are
+// independent and, with restrict-qualified arguments, the compiler overlaps
+// their loads.
+template <bool ADJ = false, class F>
+__device__ __forceinline__ void for_each_point(const Loop &L, F &&f);
+template <bool ADJ, class F>
+__device__ __forceinline__ void for_each_point_as(const Loop &L, F &&f);
+
+template <bool ADJ, class F>
 __device__ __forceinline__ void for_each_point(const Loop &L, F &&f) {
+  if (ADJ && L.spread) for_each_point_as<false>(L, f);
+  else for_each_point_as<ADJ>(L, f);
+}
+
+template <bool ADJ, class F>
+__device__ __forceinline__ void for_each_point_as(const Loop &L, F &&f) {
   constexpr int XP = OPSMI_GEN_XP;
+  constexpr int STEP = ADJ ? 1 : kWave;  // x distance between a lane's points
   const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
   const unsigned l = xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nb);
   const unsigned tx = l % gridDim.x, rest = l / gridDim.x;
   const unsigned ty = rest % gridDim.y, tz = rest / gridDim.y;
-  const int x = L.x0 + (int)tx * kWave * XP + (int)threadIdx.x;
+  const int x = L.x0 + (int)tx * kWave * XP + (int)threadIdx.x * (ADJ ? XP : 1);
   if (x >= L.x1) return;
   // a lane whose points all lie in the range calls f for them with no branch
   // in between (a branch join makes the compiler wait for every load in
   // flight, which would serialise the points)
-  const bool whole = x + (XP - 1) * kWave < L.x1;
+  const bool whole = x + (XP - 1) * STEP < L.x1;
   auto row = [&](int y, int z) {
     if (whole) {
 #pragma unroll
-      for (int k = 0; k < XP; k++) f(x + k * kWave, y, z);
+      for (int k = 0; k < XP; k++) f(x + k * STEP, y, z);
     } else {
 #pragma unroll
       for (int k = 0; k < XP; k++)
-        if (x + k * kWave < L.x1) f(x + k * kWave, y, z);
+        if (x + k * STEP < L.x1) f(x + k * STEP, y, z);
     }
   };
   if (L.three_d) {

@@ -58,6 +58,8 @@ typedef struct opsmi_gen_launch {
   unsigned grid[3];  /* workgroups of 64 x 4 threads */
   int alias;         /* 1: the general kernel (plain pointers, x steps from md): one dat is
                         passed in two arguments, or a stencil does not advance in a dimension */
+  int spread;        /* 1: a lane's points 64 apart also where they may be adjacent
+                        (OPSMI_GEN_ADJ=0) */
   opsmi_gen_dat dat[OPSMI_GEN_MAX_ARGS];    /* dat arguments, by argument index */
   void *partials[OPSMI_GEN_MAX_ARGS];       /* reduction arguments: dim values per workgroup */
 } opsmi_gen_launch;

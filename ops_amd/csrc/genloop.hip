@@ -101,6 +101,7 @@ struct GenLoop {
     L.grid[1] = t.grid.y;
     L.grid[2] = t.grid.z;
     L.alias = alias || strided;
+    L.spread = inst().gen_spread ? 1 : 0;
     for (size_t i = 0; i < dats.size(); i++)
       if (dats[i]) {
         const DatView v = view_of(dats[i]);

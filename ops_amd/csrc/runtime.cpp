@@ -757,6 +757,8 @@ void ops_init(const int argc, const char *const argv[], const int diags_level) {
   I.fusion = fz ? atoi(fz) : 2;
   const char *hf = getenv("OPSMI_HOST_FOLD");
   I.host_fold = hf ? atoi(hf) != 0 : true;
+  const char *ga = getenv("OPSMI_GEN_ADJ");
+  I.gen_spread = ga && atoi(ga) == 0;
   if (tiling) I.fusion = 2;
   if (I.diags > 0 && (tiling || I.block_y_set() || I.block_z_set()))
     printf(" OPS (MI355X): loop-chain fusion level %d, generated-loop workgroups %d rows (2D) / %d planes (3D)\n",

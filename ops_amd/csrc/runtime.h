@@ -165,6 +165,7 @@ struct Instance {
   int hf_dim = 1;
   double hf_init[4] = {0, 0, 0, 0};  // the handle's initial values
   bool host_fold = true;             // ops_mi_set_host_fold
+  bool gen_spread = false;           // OPSMI_GEN_ADJ=0: generated loops keep lane points 64 apart
 
   // random fills
   uint64_t rng_seed = 0;

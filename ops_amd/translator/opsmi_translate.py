@@ -588,7 +588,10 @@ def kernel_code(loop, kern, consts):
                        for i in dats)
         L.append(f"__global__ __launch_bounds__(256) void opsg_{K}{suffix}(const opsa_{K} A{ptrs}) {{")
         L.extend(red_init_lines(loop, ""))
-        L.append("  ::opsmi_gen::for_each_point(A.L, [&](int x, int y, int z) {")
+        # adjacent points per lane where the accesses merge (restrict) and no
+        # reduction's order depends on the layout
+        adj = "true" if qual and not any(a.kind == "reduce" for a in loop.args) else "false"
+        L.append(f"  ::opsmi_gen::for_each_point<{adj}>(A.L, [&](int x, int y, int z) {{")
         L.extend(point_lines(loop, "A", "", gen))
         L.append("  });")
         L.extend(red_store_lines(loop, "A", ""))
