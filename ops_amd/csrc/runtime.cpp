@@ -661,8 +661,18 @@ void flush_for(ops_reduction h) {
     I.hf_slot = ri->d_slot;
     memcpy(I.hf_init, only->ctl.init, sizeof(I.hf_init));
   }
-  execute(run);
-  I.hf_slot = nullptr;
+  // disarmed however execute() ends (an error thrown out of a launch
+  // included), so no later flush takes the host fold for this handle
+  struct Disarm {
+    Instance &I;
+    ~Disarm() { I.hf_slot = nullptr; }
+  } disarm{I};
+  try {
+    execute(run);
+  } catch (...) {
+    I.hf_used = false;
+    throw;
+  }
 }
 
 static bool same_range6(const int *a, const Range3 &r) {
