@@ -623,8 +623,9 @@ int ops_mi_stream_probe(int nread, int nwrite, int nt, long points, int reps, do
 /* loop-chain fusion in the queue: 0 off, 1 the pointwise CG axpy+axpby pair,
  * 2 also the column chains (TeaLeaf and Poisson CG axpy+axpby+calc_w, the
  * mixed solver's e update + p update + FP32 sweep, the Laplace tutorial's
- * copy + next apply_stencil), each one pass.  Every level gives the same
- * bits.  Initial level from OPSMI_FUSION (default 2). */
+ * copy + next apply_stencil), each one pass; 3 also runs of three or four
+ * generated point-wise loops (ops_mi_gen_fused_chain).  Every level gives
+ * the same bits.  Initial level from OPSMI_FUSION (default 2). */
 void ops_mi_set_fusion(int level);
 
 #ifdef __cplusplus

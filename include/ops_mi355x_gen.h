@@ -143,6 +143,27 @@ void ops_mi_gen_fused(opsmi_gen_launch_fn first, opsmi_gen_launch_fn second, ops
 void ops_mi_gen_fused_nb(opsmi_gen_launch_fn first, opsmi_gen_launch_fn second, opsmi_gen_fused_fn fused,
                          const int *links, int nlinks);
 
+/* launches one kernel that runs, at each point, the kernels of n loops in
+ * turn (L[k], blob[k]: the k-th loop's launch data; they share range and
+ * tiling) */
+typedef void (*opsmi_gen_chain_fn)(const opsmi_gen_launch *const *L, const void *const *blob, void *stream);
+#define OPSMI_GEN_MAX_CHAIN 4
+
+/* Registers `fused` as the one-pass form of n (3 to OPSMI_GEN_MAX_CHAIN)
+ * loops launched by loops[0], ..., loops[n - 1] in that order (no
+ * counterpart in the reference, whose lazy queue,
+ * ops/c/src/core/ops_lazy.cpp:380-1187, launches each loop of a chain
+ * separately).  The translator registers one per run of ops_par_loop calls
+ * that follow each other in an application's source and touch what they
+ * write only at the point.  The queue launches the longest registered form
+ * that matches the loops it runs back to back, under the conditions of
+ * ops_mi_gen_fused extended to every loop of the run (same block, range and
+ * dimension; no reduction handle shared by two of them; every dat any of
+ * them writes accessed by all of them only at the point itself) at fusion
+ * level 3 (ops_mi_set_fusion).  Fields and reductions are the same bits as the loops run one
+ * after the other. */
+void ops_mi_gen_fused_chain(const opsmi_gen_launch_fn *loops, int n, opsmi_gen_chain_fn fused);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -252,6 +252,81 @@ int gen_pair_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
   return 2;
 }
 
+std::map<std::vector<opsmi_gen_launch_fn>, opsmi_gen_chain_fn> &chain_table() {
+  static std::map<std::vector<opsmi_gen_launch_fn>, opsmi_gen_chain_fn> m;
+  return m;
+}
+int g_longest_chain = 0;
+
+// n generated loops back to back as one pass (ops_mi_gen_fused_chain): the
+// pair rule's conditions over every loop of the run
+int gen_chain_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
+  (void)s;
+  // fusion level 3: the chain's kernel takes plain pointers and one point
+  // layout for all its loops, and measured slower on CloverLeaf 3D than the
+  // loops' own restrict-qualified kernels run as pairs
+  if (inst().fusion < 3) return 0;
+  const int avail = (int)std::min<size_t>((size_t)g_longest_chain, run.size() - at);
+  for (int n = avail; n >= 3; n--) {
+    std::vector<opsmi_gen_launch_fn> key;
+    const GenLoop *G[OPSMI_GEN_MAX_CHAIN];
+    bool ok = true;
+    for (int k = 0; k < n && ok; k++) {
+      ok = run[at + k].gen != nullptr;
+      if (ok) {
+        G[k] = run[at + k].gen.get();
+        key.push_back(G[k]->fn);
+      }
+    }
+    if (!ok) continue;
+    const auto it = chain_table().find(key);
+    if (it == chain_table().end()) continue;
+    for (int k = 0; k < n && ok; k++) {
+      const GenLoop &A = *G[k];
+      ok = A.block == G[0]->block && A.dims == G[0]->dims && same_range(A.r, G[0]->r) && A.r.npts() > 0 &&
+           !A.alias && !A.strided && A.resets.empty();
+      for (int j = 0; j < k && ok; j++) ok = !share_handle(*G[j], A);
+      for (size_t i = 0; i < A.dats.size() && ok; i++) {
+        const ops_dat d = A.dats[i];
+        if (d && A.acc[i] != OPS_READ)
+          for (int j = 0; j < n && ok; j++) ok = G[j]->only_at_point(d);
+      }
+    }
+    if (!ok) continue;
+    const opsmi_gen_chain_fn fused = it->second;
+    size_t off[OPSMI_GEN_MAX_CHAIN], total = 0, bytes = 0;
+    std::string name = "fused(";
+    for (int k = 0; k < n; k++) {
+      off[k] = total;
+      total += G[k]->pbytes;
+      bytes += run[at + k].bytes;
+      name += (k ? "+" : "") + run[at + k].name;
+    }
+    name += ")";
+    ensure_partials(total);
+    run_loop(name.c_str(), bytes, [&](hipStream_t st) {
+      char *pbase = (char *)inst().d_partials;
+      opsmi_gen_launch L[OPSMI_GEN_MAX_CHAIN];
+      const opsmi_gen_launch *Lp[OPSMI_GEN_MAX_CHAIN];
+      const void *blobs[OPSMI_GEN_MAX_CHAIN];
+      for (int k = 0; k < n; k++) {
+        G[k]->fill(L[k], pbase + off[k]);
+        Lp[k] = &L[k];
+        blobs[k] = G[k]->blob();
+      }
+      fused(Lp, blobs, (void *)st);
+      OPSMI_HIP(hipGetLastError());
+      for (int k = 0; k < n; k++) G[k]->finish_reds(pbase + off[k], st);
+    });
+    for (int k = 0; k < n; k++) {
+      note_fused(run[at + k].name.c_str(), run[at + k].bytes);
+      for (ops_dat d : run[at + k].writes) note_external_write(d);
+    }
+    return n;
+  }
+  return 0;
+}
+
 bool reads(const GenLoop &G, ops_dat d) {
   for (size_t i = 0; i < G.dats.size(); i++)
     if (G.dats[i] == d && G.acc[i] != OPS_WRITE) return true;
@@ -329,6 +404,7 @@ int gen_nb_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
 
 struct RegisterGen {
   RegisterGen() {
+    register_fusion(gen_chain_rule);
     register_fusion(gen_pair_rule);
     register_fusion(gen_nb_rule);
   }
@@ -464,6 +540,18 @@ void ops_mi_gen_fused(opsmi_gen_launch_fn first, opsmi_gen_launch_fn second, ops
   OPSMI_API_BEGIN
   if (!first || !second || !fused) fail(OPS_INVALID_ARGUMENT, "ops_mi_gen_fused: NULL function");
   fused_table()[{first, second}] = fused;
+  OPSMI_API_END()
+}
+
+void ops_mi_gen_fused_chain(const opsmi_gen_launch_fn *loops, int n, opsmi_gen_chain_fn fused) {
+  OPSMI_API_BEGIN
+  if (!loops || !fused || n < 3 || n > OPSMI_GEN_MAX_CHAIN)
+    fail(OPS_INVALID_ARGUMENT, "ops_mi_gen_fused_chain: bad arguments");
+  std::vector<opsmi_gen_launch_fn> key(loops, loops + n);
+  for (opsmi_gen_launch_fn f : key)
+    if (!f) fail(OPS_INVALID_ARGUMENT, "ops_mi_gen_fused_chain: NULL function");
+  chain_table()[key] = fused;
+  g_longest_chain = std::max(g_longest_chain, n);
   OPSMI_API_END()
 }
 

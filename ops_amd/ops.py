@@ -436,7 +436,8 @@ def ops_mi_set_host_fold(on):
 
 
 def ops_mi_set_fusion(level):
-    """loop-chain fusion level of the queue: 0 off, 1 point-wise pairs, 2 chains"""
+    """loop-chain fusion level of the queue: 0 off, 1 point-wise pairs, 2 chains,
+    3 also runs of three or four generated point-wise loops"""
     _call("ops_mi_set_fusion", level)
 
 

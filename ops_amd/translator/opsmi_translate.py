@@ -687,6 +687,43 @@ def fused_code(l1, l2):
     return "\n".join(L), f"ops_mi_gen_fused(&opsl_{K1}, &opsl_{K2}, &opsf_{F});"
 
 
+def chain_code(ls):
+    """One kernel running the kernels of the loops ls (three or more) in turn
+    at every point, and its launch function (registered for the chain; the
+    library decides per call whether the queued loops may run this way)."""
+    F = "__".join(lp.kernel for lp in ls)
+    ptrs = []
+    for k, lp in enumerate(ls):
+        ptrs += [f"{'const ' if a.acc == 'OPS_READ' else ''}{a.typ} *q{k}_p{i}"
+                 for i, a in enumerate(lp.args) if a.kind == "dat"]
+    L = [f"// ---------------------------------------------------------------- {' then '.join(lp.kernel for lp in ls)}",
+         "// (plain pointers: the loops may share dats)",
+         f"__global__ __launch_bounds__(256) void opsg_chain_{F}("
+         + ", ".join(f"const opsa_{lp.kernel} A{k}" for k, lp in enumerate(ls))
+         + f"{''.join(', ' + p for p in ptrs)}) {{"]
+    for k, lp in enumerate(ls):
+        L += red_init_lines(lp, f"q{k}_")
+    L.append("  ::opsmi_gen::for_each_point(A0.L, [&](int x, int y, int z) {")
+    for k, lp in enumerate(ls):
+        L += point_lines(lp, f"A{k}", f"q{k}_")
+    L.append("  });")
+    for k, lp in enumerate(ls):
+        L += red_store_lines(lp, f"A{k}", f"q{k}_")
+    L.append("}")
+    L.append(f"void opsc_{F}(const opsmi_gen_launch *const *L, const void *const *blob, void *stream) {{")
+    for k, lp in enumerate(ls):
+        L.append(f"  opsa_{lp.kernel} A{k};")
+        L.append(f"  opsfill_{lp.kernel}(A{k}, L[{k}], blob[{k}]);")
+    L.append(f"  hipLaunchKernelGGL(opsg_chain_{F}, dim3(L[0]->grid[0], L[0]->grid[1], L[0]->grid[2]), "
+             f"dim3(::opsmi_gen::kWave, ::opsmi_gen::kRows), 0, (hipStream_t)stream, "
+             + ", ".join(f"A{k}" for k in range(len(ls)))
+             + "".join(launch_ptrs(lp, f"L[{k}]") for k, lp in enumerate(ls)) + ");")
+    L.append("}")
+    fns = ", ".join(f"&opsl_{lp.kernel}" for lp in ls)
+    return "\n".join(L), (f"{{ static const opsmi_gen_launch_fn c[] = {{{fns}}}; "
+                          f"ops_mi_gen_fused_chain(c, {len(ls)}, &opsc_{F}); }}")
+
+
 def nb_links(l1, l2, pts2):
     """(i, j): argument i of l2 reads, at stencil offsets other than the
     point itself, the dat l1 writes through argument j (same dat
@@ -864,6 +901,32 @@ def adjacent_pairs(loops):
     return out
 
 
+def point_chains(loops, kpoints, n):
+    """Runs of n ops_par_loop calls that follow each other in the source
+    (same dimension, once per kernel sequence) in which every dat
+    expression any call writes is accessed by all n kernels only at the
+    point itself: the chains the library may run as one pass (it checks the
+    dats themselves at run time)."""
+    seen, out = set(), []
+    for w in range(len(loops) - n + 1):
+        ls = loops[w:w + n]
+        key = tuple(l.kernel for l in ls)
+        if key in seen or any(l.ndim != ls[0].ndim for l in ls):
+            continue
+        written = {a.expr for l in ls for a in l.args if a.kind == "dat" and a.acc != "OPS_READ"}
+        ok = True
+        for l in ls:
+            for i, a in enumerate(l.args):
+                if a.kind == "dat" and a.expr in written:
+                    pts = kpoints[l.kernel][i]
+                    if pts is None or any(tuple(q[1:]) != (0, 0, 0) for q in pts):
+                        ok = False
+        if ok:
+            seen.add(key)
+            out.append(ls)
+    return out
+
+
 def stub_code(loop, used, consts, points):
     K = loop.kernel
     n = len(loop.args)
@@ -957,6 +1020,12 @@ def translate(app, out_dir, inc_dirs=(), program=None):
         code, reg = fused_code(kernels[a.kernel][0], kernels[b.kernel][0])
         H.append(code)
         regs.append(reg)
+    # runs of three and four calls touching what they write only at the point
+    for n in (4, 3):
+        for ls in point_chains(loops, kpoints, n):
+            code, reg = chain_code([kernels[l.kernel][0] for l in ls])
+            H.append(code)
+            regs.append(reg)
     # pairs whose second loop reads the first's output at neighbours:
     # adjacent calls and the last and first call of a loop body, matched by
     # the dat expressions as written (the library checks the dats at run time)

@@ -153,3 +153,46 @@ extern "C" int gen_cg_tail(ops_block block, ops_dat u, ops_dat p, ops_dat r, int
   }
   return 0;
 }
+
+void cg3_u(ACC<double> &u, const ACC<double> &p, const double *a) {
+  u(0, 0, 0) = u(0, 0, 0) + (*a) * p(0, 0, 0);
+}
+
+void cg3_r(ACC<double> &r, const ACC<double> &w, const double *a, double *rr) {
+  r(0, 0, 0) = r(0, 0, 0) - (*a) * w(0, 0, 0);
+  *rr = *rr + r(0, 0, 0) * r(0, 0, 0);
+}
+
+void cg3_p(ACC<double> &p, const ACC<double> &r, const double *b, double *pp) {
+  p(0, 0, 0) = r(0, 0, 0) + (*b) * p(0, 0, 0);
+  *pp = *pp + p(0, 0, 0) * p(0, 0, 0);
+}
+
+// Three loops on shared dats that touch them only at the point, `iters`
+// times: u = u + a p; r = r - a w with r.r into rr[it]; p = r + b p with p.p
+// into pp[it].  short_p != 0 runs the third loop one z plane short, so the
+// three cannot run as one pass (the first two still can).
+extern "C" int gen_cg_three(ops_block block, ops_dat u, ops_dat p, ops_dat r, ops_dat w, int nx, int ny, int nz,
+                            double a, double b, int iters, int short_p, double *rr, double *pp) {
+  int s0[] = {0, 0, 0};
+  ops_stencil S0 = ops_decl_stencil(3, 1, s0, "c0");
+  ops_reduction hr = ops_decl_reduction_handle(sizeof(double), "double", "rr");
+  ops_reduction hp = ops_decl_reduction_handle(sizeof(double), "double", "pp");
+  int inner[] = {0, nx, 0, ny, 0, nz};
+  int pr[] = {0, nx, 0, ny, 0, short_p ? nz - 1 : nz};
+  for (int it = 0; it < iters; it++) {
+    ops_par_loop(cg3_u, "cg3_u", block, 3, inner, ops_arg_dat(u, 1, S0, "double", OPS_RW),
+                 ops_arg_dat(p, 1, S0, "double", OPS_READ), ops_arg_gbl(&a, 1, "double", OPS_READ));
+    ops_par_loop(cg3_r, "cg3_r", block, 3, inner, ops_arg_dat(r, 1, S0, "double", OPS_RW),
+                 ops_arg_dat(w, 1, S0, "double", OPS_READ), ops_arg_gbl(&a, 1, "double", OPS_READ),
+                 ops_arg_reduce(hr, 1, "double", OPS_INC));
+    ops_par_loop(cg3_p, "cg3_p", block, 3, pr, ops_arg_dat(p, 1, S0, "double", OPS_RW),
+                 ops_arg_dat(r, 1, S0, "double", OPS_READ), ops_arg_gbl(&b, 1, "double", OPS_READ),
+                 ops_arg_reduce(hp, 1, "double", OPS_INC));
+    // p.p first: its flush needs all three loops (r.r alone would need only
+    // the second)
+    ops_reduction_result(hp, &pp[it]);
+    ops_reduction_result(hr, &rr[it]);
+  }
+  return 0;
+}

@@ -37,6 +37,7 @@ def gen(gpu_ops):
     L.gen_fill2d.argtypes = [vp, vp, ci, ci, cd]
     L.gen_double_in_place.argtypes = [vp, vp, ci, ci, ci]
     L.gen_cg_tail.argtypes = [vp, vp, vp, vp, ci, ci, ci, cd, cd, ci, dp]
+    L.gen_cg_three.argtypes = [vp, vp, vp, vp, vp, ci, ci, ci, cd, cd, ci, ci, dp, dp]
     L.gen_lowdim3d.argtypes = [vp, ci, ci, ci, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_float), dp]
     L.gen_lowdim2d.argtypes = [vp, ci, ci, dp]
     L.gen_lowdim_bad.argtypes = [vp, ci, ci, ci, ci]
@@ -336,6 +337,75 @@ def test_generated_pair_runs_fused_and_bit_exact(gpu_ops, gen, n):
         assert abs(pp_f[it] - ref) <= 1e-12 * abs(ref)
     assert np.array_equal(fused["u"], u) and np.array_equal(fused["p"], p)
     assert np.array_equal(fused["r"], s0["r"])
+
+
+def _cg_three(ops, gen, n, iters, fusion, short_p):
+    nx, ny, nz = n
+    blk = ops.ops_decl_block(3, "genthree")
+    ext = (nz + 2, ny + 2, nx + 2)
+    D = {}
+    for s, k in enumerate(["u", "p", "r", "w"]):
+        D[k] = ops.ops_decl_dat(blk, [nx, ny, nz], name=k)
+        ops.set_whole(D[k], O.u01(71, s, int(np.prod(ext))).reshape(ext) - 0.5)
+    start = {k: ops.get_whole(D[k]) for k in D}
+    rr, pp = np.zeros(iters), np.zeros(iters)
+    ops.lib().ops_mi_set_fusion(fusion)
+    ops.ops_mi_reset_stats()
+    try:
+        gen.gen_cg_three(blk.ptr, D["u"].ptr, D["p"].ptr, D["r"].ptr, D["w"].ptr, nx, ny, nz, 0.29, -0.53, iters,
+                         short_p, _p(rr), _p(pp))
+        ops._check()
+        st = {"three": ops.ops_mi_loop_stats("fused(cg3_u+cg3_r+cg3_p)"),
+              "pair": ops.ops_mi_loop_stats("fused(cg3_u+cg3_r)")}
+    finally:
+        ops.lib().ops_mi_set_fusion(2)
+    return start, {k: ops.get_whole(D[k]) for k in D}, rr, pp, st
+
+
+@pytest.mark.parametrize("n", [(64, 20, 9), (131, 7, 5)])
+def test_generated_three_loops_run_as_one_pass(gpu_ops, gen, n):
+    """Three point-wise loops on shared dats (u += a p; r -= a w with r.r;
+    p = r + b p with p.p): one pass per iteration at fusion level 3, none
+    below it, the same bits as the loops one by one (level 0) at every
+    level, and numpy's values (fields bit-exact, sums within 1e-12
+    relative)."""
+    iters = 3
+    s0, apart, rr_a, pp_a, st_a = _cg_three(gpu_ops, gen, n, iters, 0, 0)
+    assert st_a["three"] is None or st_a["three"][0] == 0
+    for level in (1, 2, 3):
+        _, fused, rr_f, pp_f, st = _cg_three(gpu_ops, gen, n, iters, level, 0)
+        if level == 3:
+            assert st["three"] is not None and st["three"][0] == iters
+        else:
+            assert st["three"] is None or st["three"][0] == 0, level
+        for k in ("u", "p", "r", "w"):
+            assert np.array_equal(fused[k], apart[k]), (level, k)
+        assert np.array_equal(rr_f, rr_a) and np.array_equal(pp_f, pp_a), level
+    u, p, r, w = (s0[k].copy() for k in ("u", "p", "r", "w"))
+    i = (slice(1, -1),) * 3
+    for it in range(iters):
+        u[i] = u[i] + 0.29 * p[i]
+        r[i] = r[i] - 0.29 * w[i]
+        ref_r = np.sum(r[i] * r[i])
+        p[i] = r[i] + -0.53 * p[i]
+        ref_p = np.sum(p[i] * p[i])
+        assert abs(rr_a[it] - ref_r) <= 1e-12 * abs(ref_r)
+        assert abs(pp_a[it] - ref_p) <= 1e-12 * abs(ref_p)
+    for k, v in (("u", u), ("p", p), ("r", r), ("w", w)):
+        assert np.array_equal(apart[k], v), k
+
+
+def test_generated_three_loops_with_a_shorter_range_stay_apart(gpu_ops, gen):
+    """The third loop one plane short: no three-loop pass (the first two
+    still run as a pair), and the results equal the unfused run's."""
+    n, iters = (64, 20, 9), 2
+    _, apart, rr_a, pp_a, _ = _cg_three(gpu_ops, gen, n, iters, 0, 1)
+    _, fused, rr_f, pp_f, st = _cg_three(gpu_ops, gen, n, iters, 3, 1)
+    assert st["three"] is None or st["three"][0] == 0
+    assert st["pair"] is not None and st["pair"][0] == iters
+    for k in ("u", "p", "r", "w"):
+        assert np.array_equal(fused[k], apart[k]), k
+    assert np.array_equal(rr_f, rr_a) and np.array_equal(pp_f, pp_a)
 
 
 def test_generated_pair_with_a_neighbour_read_is_not_fused(gpu_ops, gen):
