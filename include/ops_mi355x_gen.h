@@ -160,9 +160,15 @@ typedef void (*opsmi_gen_chain_fn)(const opsmi_gen_launch *const *L, const void 
  * ops_mi_gen_fused extended to every loop of the run (same block, range and
  * dimension; no reduction handle shared by two of them; every dat any of
  * them writes accessed by all of them only at the point itself) at fusion
- * level 3 (ops_mi_set_fusion).  Fields and reductions are the same bits as the loops run one
+ * level 3 (ops_mi_set_fusion), and when the arguments agree with `groups`:
+ * one entry per argument of each loop in turn (ngroups in all), the index
+ * of the fused kernel's pointer a dat argument uses, -1 for other
+ * arguments; arguments with one index must name one dat, arguments with
+ * different indices different dats (the kernel's pointers are
+ * restrict-qualified), and every dat argument must be present.  Fields and reductions are the same bits as the loops run one
  * after the other. */
-void ops_mi_gen_fused_chain(const opsmi_gen_launch_fn *loops, int n, opsmi_gen_chain_fn fused);
+void ops_mi_gen_fused_chain(const opsmi_gen_launch_fn *loops, int n, const int *groups, int ngroups,
+                            opsmi_gen_chain_fn fused);
 
 #ifdef __cplusplus
 } /* extern "C" */

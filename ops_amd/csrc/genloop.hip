@@ -252,9 +252,35 @@ int gen_pair_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
   return 2;
 }
 
-std::map<std::vector<opsmi_gen_launch_fn>, opsmi_gen_chain_fn> &chain_table() {
-  static std::map<std::vector<opsmi_gen_launch_fn>, opsmi_gen_chain_fn> m;
+struct ChainFused {
+  opsmi_gen_chain_fn fn = nullptr;
+  std::vector<int> groups;  // per argument of each loop in turn: pointer index, -1 for non-dats
+};
+std::map<std::vector<opsmi_gen_launch_fn>, ChainFused> &chain_table() {
+  static std::map<std::vector<opsmi_gen_launch_fn>, ChainFused> m;
   return m;
+}
+
+// the run's dat arguments agree with the kernel's pointers: one dat per
+// pointer, distinct dats for distinct pointers, none absent
+bool groups_match(const GenLoop *const *G, int n, const std::vector<int> &groups) {
+  std::vector<ops_dat> d;
+  std::vector<int> g;
+  size_t pos = 0;
+  for (int k = 0; k < n; k++)
+    for (size_t i = 0; i < G[k]->dats.size(); i++, pos++) {
+      if (pos >= groups.size()) return false;
+      const int j = groups[pos];
+      if (j < 0) continue;
+      if (!G[k]->dats[i]) return false;
+      d.push_back(G[k]->dats[i]);
+      g.push_back(j);
+    }
+  if (pos != groups.size()) return false;
+  for (size_t a = 0; a < d.size(); a++)
+    for (size_t b = 0; b < a; b++)
+      if ((g[a] == g[b]) != (d[a] == d[b])) return false;
+  return true;
 }
 int g_longest_chain = 0;
 
@@ -262,9 +288,6 @@ int g_longest_chain = 0;
 // pair rule's conditions over every loop of the run
 int gen_chain_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
   (void)s;
-  // fusion level 3: the chain's kernel takes plain pointers and one point
-  // layout for all its loops, and measured slower on CloverLeaf 3D than the
-  // loops' own restrict-qualified kernels run as pairs
   if (inst().fusion < 3) return 0;
   const int avail = (int)std::min<size_t>((size_t)g_longest_chain, run.size() - at);
   for (int n = avail; n >= 3; n--) {
@@ -292,8 +315,8 @@ int gen_chain_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
           for (int j = 0; j < n && ok; j++) ok = G[j]->only_at_point(d);
       }
     }
-    if (!ok) continue;
-    const opsmi_gen_chain_fn fused = it->second;
+    if (!ok || !groups_match(G, n, it->second.groups)) continue;
+    const opsmi_gen_chain_fn fused = it->second.fn;
     size_t off[OPSMI_GEN_MAX_CHAIN], total = 0, bytes = 0;
     std::string name = "fused(";
     for (int k = 0; k < n; k++) {
@@ -543,14 +566,19 @@ void ops_mi_gen_fused(opsmi_gen_launch_fn first, opsmi_gen_launch_fn second, ops
   OPSMI_API_END()
 }
 
-void ops_mi_gen_fused_chain(const opsmi_gen_launch_fn *loops, int n, opsmi_gen_chain_fn fused) {
+void ops_mi_gen_fused_chain(const opsmi_gen_launch_fn *loops, int n, const int *groups, int ngroups,
+                            opsmi_gen_chain_fn fused) {
   OPSMI_API_BEGIN
-  if (!loops || !fused || n < 3 || n > OPSMI_GEN_MAX_CHAIN)
+  if (!loops || !fused || n < 3 || n > OPSMI_GEN_MAX_CHAIN || ngroups < 0 ||
+      ngroups > n * OPSMI_GEN_MAX_ARGS || (ngroups > 0 && !groups))
     fail(OPS_INVALID_ARGUMENT, "ops_mi_gen_fused_chain: bad arguments");
   std::vector<opsmi_gen_launch_fn> key(loops, loops + n);
   for (opsmi_gen_launch_fn f : key)
     if (!f) fail(OPS_INVALID_ARGUMENT, "ops_mi_gen_fused_chain: NULL function");
-  chain_table()[key] = fused;
+  ChainFused c;
+  c.fn = fused;
+  c.groups.assign(groups, groups + ngroups);
+  chain_table()[key] = c;
   g_longest_chain = std::max(g_longest_chain, n);
   OPSMI_API_END()
 }

@@ -687,29 +687,60 @@ def fused_code(l1, l2):
     return "\n".join(L), f"ops_mi_gen_fused(&opsl_{K1}, &opsl_{K2}, &opsf_{F});"
 
 
-def chain_code(ls):
+def chain_code(ls, calls):
     """One kernel running the kernels of the loops ls (three or more) in turn
     at every point, and its launch function (registered for the chain; the
-    library decides per call whether the queued loops may run this way)."""
+    library decides per call whether the queued loops may run this way).
+    calls are the run's ops_par_loop calls: dat arguments written with the
+    same expression share one restrict-qualified pointer (the library runs
+    the kernel only when those name one dat and the others distinct dats).
+    None when two arguments with one expression differ in type."""
     F = "__".join(lp.kernel for lp in ls)
-    ptrs = []
-    for k, lp in enumerate(ls):
-        ptrs += [f"{'const ' if a.acc == 'OPS_READ' else ''}{a.typ} *q{k}_p{i}"
-                 for i, a in enumerate(lp.args) if a.kind == "dat"]
+    groups, gtyp, gconst, flat = {}, [], [], []
+    for k, (lp, c) in enumerate(zip(ls, calls)):
+        for i, a in enumerate(lp.args):
+            if a.kind != "dat":
+                flat.append(-1)
+                continue
+            e = c.args[i].expr
+            if e not in groups:
+                groups[e] = len(gtyp)
+                gtyp.append((a.typ, a.multi))
+                gconst.append(True)
+            j = groups[e]
+            if gtyp[j] != (a.typ, a.multi):
+                return None
+            gconst[j] = gconst[j] and a.acc == "OPS_READ"
+            flat.append(j)
+    adj = "false" if any(a.kind == "reduce" for lp in ls for a in lp.args) else "true"
     L = [f"// ---------------------------------------------------------------- {' then '.join(lp.kernel for lp in ls)}",
-         "// (plain pointers: the loops may share dats)",
+         "// (one restrict-qualified pointer per distinct dat)",
          f"__global__ __launch_bounds__(256) void opsg_chain_{F}("
          + ", ".join(f"const opsa_{lp.kernel} A{k}" for k, lp in enumerate(ls))
-         + f"{''.join(', ' + p for p in ptrs)}) {{"]
+         + "".join(f", {'const ' if gconst[j] else ''}{t} *__restrict__ g{j}" for j, (t, _) in enumerate(gtyp))
+         + ") {"]
+    pos = 0
+    for k, lp in enumerate(ls):
+        for i, a in enumerate(lp.args):
+            if a.kind == "dat":
+                L.append(f"  {'const ' if a.acc == 'OPS_READ' else ''}{a.typ} *q{k}_p{i} = g{flat[pos]};")
+            pos += 1
     for k, lp in enumerate(ls):
         L += red_init_lines(lp, f"q{k}_")
-    L.append("  ::opsmi_gen::for_each_point(A0.L, [&](int x, int y, int z) {")
+    L.append(f"  ::opsmi_gen::for_each_point<{adj}>(A0.L, [&](int x, int y, int z) {{")
     for k, lp in enumerate(ls):
         L += point_lines(lp, f"A{k}", f"q{k}_")
     L.append("  });")
     for k, lp in enumerate(ls):
         L += red_store_lines(lp, f"A{k}", f"q{k}_")
     L.append("}")
+    first = {}
+    pos = 0
+    for k, lp in enumerate(ls):
+        for i, a in enumerate(lp.args):
+            if flat[pos] >= 0:
+                first.setdefault(flat[pos], (k, i))
+            pos += 1
     L.append(f"void opsc_{F}(const opsmi_gen_launch *const *L, const void *const *blob, void *stream) {{")
     for k, lp in enumerate(ls):
         L.append(f"  opsa_{lp.kernel} A{k};")
@@ -717,11 +748,13 @@ def chain_code(ls):
     L.append(f"  hipLaunchKernelGGL(opsg_chain_{F}, dim3(L[0]->grid[0], L[0]->grid[1], L[0]->grid[2]), "
              f"dim3(::opsmi_gen::kWave, ::opsmi_gen::kRows), 0, (hipStream_t)stream, "
              + ", ".join(f"A{k}" for k in range(len(ls)))
-             + "".join(launch_ptrs(lp, f"L[{k}]") for k, lp in enumerate(ls)) + ");")
+             + "".join(f", ({gtyp[j][0]} *)L[{first[j][0]}]->dat[{first[j][1]}].base" for j in range(len(gtyp)))
+             + ");")
     L.append("}")
     fns = ", ".join(f"&opsl_{lp.kernel}" for lp in ls)
     return "\n".join(L), (f"{{ static const opsmi_gen_launch_fn c[] = {{{fns}}}; "
-                          f"ops_mi_gen_fused_chain(c, {len(ls)}, &opsc_{F}); }}")
+                          f"static const int g[] = {{{', '.join(map(str, flat))}}}; "
+                          f"ops_mi_gen_fused_chain(c, {len(ls)}, g, {len(flat)}, &opsc_{F}); }}")
 
 
 def nb_links(l1, l2, pts2):
@@ -1023,9 +1056,10 @@ def translate(app, out_dir, inc_dirs=(), program=None):
     # runs of three and four calls touching what they write only at the point
     for n in (4, 3):
         for ls in point_chains(loops, kpoints, n):
-            code, reg = chain_code([kernels[l.kernel][0] for l in ls])
-            H.append(code)
-            regs.append(reg)
+            got = chain_code([kernels[l.kernel][0] for l in ls], ls)
+            if got:
+                H.append(got[0])
+                regs.append(got[1])
     # pairs whose second loop reads the first's output at neighbours:
     # adjacent calls and the last and first call of a loop body, matched by
     # the dat expressions as written (the library checks the dats at run time)

@@ -339,12 +339,15 @@ def test_generated_pair_runs_fused_and_bit_exact(gpu_ops, gen, n):
     assert np.array_equal(fused["r"], s0["r"])
 
 
-def _cg_three(ops, gen, n, iters, fusion, short_p):
+def _cg_three(ops, gen, n, iters, fusion, short_p, w_is_u=False):
     nx, ny, nz = n
     blk = ops.ops_decl_block(3, "genthree")
     ext = (nz + 2, ny + 2, nx + 2)
     D = {}
     for s, k in enumerate(["u", "p", "r", "w"]):
+        if k == "w" and w_is_u:
+            D[k] = D["u"]
+            continue
         D[k] = ops.ops_decl_dat(blk, [nx, ny, nz], name=k)
         ops.set_whole(D[k], O.u01(71, s, int(np.prod(ext))).reshape(ext) - 0.5)
     start = {k: ops.get_whole(D[k]) for k in D}
@@ -404,6 +407,20 @@ def test_generated_three_loops_with_a_shorter_range_stay_apart(gpu_ops, gen):
     assert st["three"] is None or st["three"][0] == 0
     assert st["pair"] is not None and st["pair"][0] == iters
     for k in ("u", "p", "r", "w"):
+        assert np.array_equal(fused[k], apart[k]), k
+    assert np.array_equal(rr_f, rr_a) and np.array_equal(pp_f, pp_a)
+
+
+def test_generated_three_loops_with_one_dat_under_two_names_stay_apart(gpu_ops, gen):
+    """w and u the same dat: the run's kernel gives the two names separate
+    restrict-qualified pointers, so the queue does not launch it (the first
+    two loops still fuse as a pair); results equal the unfused run's."""
+    n, iters = (64, 20, 9), 2
+    _, apart, rr_a, pp_a, _ = _cg_three(gpu_ops, gen, n, iters, 0, 0, w_is_u=True)
+    _, fused, rr_f, pp_f, st = _cg_three(gpu_ops, gen, n, iters, 3, 0, w_is_u=True)
+    assert st["three"] is None or st["three"][0] == 0
+    assert st["pair"] is not None and st["pair"][0] == iters
+    for k in ("u", "p", "r"):
         assert np.array_equal(fused[k], apart[k]), k
     assert np.array_equal(rr_f, rr_a) and np.array_equal(pp_f, pp_a)
 
