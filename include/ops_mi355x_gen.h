@@ -149,23 +149,25 @@ void ops_mi_gen_fused_nb(opsmi_gen_launch_fn first, opsmi_gen_launch_fn second, 
 typedef void (*opsmi_gen_chain_fn)(const opsmi_gen_launch *const *L, const void *const *blob, void *stream);
 #define OPSMI_GEN_MAX_CHAIN 4
 
-/* Registers `fused` as the one-pass form of n (3 to OPSMI_GEN_MAX_CHAIN)
+/* Registers `fused` as the one-pass form of n (2 to OPSMI_GEN_MAX_CHAIN)
  * loops launched by loops[0], ..., loops[n - 1] in that order (no
  * counterpart in the reference, whose lazy queue,
  * ops/c/src/core/ops_lazy.cpp:380-1187, launches each loop of a chain
- * separately).  The translator registers one per run of ops_par_loop calls
- * that follow each other in an application's source and touch what they
- * write only at the point.  The queue launches the longest registered form
- * that matches the loops it runs back to back, under the conditions of
- * ops_mi_gen_fused extended to every loop of the run (same block, range and
- * dimension; no reduction handle shared by two of them; every dat any of
- * them writes accessed by all of them only at the point itself) at fusion
- * level 3 (ops_mi_set_fusion), and when the arguments agree with `groups`:
- * one entry per argument of each loop in turn (ngroups in all), the index
- * of the fused kernel's pointer a dat argument uses, -1 for other
- * arguments; arguments with one index must name one dat, arguments with
- * different indices different dats (the kernel's pointers are
- * restrict-qualified), and every dat argument must be present.  Fields and reductions are the same bits as the loops run one
+ * separately).  The translator registers one per pair of ops_par_loop calls
+ * that follow each other in an application's source, and one per run of
+ * three or four such calls that touch what they write only at the point.
+ * The queue launches the longest registered form that matches the loops it
+ * runs back to back, under the conditions of ops_mi_gen_fused extended to
+ * every loop of the run (same block, range and dimension; no reduction
+ * handle shared by two of them; every dat any of them writes accessed by
+ * all of them only at the point itself), pairs at fusion level >= 1 and
+ * longer runs at level 3 (ops_mi_set_fusion), and only when the arguments
+ * agree with `groups`: one entry per argument of each loop in turn
+ * (ngroups in all), the index of the fused kernel's pointer a dat argument
+ * uses, -1 for other arguments; arguments with one index must name one
+ * dat, arguments with different indices different dats (the kernel's
+ * pointers are restrict-qualified), and every dat argument must be
+ * present.  Fields and reductions are the same bits as the loops run one
  * after the other. */
 void ops_mi_gen_fused_chain(const opsmi_gen_launch_fn *loops, int n, const int *groups, int ngroups,
                             opsmi_gen_chain_fn fused);

@@ -288,9 +288,10 @@ int g_longest_chain = 0;
 // pair rule's conditions over every loop of the run
 int gen_chain_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
   (void)s;
-  if (inst().fusion < 3) return 0;
   const int avail = (int)std::min<size_t>((size_t)g_longest_chain, run.size() - at);
-  for (int n = avail; n >= 3; n--) {
+  // pairs at level >= 1, longer runs at level 3
+  for (int n = avail; n >= 2; n--) {
+    if (n >= 3 && inst().fusion < 3) continue;
     std::vector<opsmi_gen_launch_fn> key;
     const GenLoop *G[OPSMI_GEN_MAX_CHAIN];
     bool ok = true;
@@ -569,7 +570,7 @@ void ops_mi_gen_fused(opsmi_gen_launch_fn first, opsmi_gen_launch_fn second, ops
 void ops_mi_gen_fused_chain(const opsmi_gen_launch_fn *loops, int n, const int *groups, int ngroups,
                             opsmi_gen_chain_fn fused) {
   OPSMI_API_BEGIN
-  if (!loops || !fused || n < 3 || n > OPSMI_GEN_MAX_CHAIN || ngroups < 0 ||
+  if (!loops || !fused || n < 2 || n > OPSMI_GEN_MAX_CHAIN || ngroups < 0 ||
       ngroups > n * OPSMI_GEN_MAX_ARGS || (ngroups > 0 && !groups))
     fail(OPS_INVALID_ARGUMENT, "ops_mi_gen_fused_chain: bad arguments");
   std::vector<opsmi_gen_launch_fn> key(loops, loops + n);

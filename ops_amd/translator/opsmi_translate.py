@@ -688,13 +688,13 @@ def fused_code(l1, l2):
 
 
 def chain_code(ls, calls):
-    """One kernel running the kernels of the loops ls (three or more) in turn
+    """One kernel running the kernels of the loops ls (two or more) in turn
     at every point, and its launch function (registered for the chain; the
     library decides per call whether the queued loops may run this way).
     calls are the run's ops_par_loop calls: dat arguments written with the
     same expression share one restrict-qualified pointer (the library runs
     the kernel only when those name one dat and the others distinct dats).
-    None when two arguments with one expression differ in type."""
+    None when two arguments with one expression differ in element type."""
     F = "__".join(lp.kernel for lp in ls)
     groups, gtyp, gconst, flat = {}, [], [], []
     for k, (lp, c) in enumerate(zip(ls, calls)):
@@ -705,10 +705,10 @@ def chain_code(ls, calls):
             e = c.args[i].expr
             if e not in groups:
                 groups[e] = len(gtyp)
-                gtyp.append((a.typ, a.multi))
+                gtyp.append(a.typ)
                 gconst.append(True)
             j = groups[e]
-            if gtyp[j] != (a.typ, a.multi):
+            if gtyp[j] != a.typ:
                 return None
             gconst[j] = gconst[j] and a.acc == "OPS_READ"
             flat.append(j)
@@ -717,7 +717,7 @@ def chain_code(ls, calls):
          "// (one restrict-qualified pointer per distinct dat)",
          f"__global__ __launch_bounds__(256) void opsg_chain_{F}("
          + ", ".join(f"const opsa_{lp.kernel} A{k}" for k, lp in enumerate(ls))
-         + "".join(f", {'const ' if gconst[j] else ''}{t} *__restrict__ g{j}" for j, (t, _) in enumerate(gtyp))
+         + "".join(f", {'const ' if gconst[j] else ''}{t} *__restrict__ g{j}" for j, t in enumerate(gtyp))
          + ") {"]
     pos = 0
     for k, lp in enumerate(ls):
@@ -748,7 +748,7 @@ def chain_code(ls, calls):
     L.append(f"  hipLaunchKernelGGL(opsg_chain_{F}, dim3(L[0]->grid[0], L[0]->grid[1], L[0]->grid[2]), "
              f"dim3(::opsmi_gen::kWave, ::opsmi_gen::kRows), 0, (hipStream_t)stream, "
              + ", ".join(f"A{k}" for k in range(len(ls)))
-             + "".join(f", ({gtyp[j][0]} *)L[{first[j][0]}]->dat[{first[j][1]}].base" for j in range(len(gtyp)))
+             + "".join(f", ({gtyp[j]} *)L[{first[j][0]}]->dat[{first[j][1]}].base" for j in range(len(gtyp)))
              + ");")
     L.append("}")
     fns = ", ".join(f"&opsl_{lp.kernel}" for lp in ls)
@@ -1049,10 +1049,12 @@ def translate(app, out_dir, inc_dirs=(), program=None):
         stubs.append(s)
         sigs.append(sig)
     regs = []
+    # adjacent pairs: the same one-pass form as the runs below
     for a, b in adjacent_pairs(loops):
-        code, reg = fused_code(kernels[a.kernel][0], kernels[b.kernel][0])
-        H.append(code)
-        regs.append(reg)
+        got = chain_code([kernels[a.kernel][0], kernels[b.kernel][0]], [a, b])
+        if got:
+            H.append(got[0])
+            regs.append(got[1])
     # runs of three and four calls touching what they write only at the point
     for n in (4, 3):
         for ls in point_chains(loops, kpoints, n):

@@ -43,10 +43,13 @@ def test_translates_dev_api_applications(tmp_path):
     assert "auto kern = L->alias ? opsg_fill_fields_alias : opsg_fill_fields;" in hip
     assert "store_partials<int, OPS_INC, 1>" in hip and "store_partials<double, OPS_INC, 2>" in hip
     # one fused kernel per pair of loops that follow each other in the source
-    assert "opsg_fuse_mix_fields__summary(const opsa_mix_fields A, const opsa_summary B" in hip
-    assert "ops_mi_gen_fused(&opsl_fill_fields, &opsl_mix_fields, &opsf_fill_fields__mix_fields);" in hip
-    assert "ops_mi_gen_fused(&opsl_mix_fields, &opsl_summary, &opsf_mix_fields__summary);" in hip
-    assert "opsf_summary__" not in hip
+    # adjacent pairs: one restrict pointer per distinct dat expression
+    assert "opsg_chain_mix_fields__summary(const opsa_mix_fields A0, const opsa_summary A1, " \
+        "const double *__restrict__ g0, double *__restrict__ g1, const int *__restrict__ g2)" in hip
+    assert "{&opsl_fill_fields, &opsl_mix_fields}; static const int g[] = {0, 1, -1, -1, 0, 2}; " \
+        "ops_mi_gen_fused_chain(c, 2, g, 6, &opsc_fill_fields__mix_fields);" in hip
+    assert "ops_mi_gen_fused_chain(c, 2, g, 9, &opsc_mix_fields__summary);" in hip
+    assert "opsc_summary__" not in hip
     decl = open(os.path.join(out, "heat3d_gen_opsmi_kernels.h")).read()
     assert 'visibility("hidden")' in decl and "ops_par_loop_heat_calc_w(" in decl
 
@@ -167,7 +170,7 @@ def test_legacy_pointer_kernels_translate(tmp_path):
     assert "static const int pts[12] = {-1, -1, 0, 0, -1, 1, 0, 0, -1, 0, 0, 0};" in stub
     stub2 = hip[hip.index("void ops_par_loop_lk2("):]
     assert "static const int npts[2] = {1, 1};" in stub2 and "1, 0, 0, 0" in stub2  # component 1 of d
-    assert "ops_mi_gen_fused(" in hip
+    assert "ops_mi_gen_fused_chain(" in hip
     assert T.legacy_points("{ b[OPS_ACC0(0,0)] = *(&a[OPS_ACC1(0,0)] + 1); }", "a", 2) is None
     assert T.accessed_points("{ w(0, 0) = *(&p(0, 0) + 1); }", "p", 2) is None
     assert T.legacy_points("{ b[OPS_ACC0(0,0)] = (a[OPS_ACC1(0,0)] > 0) && (a[OPS_ACC1(1,0)] > 0); }", "a", 2) \
