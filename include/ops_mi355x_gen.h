@@ -102,23 +102,11 @@ void ops_mi_gen_const_sized(const char *name, const char *type, size_t elem_size
 #define OPSMI_CONST_CAP(T) ((OPSMI_CONST_BYTES + sizeof(T) - 1) / sizeof(T))
 
 /* launches one kernel that runs, at each point, the first loop's kernel and
- * then the second's (L1, L2 share range and tiling) */
+ * then the second's (L1, L2 share range and tiling): the neighbour form
+ * below */
 typedef void (*opsmi_gen_fused_fn)(const opsmi_gen_launch *L1, const void *blob1, const opsmi_gen_launch *L2,
                                    const void *blob2, void *stream);
 
-/* Registers `fused` as the one-pass form of a loop launched by `first`
- * directly followed by a loop launched by `second` (no counterpart in the
- * reference: its lazy queue, ops/c/src/core/ops_lazy.cpp:380-1187, tiles
- * loop chains but launches each loop separately).  The translator registers
- * one per pair of ops_par_loop calls that follow each other in an
- * application's source.  When the queue runs two such loops back to back it
- * launches the fused form if running them point by point gives what running
- * them one after the other gives: the same block, range and dimension, no
- * shared reduction handle, every dat that either loop writes accessed by
- * both only at the point itself (listed points, offset 0), and fusion
- * enabled (ops_mi_set_fusion level >= 1).  Reductions and results are the
- * same bits either way. */
-void ops_mi_gen_fused(opsmi_gen_launch_fn first, opsmi_gen_launch_fn second, opsmi_gen_fused_fn fused);
 
 /* Registers `fused` as the one-pass form of a loop launched by `first`
  * directly followed by a loop launched by `second` that reads, at stencil
@@ -157,10 +145,11 @@ typedef void (*opsmi_gen_chain_fn)(const opsmi_gen_launch *const *L, const void 
  * that follow each other in an application's source, and one per run of
  * three or four such calls that touch what they write only at the point.
  * The queue launches the longest registered form that matches the loops it
- * runs back to back, under the conditions of ops_mi_gen_fused extended to
- * every loop of the run (same block, range and dimension; no reduction
- * handle shared by two of them; every dat any of them writes accessed by
- * all of them only at the point itself), pairs at fusion level >= 1 and
+ * runs back to back when running them point by point gives what running
+ * them one after the other gives: the same block, range and dimension; no
+ * reduction handle shared by two of them; every dat any of them writes
+ * accessed by all of them only at the point itself (listed points, offset
+ * 0); pairs at fusion level >= 1 and
  * longer runs at level 3 (ops_mi_set_fusion), and only when the arguments
  * agree with `groups`: one entry per argument of each loop in turn
  * (ngroups in all), the index of the fused kernel's pointer a dat argument

@@ -186,11 +186,6 @@ void GenLoop::run_resets(hipStream_t s) const {
 
 namespace {
 
-std::map<std::pair<opsmi_gen_launch_fn, opsmi_gen_launch_fn>, opsmi_gen_fused_fn> &fused_table() {
-  static std::map<std::pair<opsmi_gen_launch_fn, opsmi_gen_launch_fn>, opsmi_gen_fused_fn> m;
-  return m;
-}
-
 struct NbFused {
   opsmi_gen_fused_fn fn = nullptr;
   std::vector<std::pair<int, int>> links;  // (argument of the second loop, argument of the first)
@@ -209,47 +204,6 @@ bool share_handle(const GenLoop &a, const GenLoop &b) {
     for (const GenRed &y : b.reds)
       if (x.ctl.slot == y.ctl.slot) return true;
   return false;
-}
-
-// Two generated loops back to back become one pass when running them point
-// by point equals running them in turn: every dat either writes is touched
-// by both only at the point itself, so no point reads what another point's
-// second loop writes, or reads before another point's first loop writes.
-int gen_pair_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
-  if (at + 1 >= run.size() || !run[at].gen || !run[at + 1].gen) return 0;
-  const GenLoop &A = *run[at].gen, &B = *run[at + 1].gen;
-  const auto it = fused_table().find({A.fn, B.fn});
-  if (it == fused_table().end()) return 0;
-  if (A.block != B.block || A.dims != B.dims || !same_range(A.r, B.r) || A.r.npts() <= 0) return 0;
-  if (share_handle(A, B)) return 0;
-  // the fused kernel takes the restrict-qualified point mapping only
-  if (A.alias || B.alias || A.strided || B.strided || !A.resets.empty() || !B.resets.empty()) return 0;
-  for (const GenLoop *W : {&A, &B})
-    for (size_t i = 0; i < W->dats.size(); i++) {
-      const ops_dat d = W->dats[i];
-      if (d && W->acc[i] != OPS_READ && !(A.only_at_point(d) && B.only_at_point(d))) return 0;
-    }
-  const opsmi_gen_fused_fn fused = it->second;
-  const size_t off_b = A.pbytes;
-  ensure_partials(off_b + B.pbytes);
-  LoopRec &ra = run[at], &rb = run[at + 1];
-  const std::string name = "fused(" + ra.name + "+" + rb.name + ")";
-  run_loop(name.c_str(), ra.bytes + rb.bytes, [&](hipStream_t st) {
-    char *pbase = (char *)inst().d_partials;
-    opsmi_gen_launch La, Lb;
-    A.fill(La, pbase);
-    B.fill(Lb, pbase + off_b);
-    fused(&La, A.blob(), &Lb, B.blob(), (void *)st);
-    OPSMI_HIP(hipGetLastError());
-    A.finish_reds(pbase, st);
-    B.finish_reds(pbase + off_b, st);
-  });
-  (void)s;
-  note_fused(ra.name.c_str(), ra.bytes);
-  note_fused(rb.name.c_str(), rb.bytes);
-  for (const LoopRec *L : {&ra, &rb})
-    for (ops_dat d : L->writes) note_external_write(d);
-  return 2;
 }
 
 struct ChainFused {
@@ -284,8 +238,11 @@ bool groups_match(const GenLoop *const *G, int n, const std::vector<int> &groups
 }
 int g_longest_chain = 0;
 
-// n generated loops back to back as one pass (ops_mi_gen_fused_chain): the
-// pair rule's conditions over every loop of the run
+// n generated loops back to back as one pass (ops_mi_gen_fused_chain) when
+// running them point by point equals running them in turn: every dat any of
+// them writes is touched by all of them only at the point itself, so no
+// point reads what another point's later loop writes, or reads before
+// another point's earlier loop writes
 int gen_chain_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
   (void)s;
   const int avail = (int)std::min<size_t>((size_t)g_longest_chain, run.size() - at);
@@ -429,7 +386,6 @@ int gen_nb_rule(std::vector<LoopRec> &run, size_t at, hipStream_t s) {
 struct RegisterGen {
   RegisterGen() {
     register_fusion(gen_chain_rule);
-    register_fusion(gen_pair_rule);
     register_fusion(gen_nb_rule);
   }
 } g_register_gen;
@@ -557,13 +513,6 @@ void ops_mi_gen_par_loop(const char *name, ops_block block, int dim, const int *
   }
   enqueue(std::move(rec));
   finish_loop(args, nargs);
-  OPSMI_API_END()
-}
-
-void ops_mi_gen_fused(opsmi_gen_launch_fn first, opsmi_gen_launch_fn second, opsmi_gen_fused_fn fused) {
-  OPSMI_API_BEGIN
-  if (!first || !second || !fused) fail(OPS_INVALID_ARGUMENT, "ops_mi_gen_fused: NULL function");
-  fused_table()[{first, second}] = fused;
   OPSMI_API_END()
 }
 

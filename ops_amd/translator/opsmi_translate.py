@@ -654,39 +654,6 @@ def launch_ptrs(loop, Lname):
     return "".join(f", ({a.typ} *){Lname}->dat[{i}].base" for i, a in enumerate(loop.args) if a.kind == "dat")
 
 
-def fused_code(l1, l2):
-    """One kernel running l1's kernel and then l2's at every point, and its
-    launch function (registered for the pair; the library decides per call
-    whether the two queued loops may run this way)."""
-    K1, K2 = l1.kernel, l2.kernel
-    F = f"{K1}__{K2}"
-    ptrs = []
-    for pfx, lp in (("f", l1), ("s", l2)):
-        ptrs += [f"{'const ' if a.acc == 'OPS_READ' else ''}{a.typ} *{pfx}p{i}"
-                 for i, a in enumerate(lp.args) if a.kind == "dat"]
-    L = [f"// ---------------------------------------------------------------- {K1} then {K2}",
-         "// (plain pointers: the two loops may share dats)",
-         f"__global__ __launch_bounds__(256) void opsg_fuse_{F}(const opsa_{K1} A, const opsa_{K2} B"
-         f"{''.join(', ' + p for p in ptrs)}) {{"]
-    L += red_init_lines(l1, "f") + red_init_lines(l2, "s")
-    L.append("  ::opsmi_gen::for_each_point(A.L, [&](int x, int y, int z) {")
-    L += point_lines(l1, "A", "f") + point_lines(l2, "B", "s")
-    L.append("  });")
-    L += red_store_lines(l1, "A", "f") + red_store_lines(l2, "B", "s")
-    L.append("}")
-    L.append(f"void opsf_{F}(const opsmi_gen_launch *L1, const void *blob1, const opsmi_gen_launch *L2, "
-             "const void *blob2, void *stream) {")
-    L.append(f"  opsa_{K1} A;")
-    L.append(f"  opsfill_{K1}(A, L1, blob1);")
-    L.append(f"  opsa_{K2} B;")
-    L.append(f"  opsfill_{K2}(B, L2, blob2);")
-    L.append(f"  hipLaunchKernelGGL(opsg_fuse_{F}, dim3(L1->grid[0], L1->grid[1], L1->grid[2]), "
-             f"dim3(::opsmi_gen::kWave, ::opsmi_gen::kRows), 0, (hipStream_t)stream, A, B"
-             f"{launch_ptrs(l1, 'L1')}{launch_ptrs(l2, 'L2')});")
-    L.append("}")
-    return "\n".join(L), f"ops_mi_gen_fused(&opsl_{K1}, &opsl_{K2}, &opsf_{F});"
-
-
 def chain_code(ls, calls):
     """One kernel running the kernels of the loops ls (two or more) in turn
     at every point, and its launch function (registered for the chain; the
